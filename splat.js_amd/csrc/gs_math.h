@@ -1,0 +1,113 @@
+// gs_math.h — portable single-precision exp/log used by every kernel whose
+// results must be reproducible bit for bit on the host.
+//
+// The GPU's expf/logf (ocml) and glibc's differ by an ULP here and there.
+// The trainer's discrete decisions (cull if opa*comp < A_MIN, the tile
+// rectangle from the opacity-aware radius, skip if alpha < A_MIN, stop when
+// T < 1e-4) turn such ULP differences into different sort keys or different
+// pixels. These two functions use only IEEE-754 basic operations (+, *, /,
+// fused multiply-add, round-to-nearest-even) and integer bit manipulation,
+// so a C compiler on x86-64 and hipcc on gfx950 produce identical bits from
+// the same source, provided both compile with -ffp-contract=off.
+//
+//   exp: Cody-Waite reduction x = k*ln2 + r, |r| <= ln2/2, then the
+//        degree-7 Taylor polynomial of e^r in Horner form (truncation error
+//        r^8/8! < 5.3e-9 relative), then scaling by 2^k through the
+//        exponent field. Max error about 1 ulp.
+//   log: x = m * 2^e with m in [sqrt(1/2), sqrt(2)), s = (m-1)/(m+1),
+//        log m = 2 atanh(s) = 2(s + s^3/3 + ... + s^9/9)
+//        (|s| <= 0.1716, truncation < 4e-10), plus e*ln2 split in two parts.
+//        Max error about 2 ulp.
+//
+// Included from C (the FP32 restatement) and from HIP device code.
+#ifndef GS_MATH_H
+#define GS_MATH_H
+
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define GS_HD __host__ __device__ __forceinline__
+#else
+#include <math.h>
+#include <string.h>
+#define GS_HD static inline
+#endif
+
+GS_HD uint32_t gs_f2u(float x) {
+#if defined(__HIPCC__)
+  return __builtin_bit_cast(uint32_t, x);
+#else
+  uint32_t u; memcpy(&u, &x, 4); return u;
+#endif
+}
+
+GS_HD float gs_u2f(uint32_t u) {
+#if defined(__HIPCC__)
+  return __builtin_bit_cast(float, u);
+#else
+  float x; memcpy(&x, &u, 4); return x;
+#endif
+}
+
+// e^x, correctly handling NaN, overflow, underflow and denormal results.
+GS_HD float gs_expf(float x) {
+  if (x != x) return x;
+  if (x > 88.72283172607421875f) return gs_u2f(0x7f800000u);  // +inf
+  if (x < -103.972084045410156f) return 0.0f;
+  const float kf = rintf(x * 1.44269502162933349609375f);      // round(x/ln2)
+  // ln2 = LN2_HI + LN2_LO, LN2_HI has 15 significant bits so k*LN2_HI is exact
+  float r = fmaf(kf, -0.693145751953125f, x);
+  r = fmaf(kf, -1.428606765330187045e-06f, r);
+  float p = 1.984126984126984e-04f;          // 1/7!
+  p = fmaf(p, r, 1.388888888888889e-03f);    // 1/6!
+  p = fmaf(p, r, 8.333333333333333e-03f);    // 1/5!
+  p = fmaf(p, r, 4.166666666666667e-02f);    // 1/4!
+  p = fmaf(p, r, 1.666666666666667e-01f);    // 1/3!
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  int k = (int)kf;
+  if (k > 127) {             // only k == 128 reaches here: p*2^127*2
+    return (p * gs_u2f(0x7f000000u)) * 2.0f;
+  }
+  if (k < -126) {            // denormal result: one rounding, in the last multiply
+    return (p * gs_u2f((uint32_t)(k + 126 + 127) << 23)) * gs_u2f(0x00800000u);
+  }
+  return p * gs_u2f((uint32_t)(k + 127) << 23);
+}
+
+// natural log; NaN for x < 0, -inf at 0, +inf at +inf.
+GS_HD float gs_logf(float x) {
+  if (x != x) return x;
+  if (x < 0.0f) return gs_u2f(0x7fc00000u);
+  if (x == 0.0f) return gs_u2f(0xff800000u);
+  uint32_t u = gs_f2u(x);
+  if (u == 0x7f800000u) return x;
+  int e = 0;
+  if (u < 0x00800000u) {     // denormal input: scale by 2^23 first
+    x = x * 8388608.0f;
+    u = gs_f2u(x);
+    e = -23;
+  }
+  e += (int)(u >> 23) - 127;
+  float m = gs_u2f((u & 0x007fffffu) | 0x3f800000u);   // [1, 2)
+  if (m > 1.41421353816986083984375f) { m = m * 0.5f; e += 1; }
+  const float s = (m - 1.0f) / (m + 1.0f);
+  const float s2 = s * s;
+  float p = 1.111111111111111e-01f;          // 1/9
+  p = fmaf(p, s2, 1.428571428571429e-01f);   // 1/7
+  p = fmaf(p, s2, 2.0e-01f);                 // 1/5
+  p = fmaf(p, s2, 3.333333333333333e-01f);   // 1/3
+  const float s2x = s + s;
+  const float lm = fmaf(s2x * s2, p, s2x);
+  const float ef = (float)e;
+  return fmaf(ef, 0.693145751953125f, fmaf(ef, 1.428606765330187045e-06f, lm));
+}
+
+// 1 / (1 + e^-clamp(x, -9, 9)): the activation for colour and opacity logits.
+GS_HD float gs_sigmoid_clamped(float x) {
+  const float c = fminf(fmaxf(x, -9.0f), 9.0f);
+  return 1.0f / (1.0f + gs_expf(-c));
+}
+
+#endif  // GS_MATH_H

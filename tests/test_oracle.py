@@ -1,0 +1,518 @@
+"""Pins the CPU restatement (oracle/) before it is trusted as the checker.
+
+Each check is chosen so that a plausible mistake in the restatement (a dropped
+term, a wrong sign or index, a transposed operand) fails: closed forms,
+textbook constants, brute-force special cases, libm, and central finite
+differences in double precision on the reference's own gradcheck rig
+(src/gs/gradcheck.js:93-185).
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+import scenes
+
+
+def _cam(W=64, H=48, f=50.0, R=None, t=None, cx=None, cy=None, g=0.0, b=0.0):
+    return {"R": R or [1, 0, 0, 0, 1, 0, 0, 0, 1], "t": t or [0, 0, 0], "f": f,
+            "cx": W / 2 if cx is None else cx, "cy": H / 2 if cy is None else cy,
+            "w": W, "h": H, "g": g, "b": b}
+
+
+def _gauss(pos, ls, q=(1, 0, 0, 0), col=(0.0, 0.0, 0.0), op=0.0):
+    p = np.zeros(16, np.float32)
+    p[0:3] = pos
+    p[3:6] = ls
+    p[6:10] = q
+    p[10:13] = col
+    p[13] = op
+    return p
+
+
+# ---------------------------------------------------------------- portable math
+def _ulp_err(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    ulp = np.spacing(np.abs(b).astype(np.float32)).astype(np.float64)
+    return np.abs(a - b) / ulp
+
+
+def test_expf_matches_libm(oracle_lib):
+    L = O.lib()
+    xs = np.concatenate([np.linspace(-103.9, 88.7, 20001), np.linspace(-10, 0, 5001),
+                         np.array([0.0, -0.0, 1e-30, -1e-30, 1.0, -1.0])]).astype(np.float32)
+    got = np.array([L.orc_expf(float(x)) for x in xs], np.float32)
+    ref = np.exp(xs.astype(np.float64))
+    normal = ref > 1.18e-38
+    assert _ulp_err(got[normal], ref[normal]).max() <= 1.5
+    assert L.orc_expf(0.0) == 1.0
+    assert L.orc_expf(-200.0) == 0.0
+    assert math.isinf(L.orc_expf(100.0))
+    assert math.isnan(L.orc_expf(float("nan")))
+
+
+def test_logf_matches_libm(oracle_lib):
+    L = O.lib()
+    xs = np.concatenate([np.geomspace(1e-37, 3e38, 20001), np.linspace(0.5, 2.0, 5001),
+                         np.array([1.0, 1.0001, 2.0, 0.5])]).astype(np.float32)
+    got = np.array([L.orc_logf(float(x)) for x in xs], np.float32)
+    ref = np.log(xs.astype(np.float64))
+    big = np.abs(ref) > 1e-3
+    assert _ulp_err(got[big], ref[big]).max() <= 2.5
+    # near 1 check absolute error relative to the tiny result
+    small = ~big
+    assert np.all(np.abs(got[small] - ref[small]) <= 4e-7 * np.abs(ref[small]) + 1e-12)
+    assert L.orc_logf(1.0) == 0.0
+    assert math.isinf(L.orc_logf(0.0))
+    assert math.isnan(L.orc_logf(-1.0))
+
+
+# ---------------------------------------------------------------- projection
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_project_on_axis_closed_form(oracle_lib, prec):
+    """Identity q, isotropic s: V = (f s / z)^2 I, conic = 1/(v + 0.3),
+    comp = v/(v+0.3) (ply.js:24-25 bakes the same factor), mean = (cx, cy)."""
+    o = O.make_opts(shDeg=0)
+    cam = O.make_cam(_cam(W=64, H=48, f=50.0))
+    st = O.Stage(o, prec)
+    z, s = 3.0, 0.05
+    p = _gauss((0, 0, z), [math.log(s)] * 3, op=2.0)[None]
+    proj, rect, cnt = st.project(cam, p, None)
+    pr = proj[0].astype(np.float64)
+    s = math.exp(float(np.float32(math.log(s))))  # the log-scale is stored in f32
+    v = (50.0 * s / z) ** 2
+    tol = 1e-6 if prec == "f32" else 1e-12
+    assert pr[11] == 1
+    assert abs(pr[0] - 32) < 1e-5 and abs(pr[1] - 24) < 1e-5
+    assert abs(pr[2] - z) < 1e-6
+    np.testing.assert_allclose([pr[12], pr[14]], [v, v], rtol=tol * 10)
+    assert abs(pr[13]) < 1e-9
+    np.testing.assert_allclose(pr[3], 1 / (v + 0.3), rtol=tol * 10)
+    np.testing.assert_allclose(pr[5], 1 / (v + 0.3), rtol=tol * 10)
+    np.testing.assert_allclose(pr[6], v / (v + 0.3), rtol=tol * 10)
+    opa = 1 / (1 + math.exp(-2.0))
+    np.testing.assert_allclose(pr[7], opa, rtol=tol * 10)
+    emax = min(4.5, math.log(opa * (v / (v + 0.3)) / 0.0039))
+    np.testing.assert_allclose(pr[15], math.sqrt(2 * emax) * math.sqrt(v + 0.3), rtol=tol * 10)
+    # tile rect = floor((m -+ rad)/16) clamped to the grid
+    r = pr[15]
+    assert list(rect[0]) == [max(0, math.floor((32 - r) / 16)), min(3, math.floor((32 + r) / 16)),
+                             max(0, math.floor((24 - r) / 16)), min(2, math.floor((24 + r) / 16))]
+    assert cnt[0] == (rect[0][1] - rect[0][0] + 1) * (rect[0][3] - rect[0][2] + 1)
+
+
+def test_project_off_axis_and_rotation(oracle_lib):
+    """Mean = f x/z + cx; a 90-degree rotation about z swaps the x/y extents."""
+    o = O.make_opts(shDeg=0)
+    cam = O.make_cam(_cam(W=200, H=100, f=80.0))
+    st = O.Stage(o, "f64")
+    x, y, z = 0.4, -0.2, 4.0
+    ls = [math.log(0.2), math.log(0.05), math.log(0.05)]
+    h = math.sqrt(0.5)
+    p = np.stack([_gauss((x, y, z), ls), _gauss((x, y, z), ls, q=(h, 0, 0, h))])
+    proj, _, _ = st.project(cam, p, None)
+    x, y, z = (float(np.float32(a)) for a in (x, y, z))  # parameters are stored in f32
+    ls = [float(np.float32(a)) for a in ls]
+    h = float(np.float32(h))
+    np.testing.assert_allclose(proj[:, 0], 80 * x / z + 100, rtol=1e-12)
+    np.testing.assert_allclose(proj[:, 1], 80 * y / z + 50, rtol=1e-12)
+    # the full EWA V for the first one, numerically: J W Sigma W^T J^T
+    def V_of(q, s):
+        qw, qx, qy, qz = q
+        Rq = np.array([[1 - 2 * (qy * qy + qz * qz), 2 * (qx * qy - qw * qz), 2 * (qx * qz + qw * qy)],
+                       [2 * (qx * qy + qw * qz), 1 - 2 * (qx * qx + qz * qz), 2 * (qy * qz - qw * qx)],
+                       [2 * (qx * qz - qw * qy), 2 * (qy * qz + qw * qx), 1 - 2 * (qx * qx + qy * qy)]])
+        S = Rq @ np.diag(np.asarray(s) ** 2) @ Rq.T
+        J = np.array([[80 / z, 0, -80 * x / z ** 2], [0, 80 / z, -80 * y / z ** 2]])
+        return J @ S @ J.T
+    s = np.exp(np.array(ls, np.float64))
+    for k, q in enumerate([(1, 0, 0, 0), (h, 0, 0, h)]):
+        V = V_of(np.array(q, np.float64) / np.linalg.norm(q), s)
+        np.testing.assert_allclose(proj[k, 12:15], [V[0, 0], V[0, 1], V[1, 1]], rtol=1e-6, atol=1e-9)
+    # rotated by 90 degrees about the view axis: x/y variances swap (approximately, J is off-axis)
+    assert proj[0, 12] > proj[0, 14] and proj[1, 14] > proj[1, 12]
+
+
+def test_project_culls(oracle_lib):
+    o = O.make_opts(shDeg=0)
+    cam = O.make_cam(_cam(W=64, H=48, f=50.0))
+    st = O.Stage(o, "f32")
+    p = np.stack([
+        _gauss((0, 0, 0.01), [-3] * 3, op=3),      # in front of the near plane
+        _gauss((0, 0, -2), [-3] * 3, op=3),        # behind the camera
+        _gauss((0, 0, 3), [-3] * 3, op=-9),        # opacity below A_MIN
+        _gauss((40, 0, 3), [-3] * 3, op=3),        # far outside the frustum
+        _gauss((0, 0, 3), [-3] * 3, op=3),         # visible
+    ])
+    proj, rect, cnt = st.project(cam, p, None)
+    assert list(proj[:, 11]) == [0, 0, 0, 0, 1]
+    assert list(cnt[:4]) == [0, 0, 0, 0] and cnt[4] >= 1
+
+
+# ---------------------------------------------------------------- SH basis
+def test_sh_basis_constants_and_gradient(oracle_lib):
+    L = O.lib()
+    import ctypes as C
+    Y = np.zeros(15)
+    D = np.zeros(45)
+
+    def basis(v):
+        v = np.ascontiguousarray(v, np.float64)
+        L.orc_sh_basis_f64(3, v.ctypes.data_as(C.c_void_p), Y.ctypes.data_as(C.c_void_p),
+                           D.ctypes.data_as(C.c_void_p))
+        return Y.copy(), D.copy().reshape(15, 3)
+
+    c1 = math.sqrt(3 / (4 * math.pi))
+    y, _ = basis([0, 0, 1.0])
+    np.testing.assert_allclose(y[:3], [0, c1, 0], atol=1e-15)
+    np.testing.assert_allclose(y[5], 0.25 * math.sqrt(5 / math.pi) * 2, rtol=1e-14)
+    np.testing.assert_allclose(y[11], 0.25 * math.sqrt(7 / math.pi) * 2, rtol=1e-12)
+    y, _ = basis([1.0, 0, 0])
+    np.testing.assert_allclose(y[2], -c1, rtol=1e-15)
+    np.testing.assert_allclose(y[7], 0.25 * math.sqrt(15 / math.pi), rtol=1e-14)
+    # orthonormality over the sphere (Monte Carlo with many samples)
+    rng = np.random.default_rng(3)
+    v = rng.normal(size=(20000, 3))
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    Ys = np.stack([basis(vi)[0] for vi in v])
+    G = 4 * math.pi * Ys.T @ Ys / len(v)
+    # shaders.js:102 writes Y[14] = C3[6] * x * (xx - yy); the textbook real SH
+    # (and INRIA 3DGS) is x * (xx - 3 yy). The restatement keeps the
+    # reference's function, so only the other 14 are orthonormal.
+    std = np.arange(14)
+    np.testing.assert_allclose(G[np.ix_(std, std)], np.eye(14), atol=0.06)
+    vx = np.array([0.3, -0.5, math.sqrt(1 - 0.34)])
+    np.testing.assert_allclose(basis(vx)[0][14], -0.5900435899266435 * 0.3 * (0.09 - 0.25), rtol=1e-14)
+    # dY/dv against central differences (the basis is a polynomial in v)
+    for vi in v[:40]:
+        _, d = basis(vi)
+        h = 1e-6
+        for a in range(3):
+            e = np.zeros(3)
+            e[a] = h
+            fd = (basis(vi + e)[0] - basis(vi - e)[0]) / (2 * h)
+            np.testing.assert_allclose(d[:, a], fd, atol=1e-8)
+
+
+# ---------------------------------------------------------------- binning + sort
+def test_bin_sort_order_and_ranges(oracle_lib):
+    o = O.make_opts(shDeg=2)
+    p, sh, cams = scenes.config1(seed=0)
+    p, sh = p[:3000], sh[:3000]
+    cam = O.make_cam(cams[0], shActiveDeg=2)
+    st = O.Stage(o, "f32")
+    proj, rect, cnt = st.project(cam, p, sh)
+    ids, ts, keys = st.bin_sort(cam, proj, rect, cnt)
+    tilesX = (cam.W + 15) // 16
+    # brute force with numpy lexsort over (tile, depth bits, id)
+    tl, dp, gi = [], [], []
+    for i in np.nonzero(cnt)[0]:
+        tx0, tx1, ty0, ty1 = rect[i]
+        for ty in range(ty0, ty1 + 1):
+            for tx in range(tx0, tx1 + 1):
+                tl.append(ty * tilesX + tx)
+                dp.append(np.float32(proj[i, 2]).view(np.uint32))
+                gi.append(i)
+    tl, dp, gi = map(np.array, (tl, dp, gi))
+    order = np.lexsort((gi, dp, tl))
+    np.testing.assert_array_equal(ids, gi[order])
+    np.testing.assert_array_equal(keys, (tl[order].astype(np.uint64) << 32) | dp[order].astype(np.uint64))
+    np.testing.assert_array_equal(np.diff(ts), np.bincount(tl, minlength=len(ts) - 1))
+
+
+# ---------------------------------------------------------------- raster forward
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_render_single_gaussian_closed_form(oracle_lib, prec):
+    o = O.make_opts(shDeg=0)
+    cam = O.make_cam(_cam(W=48, H=40, f=60.0))
+    st = O.Stage(o, prec)
+    p = _gauss((0.02, -0.01, 3.0), [math.log(0.08), math.log(0.05), math.log(0.06)],
+               q=(0.9, 0.1, -0.2, 0.3), col=(0.3, -0.4, 1.0), op=1.0)[None]
+    proj, rect, cnt = st.project(cam, p, None)
+    ids, ts, _ = st.bin_sort(cam, proj, rect, cnt)
+    img, T, end = st.render_fwd(cam, proj, ids, ts)
+    pr = proj[0].astype(np.float64)
+    ys, xs = np.mgrid[0:40, 0:48]
+    dx = xs + 0.5 - pr[0]
+    dy = ys + 0.5 - pr[1]
+    e = np.maximum(0.5 * (pr[3] * dx * dx + pr[5] * dy * dy) + pr[4] * dx * dy, 0)
+    a = np.minimum(0.99, pr[7] * pr[6] * np.exp(-e))
+    a = np.where((e > 4.5) | (a < 0.0039), 0, a)
+    # pixels outside the binned tiles get nothing
+    tx, ty = xs // 16, ys // 16
+    inrect = (tx >= rect[0, 0]) & (tx <= rect[0, 1]) & (ty >= rect[0, 2]) & (ty <= rect[0, 3])
+    a = np.where(inrect, a, 0).reshape(-1)
+    tol = 2e-6 if prec == "f32" else 1e-12
+    np.testing.assert_allclose(T, 1 - a, atol=tol)
+    for c in range(3):
+        np.testing.assert_allclose(img[:, c], a * pr[8 + c], atol=tol)
+    np.testing.assert_allclose(img[:, 3], a, atol=tol)
+
+
+def test_render_two_disjoint_and_termination(oracle_lib):
+    o = O.make_opts(shDeg=0)
+    cam = O.make_cam(_cam(W=64, H=32, f=60.0))
+    st = O.Stage(o, "f64")
+    # two opaque, disjoint splats; plus a third hidden behind splat A
+    A = _gauss((-0.6, 0, 2.0), [math.log(0.2)] * 3, col=(3, -3, -3), op=9.0)
+    B = _gauss((0.6, 0, 2.0), [math.log(0.02)] * 3, col=(-3, 3, -3), op=0.0)
+    Cb = _gauss((-0.6, 0, 2.5), [math.log(0.2)] * 3, col=(-3, -3, 3), op=9.0)
+    p = np.stack([A, B, Cb])
+    proj, rect, cnt = st.project(cam, p, None)
+    ids, ts, _ = st.bin_sort(cam, proj, rect, cnt)
+    img, T, end = st.render_fwd(cam, proj, ids, ts)
+    onlyB = np.stack([B])
+    projB, rectB, cntB = st.project(cam, onlyB, None)
+    idsB, tsB, _ = st.bin_sort(cam, projB, rectB, cntB)
+    imgB, TB, _ = st.render_fwd(cam, projB, idsB, tsB)
+    right = (np.arange(64 * 32) % 64) >= 40
+    np.testing.assert_allclose(img[right], imgB[right], atol=1e-12)
+    # at A's centre alpha = 0.99 twice (A then C): T = 0.01 * 0.01 = 1e-4, not < 1e-4,
+    # so nothing terminates early there; check the sequential product exactly
+    cx = int(proj[0, 0])
+    cy = int(proj[0, 1])
+    pi = cy * 64 + cx
+    seg = ids[ts[(cy // 16) * 4 + cx // 16]:ts[(cy // 16) * 4 + cx // 16 + 1]]
+    Tm = 1.0
+    for i in seg:
+        pr = proj[i]
+        dx = cx + 0.5 - pr[0]
+        dy = cy + 0.5 - pr[1]
+        e = max(0.5 * (pr[3] * dx * dx + pr[5] * dy * dy) + pr[4] * dx * dy, 0)
+        if e > 4.5:
+            continue
+        a = min(0.99, pr[7] * pr[6] * math.exp(-e))
+        if a < 0.0039:
+            continue
+        Tm *= 1 - a
+        if Tm < 1e-4:
+            break
+    assert abs(T[pi] - Tm) < 1e-12
+
+
+def test_render_terminating_splat_is_included(oracle_lib):
+    """shaders.js:572-574: the splat that drives T below 1e-4 is composited,
+    later ones are not, and end points one past it."""
+    o = O.make_opts(shDeg=0)
+    cam = O.make_cam(_cam(W=16, H=16, f=60.0))
+    st = O.Stage(o, "f64")
+    p = np.stack([_gauss((0, 0, 2.0 + 0.1 * k), [math.log(0.3)] * 3, col=(9, 9, 9), op=9.0)
+                  for k in range(4)])
+    proj, rect, cnt = st.project(cam, p, None)
+    ids, ts, _ = st.bin_sort(cam, proj, rect, cnt)
+    img, T, end = st.render_fwd(cam, proj, ids, ts)
+    pi = 8 * 16 + 8
+    a = [min(0.99, proj[i, 7] * proj[i, 6] * math.exp(-max(0.5 * (proj[i, 3] * (8.5 - proj[i, 0]) ** 2
+                     + proj[i, 5] * (8.5 - proj[i, 1]) ** 2) + proj[i, 4] * (8.5 - proj[i, 0]) * (8.5 - proj[i, 1]), 0)))
+         for i in range(4)]
+    Tm, k = 1.0, 0
+    for k in range(4):
+        Tm *= 1 - a[k]
+        if Tm < 1e-4:
+            break
+    assert end[pi] == k + 1
+    assert abs(T[pi] - Tm) < 1e-12
+
+
+# ---------------------------------------------------------------- loss
+def test_charbonnier_closed_form(oracle_lib):
+    cam = O.make_cam(_cam(W=4, H=2, g=0.1, b=0.02))
+    st = O.Stage(O.make_opts(shDeg=0), "f64")
+    rng = np.random.default_rng(0)
+    img = np.zeros((8, 4))
+    img[:, :3] = rng.random((8, 3))
+    rgb = rng.random((8, 3)).astype(np.float32)
+    rgb[3, 0] = -1  # invalid
+    tgt = O.pack_targets(rgb)
+    gC, valid, stats = st.loss_charb(cam, img, tgt)
+    gain = float(np.float32(math.exp(0.1)))
+    bias = float(np.float32(0.02))
+    t = np.stack([(tgt >> (8 * c)) & 255 for c in range(3)], 1) / 255.0
+    err = gain * img[:, :3] + bias - t
+    root = np.sqrt(err ** 2 + 0.03 ** 2)
+    v = valid.astype(bool)
+    assert list(v) == [True, True, True, False, True, True, True, True]
+    np.testing.assert_allclose(stats[1], (root[v].sum(1) - 0.09).sum(), rtol=1e-9)
+    np.testing.assert_allclose(stats[0], (err[v] ** 2).sum(), rtol=1e-12)
+    np.testing.assert_allclose(gC[v], gain * err[v] / root[v], rtol=1e-12)
+    assert np.all(gC[~v] == 0)
+
+
+def test_pack_targets_js_rounding(oracle_lib):
+    rgb = np.array([[0.5, 0.0, 1.0], [-0.1, 0.5, 0.5], [2.0, 1 / 255 * 0.5, 0.2]], np.float32)
+    out = O.pack_targets(rgb)
+    # Math.round(0.5*255 = 127.5) = 128 (ties toward +inf)
+    assert out[0] == (255 << 24) | (255 << 16) | (0 << 8) | 128
+    assert out[1] == 0
+    g = int(math.floor(float(np.float32(1 / 255 * 0.5)) * 255 + 0.5))
+    b = int(math.floor(float(np.float32(0.2)) * 255 + 0.5))
+    assert out[2] == (255 << 24) | (b << 16) | (g << 8) | 255
+
+
+def test_l1ssim_identity_and_fd(oracle_lib):
+    """SSIM of an image with itself is 1; the analytic dL/dC matches central
+    differences (the textbook loss the reference does not have)."""
+    W, H = 20, 17
+    cam = O.make_cam(_cam(W=W, H=H))
+    st = O.Stage(O.make_opts(shDeg=0), "f64")
+    rng = np.random.default_rng(1)
+    rgb = rng.random((W * H, 3)).astype(np.float32)
+    tgt = O.pack_targets(rgb)
+    t = np.stack([(tgt >> (8 * c)) & 255 for c in range(3)], 1) / 255.0
+    img = np.zeros((W * H, 4))
+    img[:, :3] = t
+    _, _, stats = st.loss_l1ssim(cam, img, tgt, lam=1.0)
+    assert abs(stats[1]) < 1e-9
+    img[:, :3] = np.clip(t + 0.1 * rng.normal(size=t.shape), 0, 1)
+    gC, _, stats = st.loss_l1ssim(cam, img, tgt, lam=0.2)
+    h = 1e-6
+    for pi, c in [(0, 0), (5, 1), (W * 8 + 9, 2), (W * H - 1, 0), (W * 3 + 17, 1)]:
+        ip = img.copy()
+        ip[pi, c] += h
+        im = img.copy()
+        im[pi, c] -= h
+        fd = (st.loss_l1ssim(cam, ip, tgt, 0.2)[2][1] - st.loss_l1ssim(cam, im, tgt, 0.2)[2][1]) / (2 * h)
+        assert abs(fd - gC[pi, c]) <= 1e-6 * max(1, abs(fd))
+
+
+# ---------------------------------------------------------------- gradients (FD)
+def _rig_stage(sh_deg=0):
+    params, cams, images, W, H, radius, opts = scenes.gradcheck_rig()
+    o = O.make_opts(eCut=opts["eCut"], aMin=opts["aMin"], radClamp=opts["radClamp"],
+                    anisoReg=opts["anisoReg"], shDeg=sh_deg)
+    tgt = O.pack_targets(images[0]["rgb"])
+    return params, cams[0], tgt, o
+
+
+def _fd_check(st, cam, params, sh, tgt, slots, h_by_slot, samples=12, seed=7, sh_mode=False):
+    """Median-of-slot relative error of central differences (the reference's
+    criterion, gradcheck.js:339-349). A sample whose step straddles one of the
+    forward pass's thresholds (alpha < A_MIN, e > E_CUT, the 0.99 clamp,
+    T < 1e-4) shows a large error that vanishes as h shrinks, while a wrong
+    analytic term does not: such samples are re-measured at h/8 and h/64."""
+    gF, shG = O.analytic_grads(st, cam, params, sh, tgt, fixed=False)
+    rng = np.random.default_rng(seed)
+    errs = []
+
+    def fd_at(i, slot, h):
+        if sh_mode:
+            pp = sh.copy(); pp[i, slot] += h
+            pm = sh.copy(); pm[i, slot] -= h
+            hh = (float(pp[i, slot]) - float(pm[i, slot])) / 2
+            return (O.loss_of(st, cam, params, pp, tgt) - O.loss_of(st, cam, params, pm, tgt)) / (2 * hh)
+        pp = params.copy(); pp[i, slot] += h
+        pm = params.copy(); pm[i, slot] -= h
+        hh = (float(pp[i, slot]) - float(pm[i, slot])) / 2
+        return (O.loss_of(st, cam, pp, sh, tgt) - O.loss_of(st, cam, pm, sh, tgt)) / (2 * hh)
+
+    for slot in slots:
+        g = shG[:, slot] if sh_mode else gF[:, slot]
+        cand = np.argsort(-np.abs(g))[:40]
+        h0 = h_by_slot if sh_mode else h_by_slot[slot]
+        for i in rng.choice(cand, size=min(samples, len(cand)), replace=False):
+            a = g[i]
+            best = None
+            for h in (h0, h0 / 8, h0 / 64):
+                fd = fd_at(i, slot, h)
+                den = max(abs(a), abs(fd))
+                e = 0.0 if den < 1e-6 else abs(a - fd) / den
+                best = e if best is None else min(best, e)
+                if best < 1e-4:
+                    break
+            errs.append((slot, best))
+    return errs
+
+
+def test_fd_gradcheck_params_f64(oracle_lib):
+    """Central differences of the double-precision loss vs the analytic chain,
+    on the reference rig (gradcheck.js:139-185). The parameters are f32, so
+    the step is taken on f32 values; h is the actual f32 difference."""
+    params, cam_d, tgt, o = _rig_stage(0)
+    cam = O.make_cam(cam_d)
+    st = O.Stage(o, "f64")
+    h = [2e-6, 2e-6, 2e-6, 1e-5, 1e-5, 1e-5, 1e-5, 1e-5, 1e-5, 1e-5, 1e-5, 1e-5, 1e-5, 1e-5]
+    errs = _fd_check(st, cam, params, None, tgt, range(14), h, samples=6)
+    by = {}
+    for s, e in errs:
+        by.setdefault(s, []).append(e)
+    for s, es in by.items():
+        assert np.median(es) < 1e-3, (s, es)
+        assert max(es) < 2e-2, (s, es)
+
+
+def test_fd_gradcheck_sh_f64(oracle_lib):
+    """gradCheckSH (gradcheck.js:192-264): degree-3 view dependence, SH
+    coefficient gradients and the position-through-direction term."""
+    params, cam_d, tgt, o = _rig_stage(3)
+    sh = scenes.sh_rig_coeffs(params.shape[0], 3)
+    cam = O.make_cam(cam_d, shActiveDeg=3)
+    st = O.Stage(o, "f64")
+    errs = _fd_check(st, cam, params, sh, tgt, [0, 5, 14, 20, 31, 44], 1e-5, samples=4, sh_mode=True)
+    es = [e for _, e in errs]
+    assert np.median(es) < 1e-3 and max(es) < 2e-2, errs
+    h = [2e-6, 2e-6, 2e-6]
+    errs = _fd_check(st, cam, params, sh, tgt, [0, 1, 2], h, samples=6)
+    es = [e for _, e in errs]
+    assert np.median(es) < 1e-3 and max(es) < 2e-2, errs
+
+
+def test_fixed_point_matches_float_sums(oracle_lib):
+    """The reference accumulates x16384 truncated i32 (shaders.js:529-541):
+    decoded sums differ from exact sums by < 1/16384 per contribution."""
+    params, cam_d, tgt, o = _rig_stage(0)
+    cam = O.make_cam(cam_d)
+    st = O.Stage(o, "f32")
+    proj, rect, cnt = st.project(cam, params, None)
+    ids, ts, _ = st.bin_sort(cam, proj, rect, cnt)
+    img, T, end = st.render_fwd(cam, proj, ids, ts)
+    gC, valid, _ = st.loss_charb(cam, img, tgt)
+    gfix = st.render_bwd(cam, proj, ids, ts, T, end, gC, valid, fixed=True)
+    gsum = st.render_bwd(cam, proj, ids, ts, T, end, gC, valid, fixed=False)
+    contrib = 64 * 64  # upper bound on contributions per Gaussian
+    assert np.all(np.abs(gfix / 16384.0 - gsum) <= contrib / 16384.0)
+    big = np.abs(gsum) > 1.0
+    np.testing.assert_allclose(gfix[big] / 16384.0, gsum[big], rtol=0.05)
+
+
+def test_f32_twin_close_to_f64(oracle_lib):
+    params, cam_d, tgt, o = _rig_stage(0)
+    cam = O.make_cam(cam_d)
+    g32, _ = O.analytic_grads(O.Stage(o, "f32"), cam, params, None, tgt, fixed=False)
+    g64, _ = O.analytic_grads(O.Stage(o, "f64"), cam, params, None, tgt, fixed=False)
+    sc = np.abs(g64).max(0) + 1e-12
+    assert np.max(np.abs(g32 - g64) / sc) < 1e-3
+
+
+# ---------------------------------------------------------------- Adam
+def test_adam_closed_form(oracle_lib):
+    rng = np.random.default_rng(5)
+    n = 64
+    P = rng.normal(size=(n, 16)).astype(np.float32)
+    P[:, 3:6] = rng.uniform(-6, 0, (n, 3))
+    G = rng.normal(size=(n, 16)).astype(np.float32)
+    G[0, 0] = np.nan
+    G[1, 1] = np.inf
+    m = rng.normal(size=(n, 16)).astype(np.float32) * 0.1
+    v = np.abs(rng.normal(size=(n, 16))).astype(np.float32) * 0.1
+    lr = [3e-4] * 3 + [5e-3] * 3 + [1e-3] * 4 + [1.5e-2] * 3 + [2.5e-2, 0, 0]
+    t = 7
+    hp = O.adam_hp(lr, t=t, minLS=-5.0, maxLS=-0.5)
+    P0, m0, v0 = P.copy(), m.copy(), v.copy()
+    O.adam(hp, P, G, m, v)
+    # textbook Adam in double with the reference's guards and clamps
+    Gd = np.where(np.abs(G) < 1e18, G, 0).astype(np.float64)
+    sg = 1 / (1 + np.exp(-np.clip(P0[:, 13].astype(np.float64), -9, 9)))
+    Gd[:, 13] += 0.01 * sg * (1 - sg)
+    md = 0.9 * m0 + 0.1 * Gd
+    vd = 0.999 * v0 + 0.001 * Gd ** 2
+    upd = np.array(lr)[None] * (md / (1 - 0.9 ** t)) / (np.sqrt(vd / (1 - 0.999 ** t)) + 1e-15)
+    Pd = P0 - upd
+    Pd[:, 3:6] = np.clip(Pd[:, 3:6], -5.0, -0.5)
+    Pd[:, 10:14] = np.clip(Pd[:, 10:14], -8, 8)
+    live = np.array(lr) != 0
+    np.testing.assert_allclose(P[:, live], Pd[:, live], rtol=2e-6, atol=2e-6)
+    np.testing.assert_allclose(m[:, live], md[:, live], rtol=1e-5, atol=1e-7)
+    assert np.all(P[:, ~live] == P0[:, ~live]) and np.all(m[:, ~live] == m0[:, ~live])
+    assert np.isfinite(P).all()
