@@ -1,0 +1,200 @@
+/* splat_gs.h — C ABI of the MI355X-native 3DGS trainer step.
+ *
+ * Drop-in boundary for the reference's GSTrainer (src/gs/trainer.js), reached
+ * through createTrainer() (src/index.js:56-58). Plain pointers and sizes only.
+ * A Node.js N-API addon (splat.js_amd/addon) binds these functions for the
+ * JavaScript shim that keeps the GSTrainer method surface; Python binds them
+ * with ctypes (splat.js_amd/py/gsabi.py).
+ *
+ * Every function returns 0 on success and a negative GS_E* code on failure;
+ * gs_last_error() then holds a message (the JS shim throws it as an Error,
+ * as the reference throws for oversize targets, trainer.js:122-126).
+ *
+ * Threading: one trainer is used from one host thread at a time. Work is
+ * enqueued on the trainer's own HIP stream; gs_step never waits for the GPU
+ * (the reference's queue.submit, trainer.js:458); readbacks synchronise.
+ */
+#ifndef SPLAT_GS_H
+#define SPLAT_GS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GS_ABI_VERSION 1
+
+/* opaque trainer handle: one per GSTrainer instance */
+typedef struct gs_trainer gs_trainer_t;
+
+enum {
+  GS_OK = 0,
+  GS_E_ARG = -1,       /* bad argument or call order */
+  GS_E_HIP = -2,       /* HIP runtime error (device lost, launch failure) */
+  GS_E_NOMEM = -3,     /* device allocation failed */
+  GS_E_LIMIT = -4,     /* a size limit was exceeded (targets, view size) */
+};
+
+enum { GS_LOSS_CHARBONNIER = 0, GS_LOSS_L1_DSSIM = 1 };
+enum { GS_ACCUM_FIXED = 0, GS_ACCUM_F32 = 1 };
+enum { GS_SORT_DEPTH_THEN_TILE = 0, GS_SORT_GLOBAL64 = 1 };
+
+/* Trainer options: the reference's opts object (trainer.js:18-36, :45-49,
+ * :74, :105-107, :302, :319, :338-341), plus the MI355X build's own knobs. */
+typedef struct gs_opts {
+  int32_t size;        /* sizeof(gs_opts); set by gs_default_opts */
+  float eCut;          /* Gaussian exponent cutoff, default 4.5 (shaders.js:44) */
+  float aMin;          /* minimum alpha, default 0.0039 (shaders.js:45) */
+  float radClamp;      /* binning radius clamp, fraction of width, default 1 (shaders.js:46-53) */
+  int32_t shDeg;       /* SH degree 0..3, default 2 (trainer.js:32) */
+  float anisoReg;      /* anisotropy regulariser, default 0.005 (trainer.js:74) */
+  float opacityReg;    /* opacity regulariser, default 0.01 (trainer.js:341) */
+  float shLr;          /* SH learning rate, default 1.25e-4 (trainer.js:302) */
+  float minScale;      /* min scale as a fraction of scene radius, default 1e-4 (trainer.js:338) */
+  int32_t maxIters;    /* schedule horizon, default 60000 (trainer.js:319) */
+  int64_t maxSplats;   /* capacity ceiling, default 600000 (trainer.js:107) */
+  float capMult;       /* capacity = n * capMult, default 4 (trainer.js:106) */
+  int64_t entriesCap;  /* (tile, Gaussian) key budget; 0 = max(12M, 24*maxSplats) (trainer.js:48-49) */
+  int32_t tileGrad;    /* accepted for compatibility (trainer.js:26); both orders give identical sums */
+  float posLr;         /* base position lr; 0 = 3e-4 * sceneRadius (trainer.js:322) */
+  int32_t loss;        /* GS_LOSS_CHARBONNIER (the reference's loss) or GS_LOSS_L1_DSSIM */
+  float ssimLambda;    /* D-SSIM weight for GS_LOSS_L1_DSSIM, default 0.2 */
+  int32_t gradAccum;   /* GS_ACCUM_FIXED (the reference's x16384 i32 sums) or GS_ACCUM_F32 */
+  int32_t sortMode;    /* key ordering strategy; both produce the identical permutation */
+  uint32_t seed;       /* camera-pick RNG seed (the reference uses Math.random, trainer.js:344) */
+  int32_t device;      /* HIP device ordinal, default 0 */
+} gs_opts;
+
+/* One training camera (trainer.js:95-100 cams[], :357-372): R row-major
+ * world-to-camera, t, focal f, principal point, size, target image index,
+ * exposure log-gain g and bias b. */
+typedef struct gs_camera {
+  float R[9];
+  float t[3];
+  float f, cx, cy;
+  int32_t w, h;
+  int32_t imgIdx;
+  float g, b;
+} gs_camera;
+
+/* One target image (trainer.js:98 images[]): tw x th RGB floats in [0, 1],
+ * r < 0 marks an invalid pixel (trainer.js:133). */
+typedef struct gs_image {
+  int32_t tw, th;
+  const float* rgb;
+} gs_image;
+
+/* Fields the reference's callers read off the trainer object
+ * (session.js:263-277, :479-497, :572-604; trainer.js:27, :102-107, :319). */
+typedef struct gs_info {
+  int64_t n, cap, iter;
+  int32_t shDeg, shK, stride, numCams;
+  int32_t lastCam, holdout;
+  int64_t lastKeys;          /* (tile, Gaussian) keys of the last raster */
+  int64_t overflowKeys;      /* keys dropped for the entry budget, cumulative */
+  int64_t entriesCap;
+  int32_t maxTiles;
+  float horizon;
+} gs_info;
+
+/* fills defaults (the reference's values) */
+int gs_default_opts(gs_opts* o);
+
+/* GSTrainer.create(opts) (trainer.js:13-16): creates the HIP device context. */
+int gs_create(const gs_opts* o, gs_trainer_t** out);
+/* device.destroy() (gradcheck.js:184) */
+void gs_destroy(gs_trainer_t* h);
+
+/* GSTrainer.setup(gaussians, cams, images, maxViewW, maxViewH, sceneRadius)
+ * (trainer.js:100-355). params: n x 16 f32 in the reference layout
+ * (shaders.js:14-16); sh: n x 3K f32 channel-major or NULL (zeros).
+ * Copies everything; the caller keeps ownership. */
+int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh,
+             const gs_camera* cams, int32_t ncams, const gs_image* images, int32_t nimages,
+             int32_t maxViewW, int32_t maxViewH, float sceneRadius);
+
+/* GSTrainer.stepOnce() (trainer.js:398-471). cam < 0 picks a camera with the
+ * seeded RNG, skipping the holdout and excluded ones (trainer.js:400-406);
+ * cam >= 0 trains on that camera. Enqueues only. */
+int gs_step(gs_trainer_t* h, int32_t cam);
+
+/* GSTrainer.readLoss() (trainer.js:615-631): mean squared error per valid
+ * pixel-channel since the last call, then resets. *has = 0 is the
+ * reference's null (no pixels seen). */
+int gs_read_loss(gs_trainer_t* h, double* mse, int32_t* has);
+
+/* GSTrainer.renderView(camParams, ctx, trainMode, offset) (trainer.js:634-655):
+ * renders into a caller buffer of w*h*4 f32 (C + T*bg, 1 - T) instead of a
+ * canvas. trainMode = 1 also accumulates the loss against the target pixels
+ * at `offset`, without gradients. */
+int gs_render(gs_trainer_t* h, const gs_camera* cam, int32_t trainMode, int64_t offset,
+              float* rgba);
+
+/* GSTrainer.readGaussians() (trainer.js:797-817): params n x 16 and, when
+ * shDeg > 0, sh n x 3K (either may be NULL). */
+int gs_read_gaussians(gs_trainer_t* h, float* params, float* sh);
+
+/* GSTrainer.evalCamPsnr(ci) (trainer.js:542-579). */
+int gs_eval_cam_psnr(gs_trainer_t* h, int32_t ci, double* psnr);
+
+/* trainer.holdout = ci (trainer.js:309, :396-397); -1 clears. */
+int gs_set_holdout(gs_trainer_t* h, int32_t ci);
+/* trainer.excluded = new Set(cams) (trainer.js:403) */
+int gs_set_excluded(gs_trainer_t* h, const int32_t* cams, int32_t count);
+/* trainer.iter = iter: the reference's callers assign it (session.js resume) */
+int gs_set_iter(gs_trainer_t* h, int64_t iter);
+
+int gs_get_info(gs_trainer_t* h, gs_info* out);
+
+/* device.queue.onSubmittedWorkDone() (session.js:520, :542) */
+int gs_sync(gs_trainer_t* h);
+
+/* message of the last failing call on this trainer ("" if none) */
+const char* gs_last_error(gs_trainer_t* h);
+
+/* ---- inspection (no reference counterpart: the parity tests and the
+ * benchmark read the step's intermediate buffers through these) ---- */
+
+/* Host buffers for one step's intermediates; NULL members are skipped. */
+typedef struct gs_debug_out {
+  float* rec;           /* n x 12 raster records (mx,my,A,B, C,opa*comp,r,g, b,opa,comp,cnorm) */
+  uint32_t* depthKey;   /* n, f32 depth bits or 0xffffffff when culled */
+  uint32_t* rect;       /* n x 4 (tx0, tx1, ty0, ty1) */
+  uint32_t* cnt;        /* n, tiles covered (0 = culled) */
+  uint32_t* ids;        /* keys, Gaussian id per sorted key (capacity entriesCap) */
+  uint64_t* keys;       /* keys, (tile << 32) | depth bits per sorted key */
+  int64_t* numKeys;     /* 1 */
+  uint32_t* tileRanges; /* numTiles x 2 (start, end) */
+  float* img;           /* w*h*4 */
+  float* T;             /* w*h final transmittance */
+  uint32_t* end;        /* w*h one past the last processed key */
+  float* gC;            /* w*h*4 dL/dC (xyz) and valid flag (w) */
+  int32_t* accum;       /* n x 10 screen-space gradient sums (i32 fixed point or f32 bits) */
+  float* gradF;         /* n x 16 parameter gradients (before Adam) */
+  float* shGrad;        /* n x 3K */
+  double* stats;        /* 4: sum |err|^2, loss, valid pixels, dropped keys */
+} gs_debug_out;
+
+/* One gs_step on camera `cam` that also copies the intermediates out
+ * (synchronises). */
+int gs_step_debug(gs_trainer_t* h, int32_t cam, gs_debug_out* out);
+
+/* Per-kernel timing over subsequent steps, measured with HIP events on the
+ * trainer's stream. enable = 0 stops and keeps the totals. */
+typedef struct gs_kernel_time {
+  char name[32];
+  double total_ms;
+  int64_t launches;
+  double bytes;         /* algorithmic HBM bytes summed over the launches */
+} gs_kernel_time;
+int gs_profile(gs_trainer_t* h, int32_t enable);
+int gs_profile_read(gs_trainer_t* h, gs_kernel_time* out, int32_t max, int32_t* count);
+
+/* the trainer's HIP stream (hipStream_t), for callers that time it */
+void* gs_stream(gs_trainer_t* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

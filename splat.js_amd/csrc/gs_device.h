@@ -1,0 +1,217 @@
+// gs_device.h — types and per-Gaussian math shared by the trainer kernels.
+//
+// Every expression here is written in the association the reference's WGSL
+// spells out (a*b*c = (a*b)*c, dot = (x + y) + z), and all HIP sources are
+// compiled with -ffp-contract=off, so single-precision results are
+// reproducible on the host bit for bit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gs_math.h"
+
+namespace gs {
+
+constexpr int kTile = 16;
+constexpr int kBlock = 256;
+constexpr float kFixed = 16384.0f;   // shaders.js:35-39 fixed-point scale
+
+// Per-view constants (the reference's CAM_STRUCT uniform, shaders.js:140-151).
+struct Cam {
+  float R[9];
+  float t[3];
+  float nearp;
+  float f, cx, cy;
+  float gain, bias;
+  float bg[3];
+  float camPos[3];
+  int W, H, tilesX, tilesY, numTiles;
+  int shActiveK;            // active SH rest coefficients (<= K)
+  int64_t tgtOffset;        // first target pixel of this view
+};
+
+struct Cut {
+  float eCut, aMin, radClamp;
+};
+
+// Adam hyperparameters for one step (the reference's AdamU / SHA uniforms,
+// shaders.js:952-1036, filled as trainer.js:315-342, :417-429 do).
+struct AdamHP {
+  float lr[16];
+  float b1, b2, eps, bc1, bc2;
+  float minLS, maxLS, maxLogit, reg;
+  float shLr;
+  float anisoReg;
+};
+
+// Parameters live in four float4 planes (plane c = slots 4c..4c+3 of every
+// Gaussian), so every per-Gaussian load is a fully coalesced 16-byte access.
+__device__ __forceinline__ void load_params(const float4* __restrict__ P, int64_t cap, int64_t i,
+                                            float* p) {
+  const float4 a = P[i], b = P[cap + i], c = P[2 * cap + i], d = P[3 * cap + i];
+  p[0] = a.x; p[1] = a.y; p[2] = a.z; p[3] = a.w;
+  p[4] = b.x; p[5] = b.y; p[6] = b.z; p[7] = b.w;
+  p[8] = c.x; p[9] = c.y; p[10] = c.z; p[11] = c.w;
+  p[12] = d.x; p[13] = d.y; p[14] = d.z; p[15] = d.w;
+}
+
+__device__ __forceinline__ float clampf(float x, float lo, float hi) {
+  return fminf(fmaxf(x, lo), hi);
+}
+
+__device__ __forceinline__ float dot3(const float* a, const float* b) {
+  return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+struct Geom {
+  bool ok;
+  float pc[3], q[4], r0[3], r1[3], r2[3], s[3], t0[3], t1[3];
+  float va, vb, vc, s00, s01, s02, s11, s12, s22;
+};
+
+// shaders.js:176-224 computeGeom
+__device__ __forceinline__ void compute_geom(const Cam& cam, const float* P, Geom& g) {
+  g.ok = false;
+  const float p[3] = {P[0], P[1], P[2]};
+  g.pc[0] = dot3(cam.R + 0, p) + cam.t[0];
+  g.pc[1] = dot3(cam.R + 3, p) + cam.t[1];
+  g.pc[2] = dot3(cam.R + 6, p) + cam.t[2];
+  if (g.pc[2] < cam.nearp) return;
+
+  float q[4] = {P[6], P[7], P[8], P[9]};
+  const float ql = sqrtf(((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3]);
+  if (ql < 1e-6f) { q[0] = 1.0f; q[1] = 0.0f; q[2] = 0.0f; q[3] = 0.0f; }
+  else { q[0] = q[0] / ql; q[1] = q[1] / ql; q[2] = q[2] / ql; q[3] = q[3] / ql; }
+  g.q[0] = q[0]; g.q[1] = q[1]; g.q[2] = q[2]; g.q[3] = q[3];
+  const float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
+  g.r0[0] = 1.0f - 2.0f * (qy * qy + qz * qz);
+  g.r0[1] = 2.0f * (qx * qy - qw * qz);
+  g.r0[2] = 2.0f * (qx * qz + qw * qy);
+  g.r1[0] = 2.0f * (qx * qy + qw * qz);
+  g.r1[1] = 1.0f - 2.0f * (qx * qx + qz * qz);
+  g.r1[2] = 2.0f * (qy * qz - qw * qx);
+  g.r2[0] = 2.0f * (qx * qz - qw * qy);
+  g.r2[1] = 2.0f * (qy * qz + qw * qx);
+  g.r2[2] = 1.0f - 2.0f * (qx * qx + qy * qy);
+
+  for (int k = 0; k < 3; k++) g.s[k] = gs_expf(clampf(P[3 + k], -12.0f, 6.0f));
+
+  float m0[3], m1[3], m2[3];
+  for (int k = 0; k < 3; k++) {
+    m0[k] = g.r0[k] * g.s[k];
+    m1[k] = g.r1[k] * g.s[k];
+    m2[k] = g.r2[k] * g.s[k];
+  }
+  g.s00 = dot3(m0, m0); g.s01 = dot3(m0, m1); g.s02 = dot3(m0, m2);
+  g.s11 = dot3(m1, m1); g.s12 = dot3(m1, m2); g.s22 = dot3(m2, m2);
+
+  const float f = cam.f;
+  const float iz = 1.0f / g.pc[2];
+  const float a = f * iz;
+  const float bx = ((-f) * g.pc[0] * iz) * iz;
+  const float by = ((-f) * g.pc[1] * iz) * iz;
+  for (int k = 0; k < 3; k++) {
+    g.t0[k] = a * cam.R[0 + k] + bx * cam.R[6 + k];
+    g.t1[k] = a * cam.R[3 + k] + by * cam.R[6 + k];
+  }
+  float st0[3], st1[3];
+  st0[0] = (g.s00 * g.t0[0] + g.s01 * g.t0[1]) + g.s02 * g.t0[2];
+  st0[1] = (g.s01 * g.t0[0] + g.s11 * g.t0[1]) + g.s12 * g.t0[2];
+  st0[2] = (g.s02 * g.t0[0] + g.s12 * g.t0[1]) + g.s22 * g.t0[2];
+  st1[0] = (g.s00 * g.t1[0] + g.s01 * g.t1[1]) + g.s02 * g.t1[2];
+  st1[1] = (g.s01 * g.t1[0] + g.s11 * g.t1[1]) + g.s12 * g.t1[2];
+  st1[2] = (g.s02 * g.t1[0] + g.s12 * g.t1[1]) + g.s22 * g.t1[2];
+  g.va = fmaxf(dot3(g.t0, st0), 0.0f);
+  g.vb = dot3(g.t1, st0);
+  g.vc = fmaxf(dot3(g.t1, st1), 0.0f);
+  g.ok = true;
+}
+
+// shaders.js:674-678 (render) / :736-740 (chain): largest eigenvalue of the
+// dilated 2D covariance; conic gradients are accumulated times 1/(1+lmax).
+__device__ __forceinline__ float conic_lmax(float va, float vb, float vc) {
+  const float cva = va + 0.3f, cvc = vc + 0.3f;
+  const float cmid = 0.5f * (cva + cvc);
+  return cmid + sqrtf(fmaxf(cmid * cmid - (cva * cvc - vb * vb), 0.0f));
+}
+
+// shaders.js:56-111 real SH basis (degrees 1..3) and dY/dv; D is K x 3.
+// Y[14] follows shaders.js:102 (x * (xx - yy)).
+template <int kDeg, bool kGrad>
+__device__ __forceinline__ void sh_basis(const float* v, float* Y, float* D) {
+  const float x = v[0], y = v[1], z = v[2];
+  const float xx = x * x, yy = y * y, zz = z * z;
+  const float C1 = 0.4886025119029199f;
+  const float C20 = 1.0925484305920792f, C21 = -1.0925484305920792f, C22 = 0.31539156525252005f,
+              C23 = -1.0925484305920792f, C24 = 0.5462742152960396f;
+  const float C30 = -0.5900435899266435f, C31 = 2.890611442640554f, C32 = -0.4570457994644658f,
+              C33 = 0.3731763325901154f, C34 = -0.4570457994644658f, C35 = 1.445305721320277f,
+              C36 = -0.5900435899266435f;
+  Y[0] = (-C1) * y; Y[1] = C1 * z; Y[2] = (-C1) * x;
+  if (kGrad) {
+    D[0] = 0.0f; D[1] = -C1; D[2] = 0.0f;
+    D[3] = 0.0f; D[4] = 0.0f; D[5] = C1;
+    D[6] = -C1; D[7] = 0.0f; D[8] = 0.0f;
+  }
+  if (kDeg >= 2) {
+    Y[3] = C20 * x * y;
+    Y[4] = C21 * y * z;
+    Y[5] = C22 * ((2.0f * zz - xx) - yy);
+    Y[6] = C23 * x * z;
+    Y[7] = C24 * (xx - yy);
+    if (kGrad) {
+      D[9] = C20 * y; D[10] = C20 * x; D[11] = 0.0f;
+      D[12] = 0.0f; D[13] = C21 * z; D[14] = C21 * y;
+      D[15] = C22 * (-2.0f * x); D[16] = C22 * (-2.0f * y); D[17] = C22 * (4.0f * z);
+      D[18] = C23 * z; D[19] = 0.0f; D[20] = C23 * x;
+      D[21] = C24 * (2.0f * x); D[22] = C24 * (-2.0f * y); D[23] = 0.0f;
+    }
+  }
+  if (kDeg >= 3) {
+    Y[8] = C30 * y * (3.0f * xx - yy);
+    Y[9] = C31 * x * y * z;
+    Y[10] = C32 * y * ((4.0f * zz - xx) - yy);
+    Y[11] = C33 * z * ((2.0f * zz - 3.0f * xx) - 3.0f * yy);
+    Y[12] = C34 * x * ((4.0f * zz - xx) - yy);
+    Y[13] = C35 * z * (xx - yy);
+    Y[14] = C36 * x * (xx - yy);
+    if (kGrad) {
+      D[24] = C30 * (6.0f * x * y); D[25] = C30 * (3.0f * xx - 3.0f * yy); D[26] = 0.0f;
+      D[27] = C31 * (y * z); D[28] = C31 * (x * z); D[29] = C31 * (x * y);
+      D[30] = C32 * (-2.0f * x * y); D[31] = C32 * ((4.0f * zz - xx) - 3.0f * yy); D[32] = C32 * (8.0f * y * z);
+      D[33] = C33 * (-6.0f * x * z); D[34] = C33 * (-6.0f * y * z); D[35] = C33 * ((6.0f * zz - 3.0f * xx) - 3.0f * yy);
+      D[36] = C34 * ((4.0f * zz - 3.0f * xx) - yy); D[37] = C34 * (-2.0f * x * y); D[38] = C34 * (8.0f * x * z);
+      D[39] = C35 * (2.0f * x * z); D[40] = C35 * (-2.0f * y * z); D[41] = C35 * (xx - yy);
+      D[42] = C36 * ((3.0f * xx) - yy); D[43] = C36 * (-2.0f * x * y); D[44] = 0.0f;
+    }
+  }
+}
+
+template <int kDeg>
+struct ShDims {
+  static constexpr int K = kDeg > 0 ? (kDeg + 1) * (kDeg + 1) - 1 : 0;
+  static constexpr int C = 3 * K;               // coefficients per Gaussian
+  static constexpr int P4 = (C + 3) / 4;        // float4 planes
+};
+
+// view direction from the camera (shaders.js:284-285, :914-916)
+__device__ __forceinline__ float view_dir(const Cam& cam, const float* P, float* v) {
+  const float un[3] = {P[0] - cam.camPos[0], P[1] - cam.camPos[1], P[2] - cam.camPos[2]};
+  const float ul = fmaxf(sqrtf(dot3(un, un)), 1e-9f);
+  v[0] = un[0] / ul; v[1] = un[1] / ul; v[2] = un[2] / ul;
+  return ul;
+}
+
+// Gaussian i's SH coefficient c lives in plane c/4, lane c%4.
+template <int kDeg>
+__device__ __forceinline__ void load_sh(const float4* __restrict__ SH, int64_t cap, int64_t i,
+                                        float* sh) {
+#pragma unroll
+  for (int c = 0; c < ShDims<kDeg>::P4; c++) {
+    const float4 v = SH[c * cap + i];
+    sh[4 * c + 0] = v.x; sh[4 * c + 1] = v.y; sh[4 * c + 2] = v.z; sh[4 * c + 3] = v.w;
+  }
+}
+
+}  // namespace gs

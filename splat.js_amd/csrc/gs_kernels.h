@@ -1,0 +1,84 @@
+// gs_kernels.h — host-side launchers of the trainer kernels (internal).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "gs_device.h"
+
+namespace gs {
+
+struct SortScratch {
+  int G;                  // blocks per radix pass (fixed grid)
+  uint32_t* hist;         // 256 x G
+  uint32_t* totals;       // 256
+  uint64_t* partials;     // G
+};
+
+struct BinArgs {
+  int mode;               // 0 depth-then-tile, 1 global64
+  int tilesX, numTiles;
+  uint64_t cap;           // key budget
+  const uint32_t* dN;     // device: number of Gaussians
+  const uint32_t* cnt;
+  const uint2* rect;
+  const uint32_t* depthKey;
+  uint32_t *dkA, *dvA, *dkB, *dvB;   // n-sized depth-sort buffers
+  void *tkA, *tkB;                    // key buffers (4 or 8 B per key)
+  uint32_t *tvA, *tvB;                // id buffers
+  uint32_t* numKeys;      // device: min(total, cap)
+  uint64_t* totalKeys;    // device: keys before the budget
+  uint2* ranges;          // numTiles
+  double* stats;          // [3] += dropped keys
+  SortScratch scratch;
+};
+
+struct BinOut {
+  const void* keys;
+  const uint32_t* ids;
+  int keyBytes;
+};
+
+struct RasterBufs {
+  const uint2* ranges;
+  const uint32_t* ids;
+  const float4* rec;
+  float4* img;
+  float* T;
+  uint32_t* end;
+  float4* gC;             // dL/dC, w = valid
+  const uint32_t* tgt;    // packed RGBA8 targets (all views)
+  double* stats;          // [0] sum |err|^2, [1] loss, [2] valid pixels
+};
+
+struct UpdateBufs {
+  int64_t n, cap;
+  float4 *P, *M, *V;      // 4 planes each
+  float4 *SH, *SHM, *SHV; // ShDims::P4 planes each
+  void* accum;            // n x 12 (10 used) i32 or f32
+  const uint32_t* cnt;
+  float* dbgGradF;        // optional n x 16
+  float* dbgShGrad;       // optional n x 3K
+};
+
+void launch_fill_u32(hipStream_t st, uint32_t* p, int64_t count, uint32_t v);
+
+void launch_project(hipStream_t st, int deg, int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
+                    const float4* P, const float4* SH, float4* rec, uint32_t* depthKey, uint2* rect,
+                    uint32_t* cnt);
+
+int tile_bits(int numTiles);
+void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out);
+
+// loss: 0 none (view render), 1 Charbonnier fused into the forward pass,
+// 2 forward only, loss computed by a separate kernel
+void launch_raster_fwd(hipStream_t st, const Cam& cam, const Cut& cut, const RasterBufs& b, int loss);
+void launch_loss_l1ssim(hipStream_t st, const Cam& cam, float lambda, const RasterBufs& b,
+                        float* scratch);
+void launch_raster_bwd(hipStream_t st, const Cam& cam, const Cut& cut, const RasterBufs& b,
+                       int accumMode, void* accum);
+
+void launch_update(hipStream_t st, int deg, int accumMode, const Cam& cam, const AdamHP& hp,
+                   const UpdateBufs& u);
+
+}  // namespace gs

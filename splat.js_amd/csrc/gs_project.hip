@@ -1,0 +1,106 @@
+// gs_project.hip — per-Gaussian projection (the reference's `project` pass,
+// shaders.js:228-315): EWA 3D->2D covariance, Mip-Splatting opacity
+// compensation, SH colour, opacity-aware radius, tile rectangle.
+//
+// Output per visible Gaussian:
+//   rec[3i..3i+2]  raster record, 48 B: (mx, my, A, B), (C, opa*comp, r, g),
+//                  (b, opa, comp, cnorm) — exactly what the raster passes read
+//   depthKey[i]    f32 depth bits (positive, so they order like the depths);
+//                  0xffffffff when culled, which sorts culled Gaussians last
+//   rect[i]        (tx0 | tx1 << 16, ty0 | ty1 << 16)
+//   cnt[i]         tiles covered, 0 when culled
+// HBM-bound: 56-64 B of params + 4*3K B of SH read, 60 B written per Gaussian.
+#include "gs_kernels.h"
+
+namespace gs {
+
+template <int kDeg>
+__global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam cam, Cut cut,
+                                                    const float4* __restrict__ P,
+                                                    const float4* __restrict__ SH,
+                                                    float4* __restrict__ rec,
+                                                    uint32_t* __restrict__ depthKey,
+                                                    uint2* __restrict__ rect,
+                                                    uint32_t* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  float p[16];
+  load_params(P, cap, i, p);
+  Geom g;
+  compute_geom(cam, p, g);
+  uint32_t key = 0xffffffffu, c = 0;
+  uint2 rc = make_uint2(0, 0);
+  while (g.ok) {  // single pass; `break` = culled (shaders.js:243-266)
+    const float detV = fmaxf(g.va * g.vc - g.vb * g.vb, 0.0f);
+    const float ad = g.va + 0.3f;
+    const float cd = g.vc + 0.3f;
+    const float detVd = ad * cd - g.vb * g.vb;
+    if (detVd < 1e-8f) break;
+    const float comp = sqrtf(fmaxf(detV / detVd, 0.0f));
+    const float opa = gs_sigmoid_clamped(p[13]);
+    if (opa * comp < cut.aMin) break;
+    const float f = cam.f;
+    const float mx = f * g.pc[0] / g.pc[2] + cam.cx;
+    const float my = f * g.pc[1] / g.pc[2] + cam.cy;
+    const float mid = 0.5f * (ad + cd);
+    const float disc = sqrtf(fmaxf(mid * mid - detVd, 0.0f));
+    const float eMax = fminf(cut.eCut, gs_logf(fmaxf(opa * comp / cut.aMin, 1.0001f)));
+    const float W = (float)cam.W, H = (float)cam.H;
+    const float rad = fminf(sqrtf(2.0f * eMax) * sqrtf(mid + disc), cut.radClamp * W);
+    if (mx + rad < 0.0f || my + rad < 0.0f || mx - rad > W || my - rad > H) break;
+    const float inv = 1.0f / detVd;
+    float col[3] = {gs_sigmoid_clamped(p[10]), gs_sigmoid_clamped(p[11]), gs_sigmoid_clamped(p[12])};
+    if constexpr (kDeg > 0) {
+      constexpr int K = ShDims<kDeg>::K;
+      float v[3];
+      view_dir(cam, p, v);
+      float Y[15];
+      sh_basis<kDeg, false>(v, Y, nullptr);
+      float sh[4 * ShDims<kDeg>::P4];
+      load_sh<kDeg>(SH, cap, i, sh);
+      const int aK = cam.shActiveK;
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        if (k < aK) {
+          col[0] = col[0] + sh[k] * Y[k];
+          col[1] = col[1] + sh[K + k] * Y[k];
+          col[2] = col[2] + sh[2 * K + k] * Y[k];
+        }
+      }
+      col[0] = fmaxf(col[0], 0.0f);
+      col[1] = fmaxf(col[1], 0.0f);
+      col[2] = fmaxf(col[2], 0.0f);
+    }
+    const float cnorm = 1.0f / (1.0f + conic_lmax(g.va, g.vb, g.vc));
+    rec[3 * i + 0] = make_float4(mx, my, cd * inv, (-g.vb) * inv);
+    rec[3 * i + 1] = make_float4(ad * inv, opa * comp, col[0], col[1]);
+    rec[3 * i + 2] = make_float4(col[2], opa, comp, cnorm);
+    key = gs_f2u(g.pc[2]);
+    const float TF = 16.0f;
+    const int tx0 = (int)clampf(floorf((mx - rad) / TF), 0.0f, (float)(cam.tilesX - 1));
+    const int tx1 = (int)clampf(floorf((mx + rad) / TF), 0.0f, (float)(cam.tilesX - 1));
+    const int ty0 = (int)clampf(floorf((my - rad) / TF), 0.0f, (float)(cam.tilesY - 1));
+    const int ty1 = (int)clampf(floorf((my + rad) / TF), 0.0f, (float)(cam.tilesY - 1));
+    rc = make_uint2((uint32_t)tx0 | ((uint32_t)tx1 << 16), (uint32_t)ty0 | ((uint32_t)ty1 << 16));
+    c = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
+    break;
+  }
+  depthKey[i] = key;
+  rect[i] = rc;
+  cnt[i] = c;
+}
+
+void launch_project(hipStream_t st, int deg, int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
+                    const float4* P, const float4* SH, float4* rec, uint32_t* depthKey, uint2* rect,
+                    uint32_t* cnt) {
+  if (n <= 0) return;
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+  switch (deg) {
+    case 0: k_project<0><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
+    case 1: k_project<1><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
+    case 2: k_project<2><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
+    default: k_project<3><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
+  }
+}
+
+}  // namespace gs

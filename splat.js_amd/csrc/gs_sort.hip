@@ -1,0 +1,386 @@
+// gs_sort.hip — global binning: one key per (Gaussian, covered tile),
+// ordered by (tile, depth, Gaussian id), the total order the reference's
+// per-tile bitonic sort produces (shaders.js:418-500, tie-break :454).
+//
+// Two strategies, identical output:
+//  * depth-then-tile (default): stable LSD radix sort of the n Gaussians by
+//    depth bits (emitted in id order, so ties keep id order), then keys are
+//    emitted in that order and stably sorted by tile id only
+//    (ceil(tileBits/8) passes). Sorting Gaussians instead of keys moves the
+//    32 depth bits off the long key array.
+//  * global64: keys (tile << 32 | depth bits) emitted in id order and sorted
+//    by a stable LSD radix sort over all 32 + tileBits bits.
+//
+// Every radix pass is reduce-then-scan with no inter-block waiting:
+//   hist    per-block digit counts; wave-level ballot matching gives each
+//           wave's digit groups, so each LDS counter takes one add per group
+//   scan    one block per digit scans that digit's per-block counts and adds
+//           the digit's base (an exclusive scan of the 256 digit totals)
+//   scatter per 256-key round: ballot matching ranks keys within their wave,
+//           per-wave counts give each wave's offset, so the pass is stable
+// Key counts that only exist on the device (the emitted key total) are read
+// from device memory by every kernel; the grids are fixed.
+#include "gs_kernels.h"
+
+namespace gs {
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  return (1ull << lane_id()) - 1ull;
+}
+
+// mask of active lanes whose 8-bit digit equals this lane's
+__device__ __forceinline__ uint64_t match_digit8(uint32_t d, uint64_t active) {
+  uint64_t m = active;
+#pragma unroll
+  for (int b = 0; b < 8; b++) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bal = __ballot(bit);
+    m &= bit ? bal : ~bal;
+  }
+  return m;
+}
+
+__device__ __forceinline__ void chunk_range(uint32_t N, int G, uint32_t& lo, uint32_t& hi) {
+  const uint32_t per = (uint32_t)(((uint64_t)N + G - 1) / G);
+  const uint32_t chunk = (per + 255u) & ~255u;
+  const uint64_t l = (uint64_t)blockIdx.x * chunk;
+  lo = (uint32_t)(l < N ? l : N);
+  const uint64_t h = (uint64_t)lo + chunk;
+  hi = (uint32_t)(h < N ? h : N);
+}
+
+__global__ void k_fill_u32(uint32_t* p, int64_t count, uint32_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) p[i] = v;
+}
+
+template <typename KeyT>
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const KeyT* __restrict__ keys,
+                                                       const uint32_t* __restrict__ dN, int shift,
+                                                       int G, uint32_t* __restrict__ hist,
+                                                       uint32_t* __restrict__ totals) {
+  __shared__ uint32_t cnt[256];
+  const int tid = threadIdx.x;
+  cnt[tid] = 0;
+  __syncthreads();
+  uint32_t lo, hi;
+  chunk_range(*dN, G, lo, hi);
+  for (uint32_t r = lo; r < hi; r += kBlock) {
+    const uint32_t i = r + tid;
+    const bool ok = i < hi;
+    const uint32_t d = ok ? (uint32_t)(keys[i] >> shift) & 0xffu : 0u;
+    const uint64_t m = match_digit8(d, __ballot(ok));
+    if (ok && (m & lanemask_lt()) == 0) atomicAdd(&cnt[d], (uint32_t)__popcll(m));
+  }
+  __syncthreads();
+  const uint32_t c = cnt[tid];
+  hist[tid * G + blockIdx.x] = c;
+  if (c) atomicAdd(&totals[tid], c);
+}
+
+// one block per digit: exclusive scan of hist[d][0..G) plus the digit's base
+__global__ __launch_bounds__(kBlock) void k_radix_scan(uint32_t* __restrict__ hist, int G,
+                                                       const uint32_t* __restrict__ totals) {
+  __shared__ uint32_t wsum[kBlock / 64];
+  __shared__ uint32_t sbase;
+  const int d = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // digit base: sum of totals[0..d)
+  {
+    uint32_t v = tid < d ? totals[tid] : 0u;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) wsum[w] = v;
+    __syncthreads();
+    if (tid == 0) sbase = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+  }
+  uint32_t carry = sbase;
+  uint32_t* row = hist + (int64_t)d * G;
+  for (int b0 = 0; b0 < G; b0 += kBlock) {
+    const int b = b0 + tid;
+    const uint32_t v = b < G ? row[b] : 0u;
+    uint32_t x = v;  // inclusive wave scan
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    __syncthreads();
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t wpre = 0;
+    for (int k = 0; k < w; k++) wpre += wsum[k];
+    if (b < G) row[b] = carry + wpre + x - v;
+    carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  }
+}
+
+template <typename KeyT, bool kIota, bool kWriteKeys>
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const KeyT* __restrict__ keysIn,
+                                                          const uint32_t* __restrict__ valsIn,
+                                                          const uint32_t* __restrict__ dN,
+                                                          int shift, int G,
+                                                          const uint32_t* __restrict__ scan,
+                                                          KeyT* __restrict__ keysOut,
+                                                          uint32_t* __restrict__ valsOut) {
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wcnt[4][256];
+  __shared__ uint32_t wpre[4][256];
+  const int tid = threadIdx.x, w = tid >> 6;
+  uint32_t lo, hi;
+  chunk_range(*dN, G, lo, hi);
+  base[tid] = scan[tid * G + blockIdx.x];
+  wcnt[0][tid] = wcnt[1][tid] = wcnt[2][tid] = wcnt[3][tid] = 0;
+  __syncthreads();
+  for (uint32_t r = lo; r < hi; r += kBlock) {
+    const uint32_t i = r + tid;
+    const bool ok = i < hi;
+    const KeyT key = ok ? keysIn[i] : (KeyT)0;
+    const uint32_t val = ok ? (kIota ? i : valsIn[i]) : 0u;
+    const uint32_t d = (uint32_t)(key >> shift) & 0xffu;
+    const uint64_t m = match_digit8(d, __ballot(ok));
+    const uint32_t rank = (uint32_t)__popcll(m & lanemask_lt());
+    if (ok && rank == 0) wcnt[w][d] = (uint32_t)__popcll(m);
+    __syncthreads();
+    {
+      const uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
+      const uint32_t b = base[tid];
+      wpre[0][tid] = b;
+      wpre[1][tid] = b + c0;
+      wpre[2][tid] = b + c0 + c1;
+      wpre[3][tid] = b + c0 + c1 + c2;
+      base[tid] = b + c0 + c1 + c2 + c3;
+      wcnt[0][tid] = wcnt[1][tid] = wcnt[2][tid] = wcnt[3][tid] = 0;
+    }
+    __syncthreads();
+    if (ok) {
+      const uint32_t pos = wpre[w][d] + rank;
+      if (kWriteKeys) keysOut[pos] = key;
+      valsOut[pos] = val;
+    }
+  }
+}
+
+// ---- emission offsets: exclusive scan of cnt[] in sorted order ----
+
+__device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total, uint32_t* wsum) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  uint32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  __syncthreads();
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  uint32_t pre = 0;
+  for (int k = 0; k < w; k++) pre += wsum[k];
+  total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  return pre + x - v;
+}
+
+// per-block key counts over Gaussians in (depth) order `order`
+// (order == nullptr: identity order)
+__global__ __launch_bounds__(kBlock) void k_count_partials(const uint32_t* __restrict__ order,
+                                                           const uint32_t* __restrict__ cnt,
+                                                           const uint32_t* __restrict__ dN, int G,
+                                                           uint64_t* __restrict__ partials) {
+  __shared__ uint64_t wsum[4];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  uint32_t lo, hi;
+  chunk_range(*dN, G, lo, hi);
+  uint64_t acc = 0;
+  for (uint32_t r = lo + tid; r < hi; r += kBlock) acc += cnt[order ? order[r] : r];
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+  if (lane == 0) wsum[w] = acc;
+  __syncthreads();
+  if (tid == 0) partials[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+// single block: exclusive scan of the G partials; numKeys = min(total, cap)
+__global__ __launch_bounds__(1024) void k_count_top(uint64_t* __restrict__ partials, int G,
+                                                    uint64_t cap, uint32_t* __restrict__ numKeys,
+                                                    uint64_t* __restrict__ totalKeys,
+                                                    double* __restrict__ stats) {
+  __shared__ uint64_t wsum[16];
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int per = (G + 1023) / 1024;
+  uint64_t mine = 0;
+  for (int k = 0; k < per; k++) {
+    const int b = tid * per + k;
+    if (b < G) mine += partials[b];
+  }
+  uint64_t x = mine;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  uint64_t pre = 0;
+  for (int k = 0; k < w; k++) pre += wsum[k];
+  uint64_t run = pre + x - mine;
+  for (int k = 0; k < per; k++) {
+    const int b = tid * per + k;
+    if (b < G) {
+      const uint64_t v = partials[b];
+      partials[b] = run;
+      run += v;
+    }
+  }
+  if (tid == 1023) {
+    const uint64_t total = pre + x;
+    *totalKeys = total;
+    *numKeys = (uint32_t)(total < cap ? total : cap);
+    if (total > cap) stats[3] += (double)(total - cap);
+  }
+}
+
+// Emits one key per covered tile. kGlobal64: Gaussians in id order, key =
+// (tile << 32) | depth bits. Otherwise Gaussians in depth order, key = tile.
+// Keys past the budget `cap` are dropped (counted by k_count_top).
+template <bool kGlobal64>
+__global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ order,
+                                                 const uint32_t* __restrict__ cnt,
+                                                 const uint2* __restrict__ rect,
+                                                 const uint32_t* __restrict__ depthKey,
+                                                 const uint32_t* __restrict__ dN, int G,
+                                                 const uint64_t* __restrict__ partials, int tilesX,
+                                                 uint64_t cap, void* __restrict__ keysOut,
+                                                 uint32_t* __restrict__ valsOut) {
+  __shared__ uint32_t wsum[4];
+  uint32_t lo, hi;
+  chunk_range(*dN, G, lo, hi);
+  uint64_t run = partials[blockIdx.x];
+  for (uint32_t r0 = lo; r0 < hi; r0 += kBlock) {
+    const uint32_t r = r0 + threadIdx.x;
+    const bool ok = r < hi;
+    const uint32_t id = ok ? (kGlobal64 ? r : order[r]) : 0u;
+    const uint32_t c = ok ? cnt[id] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan(c, tot, wsum);
+    if (c) {
+      uint64_t pos = run + ex;
+      const uint2 rc = rect[id];
+      const uint32_t tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu, ty1 = rc.y >> 16;
+      const uint32_t dk = kGlobal64 ? depthKey[id] : 0u;
+      for (uint32_t ty = ty0; ty <= ty1; ty++) {
+        for (uint32_t tx = tx0; tx <= tx1; tx++, pos++) {
+          if (pos >= cap) break;
+          const uint32_t tile = ty * (uint32_t)tilesX + tx;
+          if (kGlobal64) ((uint64_t*)keysOut)[pos] = ((uint64_t)tile << 32) | dk;
+          else ((uint32_t*)keysOut)[pos] = tile;
+          valsOut[pos] = id;
+        }
+      }
+    }
+    run += tot;
+  }
+}
+
+// tile ranges from the sorted keys: ranges[t] = (first, one past last)
+template <typename KeyT>
+__global__ void k_tile_ranges(const KeyT* __restrict__ keys, const uint32_t* __restrict__ dN,
+                              int shift, uint2* __restrict__ ranges) {
+  const uint32_t N = *dN;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
+    const uint32_t t = (uint32_t)(keys[k] >> shift);
+    if (k == 0 || (uint32_t)(keys[k - 1] >> shift) != t) ranges[t].x = k;
+    if (k == N - 1 || (uint32_t)(keys[k + 1] >> shift) != t) ranges[t].y = k + 1;
+  }
+}
+
+// ---- host launchers ----
+
+void launch_fill_u32(hipStream_t st, uint32_t* p, int64_t count, uint32_t v) {
+  if (count <= 0) return;
+  k_fill_u32<<<(unsigned)((count + 255) / 256), 256, 0, st>>>(p, count, v);
+}
+
+template <typename KeyT>
+static void radix_pass(hipStream_t st, const SortScratch& s, const KeyT* kin, const uint32_t* vin,
+                       KeyT* kout, uint32_t* vout, const uint32_t* dN, int shift, bool iota,
+                       bool writeKeys) {
+  launch_fill_u32(st, s.totals, 256, 0u);
+  k_radix_hist<KeyT><<<s.G, kBlock, 0, st>>>(kin, dN, shift, s.G, s.hist, s.totals);
+  k_radix_scan<<<256, kBlock, 0, st>>>(s.hist, s.G, s.totals);
+  if (iota) {
+    if (writeKeys) k_radix_scatter<KeyT, true, true><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout);
+    else k_radix_scatter<KeyT, true, false><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout);
+  } else {
+    if (writeKeys) k_radix_scatter<KeyT, false, true><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout);
+    else k_radix_scatter<KeyT, false, false><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout);
+  }
+}
+
+// Stable LSD sort of (key, val) pairs by key bits [bit0, bit1), 8 per pass.
+// Pass p reads the previous pass's output (the input for p = 0; vals = iota
+// when vin is null) and writes (k1, v1) for even p, (k2, v2) for odd p. The
+// input may alias (k2, v2): it is consumed by the first pass.
+template <typename KeyT>
+static void radix_sort(hipStream_t st, const SortScratch& s, const KeyT* kin, const uint32_t* vin,
+                       KeyT* k1, uint32_t* v1, KeyT* k2, uint32_t* v2, const uint32_t* dN,
+                       int bit0, int bit1, const KeyT** kres, const uint32_t** vres) {
+  const KeyT* ki = kin;
+  const uint32_t* vi = vin;
+  bool odd = false;
+  for (int b = bit0; b < bit1; b += 8) {
+    KeyT* ko = odd ? k2 : k1;
+    uint32_t* vo = odd ? v2 : v1;
+    radix_pass<KeyT>(st, s, ki, vi, ko, vo, dN, b, vi == nullptr, true);
+    ki = ko;
+    vi = vo;
+    odd = !odd;
+  }
+  *kres = ki;
+  *vres = vi;
+}
+
+int tile_bits(int numTiles) {
+  int b = 0;
+  while ((1 << b) < numTiles) b++;
+  return b < 1 ? 1 : b;
+}
+
+void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
+  const SortScratch& s = a.scratch;
+  const int tb = tile_bits(a.numTiles);
+  const int tpasses = (tb + 7) / 8;
+  if (a.mode == 0) {
+    // 1) stable sort of the Gaussians by depth bits, values = ids
+    const uint32_t* dk;
+    const uint32_t* order;
+    radix_sort<uint32_t>(st, s, a.depthKey, nullptr, a.dkA, a.dvA, a.dkB, a.dvB, a.dN, 0, 32, &dk, &order);
+    // 2) per-Gaussian key counts in depth order -> offsets -> one key per tile
+    k_count_partials<<<s.G, kBlock, 0, st>>>(order, a.cnt, a.dN, s.G, s.partials);
+    k_count_top<<<1, 1024, 0, st>>>(s.partials, s.G, a.cap, a.numKeys, a.totalKeys, a.stats);
+    k_emit<false><<<s.G, kBlock, 0, st>>>(order, a.cnt, a.rect, a.depthKey, a.dN, s.G, s.partials,
+                                          a.tilesX, a.cap, a.tkA, a.tvA);
+    // 3) stable sort by tile id
+    const uint32_t* tk;
+    const uint32_t* tv;
+    radix_sort<uint32_t>(st, s, (const uint32_t*)a.tkA, a.tvA, (uint32_t*)a.tkB, a.tvB,
+                         (uint32_t*)a.tkA, a.tvA, a.numKeys, 0, 8 * tpasses, &tk, &tv);
+    out.keys = tk;
+    out.ids = tv;
+    out.keyBytes = 4;
+    launch_fill_u32(st, (uint32_t*)a.ranges, 2 * (int64_t)a.numTiles, 0u);
+    k_tile_ranges<uint32_t><<<1024, 256, 0, st>>>(tk, a.numKeys, 0, a.ranges);
+  } else {
+    k_count_partials<<<s.G, kBlock, 0, st>>>(nullptr, a.cnt, a.dN, s.G, s.partials);
+    k_count_top<<<1, 1024, 0, st>>>(s.partials, s.G, a.cap, a.numKeys, a.totalKeys, a.stats);
+    k_emit<true><<<s.G, kBlock, 0, st>>>(nullptr, a.cnt, a.rect, a.depthKey, a.dN, s.G, s.partials,
+                                         a.tilesX, a.cap, a.tkA, a.tvA);
+    const uint64_t* tk;
+    const uint32_t* tv;
+    radix_sort<uint64_t>(st, s, (const uint64_t*)a.tkA, a.tvA, (uint64_t*)a.tkB, a.tvB,
+                         (uint64_t*)a.tkA, a.tvA, a.numKeys, 0, 32 + 8 * tpasses, &tk, &tv);
+    out.keys = tk;
+    out.ids = tv;
+    out.keyBytes = 8;
+    launch_fill_u32(st, (uint32_t*)a.ranges, 2 * (int64_t)a.numTiles, 0u);
+    k_tile_ranges<uint64_t><<<1024, 256, 0, st>>>(tk, a.numKeys, 32, a.ranges);
+  }
+}
+
+}  // namespace gs

@@ -1,0 +1,830 @@
+// gs_trainer.cpp — host side of the trainer: the C ABI in include/splat_gs.h.
+//
+// Mirrors GSTrainer (src/gs/trainer.js): buffer setup and target packing
+// (:100-355), the per-step host sequencing (:398-471), readLoss (:615-631),
+// renderView (:634-655), readGaussians (:797-817), evalCamPsnr (:542-579).
+// All device work goes to one HIP stream owned by the trainer; the step
+// enqueues only.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/splat_gs.h"
+#include "gs_kernels.h"
+
+using namespace gs;
+
+namespace {
+
+constexpr int kSortBlocks = 1024;
+constexpr int64_t kDefaultEntries = 12000000;   // shaders.js:34 ENTRIES_CAP
+constexpr int64_t kTargetLimitBytes = int64_t(64) << 30;
+
+struct CamMeta {
+  gs_camera c;
+  int64_t offset;   // first target pixel
+  float f0;
+  int activeDeg;    // SH degree in this camera's uniform (trainer.js:367, :411)
+};
+
+struct ProfSlot {
+  std::string name;
+  double ms = 0.0;
+  int64_t launches = 0;
+  double bytes = 0.0;
+};
+
+struct PendingEv {
+  int slot;
+  hipEvent_t a, b;
+  double bytes;
+};
+
+uint32_t mulberry32(uint32_t& a) {
+  // src/sfm/geometry.js:751-759
+  a += 0x6D2B79F5u;
+  uint32_t t = (a ^ (a >> 15)) * (1u | a);
+  t = (t + ((t ^ (t >> 7)) * (61u | t))) ^ t;
+  return t ^ (t >> 14);
+}
+
+// Math.round then Math.min(255, .) then ToInt32 (trainer.js:134-137)
+int32_t js_round255(double x) {
+  if (std::isnan(x)) return 0;
+  double r = std::floor(x);
+  if (x - r >= 0.5) r += 1.0;
+  if (r > 255.0) r = 255.0;
+  if (std::isinf(r)) return 0;
+  double m = std::fmod(r, 4294967296.0);
+  if (m < 0) m += 4294967296.0;
+  return (int32_t)(uint32_t)m;
+}
+
+}  // namespace
+
+struct gs_trainer {
+  gs_opts o{};
+  hipStream_t st = nullptr;
+  std::string err;
+  bool ready = false;
+
+  int64_t n = 0, cap = 0;
+  int shDeg = 0, shK = 0, shC = 0, shP4 = 0;
+  int64_t entriesCap = 0;
+  int maxTiles = 0;
+  int64_t maxPix = 0;
+
+  std::vector<void*> allocs;
+  float4 *P = nullptr, *M = nullptr, *V = nullptr, *SH = nullptr, *SHM = nullptr, *SHV = nullptr;
+  float4* rec = nullptr;
+  uint32_t *depthKey = nullptr, *cnt = nullptr;
+  uint2* rect = nullptr;
+  void* accum = nullptr;
+  uint32_t *dkA = nullptr, *dvA = nullptr, *dkB = nullptr, *dvB = nullptr;
+  void *tkA = nullptr, *tkB = nullptr;
+  uint32_t *tvA = nullptr, *tvB = nullptr;
+  SortScratch sort{};
+  uint32_t* dN = nullptr;
+  uint32_t* numKeys = nullptr;
+  uint64_t* totalKeys = nullptr;
+  uint2* ranges = nullptr;
+  float4* img = nullptr;
+  float* Tfin = nullptr;
+  uint32_t* endIdx = nullptr;
+  float4* gC = nullptr;
+  uint32_t* targets = nullptr;
+  int64_t totalPx = 0;
+  double* stats = nullptr;      // training stats [4]
+  double* evalStats = nullptr;  // eval/view stats [4]
+
+  std::vector<CamMeta> cams;
+  int64_t iter = 0;
+  int64_t pixelsSeen = 0;
+  float sceneRadius = 1.0f;
+  double basePosLr = 0.0;
+  double horizon = 60000.0;
+  int holdout = -1;
+  std::vector<char> excluded;
+  uint32_t rng = 0;
+  int lastCam = -1;
+  double overflowTotal = 0.0;
+  BinOut lastBin{};
+
+  bool prof = false;
+  std::vector<ProfSlot> slots;
+  std::vector<PendingEv> pending;
+  std::vector<hipEvent_t> evPool;
+};
+
+namespace {
+
+int fail(gs_trainer* h, int code, const std::string& msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+int hipfail(gs_trainer* h, const char* what, hipError_t e) {
+  return fail(h, GS_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCK(expr)                                   \
+  do {                                                \
+    hipError_t e_ = (expr);                           \
+    if (e_ != hipSuccess) return hipfail(h, #expr, e_); \
+  } while (0)
+
+template <typename T>
+int dalloc(gs_trainer* h, T** p, size_t bytes) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, bytes < 16 ? 16 : bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(h, GS_E_NOMEM, "hipMalloc(" + std::to_string(bytes) + " B) failed: " + hipGetErrorString(e));
+  }
+  h->allocs.push_back(q);
+  *p = (T*)q;
+  return GS_OK;
+}
+
+void free_all(gs_trainer* h) {
+  for (void* p : h->allocs) (void)hipFree(p);
+  h->allocs.clear();
+  h->ready = false;
+}
+
+hipEvent_t ev_get(gs_trainer* h) {
+  if (!h->evPool.empty()) {
+    hipEvent_t e = h->evPool.back();
+    h->evPool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+int prof_slot(gs_trainer* h, const char* name) {
+  for (size_t k = 0; k < h->slots.size(); k++)
+    if (h->slots[k].name == name) return (int)k;
+  h->slots.push_back(ProfSlot{name});
+  return (int)h->slots.size() - 1;
+}
+
+// Brackets one stage with events on the trainer's stream when profiling.
+struct Scope {
+  gs_trainer* h;
+  int slot = -1;
+  hipEvent_t a = nullptr;
+  double bytes;
+  Scope(gs_trainer* h_, const char* name, double bytes_) : h(h_), bytes(bytes_) {
+    if (!h->prof) return;
+    slot = prof_slot(h, name);
+    a = ev_get(h);
+    (void)hipEventRecord(a, h->st);
+  }
+  ~Scope() {
+    if (slot < 0) return;
+    hipEvent_t b = ev_get(h);
+    (void)hipEventRecord(b, h->st);
+    h->pending.push_back(PendingEv{slot, a, b, bytes});
+  }
+};
+
+void prof_collect(gs_trainer* h) {
+  if (h->pending.empty()) return;
+  (void)hipStreamSynchronize(h->st);
+  for (auto& p : h->pending) {
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, p.a, p.b);
+    h->slots[p.slot].ms += ms;
+    h->slots[p.slot].launches += 1;
+    h->slots[p.slot].bytes += p.bytes;
+    h->evPool.push_back(p.a);
+    h->evPool.push_back(p.b);
+  }
+  h->pending.clear();
+}
+
+Cam make_cam(const gs_trainer* h, const gs_camera& c, int trainMode, int64_t offset, int activeDeg) {
+  Cam k{};
+  for (int i = 0; i < 9; i++) k.R[i] = c.R[i];
+  for (int i = 0; i < 3; i++) { k.t[i] = c.t[i]; k.bg[i] = 0.0f; }
+  k.nearp = 0.05f;                       // trainer.js:362
+  k.f = c.f; k.cx = c.cx; k.cy = c.cy;
+  k.gain = (float)std::exp((double)c.g); // trainer.js:363
+  k.bias = c.b;
+  // shaders.js:121-126, in f32
+  const float* R = k.R;
+  const float* t = k.t;
+  k.camPos[0] = -((R[0] * t[0] + R[3] * t[1]) + R[6] * t[2]);
+  k.camPos[1] = -((R[1] * t[0] + R[4] * t[1]) + R[7] * t[2]);
+  k.camPos[2] = -((R[2] * t[0] + R[5] * t[1]) + R[8] * t[2]);
+  k.W = c.w; k.H = c.h;
+  k.tilesX = (c.w + kTile - 1) / kTile;
+  k.tilesY = (c.h + kTile - 1) / kTile;
+  k.numTiles = k.tilesX * k.tilesY;
+  const int a = (activeDeg + 1) * (activeDeg + 1) - 1;
+  k.shActiveK = std::min(a, h->shK);
+  k.tgtOffset = offset;
+  (void)trainMode;
+  return k;
+}
+
+AdamHP make_hp(const gs_trainer* h) {
+  // trainer.js:320-342, :417-424
+  AdamHP a{};
+  const double posLr = h->basePosLr * std::pow(0.01, std::min(1.0, (double)h->iter / (0.75 * h->horizon)));
+  for (int k = 0; k < 3; k++) a.lr[k] = (float)posLr;
+  for (int k = 3; k < 6; k++) a.lr[k] = 5e-3f;
+  for (int k = 6; k < 10; k++) a.lr[k] = 1e-3f;
+  for (int k = 10; k < 13; k++) a.lr[k] = 1.5e-2f;
+  a.lr[13] = 2.5e-2f;
+  a.lr[14] = a.lr[15] = 0.0f;
+  a.b1 = 0.9f; a.b2 = 0.999f; a.eps = 1e-15f;
+  a.bc1 = (float)(1.0 - std::pow((double)0.9f, (double)h->iter));
+  a.bc2 = (float)(1.0 - std::pow((double)0.999f, (double)h->iter));
+  const double r = h->sceneRadius;
+  a.minLS = (float)std::log(r * (double)h->o.minScale);
+  a.maxLS = (float)std::log(r * 0.05);
+  a.maxLogit = 8.0f;
+  a.reg = h->o.opacityReg;
+  a.shLr = h->o.shLr;
+  a.anisoReg = h->o.anisoReg;
+  return a;
+}
+
+Cut make_cut(const gs_trainer* h) {
+  Cut c;
+  c.eCut = h->o.eCut;
+  c.aMin = h->o.aMin;
+  c.radClamp = h->o.radClamp;
+  return c;
+}
+
+// ---- one raster: project -> bin/sort -> forward (+ loss) ----
+int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
+  const int64_t n = h->n;
+  {
+    Scope s(h, "project", (double)n * (56.0 + 4.0 * h->shC + 48.0 + 16.0));
+    launch_project(h->st, h->shDeg, n, h->cap, cam, make_cut(h), h->P, h->SH, h->rec, h->depthKey,
+                   h->rect, h->cnt);
+  }
+  {
+    BinArgs a{};
+    a.mode = h->o.sortMode;
+    a.tilesX = cam.tilesX;
+    a.numTiles = cam.numTiles;
+    a.cap = (uint64_t)h->entriesCap;
+    a.dN = h->dN;
+    a.cnt = h->cnt;
+    a.rect = h->rect;
+    a.depthKey = h->depthKey;
+    a.dkA = h->dkA; a.dvA = h->dvA; a.dkB = h->dkB; a.dvB = h->dvB;
+    a.tkA = h->tkA; a.tkB = h->tkB; a.tvA = h->tvA; a.tvB = h->tvB;
+    a.numKeys = h->numKeys;
+    a.totalKeys = h->totalKeys;
+    a.ranges = h->ranges;
+    a.stats = statsBuf;
+    a.scratch = h->sort;
+    Scope s(h, "bin_sort", 0.0);
+    launch_bin_sort(h->st, a, h->lastBin);
+  }
+  RasterBufs b{};
+  b.ranges = h->ranges;
+  b.ids = h->lastBin.ids;
+  b.rec = h->rec;
+  b.img = h->img;
+  b.T = h->Tfin;
+  b.end = h->endIdx;
+  b.gC = h->gC;
+  b.tgt = h->targets;
+  b.stats = statsBuf;
+  {
+    Scope s(h, loss ? "raster_fwd_loss" : "raster_fwd", 0.0);
+    launch_raster_fwd(h->st, cam, make_cut(h), b, loss);
+  }
+  HIPCK(hipGetLastError());
+  return GS_OK;
+}
+
+int pick_camera(gs_trainer* h) {
+  // trainer.js:400-406 (Math.random replaced by a seeded mulberry32)
+  const int nc = (int)h->cams.size();
+  int ci = (int)((double)mulberry32(h->rng) / 4294967296.0 * nc);
+  if (ci >= nc) ci = nc - 1;
+  for (int tries = 0; tries < 16; tries++) {
+    const bool bad = ci == h->holdout || (!h->excluded.empty() && h->excluded[ci]);
+    if (!bad) break;
+    ci = (ci + 1) % nc;
+  }
+  return ci;
+}
+
+int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  if (h->o.loss != GS_LOSS_CHARBONNIER)
+    return fail(h, GS_E_ARG, "only the Charbonnier loss is available in this build");
+  if (ci < 0) ci = pick_camera(h);
+  if (ci >= (int)h->cams.size()) return fail(h, GS_E_ARG, "camera index out of range");
+  CamMeta& meta = h->cams[ci];
+  h->lastCam = ci;
+  // trainer.js:409-412: one SH band per 1000 iterations
+  meta.activeDeg = (int)std::min<int64_t>(h->shDeg, h->iter / 1000);
+  const Cam cam = make_cam(h, meta.c, 1, meta.offset, meta.activeDeg);
+  h->iter++;
+  h->pixelsSeen += (int64_t)meta.c.w * meta.c.h;
+  const AdamHP hp = make_hp(h);
+
+  int rc = run_raster(h, cam, 1, h->stats);
+  if (rc) return rc;
+  const int64_t npx = (int64_t)cam.W * cam.H;
+  if (dbg) {
+    HIPCK(hipStreamSynchronize(h->st));
+    uint32_t nk = 0;
+    HIPCK(hipMemcpy(&nk, h->numKeys, 4, hipMemcpyDeviceToHost));
+    if (dbg->rec) HIPCK(hipMemcpy(dbg->rec, h->rec, 48 * h->n, hipMemcpyDeviceToHost));
+    std::vector<uint32_t> dk(h->n);
+    HIPCK(hipMemcpy(dk.data(), h->depthKey, 4 * h->n, hipMemcpyDeviceToHost));
+    if (dbg->depthKey) memcpy(dbg->depthKey, dk.data(), 4 * h->n);
+    if (dbg->rect) {
+      std::vector<uint2> r(h->n);
+      HIPCK(hipMemcpy(r.data(), h->rect, 8 * h->n, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < h->n; i++) {
+        dbg->rect[4 * i + 0] = r[i].x & 0xffff; dbg->rect[4 * i + 1] = r[i].x >> 16;
+        dbg->rect[4 * i + 2] = r[i].y & 0xffff; dbg->rect[4 * i + 3] = r[i].y >> 16;
+      }
+    }
+    if (dbg->cnt) HIPCK(hipMemcpy(dbg->cnt, h->cnt, 4 * h->n, hipMemcpyDeviceToHost));
+    if (dbg->numKeys) *dbg->numKeys = nk;
+    if (dbg->ids || dbg->keys) {
+      std::vector<uint32_t> ids(nk ? nk : 1);
+      if (nk) HIPCK(hipMemcpy(ids.data(), h->lastBin.ids, 4ull * nk, hipMemcpyDeviceToHost));
+      if (dbg->ids) memcpy(dbg->ids, ids.data(), 4ull * nk);
+      if (dbg->keys && nk) {
+        if (h->lastBin.keyBytes == 8) {
+          HIPCK(hipMemcpy(dbg->keys, h->lastBin.keys, 8ull * nk, hipMemcpyDeviceToHost));
+        } else {
+          std::vector<uint32_t> tk(nk);
+          HIPCK(hipMemcpy(tk.data(), h->lastBin.keys, 4ull * nk, hipMemcpyDeviceToHost));
+          for (uint32_t k = 0; k < nk; k++) dbg->keys[k] = ((uint64_t)tk[k] << 32) | dk[ids[k]];
+        }
+      }
+    }
+    if (dbg->tileRanges) HIPCK(hipMemcpy(dbg->tileRanges, h->ranges, 8ull * cam.numTiles, hipMemcpyDeviceToHost));
+    if (dbg->img) HIPCK(hipMemcpy(dbg->img, h->img, 16 * npx, hipMemcpyDeviceToHost));
+    if (dbg->T) HIPCK(hipMemcpy(dbg->T, h->Tfin, 4 * npx, hipMemcpyDeviceToHost));
+    if (dbg->end) HIPCK(hipMemcpy(dbg->end, h->endIdx, 4 * npx, hipMemcpyDeviceToHost));
+    if (dbg->gC) HIPCK(hipMemcpy(dbg->gC, h->gC, 16 * npx, hipMemcpyDeviceToHost));
+  }
+  RasterBufs b{};
+  b.ranges = h->ranges;
+  b.ids = h->lastBin.ids;
+  b.rec = h->rec;
+  b.T = h->Tfin;
+  b.end = h->endIdx;
+  b.gC = h->gC;
+  {
+    Scope s(h, "raster_bwd", 0.0);
+    launch_raster_bwd(h->st, cam, make_cut(h), b, h->o.gradAccum, h->accum);
+  }
+  HIPCK(hipGetLastError());
+  float* dGF = nullptr;
+  float* dSG = nullptr;
+  if (dbg) {
+    HIPCK(hipStreamSynchronize(h->st));
+    if (dbg->accum) {
+      std::vector<int32_t> a(12 * h->n);
+      HIPCK(hipMemcpy(a.data(), h->accum, 48 * h->n, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < h->n; i++)
+        for (int k = 0; k < 10; k++) dbg->accum[10 * i + k] = a[12 * i + k];
+    }
+    if (dbg->gradF || dbg->shGrad) {
+      HIPCK(hipMalloc(&dGF, 64 * h->n));
+      HIPCK(hipMalloc(&dSG, 4 * (h->shC ? h->shC : 1) * h->n));
+    }
+  }
+  UpdateBufs u{};
+  u.n = h->n;
+  u.cap = h->cap;
+  u.P = h->P; u.M = h->M; u.V = h->V;
+  u.SH = h->SH; u.SHM = h->SHM; u.SHV = h->SHV;
+  u.accum = h->accum;
+  u.cnt = h->cnt;
+  u.dbgGradF = dGF;
+  u.dbgShGrad = dSG;
+  {
+    Scope s(h, "update", (double)h->n * (100.0 + 6 * 64.0 + 6 * 4.0 * h->shC));
+    launch_update(h->st, h->shDeg, h->o.gradAccum, cam, hp, u);
+  }
+  HIPCK(hipGetLastError());
+  if (dbg) {
+    HIPCK(hipStreamSynchronize(h->st));
+    if (dGF) {
+      if (dbg->gradF) HIPCK(hipMemcpy(dbg->gradF, dGF, 64 * h->n, hipMemcpyDeviceToHost));
+      if (dbg->shGrad && h->shC) HIPCK(hipMemcpy(dbg->shGrad, dSG, 4ull * h->shC * h->n, hipMemcpyDeviceToHost));
+      (void)hipFree(dGF);
+      (void)hipFree(dSG);
+    }
+    if (dbg->stats) HIPCK(hipMemcpy(dbg->stats, h->stats, 32, hipMemcpyDeviceToHost));
+  }
+  return GS_OK;
+}
+
+void repack_to_planes(const float* src, int64_t n, int width, int planes, int64_t cap,
+                      std::vector<float>& dst) {
+  // src: n x width (row-major); dst: planes x cap x 4
+  dst.assign((size_t)planes * cap * 4, 0.0f);
+  for (int64_t i = 0; i < n; i++)
+    for (int j = 0; j < width; j++) dst[((size_t)(j / 4) * cap + i) * 4 + (j % 4)] = src[i * width + j];
+}
+
+void planes_to_rows(const std::vector<float>& src, int64_t n, int width, int64_t cap, float* dst) {
+  for (int64_t i = 0; i < n; i++)
+    for (int j = 0; j < width; j++) dst[i * width + j] = src[((size_t)(j / 4) * cap + i) * 4 + (j % 4)];
+}
+
+}  // namespace
+
+extern "C" {
+
+int gs_default_opts(gs_opts* o) {
+  if (!o) return GS_E_ARG;
+  memset(o, 0, sizeof(*o));
+  o->size = (int32_t)sizeof(gs_opts);
+  o->eCut = 4.5f;
+  o->aMin = 0.0039f;
+  o->radClamp = 1.0f;
+  o->shDeg = 2;
+  o->anisoReg = 0.005f;
+  o->opacityReg = 0.01f;
+  o->shLr = 1.25e-4f;
+  o->minScale = 1e-4f;
+  o->maxIters = 60000;
+  o->maxSplats = 600000;
+  o->capMult = 4.0f;
+  o->entriesCap = 0;
+  o->tileGrad = 1;
+  o->posLr = 0.0f;
+  o->loss = GS_LOSS_CHARBONNIER;
+  o->ssimLambda = 0.2f;
+  o->gradAccum = GS_ACCUM_FIXED;
+  o->sortMode = GS_SORT_DEPTH_THEN_TILE;
+  o->seed = 0;
+  o->device = 0;
+  return GS_OK;
+}
+
+int gs_create(const gs_opts* o, gs_trainer_t** out) {
+  if (!out) return GS_E_ARG;
+  *out = nullptr;
+  gs_trainer* h = new gs_trainer();
+  gs_default_opts(&h->o);
+  if (o) {
+    if (o->size != (int32_t)sizeof(gs_opts)) {
+      delete h;
+      return GS_E_ARG;
+    }
+    h->o = *o;
+  }
+  if (h->o.shDeg < 0 || h->o.shDeg > 3 || h->o.sortMode < 0 || h->o.sortMode > 1 ||
+      h->o.gradAccum < 0 || h->o.gradAccum > 1) {
+    delete h;
+    return GS_E_ARG;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || h->o.device >= ndev) {
+    (void)hipGetLastError();
+    delete h;
+    return GS_E_HIP;
+  }
+  if (hipSetDevice(h->o.device) != hipSuccess || hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return GS_E_HIP;
+  }
+  h->shDeg = h->o.shDeg;
+  h->shK = h->shDeg > 0 ? (h->shDeg + 1) * (h->shDeg + 1) - 1 : 0;
+  h->shC = 3 * h->shK;
+  h->shP4 = (h->shC + 3) / 4;
+  h->rng = h->o.seed;
+  *out = h;
+  return GS_OK;
+}
+
+void gs_destroy(gs_trainer_t* h) {
+  if (!h) return;
+  (void)hipSetDevice(h->o.device);
+  (void)hipStreamSynchronize(h->st);
+  free_all(h);
+  for (auto& p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
+  for (auto e : h->evPool) (void)hipEventDestroy(e);
+  if (h->st) (void)hipStreamDestroy(h->st);
+  delete h;
+}
+
+int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, const gs_camera* cams,
+             int32_t ncams, const gs_image* images, int32_t nimages, int32_t maxViewW,
+             int32_t maxViewH, float sceneRadius) {
+  if (!h) return GS_E_ARG;
+  if (!params || n <= 0 || !cams || ncams <= 0 || !images || nimages <= 0)
+    return fail(h, GS_E_ARG, "setup: empty gaussians, cameras or images");
+  HIPCK(hipSetDevice(h->o.device));
+  HIPCK(hipStreamSynchronize(h->st));
+  free_all(h);
+  h->n = n;
+  // trainer.js:105-107; capacity never below n
+  const int64_t maxSplats = h->o.maxSplats > 0 ? h->o.maxSplats : 600000;
+  h->cap = std::max<int64_t>(n, std::min<int64_t>(std::max<int64_t>((int64_t)std::floor(n * (double)h->o.capMult), n), maxSplats));
+  if (h->cap >= (int64_t)1 << 31) return fail(h, GS_E_LIMIT, "setup: more than 2^31 Gaussians");
+  // trainer.js:48-49
+  h->entriesCap = h->o.entriesCap > 0 ? h->o.entriesCap : std::max<int64_t>(kDefaultEntries, 24 * maxSplats);
+  h->entriesCap = std::min<int64_t>(h->entriesCap, 0xffffffffll);
+
+  // targets (trainer.js:115-139)
+  h->cams.clear();
+  int64_t total = 0;
+  for (int i = 0; i < ncams; i++) {
+    const gs_camera& c = cams[i];
+    if (c.imgIdx < 0 || c.imgIdx >= nimages) return fail(h, GS_E_ARG, "setup: camera imgIdx out of range");
+    const gs_image& im = images[c.imgIdx];
+    CamMeta m;
+    m.c = c;
+    m.c.w = im.tw;
+    m.c.h = im.th;
+    m.offset = total;
+    m.f0 = c.f;
+    m.activeDeg = h->shDeg;
+    total += (int64_t)im.tw * im.th;
+    h->cams.push_back(m);
+  }
+  if (total * 4 > kTargetLimitBytes) {
+    char buf[256];
+    snprintf(buf, sizeof buf, "training targets (%.0fMB) exceed the device binding limit (%.0fMB) — reduce image count or resolution",
+             total * 4 / 1e6, kTargetLimitBytes / 1e6);
+    return fail(h, GS_E_LIMIT, buf);
+  }
+  h->totalPx = total;
+  std::vector<uint32_t> tgt((size_t)std::max<int64_t>(total, 4));
+  for (const CamMeta& m : h->cams) {
+    const float* rgb = images[m.c.imgIdx].rgb;
+    const int64_t np = (int64_t)m.c.w * m.c.h;
+    for (int64_t p = 0; p < np; p++) {
+      const double r = rgb[3 * p];
+      if (r < 0) { tgt[m.offset + p] = 0; continue; }
+      const uint32_t vb = (uint32_t)js_round255((double)rgb[3 * p + 2] * 255.0) << 16;
+      const uint32_t vg = (uint32_t)js_round255((double)rgb[3 * p + 1] * 255.0) << 8;
+      const uint32_t vr = (uint32_t)js_round255(r * 255.0);
+      tgt[m.offset + p] = (255u << 24) | vb | vg | vr;
+    }
+  }
+  h->maxPix = (int64_t)maxViewW * maxViewH;
+  h->maxTiles = ((maxViewW + 15) / 16) * ((maxViewH + 15) / 16);
+  for (const CamMeta& m : h->cams) {
+    h->maxPix = std::max<int64_t>(h->maxPix, (int64_t)m.c.w * m.c.h);
+    h->maxTiles = std::max(h->maxTiles, ((m.c.w + 15) / 16) * ((m.c.h + 15) / 16));
+  }
+
+  const int64_t cap = h->cap;
+  const size_t pb = (size_t)cap * 64;
+  int rc = 0;
+  rc |= dalloc(h, &h->P, pb);
+  rc |= dalloc(h, &h->M, pb);
+  rc |= dalloc(h, &h->V, pb);
+  const size_t sb = (size_t)cap * 16 * std::max(h->shP4, 1);
+  rc |= dalloc(h, &h->SH, sb);
+  rc |= dalloc(h, &h->SHM, sb);
+  rc |= dalloc(h, &h->SHV, sb);
+  rc |= dalloc(h, &h->rec, (size_t)cap * 48);
+  rc |= dalloc(h, &h->depthKey, (size_t)cap * 4);
+  rc |= dalloc(h, &h->cnt, (size_t)cap * 4);
+  rc |= dalloc(h, &h->rect, (size_t)cap * 8);
+  rc |= dalloc(h, &h->accum, (size_t)cap * 48);
+  rc |= dalloc(h, &h->dkA, (size_t)cap * 4);
+  rc |= dalloc(h, &h->dvA, (size_t)cap * 4);
+  rc |= dalloc(h, &h->dkB, (size_t)cap * 4);
+  rc |= dalloc(h, &h->dvB, (size_t)cap * 4);
+  const size_t kb = h->o.sortMode == GS_SORT_GLOBAL64 ? 8 : 4;
+  rc |= dalloc(h, &h->tkA, (size_t)h->entriesCap * kb);
+  rc |= dalloc(h, &h->tkB, (size_t)h->entriesCap * kb);
+  rc |= dalloc(h, &h->tvA, (size_t)h->entriesCap * 4);
+  rc |= dalloc(h, &h->tvB, (size_t)h->entriesCap * 4);
+  h->sort.G = kSortBlocks;
+  rc |= dalloc(h, &h->sort.hist, (size_t)256 * kSortBlocks * 4);
+  rc |= dalloc(h, &h->sort.totals, 256 * 4);
+  rc |= dalloc(h, &h->sort.partials, (size_t)kSortBlocks * 8);
+  rc |= dalloc(h, &h->dN, 4);
+  rc |= dalloc(h, &h->numKeys, 4);
+  rc |= dalloc(h, &h->totalKeys, 8);
+  rc |= dalloc(h, &h->ranges, (size_t)h->maxTiles * 8);
+  rc |= dalloc(h, &h->img, (size_t)h->maxPix * 16);
+  rc |= dalloc(h, &h->Tfin, (size_t)h->maxPix * 4);
+  rc |= dalloc(h, &h->endIdx, (size_t)h->maxPix * 4);
+  rc |= dalloc(h, &h->gC, (size_t)h->maxPix * 16);
+  rc |= dalloc(h, &h->targets, tgt.size() * 4);
+  rc |= dalloc(h, &h->stats, 32);
+  rc |= dalloc(h, &h->evalStats, 32);
+  if (rc) { free_all(h); return rc < 0 ? rc : GS_E_NOMEM; }
+
+  std::vector<float> planes;
+  repack_to_planes(params, n, 16, 4, cap, planes);
+  HIPCK(hipMemcpy(h->P, planes.data(), pb, hipMemcpyHostToDevice));
+  if (h->shC) {
+    std::vector<float> shz;
+    if (sh) repack_to_planes(sh, n, h->shC, h->shP4, cap, planes);
+    else planes.assign((size_t)h->shP4 * cap * 4, 0.0f);
+    HIPCK(hipMemcpy(h->SH, planes.data(), sb, hipMemcpyHostToDevice));
+  }
+  HIPCK(hipMemset(h->M, 0, pb));
+  HIPCK(hipMemset(h->V, 0, pb));
+  HIPCK(hipMemset(h->SHM, 0, sb));
+  HIPCK(hipMemset(h->SHV, 0, sb));
+  HIPCK(hipMemset(h->accum, 0, (size_t)cap * 48));
+  HIPCK(hipMemset(h->stats, 0, 32));
+  HIPCK(hipMemset(h->evalStats, 0, 32));
+  const uint32_t n32 = (uint32_t)n;
+  HIPCK(hipMemcpy(h->dN, &n32, 4, hipMemcpyHostToDevice));
+  HIPCK(hipMemcpy(h->targets, tgt.data(), tgt.size() * 4, hipMemcpyHostToDevice));
+
+  h->sceneRadius = sceneRadius;
+  h->horizon = h->o.maxIters > 0 ? h->o.maxIters : 60000;
+  h->basePosLr = h->o.posLr > 0 ? (double)h->o.posLr : 3e-4 * (double)sceneRadius;
+  h->holdout = -1;
+  h->excluded.assign(ncams, 0);
+  h->iter = 0;
+  h->pixelsSeen = 0;
+  h->ready = true;
+  return GS_OK;
+}
+
+int gs_step(gs_trainer_t* h, int32_t cam) {
+  if (!h) return GS_E_ARG;
+  return step_impl(h, cam, nullptr);
+}
+
+int gs_step_debug(gs_trainer_t* h, int32_t cam, gs_debug_out* out) {
+  if (!h || !out) return GS_E_ARG;
+  return step_impl(h, cam, out);
+}
+
+int gs_read_loss(gs_trainer_t* h, double* mse, int32_t* has) {
+  if (!h || !mse || !has) return GS_E_ARG;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  double s[4];
+  HIPCK(hipMemcpyAsync(s, h->stats, 32, hipMemcpyDeviceToHost, h->st));
+  HIPCK(hipMemsetAsync(h->stats, 0, 32, h->st));
+  HIPCK(hipStreamSynchronize(h->st));
+  const int64_t px = h->pixelsSeen;
+  h->pixelsSeen = 0;
+  h->overflowTotal += s[3];
+  *has = 0;
+  *mse = 0.0;
+  if (px == 0 || s[2] == 0.0) return GS_OK;
+  *has = 1;
+  *mse = s[0] / (s[2] * 3.0);
+  return GS_OK;
+}
+
+int gs_render(gs_trainer_t* h, const gs_camera* c, int32_t trainMode, int64_t offset, float* rgba) {
+  if (!h || !c || !rgba) return GS_E_ARG;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  if ((int64_t)c->w * c->h > h->maxPix || ((c->w + 15) / 16) * ((c->h + 15) / 16) > h->maxTiles)
+    return fail(h, GS_E_LIMIT, "renderView: view larger than maxViewW x maxViewH");
+  if (c->w <= 0 || c->h <= 0) return fail(h, GS_E_ARG, "renderView: empty view");
+  if (trainMode && (offset < 0 || offset + (int64_t)c->w * c->h > h->totalPx))
+    return fail(h, GS_E_ARG, "renderView: target offset outside the packed targets");
+  const Cam cam = make_cam(h, *c, trainMode, offset, h->shDeg);
+  int rc = run_raster(h, cam, trainMode ? 1 : 0, h->evalStats);
+  if (rc) return rc;
+  HIPCK(hipMemcpyAsync(rgba, h->img, 16ull * c->w * c->h, hipMemcpyDeviceToHost, h->st));
+  HIPCK(hipStreamSynchronize(h->st));
+  return GS_OK;
+}
+
+int gs_eval_cam_psnr(gs_trainer_t* h, int32_t ci, double* psnr) {
+  if (!h || !psnr) return GS_E_ARG;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  if (ci < 0 || ci >= (int)h->cams.size()) return fail(h, GS_E_ARG, "camera index out of range");
+  const CamMeta& m = h->cams[ci];
+  // the reference evaluates with the camera's training uniform (trainer.js:547)
+  const Cam cam = make_cam(h, m.c, 1, m.offset, m.activeDeg);
+  HIPCK(hipMemsetAsync(h->evalStats, 0, 32, h->st));
+  int rc = run_raster(h, cam, 1, h->evalStats);
+  if (rc) return rc;
+  double s[4];
+  HIPCK(hipMemcpyAsync(s, h->evalStats, 32, hipMemcpyDeviceToHost, h->st));
+  HIPCK(hipStreamSynchronize(h->st));
+  // trainer.js:570-571: the training stats window is reset too
+  HIPCK(hipMemsetAsync(h->stats, 0, 32, h->st));
+  h->pixelsSeen = 0;
+  const double mse = s[0] / (std::max(1.0, s[2]) * 3.0);
+  *psnr = mse > 0 ? -10.0 * std::log10(mse) : INFINITY;
+  return GS_OK;
+}
+
+int gs_read_gaussians(gs_trainer_t* h, float* params, float* sh) {
+  if (!h) return GS_E_ARG;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  HIPCK(hipStreamSynchronize(h->st));
+  if (params) {
+    std::vector<float> planes((size_t)4 * h->cap * 4);
+    HIPCK(hipMemcpy(planes.data(), h->P, planes.size() * 4, hipMemcpyDeviceToHost));
+    planes_to_rows(planes, h->n, 16, h->cap, params);
+  }
+  if (sh && h->shC) {
+    std::vector<float> planes((size_t)h->shP4 * h->cap * 4);
+    HIPCK(hipMemcpy(planes.data(), h->SH, planes.size() * 4, hipMemcpyDeviceToHost));
+    planes_to_rows(planes, h->n, h->shC, h->cap, sh);
+  }
+  return GS_OK;
+}
+
+int gs_set_holdout(gs_trainer_t* h, int32_t ci) {
+  if (!h) return GS_E_ARG;
+  h->holdout = ci;
+  return GS_OK;
+}
+
+int gs_set_excluded(gs_trainer_t* h, const int32_t* c, int32_t count) {
+  if (!h || (count > 0 && !c)) return GS_E_ARG;
+  std::fill(h->excluded.begin(), h->excluded.end(), 0);
+  for (int i = 0; i < count; i++)
+    if (c[i] >= 0 && c[i] < (int)h->excluded.size()) h->excluded[c[i]] = 1;
+  return GS_OK;
+}
+
+int gs_set_iter(gs_trainer_t* h, int64_t iter) {
+  if (!h || iter < 0) return GS_E_ARG;
+  h->iter = iter;
+  return GS_OK;
+}
+
+int gs_get_info(gs_trainer_t* h, gs_info* out) {
+  if (!h || !out) return GS_E_ARG;
+  memset(out, 0, sizeof(*out));
+  out->n = h->n;
+  out->cap = h->cap;
+  out->iter = h->iter;
+  out->shDeg = h->shDeg;
+  out->shK = h->shK;
+  out->stride = 16;
+  out->numCams = (int32_t)h->cams.size();
+  out->lastCam = h->lastCam;
+  out->holdout = h->holdout;
+  out->entriesCap = h->entriesCap;
+  out->maxTiles = h->maxTiles;
+  out->horizon = (float)h->horizon;
+  if (h->ready) {
+    uint32_t nk = 0;
+    uint64_t tk = 0;
+    double s[4];
+    HIPCK(hipStreamSynchronize(h->st));
+    HIPCK(hipMemcpy(&nk, h->numKeys, 4, hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpy(&tk, h->totalKeys, 8, hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpy(s, h->stats, 32, hipMemcpyDeviceToHost));
+    out->lastKeys = nk;
+    out->overflowKeys = (int64_t)(h->overflowTotal + s[3]);
+  }
+  return GS_OK;
+}
+
+int gs_sync(gs_trainer_t* h) {
+  if (!h) return GS_E_ARG;
+  HIPCK(hipStreamSynchronize(h->st));
+  return GS_OK;
+}
+
+const char* gs_last_error(gs_trainer_t* h) {
+  return h ? h->err.c_str() : "null trainer";
+}
+
+int gs_profile(gs_trainer_t* h, int32_t enable) {
+  if (!h) return GS_E_ARG;
+  if (enable && !h->prof) {
+    h->slots.clear();
+  }
+  if (!enable) prof_collect(h);
+  h->prof = enable != 0;
+  return GS_OK;
+}
+
+int gs_profile_read(gs_trainer_t* h, gs_kernel_time* out, int32_t max, int32_t* count) {
+  if (!h || !count) return GS_E_ARG;
+  prof_collect(h);
+  *count = (int32_t)h->slots.size();
+  for (int k = 0; k < (int)h->slots.size() && k < max; k++) {
+    memset(&out[k], 0, sizeof(gs_kernel_time));
+    snprintf(out[k].name, sizeof out[k].name, "%s", h->slots[k].name.c_str());
+    out[k].total_ms = h->slots[k].ms;
+    out[k].launches = h->slots[k].launches;
+    out[k].bytes = h->slots[k].bytes;
+  }
+  return GS_OK;
+}
+
+void* gs_stream(gs_trainer_t* h) { return h ? (void*)h->st : nullptr; }
+
+}  // extern "C"

@@ -1,0 +1,291 @@
+// gs_update.hip — fused per-Gaussian update: the reference's `chain`
+// (shaders.js:704-948), `adam` (:984-1036) and `sh-adam` (:952-982) passes in
+// one kernel.
+//
+// Per Gaussian it reads the screen-space gradient sums (and clears them, as
+// chain does at :727-732), recomputes the projection geometry, chains the
+// gradient to the 14 parameters and the SH coefficients in registers, and
+// applies Adam to params/m/v and sh/m/v. The parameter gradients never
+// touch HBM (the reference writes and re-reads gradF and shGrad), so the pass
+// moves params, SH and both moment buffers once in each direction plus 48 B
+// of accumulators: ~1.06 KB per Gaussian at SH degree 2, HBM-bound.
+//
+// Invisible Gaussians still take the Adam step with a zero gradient (plus the
+// opacity regulariser), like the reference, whose Adam runs over all n.
+#include "gs_kernels.h"
+
+namespace gs {
+
+template <int kDeg, int kAccum, bool kDebug>
+__global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBufs u) {
+  constexpr int K = ShDims<kDeg>::K;
+  constexpr int NC = ShDims<kDeg>::C;
+  constexpr int P4 = ShDims<kDeg>::P4;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= u.n) return;
+  const int64_t cap = u.cap;
+
+  // ---- screen-space gradients (decode + clear), shaders.js:727-744 ----
+  float gp[10];
+  {
+    int4* acc = (int4*)u.accum + 3 * i;
+    const int4 a0 = acc[0], a1 = acc[1], a2 = acc[2];
+    const int raw[10] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, a2.x, a2.y};
+    const int4 z = make_int4(0, 0, 0, 0);
+    acc[0] = z; acc[1] = z; acc[2] = z;
+#pragma unroll
+    for (int k = 0; k < 10; k++)
+      gp[k] = kAccum == 0 ? (float)raw[k] / kFixed : __int_as_float(raw[k]);
+  }
+
+  float p[16];
+  load_params(u.P, cap, i, p);
+  float sh[P4 > 0 ? 4 * P4 : 1];
+  if constexpr (kDeg > 0) load_sh<kDeg>(u.SH, cap, i, sh);
+
+  float gF[14];
+#pragma unroll
+  for (int k = 0; k < 14; k++) gF[k] = 0.0f;
+  float shG[NC > 0 ? NC : 1];
+#pragma unroll
+  for (int k = 0; k < NC; k++) shG[k] = 0.0f;
+
+  Geom g;
+  g.ok = false;
+  if (u.cnt[i] != 0u) compute_geom(cam, p, g);
+  if (g.ok) {
+    const float cden = 1.0f + conic_lmax(g.va, g.vb, g.vc);
+    gp[2] = gp[2] * cden;
+    gp[3] = gp[3] * cden;
+    gp[4] = gp[4] * cden;
+    const float f = cam.f;
+    const float iz = 1.0f / g.pc[2];
+
+    // conic -> dilated 2D covariance (:759-777)
+    const float ad = g.va + 0.3f, cd = g.vc + 0.3f;
+    const float detVd = ad * cd - g.vb * g.vb;
+    const float inv = 1.0f / detVd;
+    const float kA = cd * inv, kB = (-g.vb) * inv, kC = ad * inv;
+    const float g11 = gp[2], g12 = 0.5f * gp[3], g22 = gp[4];
+    const float m100 = g11 * kA + g12 * kB, m101 = g11 * kB + g12 * kC;
+    const float m110 = g12 * kA + g22 * kB, m111 = g12 * kB + g22 * kC;
+    const float p00 = kA * m100 + kB * m110, p01 = kA * m101 + kB * m111;
+    const float p10 = kB * m100 + kC * m110, p11 = kB * m101 + kC * m111;
+    float gva = -p00, gvb = -(p01 + p10), gvc = -p11;
+
+    // comp -> raw 2D covariance (:780-788)
+    const float detV = fmaxf(g.va * g.vc - g.vb * g.vb, 0.0f);
+    const float comp = sqrtf(fmaxf(detV / detVd, 0.0f));
+    if (comp > 1e-4f) {
+      const float gcomp = gp[5];
+      const float denom = 2.0f * comp * detVd;
+      gva = gva + gcomp * (g.vc - comp * comp * cd) / denom;
+      gvc = gvc + gcomp * (g.va - comp * comp * ad) / denom;
+      gvb = gvb + gcomp * (2.0f * g.vb * (comp * comp - 1.0f)) / denom;
+    }
+
+    // V = T Sigma T^T backward (:791-810)
+    const float h11 = gva, h12 = 0.5f * gvb, h22 = gvc;
+    const float* t0 = g.t0;
+    const float* t1 = g.t1;
+    float u0[3], u1[3];
+    for (int k = 0; k < 3; k++) { u0[k] = h11 * t0[k] + h12 * t1[k]; u1[k] = h12 * t0[k] + h22 * t1[k]; }
+    const float dS00 = t0[0] * u0[0] + t1[0] * u1[0];
+    const float dS01 = 0.5f * ((t0[0] * u0[1] + t1[0] * u1[1]) + (t0[1] * u0[0] + t1[1] * u1[0]));
+    const float dS02 = 0.5f * ((t0[0] * u0[2] + t1[0] * u1[2]) + (t0[2] * u0[0] + t1[2] * u1[0]));
+    const float dS11 = t0[1] * u0[1] + t1[1] * u1[1];
+    const float dS12 = 0.5f * ((t0[1] * u0[2] + t1[1] * u1[2]) + (t0[2] * u0[1] + t1[2] * u1[1]));
+    const float dS22 = t0[2] * u0[2] + t1[2] * u1[2];
+    float sT0[3], sT1[3];
+    sT0[0] = (g.s00 * t0[0] + g.s01 * t0[1]) + g.s02 * t0[2];
+    sT0[1] = (g.s01 * t0[0] + g.s11 * t0[1]) + g.s12 * t0[2];
+    sT0[2] = (g.s02 * t0[0] + g.s12 * t0[1]) + g.s22 * t0[2];
+    sT1[0] = (g.s00 * t1[0] + g.s01 * t1[1]) + g.s02 * t1[2];
+    sT1[1] = (g.s01 * t1[0] + g.s11 * t1[1]) + g.s12 * t1[2];
+    sT1[2] = (g.s02 * t1[0] + g.s12 * t1[1]) + g.s22 * t1[2];
+    float dT0[3], dT1[3];
+    for (int k = 0; k < 3; k++) {
+      dT0[k] = 2.0f * (h11 * sT0[k] + h12 * sT1[k]);
+      dT1[k] = 2.0f * (h12 * sT0[k] + h22 * sT1[k]);
+    }
+
+    // T = J W (:813-822) and the mean path (:825-827)
+    const float dJ00 = dot3(dT0, cam.R + 0);
+    const float dJ02 = dot3(dT0, cam.R + 6);
+    const float dJ11 = dot3(dT1, cam.R + 3);
+    const float dJ12 = dot3(dT1, cam.R + 6);
+    float dpc[3] = {0.0f, 0.0f, 0.0f};
+    dpc[0] = dpc[0] + dJ02 * ((-f) * iz * iz);
+    dpc[1] = dpc[1] + dJ12 * ((-f) * iz * iz);
+    dpc[2] = dpc[2] + ((dJ00 + dJ11) * ((-f) * iz * iz)
+           + dJ02 * (2.0f * f * g.pc[0] * iz * iz * iz))
+           + dJ12 * (2.0f * f * g.pc[1] * iz * iz * iz);
+    dpc[0] = dpc[0] + gp[0] * f * iz;
+    dpc[1] = dpc[1] + gp[1] * f * iz;
+    dpc[2] = dpc[2] + (-f) * (gp[0] * g.pc[0] + gp[1] * g.pc[1]) * iz * iz;
+    const float* R = cam.R;
+    gF[0] = (R[0] * dpc[0] + R[3] * dpc[1]) + R[6] * dpc[2];
+    gF[1] = (R[1] * dpc[0] + R[4] * dpc[1]) + R[7] * dpc[2];
+    gF[2] = (R[2] * dpc[0] + R[5] * dpc[1]) + R[8] * dpc[2];
+
+    // Sigma = M M^T backward (:861-884)
+    float m0[3], m1[3], m2[3], dM0[3], dM1[3], dM2[3];
+    for (int k = 0; k < 3; k++) { m0[k] = g.r0[k] * g.s[k]; m1[k] = g.r1[k] * g.s[k]; m2[k] = g.r2[k] * g.s[k]; }
+    for (int k = 0; k < 3; k++) {
+      dM0[k] = 2.0f * ((dS00 * m0[k] + dS01 * m1[k]) + dS02 * m2[k]);
+      dM1[k] = 2.0f * ((dS01 * m0[k] + dS11 * m1[k]) + dS12 * m2[k]);
+      dM2[k] = 2.0f * ((dS02 * m0[k] + dS12 * m1[k]) + dS22 * m2[k]);
+    }
+    float ls[3];
+    for (int k = 0; k < 3; k++) ls[k] = clampf(p[3 + k], -12.0f, 6.0f);
+    const float mls = ((ls[0] + ls[1]) + ls[2]) / 3.0f;
+    for (int k = 0; k < 3; k++) {
+      const float dsv = (dM0[k] * g.r0[k] + dM1[k] * g.r1[k]) + dM2[k] * g.r2[k];
+      gF[3 + k] = dsv * g.s[k] + hp.anisoReg * (ls[k] - mls);
+    }
+
+    // quaternion backward (:887-904)
+    float dR0[3], dR1[3], dR2[3];
+    for (int k = 0; k < 3; k++) { dR0[k] = dM0[k] * g.s[k]; dR1[k] = dM1[k] * g.s[k]; dR2[k] = dM2[k] * g.s[k]; }
+    const float qw = g.q[0], qx = g.q[1], qy = g.q[2], qz = g.q[3];
+    const float gw = 2.0f * (((((dR0[1] * (-qz) + dR0[2] * qy) + dR1[0] * qz) + dR1[2] * (-qx)) + dR2[0] * (-qy)) + dR2[1] * qx);
+    const float gx = 2.0f * (((((((dR0[1] * qy + dR0[2] * qz) + dR1[0] * qy) + dR1[1] * (-2.0f * qx)) + dR1[2] * (-qw)) + dR2[0] * qz) + dR2[1] * qw) + dR2[2] * (-2.0f * qx));
+    const float gy = 2.0f * (((((((dR0[0] * (-2.0f * qy) + dR0[1] * qx) + dR0[2] * qw) + dR1[0] * qx) + dR1[2] * qz) + dR2[0] * (-qw)) + dR2[1] * qz) + dR2[2] * (-2.0f * qy));
+    const float gz = 2.0f * (((((((dR0[0] * (-2.0f * qz) + dR0[1] * (-qw)) + dR0[2] * qx) + dR1[0] * qw) + dR1[1] * (-2.0f * qz)) + dR1[2] * qy) + dR2[0] * qx) + dR2[1] * qy);
+    const float gq[4] = {gw, gx, gy, gz};
+    const float ql = fmaxf(sqrtf(((p[6] * p[6] + p[7] * p[7]) + p[8] * p[8]) + p[9] * p[9]), 1e-6f);
+    const float qdg = ((g.q[0] * gq[0] + g.q[1] * gq[1]) + g.q[2] * gq[2]) + g.q[3] * gq[3];
+    for (int k = 0; k < 4; k++) gF[6 + k] = (gq[k] - g.q[k] * qdg) / ql;
+
+    // colour (:907-947)
+    const float sCol[3] = {gs_sigmoid_clamped(p[10]), gs_sigmoid_clamped(p[11]), gs_sigmoid_clamped(p[12])};
+    float dRGB[3] = {gp[7], gp[8], gp[9]};
+    if constexpr (kDeg > 0) {
+      float v[3];
+      const float ulen = view_dir(cam, p, v);
+      float Y[15], D[45];
+      sh_basis<kDeg, true>(v, Y, D);
+      const int aK = cam.shActiveK;
+      float col[3] = {sCol[0], sCol[1], sCol[2]};
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        if (k < aK) {
+          col[0] = col[0] + sh[k] * Y[k];
+          col[1] = col[1] + sh[K + k] * Y[k];
+          col[2] = col[2] + sh[2 * K + k] * Y[k];
+        }
+      }
+      for (int c = 0; c < 3; c++) dRGB[c] = dRGB[c] * (col[c] > 0.0f ? 1.0f : 0.0f);
+      float gv[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+      for (int k = 0; k < K; k++) {
+        if (k < aK) {
+          shG[k] = dRGB[0] * Y[k];
+          shG[K + k] = dRGB[1] * Y[k];
+          shG[2 * K + k] = dRGB[2] * Y[k];
+          const float w = (dRGB[0] * sh[k] + dRGB[1] * sh[K + k]) + dRGB[2] * sh[2 * K + k];
+          for (int c = 0; c < 3; c++) gv[c] = gv[c] + w * D[3 * k + c];
+        }
+      }
+      const float vg = dot3(v, gv);
+      for (int c = 0; c < 3; c++) gF[c] = gF[c] + (gv[c] - v[c] * vg) / ulen;
+    }
+    for (int c = 0; c < 3; c++) gF[10 + c] = dRGB[c] * sCol[c] * (1.0f - sCol[c]);
+    gF[13] = gp[6];
+  }
+
+  if (kDebug) {
+    for (int k = 0; k < 14; k++) u.dbgGradF[16 * i + k] = gF[k];
+    u.dbgGradF[16 * i + 14] = 0.0f;
+    u.dbgGradF[16 * i + 15] = 0.0f;
+    for (int k = 0; k < NC; k++) u.dbgShGrad[(int64_t)NC * i + k] = shG[k];
+  }
+
+  // ---- Adam over the 16 param slots (shaders.js:1000-1035) ----
+  float m[16], vv[16];
+  load_params(u.M, cap, i, m);
+  load_params(u.V, cap, i, vv);
+#pragma unroll
+  for (int j = 0; j < 14; j++) {
+    const float lr = hp.lr[j];
+    if (lr == 0.0f) continue;
+    float gr = gF[j];
+    if (!(fabsf(gr) < 1e18f)) gr = 0.0f;
+    if (j == 13) {
+      const float sg = gs_sigmoid_clamped(p[j]);
+      gr = gr + hp.reg * sg * (1.0f - sg);
+    }
+    const float mm = hp.b1 * m[j] + (1.0f - hp.b1) * gr;
+    const float vn = hp.b2 * vv[j] + (1.0f - hp.b2) * gr * gr;
+    m[j] = mm;
+    vv[j] = vn;
+    const float mh = mm / hp.bc1;
+    const float vh = vn / hp.bc2;
+    float pn = p[j] - lr * mh / (sqrtf(vh) + hp.eps);
+    if (j >= 3 && j <= 5) pn = clampf(pn, hp.minLS, hp.maxLS);
+    if (j >= 10 && j <= 13) pn = clampf(pn, -hp.maxLogit, hp.maxLogit);
+    p[j] = pn;
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    u.P[c * cap + i] = make_float4(p[4 * c], p[4 * c + 1], p[4 * c + 2], p[4 * c + 3]);
+    u.M[c * cap + i] = make_float4(m[4 * c], m[4 * c + 1], m[4 * c + 2], m[4 * c + 3]);
+    u.V[c * cap + i] = make_float4(vv[4 * c], vv[4 * c + 1], vv[4 * c + 2], vv[4 * c + 3]);
+  }
+
+  // ---- Adam over the SH coefficients (shaders.js:963-981) ----
+  if constexpr (kDeg > 0) {
+#pragma unroll
+    for (int c = 0; c < P4; c++) {
+      const float4 m4 = u.SHM[c * cap + i];
+      const float4 v4 = u.SHV[c * cap + i];
+      float ms[4] = {m4.x, m4.y, m4.z, m4.w};
+      float vs[4] = {v4.x, v4.y, v4.z, v4.w};
+      float ss[4] = {sh[4 * c], sh[4 * c + 1], sh[4 * c + 2], sh[4 * c + 3]};
+#pragma unroll
+      for (int l = 0; l < 4; l++) {
+        const int j = 4 * c + l;
+        if (j >= NC) break;
+        float gr = shG[j];
+        if (!(fabsf(gr) < 1e18f)) gr = 0.0f;
+        const float mm = hp.b1 * ms[l] + (1.0f - hp.b1) * gr;
+        const float vn = hp.b2 * vs[l] + (1.0f - hp.b2) * gr * gr;
+        ms[l] = mm;
+        vs[l] = vn;
+        const float mh = mm / hp.bc1;
+        const float vh = vn / hp.bc2;
+        ss[l] = ss[l] - hp.shLr * mh / (sqrtf(vh) + hp.eps);
+      }
+      u.SH[c * cap + i] = make_float4(ss[0], ss[1], ss[2], ss[3]);
+      u.SHM[c * cap + i] = make_float4(ms[0], ms[1], ms[2], ms[3]);
+      u.SHV[c * cap + i] = make_float4(vs[0], vs[1], vs[2], vs[3]);
+    }
+  }
+}
+
+template <int kDeg>
+static void launch_deg(hipStream_t st, int accumMode, bool dbg, const Cam& cam, const AdamHP& hp,
+                       const UpdateBufs& u) {
+  const dim3 grid((unsigned)((u.n + kBlock - 1) / kBlock));
+  if (accumMode == 0) {
+    if (dbg) k_update<kDeg, 0, true><<<grid, kBlock, 0, st>>>(cam, hp, u);
+    else k_update<kDeg, 0, false><<<grid, kBlock, 0, st>>>(cam, hp, u);
+  } else {
+    if (dbg) k_update<kDeg, 1, true><<<grid, kBlock, 0, st>>>(cam, hp, u);
+    else k_update<kDeg, 1, false><<<grid, kBlock, 0, st>>>(cam, hp, u);
+  }
+}
+
+void launch_update(hipStream_t st, int deg, int accumMode, const Cam& cam, const AdamHP& hp,
+                   const UpdateBufs& u) {
+  if (u.n <= 0) return;
+  const bool dbg = u.dbgGradF != nullptr;
+  switch (deg) {
+    case 0: launch_deg<0>(st, accumMode, dbg, cam, hp, u); break;
+    case 1: launch_deg<1>(st, accumMode, dbg, cam, hp, u); break;
+    case 2: launch_deg<2>(st, accumMode, dbg, cam, hp, u); break;
+    default: launch_deg<3>(st, accumMode, dbg, cam, hp, u); break;
+  }
+}
+
+}  // namespace gs

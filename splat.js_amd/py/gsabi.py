@@ -1,0 +1,299 @@
+"""Python binding of the C ABI (include/splat_gs.h) with the reference's
+GSTrainer method surface (src/gs/trainer.js): create / setup / stepOnce /
+readLoss / renderView / readGaussians / evalCamPsnr, plus the inspection
+entry points the tests and the benchmark use.
+
+The library must be built (splat.js_amd/build.py); loading fails loudly
+otherwise. There is no fallback path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libsplatgs.so")
+
+GS_LOSS_CHARBONNIER, GS_LOSS_L1_DSSIM = 0, 1
+GS_ACCUM_FIXED, GS_ACCUM_F32 = 0, 1
+GS_SORT_DEPTH_THEN_TILE, GS_SORT_GLOBAL64 = 0, 1
+
+
+class GsOpts(C.Structure):
+    _fields_ = [("size", C.c_int32), ("eCut", C.c_float), ("aMin", C.c_float), ("radClamp", C.c_float),
+                ("shDeg", C.c_int32), ("anisoReg", C.c_float), ("opacityReg", C.c_float),
+                ("shLr", C.c_float), ("minScale", C.c_float), ("maxIters", C.c_int32),
+                ("maxSplats", C.c_int64), ("capMult", C.c_float), ("entriesCap", C.c_int64),
+                ("tileGrad", C.c_int32), ("posLr", C.c_float), ("loss", C.c_int32),
+                ("ssimLambda", C.c_float), ("gradAccum", C.c_int32), ("sortMode", C.c_int32),
+                ("seed", C.c_uint32), ("device", C.c_int32)]
+
+
+class GsCamera(C.Structure):
+    _fields_ = [("R", C.c_float * 9), ("t", C.c_float * 3), ("f", C.c_float), ("cx", C.c_float),
+                ("cy", C.c_float), ("w", C.c_int32), ("h", C.c_int32), ("imgIdx", C.c_int32),
+                ("g", C.c_float), ("b", C.c_float)]
+
+
+class GsImage(C.Structure):
+    _fields_ = [("tw", C.c_int32), ("th", C.c_int32), ("rgb", C.c_void_p)]
+
+
+class GsInfo(C.Structure):
+    _fields_ = [("n", C.c_int64), ("cap", C.c_int64), ("iter", C.c_int64), ("shDeg", C.c_int32),
+                ("shK", C.c_int32), ("stride", C.c_int32), ("numCams", C.c_int32),
+                ("lastCam", C.c_int32), ("holdout", C.c_int32), ("lastKeys", C.c_int64),
+                ("overflowKeys", C.c_int64), ("entriesCap", C.c_int64), ("maxTiles", C.c_int32),
+                ("horizon", C.c_float)]
+
+
+class GsDebugOut(C.Structure):
+    _fields_ = [(name, C.c_void_p) for name in
+                ("rec", "depthKey", "rect", "cnt", "ids", "keys", "numKeys", "tileRanges", "img", "T",
+                 "end", "gC", "accum", "gradF", "shGrad", "stats")]
+
+
+class GsKernelTime(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("total_ms", C.c_double), ("launches", C.c_int64),
+                ("bytes", C.c_double)]
+
+
+# every function the header declares, with its ctypes signature
+SIGNATURES = {
+    "gs_default_opts": ([C.c_void_p], C.c_int),
+    "gs_create": ([C.c_void_p, C.c_void_p], C.c_int),
+    "gs_destroy": ([C.c_void_p], None),
+    "gs_setup": ([C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                  C.c_int32, C.c_int32, C.c_int32, C.c_float], C.c_int),
+    "gs_step": ([C.c_void_p, C.c_int32], C.c_int),
+    "gs_read_loss": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "gs_render": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, C.c_void_p], C.c_int),
+    "gs_read_gaussians": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "gs_eval_cam_psnr": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
+    "gs_set_holdout": ([C.c_void_p, C.c_int32], C.c_int),
+    "gs_set_excluded": ([C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
+    "gs_set_iter": ([C.c_void_p, C.c_int64], C.c_int),
+    "gs_get_info": ([C.c_void_p, C.c_void_p], C.c_int),
+    "gs_sync": ([C.c_void_p], C.c_int),
+    "gs_last_error": ([C.c_void_p], C.c_char_p),
+    "gs_step_debug": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
+    "gs_profile": ([C.c_void_p, C.c_int32], C.c_int),
+    "gs_profile_read": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
+    "gs_stream": ([C.c_void_p], C.c_void_p),
+}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python splat.js_amd/build.py`")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+class GsError(RuntimeError):
+    pass
+
+
+def default_opts(**kw):
+    o = GsOpts()
+    lib().gs_default_opts(C.byref(o))
+    names = {f[0] for f in GsOpts._fields_}
+    for k, v in kw.items():
+        if k not in names:
+            raise KeyError(f"unknown trainer option {k}")
+        setattr(o, k, v)
+    return o
+
+
+def make_camera(c):
+    k = GsCamera()
+    for i in range(9):
+        k.R[i] = float(c["R"][i])
+    for i in range(3):
+        k.t[i] = float(c["t"][i])
+    k.f, k.cx, k.cy = float(c["f"]), float(c["cx"]), float(c["cy"])
+    k.w, k.h = int(c.get("w", 0)), int(c.get("h", 0))
+    k.imgIdx = int(c.get("imgIdx", 0))
+    k.g, k.b = float(c.get("g", 0.0)), float(c.get("b", 0.0))
+    return k
+
+
+class GSTrainer:
+    """Mirror of the reference GSTrainer over the HIP library."""
+
+    def __init__(self, **opts):
+        self.opts = default_opts(**opts)
+        h = C.c_void_p()
+        rc = lib().gs_create(C.byref(self.opts), C.byref(h))
+        if rc != 0:
+            raise GsError(f"gs_create failed ({rc}): no usable HIP device?")
+        self._h = h
+        self.shDeg = self.opts.shDeg
+        self.shK = (self.shDeg + 1) ** 2 - 1 if self.shDeg > 0 else 0
+        self.stride = 16
+
+    @classmethod
+    def create(cls, **opts):
+        return cls(**opts)
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise GsError(lib().gs_last_error(self._h).decode() or f"error {rc}")
+
+    def destroy(self):
+        if self._h:
+            lib().gs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    # trainer.js:100
+    def setup(self, gaussians, cams, images, maxViewW, maxViewH, sceneRadius):
+        data = np.ascontiguousarray(gaussians["data"], np.float32).reshape(-1, 16)
+        n = int(gaussians.get("n", data.shape[0]))
+        sh = gaussians.get("sh")
+        sh = None if sh is None else np.ascontiguousarray(sh, np.float32).reshape(n, -1)
+        self._cams = (GsCamera * len(cams))(*[make_camera(c) for c in cams])
+        self._rgb = [np.ascontiguousarray(im["rgb"], np.float32).reshape(-1) for im in images]
+        ims = (GsImage * len(images))()
+        for i, im in enumerate(images):
+            ims[i].tw, ims[i].th = int(im["tw"]), int(im["th"])
+            ims[i].rgb = self._rgb[i].ctypes.data
+        self._ck(lib().gs_setup(self._h, _p(data), n, _p(sh), self._cams, len(cams), ims, len(images),
+                                int(maxViewW), int(maxViewH), float(sceneRadius)))
+        self.camMeta = [dict(c, w=images[c["imgIdx"]]["tw"], h=images[c["imgIdx"]]["th"]) for c in cams]
+        off = 0
+        for m in self.camMeta:
+            m["offset"] = off
+            off += m["w"] * m["h"]
+
+    def stepOnce(self, cam=-1):
+        self._ck(lib().gs_step(self._h, int(cam)))
+
+    step_once = stepOnce
+
+    def readLoss(self):
+        mse = C.c_double()
+        has = C.c_int32()
+        self._ck(lib().gs_read_loss(self._h, C.byref(mse), C.byref(has)))
+        return mse.value if has.value else None
+
+    def renderView(self, cam, trainMode=0, offset=0):
+        k = make_camera(cam)
+        out = np.zeros((k.h, k.w, 4), np.float32)
+        self._ck(lib().gs_render(self._h, C.byref(k), int(trainMode), int(offset), _p(out)))
+        return out
+
+    def renderTrainCam(self, ci):
+        m = self.camMeta[ci]
+        return self.renderView(m, 0, m["offset"])
+
+    def readGaussians(self):
+        info = self.info()
+        data = np.zeros((info.n, 16), np.float32)
+        sh = np.zeros((info.n, 3 * self.shK), np.float32) if self.shK else None
+        self._ck(lib().gs_read_gaussians(self._h, _p(data), _p(sh)))
+        return {"data": data, "n": info.n, "sh": sh, "shK": self.shK}
+
+    def evalCamPsnr(self, ci):
+        v = C.c_double()
+        self._ck(lib().gs_eval_cam_psnr(self._h, int(ci), C.byref(v)))
+        return v.value
+
+    @property
+    def holdout(self):
+        return self.info().holdout
+
+    @holdout.setter
+    def holdout(self, ci):
+        self._ck(lib().gs_set_holdout(self._h, int(ci)))
+
+    def set_excluded(self, cams):
+        a = np.ascontiguousarray(list(cams), np.int32)
+        self._ck(lib().gs_set_excluded(self._h, _p(a) if len(a) else None, len(a)))
+
+    @property
+    def iter(self):
+        return self.info().iter
+
+    @iter.setter
+    def iter(self, v):
+        self._ck(lib().gs_set_iter(self._h, int(v)))
+
+    @property
+    def n(self):
+        return self.info().n
+
+    def info(self):
+        i = GsInfo()
+        self._ck(lib().gs_get_info(self._h, C.byref(i)))
+        return i
+
+    def sync(self):
+        self._ck(lib().gs_sync(self._h))
+
+    def stream(self):
+        return lib().gs_stream(self._h)
+
+    # ---- inspection ----
+    def step_debug(self, cam, want=("rec", "depthKey", "rect", "cnt", "ids", "keys", "tileRanges", "img", "T",
+                                    "end", "gC", "accum", "gradF", "shGrad", "stats")):
+        info = self.info()
+        n = info.n
+        m = self.camMeta[cam]
+        npx = m["w"] * m["h"]
+        nt = ((m["w"] + 15) // 16) * ((m["h"] + 15) // 16)
+        cap = info.entriesCap
+        shapes = {"rec": ((n, 12), np.float32), "depthKey": ((n,), np.uint32), "rect": ((n, 4), np.uint32),
+                  "cnt": ((n,), np.uint32), "ids": ((cap,), np.uint32), "keys": ((cap,), np.uint64),
+                  "tileRanges": ((nt, 2), np.uint32), "img": ((npx, 4), np.float32), "T": ((npx,), np.float32),
+                  "end": ((npx,), np.uint32), "gC": ((npx, 4), np.float32), "accum": ((n, 10), np.int32),
+                  "gradF": ((n, 16), np.float32), "shGrad": ((n, max(3 * self.shK, 1)), np.float32),
+                  "stats": ((4,), np.float64)}
+        out = {k: np.zeros(*shapes[k]) for k in want}
+        nk = np.zeros(1, np.int64)
+        d = GsDebugOut()
+        for k, a in out.items():
+            setattr(d, k, a.ctypes.data)
+        d.numKeys = nk.ctypes.data
+        self._ck(lib().gs_step_debug(self._h, int(cam), C.byref(d)))
+        K = int(nk[0])
+        for k in ("ids", "keys"):
+            if k in out:
+                out[k] = out[k][:K]
+        out["numKeys"] = K
+        if "shGrad" in out:
+            out["shGrad"] = out["shGrad"][:, :3 * self.shK]
+        return out
+
+    def profile(self, enable=True):
+        self._ck(lib().gs_profile(self._h, 1 if enable else 0))
+
+    def profile_read(self):
+        buf = (GsKernelTime * 64)()
+        cnt = C.c_int32()
+        self._ck(lib().gs_profile_read(self._h, buf, 64, C.byref(cnt)))
+        return {buf[i].name.decode(): {"ms": buf[i].total_ms, "launches": buf[i].launches, "bytes": buf[i].bytes}
+                for i in range(min(cnt.value, 64))}
+
+
+def createTrainer(**opts):
+    """src/index.js:56-58"""
+    return GSTrainer.create(**opts)

@@ -1,0 +1,101 @@
+"""Shared test helpers: oracle-side training steps with the same host
+arithmetic the trainer uses (trainer.js:320-342, :409-424)."""
+import math
+
+import numpy as np
+
+import oracle as O
+import scenes
+
+
+def f32(x):
+    return float(np.float32(x))
+
+
+def adam_for_iter(it, radius, posLr=0.0, horizon=60000, minScale=1e-4, opacityReg=0.01, shLr=1.25e-4):
+    """AdamHP for the step whose Adam t = it (after iter++)."""
+    r = f32(radius)
+    base = float(posLr) if posLr > 0 else 3e-4 * r
+    pos = f32(base * 0.01 ** min(1.0, it / (0.75 * horizon)))
+    lr = [pos] * 3 + [5e-3] * 3 + [1e-3] * 4 + [1.5e-2] * 3 + [2.5e-2, 0, 0]
+    return O.adam_hp(lr, t=it, minLS=f32(math.log(r * f32(minScale))), maxLS=f32(math.log(r * 0.05)),
+                     maxLogit=8.0, reg=f32(opacityReg), shLr=f32(shLr))
+
+
+class OracleTrainer:
+    """The f32 restatement driven like GSTrainer.stepOnce on a fixed camera."""
+
+    def __init__(self, params, sh, cams, images, radius, opts=None, shDeg=2, fixed=True):
+        opts = dict(opts or {})
+        self.o = O.make_opts(eCut=opts.get("eCut", 4.5), aMin=opts.get("aMin", 0.0039),
+                             radClamp=opts.get("radClamp", 1.0), anisoReg=opts.get("anisoReg", 0.005),
+                             shDeg=shDeg)
+        self.opts = opts
+        self.st = O.Stage(self.o, "f32")
+        self.P = np.array(params, np.float32).reshape(-1, 16)
+        n = self.P.shape[0]
+        self.K = O.sh_k(shDeg)
+        self.SH = (np.zeros((n, 3 * self.K), np.float32) if sh is None
+                   else np.array(sh, np.float32).reshape(n, 3 * self.K))
+        self.M = np.zeros_like(self.P)
+        self.V = np.zeros_like(self.P)
+        self.SM = np.zeros_like(self.SH)
+        self.SV = np.zeros_like(self.SH)
+        self.cams = cams
+        self.radius = radius
+        self.shDeg = shDeg
+        self.fixed = fixed
+        self.iter = 0
+        self.tgt = []
+        for c in cams:
+            im = images[c["imgIdx"]]
+            self.tgt.append(O.pack_targets(np.asarray(im["rgb"], np.float32).reshape(-1, 3)))
+        self.cam_deg = [shDeg] * len(cams)
+
+    def step(self, ci):
+        deg = min(self.shDeg, self.iter // 1000)
+        self.cam_deg[ci] = deg
+        cam = O.make_cam(self.cams[ci], shActiveDeg=deg)
+        self.iter += 1
+        st = self.st
+        proj, rect, cnt = st.project(cam, self.P, self.SH if self.K else None)
+        ids, ts, keys = st.bin_sort(cam, proj, rect, cnt)
+        img, T, end = st.render_fwd(cam, proj, ids, ts)
+        gC, valid, stats = st.loss_charb(cam, img, self.tgt[ci])
+        g = st.render_bwd(cam, proj, ids, ts, T, end, gC, valid, fixed=self.fixed)
+        if self.fixed:
+            gradF, shG = st.chain(cam, self.P, self.SH if self.K else None, proj, gfix=g)
+        else:
+            gradF, shG = st.chain(cam, self.P, self.SH if self.K else None, proj, gsum=g)
+        hp = adam_for_iter(self.iter, self.radius, posLr=self.opts.get("posLr", 0.0),
+                           horizon=self.opts.get("maxIters", 60000),
+                           opacityReg=self.opts.get("opacityReg", 0.01), shLr=self.opts.get("shLr", 1.25e-4))
+        pre = {"proj": proj, "rect": rect, "cnt": cnt, "ids": ids, "ts": ts, "keys": keys, "img": img, "T": T,
+               "end": end, "gC": gC, "valid": valid, "stats": stats, "accum": g, "gradF": gradF, "shGrad": shG}
+        O.adam(hp, self.P, gradF, self.M, self.V)
+        if self.K:
+            O.sh_adam(hp, self.SH, shG, self.SM, self.SV)
+        return pre
+
+
+def rig_setup(sh_deg=0):
+    """The reference gradcheck rig as trainer inputs."""
+    params, cams, images, W, H, radius, opts = scenes.gradcheck_rig()
+    sh = scenes.sh_rig_coeffs(params.shape[0], sh_deg) if sh_deg > 0 else None
+    return params, sh, cams, images, W, H, radius, opts
+
+
+def config1_setup(n=20000, seed=0, target_seed=1):
+    """Config 1 with its target rendered (f32 restatement) from a second set."""
+    p, sh, cams = scenes.config1(seed)
+    p, sh = p[:n], sh[:n]
+    gt_p, gt_sh, _ = scenes.config1(target_seed)
+    o = O.make_opts(shDeg=2)
+    st = O.Stage(o, "f32")
+    cam = O.make_cam(cams[0], shActiveDeg=2)
+    proj, rect, cnt = st.project(cam, gt_p, gt_sh)
+    ids, ts, _ = st.bin_sort(cam, proj, rect, cnt)
+    img, _, _ = st.render_fwd(cam, proj, ids, ts)
+    rgb = np.clip(img[:, :3], 0, 1).astype(np.float32)
+    images = [{"tw": 320, "th": 240, "rgb": rgb.reshape(-1)}]
+    return p, sh, cams, images, 320, 240, 1.0

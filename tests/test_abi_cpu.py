@@ -1,0 +1,79 @@
+"""CPU-only checks of the drop-in boundary: the C-ABI library loads, exports
+every entry point include/splat_gs.h declares, and its structs match the
+ctypes mirrors byte for byte. No GPU calls are made."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+import gsabi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "splat_gs.h")
+
+
+def _declared():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(gs_[a-z_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = gsabi.lib()
+    declared = _declared()
+    assert "gs_setup" in declared and "gs_step" in declared
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+    assert set(declared) == set(gsabi.SIGNATURES), set(declared) ^ set(gsabi.SIGNATURES)
+    out = subprocess.run(["nm", "-D", "--defined-only", gsabi.LIB_PATH], stdout=subprocess.PIPE, text=True).stdout
+    for s in declared:
+        assert re.search(rf"\bT {s}\b", out), s
+
+
+def test_default_opts_are_the_references():
+    o = gsabi.default_opts()
+    assert o.size == C.sizeof(gsabi.GsOpts)
+    assert o.eCut == pytest.approx(4.5) and o.aMin == pytest.approx(0.0039)
+    assert o.radClamp == 1.0 and o.shDeg == 2
+    assert o.anisoReg == pytest.approx(0.005) and o.opacityReg == pytest.approx(0.01)
+    assert o.shLr == pytest.approx(1.25e-4) and o.minScale == pytest.approx(1e-4)
+    assert o.maxIters == 60000 and o.maxSplats == 600000 and o.capMult == 4.0
+    assert o.loss == gsabi.GS_LOSS_CHARBONNIER and o.gradAccum == gsabi.GS_ACCUM_FIXED
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """Compile a probe against the header and compare sizes/offsets."""
+    probe = tmp_path / "probe.c"
+    fields = {
+        "gs_opts": [f[0] for f in gsabi.GsOpts._fields_],
+        "gs_camera": [f[0] for f in gsabi.GsCamera._fields_],
+        "gs_image": [f[0] for f in gsabi.GsImage._fields_],
+        "gs_info": [f[0] for f in gsabi.GsInfo._fields_],
+        "gs_debug_out": [f[0] for f in gsabi.GsDebugOut._fields_],
+        "gs_kernel_time": [f[0] for f in gsabi.GsKernelTime._fields_],
+    }
+    lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
+    for s, fs in fields.items():
+        lines.append(f'printf("{s} %zu\\n", sizeof({s}));')
+        for f in fs:
+            lines.append(f'printf("{s}.{f} %zu\\n", offsetof({s}, {f}));')
+    lines.append("return 0;}")
+    probe.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.check_call(["gcc", "-o", str(exe), str(probe)])
+    got = dict(l.split() for l in subprocess.check_output([str(exe)], text=True).splitlines())
+    py = {"gs_opts": gsabi.GsOpts, "gs_camera": gsabi.GsCamera, "gs_image": gsabi.GsImage,
+          "gs_info": gsabi.GsInfo, "gs_debug_out": gsabi.GsDebugOut, "gs_kernel_time": gsabi.GsKernelTime}
+    for s, cls in py.items():
+        assert int(got[s]) == C.sizeof(cls), s
+        for f, _ in cls._fields_:
+            assert int(got[f"{s}.{f}"]) == getattr(cls, f).offset, (s, f)
+
+
+def test_library_is_built_for_gfx950():
+    out = subprocess.run(["/opt/rocm/llvm/bin/llvm-readelf", "-n", gsabi.LIB_PATH], stdout=subprocess.PIPE,
+                         stderr=subprocess.STDOUT, text=True).stdout
+    blob = open(gsabi.LIB_PATH, "rb").read()
+    assert b"gfx950" in blob

@@ -1,0 +1,282 @@
+"""GPU parity: the HIP trainer step against the CPU restatement (oracle/),
+through the C ABI.
+
+Bar: bit-exact for everything the f32 restatement computes in the same order
+(projection records, depth keys, tile rectangles, the sorted (tile, depth,
+id) permutation, tile ranges, images, final T, end indices, dL/dC, the
+reference's fixed-point gradient sums, parameter gradients, Adam updates);
+the f32 atomic-accumulation mode is checked against the f64 reference
+within 1e-3 relative.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import gsabi
+import oracle as O
+import scenes
+from helpers import OracleTrainer, config1_setup, f32, rig_setup
+
+pytestmark = pytest.mark.gpu
+
+
+def _trainer(params, sh, cams, images, W, H, radius, opts=None, shDeg=2, **kw):
+    o = dict(opts or {})
+    o.update(kw)
+    t = gsabi.GSTrainer(shDeg=shDeg, **o)
+    g = {"data": params, "n": params.shape[0]}
+    if sh is not None:
+        g["sh"] = sh
+    t.setup(g, cams, images, W, H, radius)
+    return t
+
+
+def _expected_rec(proj):
+    """Raster record the kernels store, from the restatement's proj rows,
+    with the same f32 arithmetic (opa*comp, 1/(1+lambda_max))."""
+    p = proj.astype(np.float32)
+    va, vb, vc = p[:, 12], p[:, 13], p[:, 14]
+    cva = va + np.float32(0.3)
+    cvc = vc + np.float32(0.3)
+    cmid = np.float32(0.5) * (cva + cvc)
+    lmax = cmid + np.sqrt(np.maximum(cmid * cmid - (cva * cvc - vb * vb), np.float32(0)))
+    cnorm = np.float32(1) / (np.float32(1) + lmax)
+    return np.stack([p[:, 0], p[:, 1], p[:, 3], p[:, 4], p[:, 5], p[:, 7] * p[:, 6], p[:, 8], p[:, 9],
+                     p[:, 10], p[:, 7], p[:, 6], cnorm], 1)
+
+
+def _check_step(gpu, ref, n_tiles, label):
+    vis = ref["cnt"] > 0
+    # projection
+    np.testing.assert_array_equal(gpu["cnt"], ref["cnt"], err_msg=label)
+    np.testing.assert_array_equal(gpu["rect"][vis], ref["rect"][vis].astype(np.uint32), err_msg=label)
+    exp_key = np.where(vis, ref["proj"][:, 2].astype(np.float32).view(np.uint32), np.uint32(0xFFFFFFFF))
+    np.testing.assert_array_equal(gpu["depthKey"], exp_key, err_msg=label)
+    np.testing.assert_array_equal(gpu["rec"][vis].view(np.uint32), _expected_rec(ref["proj"])[vis].view(np.uint32),
+                                  err_msg=label)
+    # binning + sort: bit-exact permutation and keys
+    assert gpu["numKeys"] == len(ref["ids"]), label
+    np.testing.assert_array_equal(gpu["ids"], ref["ids"], err_msg=label)
+    np.testing.assert_array_equal(gpu["keys"], ref["keys"], err_msg=label)
+    cnt_t = np.diff(ref["ts"].astype(np.int64))
+    tr = gpu["tileRanges"].astype(np.int64)
+    np.testing.assert_array_equal(tr[:, 1] - tr[:, 0], cnt_t, err_msg=label)
+    ne = cnt_t > 0
+    np.testing.assert_array_equal(tr[ne, 0], ref["ts"][:-1][ne], err_msg=label)
+    # forward composite
+    np.testing.assert_array_equal(gpu["img"].view(np.uint32), ref["img"].astype(np.float32).view(np.uint32),
+                                  err_msg=label)
+    np.testing.assert_array_equal(gpu["T"].view(np.uint32), ref["T"].view(np.uint32), err_msg=label)
+    np.testing.assert_array_equal(gpu["end"], ref["end"], err_msg=label)
+    # loss gradient
+    np.testing.assert_array_equal(gpu["gC"][:, :3].view(np.uint32), ref["gC"].view(np.uint32), err_msg=label)
+    np.testing.assert_array_equal(gpu["gC"][:, 3] != 0, ref["valid"] != 0, err_msg=label)
+    np.testing.assert_allclose(gpu["stats"][:3], ref["stats"], rtol=1e-9, atol=1e-9, err_msg=label)
+    # backward (fixed point: deterministic, bit-exact) and chain
+    np.testing.assert_array_equal(gpu["accum"], ref["accum"], err_msg=label)
+    np.testing.assert_array_equal(gpu["gradF"][:, :14].view(np.uint32),
+                                  ref["gradF"][:, :14].astype(np.float32).view(np.uint32), err_msg=label)
+    if ref["shGrad"].size:
+        np.testing.assert_array_equal(gpu["shGrad"].view(np.uint32), ref["shGrad"].view(np.uint32),
+                                      err_msg=label)
+
+
+def _run_pair(params, sh, cams, images, W, H, radius, opts, shDeg, steps, start_iter=0, cams_seq=None, **kw):
+    gpu = _trainer(params, sh, cams, images, W, H, radius, opts, shDeg=shDeg, **kw)
+    ref = OracleTrainer(params, sh, cams, images, radius, opts=opts, shDeg=shDeg)
+    gpu.iter = start_iter
+    ref.iter = start_iter
+    n_tiles = ((W + 15) // 16) * ((H + 15) // 16)
+    for s in range(steps):
+        ci = cams_seq[s] if cams_seq else 0
+        g = gpu.step_debug(ci)
+        r = ref.step(ci)
+        _check_step(g, r, n_tiles, f"step {s}")
+        out = gpu.readGaussians()
+        np.testing.assert_array_equal(out["data"].view(np.uint32), ref.P.view(np.uint32), err_msg=f"params step {s}")
+        if ref.K:
+            np.testing.assert_array_equal(out["sh"].view(np.uint32), ref.SH.view(np.uint32), err_msg=f"sh step {s}")
+    gpu.destroy()
+    return ref
+
+
+def test_rig_deg0_bit_exact_three_steps():
+    """The reference's gradcheck rig (gradcheck.js:139-185), SH off."""
+    params, sh, cams, images, W, H, radius, opts = rig_setup(0)
+    _run_pair(params, None, cams, images, W, H, radius, opts, 0, steps=3)
+
+
+def test_rig_deg3_sh_bit_exact():
+    """gradCheckSH rig (gradcheck.js:192-201): degree-3 SH, all bands active."""
+    params, sh, cams, images, W, H, radius, opts = rig_setup(3)
+    _run_pair(params, sh, cams, images, W, H, radius, opts, 3, steps=2, start_iter=5000)
+
+
+def test_config1_bit_exact():
+    """Config 1: 20k Gaussians, 320x240, SH degree 2 (first step: band ramp
+    at degree 0, then full degree)."""
+    p, sh, cams, images, W, H, radius = config1_setup()
+    _run_pair(p, sh, cams, images, W, H, radius, {}, 2, steps=1, start_iter=0)
+    _run_pair(p, sh, cams, images, W, H, radius, {}, 2, steps=2, start_iter=2500)
+
+
+def test_global64_sort_gives_same_permutation():
+    p, sh, cams, images, W, H, radius = config1_setup(n=8000)
+    outs = []
+    for mode in (gsabi.GS_SORT_DEPTH_THEN_TILE, gsabi.GS_SORT_GLOBAL64):
+        t = _trainer(p, sh, cams, images, W, H, radius, {}, shDeg=2, sortMode=mode)
+        outs.append(t.step_debug(0, want=("ids", "keys", "tileRanges", "img")))
+        t.destroy()
+    a, b = outs
+    assert a["numKeys"] == b["numKeys"] > 0
+    np.testing.assert_array_equal(a["ids"], b["ids"])
+    np.testing.assert_array_equal(a["keys"], b["keys"])
+    np.testing.assert_array_equal(a["img"], b["img"])
+
+
+def test_f32_accumulation_matches_f64_gradients():
+    """GS_ACCUM_F32: atomic f32 sums, checked against the double-precision
+    analytic gradients (which central differences pin, test_oracle.py)."""
+    params, sh, cams, images, W, H, radius, opts = rig_setup(0)
+    t = _trainer(params, None, cams, images, W, H, radius, opts, shDeg=0, gradAccum=gsabi.GS_ACCUM_F32)
+    g = t.step_debug(0, want=("gradF", "accum"))
+    t.destroy()
+    o = O.make_opts(eCut=opts["eCut"], aMin=opts["aMin"], radClamp=opts["radClamp"], anisoReg=0.0, shDeg=0)
+    tgt = O.pack_targets(images[0]["rgb"])
+    g64, _ = O.analytic_grads(O.Stage(o, "f64"), O.make_cam(cams[0]), params, None, tgt, fixed=False)
+    for s in range(14):
+        a, b = g["gradF"][:, s].astype(np.float64), g64[:, s]
+        big = np.abs(b) > 1e-3 * max(np.abs(b).max(), 1e-12)
+        rel = np.abs(a[big] - b[big]) / np.abs(b[big])
+        assert np.median(rel) < 1e-4 and rel.max() < 1e-3, (s, rel.max())
+
+
+def test_fixed_point_step_is_deterministic():
+    p, sh, cams, images, W, H, radius = config1_setup(n=6000)
+    res = []
+    for _ in range(2):
+        t = _trainer(p, sh, cams, images, W, H, radius, {}, shDeg=2)
+        for _ in range(3):
+            t.stepOnce(0)
+        res.append(t.readGaussians()["data"])
+        t.destroy()
+    np.testing.assert_array_equal(res[0].view(np.uint32), res[1].view(np.uint32))
+
+
+def test_render_view_and_read_loss():
+    p, sh, cams, images, W, H, radius = config1_setup(n=6000)
+    t = _trainer(p, sh, cams, images, W, H, radius, {}, shDeg=2)
+    ref = OracleTrainer(p, sh, cams, images, radius, shDeg=2)
+    img = t.renderView(cams[0])
+    o = O.make_opts(shDeg=2)
+    st = O.Stage(o, "f32")
+    cam = O.make_cam(cams[0], shActiveDeg=2)
+    proj, rect, cnt = st.project(cam, p, sh)
+    ids, ts, _ = st.bin_sort(cam, proj, rect, cnt)
+    rimg, _, _ = st.render_fwd(cam, proj, ids, ts)
+    np.testing.assert_array_equal(img.reshape(-1, 4).view(np.uint32), rimg.view(np.uint32))
+    assert t.readLoss() is None  # reference returns null before any step
+    t.stepOnce(0)
+    r = ref.step(0)
+    mse = t.readLoss()
+    assert abs(mse - r["stats"][0] / (r["stats"][2] * 3)) <= 1e-9 * mse
+    assert t.readLoss() is None
+    # evalCamPsnr = -10 log10(mse) of a fresh pass at the current parameters
+    psnr = t.evalCamPsnr(0)
+    assert 5 < psnr < 60
+    t.destroy()
+
+
+def test_dense_tiles_and_ragged_image():
+    """Tiles with > 2048 keys (the reference's global-memory bitonic path,
+    shaders.js:471-499), several LDS batches per tile, and a 70x45 view whose
+    edge tiles are partial."""
+    rng = np.random.default_rng(11)
+    n = 3000
+    p = np.zeros((n, 16), np.float32)
+    p[:, 0] = rng.normal(0, 0.15, n)
+    p[:, 1] = rng.normal(0, 0.1, n)
+    p[:, 2] = rng.uniform(2.5, 3.5, n)
+    p[:, 3:6] = rng.normal(-2.0, 0.3, (n, 3))
+    q = rng.normal(size=(n, 4))
+    p[:, 6:10] = q / np.linalg.norm(q, axis=1, keepdims=True)
+    p[:, 10:13] = rng.normal(0, 1, (n, 3))
+    p[:, 13] = rng.normal(-2.5, 0.5, n)
+    W, H = 70, 45
+    cams = [{"imgIdx": 0, "R": [1, 0, 0, 0, 1, 0, 0, 0, 1], "t": [0, 0, 0], "f": 60, "cx": 35, "cy": 22.5,
+             "w": W, "h": H}]
+    rgb = rng.random((H * W, 3)).astype(np.float32)
+    rgb[rng.random(H * W) < 0.05, 0] = -1  # invalid pixels
+    images = [{"tw": W, "th": H, "rgb": rgb}]
+    ref = _run_pair(p, None, cams, images, W, H, 1.0, {}, 0, steps=2)
+    proj, rect, cnt = ref.st.project(O.make_cam(cams[0]), ref.P, None)
+    _, ts, _ = ref.st.bin_sort(O.make_cam(cams[0]), proj, rect, cnt)
+    assert np.diff(ts.astype(np.int64)).max() > 2048
+
+
+def test_all_culled_and_single_gaussian():
+    W, H = 48, 32
+    cams = [{"imgIdx": 0, "R": [1, 0, 0, 0, 1, 0, 0, 0, 1], "t": [0, 0, 0], "f": 40, "cx": 24, "cy": 16,
+             "w": W, "h": H}]
+    images = [{"tw": W, "th": H, "rgb": np.full((W * H, 3), 0.5, np.float32)}]
+    behind = np.zeros((5, 16), np.float32)
+    behind[:, 2] = -3.0
+    behind[:, 6] = 1
+    _run_pair(behind, None, cams, images, W, H, 1.0, {}, 0, steps=2)
+    one = np.zeros((1, 16), np.float32)
+    one[0, 2] = 2.0
+    one[0, 3:6] = math.log(0.3)
+    one[0, 6] = 1
+    one[0, 13] = 1.0
+    _run_pair(one, None, cams, images, W, H, 1.0, {}, 0, steps=2)
+
+
+def test_multi_camera_seeded_pick_and_holdout():
+    """stepOnce() picks cameras with the seeded RNG and skips the holdout
+    (trainer.js:400-406); the picks match a host replay of mulberry32."""
+    p, sh, cams = scenes.config2(n=4000, views=6, W=96, H=64, f=70.0)
+    rng = np.random.default_rng(2)
+    images = [{"tw": 96, "th": 64, "rgb": rng.random((96 * 64, 3)).astype(np.float32)} for _ in cams]
+    t = _trainer(p, sh, cams, images, 96, 64, 5.0, {}, shDeg=2, seed=1234)
+    t.holdout = 2
+    seen = []
+    for _ in range(20):
+        t.stepOnce()
+        seen.append(t.info().lastCam)
+    t.destroy()
+
+    def mulberry(a):
+        a = (a + 0x6D2B79F5) & 0xFFFFFFFF
+        x = ((a ^ (a >> 15)) * (1 | a)) & 0xFFFFFFFF
+        x = ((x + (((x ^ (x >> 7)) * (61 | x)) & 0xFFFFFFFF)) & 0xFFFFFFFF) ^ x
+        return a, (x ^ (x >> 14)) & 0xFFFFFFFF
+    a, exp = 1234, []
+    for _ in range(20):
+        a, r = mulberry(a)
+        ci = int(r / 4294967296.0 * 6)
+        if ci == 2:
+            ci = 3
+        exp.append(ci)
+    assert seen == exp
+
+
+def test_entry_budget_overflow_is_reported():
+    p, sh, cams, images, W, H, radius = config1_setup(n=4000)
+    t = _trainer(p, sh, cams, images, W, H, radius, {}, shDeg=2, entriesCap=500)
+    t.stepOnce(0)
+    info = t.info()
+    assert info.lastKeys == 500
+    assert info.overflowKeys > 0
+    t.destroy()
+
+
+def test_setup_rejects_bad_input():
+    t = gsabi.GSTrainer()
+    with pytest.raises(gsabi.GsError):
+        t.stepOnce()
+    with pytest.raises(gsabi.GsError):
+        t.setup({"data": np.zeros((1, 16), np.float32), "n": 1},
+                [{"imgIdx": 3, "R": [1, 0, 0, 0, 1, 0, 0, 0, 1], "t": [0, 0, 0], "f": 1, "cx": 0, "cy": 0}],
+                [{"tw": 4, "th": 4, "rgb": np.zeros(48, np.float32)}], 4, 4, 1.0)
+    t.destroy()
