@@ -1,0 +1,180 @@
+"""Trainer-step benchmark on one MI355X.
+
+Workload (the benchmark's FP32 single-view configuration): 1M Gaussians
+(means U(-5,5)^3, log-scales N(-4,0.5), random unit quaternions, opacity
+logits N(0,1), DC U(0,1) as logits, SH degree 2 rest N(0,0.05)), a ring of
+64 views at radius 8 looking at the origin, 1600x900, f=1100. Targets are
+renders of a second seeded 1M-Gaussian set. Each step = projection, 16x16
+tile binning + radix sort, raster forward + Charbonnier loss, raster
+backward, chain + Adam (position lr 1.6e-4) on one view picked by the
+seeded camera RNG. All three SH bands are active (iteration counter starts
+at 2000, past the one-band-per-1000-iterations ramp).
+
+Prints one JSON line (the driver's contract) with the roofline of the
+dominant kernel (HIP events on the trainer's stream over the timed region)
+and a CPU baseline (the C restatement under oracle/, one thread, one full
+step of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "splat.js_amd", "py"))
+
+import numpy as np  # noqa: E402
+
+import gsabi  # noqa: E402
+import scenes  # noqa: E402
+
+METRIC = "train iters/s @1M splats 1600x900 (fwd+bwd+sort), mixed vs FP32, ≤0.05dB PSNR"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_workload(n, views, W, H, f, seed=0, gt_seed=1):
+    p, sh, cams = scenes.config2(n=n, seed=seed, views=views, W=W, H=H, f=f)
+    gt_p, gt_sh, _ = scenes.config2(n=n, seed=gt_seed, views=views, W=W, H=H, f=f)
+    t0 = time.time()
+    gt = gsabi.GSTrainer(shDeg=2, maxSplats=n)
+    blank = [{"tw": W, "th": H, "rgb": np.zeros(W * H * 3, np.float32)}]
+    gt.setup({"data": gt_p, "n": n, "sh": gt_sh}, [dict(cams[0], imgIdx=0)], blank, W, H, 8.0)
+    images = []
+    for c in cams:
+        img = gt.renderView(c)
+        images.append({"tw": W, "th": H, "rgb": np.clip(img[..., :3], 0, 1).reshape(-1).copy()})
+    gt.destroy()
+    log(f"rendered {views} targets in {time.time() - t0:.1f}s")
+    return p, sh, cams, images
+
+
+def algorithmic_bytes(name, n, vis, keys, npx, shC):
+    """HBM bytes each kernel has to move per launch (f32):
+    project   n*(56 params + 16 key/rect/count) + vis*(4*shC SH + 48 record)
+    raster fwd  keys*(4 id + 48 record) + px*(16 img + 4 T + 4 end + 16 dL/dC + 4 target)
+    raster bwd  keys*(4 id + 48 record) + px*(16 dL/dC + 4 T + 4 end) + vis*40 gradient sums
+    update    n*(40+40 sums + 4 count + 3*(56+56) params/m/v) + n*3*2*4*shC SH/m/v"""
+    if name == "project":
+        return n * 72.0 + vis * (4.0 * shC + 48.0)
+    if name == "raster_fwd_loss":
+        return keys * 52.0 + npx * 44.0
+    if name == "raster_bwd":
+        return keys * 52.0 + npx * 24.0 + vis * 40.0
+    if name == "update":
+        return n * (84.0 + 3 * 112.0) + n * 24.0 * shC
+    return None
+
+
+def cpu_baseline(p, sh, cam, image, radius):
+    """One full f32 step of the C restatement (single thread) on the same
+    inputs: projection, brute-force binning sort, forward, loss, backward,
+    chain, Adam."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import OracleTrainer  # CPU restatement, used here only as the baseline
+    ref = OracleTrainer(p, sh, [dict(cam, imgIdx=0)], [image], radius, opts={"posLr": 1.6e-4}, shDeg=2)
+    ref.iter = 2000
+    t0 = time.perf_counter()
+    ref.step(0)
+    return time.perf_counter() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--views", type=int, default=64)
+    ap.add_argument("--width", type=int, default=1600)
+    ap.add_argument("--height", type=int, default=900)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    W, H, f = args.width, args.height, 1100.0 * args.width / 1600.0
+    p, sh, cams, images = build_workload(args.n, args.views, W, H, f)
+    t = gsabi.GSTrainer(shDeg=2, maxSplats=args.n, posLr=1.6e-4, seed=7)
+    t.setup({"data": p, "n": args.n, "sh": sh}, cams, images, W, H, 8.0)
+    t.iter = 2000
+    log("warmup")
+    for _ in range(args.warmup):
+        t.stepOnce()
+    t.sync()
+
+    # timed region: K steps, per-stage HIP events on the trainer's stream
+    t.profile(True)
+    keys, vis = [], []
+    t.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t.stepOnce()
+    t.sync()
+    dt = time.perf_counter() - t0
+    prof = t.profile_read()
+    t.profile(False)
+    info = t.info()
+    # key and visible counts vary by view: sample them over a pass of steps after the timed region
+    for _ in range(min(16, args.steps)):
+        t.stepOnce()
+        i = t.info()
+        keys.append(i.lastKeys)
+        vis.append(i.lastVisible)
+    mean_keys = float(np.mean(keys))
+    mean_vis = float(np.mean(vis))
+    mse = t.readLoss()
+    t.destroy()
+
+    its = args.steps / dt
+    npx = W * H
+    shC = 24
+    kernels = {}
+    for name, v in prof.items():
+        avg_ms = v["ms"] / max(v["launches"], 1)
+        b = algorithmic_bytes(name, args.n, mean_vis, mean_keys, npx, shC)
+        kernels[name] = {"avg_ms": round(avg_ms, 4),
+                         "GBps": None if b is None else round(b / (avg_ms * 1e-3) / 1e9, 1)}
+    single = {k: v for k, v in kernels.items() if v["GBps"] is not None}
+    dom = max(single, key=lambda k: single[k]["avg_ms"])
+    dom_b = algorithmic_bytes(dom, args.n, mean_vis, mean_keys, npx, shC)
+    achieved = dom_b / (single[dom]["avg_ms"] * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_1M_1600x900.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline (one f32 step of the C restatement, 1 thread)")
+        sec = cpu_baseline(p, sh, cams[0], images[0], 8.0)
+        cpu = {"value": round(1.0 / sec, 5), "unit": "iters/s", "cores": 1, "kind": "port",
+               "sample": f"1 full training step (project, sort, fwd, loss, bwd, chain, Adam) of the same "
+                         f"{args.n}-Gaussian {W}x{H} workload on view 0, f32 C restatement, {sec:.1f}s"}
+
+    line = {
+        "metric": METRIC, "value": round(its, 2), "unit": "iters/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"FP32 single-view trainer step, {args.n} Gaussians SH deg 2, "
+                               f"{args.views}-view ring r=8, {W}x{H} f={f:g}",
+                   "gaussians": args.n, "width": W, "height": H, "views": args.views, "sh_degree": 2,
+                   "keys_per_step": round(mean_keys), "visible_per_step": round(mean_vis),
+                   "sorted_keys_per_s": round(mean_keys * its), "kernels": kernels,
+                   "train_mse": mse},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

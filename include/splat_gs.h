@@ -92,6 +92,7 @@ typedef struct gs_info {
   int32_t shDeg, shK, stride, numCams;
   int32_t lastCam, holdout;
   int64_t lastKeys;          /* (tile, Gaussian) keys of the last raster */
+  int64_t lastVisible;       /* Gaussians that passed projection culling in the last raster */
   int64_t overflowKeys;      /* keys dropped for the entry budget, cumulative */
   int64_t entriesCap;
   int32_t maxTiles;

@@ -76,7 +76,7 @@ def build_addon():
     cxx = shutil.which("g++") or "g++"
     tmp = ADDON + ".tmp"
     _run([cxx, "-O2", "-fPIC", "-shared", "-std=c++17", "-I" + inc, "-I" + os.path.join(ROOT, "include"),
-          "-DNAPI_VERSION=8", src, "-o", tmp, "-ldl"])
+          "-DNAPI_VERSION=8", "-DNODE_GYP_MODULE_NAME=gs_napi", src, "-o", tmp, "-ldl"])
     os.replace(tmp, ADDON)
     return ADDON
 

@@ -65,7 +65,7 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, int64_t count, uint32_t v);
 
 void launch_project(hipStream_t st, int deg, int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
                     const float4* P, const float4* SH, float4* rec, uint32_t* depthKey, uint2* rect,
-                    uint32_t* cnt);
+                    uint32_t* cnt, uint32_t* numVisible);
 
 int tile_bits(int numTiles);
 void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out);

@@ -21,7 +21,8 @@ __global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam 
                                                     float4* __restrict__ rec,
                                                     uint32_t* __restrict__ depthKey,
                                                     uint2* __restrict__ rect,
-                                                    uint32_t* __restrict__ cnt) {
+                                                    uint32_t* __restrict__ cnt,
+                                                    uint32_t* __restrict__ numVisible) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   float p[16];
@@ -88,18 +89,22 @@ __global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam 
   depthKey[i] = key;
   rect[i] = rc;
   cnt[i] = c;
+  const uint64_t vis = __ballot(c != 0u);
+  if (__lane_id() == (uint32_t)__ffsll((unsigned long long)__ballot(1)) - 1u && vis)
+    atomicAdd(numVisible, (uint32_t)__popcll(vis));
 }
 
 void launch_project(hipStream_t st, int deg, int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
                     const float4* P, const float4* SH, float4* rec, uint32_t* depthKey, uint2* rect,
-                    uint32_t* cnt) {
+                    uint32_t* cnt, uint32_t* numVisible) {
   if (n <= 0) return;
+  launch_fill_u32(st, numVisible, 1, 0u);
   const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
   switch (deg) {
-    case 0: k_project<0><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
-    case 1: k_project<1><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
-    case 2: k_project<2><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
-    default: k_project<3><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
+    case 0: k_project<0><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, numVisible); break;
+    case 1: k_project<1><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, numVisible); break;
+    case 2: k_project<2><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, numVisible); break;
+    default: k_project<3><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, numVisible); break;
   }
 }
 

@@ -91,6 +91,7 @@ struct gs_trainer {
   SortScratch sort{};
   uint32_t* dN = nullptr;
   uint32_t* numKeys = nullptr;
+  uint32_t* numVisible = nullptr;
   uint64_t* totalKeys = nullptr;
   uint2* ranges = nullptr;
   float4* img = nullptr;
@@ -272,7 +273,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
   {
     Scope s(h, "project", (double)n * (56.0 + 4.0 * h->shC + 48.0 + 16.0));
     launch_project(h->st, h->shDeg, n, h->cap, cam, make_cut(h), h->P, h->SH, h->rec, h->depthKey,
-                   h->rect, h->cnt);
+                   h->rect, h->cnt, h->numVisible);
   }
   {
     BinArgs a{};
@@ -618,6 +619,7 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   rc |= dalloc(h, &h->sort.partials, (size_t)kSortBlocks * 8);
   rc |= dalloc(h, &h->dN, 4);
   rc |= dalloc(h, &h->numKeys, 4);
+  rc |= dalloc(h, &h->numVisible, 4);
   rc |= dalloc(h, &h->totalKeys, 8);
   rc |= dalloc(h, &h->ranges, (size_t)h->maxTiles * 8);
   rc |= dalloc(h, &h->img, (size_t)h->maxPix * 16);
@@ -778,11 +780,13 @@ int gs_get_info(gs_trainer_t* h, gs_info* out) {
   out->maxTiles = h->maxTiles;
   out->horizon = (float)h->horizon;
   if (h->ready) {
-    uint32_t nk = 0;
+    uint32_t nk = 0, nv = 0;
     uint64_t tk = 0;
     double s[4];
     HIPCK(hipStreamSynchronize(h->st));
     HIPCK(hipMemcpy(&nk, h->numKeys, 4, hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpy(&nv, h->numVisible, 4, hipMemcpyDeviceToHost));
+    out->lastVisible = nv;
     HIPCK(hipMemcpy(&tk, h->totalKeys, 8, hipMemcpyDeviceToHost));
     HIPCK(hipMemcpy(s, h->stats, 32, hipMemcpyDeviceToHost));
     out->lastKeys = nk;

@@ -1,0 +1,134 @@
+// trainer.mjs — GSTrainer over the MI355X HIP trainer (N-API addon).
+//
+// Keeps the method surface of the reference's WebGPU GSTrainer
+// (src/gs/trainer.js) so Session-style callers can drop it in:
+//   GSTrainer.create(opts) / setup(gaussians, cams, images, maxViewW,
+//   maxViewH, sceneRadius) / stepOnce() / readLoss() / renderView(cam, out,
+//   trainMode, offset) / renderTrainCam(ci, out) / readGaussians() /
+//   evalCamPsnr(ci), the fields callers read (iter, n, cap, camMeta,
+//   holdout, excluded, lastCam, horizon, shK, stride, tileGrad, opts) and
+//   device.queue.onSubmittedWorkDone().
+// Differences: renderView writes into a caller Float32Array (w*h*4) instead
+// of a canvas context (Node has no canvas); refine() is not provided by this
+// build. Syntax stays ES2019 (no ?? or ?.) for Node 12.
+import { createRequire } from 'module';
+
+const require = createRequire(import.meta.url);
+const addon = require('../addon/gs_napi.node');
+
+export const STRIDE = 16;
+export const TILE = 16;
+export const LOSS = { charbonnier: 0, l1_dssim: 1 };
+export const ACCUM = { fixed: 0, f32: 1 };
+export const SORT = { depthThenTile: 0, global64: 1 };
+
+const shRestCoefs = (deg) => (deg + 1) * (deg + 1) - 1;
+
+export class GSTrainer {
+  /** trainer.js:13-16 */
+  static async create(opts = {}) {
+    return new GSTrainer(opts);
+  }
+
+  constructor(opts = {}) {
+    if (!addon.loaded()) throw new Error(addon.loadError());
+    this.opts = opts;
+    const o = Object.assign({}, opts);
+    if (typeof o.loss === 'string') o.loss = LOSS[o.loss];
+    if (typeof o.gradAccum === 'string') o.gradAccum = ACCUM[o.gradAccum];
+    if (typeof o.sortMode === 'string') o.sortMode = SORT[o.sortMode];
+    this._h = addon.create(o);
+    this.tileGrad = opts.tileGrad === undefined ? true : opts.tileGrad;
+    this.shDeg = opts.shDeg === undefined ? 2 : opts.shDeg;
+    this.shK = this.shDeg > 0 ? shRestCoefs(this.shDeg) : 0;
+    this.stride = STRIDE;
+    this.pixelsSeen = 0;
+    this.lastRefine = 0;
+    this.excluded = null;
+    const self = this;
+    // session.js:520, :542 wait on the queue between frames
+    this.device = {
+      queue: { onSubmittedWorkDone() { addon.sync(self._h); return Promise.resolve(); } },
+      destroy() { self.destroy(); },
+    };
+  }
+
+  /** trainer.js:100-355 */
+  setup(gaussians, cams, images, maxViewW, maxViewH, sceneRadius) {
+    const sh = gaussians.sh || null;
+    addon.setup(this._h, gaussians.data, gaussians.n, sh, cams, images, maxViewW | 0, maxViewH | 0, sceneRadius);
+    this.cams = cams;
+    this.sceneRadius = sceneRadius;
+    let total = 0;
+    this.camMeta = cams.map((c) => {
+      const im = images[c.imgIdx];
+      const meta = Object.assign({}, c, { w: im.tw, h: im.th, offset: total });
+      total += im.tw * im.th;
+      return meta;
+    });
+    for (const m of this.camMeta) m.f0 = m.f;
+    this.horizon = this.opts.maxIters === undefined ? 60000 : this.opts.maxIters;
+  }
+
+  get iter() { return addon.info(this._h).iter; }
+  set iter(v) { addon.setIter(this._h, v); }
+  get n() { return addon.info(this._h).n; }
+  get cap() { return addon.info(this._h).cap; }
+  get lastCam() { return addon.info(this._h).lastCam; }
+  get holdout() { return addon.info(this._h).holdout; }
+  set holdout(ci) { addon.setHoldout(this._h, ci); }
+  get entryOverflowTiles() { return addon.info(this._h).overflowKeys; }
+
+  /** trainer.js:398-471; `excluded` is a Set of camera indices as in :403 */
+  stepOnce(cam = -1) {
+    if (this.excluded !== this._excludedSent) {
+      addon.setExcluded(this._h, this.excluded ? Array.from(this.excluded) : []);
+      this._excludedSent = this.excluded;
+    }
+    addon.step(this._h, cam);
+  }
+
+  /** trainer.js:615-631: MSE per valid pixel-channel since the last call */
+  async readLoss() {
+    return addon.readLoss(this._h);
+  }
+
+  /** trainer.js:634-655: renders into `out` (Float32Array w*h*4) */
+  renderView(camParams, out, trainMode = 0, offset = 0) {
+    const buf = out || new Float32Array(camParams.w * camParams.h * 4);
+    addon.render(this._h, camParams, trainMode, offset, buf);
+    return buf;
+  }
+
+  /** trainer.js:657-660 */
+  renderTrainCam(ci, out) {
+    const meta = this.camMeta[ci];
+    return this.renderView(meta, out, 0, meta.offset);
+  }
+
+  /** trainer.js:797-817 */
+  async readGaussians() {
+    const n = this.n;
+    const data = new Float32Array(n * STRIDE);
+    const sh = this.shK ? new Float32Array(n * this.shK * 3) : null;
+    addon.readGaussians(this._h, data, sh);
+    return { data, n, sh, shK: this.shK };
+  }
+
+  /** trainer.js:577-579 */
+  async evalCamPsnr(ci) {
+    this.pixelsSeen = 0;
+    return addon.evalCamPsnr(this._h, ci);
+  }
+
+  async refine() {
+    throw new Error('refine() (MCMC-lite relocation) is not part of this build');
+  }
+
+  info() { return addon.info(this._h); }
+
+  destroy() {
+    // the addon frees the trainer when the handle is collected; sync first
+    addon.sync(this._h);
+  }
+}

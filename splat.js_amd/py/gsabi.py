@@ -42,7 +42,7 @@ class GsImage(C.Structure):
 class GsInfo(C.Structure):
     _fields_ = [("n", C.c_int64), ("cap", C.c_int64), ("iter", C.c_int64), ("shDeg", C.c_int32),
                 ("shK", C.c_int32), ("stride", C.c_int32), ("numCams", C.c_int32),
-                ("lastCam", C.c_int32), ("holdout", C.c_int32), ("lastKeys", C.c_int64),
+                ("lastCam", C.c_int32), ("holdout", C.c_int32), ("lastKeys", C.c_int64), ("lastVisible", C.c_int64),
                 ("overflowKeys", C.c_int64), ("entriesCap", C.c_int64), ("maxTiles", C.c_int32),
                 ("horizon", C.c_float)]
 

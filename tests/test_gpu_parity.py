@@ -46,7 +46,7 @@ def _expected_rec(proj):
                      p[:, 10], p[:, 7], p[:, 6], cnorm], 1)
 
 
-def _check_step(gpu, ref, n_tiles, label):
+def _check_step(gpu, ref, n_tiles, label, stats_acc=None):
     vis = ref["cnt"] > 0
     # projection
     np.testing.assert_array_equal(gpu["cnt"], ref["cnt"], err_msg=label)
@@ -72,7 +72,9 @@ def _check_step(gpu, ref, n_tiles, label):
     # loss gradient
     np.testing.assert_array_equal(gpu["gC"][:, :3].view(np.uint32), ref["gC"].view(np.uint32), err_msg=label)
     np.testing.assert_array_equal(gpu["gC"][:, 3] != 0, ref["valid"] != 0, err_msg=label)
-    np.testing.assert_allclose(gpu["stats"][:3], ref["stats"], rtol=1e-9, atol=1e-9, err_msg=label)
+    # the loss stats accumulate over steps until readLoss() (trainer.js:615-631)
+    exp_stats = ref["stats"] if stats_acc is None else stats_acc
+    np.testing.assert_allclose(gpu["stats"][:3], exp_stats, rtol=1e-9, atol=1e-9, err_msg=label)
     # backward (fixed point: deterministic, bit-exact) and chain
     np.testing.assert_array_equal(gpu["accum"], ref["accum"], err_msg=label)
     np.testing.assert_array_equal(gpu["gradF"][:, :14].view(np.uint32),
@@ -88,11 +90,13 @@ def _run_pair(params, sh, cams, images, W, H, radius, opts, shDeg, steps, start_
     gpu.iter = start_iter
     ref.iter = start_iter
     n_tiles = ((W + 15) // 16) * ((H + 15) // 16)
+    acc = np.zeros(3)
     for s in range(steps):
         ci = cams_seq[s] if cams_seq else 0
         g = gpu.step_debug(ci)
         r = ref.step(ci)
-        _check_step(g, r, n_tiles, f"step {s}")
+        acc = acc + r["stats"]
+        _check_step(g, r, n_tiles, f"step {s}", stats_acc=acc)
         out = gpu.readGaussians()
         np.testing.assert_array_equal(out["data"].view(np.uint32), ref.P.view(np.uint32), err_msg=f"params step {s}")
         if ref.K:
