@@ -159,7 +159,7 @@ const char* gs_last_error(gs_trainer_t* h);
 
 /* Host buffers for one step's intermediates; NULL members are skipped. */
 typedef struct gs_debug_out {
-  float* rec;           /* n x 12 raster records (mx,my,A,B, C,opa*comp,r,g, b,opa,comp,cnorm) */
+  float* rec;           /* n x 16 raster records (mx,my,A,B, C,opa*comp,ex,ey, r,g,b,opa, comp,cnorm,0,0) */
   uint32_t* depthKey;   /* n, f32 depth bits or 0xffffffff when culled */
   uint32_t* rect;       /* n x 4 (tx0, tx1, ty0, ty1) */
   uint32_t* cnt;        /* n, tiles covered (0 = culled) */

@@ -344,11 +344,13 @@ void FN(orc_render_bwd)(const orc_opts* o, const orc_cam* cam, const REAL* proj,
         const REAL alpha = FMIN(RL(0.99), araw);
         if (!(alpha >= A_MIN)) continue;
         const REAL c[3] = {pr[8], pr[9], pr[10]};
-        const REAL om = RL(1.0) - alpha;
-        const REAL Tb = Ta / om;
+        /* Tb = Ta / (1 - alpha) and S / (1 - alpha) (shaders.js:656, :660),
+         * both through one reciprocal of (1 - alpha) */
+        const REAL rom = RL(1.0) / (RL(1.0) - alpha);
+        const REAL Tb = Ta * rom;
         const REAL aT = alpha * Tb;
         REAL u[3];
-        for (int ch = 0; ch < 3; ch++) u[ch] = c[ch] * Tb - S[ch] / om;
+        for (int ch = 0; ch < 3; ch++) u[ch] = c[ch] * Tb - S[ch] * rom;
         REAL galpha = FN(dot3)(gC, u);
         if (araw > RL(0.99)) galpha = 0;
         const REAL ga = galpha * araw;

@@ -13,6 +13,7 @@ struct SortScratch {
   uint32_t* hist;         // 256 x G
   uint32_t* totals;       // 256
   uint64_t* partials;     // G
+  uint32_t* vpartials;    // G: visible Gaussians per block
 };
 
 struct BinArgs {
@@ -27,6 +28,7 @@ struct BinArgs {
   void *tkA, *tkB;                    // key buffers (4 or 8 B per key)
   uint32_t *tvA, *tvB;                // id buffers
   uint32_t* numKeys;      // device: min(total, cap)
+  uint32_t* numVisible;   // device: Gaussians with at least one key
   uint64_t* totalKeys;    // device: keys before the budget
   uint2* ranges;          // numTiles
   double* stats;          // [3] += dropped keys
@@ -65,7 +67,7 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, int64_t count, uint32_t v);
 
 void launch_project(hipStream_t st, int deg, int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
                     const float4* P, const float4* SH, float4* rec, uint32_t* depthKey, uint2* rect,
-                    uint32_t* cnt, uint32_t* numVisible);
+                    uint32_t* cnt);
 
 int tile_bits(int numTiles);
 void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out);

@@ -3,8 +3,12 @@
 // compensation, SH colour, opacity-aware radius, tile rectangle.
 //
 // Output per visible Gaussian:
-//   rec[3i..3i+2]  raster record, 48 B: (mx, my, A, B), (C, opa*comp, r, g),
-//                  (b, opa, comp, cnorm) — exactly what the raster passes read
+//   rec[4i..4i+3]  raster record, 64 B: (mx, my, A, B), (C, opa*comp, ex, ey),
+//                  (r, g, b, opa), (comp, cnorm, 0, 0). ex, ey bound the
+//                  ellipse where the splat can reach alpha >= A_MIN (half
+//                  extents sqrt(2 eMax Vd_xx), sqrt(2 eMax Vd_yy), widened by
+//                  1% + 0.05 px); the raster passes use them only to skip
+//                  whole waves, never to change a pixel's value.
 //   depthKey[i]    f32 depth bits (positive, so they order like the depths);
 //                  0xffffffff when culled, which sorts culled Gaussians last
 //   rect[i]        (tx0 | tx1 << 16, ty0 | ty1 << 16)
@@ -21,8 +25,7 @@ __global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam 
                                                     float4* __restrict__ rec,
                                                     uint32_t* __restrict__ depthKey,
                                                     uint2* __restrict__ rect,
-                                                    uint32_t* __restrict__ cnt,
-                                                    uint32_t* __restrict__ numVisible) {
+                                                    uint32_t* __restrict__ cnt) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   float p[16];
@@ -73,9 +76,12 @@ __global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam 
       col[2] = fmaxf(col[2], 0.0f);
     }
     const float cnorm = 1.0f / (1.0f + conic_lmax(g.va, g.vb, g.vc));
-    rec[3 * i + 0] = make_float4(mx, my, cd * inv, (-g.vb) * inv);
-    rec[3 * i + 1] = make_float4(ad * inv, opa * comp, col[0], col[1]);
-    rec[3 * i + 2] = make_float4(col[2], opa, comp, cnorm);
+    const float ex = sqrtf(2.0f * eMax * ad) * 1.01f + 0.05f;
+    const float ey = sqrtf(2.0f * eMax * cd) * 1.01f + 0.05f;
+    rec[4 * i + 0] = make_float4(mx, my, cd * inv, (-g.vb) * inv);
+    rec[4 * i + 1] = make_float4(ad * inv, opa * comp, ex, ey);
+    rec[4 * i + 2] = make_float4(col[0], col[1], col[2], opa);
+    rec[4 * i + 3] = make_float4(comp, cnorm, 0.0f, 0.0f);
     key = gs_f2u(g.pc[2]);
     const float TF = 16.0f;
     const int tx0 = (int)clampf(floorf((mx - rad) / TF), 0.0f, (float)(cam.tilesX - 1));
@@ -89,22 +95,18 @@ __global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam 
   depthKey[i] = key;
   rect[i] = rc;
   cnt[i] = c;
-  const uint64_t vis = __ballot(c != 0u);
-  if (__lane_id() == (uint32_t)__ffsll((unsigned long long)__ballot(1)) - 1u && vis)
-    atomicAdd(numVisible, (uint32_t)__popcll(vis));
 }
 
 void launch_project(hipStream_t st, int deg, int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
                     const float4* P, const float4* SH, float4* rec, uint32_t* depthKey, uint2* rect,
-                    uint32_t* cnt, uint32_t* numVisible) {
+                    uint32_t* cnt) {
   if (n <= 0) return;
-  launch_fill_u32(st, numVisible, 1, 0u);
   const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
   switch (deg) {
-    case 0: k_project<0><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, numVisible); break;
-    case 1: k_project<1><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, numVisible); break;
-    case 2: k_project<2><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, numVisible); break;
-    default: k_project<3><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, numVisible); break;
+    case 0: k_project<0><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
+    case 1: k_project<1><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
+    case 2: k_project<2><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
+    default: k_project<3><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
   }
 }
 

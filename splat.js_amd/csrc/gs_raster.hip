@@ -1,11 +1,13 @@
 // gs_raster.hip — per-tile alpha compositing, forward and backward
 // (the reference's `render` pass, shaders.js:511-702).
 //
-// One 256-thread workgroup (4 waves) per 16x16 tile, one thread per pixel.
-// The tile's sorted keys are consumed in batches of 256: every thread loads
-// one raster record (48 B) into LDS, then every pixel walks the batch from
-// LDS (broadcast reads). The forward stops the whole tile once every pixel
-// has reached T < 1e-4.
+// One 256-thread workgroup (4 waves) per 16x16 tile, one thread per pixel,
+// each wave an 8x8 block. The tile's sorted keys are consumed in batches of
+// 256: every thread loads one raster record into LDS, then every pixel walks
+// the batch from LDS (broadcast reads). A wave skips, as a whole, every splat
+// whose alpha >= A_MIN ellipse misses its 8x8 block (the pixels it skips
+// could not have taken the splat). The forward stops the whole tile once
+// every pixel has reached T < 1e-4.
 //
 // Forward (shaders.js:556-585) writes the image, the final transmittance T
 // and `end` (one past the last key the pixel processed); in training it also
@@ -14,7 +16,8 @@
 //
 // Backward (shaders.js:620-700) walks each tile from the largest `end` of
 // its pixels down to the first key, recomputing each pixel's transmittance
-// in front of a splat by division from the stored final T (Tb = Ta/(1-a)).
+// in front of a splat from the stored final T (Tb = Ta/(1-a), through one
+// reciprocal that also scales the colour-behind sum S).
 // Per key, the 10 screen-space gradient terms of the wave's 64 pixels are
 // summed with a transpose reduction (13 lane shuffles instead of 60) and
 // added to the Gaussian's accumulator with one atomic per term per wave:
@@ -30,16 +33,40 @@ namespace gs {
 
 __device__ __forceinline__ int tile_of_block(const Cam& cam) { return (int)blockIdx.x; }
 
+// Each wave owns an 8x8 pixel block of the tile (wave w: columns 8*(w&1),
+// rows 8*(w>>1)); lane l is pixel (l & 7, l >> 3) of that block.
+struct PixelMap {
+  int x, y;        // pixel
+  float bx0, by0;  // centre of the block's first pixel
+};
+__device__ __forceinline__ PixelMap pixel_of_thread(const Cam& cam, int tile, int tid) {
+  const int tx = tile % cam.tilesX, ty = tile / cam.tilesX;
+  const int w = tid >> 6, l = tid & 63;
+  const int bx = tx * kTile + (w & 1) * 8, by = ty * kTile + (w >> 1) * 8;
+  PixelMap m;
+  m.x = bx + (l & 7);
+  m.y = by + (l >> 3);
+  m.bx0 = (float)bx + 0.5f;
+  m.by0 = (float)by + 0.5f;
+  return m;
+}
+
+// true when the splat's alpha >= A_MIN ellipse (bounding box mx +- ex,
+// my +- ey) cannot reach any pixel centre of the wave's 8x8 block
+__device__ __forceinline__ bool wave_misses(float mx, float my, float ex, float ey, const PixelMap& m) {
+  return mx + ex < m.bx0 || mx - ex > m.bx0 + 7.0f || my + ey < m.by0 || my - ey > m.by0 + 7.0f;
+}
+
 template <int kLoss>
 __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterBufs b) {
-  __shared__ float4 sA[kBlock];
-  __shared__ float4 sB[kBlock];
-  __shared__ float sC[kBlock];
+  __shared__ float4 sA[kBlock];   // mx, my, A, B
+  __shared__ float4 sB[kBlock];   // C, opa*comp, ex, ey
+  __shared__ float4 sC[kBlock];   // r, g, b, opa
   __shared__ double sStat[3][kBlock / 64];
   const int tid = threadIdx.x;
   const int tile = tile_of_block(cam);
-  const int tx = tile % cam.tilesX, ty = tile / cam.tilesX;
-  const int x = tx * kTile + (tid & 15), y = ty * kTile + (tid >> 4);
+  const PixelMap pm = pixel_of_thread(cam, tile, tid);
+  const int x = pm.x, y = pm.y;
   const bool inside = x < cam.W && y < cam.H;
   const float px = (float)x + 0.5f, py = (float)y + 0.5f;
   const uint2 rg = b.ranges[tile];
@@ -52,9 +79,9 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
     const uint32_t k = base + tid;
     if (k < e) {
       const uint32_t id = b.ids[k];
-      sA[tid] = b.rec[3 * id + 0];
-      sB[tid] = b.rec[3 * id + 1];
-      sC[tid] = b.rec[3 * id + 2].x;
+      sA[tid] = b.rec[4 * id + 0];
+      sB[tid] = b.rec[4 * id + 1];
+      sC[tid] = b.rec[4 * id + 2];
     }
     __syncthreads();
     if (!done) {
@@ -63,6 +90,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
       for (; j < cnt; j++) {
         const float4 a = sA[j];
         const float4 bb = sB[j];
+        if (wave_misses(a.x, a.y, bb.z, bb.w, pm)) continue;
         const float dx = px - a.x, dy = py - a.y;
         const float t2 = fmaf(bb.x * dy, dy, (a.z * dx) * dx);
         const float ee = fmaxf(fmaf(0.5f, t2, (a.w * dx) * dy), 0.0f);
@@ -70,10 +98,11 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
         const float araw = bb.y * gs_expf(-ee);
         const float alpha = fminf(0.99f, araw);
         if (alpha < cut.aMin) continue;
+        const float4 c4 = sC[j];
         const float w = T * alpha;
-        C0 = fmaf(w, bb.z, C0);
-        C1 = fmaf(w, bb.w, C1);
-        C2 = fmaf(w, sC[j], C2);
+        C0 = fmaf(w, c4.x, C0);
+        C1 = fmaf(w, c4.y, C1);
+        C2 = fmaf(w, c4.z, C2);
         T = T * (1.0f - alpha);
         if (T < 1e-4f) { done = true; break; }
       }
@@ -176,15 +205,16 @@ __device__ __forceinline__ T wave_reduce10(const T* v, int& slot) {
 
 template <int kAccum>
 __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterBufs b, void* accum) {
-  __shared__ float4 sA[kBlock];
-  __shared__ float4 sB[kBlock];
-  __shared__ float4 sC[kBlock];
+  __shared__ float4 sA[kBlock];   // mx, my, A, B
+  __shared__ float4 sB[kBlock];   // C, opa*comp, ex, ey
+  __shared__ float4 sC[kBlock];   // r, g, b, opa
+  __shared__ float2 sD[kBlock];   // comp, cnorm
   __shared__ uint32_t sId[kBlock];
   __shared__ uint32_t sEnd;
   const int tid = threadIdx.x;
   const int tile = tile_of_block(cam);
-  const int tx = tile % cam.tilesX, ty = tile / cam.tilesX;
-  const int x = tx * kTile + (tid & 15), y = ty * kTile + (tid >> 4);
+  const PixelMap pm = pixel_of_thread(cam, tile, tid);
+  const int x = pm.x, y = pm.y;
   const bool inside = x < cam.W && y < cam.H;
   const float px = (float)x + 0.5f, py = (float)y + 0.5f;
   const uint2 rg = b.ranges[tile];
@@ -215,20 +245,23 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
     if (tid < n) {
       const uint32_t id = b.ids[lo + tid];
       sId[tid] = id;
-      sA[tid] = b.rec[3 * id + 0];
-      sB[tid] = b.rec[3 * id + 1];
-      sC[tid] = b.rec[3 * id + 2];
+      sA[tid] = b.rec[4 * id + 0];
+      sB[tid] = b.rec[4 * id + 1];
+      sC[tid] = b.rec[4 * id + 2];
+      const float4 d = b.rec[4 * id + 3];
+      sD[tid] = make_float2(d.x, d.y);
     }
     __syncthreads();
     for (int j = n - 1; j >= 0; j--) {
+      const float4 a = sA[j];
+      const float4 bb = sB[j];
+      if (wave_misses(a.x, a.y, bb.z, bb.w, pm)) continue;
       const uint32_t k = lo + (uint32_t)j;
       bool contrib = false;
       float v[10];
 #pragma unroll
       for (int q = 0; q < 10; q++) v[q] = 0.0f;
       if (active && k < myEnd) {
-        const float4 a = sA[j];
-        const float4 bb = sB[j];
         const float dx = px - a.x, dy = py - a.y;
         const float cA = a.z, cB = a.w, cC = bb.x;
         const float t2 = fmaf(cC * dy, dy, (cA * dx) * dx);
@@ -240,12 +273,14 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
           if (alpha >= cut.aMin) {
             contrib = true;
             const float4 c4 = sC[j];
-            const float col[3] = {bb.z, bb.w, c4.x};
-            const float opa = c4.y, cnorm = c4.w;
-            const float om = 1.0f - alpha;
-            const float Tb = Ta / om;
+            const float2 d2 = sD[j];
+            const float col[3] = {c4.x, c4.y, c4.z};
+            const float opa = c4.w, cnorm = d2.y;
+            // Tb = Ta / (1 - alpha), S / (1 - alpha) through one reciprocal
+            const float rom = 1.0f / (1.0f - alpha);
+            const float Tb = Ta * rom;
             const float aT = alpha * Tb;
-            const float u[3] = {col[0] * Tb - S0 / om, col[1] * Tb - S1 / om, col[2] * Tb - S2 / om};
+            const float u[3] = {col[0] * Tb - S0 * rom, col[1] * Tb - S1 * rom, col[2] * Tb - S2 * rom};
             float galpha = dot3(gC, u);
             if (araw > 0.99f) galpha = 0.0f;
             const float ga = galpha * araw;

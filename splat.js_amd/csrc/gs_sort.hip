@@ -180,36 +180,55 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total,
 }
 
 // per-block key counts over Gaussians in (depth) order `order`
-// (order == nullptr: identity order)
+// (order == nullptr: identity order), and the number of visible ones
 __global__ __launch_bounds__(kBlock) void k_count_partials(const uint32_t* __restrict__ order,
                                                            const uint32_t* __restrict__ cnt,
                                                            const uint32_t* __restrict__ dN, int G,
-                                                           uint64_t* __restrict__ partials) {
+                                                           uint64_t* __restrict__ partials,
+                                                           uint32_t* __restrict__ vpartials) {
   __shared__ uint64_t wsum[4];
+  __shared__ uint32_t wvis[4];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   uint32_t lo, hi;
   chunk_range(*dN, G, lo, hi);
   uint64_t acc = 0;
-  for (uint32_t r = lo + tid; r < hi; r += kBlock) acc += cnt[order ? order[r] : r];
-  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-  if (lane == 0) wsum[w] = acc;
+  uint32_t vis = 0;
+  for (uint32_t r = lo + tid; r < hi; r += kBlock) {
+    const uint32_t c = cnt[order ? order[r] : r];
+    acc += c;
+    vis += c != 0u;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    acc += __shfl_xor(acc, o);
+    vis += __shfl_xor(vis, o);
+  }
+  if (lane == 0) { wsum[w] = acc; wvis[w] = vis; }
   __syncthreads();
-  if (tid == 0) partials[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  if (tid == 0) {
+    partials[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    vpartials[blockIdx.x] = wvis[0] + wvis[1] + wvis[2] + wvis[3];
+  }
 }
 
 // single block: exclusive scan of the G partials; numKeys = min(total, cap)
-__global__ __launch_bounds__(1024) void k_count_top(uint64_t* __restrict__ partials, int G,
+__global__ __launch_bounds__(1024) void k_count_top(uint64_t* __restrict__ partials,
+                                                    const uint32_t* __restrict__ vpartials, int G,
                                                     uint64_t cap, uint32_t* __restrict__ numKeys,
                                                     uint64_t* __restrict__ totalKeys,
+                                                    uint32_t* __restrict__ numVisible,
                                                     double* __restrict__ stats) {
   __shared__ uint64_t wsum[16];
+  __shared__ uint32_t wvis[16];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int per = (G + 1023) / 1024;
   uint64_t mine = 0;
+  uint32_t vis = 0;
   for (int k = 0; k < per; k++) {
     const int b = tid * per + k;
-    if (b < G) mine += partials[b];
+    if (b < G) { mine += partials[b]; vis += vpartials[b]; }
   }
+  for (int o = 32; o > 0; o >>= 1) vis += __shfl_xor(vis, o);
+  if (lane == 0) wvis[w] = vis;
   uint64_t x = mine;
   for (int o = 1; o < 64; o <<= 1) {
     const uint64_t y = __shfl_up(x, o);
@@ -227,6 +246,11 @@ __global__ __launch_bounds__(1024) void k_count_top(uint64_t* __restrict__ parti
       partials[b] = run;
       run += v;
     }
+  }
+  if (tid == 0) {
+    uint32_t v = 0;
+    for (int k = 0; k < 16; k++) v += wvis[k];
+    *numVisible = v;
   }
   if (tid == 1023) {
     const uint64_t total = pre + x;
@@ -278,15 +302,28 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ or
   }
 }
 
-// tile ranges from the sorted keys: ranges[t] = (first, one past last)
+// Tile ranges from the sorted keys: ranges[t] = (first key, one past the
+// last). A tile with no keys gets (o, o), o = number of keys of lower tiles,
+// as the reference's exclusive scan gives it (shaders.js:366-371); the
+// threads at key boundaries fill the empty tiles in between, and the thread
+// holding the last key fills the tiles after it.
 template <typename KeyT>
 __global__ void k_tile_ranges(const KeyT* __restrict__ keys, const uint32_t* __restrict__ dN,
-                              int shift, uint2* __restrict__ ranges) {
+                              int shift, int numTiles, uint2* __restrict__ ranges) {
   const uint32_t N = *dN;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < N; k += gridDim.x * blockDim.x) {
     const uint32_t t = (uint32_t)(keys[k] >> shift);
-    if (k == 0 || (uint32_t)(keys[k - 1] >> shift) != t) ranges[t].x = k;
-    if (k == N - 1 || (uint32_t)(keys[k + 1] >> shift) != t) ranges[t].y = k + 1;
+    const uint32_t tp = k == 0 ? 0xffffffffu : (uint32_t)(keys[k - 1] >> shift);
+    if (tp != t) {
+      ranges[t].x = k;
+      for (uint32_t e = tp + 1u; e < t; e++) ranges[e] = make_uint2(k, k);  // tp + 1 wraps to 0 for k = 0
+    }
+    if (k == N - 1) {
+      ranges[t].y = N;
+      for (uint32_t e = t + 1u; e < (uint32_t)numTiles; e++) ranges[e] = make_uint2(N, N);
+    } else if ((uint32_t)(keys[k + 1] >> shift) != t) {
+      ranges[t].y = k + 1;
+    }
   }
 }
 
@@ -352,8 +389,8 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
     const uint32_t* order;
     radix_sort<uint32_t>(st, s, a.depthKey, nullptr, a.dkA, a.dvA, a.dkB, a.dvB, a.dN, 0, 32, &dk, &order);
     // 2) per-Gaussian key counts in depth order -> offsets -> one key per tile
-    k_count_partials<<<s.G, kBlock, 0, st>>>(order, a.cnt, a.dN, s.G, s.partials);
-    k_count_top<<<1, 1024, 0, st>>>(s.partials, s.G, a.cap, a.numKeys, a.totalKeys, a.stats);
+    k_count_partials<<<s.G, kBlock, 0, st>>>(order, a.cnt, a.dN, s.G, s.partials, s.vpartials);
+    k_count_top<<<1, 1024, 0, st>>>(s.partials, s.vpartials, s.G, a.cap, a.numKeys, a.totalKeys, a.numVisible, a.stats);
     k_emit<false><<<s.G, kBlock, 0, st>>>(order, a.cnt, a.rect, a.depthKey, a.dN, s.G, s.partials,
                                           a.tilesX, a.cap, a.tkA, a.tvA);
     // 3) stable sort by tile id
@@ -365,10 +402,10 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
     out.ids = tv;
     out.keyBytes = 4;
     launch_fill_u32(st, (uint32_t*)a.ranges, 2 * (int64_t)a.numTiles, 0u);
-    k_tile_ranges<uint32_t><<<1024, 256, 0, st>>>(tk, a.numKeys, 0, a.ranges);
+    k_tile_ranges<uint32_t><<<1024, 256, 0, st>>>(tk, a.numKeys, 0, a.numTiles, a.ranges);
   } else {
-    k_count_partials<<<s.G, kBlock, 0, st>>>(nullptr, a.cnt, a.dN, s.G, s.partials);
-    k_count_top<<<1, 1024, 0, st>>>(s.partials, s.G, a.cap, a.numKeys, a.totalKeys, a.stats);
+    k_count_partials<<<s.G, kBlock, 0, st>>>(nullptr, a.cnt, a.dN, s.G, s.partials, s.vpartials);
+    k_count_top<<<1, 1024, 0, st>>>(s.partials, s.vpartials, s.G, a.cap, a.numKeys, a.totalKeys, a.numVisible, a.stats);
     k_emit<true><<<s.G, kBlock, 0, st>>>(nullptr, a.cnt, a.rect, a.depthKey, a.dN, s.G, s.partials,
                                          a.tilesX, a.cap, a.tkA, a.tvA);
     const uint64_t* tk;
@@ -379,7 +416,7 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
     out.ids = tv;
     out.keyBytes = 8;
     launch_fill_u32(st, (uint32_t*)a.ranges, 2 * (int64_t)a.numTiles, 0u);
-    k_tile_ranges<uint64_t><<<1024, 256, 0, st>>>(tk, a.numKeys, 32, a.ranges);
+    k_tile_ranges<uint64_t><<<1024, 256, 0, st>>>(tk, a.numKeys, 32, a.numTiles, a.ranges);
   }
 }
 

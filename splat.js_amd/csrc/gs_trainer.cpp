@@ -98,6 +98,7 @@ struct gs_trainer {
   float* Tfin = nullptr;
   uint32_t* endIdx = nullptr;
   float4* gC = nullptr;
+  float* ssimScratch = nullptr;  // 9 x maxPix SSIM partials (L1 + D-SSIM loss only)
   uint32_t* targets = nullptr;
   int64_t totalPx = 0;
   double* stats = nullptr;      // training stats [4]
@@ -273,7 +274,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
   {
     Scope s(h, "project", (double)n * (56.0 + 4.0 * h->shC + 48.0 + 16.0));
     launch_project(h->st, h->shDeg, n, h->cap, cam, make_cut(h), h->P, h->SH, h->rec, h->depthKey,
-                   h->rect, h->cnt, h->numVisible);
+                   h->rect, h->cnt);
   }
   {
     BinArgs a{};
@@ -288,6 +289,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
     a.dkA = h->dkA; a.dvA = h->dvA; a.dkB = h->dkB; a.dvB = h->dvB;
     a.tkA = h->tkA; a.tkB = h->tkB; a.tvA = h->tvA; a.tvB = h->tvB;
     a.numKeys = h->numKeys;
+    a.numVisible = h->numVisible;
     a.totalKeys = h->totalKeys;
     a.ranges = h->ranges;
     a.stats = statsBuf;
@@ -328,8 +330,6 @@ int pick_camera(gs_trainer* h) {
 
 int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
   if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
-  if (h->o.loss != GS_LOSS_CHARBONNIER)
-    return fail(h, GS_E_ARG, "only the Charbonnier loss is available in this build");
   if (ci < 0) ci = pick_camera(h);
   if (ci >= (int)h->cams.size()) return fail(h, GS_E_ARG, "camera index out of range");
   CamMeta& meta = h->cams[ci];
@@ -341,14 +341,25 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
   h->pixelsSeen += (int64_t)meta.c.w * meta.c.h;
   const AdamHP hp = make_hp(h);
 
-  int rc = run_raster(h, cam, 1, h->stats);
+  const bool ssim = h->o.loss == GS_LOSS_L1_DSSIM;
+  int rc = run_raster(h, cam, ssim ? 2 : 1, h->stats);
   if (rc) return rc;
+  if (ssim) {
+    RasterBufs lb{};
+    lb.img = h->img;
+    lb.gC = h->gC;
+    lb.tgt = h->targets;
+    lb.stats = h->stats;
+    Scope sc(h, "loss_l1ssim", (double)cam.W * cam.H * 116.0);
+    launch_loss_l1ssim(h->st, cam, h->o.ssimLambda, lb, h->ssimScratch);
+  }
+  HIPCK(hipGetLastError());
   const int64_t npx = (int64_t)cam.W * cam.H;
   if (dbg) {
     HIPCK(hipStreamSynchronize(h->st));
     uint32_t nk = 0;
     HIPCK(hipMemcpy(&nk, h->numKeys, 4, hipMemcpyDeviceToHost));
-    if (dbg->rec) HIPCK(hipMemcpy(dbg->rec, h->rec, 48 * h->n, hipMemcpyDeviceToHost));
+    if (dbg->rec) HIPCK(hipMemcpy(dbg->rec, h->rec, 64 * h->n, hipMemcpyDeviceToHost));
     std::vector<uint32_t> dk(h->n);
     HIPCK(hipMemcpy(dk.data(), h->depthKey, 4 * h->n, hipMemcpyDeviceToHost));
     if (dbg->depthKey) memcpy(dbg->depthKey, dk.data(), 4 * h->n);
@@ -493,7 +504,7 @@ int gs_create(const gs_opts* o, gs_trainer_t** out) {
     h->o = *o;
   }
   if (h->o.shDeg < 0 || h->o.shDeg > 3 || h->o.sortMode < 0 || h->o.sortMode > 1 ||
-      h->o.gradAccum < 0 || h->o.gradAccum > 1) {
+      h->o.gradAccum < 0 || h->o.gradAccum > 1 || h->o.loss < 0 || h->o.loss > 1) {
     delete h;
     return GS_E_ARG;
   }
@@ -599,7 +610,7 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   rc |= dalloc(h, &h->SH, sb);
   rc |= dalloc(h, &h->SHM, sb);
   rc |= dalloc(h, &h->SHV, sb);
-  rc |= dalloc(h, &h->rec, (size_t)cap * 48);
+  rc |= dalloc(h, &h->rec, (size_t)cap * 64);
   rc |= dalloc(h, &h->depthKey, (size_t)cap * 4);
   rc |= dalloc(h, &h->cnt, (size_t)cap * 4);
   rc |= dalloc(h, &h->rect, (size_t)cap * 8);
@@ -617,6 +628,7 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   rc |= dalloc(h, &h->sort.hist, (size_t)256 * kSortBlocks * 4);
   rc |= dalloc(h, &h->sort.totals, 256 * 4);
   rc |= dalloc(h, &h->sort.partials, (size_t)kSortBlocks * 8);
+  rc |= dalloc(h, &h->sort.vpartials, (size_t)kSortBlocks * 4);
   rc |= dalloc(h, &h->dN, 4);
   rc |= dalloc(h, &h->numKeys, 4);
   rc |= dalloc(h, &h->numVisible, 4);
@@ -626,6 +638,7 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   rc |= dalloc(h, &h->Tfin, (size_t)h->maxPix * 4);
   rc |= dalloc(h, &h->endIdx, (size_t)h->maxPix * 4);
   rc |= dalloc(h, &h->gC, (size_t)h->maxPix * 16);
+  if (h->o.loss == GS_LOSS_L1_DSSIM) rc |= dalloc(h, &h->ssimScratch, (size_t)h->maxPix * 36);
   rc |= dalloc(h, &h->targets, tgt.size() * 4);
   rc |= dalloc(h, &h->stats, 32);
   rc |= dalloc(h, &h->evalStats, 32);

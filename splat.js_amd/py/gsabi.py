@@ -261,7 +261,7 @@ class GSTrainer:
         npx = m["w"] * m["h"]
         nt = ((m["w"] + 15) // 16) * ((m["h"] + 15) // 16)
         cap = info.entriesCap
-        shapes = {"rec": ((n, 12), np.float32), "depthKey": ((n,), np.uint32), "rect": ((n, 4), np.uint32),
+        shapes = {"rec": ((n, 16), np.float32), "depthKey": ((n,), np.uint32), "rect": ((n, 4), np.uint32),
                   "cnt": ((n,), np.uint32), "ids": ((cap,), np.uint32), "keys": ((cap,), np.uint64),
                   "tileRanges": ((nt, 2), np.uint32), "img": ((npx, 4), np.float32), "T": ((npx,), np.float32),
                   "end": ((npx,), np.uint32), "gC": ((npx, 4), np.float32), "accum": ((n, 10), np.int32),
