@@ -37,24 +37,49 @@ __device__ __forceinline__ int tile_of_block(const Cam& cam) { return (int)block
 // rows 8*(w>>1)); lane l is pixel (l & 7, l >> 3) of that block.
 struct PixelMap {
   int x, y;        // pixel
-  float bx0, by0;  // centre of the block's first pixel
 };
 __device__ __forceinline__ PixelMap pixel_of_thread(const Cam& cam, int tile, int tid) {
   const int tx = tile % cam.tilesX, ty = tile / cam.tilesX;
   const int w = tid >> 6, l = tid & 63;
-  const int bx = tx * kTile + (w & 1) * 8, by = ty * kTile + (w >> 1) * 8;
   PixelMap m;
-  m.x = bx + (l & 7);
-  m.y = by + (l >> 3);
-  m.bx0 = (float)bx + 0.5f;
-  m.by0 = (float)by + 0.5f;
+  m.x = tx * kTile + (w & 1) * 8 + (l & 7);
+  m.y = ty * kTile + (w >> 1) * 8 + (l >> 3);
   return m;
 }
 
-// true when the splat's alpha >= A_MIN ellipse (bounding box mx +- ex,
-// my +- ey) cannot reach any pixel centre of the wave's 8x8 block
-__device__ __forceinline__ bool wave_misses(float mx, float my, float ex, float ey, const PixelMap& m) {
-  return mx + ex < m.bx0 || mx - ex > m.bx0 + 7.0f || my + ey < m.by0 || my - ey > m.by0 + 7.0f;
+// Bit w set when the splat's alpha >= A_MIN ellipse (bounding box mx +- ex,
+// my +- ey) can reach a pixel centre of wave w's 8x8 block. A wave only
+// visits the splats of its bits, so the pixels of a cleared block are the
+// ones that could not have taken the splat.
+__device__ __forceinline__ uint32_t wave_cover(const Cam& cam, int tile, float4 a, float4 bb) {
+  const int tx = tile % cam.tilesX, ty = tile / cam.tilesX;
+  const float x0 = (float)(tx * kTile) + 0.5f, y0 = (float)(ty * kTile) + 0.5f;
+  const float mx = a.x, my = a.y, ex = bb.z, ey = bb.w;
+  uint32_t m = 0;
+#pragma unroll
+  for (int w = 0; w < 4; w++) {
+    const float bx = x0 + (float)((w & 1) * 8), by = y0 + (float)((w >> 1) * 8);
+    const bool miss = mx + ex < bx || mx - ex > bx + 7.0f || my + ey < by || my - ey > by + 7.0f;
+    m |= miss ? 0u : (1u << w);
+  }
+  return m;
+}
+
+// Stores, for the staged batch, wave c's ballot of "entry touches block w"
+// in sMask[c][w]; entries past the batch end contribute no bits.
+__device__ __forceinline__ void build_masks(uint32_t cover, uint64_t (*sMask)[4]) {
+  const int tid = threadIdx.x, c = tid >> 6;
+#pragma unroll
+  for (int w = 0; w < 4; w++) {
+    const uint64_t bal = __ballot((cover >> w) & 1u);
+    if ((tid & 63) == 0) sMask[c][w] = bal;
+  }
+}
+
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
 }
 
 template <int kLoss>
@@ -62,8 +87,9 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
   __shared__ float4 sA[kBlock];   // mx, my, A, B
   __shared__ float4 sB[kBlock];   // C, opa*comp, ex, ey
   __shared__ float4 sC[kBlock];   // r, g, b, opa
+  __shared__ uint64_t sMask[4][4];
   __shared__ double sStat[3][kBlock / 64];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wv = tid >> 6;
   const int tile = tile_of_block(cam);
   const PixelMap pm = pixel_of_thread(cam, tile, tid);
   const int x = pm.x, y = pm.y;
@@ -77,36 +103,51 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
   for (uint32_t base = s; base < e; base += kBlock) {
     if (__syncthreads_count(!done) == 0) break;
     const uint32_t k = base + tid;
+    uint32_t cover = 0;
     if (k < e) {
       const uint32_t id = b.ids[k];
-      sA[tid] = b.rec[4 * id + 0];
-      sB[tid] = b.rec[4 * id + 1];
+      const float4 a = b.rec[4 * id + 0];
+      const float4 bb = b.rec[4 * id + 1];
+      sA[tid] = a;
+      sB[tid] = bb;
       sC[tid] = b.rec[4 * id + 2];
+      cover = wave_cover(cam, tile, a, bb);
     }
+    build_masks(cover, sMask);
     __syncthreads();
-    if (!done) {
-      const int cnt = (int)min((uint32_t)kBlock, e - base);
-      int j = 0;
-      for (; j < cnt; j++) {
-        const float4 a = sA[j];
-        const float4 bb = sB[j];
-        if (wave_misses(a.x, a.y, bb.z, bb.w, pm)) continue;
-        const float dx = px - a.x, dy = py - a.y;
-        const float t2 = fmaf(bb.x * dy, dy, (a.z * dx) * dx);
-        const float ee = fmaxf(fmaf(0.5f, t2, (a.w * dx) * dy), 0.0f);
-        if (ee > cut.eCut) continue;
-        const float araw = bb.y * gs_expf(-ee);
-        const float alpha = fminf(0.99f, araw);
-        if (alpha < cut.aMin) continue;
-        const float4 c4 = sC[j];
-        const float w = T * alpha;
-        C0 = fmaf(w, c4.x, C0);
-        C1 = fmaf(w, c4.y, C1);
-        C2 = fmaf(w, c4.z, C2);
-        T = T * (1.0f - alpha);
-        if (T < 1e-4f) { done = true; break; }
+    const uint32_t cnt = min((uint32_t)kBlock, e - base);
+    if (__ballot(!done) != 0ull) {
+      uint32_t term = 0xffffffffu;  // batch index of this pixel's last splat
+      for (int c = 0; c < 4; c++) {
+        uint64_t m = uniform64(sMask[c][wv]);
+        while (m) {
+          const int j = c * 64 + (int)__builtin_ctzll(m);
+          m &= m - 1ull;
+          if (!done) {
+            const float4 a = sA[j];
+            const float4 bb = sB[j];
+            const float dx = px - a.x, dy = py - a.y;
+            const float t2 = fmaf(bb.x * dy, dy, (a.z * dx) * dx);
+            const float ee = fmaxf(fmaf(0.5f, t2, (a.w * dx) * dy), 0.0f);
+            if (ee <= cut.eCut) {
+              const float araw = bb.y * gs_expf(-ee);
+              const float alpha = fminf(0.99f, araw);
+              if (alpha >= cut.aMin) {
+                const float4 c4 = sC[j];
+                const float w = T * alpha;
+                C0 = fmaf(w, c4.x, C0);
+                C1 = fmaf(w, c4.y, C1);
+                C2 = fmaf(w, c4.z, C2);
+                T = T * (1.0f - alpha);
+                if (T < 1e-4f) { done = true; term = (uint32_t)j; }
+              }
+            }
+          }
+          if (__ballot(!done) == 0ull) { m = 0ull; c = 4; }
+        }
       }
-      end = done ? base + (uint32_t)j + 1u : base + (uint32_t)cnt;
+      if (term != 0xffffffffu) end = base + term + 1u;
+      else if (!done) end = base + cnt;
     }
   }
   double st0 = 0.0, st1 = 0.0, st2 = 0.0;
@@ -144,8 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
       st1 += __shfl_xor(st1, o);
       st2 += __shfl_xor(st2, o);
     }
-    const int w = tid >> 6;
-    if ((tid & 63) == 0) { sStat[0][w] = st0; sStat[1][w] = st1; sStat[2][w] = st2; }
+    if ((tid & 63) == 0) { sStat[0][wv] = st0; sStat[1][wv] = st1; sStat[2][wv] = st2; }
     __syncthreads();
     if (tid < 3) {
       const double v = sStat[tid][0] + sStat[tid][1] + sStat[tid][2] + sStat[tid][3];
@@ -154,53 +194,81 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
   }
 }
 
-// ---- wave transpose reduction of 10 values ----
-// Step masks 32, 16, 8, 4 halve the value count (10 -> 5 -> 3 -> 2 -> 1) by
-// swapping halves with the partner lane; masks 2 and 1 finish the sum. Lane
-// l ends with the 64-lane total of slot `sid` (or sid < 0: padding).
-template <typename T>
-__device__ __forceinline__ T shfl_x(T v, int m);
-template <>
-__device__ __forceinline__ uint32_t shfl_x<uint32_t>(uint32_t v, int m) {
-  return (uint32_t)__shfl_xor((int)v, m);
-}
-template <>
-__device__ __forceinline__ float shfl_x<float>(float v, int m) {
-  return __shfl_xor(v, m);
+// ---- wave transpose reduction of 10 values (VALU lane exchanges) ----
+// Halving steps: 10 -> 5 (lanes l, l^32: v_permlane32_swap), 5 -> 3
+// (l, l^16: v_permlane16_swap), 3 -> 2 (l, l^15: DPP row_mirror), 2 -> 1
+// (l, l^7: DPP row_half_mirror); then the quad sums (DPP quad_perm xor 2,
+// xor 1). In each halving step a lane keeps one half of its values and adds
+// its partner's copy of that half, so after the last step each lane holds
+// the 64-lane total of one slot (or of padding). The swap builtins return
+// both operands after exchanging one half of the lanes, so lo' + hi' is that
+// sum whichever half a lane ends up with; the slot each lane ends with is
+// measured once per kernel by running the network on the constants 1..10.
+template <int kCtrl>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, 0xf, 0xf, false);
 }
 
-template <typename T, int N>
-__device__ __forceinline__ void halve(const T* v, const int* sid, T* out, int* osid, int mask) {
+template <typename T>
+__device__ __forceinline__ uint32_t as_u(T v) { return __builtin_bit_cast(uint32_t, v); }
+template <typename T>
+__device__ __forceinline__ T from_u(uint32_t v) { return __builtin_bit_cast(T, v); }
+
+template <typename T, int N, bool k32>
+__device__ __forceinline__ void swap_halve(const T* v, T* out) {
   constexpr int H = (N + 1) / 2;
-  const bool hi = (__lane_id() & mask) != 0;
 #pragma unroll
   for (int i = 0; i < H; i++) {
-    const bool hasHi = i + H < N;
-    const T lo_v = v[i];
-    const T hi_v = hasHi ? v[i + H] : (T)0;
-    const T keep = hi ? hi_v : lo_v;
-    const T send = hi ? lo_v : hi_v;
-    out[i] = keep + shfl_x<T>(send, mask);
-    osid[i] = hi ? (hasHi ? sid[i + H] : -1) : sid[i];
+    const uint32_t lo = as_u(v[i]);
+    const uint32_t hi = i + H < N ? as_u(v[i + H]) : as_u((T)0);
+    uint32_t a, bq;
+    if (k32) {
+      const auto r = __builtin_amdgcn_permlane32_swap(lo, hi, false, false);
+      a = r[0];
+      bq = r[1];
+    } else {
+      const auto r = __builtin_amdgcn_permlane16_swap(lo, hi, false, false);
+      a = r[0];
+      bq = r[1];
+    }
+    out[i] = from_u<T>(a) + from_u<T>(bq);
+  }
+}
+
+template <typename T, int N, int kCtrl>
+__device__ __forceinline__ void dpp_halve(const T* v, T* out, bool upper) {
+  constexpr int H = (N + 1) / 2;
+#pragma unroll
+  for (int i = 0; i < H; i++) {
+    const T lo = v[i];
+    const T hi = i + H < N ? v[i + H] : (T)0;
+    const T keep = upper ? hi : lo;
+    const T send = upper ? lo : hi;
+    out[i] = keep + from_u<T>(dpp<kCtrl>(as_u(send)));
   }
 }
 
 template <typename T>
-__device__ __forceinline__ T wave_reduce10(const T* v, int& slot) {
-  int s0[10];
-#pragma unroll
-  for (int i = 0; i < 10; i++) s0[i] = i;
+__device__ __forceinline__ T wave_reduce10(const T* v) {
+  const uint32_t lane = __lane_id();
   T a[5], b3[3], c[2], d[1];
-  int sa[5], sb[3], sc[2], sd[1];
-  halve<T, 10>(v, s0, a, sa, 32);
-  halve<T, 5>(a, sa, b3, sb, 16);
-  halve<T, 3>(b3, sb, c, sc, 8);
-  halve<T, 2>(c, sc, d, sd, 4);
+  swap_halve<T, 10, true>(v, a);
+  swap_halve<T, 5, false>(a, b3);
+  dpp_halve<T, 3, 0x140>(b3, c, (lane & 8) != 0);   // row_mirror: l ^ 15
+  dpp_halve<T, 2, 0x141>(c, d, (lane & 4) != 0);    // row_half_mirror: l ^ 7
   T r = d[0];
-  r = r + shfl_x<T>(r, 2);
-  r = r + shfl_x<T>(r, 1);
-  slot = sd[0];
+  r = r + from_u<T>(dpp<0x4E>(as_u(r)));            // quad_perm [2,3,0,1]: l ^ 2
+  r = r + from_u<T>(dpp<0xB1>(as_u(r)));            // quad_perm [1,0,3,2]: l ^ 1
   return r;
+}
+
+// slot whose 64-lane total this lane holds after wave_reduce10 (-1: none)
+__device__ __forceinline__ int reduce10_slot() {
+  uint32_t v[10];
+#pragma unroll
+  for (int i = 0; i < 10; i++) v[i] = (uint32_t)(i + 1);
+  const int s = (int)(wave_reduce10<uint32_t>(v) / 64u) - 1;
+  return s >= 0 && s < 10 ? s : -1;
 }
 
 template <int kAccum>
@@ -210,8 +278,9 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
   __shared__ float4 sC[kBlock];   // r, g, b, opa
   __shared__ float2 sD[kBlock];   // comp, cnorm
   __shared__ uint32_t sId[kBlock];
+  __shared__ uint64_t sMask[4][4];
   __shared__ uint32_t sEnd;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wv = tid >> 6;
   const int tile = tile_of_block(cam);
   const PixelMap pm = pixel_of_thread(cam, tile, tid);
   const int x = pm.x, y = pm.y;
@@ -220,6 +289,8 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
   const uint2 rg = b.ranges[tile];
   const uint32_t s = rg.x;
   const int64_t pi = (int64_t)y * cam.W + x;
+  const int mySlot = reduce10_slot();
+  const bool flusher = (__lane_id() & 3) == 0 && mySlot >= 0;
   float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float Ta = 0.0f;
   uint32_t myEnd = s;
@@ -242,82 +313,89 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
     const uint32_t lo = hi - s > (uint32_t)kBlock ? hi - kBlock : s;
     const int n = (int)(hi - lo);
     __syncthreads();
+    uint32_t cover = 0;
     if (tid < n) {
       const uint32_t id = b.ids[lo + tid];
       sId[tid] = id;
-      sA[tid] = b.rec[4 * id + 0];
-      sB[tid] = b.rec[4 * id + 1];
+      const float4 a = b.rec[4 * id + 0];
+      const float4 bb = b.rec[4 * id + 1];
+      sA[tid] = a;
+      sB[tid] = bb;
       sC[tid] = b.rec[4 * id + 2];
       const float4 d = b.rec[4 * id + 3];
       sD[tid] = make_float2(d.x, d.y);
+      cover = wave_cover(cam, tile, a, bb);
     }
+    build_masks(cover, sMask);
     __syncthreads();
-    for (int j = n - 1; j >= 0; j--) {
-      const float4 a = sA[j];
-      const float4 bb = sB[j];
-      if (wave_misses(a.x, a.y, bb.z, bb.w, pm)) continue;
-      const uint32_t k = lo + (uint32_t)j;
-      bool contrib = false;
-      float v[10];
+    for (int c = 3; c >= 0; c--) {
+      uint64_t m = uniform64(sMask[c][wv]);
+      while (m) {
+        const int bit = 63 - (int)__builtin_clzll(m);
+        m &= ~(1ull << bit);
+        const int j = c * 64 + bit;
+        const uint32_t k = lo + (uint32_t)j;
+        bool contrib = false;
+        float v[10];
 #pragma unroll
-      for (int q = 0; q < 10; q++) v[q] = 0.0f;
-      if (active && k < myEnd) {
-        const float dx = px - a.x, dy = py - a.y;
-        const float cA = a.z, cB = a.w, cC = bb.x;
-        const float t2 = fmaf(cC * dy, dy, (cA * dx) * dx);
-        const float e = fmaxf(fmaf(0.5f, t2, (cB * dx) * dy), 0.0f);
-        if (e <= cut.eCut) {
-          const float G = gs_expf(-e);
-          const float araw = bb.y * G;
-          const float alpha = fminf(0.99f, araw);
-          if (alpha >= cut.aMin) {
-            contrib = true;
-            const float4 c4 = sC[j];
-            const float2 d2 = sD[j];
-            const float col[3] = {c4.x, c4.y, c4.z};
-            const float opa = c4.w, cnorm = d2.y;
-            // Tb = Ta / (1 - alpha), S / (1 - alpha) through one reciprocal
-            const float rom = 1.0f / (1.0f - alpha);
-            const float Tb = Ta * rom;
-            const float aT = alpha * Tb;
-            const float u[3] = {col[0] * Tb - S0 * rom, col[1] * Tb - S1 * rom, col[2] * Tb - S2 * rom};
-            float galpha = dot3(gC, u);
-            if (araw > 0.99f) galpha = 0.0f;
-            const float ga = galpha * araw;
-            v[0] = ga * (cA * dx + cB * dy);
-            v[1] = ga * (cB * dx + cC * dy);
-            v[2] = (-ga) * 0.5f * dx * dx * cnorm;
-            v[3] = (-ga) * dx * dy * cnorm;
-            v[4] = (-ga) * 0.5f * dy * dy * cnorm;
-            v[5] = galpha * opa * G;
-            v[6] = ga * (1.0f - opa);
-            v[7] = gC[0] * aT;
-            v[8] = gC[1] * aT;
-            v[9] = gC[2] * aT;
-            S0 = S0 + col[0] * alpha * Tb;
-            S1 = S1 + col[1] * alpha * Tb;
-            S2 = S2 + col[2] * alpha * Tb;
-            Ta = Tb;
+        for (int q = 0; q < 10; q++) v[q] = 0.0f;
+        if (active && k < myEnd) {
+          const float4 a = sA[j];
+          const float4 bb = sB[j];
+          const float dx = px - a.x, dy = py - a.y;
+          const float cA = a.z, cB = a.w, cC = bb.x;
+          const float t2 = fmaf(cC * dy, dy, (cA * dx) * dx);
+          const float e = fmaxf(fmaf(0.5f, t2, (cB * dx) * dy), 0.0f);
+          if (e <= cut.eCut) {
+            const float G = gs_expf(-e);
+            const float araw = bb.y * G;
+            const float alpha = fminf(0.99f, araw);
+            if (alpha >= cut.aMin) {
+              contrib = true;
+              const float4 c4 = sC[j];
+              const float2 d2 = sD[j];
+              const float col[3] = {c4.x, c4.y, c4.z};
+              const float opa = c4.w, cnorm = d2.y;
+              // Tb = Ta / (1 - alpha), S / (1 - alpha) through one reciprocal
+              const float rom = 1.0f / (1.0f - alpha);
+              const float Tb = Ta * rom;
+              const float aT = alpha * Tb;
+              const float u[3] = {col[0] * Tb - S0 * rom, col[1] * Tb - S1 * rom, col[2] * Tb - S2 * rom};
+              float galpha = dot3(gC, u);
+              if (araw > 0.99f) galpha = 0.0f;
+              const float ga = galpha * araw;
+              v[0] = ga * (cA * dx + cB * dy);
+              v[1] = ga * (cB * dx + cC * dy);
+              v[2] = (-ga) * 0.5f * dx * dx * cnorm;
+              v[3] = (-ga) * dx * dy * cnorm;
+              v[4] = (-ga) * 0.5f * dy * dy * cnorm;
+              v[5] = galpha * opa * G;
+              v[6] = ga * (1.0f - opa);
+              v[7] = gC[0] * aT;
+              v[8] = gC[1] * aT;
+              v[9] = gC[2] * aT;
+              S0 = S0 + col[0] * alpha * Tb;
+              S1 = S1 + col[1] * alpha * Tb;
+              S2 = S2 + col[2] * alpha * Tb;
+              Ta = Tb;
+            }
           }
         }
-      }
-      if (__ballot(contrib) == 0ull) continue;
-      const uint32_t gid = sId[j];
-      int slot;
-      if (kAccum == 0) {
-        uint32_t iv[10];
+        if (__ballot(contrib) == 0ull) continue;
+        const uint32_t gid = sId[j];
+        if (kAccum == 0) {
+          uint32_t iv[10];
 #pragma unroll
-        for (int q = 0; q < 10; q++) {
-          const float cl = fminf(fmaxf(v[q] * kFixed, -1.0e9f), 1.0e9f);
-          iv[q] = contrib ? (uint32_t)(int32_t)cl : 0u;
+          for (int q = 0; q < 10; q++) {
+            const float cl = fminf(fmaxf(v[q] * kFixed, -1.0e9f), 1.0e9f);
+            iv[q] = contrib ? (uint32_t)(int32_t)cl : 0u;
+          }
+          const uint32_t r = wave_reduce10<uint32_t>(iv);
+          if (flusher && r != 0u) atomicAdd((uint32_t*)accum + (int64_t)gid * 12 + mySlot, r);
+        } else {
+          const float r = wave_reduce10<float>(v);
+          if (flusher && r != 0.0f) unsafeAtomicAdd((float*)accum + (int64_t)gid * 12 + mySlot, r);
         }
-        const uint32_t r = wave_reduce10<uint32_t>(iv, slot);
-        if ((__lane_id() & 3) == 0 && slot >= 0 && r != 0u)
-          atomicAdd((uint32_t*)accum + (int64_t)gid * 12 + slot, r);
-      } else {
-        const float r = wave_reduce10<float>(v, slot);
-        if ((__lane_id() & 3) == 0 && slot >= 0 && r != 0.0f)
-          unsafeAtomicAdd((float*)accum + (int64_t)gid * 12 + slot, r);
       }
     }
     hi = lo;
