@@ -11,13 +11,19 @@
 //  * global64: keys (tile << 32 | depth bits) emitted in id order and sorted
 //    by a stable LSD radix sort over all 32 + tileBits bits.
 //
-// Every radix pass is reduce-then-scan with no inter-block waiting:
-//   hist    per-block digit counts; wave-level ballot matching gives each
-//           wave's digit groups, so each LDS counter takes one add per group
-//   scan    one block per digit scans that digit's per-block counts and adds
-//           the digit's base (an exclusive scan of the 256 digit totals)
-//   scatter per 256-key round: ballot matching ranks keys within their wave,
-//           per-wave counts give each wave's offset, so the pass is stable
+// Every radix pass (8-bit digit) is reduce-then-scan, with no inter-block
+// waiting, over a fixed grid of G blocks, each owning a contiguous chunk
+// processed in rounds of 4096 keys (16 per thread, wave-striped so every
+// load is coalesced):
+//   hist    per-wave digit counts in LDS; each wave groups its 64 keys by
+//           digit with 8 ballots, and one lane per group adds the group size
+//   scan    digit base (exclusive scan of the digit totals) plus the
+//           exclusive scan of each digit's per-block counts (table is
+//           block-major, so every access is a contiguous row)
+//   scatter wave-local stable ranks from the same ballot groups, one
+//           block-wide digit prefix per round, keys staged in LDS in
+//           (digit, position) order, then written out in digit runs by
+//           consecutive threads
 // Key counts that only exist on the device (the emitted key total) are read
 // from device memory by every kernel; the grids are fixed.
 #include "gs_kernels.h"
@@ -42,9 +48,14 @@ __device__ __forceinline__ uint64_t match_digit8(uint32_t d, uint64_t active) {
   return m;
 }
 
-__device__ __forceinline__ void chunk_range(uint32_t N, int G, uint32_t& lo, uint32_t& hi) {
+constexpr int kItems = 16;
+constexpr int kRound = kBlock * kItems;   // keys per block round
+
+// block b's contiguous chunk [lo, hi) of N elements split over G blocks,
+// rounded to `align` elements
+__device__ __forceinline__ void chunk_range(uint32_t N, int G, uint32_t align, uint32_t& lo, uint32_t& hi) {
   const uint32_t per = (uint32_t)(((uint64_t)N + G - 1) / G);
-  const uint32_t chunk = (per + 255u) & ~255u;
+  const uint32_t chunk = (per + align - 1u) / align * align;
   const uint64_t l = (uint64_t)blockIdx.x * chunk;
   lo = (uint32_t)(l < N ? l : N);
   const uint64_t h = (uint64_t)lo + chunk;
@@ -57,107 +68,167 @@ __global__ void k_fill_u32(uint32_t* p, int64_t count, uint32_t v) {
 }
 
 template <typename KeyT>
+__device__ __forceinline__ uint32_t digit_of(KeyT k, int shift) {
+  return (uint32_t)(k >> shift) & 0xffu;
+}
+
+template <typename KeyT>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const KeyT* __restrict__ keys,
                                                        const uint32_t* __restrict__ dN, int shift,
                                                        int G, uint32_t* __restrict__ hist,
                                                        uint32_t* __restrict__ totals) {
-  __shared__ uint32_t cnt[256];
-  const int tid = threadIdx.x;
-  cnt[tid] = 0;
+  __shared__ uint32_t wc[4][256];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  wc[0][tid] = wc[1][tid] = wc[2][tid] = wc[3][tid] = 0;
   __syncthreads();
   uint32_t lo, hi;
-  chunk_range(*dN, G, lo, hi);
-  for (uint32_t r = lo; r < hi; r += kBlock) {
-    const uint32_t i = r + tid;
-    const bool ok = i < hi;
-    const uint32_t d = ok ? (uint32_t)(keys[i] >> shift) & 0xffu : 0u;
-    const uint64_t m = match_digit8(d, __ballot(ok));
-    if (ok && (m & lanemask_lt()) == 0) atomicAdd(&cnt[d], (uint32_t)__popcll(m));
+  chunk_range(*dN, G, kRound, lo, hi);
+  for (uint32_t r = lo; r < hi; r += kRound) {
+    const uint32_t base = r + (uint32_t)w * 64u * kItems;
+    uint32_t d[kItems];
+#pragma unroll
+    for (int i = 0; i < kItems; i++) {
+      const uint32_t idx = base + (uint32_t)i * 64u + lane;
+      d[i] = idx < hi ? digit_of(keys[idx], shift) : 0x100u;
+    }
+#pragma unroll
+    for (int i = 0; i < kItems; i++) {
+      const bool ok = d[i] < 0x100u;
+      const uint64_t act = __ballot(ok);
+      if (act == 0ull) break;
+      const uint64_t m = match_digit8(d[i] & 0xffu, act);
+      if (ok && (m & lanemask_lt()) == 0ull) wc[w][d[i]] += (uint32_t)__popcll(m);
+    }
   }
   __syncthreads();
-  const uint32_t c = cnt[tid];
-  hist[tid * G + blockIdx.x] = c;
+  const uint32_t c = wc[0][tid] + wc[1][tid] + wc[2][tid] + wc[3][tid];
+  hist[(int64_t)blockIdx.x * 256 + tid] = c;
   if (c) atomicAdd(&totals[tid], c);
 }
 
-// one block per digit: exclusive scan of hist[d][0..G) plus the digit's base
+// 8 blocks x 256 threads: block k owns digits [32k, 32k+32); thread t sums
+// digit 32k + (t & 31) over rows group t >> 5 of the G-row table, then the
+// exclusive prefix (digit base + earlier blocks) replaces the counts.
 __global__ __launch_bounds__(kBlock) void k_radix_scan(uint32_t* __restrict__ hist, int G,
                                                        const uint32_t* __restrict__ totals) {
-  __shared__ uint32_t wsum[kBlock / 64];
-  __shared__ uint32_t sbase;
-  const int d = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  // digit base: sum of totals[0..d)
-  {
-    uint32_t v = tid < d ? totals[tid] : 0u;
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if (lane == 0) wsum[w] = v;
-    __syncthreads();
-    if (tid == 0) sbase = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    __syncthreads();
+  __shared__ uint32_t part[8][32];
+  __shared__ uint32_t dbase[32];
+  const int tid = threadIdx.x, dl = tid & 31, grp = tid >> 5;
+  const int d = 32 * blockIdx.x + dl;
+  const int R = (G + 7) / 8;
+  const int b0 = grp * R, b1 = min(G, b0 + R);
+  uint32_t sum = 0;
+  for (int b = b0; b < b1; b++) sum += hist[(int64_t)b * 256 + d];
+  part[grp][dl] = sum;
+  if (tid < 32) {
+    uint32_t acc = 0;
+    for (int e = 0; e < d; e++) acc += totals[e];
+    dbase[dl] = acc;
   }
-  uint32_t carry = sbase;
-  uint32_t* row = hist + (int64_t)d * G;
-  for (int b0 = 0; b0 < G; b0 += kBlock) {
-    const int b = b0 + tid;
-    const uint32_t v = b < G ? row[b] : 0u;
-    uint32_t x = v;  // inclusive wave scan
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t y = __shfl_up(x, o);
-      if (lane >= o) x += y;
-    }
-    __syncthreads();
-    if (lane == 63) wsum[w] = x;
-    __syncthreads();
-    uint32_t wpre = 0;
-    for (int k = 0; k < w; k++) wpre += wsum[k];
-    if (b < G) row[b] = carry + wpre + x - v;
-    carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  __syncthreads();
+  uint32_t pre = dbase[dl];
+  for (int g = 0; g < grp; g++) pre += part[g][dl];
+  for (int b = b0; b < b1; b++) {
+    const int64_t q = (int64_t)b * 256 + d;
+    const uint32_t v = hist[q];
+    hist[q] = pre;
+    pre += v;
   }
 }
 
-template <typename KeyT, bool kIota, bool kWriteKeys>
+template <typename KeyT, bool kWriteKeys>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const KeyT* __restrict__ keysIn,
                                                           const uint32_t* __restrict__ valsIn,
                                                           const uint32_t* __restrict__ dN,
                                                           int shift, int G,
                                                           const uint32_t* __restrict__ scan,
                                                           KeyT* __restrict__ keysOut,
-                                                          uint32_t* __restrict__ valsOut) {
-  __shared__ uint32_t base[256];
-  __shared__ uint32_t wcnt[4][256];
+                                                          uint32_t* __restrict__ valsOut,
+                                                          uint32_t* __restrict__ totalsNext) {
+  __shared__ uint32_t wc[4][256];
   __shared__ uint32_t wpre[4][256];
-  const int tid = threadIdx.x, w = tid >> 6;
+  __shared__ uint32_t gbase[256];
+  __shared__ uint32_t lstart[256];
+  __shared__ uint32_t wsum[4];
+  __shared__ KeyT sk[kRound];
+  __shared__ uint32_t sv[kRound];
+  const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+  // the next pass's histogram accumulates into totalsNext: clear it here
+  if (blockIdx.x == 0 && totalsNext) totalsNext[tid] = 0u;
   uint32_t lo, hi;
-  chunk_range(*dN, G, lo, hi);
-  base[tid] = scan[tid * G + blockIdx.x];
-  wcnt[0][tid] = wcnt[1][tid] = wcnt[2][tid] = wcnt[3][tid] = 0;
+  chunk_range(*dN, G, kRound, lo, hi);
+  gbase[tid] = scan[(int64_t)blockIdx.x * 256 + tid];
+  wc[0][tid] = wc[1][tid] = wc[2][tid] = wc[3][tid] = 0;
   __syncthreads();
-  for (uint32_t r = lo; r < hi; r += kBlock) {
-    const uint32_t i = r + tid;
-    const bool ok = i < hi;
-    const KeyT key = ok ? keysIn[i] : (KeyT)0;
-    const uint32_t val = ok ? (kIota ? i : valsIn[i]) : 0u;
-    const uint32_t d = (uint32_t)(key >> shift) & 0xffu;
-    const uint64_t m = match_digit8(d, __ballot(ok));
-    const uint32_t rank = (uint32_t)__popcll(m & lanemask_lt());
-    if (ok && rank == 0) wcnt[w][d] = (uint32_t)__popcll(m);
-    __syncthreads();
-    {
-      const uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
-      const uint32_t b = base[tid];
-      wpre[0][tid] = b;
-      wpre[1][tid] = b + c0;
-      wpre[2][tid] = b + c0 + c1;
-      wpre[3][tid] = b + c0 + c1 + c2;
-      base[tid] = b + c0 + c1 + c2 + c3;
-      wcnt[0][tid] = wcnt[1][tid] = wcnt[2][tid] = wcnt[3][tid] = 0;
+  for (uint32_t r = lo; r < hi; r += kRound) {
+    // 1. load, then rank within the wave (stable: item-major, then lane)
+    const uint32_t base = r + (uint32_t)w * 64u * kItems;
+    KeyT key[kItems];
+    uint32_t val[kItems], rank[kItems];
+#pragma unroll
+    for (int i = 0; i < kItems; i++) {
+      const uint32_t idx = base + (uint32_t)i * 64u + lane;
+      const bool ok = idx < hi;
+      key[i] = ok ? keysIn[idx] : (KeyT)0;
+      val[i] = ok ? (valsIn ? valsIn[idx] : idx) : 0u;
+      rank[i] = ok ? 0u : 0xffffffffu;
+    }
+#pragma unroll
+    for (int i = 0; i < kItems; i++) {
+      const bool ok = rank[i] == 0u;
+      const uint64_t act = __ballot(ok);
+      if (act == 0ull) break;
+      const uint32_t d = digit_of(key[i], shift);
+      const uint64_t m = match_digit8(d, act);
+      if (ok) {
+        const uint32_t lt = (uint32_t)__popcll(m & lanemask_lt());
+        rank[i] = wc[w][d] + lt;
+        if (lt == 0u) wc[w][d] += (uint32_t)__popcll(m);
+      }
     }
     __syncthreads();
-    if (ok) {
-      const uint32_t pos = wpre[w][d] + rank;
-      if (kWriteKeys) keysOut[pos] = key;
-      valsOut[pos] = val;
+    // 2. per digit: wave prefixes, round total, local start (block scan)
+    const uint32_t c0 = wc[0][tid], c1 = wc[1][tid], c2 = wc[2][tid], c3 = wc[3][tid];
+    const uint32_t tot = c0 + c1 + c2 + c3;
+    wpre[0][tid] = 0u;
+    wpre[1][tid] = c0;
+    wpre[2][tid] = c0 + c1;
+    wpre[3][tid] = c0 + c1 + c2;
+    wc[0][tid] = wc[1][tid] = wc[2][tid] = wc[3][tid] = 0;
+    uint32_t x = tot;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
     }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t wp = 0;
+    for (int k = 0; k < w; k++) wp += wsum[k];
+    const uint32_t myStart = wp + x - tot;
+    lstart[tid] = myStart;
+    __syncthreads();
+    // 3. stage in local (digit, rank) order
+#pragma unroll
+    for (int i = 0; i < kItems; i++) {
+      if (rank[i] != 0xffffffffu) {
+        const uint32_t d = digit_of(key[i], shift);
+        const uint32_t p = lstart[d] + wpre[w][d] + rank[i];
+        sk[p] = key[i];
+        sv[p] = val[i];
+      }
+    }
+    __syncthreads();
+    // 4. write out: consecutive threads write consecutive slots of a digit run
+    const uint32_t cnt = min((uint32_t)kRound, hi - r);
+    for (uint32_t j = tid; j < cnt; j += kBlock) {
+      const KeyT k = sk[j];
+      const uint32_t d = digit_of(k, shift);
+      const uint32_t g = gbase[d] + (j - lstart[d]);
+      if (kWriteKeys) keysOut[g] = k;
+      valsOut[g] = sv[j];
+    }
+    __syncthreads();
+    gbase[tid] += tot;
   }
 }
 
@@ -190,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_count_partials(const uint32_t* __res
   __shared__ uint32_t wvis[4];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   uint32_t lo, hi;
-  chunk_range(*dN, G, lo, hi);
+  chunk_range(*dN, G, kBlock, lo, hi);
   uint64_t acc = 0;
   uint32_t vis = 0;
   for (uint32_t r = lo + tid; r < hi; r += kBlock) {
@@ -260,9 +331,14 @@ __global__ __launch_bounds__(1024) void k_count_top(uint64_t* __restrict__ parti
   }
 }
 
-// Emits one key per covered tile. kGlobal64: Gaussians in id order, key =
-// (tile << 32) | depth bits. Otherwise Gaussians in depth order, key = tile.
-// Keys past the budget `cap` are dropped (counted by k_count_top).
+// Emits one key per covered tile, cooperatively: for every round of 256
+// Gaussians (in sort order) the block scans their key counts, then each
+// thread produces the keys at consecutive output positions, finding its
+// Gaussian by binary search over the round's offsets in LDS. Writes are
+// coalesced and the work per thread is balanced whatever the splat sizes.
+// kGlobal64: Gaussians in id order, key = (tile << 32) | depth bits.
+// Otherwise Gaussians in depth order, key = tile. Keys past the budget `cap`
+// are dropped (counted by k_count_top).
 template <bool kGlobal64>
 __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ order,
                                                  const uint32_t* __restrict__ cnt,
@@ -273,32 +349,46 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ or
                                                  uint64_t cap, void* __restrict__ keysOut,
                                                  uint32_t* __restrict__ valsOut) {
   __shared__ uint32_t wsum[4];
+  __shared__ uint32_t sOff[kBlock];
+  __shared__ uint2 sRect[kBlock];
+  __shared__ uint32_t sId[kBlock];
+  __shared__ uint32_t sDk[kBlock];
+  const int tid = threadIdx.x;
   uint32_t lo, hi;
-  chunk_range(*dN, G, lo, hi);
+  chunk_range(*dN, G, kBlock, lo, hi);
   uint64_t run = partials[blockIdx.x];
   for (uint32_t r0 = lo; r0 < hi; r0 += kBlock) {
-    const uint32_t r = r0 + threadIdx.x;
+    const uint32_t r = r0 + tid;
     const bool ok = r < hi;
     const uint32_t id = ok ? (kGlobal64 ? r : order[r]) : 0u;
     const uint32_t c = ok ? cnt[id] : 0u;
     uint32_t tot;
     const uint32_t ex = block_excl_scan(c, tot, wsum);
-    if (c) {
-      uint64_t pos = run + ex;
-      const uint2 rc = rect[id];
-      const uint32_t tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu, ty1 = rc.y >> 16;
-      const uint32_t dk = kGlobal64 ? depthKey[id] : 0u;
-      for (uint32_t ty = ty0; ty <= ty1; ty++) {
-        for (uint32_t tx = tx0; tx <= tx1; tx++, pos++) {
-          if (pos >= cap) break;
-          const uint32_t tile = ty * (uint32_t)tilesX + tx;
-          if (kGlobal64) ((uint64_t*)keysOut)[pos] = ((uint64_t)tile << 32) | dk;
-          else ((uint32_t*)keysOut)[pos] = tile;
-          valsOut[pos] = id;
-        }
+    sOff[tid] = ex;
+    sRect[tid] = c ? rect[id] : make_uint2(0, 0);
+    sId[tid] = id;
+    if (kGlobal64) sDk[tid] = c ? depthKey[id] : 0u;
+    __syncthreads();
+    for (uint32_t q = tid; q < tot; q += kBlock) {
+      int g = 0;
+#pragma unroll
+      for (int step = kBlock / 2; step > 0; step >>= 1)
+        if (sOff[g + step] <= q) g += step;
+      const uint64_t pos = run + q;
+      if (pos < cap) {
+        const uint2 rc = sRect[g];
+        const uint32_t tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu;
+        const uint32_t wdt = tx1 - tx0 + 1u;
+        const uint32_t li = q - sOff[g];
+        const uint32_t dy = li / wdt;
+        const uint32_t tile = (ty0 + dy) * (uint32_t)tilesX + tx0 + (li - dy * wdt);
+        if (kGlobal64) ((uint64_t*)keysOut)[pos] = ((uint64_t)tile << 32) | sDk[g];
+        else ((uint32_t*)keysOut)[pos] = tile;
+        valsOut[pos] = sId[g];
       }
     }
     run += tot;
+    __syncthreads();
   }
 }
 
@@ -334,20 +424,14 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, int64_t count, uint32_t v) {
   k_fill_u32<<<(unsigned)((count + 255) / 256), 256, 0, st>>>(p, count, v);
 }
 
+// one pass; the scatter clears `totals` for the following pass
 template <typename KeyT>
 static void radix_pass(hipStream_t st, const SortScratch& s, const KeyT* kin, const uint32_t* vin,
-                       KeyT* kout, uint32_t* vout, const uint32_t* dN, int shift, bool iota,
-                       bool writeKeys) {
-  launch_fill_u32(st, s.totals, 256, 0u);
+                       KeyT* kout, uint32_t* vout, const uint32_t* dN, int shift, bool lastPass) {
   k_radix_hist<KeyT><<<s.G, kBlock, 0, st>>>(kin, dN, shift, s.G, s.hist, s.totals);
-  k_radix_scan<<<256, kBlock, 0, st>>>(s.hist, s.G, s.totals);
-  if (iota) {
-    if (writeKeys) k_radix_scatter<KeyT, true, true><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout);
-    else k_radix_scatter<KeyT, true, false><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout);
-  } else {
-    if (writeKeys) k_radix_scatter<KeyT, false, true><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout);
-    else k_radix_scatter<KeyT, false, false><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout);
-  }
+  k_radix_scan<<<8, kBlock, 0, st>>>(s.hist, s.G, s.totals);
+  k_radix_scatter<KeyT, true><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout,
+                                                      lastPass ? nullptr : s.totals);
 }
 
 // Stable LSD sort of (key, val) pairs by key bits [bit0, bit1), 8 per pass.
@@ -361,10 +445,11 @@ static void radix_sort(hipStream_t st, const SortScratch& s, const KeyT* kin, co
   const KeyT* ki = kin;
   const uint32_t* vi = vin;
   bool odd = false;
+  launch_fill_u32(st, s.totals, 256, 0u);
   for (int b = bit0; b < bit1; b += 8) {
     KeyT* ko = odd ? k2 : k1;
     uint32_t* vo = odd ? v2 : v1;
-    radix_pass<KeyT>(st, s, ki, vi, ko, vo, dN, b, vi == nullptr, true);
+    radix_pass<KeyT>(st, s, ki, vi, ko, vo, dN, b, b + 8 >= bit1);
     ki = ko;
     vi = vo;
     odd = !odd;
