@@ -12,13 +12,17 @@ struct SortScratch {
   int G;                  // blocks per radix pass (fixed grid)
   uint32_t* hist;         // 256 x G
   uint32_t* totals;       // 256
-  uint64_t* partials;     // G
-  uint32_t* vpartials;    // G: visible Gaussians per block
+  uint64_t* partials;     // ceil(cap / 256): key count per 256 Gaussians
+  uint32_t* vpartials;    // ceil(cap / 256): visible Gaussians per 256
+  uint32_t* cntS;         // cap: key counts in sort order
+  uint2* rectS;           // cap: tile rectangles in sort order
+  uint32_t* dkS;          // cap: depth bits in id order (global64 only)
 };
 
 struct BinArgs {
   int mode;               // 0 depth-then-tile, 1 global64
   int tilesX, numTiles;
+  int64_t n;              // Gaussians
   uint64_t cap;           // key budget
   const uint32_t* dN;     // device: number of Gaussians
   const uint32_t* cnt;

@@ -48,7 +48,7 @@ __device__ __forceinline__ uint64_t match_digit8(uint32_t d, uint64_t active) {
   return m;
 }
 
-constexpr int kItems = 16;
+constexpr int kItems = 8;
 constexpr int kRound = kBlock * kItems;   // keys per block round
 
 // block b's contiguous chunk [lo, hi) of N elements split over G blocks,
@@ -97,42 +97,47 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const KeyT* __restrict__ 
       const uint64_t act = __ballot(ok);
       if (act == 0ull) break;
       const uint64_t m = match_digit8(d[i] & 0xffu, act);
-      if (ok && (m & lanemask_lt()) == 0ull) wc[w][d[i]] += (uint32_t)__popcll(m);
+      // one lane per digit group: distinct addresses within an instruction
+      if (ok && (m & lanemask_lt()) == 0ull) atomicAdd(&wc[w][d[i]], (uint32_t)__popcll(m));
     }
   }
   __syncthreads();
   const uint32_t c = wc[0][tid] + wc[1][tid] + wc[2][tid] + wc[3][tid];
-  hist[(int64_t)blockIdx.x * 256 + tid] = c;
+  hist[(int64_t)tid * G + blockIdx.x] = c;
   if (c) atomicAdd(&totals[tid], c);
 }
 
-// 8 blocks x 256 threads: block k owns digits [32k, 32k+32); thread t sums
-// digit 32k + (t & 31) over rows group t >> 5 of the G-row table, then the
-// exclusive prefix (digit base + earlier blocks) replaces the counts.
+// one block per digit: exclusive scan of hist[d][0..G) plus the digit's base
 __global__ __launch_bounds__(kBlock) void k_radix_scan(uint32_t* __restrict__ hist, int G,
                                                        const uint32_t* __restrict__ totals) {
-  __shared__ uint32_t part[8][32];
-  __shared__ uint32_t dbase[32];
-  const int tid = threadIdx.x, dl = tid & 31, grp = tid >> 5;
-  const int d = 32 * blockIdx.x + dl;
-  const int R = (G + 7) / 8;
-  const int b0 = grp * R, b1 = min(G, b0 + R);
-  uint32_t sum = 0;
-  for (int b = b0; b < b1; b++) sum += hist[(int64_t)b * 256 + d];
-  part[grp][dl] = sum;
-  if (tid < 32) {
-    uint32_t acc = 0;
-    for (int e = 0; e < d; e++) acc += totals[e];
-    dbase[dl] = acc;
+  __shared__ uint32_t wsum[kBlock / 64];
+  __shared__ uint32_t sbase;
+  const int d = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  {
+    uint32_t v = tid < d ? totals[tid] : 0u;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (lane == 0) wsum[w] = v;
+    __syncthreads();
+    if (tid == 0) sbase = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
   }
-  __syncthreads();
-  uint32_t pre = dbase[dl];
-  for (int g = 0; g < grp; g++) pre += part[g][dl];
-  for (int b = b0; b < b1; b++) {
-    const int64_t q = (int64_t)b * 256 + d;
-    const uint32_t v = hist[q];
-    hist[q] = pre;
-    pre += v;
+  uint32_t carry = sbase;
+  uint32_t* row = hist + (int64_t)d * G;
+  for (int b0 = 0; b0 < G; b0 += kBlock) {
+    const int b = b0 + tid;
+    const uint32_t v = b < G ? row[b] : 0u;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    __syncthreads();
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t wpre = 0;
+    for (int k = 0; k < w; k++) wpre += wsum[k];
+    if (b < G) row[b] = carry + wpre + x - v;
+    carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
   }
 }
 
@@ -157,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const KeyT* __restrict
   if (blockIdx.x == 0 && totalsNext) totalsNext[tid] = 0u;
   uint32_t lo, hi;
   chunk_range(*dN, G, kRound, lo, hi);
-  gbase[tid] = scan[(int64_t)blockIdx.x * 256 + tid];
+  gbase[tid] = scan[(int64_t)tid * G + blockIdx.x];
   wc[0][tid] = wc[1][tid] = wc[2][tid] = wc[3][tid] = 0;
   __syncthreads();
   for (uint32_t r = lo; r < hi; r += kRound) {
@@ -173,18 +178,25 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const KeyT* __restrict
       val[i] = ok ? (valsIn ? valsIn[idx] : idx) : 0u;
       rank[i] = ok ? 0u : 0xffffffffu;
     }
+    // ranks: all groups' counter updates are issued back to back (the LDS
+    // applies one wave's operations in order), then each lane fetches its
+    // group leader's pre-add count
+    uint64_t mm[kItems];
+    uint32_t old[kItems];
 #pragma unroll
     for (int i = 0; i < kItems; i++) {
       const bool ok = rank[i] == 0u;
-      const uint64_t act = __ballot(ok);
-      if (act == 0ull) break;
-      const uint32_t d = digit_of(key[i], shift);
-      const uint64_t m = match_digit8(d, act);
-      if (ok) {
-        const uint32_t lt = (uint32_t)__popcll(m & lanemask_lt());
-        rank[i] = wc[w][d] + lt;
-        if (lt == 0u) wc[w][d] += (uint32_t)__popcll(m);
-      }
+      mm[i] = match_digit8(digit_of(key[i], shift), __ballot(ok)) ;
+      old[i] = 0u;
+      if (ok && (mm[i] & lanemask_lt()) == 0ull)
+        old[i] = atomicAdd(&wc[w][digit_of(key[i], shift)], (uint32_t)__popcll(mm[i]));
+    }
+#pragma unroll
+    for (int i = 0; i < kItems; i++) {
+      const bool ok = rank[i] == 0u;
+      const int leader = mm[i] ? (int)__builtin_ctzll(mm[i]) : 0;
+      const uint32_t b = (uint32_t)__shfl((int)old[i], leader);
+      if (ok) rank[i] = b + (uint32_t)__popcll(mm[i] & lanemask_lt());
     }
     __syncthreads();
     // 2. per digit: wave prefixes, round total, local start (block scan)
@@ -250,25 +262,34 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t v, uint32_t& total,
   return pre + x - v;
 }
 
-// per-block key counts over Gaussians in (depth) order `order`
-// (order == nullptr: identity order), and the number of visible ones
+// One block per 256 Gaussians in (depth) order `order` (nullptr: id
+// order): gathers each one's key count and tile rectangle once, stores them
+// in sort order for the emission pass, and writes the block's key total and
+// visible count.
 __global__ __launch_bounds__(kBlock) void k_count_partials(const uint32_t* __restrict__ order,
                                                            const uint32_t* __restrict__ cnt,
-                                                           const uint32_t* __restrict__ dN, int G,
+                                                           const uint2* __restrict__ rect,
+                                                           const uint32_t* __restrict__ depthKey,
+                                                           const uint32_t* __restrict__ dN,
+                                                           uint32_t* __restrict__ cntS,
+                                                           uint2* __restrict__ rectS,
+                                                           uint32_t* __restrict__ dkS,
                                                            uint64_t* __restrict__ partials,
                                                            uint32_t* __restrict__ vpartials) {
-  __shared__ uint64_t wsum[4];
+  __shared__ uint32_t wsum[4];
   __shared__ uint32_t wvis[4];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  uint32_t lo, hi;
-  chunk_range(*dN, G, kBlock, lo, hi);
-  uint64_t acc = 0;
-  uint32_t vis = 0;
-  for (uint32_t r = lo + tid; r < hi; r += kBlock) {
-    const uint32_t c = cnt[order ? order[r] : r];
-    acc += c;
-    vis += c != 0u;
+  const uint32_t N = *dN;
+  const uint32_t r = blockIdx.x * kBlock + tid;
+  uint32_t c = 0;
+  if (r < N) {
+    const uint32_t id = order ? order[r] : r;
+    c = cnt[id];
+    cntS[r] = c;
+    rectS[r] = c ? rect[id] : make_uint2(0, 0);
+    if (dkS) dkS[r] = c ? depthKey[id] : 0u;
   }
+  uint32_t acc = c, vis = c != 0u;
   for (int o = 32; o > 0; o >>= 1) {
     acc += __shfl_xor(acc, o);
     vis += __shfl_xor(vis, o);
@@ -276,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_count_partials(const uint32_t* __res
   if (lane == 0) { wsum[w] = acc; wvis[w] = vis; }
   __syncthreads();
   if (tid == 0) {
-    partials[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    partials[blockIdx.x] = (uint64_t)wsum[0] + wsum[1] + wsum[2] + wsum[3];
     vpartials[blockIdx.x] = wvis[0] + wvis[1] + wvis[2] + wvis[3];
   }
 }
@@ -341,10 +362,10 @@ __global__ __launch_bounds__(1024) void k_count_top(uint64_t* __restrict__ parti
 // are dropped (counted by k_count_top).
 template <bool kGlobal64>
 __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ order,
-                                                 const uint32_t* __restrict__ cnt,
-                                                 const uint2* __restrict__ rect,
-                                                 const uint32_t* __restrict__ depthKey,
-                                                 const uint32_t* __restrict__ dN, int G,
+                                                 const uint32_t* __restrict__ cntS,
+                                                 const uint2* __restrict__ rectS,
+                                                 const uint32_t* __restrict__ dkS,
+                                                 const uint32_t* __restrict__ dN,
                                                  const uint64_t* __restrict__ partials, int tilesX,
                                                  uint64_t cap, void* __restrict__ keysOut,
                                                  uint32_t* __restrict__ valsOut) {
@@ -354,41 +375,35 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ or
   __shared__ uint32_t sId[kBlock];
   __shared__ uint32_t sDk[kBlock];
   const int tid = threadIdx.x;
-  uint32_t lo, hi;
-  chunk_range(*dN, G, kBlock, lo, hi);
-  uint64_t run = partials[blockIdx.x];
-  for (uint32_t r0 = lo; r0 < hi; r0 += kBlock) {
-    const uint32_t r = r0 + tid;
-    const bool ok = r < hi;
-    const uint32_t id = ok ? (kGlobal64 ? r : order[r]) : 0u;
-    const uint32_t c = ok ? cnt[id] : 0u;
-    uint32_t tot;
-    const uint32_t ex = block_excl_scan(c, tot, wsum);
-    sOff[tid] = ex;
-    sRect[tid] = c ? rect[id] : make_uint2(0, 0);
-    sId[tid] = id;
-    if (kGlobal64) sDk[tid] = c ? depthKey[id] : 0u;
-    __syncthreads();
-    for (uint32_t q = tid; q < tot; q += kBlock) {
-      int g = 0;
+  const uint32_t N = *dN;
+  const uint32_t r = blockIdx.x * kBlock + tid;
+  const bool ok = r < N;
+  const uint32_t c = ok ? cntS[r] : 0u;
+  uint32_t tot;
+  const uint32_t ex = block_excl_scan(c, tot, wsum);
+  sOff[tid] = ex;
+  sRect[tid] = ok ? rectS[r] : make_uint2(0, 0);
+  sId[tid] = ok ? (kGlobal64 ? r : order[r]) : 0u;
+  if (kGlobal64) sDk[tid] = ok ? dkS[r] : 0u;
+  __syncthreads();
+  const uint64_t run = partials[blockIdx.x];
+  for (uint32_t q = tid; q < tot; q += kBlock) {
+    int g = 0;
 #pragma unroll
-      for (int step = kBlock / 2; step > 0; step >>= 1)
-        if (sOff[g + step] <= q) g += step;
-      const uint64_t pos = run + q;
-      if (pos < cap) {
-        const uint2 rc = sRect[g];
-        const uint32_t tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu;
-        const uint32_t wdt = tx1 - tx0 + 1u;
-        const uint32_t li = q - sOff[g];
-        const uint32_t dy = li / wdt;
-        const uint32_t tile = (ty0 + dy) * (uint32_t)tilesX + tx0 + (li - dy * wdt);
-        if (kGlobal64) ((uint64_t*)keysOut)[pos] = ((uint64_t)tile << 32) | sDk[g];
-        else ((uint32_t*)keysOut)[pos] = tile;
-        valsOut[pos] = sId[g];
-      }
+    for (int step = kBlock / 2; step > 0; step >>= 1)
+      if (sOff[g + step] <= q) g += step;
+    const uint64_t pos = run + q;
+    if (pos < cap) {
+      const uint2 rc = sRect[g];
+      const uint32_t tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu;
+      const uint32_t wdt = tx1 - tx0 + 1u;
+      const uint32_t li = q - sOff[g];
+      const uint32_t dy = li / wdt;
+      const uint32_t tile = (ty0 + dy) * (uint32_t)tilesX + tx0 + (li - dy * wdt);
+      if (kGlobal64) ((uint64_t*)keysOut)[pos] = ((uint64_t)tile << 32) | sDk[g];
+      else ((uint32_t*)keysOut)[pos] = tile;
+      valsOut[pos] = sId[g];
     }
-    run += tot;
-    __syncthreads();
   }
 }
 
@@ -429,7 +444,7 @@ template <typename KeyT>
 static void radix_pass(hipStream_t st, const SortScratch& s, const KeyT* kin, const uint32_t* vin,
                        KeyT* kout, uint32_t* vout, const uint32_t* dN, int shift, bool lastPass) {
   k_radix_hist<KeyT><<<s.G, kBlock, 0, st>>>(kin, dN, shift, s.G, s.hist, s.totals);
-  k_radix_scan<<<8, kBlock, 0, st>>>(s.hist, s.G, s.totals);
+  k_radix_scan<<<256, kBlock, 0, st>>>(s.hist, s.G, s.totals);
   k_radix_scatter<KeyT, true><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout,
                                                       lastPass ? nullptr : s.totals);
 }
@@ -468,16 +483,18 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
   const SortScratch& s = a.scratch;
   const int tb = tile_bits(a.numTiles);
   const int tpasses = (tb + 7) / 8;
+  const int Gc = (int)((a.n + kBlock - 1) / kBlock);
   if (a.mode == 0) {
     // 1) stable sort of the Gaussians by depth bits, values = ids
     const uint32_t* dk;
     const uint32_t* order;
     radix_sort<uint32_t>(st, s, a.depthKey, nullptr, a.dkA, a.dvA, a.dkB, a.dvB, a.dN, 0, 32, &dk, &order);
     // 2) per-Gaussian key counts in depth order -> offsets -> one key per tile
-    k_count_partials<<<s.G, kBlock, 0, st>>>(order, a.cnt, a.dN, s.G, s.partials, s.vpartials);
-    k_count_top<<<1, 1024, 0, st>>>(s.partials, s.vpartials, s.G, a.cap, a.numKeys, a.totalKeys, a.numVisible, a.stats);
-    k_emit<false><<<s.G, kBlock, 0, st>>>(order, a.cnt, a.rect, a.depthKey, a.dN, s.G, s.partials,
-                                          a.tilesX, a.cap, a.tkA, a.tvA);
+    k_count_partials<<<Gc, kBlock, 0, st>>>(order, a.cnt, a.rect, a.depthKey, a.dN, s.cntS, s.rectS, nullptr,
+                                            s.partials, s.vpartials);
+    k_count_top<<<1, 1024, 0, st>>>(s.partials, s.vpartials, Gc, a.cap, a.numKeys, a.totalKeys, a.numVisible, a.stats);
+    k_emit<false><<<Gc, kBlock, 0, st>>>(order, s.cntS, s.rectS, nullptr, a.dN, s.partials, a.tilesX, a.cap,
+                                         a.tkA, a.tvA);
     // 3) stable sort by tile id
     const uint32_t* tk;
     const uint32_t* tv;
@@ -489,10 +506,11 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
     launch_fill_u32(st, (uint32_t*)a.ranges, 2 * (int64_t)a.numTiles, 0u);
     k_tile_ranges<uint32_t><<<1024, 256, 0, st>>>(tk, a.numKeys, 0, a.numTiles, a.ranges);
   } else {
-    k_count_partials<<<s.G, kBlock, 0, st>>>(nullptr, a.cnt, a.dN, s.G, s.partials, s.vpartials);
-    k_count_top<<<1, 1024, 0, st>>>(s.partials, s.vpartials, s.G, a.cap, a.numKeys, a.totalKeys, a.numVisible, a.stats);
-    k_emit<true><<<s.G, kBlock, 0, st>>>(nullptr, a.cnt, a.rect, a.depthKey, a.dN, s.G, s.partials,
-                                         a.tilesX, a.cap, a.tkA, a.tvA);
+    k_count_partials<<<Gc, kBlock, 0, st>>>(nullptr, a.cnt, a.rect, a.depthKey, a.dN, s.cntS, s.rectS, s.dkS,
+                                            s.partials, s.vpartials);
+    k_count_top<<<1, 1024, 0, st>>>(s.partials, s.vpartials, Gc, a.cap, a.numKeys, a.totalKeys, a.numVisible, a.stats);
+    k_emit<true><<<Gc, kBlock, 0, st>>>(nullptr, s.cntS, s.rectS, s.dkS, a.dN, s.partials, a.tilesX, a.cap,
+                                        a.tkA, a.tvA);
     const uint64_t* tk;
     const uint32_t* tv;
     radix_sort<uint64_t>(st, s, (const uint64_t*)a.tkA, a.tvA, (uint64_t*)a.tkB, a.tvB,

@@ -21,7 +21,7 @@ using namespace gs;
 
 namespace {
 
-constexpr int kSortBlocks = 512;
+constexpr int kSortBlocks = 1024;
 constexpr int64_t kDefaultEntries = 12000000;   // shaders.js:34 ENTRIES_CAP
 constexpr int64_t kTargetLimitBytes = int64_t(64) << 30;
 
@@ -281,6 +281,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
     a.mode = h->o.sortMode;
     a.tilesX = cam.tilesX;
     a.numTiles = cam.numTiles;
+    a.n = n;
     a.cap = (uint64_t)h->entriesCap;
     a.dN = h->dN;
     a.cnt = h->cnt;
@@ -627,8 +628,12 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   h->sort.G = kSortBlocks;
   rc |= dalloc(h, &h->sort.hist, (size_t)256 * kSortBlocks * 4);
   rc |= dalloc(h, &h->sort.totals, 256 * 4);
-  rc |= dalloc(h, &h->sort.partials, (size_t)kSortBlocks * 8);
-  rc |= dalloc(h, &h->sort.vpartials, (size_t)kSortBlocks * 4);
+  const size_t gc = (size_t)((cap + 255) / 256);
+  rc |= dalloc(h, &h->sort.partials, gc * 8);
+  rc |= dalloc(h, &h->sort.vpartials, gc * 4);
+  rc |= dalloc(h, &h->sort.cntS, (size_t)cap * 4);
+  rc |= dalloc(h, &h->sort.rectS, (size_t)cap * 8);
+  rc |= dalloc(h, &h->sort.dkS, (size_t)cap * 4);
   rc |= dalloc(h, &h->dN, 4);
   rc |= dalloc(h, &h->numKeys, 4);
   rc |= dalloc(h, &h->numVisible, 4);
