@@ -44,7 +44,7 @@ enum { GS_SORT_DEPTH_THEN_TILE = 0, GS_SORT_GLOBAL64 = 1 };
  * :74, :105-107, :302, :319, :338-341), plus the MI355X build's own knobs. */
 typedef struct gs_opts {
   int32_t size;        /* sizeof(gs_opts); set by gs_default_opts */
-  float eCut;          /* Gaussian exponent cutoff, default 4.5 (shaders.js:44) */
+  float eCut;          /* Gaussian exponent cutoff, default 4.5 (shaders.js:44); must lie in [0, 80] */
   float aMin;          /* minimum alpha, default 0.0039 (shaders.js:45) */
   float radClamp;      /* binning radius clamp, fraction of width, default 1 (shaders.js:46-53) */
   int32_t shDeg;       /* SH degree 0..3, default 2 (trainer.js:32) */

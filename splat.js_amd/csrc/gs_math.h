@@ -76,6 +76,28 @@ GS_HD float gs_expf(float x) {
   return p * gs_u2f((uint32_t)(k + 127) << 23);
 }
 
+// gs_expf without branches for the rasterizer's inner loops. For x in
+// [-87, 0] the operations are exactly those of gs_expf (k = round(x/ln2) lies
+// in [-126, 0], the normal-result path), so the bits agree. Other inputs
+// (including NaN) are clamped into that range first and give a finite,
+// positive value that callers discard.
+GS_HD float gs_expf_nonpos(float x) {
+  const float xc = fminf(fmaxf(x, -87.0f), 0.0f);
+  const float kf = rintf(xc * 1.44269502162933349609375f);
+  float r = fmaf(kf, -0.693145751953125f, xc);
+  r = fmaf(kf, -1.428606765330187045e-06f, r);
+  float p = 1.984126984126984e-04f;
+  p = fmaf(p, r, 1.388888888888889e-03f);
+  p = fmaf(p, r, 8.333333333333333e-03f);
+  p = fmaf(p, r, 4.166666666666667e-02f);
+  p = fmaf(p, r, 1.666666666666667e-01f);
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  const int k = (int)kf;
+  return p * gs_u2f((uint32_t)(k + 127) << 23);
+}
+
 // natural log; NaN for x < 0, -inf at 0, +inf at +inf.
 GS_HD float gs_logf(float x) {
   if (x != x) return x;

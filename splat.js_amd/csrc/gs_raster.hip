@@ -82,6 +82,15 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// 1/x for x in [2^-9, 1]: hardware reciprocal estimate plus one Newton step
+// with fused residual. Equal to IEEE 1.0f/x for all 75,497,473 floats of that
+// range (exhaustive check on gfx950); callers guarantee the range.
+__device__ __forceinline__ float rcp_exact_unit(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  const float e = fmaf(-x, r, 1.0f);
+  return fmaf(r, e, r);
+}
+
 template <int kLoss>
 __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterBufs b) {
   __shared__ float4 sA[kBlock];   // mx, my, A, B
@@ -123,25 +132,23 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
         while (m) {
           const int j = c * 64 + (int)__builtin_ctzll(m);
           m &= m - 1ull;
-          if (!done) {
-            const float4 a = sA[j];
-            const float4 bb = sB[j];
-            const float dx = px - a.x, dy = py - a.y;
-            const float t2 = fmaf(bb.x * dy, dy, (a.z * dx) * dx);
-            const float ee = fmaxf(fmaf(0.5f, t2, (a.w * dx) * dy), 0.0f);
-            if (ee <= cut.eCut) {
-              const float araw = bb.y * gs_expf(-ee);
-              const float alpha = fminf(0.99f, araw);
-              if (alpha >= cut.aMin) {
-                const float4 c4 = sC[j];
-                const float w = T * alpha;
-                C0 = fmaf(w, c4.x, C0);
-                C1 = fmaf(w, c4.y, C1);
-                C2 = fmaf(w, c4.z, C2);
-                T = T * (1.0f - alpha);
-                if (T < 1e-4f) { done = true; term = (uint32_t)j; }
-              }
-            }
+          // Straight-line for all lanes; only lanes that take the splat
+          // (not done, inside the cutoff, alpha >= A_MIN) update their state.
+          const float4 a = sA[j];
+          const float4 bb = sB[j];
+          const float dx = px - a.x, dy = py - a.y;
+          const float t2 = fmaf(bb.x * dy, dy, (a.z * dx) * dx);
+          const float ee = fmaxf(fmaf(0.5f, t2, (a.w * dx) * dy), 0.0f);
+          const float araw = bb.y * gs_expf_nonpos(-ee);   // == gs_expf(-ee) for ee <= eCut <= 80
+          const float alpha = fminf(0.99f, araw);
+          if (!done && ee <= cut.eCut && alpha >= cut.aMin) {
+            const float4 c4 = sC[j];
+            const float w = T * alpha;
+            C0 = fmaf(w, c4.x, C0);
+            C1 = fmaf(w, c4.y, C1);
+            C2 = fmaf(w, c4.z, C2);
+            T = T * (1.0f - alpha);
+            if (T < 1e-4f) { done = true; term = (uint32_t)j; }
           }
           if (__ballot(!done) == 0ull) { m = 0ull; c = 4; }
         }
@@ -335,53 +342,54 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
         m &= ~(1ull << bit);
         const int j = c * 64 + bit;
         const uint32_t k = lo + (uint32_t)j;
-        bool contrib = false;
+        // Straight-line for all 64 lanes; lanes that do not take the splat
+        // (outside the image, past their `end`, outside the cutoff, alpha
+        // below A_MIN) keep their state and contribute zero terms.
+        const float4 a = sA[j];
+        const float4 bb = sB[j];
+        const float dx = px - a.x, dy = py - a.y;
+        const float cA = a.z, cB = a.w, cC = bb.x;
+        const float t2 = fmaf(cC * dy, dy, (cA * dx) * dx);
+        const float e = fmaxf(fmaf(0.5f, t2, (cB * dx) * dy), 0.0f);
+        const float G = gs_expf_nonpos(-e);   // == gs_expf(-e) for e <= eCut <= 80
+        const float araw = bb.y * G;
+        const float alpha = fminf(0.99f, araw);
+        const bool contrib = active && k < myEnd && e <= cut.eCut && alpha >= cut.aMin;
+        if (__ballot(contrib) == 0ull) continue;
         float v[10];
-#pragma unroll
-        for (int q = 0; q < 10; q++) v[q] = 0.0f;
-        if (active && k < myEnd) {
-          const float4 a = sA[j];
-          const float4 bb = sB[j];
-          const float dx = px - a.x, dy = py - a.y;
-          const float cA = a.z, cB = a.w, cC = bb.x;
-          const float t2 = fmaf(cC * dy, dy, (cA * dx) * dx);
-          const float e = fmaxf(fmaf(0.5f, t2, (cB * dx) * dy), 0.0f);
-          if (e <= cut.eCut) {
-            const float G = gs_expf(-e);
-            const float araw = bb.y * G;
-            const float alpha = fminf(0.99f, araw);
-            if (alpha >= cut.aMin) {
-              contrib = true;
-              const float4 c4 = sC[j];
-              const float2 d2 = sD[j];
-              const float col[3] = {c4.x, c4.y, c4.z};
-              const float opa = c4.w, cnorm = d2.y;
-              // Tb = Ta / (1 - alpha), S / (1 - alpha) through one reciprocal
-              const float rom = 1.0f / (1.0f - alpha);
-              const float Tb = Ta * rom;
-              const float aT = alpha * Tb;
-              const float u[3] = {col[0] * Tb - S0 * rom, col[1] * Tb - S1 * rom, col[2] * Tb - S2 * rom};
-              float galpha = dot3(gC, u);
-              if (araw > 0.99f) galpha = 0.0f;
-              const float ga = galpha * araw;
-              v[0] = ga * (cA * dx + cB * dy);
-              v[1] = ga * (cB * dx + cC * dy);
-              v[2] = (-ga) * 0.5f * dx * dx * cnorm;
-              v[3] = (-ga) * dx * dy * cnorm;
-              v[4] = (-ga) * 0.5f * dy * dy * cnorm;
-              v[5] = galpha * opa * G;
-              v[6] = ga * (1.0f - opa);
-              v[7] = gC[0] * aT;
-              v[8] = gC[1] * aT;
-              v[9] = gC[2] * aT;
-              S0 = S0 + col[0] * alpha * Tb;
-              S1 = S1 + col[1] * alpha * Tb;
-              S2 = S2 + col[2] * alpha * Tb;
-              Ta = Tb;
-            }
+        {
+          const float4 c4 = sC[j];
+          const float2 d2 = sD[j];
+          const float col[3] = {c4.x, c4.y, c4.z};
+          const float opa = c4.w, cnorm = d2.y;
+          // Tb = Ta / (1 - alpha), S / (1 - alpha) through one reciprocal.
+          // 1 - alpha lies in [0.0099, 1] (alpha in [0, 0.99]); on [2^-9, 1]
+          // v_rcp_f32 refined by one FMA Newton step equals the correctly
+          // rounded 1/x for every float (checked exhaustively on gfx950).
+          const float rom = rcp_exact_unit(1.0f - alpha);
+          const float Tb = Ta * rom;
+          const float aT = alpha * Tb;
+          const float u[3] = {col[0] * Tb - S0 * rom, col[1] * Tb - S1 * rom, col[2] * Tb - S2 * rom};
+          float galpha = dot3(gC, u);
+          if (araw > 0.99f) galpha = 0.0f;
+          const float ga = galpha * araw;
+          v[0] = ga * (cA * dx + cB * dy);
+          v[1] = ga * (cB * dx + cC * dy);
+          v[2] = (-ga) * 0.5f * dx * dx * cnorm;
+          v[3] = (-ga) * dx * dy * cnorm;
+          v[4] = (-ga) * 0.5f * dy * dy * cnorm;
+          v[5] = galpha * opa * G;
+          v[6] = ga * (1.0f - opa);
+          v[7] = gC[0] * aT;
+          v[8] = gC[1] * aT;
+          v[9] = gC[2] * aT;
+          if (contrib) {
+            S0 = S0 + col[0] * alpha * Tb;
+            S1 = S1 + col[1] * alpha * Tb;
+            S2 = S2 + col[2] * alpha * Tb;
+            Ta = Tb;
           }
         }
-        if (__ballot(contrib) == 0ull) continue;
         const uint32_t gid = sId[j];
         if (kAccum == 0) {
           uint32_t iv[10];
@@ -393,6 +401,8 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
           const uint32_t r = wave_reduce10<uint32_t>(iv);
           if (flusher && r != 0u) atomicAdd((uint32_t*)accum + (int64_t)gid * 12 + mySlot, r);
         } else {
+#pragma unroll
+          for (int q = 0; q < 10; q++) v[q] = contrib ? v[q] : 0.0f;
           const float r = wave_reduce10<float>(v);
           if (flusher && r != 0.0f) unsafeAtomicAdd((float*)accum + (int64_t)gid * 12 + mySlot, r);
         }

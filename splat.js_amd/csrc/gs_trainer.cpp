@@ -505,7 +505,8 @@ int gs_create(const gs_opts* o, gs_trainer_t** out) {
     h->o = *o;
   }
   if (h->o.shDeg < 0 || h->o.shDeg > 3 || h->o.sortMode < 0 || h->o.sortMode > 1 ||
-      h->o.gradAccum < 0 || h->o.gradAccum > 1 || h->o.loss < 0 || h->o.loss > 1) {
+      h->o.gradAccum < 0 || h->o.gradAccum > 1 || h->o.loss < 0 || h->o.loss > 1 ||
+      !(h->o.eCut >= 0.0f && h->o.eCut <= 80.0f)) {
     delete h;
     return GS_E_ARG;
   }

@@ -77,3 +77,57 @@ def test_library_is_built_for_gfx950():
                          stderr=subprocess.STDOUT, text=True).stdout
     blob = open(gsabi.LIB_PATH, "rb").read()
     assert b"gfx950" in blob
+
+
+def test_create_rejects_ecut_outside_supported_range():
+    """The rasterizer's branch-free exp is exact only for exponents in
+    [-80, 0]; options outside that range are refused before any HIP call."""
+    L = gsabi.lib()
+    for bad in (80.5, -1.0, float("nan"), float("inf")):
+        o = gsabi.default_opts()
+        o.eCut = bad
+        h = C.c_void_p()
+        assert L.gs_create(C.byref(o), C.byref(h)) == -1, bad  # GS_E_ARG
+        assert not h.value
+
+
+MATH_PROBE = r"""
+#include <stdio.h>
+#include "gs_math.h"
+int main(void) {
+  /* every float in [-87, 0] (sign bit set, magnitude 0 .. 87) */
+  const uint32_t top = gs_f2u(87.0f);
+  unsigned long long bad = 0, n = 0;
+#pragma omp parallel for reduction(+ : bad, n) schedule(static)
+  for (long long mm = 0; mm <= (long long)top; mm++) {
+    const float x = gs_u2f(0x80000000u | (uint32_t)mm);
+    n++;
+    if (gs_f2u(gs_expf_nonpos(x)) != gs_f2u(gs_expf(x))) bad++;
+  }
+  /* outside the range: finite and positive */
+  const float odd[] = {-87.5f, -1e30f, gs_u2f(0xff800000u), gs_u2f(0x7fc00000u), 1.0f, 1e30f};
+  int oddbad = 0;
+  for (int i = 0; i < 6; i++) {
+    const float r = gs_expf_nonpos(odd[i]);
+    if (!(r > 0.0f && r <= 1.0f)) oddbad++;
+  }
+  printf("%llu %llu %d\n", n, bad, oddbad);
+  return 0;
+}
+"""
+
+
+def test_branch_free_exp_matches_portable_exp_exhaustively(tmp_path):
+    """gs_expf_nonpos (raster inner loops) == gs_expf bit for bit on every
+    float of [-87, 0], the range eCut <= 80 confines it to."""
+    src = tmp_path / "m.c"
+    src.write_text(MATH_PROBE)
+    exe = tmp_path / "m"
+    flags = ["-O2", "-ffp-contract=off", "-fopenmp"]
+    cpu = open("/proc/cpuinfo").read() if os.path.exists("/proc/cpuinfo") else ""
+    if re.search(r"\bfma\b", cpu) and re.search(r"\bsse4_1\b", cpu):
+        flags += ["-mfma", "-msse4.1"]   # hardware fmaf/rintf: same IEEE results, 20x faster than libm
+    subprocess.check_call(["gcc", *flags, "-I", os.path.join(ROOT, "splat.js_amd", "csrc"),
+                           "-o", str(exe), str(src), "-lm"])
+    n, bad, oddbad = map(int, subprocess.check_output([str(exe)], text=True).split())
+    assert n == 0x42ae0000 + 1 and bad == 0 and oddbad == 0
