@@ -10,6 +10,7 @@
 #include <stdint.h>
 
 #include "gs_math.h"
+#include "gs_cover.h"
 
 namespace gs {
 

@@ -3,12 +3,12 @@
 // compensation, SH colour, opacity-aware radius, tile rectangle.
 //
 // Output per visible Gaussian:
-//   rec[4i..4i+3]  raster record, 64 B: (mx, my, A, B), (C, opa*comp, ex, ey),
-//                  (r, g, b, opa), (comp, cnorm, 0, 0). ex, ey bound the
-//                  ellipse where the splat can reach alpha >= A_MIN (half
-//                  extents sqrt(2 eMax Vd_xx), sqrt(2 eMax Vd_yy), widened by
-//                  1% + 0.05 px); the raster passes use them only to skip
-//                  whole waves, never to change a pixel's value.
+//   rec[4i..4i+3]  raster record, 64 B: (mx, my, A, B), (C, opa*comp, L, 0),
+//                  (r, g, b, opa), (comp, cnorm, 0, 0). L (gs_cover.h) bounds
+//                  the exponent at which a pixel can still take the splat
+//                  (min(eCut, ln(opa*comp / A_MIN)) plus rounding slack); the
+//                  raster passes use it only to skip whole waves, never to
+//                  change a pixel's value.
 //   depthKey[i]    f32 depth bits (positive, so they order like the depths);
 //                  0xffffffff when culled, which sorts culled Gaussians last
 //   rect[i]        (tx0 | tx1 << 16, ty0 | ty1 << 16)
@@ -76,10 +76,8 @@ __global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam 
       col[2] = fmaxf(col[2], 0.0f);
     }
     const float cnorm = 1.0f / (1.0f + conic_lmax(g.va, g.vb, g.vc));
-    const float ex = sqrtf(2.0f * eMax * ad) * 1.01f + 0.05f;
-    const float ey = sqrtf(2.0f * eMax * cd) * 1.01f + 0.05f;
     rec[4 * i + 0] = make_float4(mx, my, cd * inv, (-g.vb) * inv);
-    rec[4 * i + 1] = make_float4(ad * inv, opa * comp, ex, ey);
+    rec[4 * i + 1] = make_float4(ad * inv, opa * comp, gs_cover_level(eMax), 0.0f);
     rec[4 * i + 2] = make_float4(col[0], col[1], col[2], opa);
     rec[4 * i + 3] = make_float4(comp, cnorm, 0.0f, 0.0f);
     key = gs_f2u(g.pc[2]);

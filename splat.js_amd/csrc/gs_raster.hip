@@ -5,8 +5,8 @@
 // each wave an 8x8 block. The tile's sorted keys are consumed in batches of
 // 256: every thread loads one raster record into LDS, then every pixel walks
 // the batch from LDS (broadcast reads). A wave skips, as a whole, every splat
-// whose alpha >= A_MIN ellipse misses its 8x8 block (the pixels it skips
-// could not have taken the splat). The forward stops the whole tile once
+// that no pixel centre of its 8x8 block can take (gs_cover.h), and leaves a
+// splat early once no lane is inside the cutoff or takes it. The forward stops the whole tile once
 // every pixel has reached T < 1e-4.
 //
 // Forward (shaders.js:556-585) writes the image, the final transmittance T
@@ -47,20 +47,21 @@ __device__ __forceinline__ PixelMap pixel_of_thread(const Cam& cam, int tile, in
   return m;
 }
 
-// Bit w set when the splat's alpha >= A_MIN ellipse (bounding box mx +- ex,
-// my +- ey) can reach a pixel centre of wave w's 8x8 block. A wave only
-// visits the splats of its bits, so the pixels of a cleared block are the
-// ones that could not have taken the splat.
+// Bit w set when some pixel centre of wave w's 8x8 block may take the splat
+// (gs_cover_rect: exact minimum of the exponent over the block, with a
+// rounding margin, against the splat's cover level L). A wave only visits
+// the splats of its bits; a cleared bit means no pixel of the block could
+// have taken the splat, so skipping it changes nothing.
 __device__ __forceinline__ uint32_t wave_cover(const Cam& cam, int tile, float4 a, float4 bb) {
   const int tx = tile % cam.tilesX, ty = tile / cam.tilesX;
   const float x0 = (float)(tx * kTile) + 0.5f, y0 = (float)(ty * kTile) + 0.5f;
-  const float mx = a.x, my = a.y, ex = bb.z, ey = bb.w;
+  const float mx = a.x, my = a.y, A = a.z, B = a.w, C = bb.x, L = bb.z;
   uint32_t m = 0;
 #pragma unroll
   for (int w = 0; w < 4; w++) {
     const float bx = x0 + (float)((w & 1) * 8), by = y0 + (float)((w >> 1) * 8);
-    const bool miss = mx + ex < bx || mx - ex > bx + 7.0f || my + ey < by || my - ey > by + 7.0f;
-    m |= miss ? 0u : (1u << w);
+    const int hit = gs_cover_rect(A, B, C, L, bx - mx, (bx + 7.0f) - mx, by - my, (by + 7.0f) - my);
+    m |= hit ? (1u << w) : 0u;
   }
   return m;
 }
@@ -94,7 +95,7 @@ __device__ __forceinline__ float rcp_exact_unit(float x) {
 template <int kLoss>
 __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterBufs b) {
   __shared__ float4 sA[kBlock];   // mx, my, A, B
-  __shared__ float4 sB[kBlock];   // C, opa*comp, ex, ey
+  __shared__ float4 sB[kBlock];   // C, opa*comp, L, 0
   __shared__ float4 sC[kBlock];   // r, g, b, opa
   __shared__ uint64_t sMask[4][4];
   __shared__ double sStat[3][kBlock / 64];
@@ -139,18 +140,21 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
           const float dx = px - a.x, dy = py - a.y;
           const float t2 = fmaf(bb.x * dy, dy, (a.z * dx) * dx);
           const float ee = fmaxf(fmaf(0.5f, t2, (a.w * dx) * dy), 0.0f);
-          const float araw = bb.y * gs_expf_nonpos(-ee);   // == gs_expf(-ee) for ee <= eCut <= 80
-          const float alpha = fminf(0.99f, araw);
-          if (!done && ee <= cut.eCut && alpha >= cut.aMin) {
-            const float4 c4 = sC[j];
-            const float w = T * alpha;
-            C0 = fmaf(w, c4.x, C0);
-            C1 = fmaf(w, c4.y, C1);
-            C2 = fmaf(w, c4.z, C2);
-            T = T * (1.0f - alpha);
-            if (T < 1e-4f) { done = true; term = (uint32_t)j; }
+          const bool live = !done && ee <= cut.eCut;
+          if (__ballot(live) != 0ull) {
+            const float araw = bb.y * gs_expf_nonpos(-ee);   // == gs_expf(-ee) for ee <= eCut <= 80
+            const float alpha = fminf(0.99f, araw);
+            if (live && alpha >= cut.aMin) {
+              const float4 c4 = sC[j];
+              const float w = T * alpha;
+              C0 = fmaf(w, c4.x, C0);
+              C1 = fmaf(w, c4.y, C1);
+              C2 = fmaf(w, c4.z, C2);
+              T = T * (1.0f - alpha);
+              if (T < 1e-4f) { done = true; term = (uint32_t)j; }
+            }
+            if (__ballot(!done) == 0ull) { m = 0ull; c = 4; }
           }
-          if (__ballot(!done) == 0ull) { m = 0ull; c = 4; }
         }
       }
       if (term != 0xffffffffu) end = base + term + 1u;
@@ -281,7 +285,7 @@ __device__ __forceinline__ int reduce10_slot() {
 template <int kAccum>
 __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterBufs b, void* accum) {
   __shared__ float4 sA[kBlock];   // mx, my, A, B
-  __shared__ float4 sB[kBlock];   // C, opa*comp, ex, ey
+  __shared__ float4 sB[kBlock];   // C, opa*comp, L, 0
   __shared__ float4 sC[kBlock];   // r, g, b, opa
   __shared__ float2 sD[kBlock];   // comp, cnorm
   __shared__ uint32_t sId[kBlock];
@@ -344,17 +348,21 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
         const uint32_t k = lo + (uint32_t)j;
         // Straight-line for all 64 lanes; lanes that do not take the splat
         // (outside the image, past their `end`, outside the cutoff, alpha
-        // below A_MIN) keep their state and contribute zero terms.
+        // below A_MIN) keep their state and contribute zero terms. The wave
+        // leaves as soon as no lane is inside the cutoff (before the exp) or
+        // no lane takes the splat.
         const float4 a = sA[j];
         const float4 bb = sB[j];
         const float dx = px - a.x, dy = py - a.y;
         const float cA = a.z, cB = a.w, cC = bb.x;
         const float t2 = fmaf(cC * dy, dy, (cA * dx) * dx);
         const float e = fmaxf(fmaf(0.5f, t2, (cB * dx) * dy), 0.0f);
+        const bool live = active && k < myEnd && e <= cut.eCut;
+        if (__ballot(live) == 0ull) continue;
         const float G = gs_expf_nonpos(-e);   // == gs_expf(-e) for e <= eCut <= 80
         const float araw = bb.y * G;
         const float alpha = fminf(0.99f, araw);
-        const bool contrib = active && k < myEnd && e <= cut.eCut && alpha >= cut.aMin;
+        const bool contrib = live && alpha >= cut.aMin;
         if (__ballot(contrib) == 0ull) continue;
         float v[10];
         {
@@ -368,10 +376,12 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
           // rounded 1/x for every float (checked exhaustively on gfx950).
           const float rom = rcp_exact_unit(1.0f - alpha);
           const float Tb = Ta * rom;
-          const float aT = alpha * Tb;
           const float u[3] = {col[0] * Tb - S0 * rom, col[1] * Tb - S1 * rom, col[2] * Tb - S2 * rom};
-          float galpha = dot3(gC, u);
-          if (araw > 0.99f) galpha = 0.0f;
+          // Every term is a product with galpha or aT; zeroing those two for
+          // the lanes that do not take the splat zeroes all ten terms (all
+          // other factors are finite), and leaves the taking lanes' bits as is.
+          const float galpha = (contrib && !(araw > 0.99f)) ? dot3(gC, u) : 0.0f;
+          const float aT = contrib ? alpha * Tb : 0.0f;
           const float ga = galpha * araw;
           v[0] = ga * (cA * dx + cB * dy);
           v[1] = ga * (cB * dx + cC * dy);
@@ -396,13 +406,11 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
 #pragma unroll
           for (int q = 0; q < 10; q++) {
             const float cl = fminf(fmaxf(v[q] * kFixed, -1.0e9f), 1.0e9f);
-            iv[q] = contrib ? (uint32_t)(int32_t)cl : 0u;
+            iv[q] = (uint32_t)(int32_t)cl;
           }
           const uint32_t r = wave_reduce10<uint32_t>(iv);
           if (flusher && r != 0u) atomicAdd((uint32_t*)accum + (int64_t)gid * 12 + mySlot, r);
         } else {
-#pragma unroll
-          for (int q = 0; q < 10; q++) v[q] = contrib ? v[q] : 0.0f;
           const float r = wave_reduce10<float>(v);
           if (flusher && r != 0.0f) unsafeAtomicAdd((float*)accum + (int64_t)gid * 12 + mySlot, r);
         }

@@ -1,0 +1,95 @@
+"""The rasterizer's wave-skip test (splat.js_amd/csrc/gs_cover.h) against
+brute force: for random splats — conics built the way the projection builds
+them (covariance + 0.3 px dilation, inverted in f32), with many needle-like,
+diagonal and huge ones — and 8x8 pixel blocks placed around the ellipse
+boundary, every pixel centre's exponent is evaluated with the kernels' exact
+f32 operations. A block the test rejects must hold no pixel with e <= L
+(else skipping it would change an image); the test should also reject most
+blocks that indeed hold none (else it saves nothing)."""
+import os
+import subprocess
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+PROBE = r"""
+#include <stdio.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <math.h>
+#include "gs_cover.h"
+
+static uint64_t s = 0x9E3779B97F4A7C15ull;
+static double urand(void) {            /* xorshift64*, [0,1) */
+  s ^= s >> 12; s ^= s << 25; s ^= s >> 27;
+  return (double)((s * 2685821657736338717ull) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+/* the raster kernels' per-pixel exponent, operation for operation */
+static float pixel_e(float px, float py, float mx, float my, float A, float B, float C) {
+  const float dx = px - mx, dy = py - my;
+  const float t2 = fmaf(C * dy, dy, (A * dx) * dx);
+  return fmaxf(fmaf(0.5f, t2, (B * dx) * dy), 0.0f);
+}
+
+int main(int argc, char** argv) {
+  const long cases = atol(argv[1]);
+  long blocks = 0, unsafe = 0, takes = 0, empty = 0, empty_rejected = 0;
+  for (long it = 0; it < cases; it++) {
+    /* covariance: log-uniform diagonal, correlation up to 1 - 1e-8 */
+    const float va = (float)pow(10.0, -4.0 + 10.0 * urand());
+    const float vc = (float)pow(10.0, -4.0 + 10.0 * urand());
+    const double rho = (urand() < 0.5 ? -1.0 : 1.0) * (1.0 - pow(10.0, -8.0 * urand()));
+    const float vb = (float)(rho * sqrt((double)va * (double)vc) * (urand() < 0.2 ? 0.0 : 1.0));
+    const float ad = va + 0.3f, cd = vc + 0.3f;
+    const float detVd = ad * cd - vb * vb;
+    if (detVd < 1e-8f) continue;
+    const float inv = 1.0f / detVd;
+    const float A = cd * inv, B = (-vb) * inv, C = ad * inv;
+    const float L = (float)pow(10.0, -4.0 + 6.0 * urand());       /* 1e-4 .. 100 */
+    const float mx = (float)(-200.0 + 2400.0 * urand()), my = (float)(-200.0 + 1400.0 * urand());
+    /* blocks near the ellipse: offsets up to ~1.3x its extent along x and y */
+    const double exx = sqrt(2.0 * L * (double)ad) * 1.3 + 12.0, eyy = sqrt(2.0 * L * (double)cd) * 1.3 + 12.0;
+    for (int b = 0; b < 8; b++) {
+      const float bx = (float)floor(mx + (2.0 * urand() - 1.0) * exx) + 0.5f;
+      const float by = (float)floor(my + (2.0 * urand() - 1.0) * eyy) + 0.5f;
+      float emin = INFINITY;
+      for (int j = 0; j < 8; j++)
+        for (int i = 0; i < 8; i++)
+          emin = fminf(emin, pixel_e(bx + (float)i, by + (float)j, mx, my, A, B, C));
+      /* every other block is borderline: L set to its smallest pixel exponent */
+      const float Lb = (b & 1) ? L : emin;
+      if (!(Lb > 0.0f) || Lb == INFINITY) continue;
+      const int any = emin <= Lb;
+      const int hit = gs_cover_rect(A, B, C, Lb, bx - mx, (bx + 7.0f) - mx, by - my, (by + 7.0f) - my);
+      blocks++;
+      if (any) { takes++; if (!hit) unsafe++; }
+      else { empty++; if (!hit) empty_rejected++; }
+    }
+  }
+  printf("%ld %ld %ld %ld %ld\n", blocks, unsafe, takes, empty, empty_rejected);
+  return 0;
+}
+"""
+
+
+def _run(tmp_path, cases, defines=()):
+    src = tmp_path / "cover.c"
+    src.write_text(PROBE)
+    exe = tmp_path / ("cover" + "".join(d.replace("=", "_") for d in defines))
+    subprocess.check_call(["gcc", "-O2", "-ffp-contract=off", *[f"-D{d}" for d in defines],
+                           "-I", os.path.join(ROOT, "splat.js_amd", "csrc"), "-o", str(exe), str(src), "-lm"])
+    return map(int, subprocess.check_output([str(exe), str(cases)], text=True).split())
+
+
+def test_cover_rect_never_rejects_a_taking_block(tmp_path):
+    blocks, unsafe, takes, empty, empty_rejected = _run(tmp_path, 400000)
+    assert blocks > 2_000_000 and takes > 1_000_000 and empty > 200_000
+    assert unsafe == 0, f"{unsafe} blocks with a taking pixel were rejected"
+    # tight: rejects nearly every block that no pixel centre takes
+    assert empty_rejected >= 0.90 * empty, (empty_rejected, empty)
+
+
+def test_cover_probe_catches_a_missing_rounding_margin(tmp_path):
+    """Without the margin, needle-like splats lose pixels: the probe sees it."""
+    blocks, unsafe, takes, empty, empty_rejected = _run(tmp_path, 400000, ["GS_COVER_MARGIN=0.0f"])
+    assert unsafe > 0
