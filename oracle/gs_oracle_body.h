@@ -328,8 +328,12 @@ void FN(orc_render_bwd)(const orc_opts* o, const orc_cam* cam, const REAL* proj,
       const uint32_t segS = tile_start[tile];
       const REAL px = (REAL)x + RL(0.5), py = (REAL)y + RL(0.5);
       const REAL gC[3] = {gCin[3 * pi], gCin[3 * pi + 1], gCin[3 * pi + 2]};
-      REAL S[3], Ta = Tfin[pi];
-      for (int c = 0; c < 3; c++) S[c] = RL(cam->bg[c]) * Ta;
+      /* The colour-behind sum S (shaders.js:627, :686) enters only through
+       * dot(gC, S), so that dot product GS is carried instead of S:
+       * GS starts at dot(gC, bg) * T_final and grows by dot(gC, c) * alpha * Tb. */
+      REAL Ta = Tfin[pi];
+      const REAL gbg = FMA(gC[2], RL(cam->bg[2]), FMA(gC[1], RL(cam->bg[1]), gC[0] * RL(cam->bg[0])));
+      REAL GS = gbg * Ta;
       for (uint32_t kk = endIn[pi]; kk > segS; kk--) {
         const uint32_t gid = ids[kk - 1];
         const REAL* pr = proj + 16 * (int64_t)gid;
@@ -345,13 +349,13 @@ void FN(orc_render_bwd)(const orc_opts* o, const orc_cam* cam, const REAL* proj,
         if (!(alpha >= A_MIN)) continue;
         const REAL c[3] = {pr[8], pr[9], pr[10]};
         /* Tb = Ta / (1 - alpha) and S / (1 - alpha) (shaders.js:656, :660),
-         * both through one reciprocal of (1 - alpha) */
+         * both through one reciprocal of (1 - alpha);
+         * galpha = dot(gC, c Tb - S/(1-alpha)) = dot(gC, c) Tb - GS/(1-alpha) */
         const REAL rom = RL(1.0) / (RL(1.0) - alpha);
         const REAL Tb = Ta * rom;
         const REAL aT = alpha * Tb;
-        REAL u[3];
-        for (int ch = 0; ch < 3; ch++) u[ch] = c[ch] * Tb - S[ch] * rom;
-        REAL galpha = FN(dot3)(gC, u);
+        const REAL gcol = FMA(gC[2], c[2], FMA(gC[1], c[1], gC[0] * c[0]));
+        REAL galpha = FMA(gcol, Tb, -(GS * rom));
         if (araw > RL(0.99)) galpha = 0;
         const REAL ga = galpha * araw;
         /* cnorm: shaders.js:674-678 */
@@ -362,9 +366,14 @@ void FN(orc_render_bwd)(const orc_opts* o, const orc_cam* cam, const REAL* proj,
         REAL v[10];
         v[0] = ga * (cA * dx + cB * dy);
         v[1] = ga * (cB * dx + cC * dy);
-        v[2] = (-ga) * RL(0.5) * dx * dx * cnorm;
-        v[3] = (-ga) * dx * dy * cnorm;
-        v[4] = (-ga) * RL(0.5) * dy * dy * cnorm;
+        /* -ga * 0.5 * d.x * d.x * cnorm etc. (shaders.js:679-681) through the
+         * shared factor gq = ga * (-0.5 cnorm); the middle term is 2 gq dx dy
+         * (the doubling is exact) */
+        const REAL gq = ga * (RL(-0.5) * cnorm);
+        const REAL gqdx = gq * dx;
+        v[2] = gqdx * dx;
+        v[3] = RL(2.0) * (gqdx * dy);
+        v[4] = (gq * dy) * dy;
         v[5] = galpha * opa * G;
         v[6] = ga * (RL(1.0) - opa);
         v[7] = gC[0] * aT; v[8] = gC[1] * aT; v[9] = gC[2] * aT;
@@ -379,7 +388,7 @@ void FN(orc_render_bwd)(const orc_opts* o, const orc_cam* cam, const REAL* proj,
         } else {
           for (int s = 0; s < 10; s++) gsum[10 * (int64_t)gid + s] += (double)v[s];
         }
-        for (int ch = 0; ch < 3; ch++) S[ch] = S[ch] + c[ch] * alpha * Tb;
+        GS = FMA(gcol, aT, GS);
         Ta = Tb;
       }
     }

@@ -57,6 +57,18 @@ GS_HD int gs_cover_rect(float A, float B, float C, float L, float u0, float u1, 
   return !(q > L + GS_COVER_MARGIN * S);
 }
 
+// Bit w (w = 0..3: column w & 1, row w >> 1 of 8x8 blocks) set when some
+// pixel centre of that block of 16x16 tile (tx, ty) may take the splat.
+GS_HD uint32_t gs_cover_tile_mask(float mx, float my, float A, float B, float C, float L, int tx, int ty) {
+  const float x0 = (float)(tx * 16) + 0.5f, y0 = (float)(ty * 16) + 0.5f;
+  uint32_t m = 0;
+  for (int w = 0; w < 4; w++) {
+    const float bx = x0 + (float)((w & 1) * 8), by = y0 + (float)((w >> 1) * 8);
+    if (gs_cover_rect(A, B, C, L, bx - mx, (bx + 7.0f) - mx, by - my, (by + 7.0f) - my)) m |= 1u << w;
+  }
+  return m;
+}
+
 // Cover level stored with each splat: an upper bound on the largest f32
 // exponent a pixel can take the splat at. eMax = min(eCut, ln(opa*comp /
 // A_MIN)) is computed in f32 (division, gs_logf: a few ulp), and alpha =

@@ -8,6 +8,11 @@
 
 namespace gs {
 
+// Sorted key values: Gaussian id in bits 0-27, the 4-bit mask of the tile's
+// 8x8 pixel blocks that may take the splat (gs_cover_tile_mask) in bits 28-31.
+constexpr int kMaskShift = 28;
+constexpr uint32_t kIdMask = (1u << kMaskShift) - 1u;
+
 struct SortScratch {
   int G;                  // blocks per radix pass (fixed grid)
   uint32_t* hist;         // 256 x G
@@ -28,6 +33,7 @@ struct BinArgs {
   const uint32_t* cnt;
   const uint2* rect;
   const uint32_t* depthKey;
+  const float4* rec;      // raster records: the emit pass derives each key's block mask
   uint32_t *dkA, *dvA, *dkB, *dvB;   // n-sized depth-sort buffers
   void *tkA, *tkB;                    // key buffers (4 or 8 B per key)
   uint32_t *tvA, *tvB;                // id buffers

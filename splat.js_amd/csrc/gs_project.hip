@@ -4,7 +4,7 @@
 //
 // Output per visible Gaussian:
 //   rec[4i..4i+3]  raster record, 64 B: (mx, my, A, B), (C, opa*comp, L, 0),
-//                  (r, g, b, opa), (comp, cnorm, 0, 0). L (gs_cover.h) bounds
+//                  (r, g, b, opa), (comp, -0.5 cnorm, 1 - opa, 0). L (gs_cover.h) bounds
 //                  the exponent at which a pixel can still take the splat
 //                  (min(eCut, ln(opa*comp / A_MIN)) plus rounding slack); the
 //                  raster passes use it only to skip whole waves, never to
@@ -79,7 +79,7 @@ __global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam 
     rec[4 * i + 0] = make_float4(mx, my, cd * inv, (-g.vb) * inv);
     rec[4 * i + 1] = make_float4(ad * inv, opa * comp, gs_cover_level(eMax), 0.0f);
     rec[4 * i + 2] = make_float4(col[0], col[1], col[2], opa);
-    rec[4 * i + 3] = make_float4(comp, cnorm, 0.0f, 0.0f);
+    rec[4 * i + 3] = make_float4(comp, -0.5f * cnorm, 1.0f - opa, 0.0f);
     key = gs_f2u(g.pc[2]);
     const float TF = 16.0f;
     const int tx0 = (int)clampf(floorf((mx - rad) / TF), 0.0f, (float)(cam.tilesX - 1));

@@ -4,9 +4,11 @@
 // One 256-thread workgroup (4 waves) per 16x16 tile, one thread per pixel,
 // each wave an 8x8 block. The tile's sorted keys are consumed in batches of
 // 256: every thread loads one raster record into LDS, then every pixel walks
-// the batch from LDS (broadcast reads). A wave skips, as a whole, every splat
-// that no pixel centre of its 8x8 block can take (gs_cover.h), and leaves a
-// splat early once no lane is inside the cutoff or takes it. The forward stops the whole tile once
+// the batch from LDS (broadcast reads). Each key carries the mask of the
+// tile's 8x8 blocks whose pixels may take the splat (gs_cover.h, set by the
+// binning pass): a wave visits only the splats of its bit, a splat with an
+// empty mask is not even loaded, and a wave leaves a splat early once no
+// lane is inside the cutoff or takes it. The forward stops the whole tile once
 // every pixel has reached T < 1e-4.
 //
 // Forward (shaders.js:556-585) writes the image, the final transmittance T
@@ -44,25 +46,6 @@ __device__ __forceinline__ PixelMap pixel_of_thread(const Cam& cam, int tile, in
   PixelMap m;
   m.x = tx * kTile + (w & 1) * 8 + (l & 7);
   m.y = ty * kTile + (w >> 1) * 8 + (l >> 3);
-  return m;
-}
-
-// Bit w set when some pixel centre of wave w's 8x8 block may take the splat
-// (gs_cover_rect: exact minimum of the exponent over the block, with a
-// rounding margin, against the splat's cover level L). A wave only visits
-// the splats of its bits; a cleared bit means no pixel of the block could
-// have taken the splat, so skipping it changes nothing.
-__device__ __forceinline__ uint32_t wave_cover(const Cam& cam, int tile, float4 a, float4 bb) {
-  const int tx = tile % cam.tilesX, ty = tile / cam.tilesX;
-  const float x0 = (float)(tx * kTile) + 0.5f, y0 = (float)(ty * kTile) + 0.5f;
-  const float mx = a.x, my = a.y, A = a.z, B = a.w, C = bb.x, L = bb.z;
-  uint32_t m = 0;
-#pragma unroll
-  for (int w = 0; w < 4; w++) {
-    const float bx = x0 + (float)((w & 1) * 8), by = y0 + (float)((w >> 1) * 8);
-    const int hit = gs_cover_rect(A, B, C, L, bx - mx, (bx + 7.0f) - mx, by - my, (by + 7.0f) - my);
-    m |= hit ? (1u << w) : 0u;
-  }
   return m;
 }
 
@@ -115,13 +98,14 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
     const uint32_t k = base + tid;
     uint32_t cover = 0;
     if (k < e) {
-      const uint32_t id = b.ids[k];
-      const float4 a = b.rec[4 * id + 0];
-      const float4 bb = b.rec[4 * id + 1];
-      sA[tid] = a;
-      sB[tid] = bb;
-      sC[tid] = b.rec[4 * id + 2];
-      cover = wave_cover(cam, tile, a, bb);
+      const uint32_t v = b.ids[k];
+      cover = v >> kMaskShift;
+      if (cover != 0u) {           // splats no block of the tile can take stay unloaded
+        const uint32_t id = v & kIdMask;
+        sA[tid] = b.rec[4 * id + 0];
+        sB[tid] = b.rec[4 * id + 1];
+        sC[tid] = b.rec[4 * id + 2];
+      }
     }
     build_masks(cover, sMask);
     __syncthreads();
@@ -287,7 +271,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
   __shared__ float4 sA[kBlock];   // mx, my, A, B
   __shared__ float4 sB[kBlock];   // C, opa*comp, L, 0
   __shared__ float4 sC[kBlock];   // r, g, b, opa
-  __shared__ float2 sD[kBlock];   // comp, cnorm
+  __shared__ float2 sD[kBlock];   // -0.5 cnorm, 1 - opa
   __shared__ uint32_t sId[kBlock];
   __shared__ uint64_t sMask[4][4];
   __shared__ uint32_t sEnd;
@@ -318,24 +302,26 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
   if (myEnd > s) atomicMax(&sEnd, myEnd);
   __syncthreads();
   const uint32_t endMax = sEnd;
-  float S0 = cam.bg[0] * Ta, S1 = cam.bg[1] * Ta, S2 = cam.bg[2] * Ta;
   const float gC[3] = {g.x, g.y, g.z};
+  // dot(gC, S) of the colour-behind sum S, which enters only through it
+  float GS = fmaf(gC[2], cam.bg[2], fmaf(gC[1], cam.bg[1], gC[0] * cam.bg[0])) * Ta;
   for (uint32_t hi = endMax; hi > s; ) {
     const uint32_t lo = hi - s > (uint32_t)kBlock ? hi - kBlock : s;
     const int n = (int)(hi - lo);
     __syncthreads();
     uint32_t cover = 0;
     if (tid < n) {
-      const uint32_t id = b.ids[lo + tid];
-      sId[tid] = id;
-      const float4 a = b.rec[4 * id + 0];
-      const float4 bb = b.rec[4 * id + 1];
-      sA[tid] = a;
-      sB[tid] = bb;
-      sC[tid] = b.rec[4 * id + 2];
-      const float4 d = b.rec[4 * id + 3];
-      sD[tid] = make_float2(d.x, d.y);
-      cover = wave_cover(cam, tile, a, bb);
+      const uint32_t v = b.ids[lo + tid];
+      cover = v >> kMaskShift;
+      if (cover != 0u) {
+        const uint32_t id = v & kIdMask;
+        sId[tid] = id;
+        sA[tid] = b.rec[4 * id + 0];
+        sB[tid] = b.rec[4 * id + 1];
+        sC[tid] = b.rec[4 * id + 2];
+        const float4 d = b.rec[4 * id + 3];
+        sD[tid] = make_float2(d.y, d.z);
+      }
     }
     build_masks(cover, sMask);
     __syncthreads();
@@ -367,50 +353,48 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
         float v[10];
         {
           const float4 c4 = sC[j];
-          const float2 d2 = sD[j];
-          const float col[3] = {c4.x, c4.y, c4.z};
-          const float opa = c4.w, cnorm = d2.y;
+          const float2 d2 = sD[j];          // -0.5 cnorm, 1 - opa
           // Tb = Ta / (1 - alpha), S / (1 - alpha) through one reciprocal.
           // 1 - alpha lies in [0.0099, 1] (alpha in [0, 0.99]); on [2^-9, 1]
           // v_rcp_f32 refined by one FMA Newton step equals the correctly
           // rounded 1/x for every float (checked exhaustively on gfx950).
           const float rom = rcp_exact_unit(1.0f - alpha);
           const float Tb = Ta * rom;
-          const float u[3] = {col[0] * Tb - S0 * rom, col[1] * Tb - S1 * rom, col[2] * Tb - S2 * rom};
+          const float gcol = fmaf(gC[2], c4.z, fmaf(gC[1], c4.y, gC[0] * c4.x));
           // Every term is a product with galpha or aT; zeroing those two for
           // the lanes that do not take the splat zeroes all ten terms (all
           // other factors are finite), and leaves the taking lanes' bits as is.
-          const float galpha = (contrib && !(araw > 0.99f)) ? dot3(gC, u) : 0.0f;
+          const float galpha = (contrib && !(araw > 0.99f)) ? fmaf(gcol, Tb, -(GS * rom)) : 0.0f;
           const float aT = contrib ? alpha * Tb : 0.0f;
           const float ga = galpha * araw;
+          const float gq = ga * d2.x;
+          const float gqdx = gq * dx;
           v[0] = ga * (cA * dx + cB * dy);
           v[1] = ga * (cB * dx + cC * dy);
-          v[2] = (-ga) * 0.5f * dx * dx * cnorm;
-          v[3] = (-ga) * dx * dy * cnorm;
-          v[4] = (-ga) * 0.5f * dy * dy * cnorm;
-          v[5] = galpha * opa * G;
-          v[6] = ga * (1.0f - opa);
+          v[2] = gqdx * dx;
+          v[3] = gqdx * dy;                 // half of the term: doubled in the fixed-point scale
+          v[4] = (gq * dy) * dy;
+          v[5] = galpha * c4.w * G;
+          v[6] = ga * d2.y;
           v[7] = gC[0] * aT;
           v[8] = gC[1] * aT;
           v[9] = gC[2] * aT;
-          if (contrib) {
-            S0 = S0 + col[0] * alpha * Tb;
-            S1 = S1 + col[1] * alpha * Tb;
-            S2 = S2 + col[2] * alpha * Tb;
-            Ta = Tb;
-          }
+          GS = fmaf(gcol, aT, GS);
+          Ta = contrib ? Tb : Ta;
         }
         const uint32_t gid = sId[j];
         if (kAccum == 0) {
           uint32_t iv[10];
 #pragma unroll
           for (int q = 0; q < 10; q++) {
-            const float cl = fminf(fmaxf(v[q] * kFixed, -1.0e9f), 1.0e9f);
+            // 2 * (v * 16384) == v * 32768 exactly (power-of-two scalings)
+            const float cl = fminf(fmaxf(v[q] * (q == 3 ? 2.0f * kFixed : kFixed), -1.0e9f), 1.0e9f);
             iv[q] = (uint32_t)(int32_t)cl;
           }
           const uint32_t r = wave_reduce10<uint32_t>(iv);
           if (flusher && r != 0u) atomicAdd((uint32_t*)accum + (int64_t)gid * 12 + mySlot, r);
         } else {
+          v[3] = 2.0f * v[3];
           const float r = wave_reduce10<float>(v);
           if (flusher && r != 0.0f) unsafeAtomicAdd((float*)accum + (int64_t)gid * 12 + mySlot, r);
         }

@@ -367,13 +367,16 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ or
                                                  const uint32_t* __restrict__ dkS,
                                                  const uint32_t* __restrict__ dN,
                                                  const uint64_t* __restrict__ partials, int tilesX,
-                                                 uint64_t cap, void* __restrict__ keysOut,
+                                                 uint64_t cap, const float4* __restrict__ rec,
+                                                 void* __restrict__ keysOut,
                                                  uint32_t* __restrict__ valsOut) {
   __shared__ uint32_t wsum[4];
   __shared__ uint32_t sOff[kBlock];
   __shared__ uint2 sRect[kBlock];
   __shared__ uint32_t sId[kBlock];
   __shared__ uint32_t sDk[kBlock];
+  __shared__ float4 sG0[kBlock];   // mx, my, A, B
+  __shared__ float2 sG1[kBlock];   // C, cover level L
   const int tid = threadIdx.x;
   const uint32_t N = *dN;
   const uint32_t r = blockIdx.x * kBlock + tid;
@@ -383,8 +386,14 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ or
   const uint32_t ex = block_excl_scan(c, tot, wsum);
   sOff[tid] = ex;
   sRect[tid] = ok ? rectS[r] : make_uint2(0, 0);
-  sId[tid] = ok ? (kGlobal64 ? r : order[r]) : 0u;
+  const uint32_t gidr = ok ? (kGlobal64 ? r : order[r]) : 0u;
+  sId[tid] = gidr;
   if (kGlobal64) sDk[tid] = ok ? dkS[r] : 0u;
+  if (c != 0u) {
+    sG0[tid] = rec[4 * (size_t)gidr + 0];
+    const float4 g1 = rec[4 * (size_t)gidr + 1];
+    sG1[tid] = make_float2(g1.x, g1.z);
+  }
   __syncthreads();
   const uint64_t run = partials[blockIdx.x];
   for (uint32_t q = tid; q < tot; q += kBlock) {
@@ -399,10 +408,14 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ or
       const uint32_t wdt = tx1 - tx0 + 1u;
       const uint32_t li = q - sOff[g];
       const uint32_t dy = li / wdt;
-      const uint32_t tile = (ty0 + dy) * (uint32_t)tilesX + tx0 + (li - dy * wdt);
+      const uint32_t tx = tx0 + (li - dy * wdt), ty = ty0 + dy;
+      const uint32_t tile = ty * (uint32_t)tilesX + tx;
       if (kGlobal64) ((uint64_t*)keysOut)[pos] = ((uint64_t)tile << 32) | sDk[g];
       else ((uint32_t*)keysOut)[pos] = tile;
-      valsOut[pos] = sId[g];
+      const float4 g0 = sG0[g];
+      const float2 g1 = sG1[g];
+      const uint32_t mask = gs_cover_tile_mask(g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, (int)tx, (int)ty);
+      valsOut[pos] = sId[g] | (mask << kMaskShift);
     }
   }
 }
@@ -494,7 +507,7 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
                                             s.partials, s.vpartials);
     k_count_top<<<1, 1024, 0, st>>>(s.partials, s.vpartials, Gc, a.cap, a.numKeys, a.totalKeys, a.numVisible, a.stats);
     k_emit<false><<<Gc, kBlock, 0, st>>>(order, s.cntS, s.rectS, nullptr, a.dN, s.partials, a.tilesX, a.cap,
-                                         a.tkA, a.tvA);
+                                         a.rec, a.tkA, a.tvA);
     // 3) stable sort by tile id
     const uint32_t* tk;
     const uint32_t* tv;
@@ -510,7 +523,7 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
                                             s.partials, s.vpartials);
     k_count_top<<<1, 1024, 0, st>>>(s.partials, s.vpartials, Gc, a.cap, a.numKeys, a.totalKeys, a.numVisible, a.stats);
     k_emit<true><<<Gc, kBlock, 0, st>>>(nullptr, s.cntS, s.rectS, s.dkS, a.dN, s.partials, a.tilesX, a.cap,
-                                        a.tkA, a.tvA);
+                                        a.rec, a.tkA, a.tvA);
     const uint64_t* tk;
     const uint32_t* tv;
     radix_sort<uint64_t>(st, s, (const uint64_t*)a.tkA, a.tvA, (uint64_t*)a.tkB, a.tvB,

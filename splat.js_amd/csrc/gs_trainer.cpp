@@ -287,6 +287,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
     a.cnt = h->cnt;
     a.rect = h->rect;
     a.depthKey = h->depthKey;
+    a.rec = h->rec;
     a.dkA = h->dkA; a.dvA = h->dvA; a.dkB = h->dkB; a.dvB = h->dvB;
     a.tkA = h->tkA; a.tkB = h->tkB; a.tvA = h->tvA; a.tvB = h->tvB;
     a.numKeys = h->numKeys;
@@ -377,6 +378,7 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
     if (dbg->ids || dbg->keys) {
       std::vector<uint32_t> ids(nk ? nk : 1);
       if (nk) HIPCK(hipMemcpy(ids.data(), h->lastBin.ids, 4ull * nk, hipMemcpyDeviceToHost));
+      for (auto& v : ids) v &= kIdMask;   // block masks ride in the top bits
       if (dbg->ids) memcpy(dbg->ids, ids.data(), 4ull * nk);
       if (dbg->keys && nk) {
         if (h->lastBin.keyBytes == 8) {
@@ -553,7 +555,7 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   // trainer.js:105-107; capacity never below n
   const int64_t maxSplats = h->o.maxSplats > 0 ? h->o.maxSplats : 600000;
   h->cap = std::max<int64_t>(n, std::min<int64_t>(std::max<int64_t>((int64_t)std::floor(n * (double)h->o.capMult), n), maxSplats));
-  if (h->cap >= (int64_t)1 << 31) return fail(h, GS_E_LIMIT, "setup: more than 2^31 Gaussians");
+  if (h->cap > (int64_t)kIdMask + 1) return fail(h, GS_E_LIMIT, "setup: more than 2^28 Gaussians");
   // trainer.js:48-49
   h->entriesCap = h->o.entriesCap > 0 ? h->o.entriesCap : std::max<int64_t>(kDefaultEntries, 24 * maxSplats);
   h->entriesCap = std::min<int64_t>(h->entriesCap, 0xffffffffll);

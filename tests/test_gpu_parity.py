@@ -34,10 +34,10 @@ def _trainer(params, sh, cams, images, W, H, radius, opts=None, shDeg=2, **kw):
 
 def _expected_rec(proj):
     """Raster record the kernels store (16 floats), from the restatement's
-    proj rows, with the same f32 arithmetic (opa*comp, 1/(1+lambda_max)).
-    Slots 6-7 (ellipse extents) only decide which waves skip a splat; a
-    wrong extent would show up as image and gradient mismatches, so they
-    are not compared here."""
+    proj rows, with the same f32 arithmetic (opa*comp, -0.5/(1+lambda_max),
+    1-opa). Slot 6 (the cover level) only decides which waves skip a splat;
+    a wrong level would show up as image and gradient mismatches, so it is
+    not compared here."""
     p = proj.astype(np.float32)
     va, vb, vc = p[:, 12], p[:, 13], p[:, 14]
     cva = va + np.float32(0.3)
@@ -47,10 +47,11 @@ def _expected_rec(proj):
     cnorm = np.float32(1) / (np.float32(1) + lmax)
     z = np.zeros_like(cnorm)
     return np.stack([p[:, 0], p[:, 1], p[:, 3], p[:, 4], p[:, 5], p[:, 7] * p[:, 6], z, z,
-                     p[:, 8], p[:, 9], p[:, 10], p[:, 7], p[:, 6], cnorm, z, z], 1)
+                     p[:, 8], p[:, 9], p[:, 10], p[:, 7], p[:, 6], np.float32(-0.5) * cnorm,
+                     np.float32(1) - p[:, 7], z], 1)
 
 
-REC_CMP = [0, 1, 2, 3, 4, 5, 8, 9, 10, 11, 12, 13]
+REC_CMP = [0, 1, 2, 3, 4, 5, 7, 8, 9, 10, 11, 12, 13, 14, 15]
 
 
 def _check_step(gpu, ref, n_tiles, label, stats_acc=None):
