@@ -33,7 +33,7 @@ enum {
   GS_E_ARG = -1,       /* bad argument or call order */
   GS_E_HIP = -2,       /* HIP runtime error (device lost, launch failure) */
   GS_E_NOMEM = -3,     /* device allocation failed */
-  GS_E_LIMIT = -4,     /* a size limit was exceeded (targets, view size) */
+  GS_E_LIMIT = -4,     /* a size limit was exceeded (targets, view size, more than 2^28 Gaussians) */
 };
 
 enum { GS_LOSS_CHARBONNIER = 0, GS_LOSS_L1_DSSIM = 1 };

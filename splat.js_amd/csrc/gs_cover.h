@@ -57,15 +57,57 @@ GS_HD int gs_cover_rect(float A, float B, float C, float L, float u0, float u1, 
   return !(q > L + GS_COVER_MARGIN * S);
 }
 
+// Per-splat constants of the tile test (gs_cover_tile_mask).
+typedef struct {
+  float mx, my, hA, B, hC, L;   // hA = A/2, hC = C/2
+  float kA, kC;                 // -B/A, -B/C: edge-vertex slopes
+  int pd;                       // conic positive definite
+} gs_cover_splat;
+
+GS_HD gs_cover_splat gs_cover_prepare(float mx, float my, float A, float B, float C, float L) {
+  gs_cover_splat s;
+  s.mx = mx; s.my = my; s.hA = 0.5f * A; s.B = B; s.hC = 0.5f * C; s.L = L;
+  s.pd = A > 0.0f && C > 0.0f;
+  s.kA = s.pd ? -B / A : 0.0f;
+  s.kC = s.pd ? -B / C : 0.0f;
+  return s;
+}
+
 // Bit w (w = 0..3: column w & 1, row w >> 1 of 8x8 blocks) set when some
 // pixel centre of that block of 16x16 tile (tx, ty) may take the splat.
-GS_HD uint32_t gs_cover_tile_mask(float mx, float my, float A, float B, float C, float L, int tx, int ty) {
+// The same test as gs_cover_rect on each block, arranged so the four blocks
+// share their edge lines: u[0..3] = x offsets of pixel-centre columns 0, 7,
+// 8, 15, w[0..3] likewise for rows; block (i, j) spans u[2i..2i+1] x
+// w[2j..2j+1]. A vertical edge line u is evaluated at the clamped vertex
+// kC*u of each block row, q = hA u^2 + wc (hC wc + B u), and symmetrically
+// for horizontal lines; the f32 error of this form is also a few ulp of S.
+GS_HD uint32_t gs_cover_tile_mask(const gs_cover_splat* s, int tx, int ty) {
+  if (!s->pd) return 0xfu;
   const float x0 = (float)(tx * 16) + 0.5f, y0 = (float)(ty * 16) + 0.5f;
-  uint32_t m = 0;
-  for (int w = 0; w < 4; w++) {
-    const float bx = x0 + (float)((w & 1) * 8), by = y0 + (float)((w >> 1) * 8);
-    if (gs_cover_rect(A, B, C, L, bx - mx, (bx + 7.0f) - mx, by - my, (by + 7.0f) - my)) m |= 1u << w;
+  const float u[4] = {x0 - s->mx, (x0 + 7.0f) - s->mx, (x0 + 8.0f) - s->mx, (x0 + 15.0f) - s->mx};
+  const float w[4] = {y0 - s->my, (y0 + 7.0f) - s->my, (y0 + 8.0f) - s->my, (y0 + 15.0f) - s->my};
+  float qv[4][2], qh[4][2];   // [line][block row / column]
+  for (int a = 0; a < 4; a++) {
+    const float au = s->hA * u[a] * u[a], bu = s->B * u[a], tu = s->kC * u[a];
+    const float aw = s->hC * w[a] * w[a], bw = s->B * w[a], tw = s->kA * w[a];
+    for (int j = 0; j < 2; j++) {
+      const float wc = gs_cover_clamp(tu, w[2 * j], w[2 * j + 1]);
+      qv[a][j] = fmaf(wc, fmaf(s->hC, wc, bu), au);
+      const float uc = gs_cover_clamp(tw, u[2 * j], u[2 * j + 1]);
+      qh[a][j] = fmaf(uc, fmaf(s->hA, uc, bw), aw);
+    }
   }
+  const float aB = fabsf(s->B);
+  uint32_t m = 0;
+  for (int bj = 0; bj < 2; bj++)
+    for (int bi = 0; bi < 2; bi++) {
+      const float u0 = u[2 * bi], u1 = u[2 * bi + 1], w0 = w[2 * bj], w1 = w[2 * bj + 1];
+      const int inside = u0 <= 0.0f && u1 >= 0.0f && w0 <= 0.0f && w1 >= 0.0f;
+      const float q = fminf(fminf(qv[2 * bi][bj], qv[2 * bi + 1][bj]), fminf(qh[2 * bj][bi], qh[2 * bj + 1][bi]));
+      const float du = fmaxf(fabsf(u0), fabsf(u1)), dw = fmaxf(fabsf(w0), fabsf(w1));
+      const float S = fmaf(s->hA * du, du, fmaf(s->hC * dw, dw, aB * du * dw));
+      if (inside || !(q > s->L + GS_COVER_MARGIN * S)) m |= 1u << (2 * bj + bi);
+    }
   return m;
 }
 

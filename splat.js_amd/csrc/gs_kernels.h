@@ -41,6 +41,7 @@ struct BinArgs {
   uint32_t* numVisible;   // device: Gaussians with at least one key
   uint64_t* totalKeys;    // device: keys before the budget
   uint2* ranges;          // numTiles
+  uint32_t* tileOrder;    // numTiles: raster launch order, most keys first
   double* stats;          // [3] += dropped keys
   SortScratch scratch;
 };
@@ -48,11 +49,13 @@ struct BinArgs {
 struct BinOut {
   const void* keys;
   const uint32_t* ids;
+  const uint32_t* tileOrder;
   int keyBytes;
 };
 
 struct RasterBufs {
   const uint2* ranges;
+  const uint32_t* tileOrder;   // block b rasterizes tile tileOrder[b]
   const uint32_t* ids;
   const float4* rec;
   float4* img;

@@ -33,7 +33,7 @@
 
 namespace gs {
 
-__device__ __forceinline__ int tile_of_block(const Cam& cam) { return (int)blockIdx.x; }
+__device__ __forceinline__ int tile_of_block(const RasterBufs& b) { return (int)b.tileOrder[blockIdx.x]; }
 
 // Each wave owns an 8x8 pixel block of the tile (wave w: columns 8*(w&1),
 // rows 8*(w>>1)); lane l is pixel (l & 7, l >> 3) of that block.
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
   __shared__ uint64_t sMask[4][4];
   __shared__ double sStat[3][kBlock / 64];
   const int tid = threadIdx.x, wv = tid >> 6;
-  const int tile = tile_of_block(cam);
+  const int tile = tile_of_block(b);
   const PixelMap pm = pixel_of_thread(cam, tile, tid);
   const int x = pm.x, y = pm.y;
   const bool inside = x < cam.W && y < cam.H;
@@ -276,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
   __shared__ uint64_t sMask[4][4];
   __shared__ uint32_t sEnd;
   const int tid = threadIdx.x, wv = tid >> 6;
-  const int tile = tile_of_block(cam);
+  const int tile = tile_of_block(b);
   const PixelMap pm = pixel_of_thread(cam, tile, tid);
   const int x = pm.x, y = pm.y;
   const bool inside = x < cam.W && y < cam.H;

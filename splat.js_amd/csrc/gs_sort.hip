@@ -13,19 +13,21 @@
 //
 // Every radix pass (8-bit digit) is reduce-then-scan, with no inter-block
 // waiting, over a fixed grid of G blocks, each owning a contiguous chunk
-// processed in rounds of 4096 keys (16 per thread, wave-striped so every
+// processed in rounds of 2048 keys (8 per thread, wave-striped so every
 // load is coalesced):
 //   hist    per-wave digit counts in LDS; each wave groups its 64 keys by
-//           digit with 8 ballots, and one lane per group adds the group size
-//   scan    digit base (exclusive scan of the digit totals) plus the
-//           exclusive scan of each digit's per-block counts (table is
-//           block-major, so every access is a contiguous row)
-//   scatter wave-local stable ranks from the same ballot groups, one
+//           digit with 8 ballots, and one lane per group adds the group size;
+//           the block's 256 counts go to a digit-major table hist[d][block]
+//   scan    one block per digit: exclusive scan of the digit's row (a
+//           contiguous row) and the digit total
+//   scatter digit bases from an in-block scan of the 256 totals;
+//           wave-local stable ranks from the same ballot groups, one
 //           block-wide digit prefix per round, keys staged in LDS in
 //           (digit, position) order, then written out in digit runs by
 //           consecutive threads
 // Key counts that only exist on the device (the emitted key total) are read
-// from device memory by every kernel; the grids are fixed.
+// from device memory by every kernel; the grids are fixed. No pass uses
+// global atomics.
 #include "gs_kernels.h"
 
 namespace gs {
@@ -75,8 +77,7 @@ __device__ __forceinline__ uint32_t digit_of(KeyT k, int shift) {
 template <typename KeyT>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const KeyT* __restrict__ keys,
                                                        const uint32_t* __restrict__ dN, int shift,
-                                                       int G, uint32_t* __restrict__ hist,
-                                                       uint32_t* __restrict__ totals) {
+                                                       int G, uint32_t* __restrict__ hist) {
   __shared__ uint32_t wc[4][256];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
   wc[0][tid] = wc[1][tid] = wc[2][tid] = wc[3][tid] = 0;
@@ -102,26 +103,16 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const KeyT* __restrict__ 
     }
   }
   __syncthreads();
-  const uint32_t c = wc[0][tid] + wc[1][tid] + wc[2][tid] + wc[3][tid];
-  hist[(int64_t)tid * G + blockIdx.x] = c;
-  if (c) atomicAdd(&totals[tid], c);
+  hist[(int64_t)tid * G + blockIdx.x] = wc[0][tid] + wc[1][tid] + wc[2][tid] + wc[3][tid];
 }
 
-// one block per digit: exclusive scan of hist[d][0..G) plus the digit's base
+// one block per digit: exclusive scan of hist[d][0..G) in place, and the
+// digit's total into totals[d] (the scatter adds the digit bases)
 __global__ __launch_bounds__(kBlock) void k_radix_scan(uint32_t* __restrict__ hist, int G,
-                                                       const uint32_t* __restrict__ totals) {
+                                                       uint32_t* __restrict__ totals) {
   __shared__ uint32_t wsum[kBlock / 64];
-  __shared__ uint32_t sbase;
   const int d = blockIdx.x, tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  {
-    uint32_t v = tid < d ? totals[tid] : 0u;
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    if (lane == 0) wsum[w] = v;
-    __syncthreads();
-    if (tid == 0) sbase = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    __syncthreads();
-  }
-  uint32_t carry = sbase;
+  uint32_t carry = 0u;
   uint32_t* row = hist + (int64_t)d * G;
   for (int b0 = 0; b0 < G; b0 += kBlock) {
     const int b = b0 + tid;
@@ -139,6 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scan(uint32_t* __restrict__ hi
     if (b < G) row[b] = carry + wpre + x - v;
     carry += wsum[0] + wsum[1] + wsum[2] + wsum[3];
   }
+  if (tid == 0) totals[d] = carry;
 }
 
 template <typename KeyT, bool kWriteKeys>
@@ -149,7 +141,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const KeyT* __restrict
                                                           const uint32_t* __restrict__ scan,
                                                           KeyT* __restrict__ keysOut,
                                                           uint32_t* __restrict__ valsOut,
-                                                          uint32_t* __restrict__ totalsNext) {
+                                                          const uint32_t* __restrict__ totals) {
   __shared__ uint32_t wc[4][256];
   __shared__ uint32_t wpre[4][256];
   __shared__ uint32_t gbase[256];
@@ -158,11 +150,22 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const KeyT* __restrict
   __shared__ KeyT sk[kRound];
   __shared__ uint32_t sv[kRound];
   const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
-  // the next pass's histogram accumulates into totalsNext: clear it here
-  if (blockIdx.x == 0 && totalsNext) totalsNext[tid] = 0u;
   uint32_t lo, hi;
   chunk_range(*dN, G, kRound, lo, hi);
-  gbase[tid] = scan[(int64_t)tid * G + blockIdx.x];
+  {
+    // digit bases: exclusive scan of the digit totals
+    const uint32_t t = totals[tid];
+    uint32_t x = t;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t wp = 0;
+    for (int k = 0; k < w; k++) wp += wsum[k];
+    gbase[tid] = wp + x - t + scan[(int64_t)tid * G + blockIdx.x];
+  }
   wc[0][tid] = wc[1][tid] = wc[2][tid] = wc[3][tid] = 0;
   __syncthreads();
   for (uint32_t r = lo; r < hi; r += kRound) {
@@ -375,8 +378,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ or
   __shared__ uint2 sRect[kBlock];
   __shared__ uint32_t sId[kBlock];
   __shared__ uint32_t sDk[kBlock];
-  __shared__ float4 sG0[kBlock];   // mx, my, A, B
-  __shared__ float2 sG1[kBlock];   // C, cover level L
+  __shared__ gs_cover_splat sG[kBlock];
   const int tid = threadIdx.x;
   const uint32_t N = *dN;
   const uint32_t r = blockIdx.x * kBlock + tid;
@@ -390,9 +392,9 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ or
   sId[tid] = gidr;
   if (kGlobal64) sDk[tid] = ok ? dkS[r] : 0u;
   if (c != 0u) {
-    sG0[tid] = rec[4 * (size_t)gidr + 0];
+    const float4 g0 = rec[4 * (size_t)gidr + 0];
     const float4 g1 = rec[4 * (size_t)gidr + 1];
-    sG1[tid] = make_float2(g1.x, g1.z);
+    sG[tid] = gs_cover_prepare(g0.x, g0.y, g0.z, g0.w, g1.x, g1.z);
   }
   __syncthreads();
   const uint64_t run = partials[blockIdx.x];
@@ -412,9 +414,8 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ or
       const uint32_t tile = ty * (uint32_t)tilesX + tx;
       if (kGlobal64) ((uint64_t*)keysOut)[pos] = ((uint64_t)tile << 32) | sDk[g];
       else ((uint32_t*)keysOut)[pos] = tile;
-      const float4 g0 = sG0[g];
-      const float2 g1 = sG1[g];
-      const uint32_t mask = gs_cover_tile_mask(g0.x, g0.y, g0.z, g0.w, g1.x, g1.y, (int)tx, (int)ty);
+      const gs_cover_splat sg = sG[g];
+      const uint32_t mask = gs_cover_tile_mask(&sg, (int)tx, (int)ty);
       valsOut[pos] = sId[g] | (mask << kMaskShift);
     }
   }
@@ -445,6 +446,38 @@ __global__ void k_tile_ranges(const KeyT* __restrict__ keys, const uint32_t* __r
   }
 }
 
+// Raster launch order: tiles by descending key count (a 256-bucket
+// pseudo-logarithmic counting sort; order within a bucket is arbitrary).
+// Launching the heaviest tiles first keeps the last partial wave of
+// workgroups short. The order changes no result: tiles are independent and
+// the fixed-point gradient sums are integer.
+__device__ __forceinline__ uint32_t tile_bucket(uint2 r) {
+  const uint32_t c = r.y - r.x;
+  if (c < 16u) return c;
+  const int e = 31 - (int)__builtin_clz(c);
+  return min(255u, (uint32_t)(e - 3) * 16u + ((c >> (e - 4)) & 15u));
+}
+
+__global__ __launch_bounds__(1024) void k_tile_order(const uint2* __restrict__ ranges, int numTiles,
+                                                     uint32_t* __restrict__ order) {
+  __shared__ uint32_t cnt[256];
+  const int tid = threadIdx.x;
+  if (tid < 256) cnt[tid] = 0u;
+  __syncthreads();
+  for (int t = tid; t < numTiles; t += 1024) atomicAdd(&cnt[tile_bucket(ranges[t])], 1u);
+  __syncthreads();
+  if (tid == 0) {   // exclusive offsets, highest bucket first
+    uint32_t run = 0;
+    for (int b = 255; b >= 0; b--) {
+      const uint32_t c = cnt[b];
+      cnt[b] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+  for (int t = tid; t < numTiles; t += 1024) order[atomicAdd(&cnt[tile_bucket(ranges[t])], 1u)] = (uint32_t)t;
+}
+
 // ---- host launchers ----
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, int64_t count, uint32_t v) {
@@ -452,14 +485,13 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, int64_t count, uint32_t v) {
   k_fill_u32<<<(unsigned)((count + 255) / 256), 256, 0, st>>>(p, count, v);
 }
 
-// one pass; the scatter clears `totals` for the following pass
+// one pass: histogram, per-digit scan (with digit totals), stable scatter
 template <typename KeyT>
 static void radix_pass(hipStream_t st, const SortScratch& s, const KeyT* kin, const uint32_t* vin,
-                       KeyT* kout, uint32_t* vout, const uint32_t* dN, int shift, bool lastPass) {
-  k_radix_hist<KeyT><<<s.G, kBlock, 0, st>>>(kin, dN, shift, s.G, s.hist, s.totals);
+                       KeyT* kout, uint32_t* vout, const uint32_t* dN, int shift) {
+  k_radix_hist<KeyT><<<s.G, kBlock, 0, st>>>(kin, dN, shift, s.G, s.hist);
   k_radix_scan<<<256, kBlock, 0, st>>>(s.hist, s.G, s.totals);
-  k_radix_scatter<KeyT, true><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout,
-                                                      lastPass ? nullptr : s.totals);
+  k_radix_scatter<KeyT, true><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout, s.totals);
 }
 
 // Stable LSD sort of (key, val) pairs by key bits [bit0, bit1), 8 per pass.
@@ -473,11 +505,10 @@ static void radix_sort(hipStream_t st, const SortScratch& s, const KeyT* kin, co
   const KeyT* ki = kin;
   const uint32_t* vi = vin;
   bool odd = false;
-  launch_fill_u32(st, s.totals, 256, 0u);
   for (int b = bit0; b < bit1; b += 8) {
     KeyT* ko = odd ? k2 : k1;
     uint32_t* vo = odd ? v2 : v1;
-    radix_pass<KeyT>(st, s, ki, vi, ko, vo, dN, b, b + 8 >= bit1);
+    radix_pass<KeyT>(st, s, ki, vi, ko, vo, dN, b);
     ki = ko;
     vi = vo;
     odd = !odd;
@@ -518,6 +549,7 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
     out.keyBytes = 4;
     launch_fill_u32(st, (uint32_t*)a.ranges, 2 * (int64_t)a.numTiles, 0u);
     k_tile_ranges<uint32_t><<<1024, 256, 0, st>>>(tk, a.numKeys, 0, a.numTiles, a.ranges);
+    k_tile_order<<<1, 1024, 0, st>>>(a.ranges, a.numTiles, a.tileOrder);
   } else {
     k_count_partials<<<Gc, kBlock, 0, st>>>(nullptr, a.cnt, a.rect, a.depthKey, a.dN, s.cntS, s.rectS, s.dkS,
                                             s.partials, s.vpartials);
@@ -533,7 +565,9 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
     out.keyBytes = 8;
     launch_fill_u32(st, (uint32_t*)a.ranges, 2 * (int64_t)a.numTiles, 0u);
     k_tile_ranges<uint64_t><<<1024, 256, 0, st>>>(tk, a.numKeys, 32, a.numTiles, a.ranges);
+    k_tile_order<<<1, 1024, 0, st>>>(a.ranges, a.numTiles, a.tileOrder);
   }
+  out.tileOrder = a.tileOrder;
 }
 
 }  // namespace gs

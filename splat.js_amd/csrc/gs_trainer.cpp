@@ -94,6 +94,7 @@ struct gs_trainer {
   uint32_t* numVisible = nullptr;
   uint64_t* totalKeys = nullptr;
   uint2* ranges = nullptr;
+  uint32_t* tileOrder = nullptr;
   float4* img = nullptr;
   float* Tfin = nullptr;
   uint32_t* endIdx = nullptr;
@@ -294,6 +295,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
     a.numVisible = h->numVisible;
     a.totalKeys = h->totalKeys;
     a.ranges = h->ranges;
+    a.tileOrder = h->tileOrder;
     a.stats = statsBuf;
     a.scratch = h->sort;
     Scope s(h, "bin_sort", 0.0);
@@ -301,6 +303,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
   }
   RasterBufs b{};
   b.ranges = h->ranges;
+  b.tileOrder = h->lastBin.tileOrder;
   b.ids = h->lastBin.ids;
   b.rec = h->rec;
   b.img = h->img;
@@ -398,6 +401,7 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
   }
   RasterBufs b{};
   b.ranges = h->ranges;
+  b.tileOrder = h->lastBin.tileOrder;
   b.ids = h->lastBin.ids;
   b.rec = h->rec;
   b.T = h->Tfin;
@@ -642,6 +646,7 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   rc |= dalloc(h, &h->numVisible, 4);
   rc |= dalloc(h, &h->totalKeys, 8);
   rc |= dalloc(h, &h->ranges, (size_t)h->maxTiles * 8);
+  rc |= dalloc(h, &h->tileOrder, (size_t)h->maxTiles * 4);
   rc |= dalloc(h, &h->img, (size_t)h->maxPix * 16);
   rc |= dalloc(h, &h->Tfin, (size_t)h->maxPix * 4);
   rc |= dalloc(h, &h->endIdx, (size_t)h->maxPix * 4);

@@ -66,7 +66,46 @@ int main(int argc, char** argv) {
       else { empty++; if (!hit) empty_rejected++; }
     }
   }
-  printf("%ld %ld %ld %ld %ld\n", blocks, unsafe, takes, empty, empty_rejected);
+  /* the tile form used by the binning pass: 16x16 tiles, 4 block bits */
+  long tblocks = 0, tunsafe = 0, ttakes = 0, tempty = 0, tempty_rejected = 0;
+  for (long it = 0; it < cases / 4; it++) {
+    const float va = (float)pow(10.0, -4.0 + 10.0 * urand());
+    const float vc = (float)pow(10.0, -4.0 + 10.0 * urand());
+    const double rho = (urand() < 0.5 ? -1.0 : 1.0) * (1.0 - pow(10.0, -8.0 * urand()));
+    const float vb = (float)(rho * sqrt((double)va * (double)vc) * (urand() < 0.2 ? 0.0 : 1.0));
+    const float ad = va + 0.3f, cd = vc + 0.3f;
+    const float detVd = ad * cd - vb * vb;
+    if (detVd < 1e-8f) continue;
+    const float inv = 1.0f / detVd;
+    const float A = cd * inv, B = (-vb) * inv, C = ad * inv;
+    const float L0 = (float)pow(10.0, -4.0 + 6.0 * urand());
+    const float mx = (float)(-200.0 + 2400.0 * urand()), my = (float)(-200.0 + 1400.0 * urand());
+    const double exx = sqrt(2.0 * L0 * (double)ad) * 1.3 + 24.0, eyy = sqrt(2.0 * L0 * (double)cd) * 1.3 + 24.0;
+    for (int b = 0; b < 4; b++) {
+      const int tx = (int)floor((mx + (2.0 * urand() - 1.0) * exx) / 16.0);
+      const int ty = (int)floor((my + (2.0 * urand() - 1.0) * eyy) / 16.0);
+      float emin[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
+      for (int j = 0; j < 16; j++)
+        for (int i = 0; i < 16; i++) {
+          const float e = pixel_e((float)(tx * 16 + i) + 0.5f, (float)(ty * 16 + j) + 0.5f, mx, my, A, B, C);
+          const int blk = (i >> 3) + 2 * (j >> 3);
+          emin[blk] = fminf(emin[blk], e);
+        }
+      /* every other tile is borderline: L = smallest exponent of one block */
+      const float L = (b & 1) ? L0 : emin[(int)(urand() * 4.0) & 3];
+      if (!(L > 0.0f) || L == INFINITY) continue;
+      const gs_cover_splat sp = gs_cover_prepare(mx, my, A, B, C, L);
+      const uint32_t m = gs_cover_tile_mask(&sp, tx, ty);
+      for (int blk = 0; blk < 4; blk++) {
+        tblocks++;
+        const int any = emin[blk] <= L, hit = (m >> blk) & 1u;
+        if (any) { ttakes++; if (!hit) tunsafe++; }
+        else { tempty++; if (!hit) tempty_rejected++; }
+      }
+    }
+  }
+  printf("%ld %ld %ld %ld %ld %ld %ld %ld %ld %ld\n", blocks, unsafe, takes, empty, empty_rejected,
+         tblocks, tunsafe, ttakes, tempty, tempty_rejected);
   return 0;
 }
 """
@@ -81,15 +120,17 @@ def _run(tmp_path, cases, defines=()):
     return map(int, subprocess.check_output([str(exe), str(cases)], text=True).split())
 
 
-def test_cover_rect_never_rejects_a_taking_block(tmp_path):
-    blocks, unsafe, takes, empty, empty_rejected = _run(tmp_path, 400000)
-    assert blocks > 2_000_000 and takes > 1_000_000 and empty > 200_000
-    assert unsafe == 0, f"{unsafe} blocks with a taking pixel were rejected"
-    # tight: rejects nearly every block that no pixel centre takes
-    assert empty_rejected >= 0.90 * empty, (empty_rejected, empty)
+def test_cover_never_rejects_a_taking_block(tmp_path):
+    r = list(_run(tmp_path, 400000))
+    for label, (blocks, unsafe, takes, empty, empty_rejected) in (("rect", r[:5]), ("tile mask", r[5:])):
+        assert blocks > 1_000_000 and takes > 200_000 and empty > 200_000, label
+        assert unsafe == 0, f"{label}: {unsafe} blocks with a taking pixel were rejected"
+        # tight: rejects nearly every block that no pixel centre takes
+        assert empty_rejected >= 0.90 * empty, (label, empty_rejected, empty)
 
 
 def test_cover_probe_catches_a_missing_rounding_margin(tmp_path):
-    """Without the margin, needle-like splats lose pixels: the probe sees it."""
-    blocks, unsafe, takes, empty, empty_rejected = _run(tmp_path, 400000, ["GS_COVER_MARGIN=0.0f"])
-    assert unsafe > 0
+    """Without the margin, needle-like splats lose pixels: the probe sees it
+    in both forms."""
+    r = list(_run(tmp_path, 400000, ["GS_COVER_MARGIN=0.0f"]))
+    assert r[1] > 0 and r[6] > 0, r
