@@ -53,20 +53,23 @@ def build_workload(n, views, W, H, f, seed=0, gt_seed=1):
     return p, sh, cams, images
 
 
-def algorithmic_bytes(name, n, vis, keys, npx, shC):
-    """HBM bytes each kernel has to move per launch (f32):
-    project   n*(56 params + 16 key/rect/count) + vis*(4*shC SH + 48 record)
+def algorithmic_bytes(name, n, vis, keys, npx, shC, mixed=False):
+    """HBM bytes each kernel has to move per launch (f32; mixed stores SH
+    coefficients and the Adam moments in bf16):
+    project   n*(56 params + 16 key/rect/count) + vis*(4*shC SH + 48 record)   (SH 2*shC mixed)
     raster fwd  keys*(4 id + 48 record) + px*(16 img + 4 T + 4 end + 16 dL/dC + 4 target)
     raster bwd  keys*(4 id + 48 record) + px*(16 dL/dC + 4 T + 4 end) + vis*40 gradient sums
-    update    n*(40+40 sums + 4 count + 3*(56+56) params/m/v) + n*3*2*4*shC SH/m/v"""
+    update    n*(40+40 sums + 4 count + 2*56 params + 2*2*56 m/v) + n*3*2*4*shC SH/m/v
+              (mixed: m/v 2*2*28, SH/m/v n*3*2*2*shC)"""
+    sb = 2.0 if mixed else 4.0
     if name == "project":
-        return n * 72.0 + vis * (4.0 * shC + 48.0)
+        return n * 72.0 + vis * (sb * shC + 48.0)
     if name == "raster_fwd_loss":
         return keys * 52.0 + npx * 44.0
     if name == "raster_bwd":
         return keys * 52.0 + npx * 24.0 + vis * 40.0
     if name == "update":
-        return n * (84.0 + 3 * 112.0) + n * 24.0 * shC
+        return n * (84.0 + 112.0 + 4.0 * 14.0 * sb) + n * 6.0 * sb * shC
     return None
 
 
@@ -93,6 +96,8 @@ def main():
     ap.add_argument("--views", type=int, default=64)
     ap.add_argument("--width", type=int, default=1600)
     ap.add_argument("--height", type=int, default=900)
+    ap.add_argument("--precision", choices=["f32", "mixed"], default="f32",
+                    help="mixed: SH coefficients and Adam moments stored in bf16 (f32 arithmetic)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device-to-device copy (scales HBM byte counters)")
@@ -104,7 +109,8 @@ def main():
 
     W, H, f = args.width, args.height, 1100.0 * args.width / 1600.0
     p, sh, cams, images = build_workload(args.n, args.views, W, H, f)
-    t = gsabi.GSTrainer(shDeg=2, maxSplats=args.n, posLr=1.6e-4, seed=7)
+    prec = gsabi.GS_PREC_MIXED if args.precision == "mixed" else gsabi.GS_PREC_F32
+    t = gsabi.GSTrainer(shDeg=2, maxSplats=args.n, posLr=1.6e-4, seed=7, precision=prec)
     t.setup({"data": p, "n": args.n, "sh": sh}, cams, images, W, H, 8.0)
     t.iter = 2000
     log("warmup")
@@ -141,12 +147,12 @@ def main():
     kernels = {}
     for name, v in prof.items():
         avg_ms = v["ms"] / max(v["launches"], 1)
-        b = algorithmic_bytes(name, args.n, mean_vis, mean_keys, npx, shC)
+        b = algorithmic_bytes(name, args.n, mean_vis, mean_keys, npx, shC, args.precision == "mixed")
         kernels[name] = {"avg_ms": round(avg_ms, 4),
                          "GBps": None if b is None else round(b / (avg_ms * 1e-3) / 1e9, 1)}
     single = {k: v for k, v in kernels.items() if v["GBps"] is not None}
     dom = max(single, key=lambda k: single[k]["avg_ms"])
-    dom_b = algorithmic_bytes(dom, args.n, mean_vis, mean_keys, npx, shC)
+    dom_b = algorithmic_bytes(dom, args.n, mean_vis, mean_keys, npx, shC, args.precision == "mixed")
     achieved = dom_b / (single[dom]["avg_ms"] * 1e-3) / 1e9
     traffic = None
     tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_1M_1600x900.json")

@@ -39,6 +39,7 @@ enum {
 enum { GS_LOSS_CHARBONNIER = 0, GS_LOSS_L1_DSSIM = 1 };
 enum { GS_ACCUM_FIXED = 0, GS_ACCUM_F32 = 1 };
 enum { GS_SORT_DEPTH_THEN_TILE = 0, GS_SORT_GLOBAL64 = 1 };
+enum { GS_PREC_F32 = 0, GS_PREC_MIXED = 1 };
 
 /* Trainer options: the reference's opts object (trainer.js:18-36, :45-49,
  * :74, :105-107, :302, :319, :338-341), plus the MI355X build's own knobs. */
@@ -64,6 +65,8 @@ typedef struct gs_opts {
   int32_t sortMode;    /* key ordering strategy; both produce the identical permutation */
   uint32_t seed;       /* camera-pick RNG seed (the reference uses Math.random, trainer.js:344) */
   int32_t device;      /* HIP device ordinal, default 0 */
+  int32_t precision;   /* GS_PREC_F32 (default), or GS_PREC_MIXED: SH coefficients and all Adam
+                          moments stored in bf16 with stochastic rounding, f32 arithmetic */
 } gs_opts;
 
 /* One training camera (trainer.js:95-100 cams[], :357-372): R row-major

@@ -109,6 +109,76 @@ void orc_sh_adam_f32(const orc_adam* hp, int64_t count, float* sh, const float* 
   }
 }
 
+/* Mixed precision: the same Adam, with the moments (and the SH coefficients)
+ * held in bf16. Each stored value is rounded stochastically with the bits
+ * gs_sr_bits(gs_sr_seed(Gaussian, step), 64 * stream + slot), streams 0/1 for
+ * the parameter moments m/v and 2/3/4 for the SH coefficient and its m/v, as
+ * the GPU update does. Slots Adam skips (lr 0) keep their bits. */
+void orc_adam_mixed_f32(const orc_adam* hp, int64_t n, float* P, const float* G,
+                        uint16_t* m, uint16_t* v, uint32_t step) {
+  for (int64_t i = 0; i < n; i++)
+    for (int slot = 0; slot < 16; slot++) {
+      const int64_t j = 16 * i + slot;
+      const uint32_t seed = gs_sr_seed((uint64_t)i, step);
+      float mf = gs_bf2f(m[j]), vf = gs_bf2f(v[j]);
+      const float lr = hp->lr[slot];
+      if (lr != 0.0f) {
+        float g = G[j];
+        if (!(fabsf(g) < 1e18f)) g = 0.0f;
+        if (slot == 13) {
+          const float sg = gs_sigmoid_clamped(P[j]);
+          g = g + hp->reg * sg * (1.0f - sg);
+        }
+        const float mm = hp->b1 * mf + (1.0f - hp->b1) * g;
+        const float vv = hp->b2 * vf + (1.0f - hp->b2) * g * g;
+        mf = mm; vf = vv;
+        const float mh = mm / hp->bc1;
+        const float vh = vv / hp->bc2;
+        float p = P[j] - lr * mh / (sqrtf(vh) + hp->eps);
+        if (slot >= 3 && slot <= 5) p = fminf(fmaxf(p, hp->minLS), hp->maxLS);
+        if (slot >= 10 && slot <= 13) p = fminf(fmaxf(p, -hp->maxLogit), hp->maxLogit);
+        P[j] = p;
+      }
+      m[j] = (uint16_t)gs_f2bf_sr(mf, gs_sr_bits(seed, 0u * 64u + (uint32_t)slot));
+      v[j] = (uint16_t)gs_f2bf_sr(vf, gs_sr_bits(seed, 1u * 64u + (uint32_t)slot));
+    }
+}
+
+void orc_sh_adam_mixed_f32(const orc_adam* hp, int64_t n, int nc, uint16_t* sh, const float* G,
+                           uint16_t* m, uint16_t* v, uint32_t step) {
+  for (int64_t i = 0; i < n; i++)
+    for (int c = 0; c < nc; c++) {
+      const int64_t j = (int64_t)nc * i + c;
+      const uint32_t seed = gs_sr_seed((uint64_t)i, step);
+      float g = G[j];
+      if (!(fabsf(g) < 1e18f)) g = 0.0f;
+      const float mm = hp->b1 * gs_bf2f(m[j]) + (1.0f - hp->b1) * g;
+      const float vv = hp->b2 * gs_bf2f(v[j]) + (1.0f - hp->b2) * g * g;
+      const float mh = mm / hp->bc1;
+      const float vh = vv / hp->bc2;
+      const float s = gs_bf2f(sh[j]) - hp->shLr * mh / (sqrtf(vh) + hp->eps);
+      sh[j] = (uint16_t)gs_f2bf_sr(s, gs_sr_bits(seed, 2u * 64u + (uint32_t)c));
+      m[j] = (uint16_t)gs_f2bf_sr(mm, gs_sr_bits(seed, 3u * 64u + (uint32_t)c));
+      v[j] = (uint16_t)gs_f2bf_sr(vv, gs_sr_bits(seed, 4u * 64u + (uint32_t)c));
+    }
+}
+
+void orc_f2bf_rne(int64_t count, const float* src, uint16_t* dst) {
+  for (int64_t j = 0; j < count; j++) dst[j] = (uint16_t)gs_f2bf_rne(src[j]);
+}
+
+void orc_f2bf_sr(int64_t count, const float* src, const uint32_t* rnd, uint16_t* dst) {
+  for (int64_t j = 0; j < count; j++) dst[j] = (uint16_t)gs_f2bf_sr(src[j], rnd[j]);
+}
+
+void orc_bf2f(int64_t count, const uint16_t* src, float* dst) {
+  for (int64_t j = 0; j < count; j++) dst[j] = gs_bf2f(src[j]);
+}
+
+uint32_t orc_sr_bits(uint64_t gaussian, uint32_t slot, uint32_t step, uint32_t stream) {
+  return gs_sr_bits(gs_sr_seed(gaussian, step), 64u * stream + slot);
+}
+
 /* JS Math.round (ties toward +inf) followed by ToInt32 */
 static int32_t js_round_to_int32(double x) {
   if (x != x || isinf(x)) return 0;

@@ -139,6 +139,15 @@ void orc_adam_f32(const orc_adam* hp, int64_t n, float* params, const float* gra
                   float* m, float* v);
 void orc_sh_adam_f32(const orc_adam* hp, int64_t count, float* sh, const float* g,
                      float* m, float* v);
+/* mixed precision: bf16 moments and SH coefficients, stochastic rounding */
+void orc_adam_mixed_f32(const orc_adam* hp, int64_t n, float* params, const float* gradF,
+                        uint16_t* m, uint16_t* v, uint32_t step);
+void orc_sh_adam_mixed_f32(const orc_adam* hp, int64_t n, int nc, uint16_t* sh, const float* g,
+                           uint16_t* m, uint16_t* v, uint32_t step);
+void orc_f2bf_rne(int64_t count, const float* src, uint16_t* dst);
+void orc_f2bf_sr(int64_t count, const float* src, const uint32_t* rnd, uint16_t* dst);
+void orc_bf2f(int64_t count, const uint16_t* src, float* dst);
+uint32_t orc_sr_bits(uint64_t gaussian, uint32_t slot, uint32_t step, uint32_t stream);
 
 /* R16: pack float RGB (r < 0 = invalid) into RGBA8 exactly as
  * trainer.js:128-139 does (JS Math.round, ToInt32 shifts). */

@@ -60,6 +60,12 @@ def lib():
             ("orc_adam_f32", [P, C.c_int64, P, P, P, P], None),
             ("orc_sh_adam_f32", [P, C.c_int64, P, P, P, P], None),
             ("orc_loss_l1ssim_f64", [P, P, P, C.c_double, P, P, P], None),
+            ("orc_adam_mixed_f32", [P, C.c_int64, P, P, P, P, C.c_uint32], None),
+            ("orc_sh_adam_mixed_f32", [P, C.c_int64, C.c_int, P, P, P, P, C.c_uint32], None),
+            ("orc_f2bf_rne", [C.c_int64, P, P], None),
+            ("orc_f2bf_sr", [C.c_int64, P, P, P], None),
+            ("orc_bf2f", [C.c_int64, P, P], None),
+            ("orc_sr_bits", [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32], C.c_uint32),
         ]:
             fn = getattr(L, name)
             fn.argtypes = args
@@ -231,6 +237,42 @@ def adam(hp, params, grad, m, v):
 def sh_adam(hp, sh, grad, m, v):
     lib().orc_sh_adam_f32(C.byref(hp), sh.size, _p(sh), _p(np.ascontiguousarray(grad, np.float32)),
                           _p(m), _p(v))
+
+
+def adam_mixed(hp, params, grad, m16, v16, step):
+    """Adam with bf16 moments (uint16 arrays, n x 16), stochastic rounding."""
+    n = params.shape[0]
+    lib().orc_adam_mixed_f32(C.byref(hp), n, _p(params), _p(np.ascontiguousarray(grad, np.float32)),
+                             _p(m16), _p(v16), step)
+
+
+def sh_adam_mixed(hp, sh16, grad, m16, v16, step):
+    """SH Adam with bf16 coefficients and moments (uint16 arrays, n x 3K)."""
+    n, nc = sh16.shape
+    lib().orc_sh_adam_mixed_f32(C.byref(hp), n, nc, _p(sh16), _p(np.ascontiguousarray(grad, np.float32)),
+                                _p(m16), _p(v16), step)
+
+
+def f2bf_rne(x):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(x.shape, np.uint16)
+    lib().orc_f2bf_rne(x.size, _p(x), _p(out))
+    return out
+
+
+def f2bf_sr(x, rnd):
+    x = np.ascontiguousarray(x, np.float32)
+    rnd = np.ascontiguousarray(rnd, np.uint32)
+    out = np.zeros(x.shape, np.uint16)
+    lib().orc_f2bf_sr(x.size, _p(x), _p(rnd), _p(out))
+    return out
+
+
+def bf2f(b):
+    b = np.ascontiguousarray(b, np.uint16)
+    out = np.zeros(b.shape, np.float32)
+    lib().orc_bf2f(b.size, _p(b), _p(out))
+    return out
 
 
 def loss_of(stage, cam, params, sh, tgt):

@@ -211,6 +211,7 @@ napi_value Create(napi_env env, napi_callback_info info) {
       o.sortMode = (int32_t)get_num(env, a, "sortMode", o.sortMode);
       o.seed = (uint32_t)get_num(env, a, "seed", o.seed);
       o.device = (int32_t)get_num(env, a, "device", o.device);
+      o.precision = (int32_t)get_num(env, a, "precision", o.precision);
     }
   }
   gs_trainer_t* h = nullptr;

@@ -44,6 +44,7 @@ struct AdamHP {
   float minLS, maxLS, maxLogit, reg;
   float shLr;
   float anisoReg;
+  uint32_t step;   // Adam step t: seeds the stochastic rounding of bf16 stores
 };
 
 // Parameters live in four float4 planes (plane c = slots 4c..4c+3 of every
@@ -193,7 +194,9 @@ template <int kDeg>
 struct ShDims {
   static constexpr int K = kDeg > 0 ? (kDeg + 1) * (kDeg + 1) - 1 : 0;
   static constexpr int C = 3 * K;               // coefficients per Gaussian
-  static constexpr int P4 = (C + 3) / 4;        // float4 planes
+  static constexpr int P4 = (C + 3) / 4;        // float4 planes (f32 storage)
+  static constexpr int Q8 = (C + 7) / 8;        // uint4 planes of 8 bf16 (mixed storage)
+  static constexpr int S = 8 * Q8 > 4 * P4 ? 8 * Q8 : 4 * P4;   // register copy size
 };
 
 // view direction from the camera (shaders.js:284-285, :914-916)
@@ -213,6 +216,32 @@ __device__ __forceinline__ void load_sh(const float4* __restrict__ SH, int64_t c
     const float4 v = SH[c * cap + i];
     sh[4 * c + 0] = v.x; sh[4 * c + 1] = v.y; sh[4 * c + 2] = v.z; sh[4 * c + 3] = v.w;
   }
+}
+
+// bf16 planes: value c of a Gaussian lives in uint4 plane c/8, half c%8
+// (even halves in the low 16 bits of each 32-bit word).
+__device__ __forceinline__ void unpack_bf8(uint4 w, float* o) {
+  o[0] = gs_bf2f(w.x); o[1] = gs_bf2f(w.x >> 16);
+  o[2] = gs_bf2f(w.y); o[3] = gs_bf2f(w.y >> 16);
+  o[4] = gs_bf2f(w.z); o[5] = gs_bf2f(w.z >> 16);
+  o[6] = gs_bf2f(w.w); o[7] = gs_bf2f(w.w >> 16);
+}
+
+__device__ __forceinline__ uint4 pack_bf8(const uint32_t* b) {
+  return make_uint4(b[0] | (b[1] << 16), b[2] | (b[3] << 16), b[4] | (b[5] << 16), b[6] | (b[7] << 16));
+}
+
+template <int kDeg>
+__device__ __forceinline__ void load_sh_bf(const uint4* __restrict__ SH, int64_t cap, int64_t i, float* sh) {
+#pragma unroll
+  for (int c = 0; c < ShDims<kDeg>::Q8; c++) unpack_bf8(SH[c * cap + i], sh + 8 * c);
+}
+
+// SH coefficients from either storage (kMixed: bf16 planes)
+template <int kDeg, bool kMixed>
+__device__ __forceinline__ void load_sh_any(const void* __restrict__ SH, int64_t cap, int64_t i, float* sh) {
+  if constexpr (kMixed) load_sh_bf<kDeg>((const uint4*)SH, cap, i, sh);
+  else load_sh<kDeg>((const float4*)SH, cap, i, sh);
 }
 
 }  // namespace gs

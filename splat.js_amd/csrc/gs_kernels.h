@@ -68,8 +68,9 @@ struct RasterBufs {
 
 struct UpdateBufs {
   int64_t n, cap;
-  float4 *P, *M, *V;      // 4 planes each
-  float4 *SH, *SHM, *SHV; // ShDims::P4 planes each
+  float4* P;              // 4 float4 planes
+  void *M, *V;            // 4 float4 planes, or 2 bf16 uint4 planes (mixed)
+  void *SH, *SHM, *SHV;   // ShDims::P4 float4 planes, or ShDims::Q8 bf16 planes (mixed)
   void* accum;            // n x 12 (10 used) i32 or f32
   const uint32_t* cnt;
   float* dbgGradF;        // optional n x 16
@@ -78,9 +79,9 @@ struct UpdateBufs {
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, int64_t count, uint32_t v);
 
-void launch_project(hipStream_t st, int deg, int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
-                    const float4* P, const float4* SH, float4* rec, uint32_t* depthKey, uint2* rect,
-                    uint32_t* cnt);
+void launch_project(hipStream_t st, int deg, bool mixed, int64_t n, int64_t cap, const Cam& cam,
+                    const Cut& cut, const float4* P, const void* SH, float4* rec, uint32_t* depthKey,
+                    uint2* rect, uint32_t* cnt);
 
 int tile_bits(int numTiles);
 void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out);
@@ -93,7 +94,7 @@ void launch_loss_l1ssim(hipStream_t st, const Cam& cam, float lambda, const Rast
 void launch_raster_bwd(hipStream_t st, const Cam& cam, const Cut& cut, const RasterBufs& b,
                        int accumMode, void* accum);
 
-void launch_update(hipStream_t st, int deg, int accumMode, const Cam& cam, const AdamHP& hp,
+void launch_update(hipStream_t st, int deg, int accumMode, bool mixed, const Cam& cam, const AdamHP& hp,
                    const UpdateBufs& u);
 
 }  // namespace gs

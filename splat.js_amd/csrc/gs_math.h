@@ -126,6 +126,55 @@ GS_HD float gs_logf(float x) {
   return fmaf(ef, 0.693145751953125f, fmaf(ef, 1.428606765330187045e-06f, lm));
 }
 
+// ---- bfloat16 storage (mixed precision) ----
+// bf16 keeps the f32 sign, exponent and top 7 mantissa bits. Values are
+// decoded exactly; arithmetic stays f32. Stores round stochastically: the
+// 16 bits below the kept mantissa are added to a random 16-bit number before
+// truncation, so a value rounds up with probability equal to its distance
+// from the lower neighbour. Small Adam updates (lr 1e-4 on O(0.1) SH
+// coefficients, the 0.1% per-step decay of v) then survive in expectation
+// instead of vanishing under round-to-nearest. Inf and NaN are truncated (a
+// NaN keeps a mantissa bit set).
+
+// 32-bit integer hash (xor-shift-multiply avalanche): the random bits of
+// stochastic rounding come from a counter of (Gaussian, step, stream, slot).
+GS_HD uint32_t gs_hash32(uint32_t x) {
+  x ^= x >> 16; x *= 0x7feb352du;
+  x ^= x >> 15; x *= 0x846ca68bu;
+  x ^= x >> 16;
+  return x;
+}
+
+// Seed of one (Gaussian, step): two full hash rounds.
+GS_HD uint32_t gs_sr_seed(uint64_t gaussian, uint32_t step) {
+  return gs_hash32((uint32_t)gaussian * 0x9E3779B1u ^ gs_hash32((uint32_t)(gaussian >> 32) + step * 0xC2B2AE3Du));
+}
+
+// 16 random bits for value k of that seed (k = 64 * stream + slot): one
+// multiply-xorshift round per pair of values, the low half for even k and
+// the high half for odd k.
+GS_HD uint32_t gs_sr_bits(uint32_t seed, uint32_t k) {
+  uint32_t x = seed ^ ((k >> 1) * 0x9E3779B9u);
+  x *= 0x85ebca6bu; x ^= x >> 13;
+  x *= 0xc2b2ae35u; x ^= x >> 16;
+  return (k & 1u) ? (x >> 16) : (x & 0xffffu);
+}
+
+GS_HD float gs_bf2f(uint32_t b) { return gs_u2f((b & 0xffffu) << 16); }
+
+// round to nearest even (initial upload of f32 data); NaN stays NaN
+GS_HD uint32_t gs_f2bf_rne(float x) {
+  const uint32_t u = gs_f2u(x);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+GS_HD uint32_t gs_f2bf_sr(float x, uint32_t rnd) {
+  const uint32_t u = gs_f2u(x);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);
+  return (u + (rnd & 0xffffu)) >> 16;
+}
+
 // 1 / (1 + e^-clamp(x, -9, 9)): the activation for colour and opacity logits.
 GS_HD float gs_sigmoid_clamped(float x) {
   const float c = fminf(fmaxf(x, -9.0f), 9.0f);

@@ -13,15 +13,16 @@
 //                  0xffffffff when culled, which sorts culled Gaussians last
 //   rect[i]        (tx0 | tx1 << 16, ty0 | ty1 << 16)
 //   cnt[i]         tiles covered, 0 when culled
-// HBM-bound: 56-64 B of params + 4*3K B of SH read, 60 B written per Gaussian.
+// HBM-bound: 56-64 B of params + 4*3K B of SH (2*3K B in mixed precision)
+// read, 64 B of record and 16 B of key/rect/count written per visible Gaussian.
 #include "gs_kernels.h"
 
 namespace gs {
 
-template <int kDeg>
+template <int kDeg, bool kMixed>
 __global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam cam, Cut cut,
                                                     const float4* __restrict__ P,
-                                                    const float4* __restrict__ SH,
+                                                    const void* __restrict__ SH,
                                                     float4* __restrict__ rec,
                                                     uint32_t* __restrict__ depthKey,
                                                     uint2* __restrict__ rect,
@@ -60,8 +61,8 @@ __global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam 
       view_dir(cam, p, v);
       float Y[15];
       sh_basis<kDeg, false>(v, Y, nullptr);
-      float sh[4 * ShDims<kDeg>::P4];
-      load_sh<kDeg>(SH, cap, i, sh);
+      float sh[ShDims<kDeg>::S];
+      load_sh_any<kDeg, kMixed>(SH, cap, i, sh);
       const int aK = cam.shActiveK;
 #pragma unroll
       for (int k = 0; k < K; k++) {
@@ -95,17 +96,25 @@ __global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam 
   cnt[i] = c;
 }
 
-void launch_project(hipStream_t st, int deg, int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
-                    const float4* P, const float4* SH, float4* rec, uint32_t* depthKey, uint2* rect,
-                    uint32_t* cnt) {
-  if (n <= 0) return;
+template <bool kMixed>
+static void launch_project_t(hipStream_t st, int deg, int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
+                             const float4* P, const void* SH, float4* rec, uint32_t* depthKey, uint2* rect,
+                             uint32_t* cnt) {
   const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
   switch (deg) {
-    case 0: k_project<0><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
-    case 1: k_project<1><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
-    case 2: k_project<2><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
-    default: k_project<3><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
+    case 0: k_project<0, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
+    case 1: k_project<1, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
+    case 2: k_project<2, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
+    default: k_project<3, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
   }
+}
+
+void launch_project(hipStream_t st, int deg, bool mixed, int64_t n, int64_t cap, const Cam& cam,
+                    const Cut& cut, const float4* P, const void* SH, float4* rec, uint32_t* depthKey,
+                    uint2* rect, uint32_t* cnt) {
+  if (n <= 0) return;
+  if (mixed) launch_project_t<true>(st, deg, n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt);
+  else launch_project_t<false>(st, deg, n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt);
 }
 
 }  // namespace gs

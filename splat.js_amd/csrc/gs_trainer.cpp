@@ -74,13 +74,15 @@ struct gs_trainer {
   bool ready = false;
 
   int64_t n = 0, cap = 0;
-  int shDeg = 0, shK = 0, shC = 0, shP4 = 0;
+  int shDeg = 0, shK = 0, shC = 0, shP4 = 0, shQ8 = 0;
+  bool mixed = false;   // GS_PREC_MIXED: bf16 SH and moments
   int64_t entriesCap = 0;
   int maxTiles = 0;
   int64_t maxPix = 0;
 
   std::vector<void*> allocs;
-  float4 *P = nullptr, *M = nullptr, *V = nullptr, *SH = nullptr, *SHM = nullptr, *SHV = nullptr;
+  float4* P = nullptr;
+  void *M = nullptr, *V = nullptr, *SH = nullptr, *SHM = nullptr, *SHV = nullptr;   // f32 or bf16 planes
   float4* rec = nullptr;
   uint32_t *depthKey = nullptr, *cnt = nullptr;
   uint2* rect = nullptr;
@@ -251,6 +253,7 @@ AdamHP make_hp(const gs_trainer* h) {
   a.b1 = 0.9f; a.b2 = 0.999f; a.eps = 1e-15f;
   a.bc1 = (float)(1.0 - std::pow((double)0.9f, (double)h->iter));
   a.bc2 = (float)(1.0 - std::pow((double)0.999f, (double)h->iter));
+  a.step = (uint32_t)h->iter;
   const double r = h->sceneRadius;
   a.minLS = (float)std::log(r * (double)h->o.minScale);
   a.maxLS = (float)std::log(r * 0.05);
@@ -274,7 +277,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
   const int64_t n = h->n;
   {
     Scope s(h, "project", (double)n * (56.0 + 4.0 * h->shC + 48.0 + 16.0));
-    launch_project(h->st, h->shDeg, n, h->cap, cam, make_cut(h), h->P, h->SH, h->rec, h->depthKey,
+    launch_project(h->st, h->shDeg, h->mixed, n, h->cap, cam, make_cut(h), h->P, h->SH, h->rec, h->depthKey,
                    h->rect, h->cnt);
   }
   {
@@ -438,7 +441,7 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
   u.dbgShGrad = dSG;
   {
     Scope s(h, "update", (double)h->n * (100.0 + 6 * 64.0 + 6 * 4.0 * h->shC));
-    launch_update(h->st, h->shDeg, h->o.gradAccum, cam, hp, u);
+    launch_update(h->st, h->shDeg, h->o.gradAccum, h->mixed, cam, hp, u);
   }
   HIPCK(hipGetLastError());
   if (dbg) {
@@ -495,6 +498,7 @@ int gs_default_opts(gs_opts* o) {
   o->sortMode = GS_SORT_DEPTH_THEN_TILE;
   o->seed = 0;
   o->device = 0;
+  o->precision = GS_PREC_F32;
   return GS_OK;
 }
 
@@ -512,6 +516,7 @@ int gs_create(const gs_opts* o, gs_trainer_t** out) {
   }
   if (h->o.shDeg < 0 || h->o.shDeg > 3 || h->o.sortMode < 0 || h->o.sortMode > 1 ||
       h->o.gradAccum < 0 || h->o.gradAccum > 1 || h->o.loss < 0 || h->o.loss > 1 ||
+      h->o.precision < 0 || h->o.precision > 1 ||
       !(h->o.eCut >= 0.0f && h->o.eCut <= 80.0f)) {
     delete h;
     return GS_E_ARG;
@@ -530,6 +535,8 @@ int gs_create(const gs_opts* o, gs_trainer_t** out) {
   h->shK = h->shDeg > 0 ? (h->shDeg + 1) * (h->shDeg + 1) - 1 : 0;
   h->shC = 3 * h->shK;
   h->shP4 = (h->shC + 3) / 4;
+  h->shQ8 = (h->shC + 7) / 8;
+  h->mixed = h->o.precision == GS_PREC_MIXED;
   h->rng = h->o.seed;
   *out = h;
   return GS_OK;
@@ -612,9 +619,10 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   const size_t pb = (size_t)cap * 64;
   int rc = 0;
   rc |= dalloc(h, &h->P, pb);
-  rc |= dalloc(h, &h->M, pb);
-  rc |= dalloc(h, &h->V, pb);
-  const size_t sb = (size_t)cap * 16 * std::max(h->shP4, 1);
+  const size_t mb = h->mixed ? (size_t)cap * 32 : pb;   // 16 moments: 2 bf16 planes or 4 f32 planes
+  rc |= dalloc(h, &h->M, mb);
+  rc |= dalloc(h, &h->V, mb);
+  const size_t sb = (size_t)cap * 16 * std::max(h->mixed ? h->shQ8 : h->shP4, 1);
   rc |= dalloc(h, &h->SH, sb);
   rc |= dalloc(h, &h->SHM, sb);
   rc |= dalloc(h, &h->SHV, sb);
@@ -660,14 +668,21 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   std::vector<float> planes;
   repack_to_planes(params, n, 16, 4, cap, planes);
   HIPCK(hipMemcpy(h->P, planes.data(), pb, hipMemcpyHostToDevice));
-  if (h->shC) {
-    std::vector<float> shz;
+  if (h->shC && !h->mixed) {
     if (sh) repack_to_planes(sh, n, h->shC, h->shP4, cap, planes);
     else planes.assign((size_t)h->shP4 * cap * 4, 0.0f);
     HIPCK(hipMemcpy(h->SH, planes.data(), sb, hipMemcpyHostToDevice));
+  } else if (h->shC) {
+    // bf16 planes: coefficient j in plane j/8, half j%8; f32 rounded to nearest even
+    std::vector<uint16_t> b16((size_t)h->shQ8 * cap * 8, 0);
+    if (sh)
+      for (int64_t i = 0; i < n; i++)
+        for (int j = 0; j < h->shC; j++)
+          b16[((size_t)(j / 8) * cap + i) * 8 + (j % 8)] = (uint16_t)gs_f2bf_rne(sh[i * h->shC + j]);
+    HIPCK(hipMemcpy(h->SH, b16.data(), sb, hipMemcpyHostToDevice));
   }
-  HIPCK(hipMemset(h->M, 0, pb));
-  HIPCK(hipMemset(h->V, 0, pb));
+  HIPCK(hipMemset(h->M, 0, mb));
+  HIPCK(hipMemset(h->V, 0, mb));
   HIPCK(hipMemset(h->SHM, 0, sb));
   HIPCK(hipMemset(h->SHV, 0, sb));
   HIPCK(hipMemset(h->accum, 0, (size_t)cap * 48));
@@ -762,10 +777,16 @@ int gs_read_gaussians(gs_trainer_t* h, float* params, float* sh) {
     HIPCK(hipMemcpy(planes.data(), h->P, planes.size() * 4, hipMemcpyDeviceToHost));
     planes_to_rows(planes, h->n, 16, h->cap, params);
   }
-  if (sh && h->shC) {
+  if (sh && h->shC && !h->mixed) {
     std::vector<float> planes((size_t)h->shP4 * h->cap * 4);
     HIPCK(hipMemcpy(planes.data(), h->SH, planes.size() * 4, hipMemcpyDeviceToHost));
     planes_to_rows(planes, h->n, h->shC, h->cap, sh);
+  } else if (sh && h->shC) {
+    std::vector<uint16_t> b16((size_t)h->shQ8 * h->cap * 8);
+    HIPCK(hipMemcpy(b16.data(), h->SH, b16.size() * 2, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < h->n; i++)
+      for (int j = 0; j < h->shC; j++)
+        sh[i * h->shC + j] = gs_bf2f(b16[((size_t)(j / 8) * h->cap + i) * 8 + (j % 8)]);
   }
   return GS_OK;
 }

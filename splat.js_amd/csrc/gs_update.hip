@@ -12,15 +12,21 @@
 //
 // Invisible Gaussians still take the Adam step with a zero gradient (plus the
 // opacity regulariser), like the reference, whose Adam runs over all n.
+//
+// Mixed precision (kMixed): the SH coefficients and all Adam moments are
+// stored in bf16 (gs_math.h: exact decode, stochastic rounding on store),
+// which takes the pass from ~1.06 KB to ~0.64 KB per Gaussian at SH degree 2;
+// parameters, gradients and all arithmetic stay f32.
 #include "gs_kernels.h"
 
 namespace gs {
 
-template <int kDeg, int kAccum, bool kDebug>
+template <int kDeg, int kAccum, bool kDebug, bool kMixed>
 __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBufs u) {
   constexpr int K = ShDims<kDeg>::K;
   constexpr int NC = ShDims<kDeg>::C;
   constexpr int P4 = ShDims<kDeg>::P4;
+  constexpr int Q8 = ShDims<kDeg>::Q8;
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= u.n) return;
   const int64_t cap = u.cap;
@@ -40,8 +46,8 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
 
   float p[16];
   load_params(u.P, cap, i, p);
-  float sh[P4 > 0 ? 4 * P4 : 1];
-  if constexpr (kDeg > 0) load_sh<kDeg>(u.SH, cap, i, sh);
+  float sh[ShDims<kDeg>::S > 0 ? ShDims<kDeg>::S : 1];
+  if constexpr (kDeg > 0) load_sh_any<kDeg, kMixed>(u.SH, cap, i, sh);
 
   float gF[14];
 #pragma unroll
@@ -203,8 +209,15 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
 
   // ---- Adam over the 16 param slots (shaders.js:1000-1035) ----
   float m[16], vv[16];
-  load_params(u.M, cap, i, m);
-  load_params(u.V, cap, i, vv);
+  if constexpr (kMixed) {
+    const uint4* M16 = (const uint4*)u.M;
+    const uint4* V16 = (const uint4*)u.V;
+    unpack_bf8(M16[i], m); unpack_bf8(M16[cap + i], m + 8);
+    unpack_bf8(V16[i], vv); unpack_bf8(V16[cap + i], vv + 8);
+  } else {
+    load_params((const float4*)u.M, cap, i, m);
+    load_params((const float4*)u.V, cap, i, vv);
+  }
 #pragma unroll
   for (int j = 0; j < 14; j++) {
     const float lr = hp.lr[j];
@@ -227,18 +240,73 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
     p[j] = pn;
   }
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-    u.P[c * cap + i] = make_float4(p[4 * c], p[4 * c + 1], p[4 * c + 2], p[4 * c + 3]);
-    u.M[c * cap + i] = make_float4(m[4 * c], m[4 * c + 1], m[4 * c + 2], m[4 * c + 3]);
-    u.V[c * cap + i] = make_float4(vv[4 * c], vv[4 * c + 1], vv[4 * c + 2], vv[4 * c + 3]);
+  for (int c = 0; c < 4; c++) u.P[c * cap + i] = make_float4(p[4 * c], p[4 * c + 1], p[4 * c + 2], p[4 * c + 3]);
+  if constexpr (kMixed) {
+    // moments stored in bf16 with stochastic rounding (streams 0: m, 1: v);
+    // slots Adam skipped re-encode to their own bits
+    const uint32_t seed = gs_sr_seed((uint64_t)i, hp.step);
+    uint32_t bm[16], bv[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      bm[j] = gs_f2bf_sr(m[j], gs_sr_bits(seed, 0u * 64u + (uint32_t)j));
+      bv[j] = gs_f2bf_sr(vv[j], gs_sr_bits(seed, 1u * 64u + (uint32_t)j));
+    }
+    uint4* M16 = (uint4*)u.M;
+    uint4* V16 = (uint4*)u.V;
+    M16[i] = pack_bf8(bm); M16[cap + i] = pack_bf8(bm + 8);
+    V16[i] = pack_bf8(bv); V16[cap + i] = pack_bf8(bv + 8);
+  } else {
+    float4* M = (float4*)u.M;
+    float4* V = (float4*)u.V;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      M[c * cap + i] = make_float4(m[4 * c], m[4 * c + 1], m[4 * c + 2], m[4 * c + 3]);
+      V[c * cap + i] = make_float4(vv[4 * c], vv[4 * c + 1], vv[4 * c + 2], vv[4 * c + 3]);
+    }
   }
 
   // ---- Adam over the SH coefficients (shaders.js:963-981) ----
-  if constexpr (kDeg > 0) {
+  if constexpr (kDeg > 0 && kMixed) {
+    // bf16 coefficients and moments (streams 2: coefficient, 3: m, 4: v);
+    // the padding halves of the last plane stay zero
+    const uint4* SHM16 = (const uint4*)u.SHM;
+    const uint4* SHV16 = (const uint4*)u.SHV;
+    const uint32_t seed = gs_sr_seed((uint64_t)i, hp.step);
+#pragma unroll
+    for (int c = 0; c < Q8; c++) {
+      float ms[8], vs[8];
+      unpack_bf8(SHM16[c * cap + i], ms);
+      unpack_bf8(SHV16[c * cap + i], vs);
+      uint32_t bs[8], bm[8], bv[8];
+#pragma unroll
+      for (int l = 0; l < 8; l++) {
+        const int j = 8 * c + l;
+        float ss = 0.0f, mm = 0.0f, vn = 0.0f;
+        if (j < NC) {
+          float gr = shG[j];
+          if (!(fabsf(gr) < 1e18f)) gr = 0.0f;
+          mm = hp.b1 * ms[l] + (1.0f - hp.b1) * gr;
+          vn = hp.b2 * vs[l] + (1.0f - hp.b2) * gr * gr;
+          const float mh = mm / hp.bc1;
+          const float vh = vn / hp.bc2;
+          ss = sh[j] - hp.shLr * mh / (sqrtf(vh) + hp.eps);
+        }
+        bs[l] = gs_f2bf_sr(ss, gs_sr_bits(seed, 2u * 64u + (uint32_t)j));
+        bm[l] = gs_f2bf_sr(mm, gs_sr_bits(seed, 3u * 64u + (uint32_t)j));
+        bv[l] = gs_f2bf_sr(vn, gs_sr_bits(seed, 4u * 64u + (uint32_t)j));
+      }
+      ((uint4*)u.SH)[c * cap + i] = pack_bf8(bs);
+      ((uint4*)u.SHM)[c * cap + i] = pack_bf8(bm);
+      ((uint4*)u.SHV)[c * cap + i] = pack_bf8(bv);
+    }
+  } else if constexpr (kDeg > 0) {
+    float4* SH = (float4*)u.SH;
+    float4* SHM = (float4*)u.SHM;
+    float4* SHV = (float4*)u.SHV;
 #pragma unroll
     for (int c = 0; c < P4; c++) {
-      const float4 m4 = u.SHM[c * cap + i];
-      const float4 v4 = u.SHV[c * cap + i];
+      const float4 m4 = SHM[c * cap + i];
+      const float4 v4 = SHV[c * cap + i];
       float ms[4] = {m4.x, m4.y, m4.z, m4.w};
       float vs[4] = {v4.x, v4.y, v4.z, v4.w};
       float ss[4] = {sh[4 * c], sh[4 * c + 1], sh[4 * c + 2], sh[4 * c + 3]};
@@ -256,36 +324,43 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
         const float vh = vn / hp.bc2;
         ss[l] = ss[l] - hp.shLr * mh / (sqrtf(vh) + hp.eps);
       }
-      u.SH[c * cap + i] = make_float4(ss[0], ss[1], ss[2], ss[3]);
-      u.SHM[c * cap + i] = make_float4(ms[0], ms[1], ms[2], ms[3]);
-      u.SHV[c * cap + i] = make_float4(vs[0], vs[1], vs[2], vs[3]);
+      SH[c * cap + i] = make_float4(ss[0], ss[1], ss[2], ss[3]);
+      SHM[c * cap + i] = make_float4(ms[0], ms[1], ms[2], ms[3]);
+      SHV[c * cap + i] = make_float4(vs[0], vs[1], vs[2], vs[3]);
     }
   }
 }
 
-template <int kDeg>
+template <int kDeg, bool kMixed>
 static void launch_deg(hipStream_t st, int accumMode, bool dbg, const Cam& cam, const AdamHP& hp,
                        const UpdateBufs& u) {
   const dim3 grid((unsigned)((u.n + kBlock - 1) / kBlock));
   if (accumMode == 0) {
-    if (dbg) k_update<kDeg, 0, true><<<grid, kBlock, 0, st>>>(cam, hp, u);
-    else k_update<kDeg, 0, false><<<grid, kBlock, 0, st>>>(cam, hp, u);
+    if (dbg) k_update<kDeg, 0, true, kMixed><<<grid, kBlock, 0, st>>>(cam, hp, u);
+    else k_update<kDeg, 0, false, kMixed><<<grid, kBlock, 0, st>>>(cam, hp, u);
   } else {
-    if (dbg) k_update<kDeg, 1, true><<<grid, kBlock, 0, st>>>(cam, hp, u);
-    else k_update<kDeg, 1, false><<<grid, kBlock, 0, st>>>(cam, hp, u);
+    if (dbg) k_update<kDeg, 1, true, kMixed><<<grid, kBlock, 0, st>>>(cam, hp, u);
+    else k_update<kDeg, 1, false, kMixed><<<grid, kBlock, 0, st>>>(cam, hp, u);
   }
 }
 
-void launch_update(hipStream_t st, int deg, int accumMode, const Cam& cam, const AdamHP& hp,
+template <bool kMixed>
+static void launch_prec(hipStream_t st, int deg, int accumMode, bool dbg, const Cam& cam, const AdamHP& hp,
+                        const UpdateBufs& u) {
+  switch (deg) {
+    case 0: launch_deg<0, kMixed>(st, accumMode, dbg, cam, hp, u); break;
+    case 1: launch_deg<1, kMixed>(st, accumMode, dbg, cam, hp, u); break;
+    case 2: launch_deg<2, kMixed>(st, accumMode, dbg, cam, hp, u); break;
+    default: launch_deg<3, kMixed>(st, accumMode, dbg, cam, hp, u); break;
+  }
+}
+
+void launch_update(hipStream_t st, int deg, int accumMode, bool mixed, const Cam& cam, const AdamHP& hp,
                    const UpdateBufs& u) {
   if (u.n <= 0) return;
   const bool dbg = u.dbgGradF != nullptr;
-  switch (deg) {
-    case 0: launch_deg<0>(st, accumMode, dbg, cam, hp, u); break;
-    case 1: launch_deg<1>(st, accumMode, dbg, cam, hp, u); break;
-    case 2: launch_deg<2>(st, accumMode, dbg, cam, hp, u); break;
-    default: launch_deg<3>(st, accumMode, dbg, cam, hp, u); break;
-  }
+  if (mixed) launch_prec<true>(st, deg, accumMode, dbg, cam, hp, u);
+  else launch_prec<false>(st, deg, accumMode, dbg, cam, hp, u);
 }
 
 }  // namespace gs

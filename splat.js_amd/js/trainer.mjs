@@ -21,6 +21,7 @@ export const TILE = 16;
 export const LOSS = { charbonnier: 0, l1_dssim: 1 };
 export const ACCUM = { fixed: 0, f32: 1 };
 export const SORT = { depthThenTile: 0, global64: 1 };
+export const PRECISION = { f32: 0, mixed: 1 };
 
 const shRestCoefs = (deg) => (deg + 1) * (deg + 1) - 1;
 
@@ -37,6 +38,7 @@ export class GSTrainer {
     if (typeof o.loss === 'string') o.loss = LOSS[o.loss];
     if (typeof o.gradAccum === 'string') o.gradAccum = ACCUM[o.gradAccum];
     if (typeof o.sortMode === 'string') o.sortMode = SORT[o.sortMode];
+    if (typeof o.precision === 'string') o.precision = PRECISION[o.precision];
     this._h = addon.create(o);
     this.tileGrad = opts.tileGrad === undefined ? true : opts.tileGrad;
     this.shDeg = opts.shDeg === undefined ? 2 : opts.shDeg;

@@ -17,6 +17,7 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "libsplatgs.so")
 GS_LOSS_CHARBONNIER, GS_LOSS_L1_DSSIM = 0, 1
 GS_ACCUM_FIXED, GS_ACCUM_F32 = 0, 1
 GS_SORT_DEPTH_THEN_TILE, GS_SORT_GLOBAL64 = 0, 1
+GS_PREC_F32, GS_PREC_MIXED = 0, 1
 
 
 class GsOpts(C.Structure):
@@ -26,7 +27,7 @@ class GsOpts(C.Structure):
                 ("maxSplats", C.c_int64), ("capMult", C.c_float), ("entriesCap", C.c_int64),
                 ("tileGrad", C.c_int32), ("posLr", C.c_float), ("loss", C.c_int32),
                 ("ssimLambda", C.c_float), ("gradAccum", C.c_int32), ("sortMode", C.c_int32),
-                ("seed", C.c_uint32), ("device", C.c_int32)]
+                ("seed", C.c_uint32), ("device", C.c_int32), ("precision", C.c_int32)]
 
 
 class GsCamera(C.Structure):

@@ -25,7 +25,7 @@ def adam_for_iter(it, radius, posLr=0.0, horizon=60000, minScale=1e-4, opacityRe
 class OracleTrainer:
     """The f32 restatement driven like GSTrainer.stepOnce on a fixed camera."""
 
-    def __init__(self, params, sh, cams, images, radius, opts=None, shDeg=2, fixed=True):
+    def __init__(self, params, sh, cams, images, radius, opts=None, shDeg=2, fixed=True, precision="f32"):
         opts = dict(opts or {})
         self.o = O.make_opts(eCut=opts.get("eCut", 4.5), aMin=opts.get("aMin", 0.0039),
                              radClamp=opts.get("radClamp", 1.0), anisoReg=opts.get("anisoReg", 0.005),
@@ -51,6 +51,15 @@ class OracleTrainer:
             im = images[c["imgIdx"]]
             self.tgt.append(O.pack_targets(np.asarray(im["rgb"], np.float32).reshape(-1, 3)))
         self.cam_deg = [shDeg] * len(cams)
+        # mixed precision: SH coefficients and all moments live in bf16 (uint16 bits)
+        self.mixed = precision == "mixed"
+        if self.mixed:
+            self.SH16 = O.f2bf_rne(self.SH)
+            self.SH = O.bf2f(self.SH16)
+            self.M16 = np.zeros(self.P.shape, np.uint16)
+            self.V16 = np.zeros(self.P.shape, np.uint16)
+            self.SM16 = np.zeros(self.SH.shape, np.uint16)
+            self.SV16 = np.zeros(self.SH.shape, np.uint16)
 
     def step(self, ci):
         deg = min(self.shDeg, self.iter // 1000)
@@ -72,6 +81,12 @@ class OracleTrainer:
                            opacityReg=self.opts.get("opacityReg", 0.01), shLr=self.opts.get("shLr", 1.25e-4))
         pre = {"proj": proj, "rect": rect, "cnt": cnt, "ids": ids, "ts": ts, "keys": keys, "img": img, "T": T,
                "end": end, "gC": gC, "valid": valid, "stats": stats, "accum": g, "gradF": gradF, "shGrad": shG}
+        if self.mixed:
+            O.adam_mixed(hp, self.P, gradF, self.M16, self.V16, self.iter)
+            if self.K:
+                O.sh_adam_mixed(hp, self.SH16, shG, self.SM16, self.SV16, self.iter)
+                self.SH = O.bf2f(self.SH16)
+            return pre
         O.adam(hp, self.P, gradF, self.M, self.V)
         if self.K:
             O.sh_adam(hp, self.SH, shG, self.SM, self.SV)

@@ -94,7 +94,8 @@ def _check_step(gpu, ref, n_tiles, label, stats_acc=None):
 
 def _run_pair(params, sh, cams, images, W, H, radius, opts, shDeg, steps, start_iter=0, cams_seq=None, **kw):
     gpu = _trainer(params, sh, cams, images, W, H, radius, opts, shDeg=shDeg, **kw)
-    ref = OracleTrainer(params, sh, cams, images, radius, opts=opts, shDeg=shDeg)
+    prec = "mixed" if kw.get("precision", gsabi.GS_PREC_F32) == gsabi.GS_PREC_MIXED else "f32"
+    ref = OracleTrainer(params, sh, cams, images, radius, opts=opts, shDeg=shDeg, precision=prec)
     gpu.iter = start_iter
     ref.iter = start_iter
     n_tiles = ((W + 15) // 16) * ((H + 15) // 16)
@@ -131,6 +132,20 @@ def test_config1_bit_exact():
     p, sh, cams, images, W, H, radius = config1_setup()
     _run_pair(p, sh, cams, images, W, H, radius, {}, 2, steps=1, start_iter=0)
     _run_pair(p, sh, cams, images, W, H, radius, {}, 2, steps=2, start_iter=2500)
+
+
+def test_mixed_precision_rig_deg3_bit_exact():
+    """Mixed precision (bf16 SH and moments, stochastic rounding) on the
+    degree-3 rig: every intermediate and the updated params/SH match the
+    restatement's mixed mode bit for bit over four steps."""
+    params, sh, cams, images, W, H, radius, opts = rig_setup(3)
+    _run_pair(params, sh, cams, images, W, H, radius, opts, 3, steps=4, start_iter=5000,
+              precision=gsabi.GS_PREC_MIXED)
+
+
+def test_mixed_precision_config1_bit_exact():
+    p, sh, cams, images, W, H, radius = config1_setup()
+    _run_pair(p, sh, cams, images, W, H, radius, {}, 2, steps=3, start_iter=2500, precision=gsabi.GS_PREC_MIXED)
 
 
 def test_global64_sort_gives_same_permutation():
