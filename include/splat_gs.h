@@ -151,6 +151,31 @@ int gs_set_iter(gs_trainer_t* h, int64_t iter);
 
 int gs_get_info(gs_trainer_t* h, gs_info* out);
 
+/* GSTrainer.refine(rng) (trainer.js:666-794), MCMC-lite: relocate dead
+ * splats (opacity < 0.02, at most ceil(5% n)) onto jittered clones of
+ * donors (opacity > 0.4), then grow by ceil(growRate * n) (until growUntil,
+ * at most growLimit) with 70% splits of the largest donors. The random
+ * stream is the reference's makeRng (mulberry32, src/sfm/geometry.js:750-758)
+ * seeded with `seed`; the reference's caller passes Math.random.
+ * Only opacities and log-scales (16 B per Gaussian) come to the host, which
+ * replays the reference's sequential choices; the copies, jitter and moment
+ * resets are applied on the device. Synchronous. */
+typedef struct gs_refine_opts {
+  int32_t size;          /* sizeof(gs_refine_opts) */
+  uint32_t seed;
+  float growRate;        /* < 0: the reference's default 0.15 (trainer.js:779) */
+  double growUntil;      /* < 0: 0.75 * horizon (trainer.js:778) */
+  int64_t growLimit;     /* <= 0: cap (trainer.growLimit, trainer.js:777) */
+} gs_refine_opts;
+
+typedef struct gs_refine_result {
+  int64_t moved, grown, n;
+  int64_t maxTile;       /* most keys in one tile of the last raster */
+  int64_t overflow;      /* keys dropped for the entry budget, cumulative */
+} gs_refine_result;
+
+int gs_refine(gs_trainer_t* h, const gs_refine_opts* o, gs_refine_result* out);
+
 /* device.queue.onSubmittedWorkDone() (session.js:520, :542) */
 int gs_sync(gs_trainer_t* h);
 

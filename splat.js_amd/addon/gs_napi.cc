@@ -35,6 +35,7 @@ struct Api {
   int (*set_iter)(gs_trainer_t*, int64_t);
   int (*get_info)(gs_trainer_t*, gs_info*);
   int (*sync)(gs_trainer_t*);
+  int (*refine)(gs_trainer_t*, const gs_refine_opts*, gs_refine_result*);
   const char* (*last_error)(gs_trainer_t*);
 };
 
@@ -78,6 +79,7 @@ bool load_api() {
   SYM(set_iter, "gs_set_iter");
   SYM(get_info, "gs_get_info");
   SYM(sync, "gs_sync");
+  SYM(refine, "gs_refine");
   SYM(last_error, "gs_last_error");
 #undef SYM
   g_api.so = so;
@@ -437,6 +439,30 @@ napi_value Sync(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// refine(h, {seed, growRate, growUntil, growLimit}) -> {moved, grown, n, maxTile, overflow}
+napi_value Refine(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  gs_trainer_t* h = get_handle(env, argv[0]);
+  gs_refine_opts ro{};
+  ro.size = (int32_t)sizeof(ro);
+  ro.seed = (uint32_t)get_num(env, argv[1], "seed", 0);
+  ro.growRate = (float)get_num(env, argv[1], "growRate", -1);
+  ro.growUntil = get_num(env, argv[1], "growUntil", -1);
+  ro.growLimit = (int64_t)get_num(env, argv[1], "growLimit", 0);
+  gs_refine_result r;
+  if (!check(env, h, g_api.refine(h, &ro, &r))) return nullptr;
+  napi_value o;
+  NAPI_CALL(env, napi_create_object(env, &o));
+  set_num(env, o, "moved", (double)r.moved);
+  set_num(env, o, "grown", (double)r.grown);
+  set_num(env, o, "n", (double)r.n);
+  set_num(env, o, "maxTile", (double)r.maxTile);
+  set_num(env, o, "overflow", (double)r.overflow);
+  return o;
+}
+
 // loaded() -> true when libsplatgs.so opened and every symbol resolved
 napi_value Loaded(napi_env env, napi_callback_info) {
   napi_value r;
@@ -459,6 +485,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"readLoss", ReadLoss},       {"render", Render},       {"readGaussians", ReadGaussians},
       {"evalCamPsnr", EvalCamPsnr}, {"setHoldout", SetHoldout}, {"setExcluded", SetExcluded},
       {"setIter", SetIter},         {"info", Info},           {"sync", Sync},
+      {"refine", Refine},
       {"loaded", Loaded},           {"loadError", LoadError},
   };
   for (const auto& f : fns) {

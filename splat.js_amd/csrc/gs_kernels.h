@@ -97,4 +97,25 @@ void launch_raster_bwd(hipStream_t st, const Cam& cam, const Cut& cut, const Ras
 void launch_update(hipStream_t st, int deg, int accumMode, bool mixed, const Cam& cam, const AdamHP& hp,
                    const UpdateBufs& u);
 
+// ---- refine (MCMC-lite relocation and growth) ----
+
+// One spawn of the refine plan: target row gets the donor's row with its
+// position jittered (f32(f64(donor pos) + jit)), the planned log-scales and
+// opacity logit, slots 14-15 zeroed, fresh Adam moments and the donor's SH
+// with fresh SH moments. A split also gives the donor the same log-scales
+// and clears the donor's moments and SH moments.
+struct RefineOp {
+  int64_t target, donor;
+  double jit[3];
+  float ls[3];
+  float opa;
+  int32_t split;
+  int32_t pad;
+};
+
+// q[i] = (opacity logit, log-scale x, y, z) for i < n
+void launch_refine_gather(hipStream_t st, int64_t n, int64_t cap, const float4* P, float4* q);
+void launch_refine_apply(hipStream_t st, int deg, bool mixed, int64_t cap, const RefineOp* ops, int64_t count,
+                         float4* P, void* M, void* V, void* SH, void* SHM, void* SHV);
+
 }  // namespace gs

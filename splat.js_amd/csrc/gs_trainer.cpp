@@ -117,6 +117,7 @@ struct gs_trainer {
   std::vector<char> excluded;
   uint32_t rng = 0;
   int lastCam = -1;
+  int lastNumTiles = 0;   // tiles of the last raster (refine's maxTile)
   double overflowTotal = 0.0;
   BinOut lastBin{};
 
@@ -285,6 +286,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
     a.mode = h->o.sortMode;
     a.tilesX = cam.tilesX;
     a.numTiles = cam.numTiles;
+    h->lastNumTiles = cam.numTiles;
     a.n = n;
     a.cap = (uint64_t)h->entriesCap;
     a.dN = h->dN;
@@ -839,6 +841,138 @@ int gs_get_info(gs_trainer_t* h, gs_info* out) {
     out->lastKeys = nk;
     out->overflowKeys = (int64_t)(h->overflowTotal + s[3]);
   }
+  return GS_OK;
+}
+
+// ---- refine (trainer.js:666-794) ----
+namespace {
+
+struct JsRng {   // makeRng (src/sfm/geometry.js:750-758): mulberry32, doubles in [0, 1)
+  uint32_t a;
+  double operator()() { return (double)mulberry32(a) / 4294967296.0; }
+};
+
+}  // namespace
+
+int gs_refine(gs_trainer_t* h, const gs_refine_opts* ro, gs_refine_result* out) {
+  if (!h || !ro || !out || ro->size != (int32_t)sizeof(gs_refine_opts)) return GS_E_ARG;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  memset(out, 0, sizeof(*out));
+  HIPCK(hipSetDevice(h->o.device));
+  HIPCK(hipStreamSynchronize(h->st));
+  const int64_t n = h->n;
+
+  // telemetry (trainer.js:784-792): most keys in one tile of the last raster,
+  // and the cumulative entry-budget drops
+  if (h->lastNumTiles > 0) {
+    std::vector<uint32_t> rg(2 * (size_t)h->lastNumTiles);
+    HIPCK(hipMemcpy(rg.data(), h->ranges, rg.size() * 4, hipMemcpyDeviceToHost));
+    for (int t = 0; t < h->lastNumTiles; t++) out->maxTile = std::max<int64_t>(out->maxTile, rg[2 * t + 1] - rg[2 * t]);
+  }
+  {
+    double s[4];
+    HIPCK(hipMemcpy(s, h->stats, 32, hipMemcpyDeviceToHost));
+    out->overflow = (int64_t)(h->overflowTotal + s[3]);
+  }
+
+  // opacity logit and log-scales of every Gaussian (16 B each)
+  std::vector<float> q(4 * (size_t)n);
+  float4* dq = nullptr;
+  HIPCK(hipMalloc(&dq, (size_t)std::max<int64_t>(n, 1) * 16));
+  launch_refine_gather(h->st, n, h->cap, h->P, dq);
+  hipError_t e = hipMemcpyAsync(q.data(), dq, q.size() * 4, hipMemcpyDeviceToHost, h->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+  (void)hipFree(dq);
+  if (e != hipSuccess) return hipfail(h, "refine gather", e);
+
+  // dead and donor sets in index order (trainer.js:687-694)
+  std::vector<int64_t> dead, donors;
+  for (int64_t i = 0; i < n; i++) {
+    const double o = 1.0 / (1.0 + std::exp(-(double)q[4 * i]));
+    if (o < 0.02) dead.push_back(i);
+    else if (o > 0.4) donors.push_back(i);
+  }
+  out->n = n;
+  if (donors.size() < 16) return GS_OK;
+  const int64_t moveCap = (int64_t)std::ceil((double)n * 0.05);
+  if ((int64_t)dead.size() > moveCap) dead.resize((size_t)moveCap);
+
+  JsRng rng{ro->seed};
+  auto gauss = [&]() {
+    const double u = std::max(1e-9, rng()), w = rng();
+    return std::sqrt(-2.0 * std::log(u)) * std::cos(2.0 * M_PI * w);
+  };
+  auto meanLogScale = [&](int64_t i) {
+    return ((double)q[4 * i + 1] + (double)q[4 * i + 2] + (double)q[4 * i + 3]) / 3.0;
+  };
+  // largest donors first, stable (Array.prototype.sort), top max(16, |donors|/4)
+  std::vector<int64_t> bigDonors(donors);
+  std::stable_sort(bigDonors.begin(), bigDonors.end(),
+                   [&](int64_t a, int64_t b) { return meanLogScale(b) - meanLogScale(a) < 0.0; });
+  bigDonors.resize(std::min(bigDonors.size(), std::max<size_t>(16, donors.size() >> 2)));
+
+  std::vector<RefineOp> ops;
+  const double shrink = std::log(1.0 / 1.6), cloneLs = std::log(0.85);
+  const float cloneOpa = (float)std::log(0.25 / 0.75);
+  auto spawnAt = [&](int64_t target, bool allowSplit) {
+    const bool doSplit = allowSplit && bigDonors.size() > 16 && rng() < 0.7;
+    int64_t don;
+    if (doSplit) {
+      const size_t di = (size_t)(int32_t)(rng() * (double)bigDonors.size());
+      don = bigDonors[di];
+      bigDonors[di] = bigDonors.back();
+      bigDonors.pop_back();   // a splat splits at most once per call
+    } else {
+      don = donors[(size_t)(int32_t)(rng() * (double)donors.size())];
+    }
+    RefineOp op{};
+    op.target = target;
+    op.donor = don;
+    const double s = std::exp(meanLogScale(don));
+    for (int c = 0; c < 3; c++) op.jit[c] = gauss() * s * 0.7;
+    for (int c = 0; c < 3; c++) op.ls[c] = q[4 * don + 1 + c];
+    if (doSplit) {
+      for (int c = 0; c < 3; c++) {
+        op.ls[c] = (float)((double)op.ls[c] + shrink);
+        q[4 * don + 1 + c] = (float)((double)q[4 * don + 1 + c] + shrink);
+      }
+      op.opa = q[4 * don];
+      op.split = 1;
+    } else {
+      for (int c = 0; c < 3; c++) op.ls[c] = (float)((double)op.ls[c] + cloneLs);
+      op.opa = cloneOpa;
+    }
+    ops.push_back(op);
+  };
+  for (int64_t i : dead) spawnAt(i, false);
+  const int64_t limit = std::min<int64_t>(h->cap, ro->growLimit > 0 ? ro->growLimit : h->cap);
+  const double growUntil = ro->growUntil >= 0.0 ? ro->growUntil : 0.75 * h->horizon;
+  const double growRate = ro->growRate >= 0.0f ? (double)ro->growRate : 0.15;
+  const int64_t grown = (double)h->iter < growUntil
+      ? std::max<int64_t>(0, std::min<int64_t>((int64_t)std::ceil((double)n * growRate), limit - n)) : 0;
+  for (int64_t k = 0; k < grown; k++) spawnAt(n + k, true);
+
+  if (!ops.empty()) {
+    RefineOp* dops = nullptr;
+    HIPCK(hipMalloc(&dops, ops.size() * sizeof(RefineOp)));
+    e = hipMemcpy(dops, ops.data(), ops.size() * sizeof(RefineOp), hipMemcpyHostToDevice);
+    if (e == hipSuccess) {
+      launch_refine_apply(h->st, h->shDeg, h->mixed, h->cap, dops, (int64_t)ops.size(), h->P, h->M, h->V, h->SH,
+                          h->SHM, h->SHV);
+      e = hipGetLastError();
+      if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+    }
+    (void)hipFree(dops);
+    if (e != hipSuccess) return hipfail(h, "refine apply", e);
+  }
+  if (grown > 0) {
+    h->n = n + grown;
+    const uint32_t n32 = (uint32_t)h->n;
+    HIPCK(hipMemcpy(h->dN, &n32, 4, hipMemcpyHostToDevice));
+  }
+  out->moved = (int64_t)dead.size();
+  out->grown = grown;
+  out->n = h->n;
   return GS_OK;
 }
 

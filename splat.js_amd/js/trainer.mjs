@@ -9,8 +9,9 @@
 //   holdout, excluded, lastCam, horizon, shK, stride, tileGrad, opts) and
 //   device.queue.onSubmittedWorkDone().
 // Differences: renderView writes into a caller Float32Array (w*h*4) instead
-// of a canvas context (Node has no canvas); refine() is not provided by this
-// build. Syntax stays ES2019 (no ?? or ?.) for Node 12.
+// of a canvas context (Node has no canvas); refine(rng) seeds the reference's
+// mulberry32 from one draw of rng. Syntax stays ES2019 (no ?? or ?.) for
+// Node 12.
 import { createRequire } from 'module';
 
 const require = createRequire(import.meta.url);
@@ -46,6 +47,7 @@ export class GSTrainer {
     this.stride = STRIDE;
     this.pixelsSeen = 0;
     this.lastRefine = 0;
+    this.growLimit = 0;   // trainer.js:777: soft ceiling below cap (0 = cap)
     this.excluded = null;
     const self = this;
     // session.js:520, :542 wait on the queue between frames
@@ -123,8 +125,20 @@ export class GSTrainer {
     return addon.evalCamPsnr(this._h, ci);
   }
 
-  async refine() {
-    throw new Error('refine() (MCMC-lite relocation) is not part of this build');
+  /** trainer.js:666-794. The plan's random stream is the reference's
+   *  makeRng (mulberry32) seeded from one draw of `rng`; the reference
+   *  consumes `rng` itself, so the same call draws different (equally
+   *  distributed) numbers. growRate / growUntil come from opts as in the
+   *  reference; growLimit is the trainer field. */
+  async refine(rng = Math.random) {
+    const o = this.opts;
+    const r = addon.refine(this._h, {
+      seed: Math.floor(rng() * 4294967296) >>> 0,
+      growRate: o.growRate === undefined ? -1 : o.growRate,
+      growUntil: o.growUntil === undefined ? -1 : o.growUntil,
+      growLimit: this.growLimit || 0,
+    });
+    return { moved: r.moved, grown: r.grown, n: r.n, maxTile: r.maxTile, overflow: r.overflow };
   }
 
   info() { return addon.info(this._h); }

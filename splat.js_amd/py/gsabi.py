@@ -76,6 +76,7 @@ SIGNATURES = {
     "gs_set_iter": ([C.c_void_p, C.c_int64], C.c_int),
     "gs_get_info": ([C.c_void_p, C.c_void_p], C.c_int),
     "gs_sync": ([C.c_void_p], C.c_int),
+    "gs_refine": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "gs_last_error": ([C.c_void_p], C.c_char_p),
     "gs_step_debug": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
     "gs_profile": ([C.c_void_p, C.c_int32], C.c_int),
@@ -106,6 +107,16 @@ def _p(a):
 
 class GsError(RuntimeError):
     pass
+
+
+class GsRefineOpts(C.Structure):
+    _fields_ = [("size", C.c_int32), ("seed", C.c_uint32), ("growRate", C.c_float), ("growUntil", C.c_double),
+                ("growLimit", C.c_int64)]
+
+
+class GsRefineResult(C.Structure):
+    _fields_ = [("moved", C.c_int64), ("grown", C.c_int64), ("n", C.c_int64), ("maxTile", C.c_int64),
+                ("overflow", C.c_int64)]
 
 
 def default_opts(**kw):
@@ -249,6 +260,16 @@ class GSTrainer:
 
     def sync(self):
         self._ck(lib().gs_sync(self._h))
+
+    def refine(self, seed=0, growRate=-1.0, growUntil=-1.0, growLimit=0):
+        """trainer.js:666-794 with the reference's mulberry32 seeded by `seed`;
+        negative growRate / growUntil and growLimit 0 select the reference's
+        defaults (0.15, 0.75 * horizon, cap)."""
+        o = GsRefineOpts(C.sizeof(GsRefineOpts), int(seed) & 0xFFFFFFFF, float(growRate), float(growUntil),
+                         int(growLimit))
+        r = GsRefineResult()
+        self._ck(lib().gs_refine(self._h, C.byref(o), C.byref(r)))
+        return {"moved": r.moved, "grown": r.grown, "n": r.n, "maxTile": r.maxTile, "overflow": r.overflow}
 
     def stream(self):
         return lib().gs_stream(self._h)

@@ -53,6 +53,8 @@ def test_struct_layouts_match_the_header(tmp_path):
         "gs_info": [f[0] for f in gsabi.GsInfo._fields_],
         "gs_debug_out": [f[0] for f in gsabi.GsDebugOut._fields_],
         "gs_kernel_time": [f[0] for f in gsabi.GsKernelTime._fields_],
+        "gs_refine_opts": [f[0] for f in gsabi.GsRefineOpts._fields_],
+        "gs_refine_result": [f[0] for f in gsabi.GsRefineResult._fields_],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for s, fs in fields.items():
@@ -65,7 +67,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     subprocess.check_call(["gcc", "-o", str(exe), str(probe)])
     got = dict(l.split() for l in subprocess.check_output([str(exe)], text=True).splitlines())
     py = {"gs_opts": gsabi.GsOpts, "gs_camera": gsabi.GsCamera, "gs_image": gsabi.GsImage,
-          "gs_info": gsabi.GsInfo, "gs_debug_out": gsabi.GsDebugOut, "gs_kernel_time": gsabi.GsKernelTime}
+          "gs_info": gsabi.GsInfo, "gs_debug_out": gsabi.GsDebugOut, "gs_kernel_time": gsabi.GsKernelTime,
+          "gs_refine_opts": gsabi.GsRefineOpts, "gs_refine_result": gsabi.GsRefineResult}
     for s, cls in py.items():
         assert int(got[s]) == C.sizeof(cls), s
         for f, _ in cls._fields_:
