@@ -123,6 +123,11 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh,
  * cam >= 0 trains on that camera. Enqueues only. */
 int gs_step(gs_trainer_t* h, int32_t cam);
 
+/* `count` training steps enqueued in one call, exactly as `count` gs_step
+ * calls with cams[k] (k-th step; < 0 picks with the seeded RNG). No
+ * counterpart in the reference, whose host submits one step per stepOnce. */
+int gs_step_batch(gs_trainer_t* h, const int32_t* cams, int32_t count);
+
 /* GSTrainer.readLoss() (trainer.js:615-631): mean squared error per valid
  * pixel-channel since the last call, then resets. *has = 0 is the
  * reference's null (no pixels seen). */

@@ -710,6 +710,15 @@ int gs_step(gs_trainer_t* h, int32_t cam) {
   return step_impl(h, cam, nullptr);
 }
 
+int gs_step_batch(gs_trainer_t* h, const int32_t* cams, int32_t count) {
+  if (!h || count < 0 || (count > 0 && !cams)) return GS_E_ARG;
+  for (int32_t k = 0; k < count; k++) {
+    const int rc = step_impl(h, cams[k], nullptr);
+    if (rc != GS_OK) return rc;
+  }
+  return GS_OK;
+}
+
 int gs_step_debug(gs_trainer_t* h, int32_t cam, gs_debug_out* out) {
   if (!h || !out) return GS_E_ARG;
   return step_impl(h, cam, out);

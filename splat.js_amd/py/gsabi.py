@@ -77,6 +77,7 @@ SIGNATURES = {
     "gs_get_info": ([C.c_void_p, C.c_void_p], C.c_int),
     "gs_sync": ([C.c_void_p], C.c_int),
     "gs_refine": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "gs_step_batch": ([C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
     "gs_last_error": ([C.c_void_p], C.c_char_p),
     "gs_step_debug": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
     "gs_profile": ([C.c_void_p, C.c_int32], C.c_int),
@@ -200,6 +201,11 @@ class GSTrainer:
         self._ck(lib().gs_step(self._h, int(cam)))
 
     step_once = stepOnce
+
+    def stepBatch(self, cams):
+        """len(cams) steps in one call, as that many stepOnce(cams[k]) calls."""
+        a = np.ascontiguousarray(cams, np.int32)
+        self._ck(lib().gs_step_batch(self._h, _p(a), int(a.size)))
 
     def readLoss(self):
         mse = C.c_double()

@@ -141,3 +141,29 @@ def config2(n=1_000_000, seed=0, views=64, W=1600, H=900, f=1100.0, radius=8.0):
     """1M Gaussians in a 10x10x10 box, ring of views at radius 8, 1600x900."""
     p, sh = gaussians(n, seed, (-5, -5, -5), (5, 5, 5))
     return p, sh, ring_cameras(views, radius, W, H, f)
+
+
+# config 4: eight views of different sizes in one trainer
+CONFIG4_SIZES = [(800, 450), (960, 540), (1024, 576), (1280, 720), (1366, 768), (1600, 900), (1760, 990),
+                 (1920, 1080)]
+
+
+def config4(n=1_000_000, seed=0, radius=8.0, sizes=None):
+    """Config 2's Gaussians seen from 8 ring views of sizes 800x450 .. 1920x1080
+    (focal 1100 * W / 1600, principal point at the centre)."""
+    p, sh = gaussians(n, seed, (-5, -5, -5), (5, 5, 5))
+    sizes = sizes or CONFIG4_SIZES
+    cams = []
+    for k, (W, H) in enumerate(sizes):
+        a = 2 * math.pi * k / len(sizes)
+        R, t = look_at((radius * math.sin(a), 0.0, -radius * math.cos(a)))
+        cams.append({"imgIdx": k, "R": R, "t": t, "f": 1100.0 * W / 1600.0, "cx": W / 2, "cy": H / 2,
+                     "w": W, "h": H})
+    return p, sh, cams
+
+
+def config5(n=3_000_000, seed=0, views=4, W=1920, H=1080, radius=8.0):
+    """Stress: 3M Gaussians, SH degree 3 (45 coefficients), log-scales
+    N(-3, 1) (large splats, ~250M keys per view), 4 views at 1920x1080."""
+    p, sh = gaussians(n, seed, (-5, -5, -5), (5, 5, 5), ls_mu=-3.0, ls_sd=1.0, sh_deg=3)
+    return p, sh, ring_cameras(views, radius, W, H, 1100.0 * W / 1600.0)

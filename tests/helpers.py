@@ -15,7 +15,7 @@ def f32(x):
 def adam_for_iter(it, radius, posLr=0.0, horizon=60000, minScale=1e-4, opacityReg=0.01, shLr=1.25e-4):
     """AdamHP for the step whose Adam t = it (after iter++)."""
     r = f32(radius)
-    base = float(posLr) if posLr > 0 else 3e-4 * r
+    base = f32(posLr) if posLr > 0 else 3e-4 * r    # gs_opts.posLr is an f32 field
     pos = f32(base * 0.01 ** min(1.0, it / (0.75 * horizon)))
     lr = [pos] * 3 + [5e-3] * 3 + [1e-3] * 4 + [1.5e-2] * 3 + [2.5e-2, 0, 0]
     return O.adam_hp(lr, t=it, minLS=f32(math.log(r * f32(minScale))), maxLS=f32(math.log(r * 0.05)),

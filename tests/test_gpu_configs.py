@@ -1,0 +1,117 @@
+"""The benchmark's other configurations as GPU test cases.
+
+Config 4: eight views of different sizes (800x450 .. 1920x1080) in one
+trainer. Every step on every view matches the f32 restatement bit for bit
+(all intermediates and the updated params/SH), and the batched entry point
+gs_step_batch gives the same state as the step-by-step calls.
+
+Config 5: the stress configuration (3M Gaussians, SH degree 3, large splats,
+4 views at 1920x1080, ~281M keys on the first view) is far beyond what the
+CPU restatement finishes in seconds, so it is checked through size-independent
+properties: the key count equals the sum of the per-Gaussian tile counts
+with no key dropped, the tile ranges partition the keys, and two trainers
+produce bit-identical params after the same steps (fixed-point sums make the
+step deterministic)."""
+import numpy as np
+import pytest
+
+import gsabi
+import scenes
+from helpers import OracleTrainer
+from test_gpu_parity import _check_step
+
+
+def _targets(gp, gsh, cams, deg, n, entries_cap=0):
+    """Targets rendered on the GPU from a second Gaussian set (any image
+    works as a target; both sides read the same pixels)."""
+    W = max(c["w"] for c in cams)
+    H = max(c["h"] for c in cams)
+    gt = gsabi.GSTrainer(shDeg=deg, maxSplats=n, entriesCap=entries_cap)
+    blank = [{"tw": W, "th": H, "rgb": np.zeros(W * H * 3, np.float32)}]
+    gt.setup({"data": gp, "n": n, "sh": gsh}, [dict(cams[0], imgIdx=0, w=W, h=H)], blank, W, H, 8.0)
+    images = []
+    for c in cams:
+        img = gt.renderView(c)
+        images.append({"tw": c["w"], "th": c["h"], "rgb": np.clip(img[..., :3], 0, 1).reshape(-1).copy()})
+    gt.destroy()
+    return images, W, H
+
+
+@pytest.mark.gpu
+def test_config4_mixed_view_sizes_bit_exact():
+    n = 30000
+    p, sh, cams = scenes.config4(n=n, seed=0)
+    gp, gsh, _ = scenes.config4(n=n, seed=1)
+    images, W, H = _targets(gp, gsh, cams, 2, n)
+    t = gsabi.GSTrainer(shDeg=2, maxSplats=n, posLr=1.6e-4)
+    t.setup({"data": p, "n": n, "sh": sh}, cams, images, W, H, 8.0)
+    ref = OracleTrainer(p, sh, cams, images, 8.0, opts={"posLr": 1.6e-4}, shDeg=2)
+    t.iter = ref.iter = 2000
+    acc = np.zeros(3)
+    for ci in range(len(cams)):
+        g = t.step_debug(ci)
+        r = ref.step(ci)
+        acc = acc + r["stats"]
+        nt = ((cams[ci]["w"] + 15) // 16) * ((cams[ci]["h"] + 15) // 16)
+        _check_step(g, r, nt, f"view {ci} {cams[ci]['w']}x{cams[ci]['h']}", stats_acc=acc)
+        out = t.readGaussians()
+        np.testing.assert_array_equal(out["data"].view(np.uint32), ref.P.view(np.uint32), err_msg=f"view {ci}")
+        np.testing.assert_array_equal(out["sh"].view(np.uint32), ref.SH.view(np.uint32), err_msg=f"view {ci}")
+    t.destroy()
+
+    # the batched entry point: the same 8 + 8 steps in two calls
+    seq = list(range(8)) + list(range(7, -1, -1))
+    a = gsabi.GSTrainer(shDeg=2, maxSplats=n, posLr=1.6e-4)
+    b = gsabi.GSTrainer(shDeg=2, maxSplats=n, posLr=1.6e-4)
+    for tr in (a, b):
+        tr.setup({"data": p, "n": n, "sh": sh}, cams, images, W, H, 8.0)
+        tr.iter = 2000
+    for ci in seq:
+        a.stepOnce(ci)
+    b.stepBatch(seq[:8])
+    b.stepBatch(seq[8:])
+    oa, ob = a.readGaussians(), b.readGaussians()
+    np.testing.assert_array_equal(oa["data"].view(np.uint32), ob["data"].view(np.uint32))
+    np.testing.assert_array_equal(oa["sh"].view(np.uint32), ob["sh"].view(np.uint32))
+    # the loss statistics are double atomics (summation order varies)
+    la, lb = a.readLoss(), b.readLoss()
+    assert abs(la - lb) <= 1e-12 * abs(la)
+    a.destroy()
+    b.destroy()
+
+
+@pytest.mark.gpu
+def test_config5_stress_properties():
+    n = 3_000_000
+    p, sh, cams = scenes.config5(n=n, seed=0)
+    gp, gsh, _ = scenes.config5(n=n, seed=1)
+    images, W, H = _targets(gp, gsh, cams, 3, n, entries_cap=400_000_000)
+    trainers = []
+    for _ in range(2):
+        t = gsabi.GSTrainer(shDeg=3, maxSplats=n, entriesCap=400_000_000)
+        t.setup({"data": p, "n": n, "sh": sh}, cams, images, W, H, 8.0)
+        t.iter = 3000
+        trainers.append(t)
+    a, b = trainers
+    for ci in range(len(cams)):
+        d = a.step_debug(ci, want=("cnt", "tileRanges"))
+        b.stepOnce(ci)
+        info = a.info()
+        total = int(d["cnt"].astype(np.int64).sum())
+        print(f"view {ci}: {total} keys, {info.lastVisible} visible")
+        # the stress regime: ~281M keys on the initial view (the count the
+        # restatement's projection gives); the first Adam step clamps the
+        # largest log-scales to ln(0.05 r) (trainer.js:338-341), so later
+        # views hold fewer
+        assert total > (250_000_000 if ci == 0 else 100_000_000)
+        assert d["numKeys"] == total == info.lastKeys
+        tr = d["tileRanges"].astype(np.int64)
+        assert tr[0, 0] == 0 and tr[-1, 1] == total
+        assert np.all(tr[1:, 0] == tr[:-1, 1]) and np.all(tr[:, 1] >= tr[:, 0])
+    assert a.info().overflowKeys == 0
+    oa, ob = a.readGaussians(), b.readGaussians()
+    np.testing.assert_array_equal(oa["data"].view(np.uint32), ob["data"].view(np.uint32))
+    np.testing.assert_array_equal(oa["sh"].view(np.uint32), ob["sh"].view(np.uint32))
+    assert np.isfinite(oa["data"]).all() and np.isfinite(oa["sh"]).all()
+    a.destroy()
+    b.destroy()
