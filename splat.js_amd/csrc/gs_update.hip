@@ -207,64 +207,6 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
     for (int k = 0; k < NC; k++) u.dbgShGrad[(int64_t)NC * i + k] = shG[k];
   }
 
-  // ---- Adam over the 16 param slots (shaders.js:1000-1035) ----
-  float m[16], vv[16];
-  if constexpr (kMixed) {
-    const uint4* M16 = (const uint4*)u.M;
-    const uint4* V16 = (const uint4*)u.V;
-    unpack_bf8(M16[i], m); unpack_bf8(M16[cap + i], m + 8);
-    unpack_bf8(V16[i], vv); unpack_bf8(V16[cap + i], vv + 8);
-  } else {
-    load_params((const float4*)u.M, cap, i, m);
-    load_params((const float4*)u.V, cap, i, vv);
-  }
-#pragma unroll
-  for (int j = 0; j < 14; j++) {
-    const float lr = hp.lr[j];
-    if (lr == 0.0f) continue;
-    float gr = gF[j];
-    if (!(fabsf(gr) < 1e18f)) gr = 0.0f;
-    if (j == 13) {
-      const float sg = gs_sigmoid_clamped(p[j]);
-      gr = gr + hp.reg * sg * (1.0f - sg);
-    }
-    const float mm = hp.b1 * m[j] + (1.0f - hp.b1) * gr;
-    const float vn = hp.b2 * vv[j] + (1.0f - hp.b2) * gr * gr;
-    m[j] = mm;
-    vv[j] = vn;
-    const float mh = mm / hp.bc1;
-    const float vh = vn / hp.bc2;
-    float pn = p[j] - lr * mh / (sqrtf(vh) + hp.eps);
-    if (j >= 3 && j <= 5) pn = clampf(pn, hp.minLS, hp.maxLS);
-    if (j >= 10 && j <= 13) pn = clampf(pn, -hp.maxLogit, hp.maxLogit);
-    p[j] = pn;
-  }
-#pragma unroll
-  for (int c = 0; c < 4; c++) u.P[c * cap + i] = make_float4(p[4 * c], p[4 * c + 1], p[4 * c + 2], p[4 * c + 3]);
-  if constexpr (kMixed) {
-    // moments stored in bf16 with stochastic rounding (streams 0: m, 1: v);
-    // slots Adam skipped re-encode to their own bits
-    const uint32_t seed = gs_sr_seed((uint64_t)i, hp.step);
-    uint32_t bm[16], bv[16];
-#pragma unroll
-    for (int j = 0; j < 16; j++) {
-      bm[j] = gs_f2bf_sr(m[j], gs_sr_bits(seed, 0u * 64u + (uint32_t)j));
-      bv[j] = gs_f2bf_sr(vv[j], gs_sr_bits(seed, 1u * 64u + (uint32_t)j));
-    }
-    uint4* M16 = (uint4*)u.M;
-    uint4* V16 = (uint4*)u.V;
-    M16[i] = pack_bf8(bm); M16[cap + i] = pack_bf8(bm + 8);
-    V16[i] = pack_bf8(bv); V16[cap + i] = pack_bf8(bv + 8);
-  } else {
-    float4* M = (float4*)u.M;
-    float4* V = (float4*)u.V;
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      M[c * cap + i] = make_float4(m[4 * c], m[4 * c + 1], m[4 * c + 2], m[4 * c + 3]);
-      V[c * cap + i] = make_float4(vv[4 * c], vv[4 * c + 1], vv[4 * c + 2], vv[4 * c + 3]);
-    }
-  }
-
   // ---- Adam over the SH coefficients (shaders.js:963-981) ----
   if constexpr (kDeg > 0 && kMixed) {
     // bf16 coefficients and moments (streams 2: coefficient, 3: m, 4: v);
@@ -327,6 +269,71 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
       SH[c * cap + i] = make_float4(ss[0], ss[1], ss[2], ss[3]);
       SHM[c * cap + i] = make_float4(ms[0], ms[1], ms[2], ms[3]);
       SHV[c * cap + i] = make_float4(vs[0], vs[1], vs[2], vs[3]);
+    }
+  }
+
+  // ---- Adam over the 16 param slots (shaders.js:1000-1035) ----
+  // One storage plane at a time (4 f32 or 8 bf16 slots), so only that
+  // plane's moments are live; the SH pass above has already retired the SH
+  // registers.
+  auto adam_slot = [&](int j, float& mj, float& vj) {
+    const float lr = hp.lr[j];
+    if (j >= 14 || lr == 0.0f) return;
+    float gr = gF[j];
+    if (!(fabsf(gr) < 1e18f)) gr = 0.0f;
+    if (j == 13) {
+      const float sg = gs_sigmoid_clamped(p[j]);
+      gr = gr + hp.reg * sg * (1.0f - sg);
+    }
+    const float mm = hp.b1 * mj + (1.0f - hp.b1) * gr;
+    const float vn = hp.b2 * vj + (1.0f - hp.b2) * gr * gr;
+    mj = mm;
+    vj = vn;
+    const float mh = mm / hp.bc1;
+    const float vh = vn / hp.bc2;
+    float pn = p[j] - lr * mh / (sqrtf(vh) + hp.eps);
+    if (j >= 3 && j <= 5) pn = clampf(pn, hp.minLS, hp.maxLS);
+    if (j >= 10 && j <= 13) pn = clampf(pn, -hp.maxLogit, hp.maxLogit);
+    p[j] = pn;
+  };
+  if constexpr (kMixed) {
+    // moments stored in bf16 with stochastic rounding (streams 0: m, 1: v);
+    // slots Adam skips re-encode to their own bits
+    uint4* M16 = (uint4*)u.M;
+    uint4* V16 = (uint4*)u.V;
+    const uint32_t seed = gs_sr_seed((uint64_t)i, hp.step);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      float m[8], vv[8];
+      unpack_bf8(M16[h * cap + i], m);
+      unpack_bf8(V16[h * cap + i], vv);
+      uint32_t bm[8], bv[8];
+#pragma unroll
+      for (int l = 0; l < 8; l++) {
+        const int j = 8 * h + l;
+        adam_slot(j, m[l], vv[l]);
+        bm[l] = gs_f2bf_sr(m[l], gs_sr_bits(seed, 0u * 64u + (uint32_t)j));
+        bv[l] = gs_f2bf_sr(vv[l], gs_sr_bits(seed, 1u * 64u + (uint32_t)j));
+      }
+      M16[h * cap + i] = pack_bf8(bm);
+      V16[h * cap + i] = pack_bf8(bv);
+#pragma unroll
+      for (int c = 2 * h; c < 2 * h + 2; c++)
+        u.P[c * cap + i] = make_float4(p[4 * c], p[4 * c + 1], p[4 * c + 2], p[4 * c + 3]);
+    }
+  } else {
+    float4* M = (float4*)u.M;
+    float4* V = (float4*)u.V;
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const float4 m4 = M[c * cap + i], v4 = V[c * cap + i];
+      float m[4] = {m4.x, m4.y, m4.z, m4.w};
+      float vv[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+      for (int l = 0; l < 4; l++) adam_slot(4 * c + l, m[l], vv[l]);
+      u.P[c * cap + i] = make_float4(p[4 * c], p[4 * c + 1], p[4 * c + 2], p[4 * c + 3]);
+      M[c * cap + i] = make_float4(m[0], m[1], m[2], m[3]);
+      V[c * cap + i] = make_float4(vv[0], vv[1], vv[2], vv[3]);
     }
   }
 }
