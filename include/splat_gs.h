@@ -156,6 +156,16 @@ int gs_set_iter(gs_trainer_t* h, int64_t iter);
 
 int gs_get_info(gs_trainer_t* h, gs_info* out);
 
+/* Standard 3DGS binary PLY of the current Gaussians (the reference's
+ * gaussiansToPly, src/io/ply.js:47-100: x y z, zero normals, f_dc, f_rest
+ * channel-major, pre-sigmoid opacity, log-scales, unit quaternion w x y z).
+ * bake != 0 first applies bakeOpacityCompensation (ply.js:12-37) as
+ * Session.exportPlyBlob does (session.js:637-645): focal of camera 0, the
+ * positions of all training cameras. The rows are built on the device in
+ * f64 (the reference's JS numbers) and stored as f32. With buf == NULL only
+ * *size is set; otherwise cap must be at least that size. */
+int gs_export_ply(gs_trainer_t* h, int32_t bake, void* buf, int64_t cap, int64_t* size);
+
 /* GSTrainer.refine(rng) (trainer.js:666-794), MCMC-lite: relocate dead
  * splats (opacity < 0.02, at most ceil(5% n)) onto jittered clones of
  * donors (opacity > 0.4), then grow by ceil(growRate * n) (until growUntil,

@@ -36,6 +36,7 @@ struct Api {
   int (*get_info)(gs_trainer_t*, gs_info*);
   int (*sync)(gs_trainer_t*);
   int (*refine)(gs_trainer_t*, const gs_refine_opts*, gs_refine_result*);
+  int (*export_ply)(gs_trainer_t*, int32_t, void*, int64_t, int64_t*);
   const char* (*last_error)(gs_trainer_t*);
 };
 
@@ -80,6 +81,7 @@ bool load_api() {
   SYM(get_info, "gs_get_info");
   SYM(sync, "gs_sync");
   SYM(refine, "gs_refine");
+  SYM(export_ply, "gs_export_ply");
   SYM(last_error, "gs_last_error");
 #undef SYM
   g_api.so = so;
@@ -463,6 +465,23 @@ napi_value Refine(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// exportPly(h, bake) -> Buffer holding the binary PLY file
+napi_value ExportPly(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  gs_trainer_t* h = get_handle(env, argv[0]);
+  bool bake = true;
+  if (argc > 1) napi_get_value_bool(env, argv[1], &bake);
+  int64_t size = 0;
+  if (!check(env, h, g_api.export_ply(h, bake ? 1 : 0, nullptr, 0, &size))) return nullptr;
+  void* data = nullptr;
+  napi_value out;
+  NAPI_CALL(env, napi_create_buffer(env, (size_t)size, &data, &out));
+  if (!check(env, h, g_api.export_ply(h, bake ? 1 : 0, data, size, &size))) return nullptr;
+  return out;
+}
+
 // loaded() -> true when libsplatgs.so opened and every symbol resolved
 napi_value Loaded(napi_env env, napi_callback_info) {
   napi_value r;
@@ -485,7 +504,7 @@ napi_value Init(napi_env env, napi_value exports) {
       {"readLoss", ReadLoss},       {"render", Render},       {"readGaussians", ReadGaussians},
       {"evalCamPsnr", EvalCamPsnr}, {"setHoldout", SetHoldout}, {"setExcluded", SetExcluded},
       {"setIter", SetIter},         {"info", Info},           {"sync", Sync},
-      {"refine", Refine},
+      {"refine", Refine},           {"exportPly", ExportPly},
       {"loaded", Loaded},           {"loadError", LoadError},
   };
   for (const auto& f : fns) {

@@ -985,6 +985,52 @@ int gs_refine(gs_trainer_t* h, const gs_refine_opts* ro, gs_refine_result* out) 
   return GS_OK;
 }
 
+// ---- PLY export (src/io/ply.js; session.js:637-645) ----
+int gs_export_ply(gs_trainer_t* h, int32_t bake, void* buf, int64_t cap, int64_t* size) {
+  if (!h || !size) return GS_E_ARG;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  const int64_t n = h->n;
+  const int W = 17 + h->shC;
+  std::string hdr = "ply\nformat binary_little_endian 1.0\nelement vertex " + std::to_string(n) + "\n";
+  const char* head[] = {"x", "y", "z", "nx", "ny", "nz", "f_dc_0", "f_dc_1", "f_dc_2"};
+  for (const char* p : head) hdr += std::string("property float ") + p + "\n";
+  for (int k = 0; k < h->shC; k++) hdr += "property float f_rest_" + std::to_string(k) + "\n";
+  const char* tail[] = {"opacity", "scale_0", "scale_1", "scale_2", "rot_0", "rot_1", "rot_2", "rot_3"};
+  for (const char* p : tail) hdr += std::string("property float ") + p + "\n";
+  hdr += "end_header\n";
+  const int64_t body = n * (int64_t)W * 4;
+  *size = (int64_t)hdr.size() + body;
+  if (!buf) return GS_OK;
+  if (cap < *size) return fail(h, GS_E_ARG, "exportPly: buffer smaller than the file");
+  HIPCK(hipSetDevice(h->o.device));
+  // camera centres -(R^T t) in JS numbers, stored through a Float32Array
+  // (session.js:32-38, :642)
+  const int nc = (int)h->cams.size();
+  std::vector<float> cp(3 * (size_t)nc);
+  for (int c = 0; c < nc; c++) {
+    const gs_camera& k = h->cams[c].c;
+    for (int a = 0; a < 3; a++)
+      cp[3 * c + a] = (float)(-(((double)k.R[a] * k.t[0] + (double)k.R[3 + a] * k.t[1]) + (double)k.R[6 + a] * k.t[2]));
+  }
+  float* dCam = nullptr;
+  float* dOut = nullptr;
+  HIPCK(hipMalloc(&dCam, cp.size() * 4 + 4));
+  hipError_t e = hipMalloc(&dOut, (size_t)std::max<int64_t>(body, 4));
+  if (e == hipSuccess) e = hipMemcpy(dCam, cp.data(), cp.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    launch_ply_rows(h->st, h->shDeg, h->mixed, n, h->cap, h->P, h->SH, bake ? 1 : 0, (double)h->cams[0].c.f, dCam,
+                    nc, dOut);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+  memcpy(buf, hdr.data(), hdr.size());
+  if (e == hipSuccess) e = hipMemcpy((char*)buf + hdr.size(), dOut, (size_t)body, hipMemcpyDeviceToHost);
+  (void)hipFree(dCam);
+  if (dOut) (void)hipFree(dOut);
+  if (e != hipSuccess) return hipfail(h, "exportPly", e);
+  return GS_OK;
+}
+
 int gs_sync(gs_trainer_t* h) {
   if (!h) return GS_E_ARG;
   HIPCK(hipStreamSynchronize(h->st));

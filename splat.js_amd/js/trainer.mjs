@@ -141,6 +141,11 @@ export class GSTrainer {
     return { moved: r.moved, grown: r.grown, n: r.n, maxTile: r.maxTile, overflow: r.overflow };
   }
 
+  /** Binary 3DGS PLY of the current Gaussians (src/io/ply.js
+   *  gaussiansToPly), built on the device; bake applies the Mip opacity
+   *  compensation as Session.exportPlyBlob does (session.js:637-645). */
+  exportPly(bake = true) { return addon.exportPly(this._h, !!bake); }
+
   info() { return addon.info(this._h); }
 
   destroy() {
