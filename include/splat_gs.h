@@ -67,6 +67,12 @@ typedef struct gs_opts {
   int32_t device;      /* HIP device ordinal, default 0 */
   int32_t precision;   /* GS_PREC_F32 (default), or GS_PREC_MIXED: SH coefficients and all Adam
                           moments stored in bf16 with stochastic rounding, f32 arithmetic */
+  int32_t camOpt;      /* camera pose + shared focal optimisation, default 0 (trainer.js:468):
+                          per-camera gradients each step (shaders.js:614-616, :834-858), one
+                          camera Adam step every 25 iterations after 1500 (trainer.js:476-537) */
+  int32_t expComp;     /* with camOpt, also per-camera exposure gain/bias, default 0
+                          (trainer.js:511); the exposure gradients come from the Charbonnier loss,
+                          so with GS_LOSS_L1_DSSIM they stay zero */
 } gs_opts;
 
 /* One training camera (trainer.js:95-100 cams[], :357-372): R row-major
@@ -80,6 +86,15 @@ typedef struct gs_camera {
   int32_t imgIdx;
   float g, b;
 } gs_camera;
+
+/* A camera's current state in double precision, the reference's camMeta
+ * entry (trainer.js:306, :518-533): pose, focal (original focal times the
+ * shared exp(logfScale)) and exposure log-gain g and bias b. */
+typedef struct gs_cam_state {
+  double R[9];
+  double t[3];
+  double f, g, b;
+} gs_cam_state;
 
 /* One target image (trainer.js:98 images[]): tw x th RGB floats in [0, 1],
  * r < 0 marks an invalid pixel (trainer.js:133). */
@@ -146,6 +161,17 @@ int gs_read_gaussians(gs_trainer_t* h, float* params, float* sh);
 
 /* GSTrainer.evalCamPsnr(ci) (trainer.js:542-579). */
 int gs_eval_cam_psnr(gs_trainer_t* h, int32_t ci, double* psnr);
+
+/* GSTrainer.evalCamPsnrRefined(ci, steps) (trainer.js:583-612): `steps`
+ * Adam steps on camera ci's pose alone (rotation lr 3e-4, translation
+ * 3e-4 * sceneRadius; exposure from zero, lr 5e-3, when expComp), each from
+ * one training-path pass with the Gaussians frozen, then the PSNR at the
+ * refined pose. The stored camera is not changed. Synchronous. */
+int gs_eval_cam_psnr_refined(gs_trainer_t* h, int32_t ci, int32_t steps, double* psnr);
+
+/* trainer.camMeta[ci] (trainer.js:306): the camera's current pose, focal and
+ * exposure, which camOpt updates. */
+int gs_get_camera(gs_trainer_t* h, int32_t ci, gs_cam_state* out);
 
 /* trainer.holdout = ci (trainer.js:309, :396-397); -1 clears. */
 int gs_set_holdout(gs_trainer_t* h, int32_t ci);
@@ -223,6 +249,12 @@ typedef struct gs_debug_out {
 /* One gs_step on camera `cam` that also copies the intermediates out
  * (synchronises). */
 int gs_step_debug(gs_trainer_t* h, int32_t cam, gs_debug_out* out);
+
+/* The camera gradient rows accumulated since the last camera Adam step or
+ * evaluation pass (trainer.js:173, bufCamGrad): (numCams + 1) x 8 i32 in
+ * x64 fixed point, [dw x,y,z, dt x,y,z, d log gain, d bias] per camera and
+ * d log f in slot 0 of the last row. count >= (numCams + 1) * 8. */
+int gs_read_cam_grads(gs_trainer_t* h, int32_t* out, int32_t count);
 
 /* Per-kernel timing over subsequent steps, measured with HIP events on the
  * trainer's stream. enable = 0 stops and keeps the totals. */

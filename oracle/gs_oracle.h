@@ -96,11 +96,14 @@ void orc_render_fwd_f64(const orc_opts* o, const orc_cam* cam, const double* pro
 
 /* R10: Charbonnier loss (delta 0.03). tgt = packed RGBA8 for this camera's
  * pixels (alpha 0 = invalid). gC = W*H*3 dL/dC (0 for invalid pixels),
- * valid = W*H flags. stats[0] = sum |err|^2, [1] = loss, [2] = valid px. */
+ * valid = W*H flags. stats[0] = sum |err|^2, [1] = loss, [2] = valid px.
+ * expFix (2 x i32, x64 fixed point, wrapping) / expSum (2 x f64), either
+ * may be NULL: exposure gradients d/d(log gain), d/d(bias) summed over the
+ * valid pixels. */
 void orc_loss_charb_f32(const orc_cam* cam, const float* img, const uint32_t* tgt,
-                        float* gC, uint8_t* valid, double* stats);
+                        float* gC, uint8_t* valid, double* stats, int32_t* expFix, double* expSum);
 void orc_loss_charb_f64(const orc_cam* cam, const double* img, const uint32_t* tgt,
-                        double* gC, uint8_t* valid, double* stats);
+                        double* gC, uint8_t* valid, double* stats, int32_t* expFix, double* expSum);
 
 /* L1 + lambda * D-SSIM (textbook; the reference has no SSIM). Loss summed
  * over valid pixel-channels: (1-lambda)|x - y| + lambda (1 - ssim(p, c)),
@@ -124,15 +127,18 @@ void orc_render_bwd_f64(const orc_opts* o, const orc_cam* cam, const double* pro
                         int32_t* gfix, double* gsum);
 
 /* R12: chain to parameter gradients. Input: gfix (decoded /16384 as the
- * reference does) or gsum. Output gradF n x 16, shGrad n x 3K. */
+ * reference does) or gsum. Output gradF n x 16, shGrad n x 3K. camFix
+ * (7 x i32, x64 fixed point, wrapping) / camSum (7 x f64), either may be
+ * NULL: camera rotation (3), translation (3) and log-focal gradients summed
+ * over the Gaussians. */
 void orc_chain_f32(const orc_opts* o, const orc_cam* cam, int64_t n,
                    const float* params, const float* sh, const float* proj,
                    const int32_t* gfix, const double* gsum,
-                   float* gradF, float* shGrad);
+                   float* gradF, float* shGrad, int32_t* camFix, double* camSum);
 void orc_chain_f64(const orc_opts* o, const orc_cam* cam, int64_t n,
                    const float* params, const float* sh, const double* proj,
                    const int32_t* gfix, const double* gsum,
-                   double* gradF, double* shGrad);
+                   double* gradF, double* shGrad, int32_t* camFix, double* camSum);
 
 /* R13/R14: Adam over params (n x 16) and SH (n x 3K), in place. */
 void orc_adam_f32(const orc_adam* hp, int64_t n, float* params, const float* gradF,

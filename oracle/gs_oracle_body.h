@@ -23,6 +23,21 @@ typedef struct {
 static REAL FN(clampr)(REAL x, REAL lo, REAL hi) { return FMIN(FMAX(x, lo), hi); }
 static REAL FN(sigm)(REAL x) { return RL(1.0) / (RL(1.0) + EXP(-FN(clampr)(x, RL(-9.0), RL(9.0)))); }
 static REAL FN(dot3)(const REAL* a, const REAL* b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+static void FN(cross3)(const REAL* a, const REAL* b, REAL* c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+/* camAdd (shaders.js:522-524, :717-719): i32(clamp(v * 64, -1e9, 1e9)) added
+ * with wrapping; dsum (optional) takes the unscaled value */
+static void FN(cam_add)(int32_t* fix, double* dsum, int slot, REAL v) {
+  if (fix) {
+    const REAL fv = v * RL(64.0);
+    const REAL cl = FMIN(FMAX(fv, RL(-1.0e9)), RL(1.0e9));
+    fix[slot] = (int32_t)((uint32_t)fix[slot] + (uint32_t)(int32_t)cl);
+  }
+  if (dsum) dsum[slot] += (double)v;
+}
 
 /* shaders.js:176-224 computeGeom */
 static void FN(compute_geom)(const orc_cam* cam, const float* P, GEOM* g) {
@@ -287,9 +302,12 @@ void FN(orc_render_fwd)(const orc_opts* o, const orc_cam* cam, const REAL* proj,
     }
 }
 
-/* shaders.js:589-618 loss (Charbonnier, delta 0.03) */
+/* shaders.js:589-618 loss (Charbonnier, delta 0.03). expFix / expSum
+ * (optional, 2 each) take the exposure gradients of :614-616: slot 0
+ * d/d(log gain) = dot(eg, C) * gain, slot 1 d/d(bias) = sum(eg), with
+ * eg = err / root per channel. */
 void FN(orc_loss_charb)(const orc_cam* cam, const REAL* img, const uint32_t* tgt,
-                        REAL* gC, uint8_t* valid, double* stats) {
+                        REAL* gC, uint8_t* valid, double* stats, int32_t* expFix, double* expSum) {
   const int64_t npx = (int64_t)cam->W * cam->H;
   const REAL gain = cam->gain, bias = cam->bias;
   stats[0] = stats[1] = stats[2] = 0;
@@ -299,12 +317,18 @@ void FN(orc_loss_charb)(const orc_cam* cam, const REAL* img, const uint32_t* tgt
     valid[pi] = 0;
     if ((pk >> 24) == 0) continue;
     valid[pi] = 1;
-    REAL err[3], root[3];
+    REAL err[3], root[3], eg[3];
     for (int c = 0; c < 3; c++) {
       const REAL tc = (REAL)((pk >> (8 * c)) & 0xffu) / RL(255.0);
       err[c] = (gain * img[4 * pi + c] + bias) - tc;
       root[c] = SQRT(err[c] * err[c] + RL(0.0009));
-      gC[3 * pi + c] = gain * (err[c] / root[c]);
+      eg[c] = err[c] / root[c];
+      gC[3 * pi + c] = gain * eg[c];
+    }
+    if (expFix || expSum) {
+      const REAL col[3] = {img[4 * pi], img[4 * pi + 1], img[4 * pi + 2]};
+      FN(cam_add)(expFix, expSum, 0, FN(dot3)(eg, col) * gain);
+      FN(cam_add)(expFix, expSum, 1, (eg[0] + eg[1]) + eg[2]);
     }
     stats[0] += (double)FN(dot3)(err, err);
     stats[1] += (double)(((root[0] + root[1]) + root[2]) - RL(0.09));
@@ -394,11 +418,13 @@ void FN(orc_render_bwd)(const orc_opts* o, const orc_cam* cam, const REAL* proj,
     }
 }
 
-/* shaders.js:722-948 makeChainSrc (camera-pose terms omitted: camOpt off) */
+/* shaders.js:722-948 makeChainSrc. camFix / camSum (optional, 7 each) take
+ * the camera terms of :834-858: rotation dw (0-2), translation dt = dpc (3-5),
+ * shared log-focal (6). */
 void FN(orc_chain)(const orc_opts* o, const orc_cam* cam, int64_t n,
                    const float* params, const float* sh, const REAL* proj,
                    const int32_t* gfix, const double* gsum,
-                   REAL* gradF, REAL* shGrad) {
+                   REAL* gradF, REAL* shGrad, int32_t* camFix, double* camSum) {
   const int K = o->shDeg > 0 ? (o->shDeg + 1) * (o->shDeg + 1) - 1 : 0;
   const REAL AREG = o->anisoReg;
   REAL R[9];
@@ -491,6 +517,29 @@ void FN(orc_chain)(const orc_opts* o, const orc_cam* cam, int64_t n,
     dpc[0] = dpc[0] + gp[0] * f * iz;
     dpc[1] = dpc[1] + gp[1] * f * iz;
     dpc[2] = dpc[2] + (-f) * (gp[0] * g.pc[0] + gp[1] * g.pc[1]) * iz * iz;
+    /* camera pose (p_c = exp(w^) R p + t) and shared log-focal (:834-858) */
+    if (camFix || camSum) {
+      const REAL rp[3] = {g.pc[0] - RL(cam->t[0]), g.pc[1] - RL(cam->t[1]), g.pc[2] - RL(cam->t[2])};
+      REAL dw[3], cs[3] = {0, 0, 0};
+      FN(cross3)(rp, dpc, dw);
+      const REAL fiz = f * iz;
+      const REAL bx = (((-f) * g.pc[0]) * iz) * iz;
+      const REAL by = (((-f) * g.pc[1]) * iz) * iz;
+      for (int k = 0; k < 3; k++) {
+        /* column k of W against (dW0[k], dW1[k], dW2[k]) */
+        const REAL wc[3] = {R[k], R[3 + k], R[6 + k]};
+        const REAL dv[3] = {fiz * dT0[k], fiz * dT1[k], bx * dT0[k] + by * dT1[k]};
+        REAL c[3];
+        FN(cross3)(wc, dv, c);
+        for (int a = 0; a < 3; a++) cs[a] = k == 0 ? c[a] : cs[a] + c[a];
+      }
+      for (int a = 0; a < 3; a++) dw[a] = dw[a] + cs[a];
+      const REAL dlogf = (((((gp[0] * f) * g.pc[0]) * iz + ((gp[1] * f) * g.pc[1]) * iz)
+                          + (dJ00 + dJ11) * fiz) + dJ02 * bx) + dJ12 * by;
+      for (int a = 0; a < 3; a++) FN(cam_add)(camFix, camSum, a, dw[a]);
+      for (int a = 0; a < 3; a++) FN(cam_add)(camFix, camSum, 3 + a, dpc[a]);
+      FN(cam_add)(camFix, camSum, 6, dlogf);
+    }
     /* W^T dpc (:830-832) */
     gF[0] = (R[0] * dpc[0] + R[3] * dpc[1]) + R[6] * dpc[2];
     gF[1] = (R[1] * dpc[0] + R[4] * dpc[1]) + R[7] * dpc[2];

@@ -9,6 +9,7 @@ line citations). `prec` selects the single-precision twin ("f32", bit-exact
 target for the HIP kernels) or the double-precision reference ("f64").
 """
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -75,9 +76,9 @@ def lib():
                 ("orc_project_", [P, P, C.c_int64, P, P, P, P, P], None),
                 ("orc_bin_sort_", [C.c_int64, P, P, P, C.c_int, C.c_int, P, P, P], C.c_int64),
                 ("orc_render_fwd_", [P, P, P, P, P, P, P, P], None),
-                ("orc_loss_charb_", [P, P, P, P, P, P], None),
+                ("orc_loss_charb_", [P, P, P, P, P, P, P, P], None),
                 ("orc_render_bwd_", [P, P, P, P, P, P, P, P, P, P, P], None),
-                ("orc_chain_", [P, P, C.c_int64, P, P, P, P, P, P, P], None),
+                ("orc_chain_", [P, P, C.c_int64, P, P, P, P, P, P, P, P, P], None),
             ]:
                 fn = getattr(L, name + s)
                 fn.argtypes = args
@@ -100,7 +101,7 @@ def make_cam(cam, shActiveDeg=0, bg=(0.0, 0.0, 0.0)):
         c.bg[k] = float(bg[k])
     c.nearp = 0.05
     c.f, c.cx, c.cy = float(cam["f"]), float(cam["cx"]), float(cam["cy"])
-    c.gain = float(np.float32(np.exp(cam.get("g", 0.0))))
+    c.gain = float(np.float32(math.exp(cam.get("g", 0.0))))   # libm exp, as the trainer's host code
     c.bias = float(cam.get("b", 0.0))
     c.W, c.H = int(cam["w"]), int(cam["h"])
     c.shActiveDeg = int(shActiveDeg)
@@ -163,13 +164,19 @@ class Stage:
                                    _p(img), _p(T), _p(end))
         return img, T, end
 
-    def loss_charb(self, cam, img, tgt):
+    def loss_charb(self, cam, img, tgt, cam_grads=False):
+        """cam_grads: also return the exposure gradients (log gain, bias) as
+        x64 fixed-point wrapping i32 sums and as plain f64 sums."""
         npx = cam.W * cam.H
         gC = np.zeros((npx, 3), self.dt)
         valid = np.zeros(npx, np.uint8)
         stats = np.zeros(3, np.float64)
+        efix = np.zeros(2, np.int32) if cam_grads else None
+        esum = np.zeros(2, np.float64) if cam_grads else None
         self.fn("orc_loss_charb_")(C.byref(cam), _p(img), _p(np.ascontiguousarray(tgt, np.uint32)),
-                                   _p(gC), _p(valid), _p(stats))
+                                   _p(gC), _p(valid), _p(stats), _p(efix), _p(esum))
+        if cam_grads:
+            return gC, valid, stats, efix, esum
         return gC, valid, stats
 
     def loss_l1ssim(self, cam, img, tgt, lam=0.2):
@@ -193,15 +200,22 @@ class Stage:
                                    _p(valid), _p(gfix), _p(gsum))
         return gfix if fixed else gsum
 
-    def chain(self, cam, params, sh, proj, gfix=None, gsum=None):
+    def chain(self, cam, params, sh, proj, gfix=None, gsum=None, cam_grads=False):
+        """cam_grads: also return the camera gradients (rotation 3,
+        translation 3, log-focal 1) as x64 fixed-point wrapping i32 sums and
+        as plain f64 sums."""
         n = params.shape[0]
         K = sh_k(self.o.shDeg)
         gradF = np.zeros((n, 16), self.dt)
         shGrad = np.zeros((n, max(3 * K, 1)), self.dt)
+        cfix = np.zeros(7, np.int32) if cam_grads else None
+        csum = np.zeros(7, np.float64) if cam_grads else None
         self.fn("orc_chain_")(C.byref(self.o), C.byref(cam), n,
                               _p(np.ascontiguousarray(params, np.float32)),
                               _p(None if sh is None else np.ascontiguousarray(sh, np.float32)),
-                              _p(proj), _p(gfix), _p(gsum), _p(gradF), _p(shGrad))
+                              _p(proj), _p(gfix), _p(gsum), _p(gradF), _p(shGrad), _p(cfix), _p(csum))
+        if cam_grads:
+            return gradF, shGrad[:, :3 * K], cfix, csum
         return gradF, shGrad[:, :3 * K]
 
 

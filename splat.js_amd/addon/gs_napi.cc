@@ -30,6 +30,9 @@ struct Api {
   int (*render)(gs_trainer_t*, const gs_camera*, int32_t, int64_t, float*);
   int (*read_gaussians)(gs_trainer_t*, float*, float*);
   int (*eval_cam_psnr)(gs_trainer_t*, int32_t, double*);
+  int (*eval_cam_psnr_refined)(gs_trainer_t*, int32_t, int32_t, double*);
+  int (*get_camera)(gs_trainer_t*, int32_t, gs_cam_state*);
+  int (*read_cam_grads)(gs_trainer_t*, int32_t*, int32_t);
   int (*set_holdout)(gs_trainer_t*, int32_t);
   int (*set_excluded)(gs_trainer_t*, const int32_t*, int32_t);
   int (*set_iter)(gs_trainer_t*, int64_t);
@@ -75,6 +78,9 @@ bool load_api() {
   SYM(render, "gs_render");
   SYM(read_gaussians, "gs_read_gaussians");
   SYM(eval_cam_psnr, "gs_eval_cam_psnr");
+  SYM(eval_cam_psnr_refined, "gs_eval_cam_psnr_refined");
+  SYM(get_camera, "gs_get_camera");
+  SYM(read_cam_grads, "gs_read_cam_grads");
   SYM(set_holdout, "gs_set_holdout");
   SYM(set_excluded, "gs_set_excluded");
   SYM(set_iter, "gs_set_iter");
@@ -128,6 +134,12 @@ double get_num(napi_env env, napi_value obj, const char* key, double dflt, bool*
   napi_get_value_double(env, v, &d);
   if (present) *present = true;
   return d;
+}
+
+void set_num(napi_env env, napi_value obj, const char* k, double v) {
+  napi_value x;
+  napi_create_double(env, v, &x);
+  napi_set_named_property(env, obj, k, x);
 }
 
 bool get_f32(napi_env env, napi_value v, float** data, size_t* len) {
@@ -216,6 +228,8 @@ napi_value Create(napi_env env, napi_callback_info info) {
       o.seed = (uint32_t)get_num(env, a, "seed", o.seed);
       o.device = (int32_t)get_num(env, a, "device", o.device);
       o.precision = (int32_t)get_num(env, a, "precision", o.precision);
+      o.camOpt = get_num(env, a, "camOpt", o.camOpt) != 0.0 ? 1 : 0;
+      o.expComp = get_num(env, a, "expComp", o.expComp) != 0.0 ? 1 : 0;
     }
   }
   gs_trainer_t* h = nullptr;
@@ -360,6 +374,69 @@ napi_value EvalCamPsnr(napi_env env, napi_callback_info info) {
   return r;
 }
 
+// evalCamPsnrRefined(h, ci, steps) -> psnr
+napi_value EvalCamPsnrRefined(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  gs_trainer_t* h = get_handle(env, argv[0]);
+  int32_t ci = 0, steps = 200;
+  napi_get_value_int32(env, argv[1], &ci);
+  if (argc > 2) napi_get_value_int32(env, argv[2], &steps);
+  double v = 0;
+  if (!check(env, h, g_api.eval_cam_psnr_refined(h, ci, steps, &v))) return nullptr;
+  napi_value r;
+  NAPI_CALL(env, napi_create_double(env, v, &r));
+  return r;
+}
+
+// getCamera(h, ci) -> {R: number[9], t: number[3], f, g, b}
+napi_value GetCamera(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  gs_trainer_t* h = get_handle(env, argv[0]);
+  int32_t ci = 0;
+  napi_get_value_int32(env, argv[1], &ci);
+  gs_cam_state st;
+  if (!check(env, h, g_api.get_camera(h, ci, &st))) return nullptr;
+  napi_value o, R, t;
+  NAPI_CALL(env, napi_create_object(env, &o));
+  NAPI_CALL(env, napi_create_array_with_length(env, 9, &R));
+  NAPI_CALL(env, napi_create_array_with_length(env, 3, &t));
+  for (uint32_t k = 0; k < 9; k++) {
+    napi_value v;
+    napi_create_double(env, st.R[k], &v);
+    napi_set_element(env, R, k, v);
+  }
+  for (uint32_t k = 0; k < 3; k++) {
+    napi_value v;
+    napi_create_double(env, st.t[k], &v);
+    napi_set_element(env, t, k, v);
+  }
+  napi_set_named_property(env, o, "R", R);
+  napi_set_named_property(env, o, "t", t);
+  set_num(env, o, "f", st.f);
+  set_num(env, o, "g", st.g);
+  set_num(env, o, "b", st.b);
+  return o;
+}
+
+// readCamGrads(h, Int32Array (numCams + 1) * 8)
+napi_value ReadCamGrads(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  gs_trainer_t* h = get_handle(env, argv[0]);
+  napi_typedarray_type ty;
+  size_t len = 0;
+  void* data = nullptr;
+  if (napi_get_typedarray_info(env, argv[1], &ty, &len, &data, nullptr, nullptr) != napi_ok || ty != napi_int32_array)
+    return throw_err(env, "readCamGrads: expected an Int32Array");
+  check(env, h, g_api.read_cam_grads(h, (int32_t*)data, (int32_t)len));
+  return nullptr;
+}
+
 napi_value SetHoldout(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
@@ -399,11 +476,6 @@ napi_value SetIter(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
-void set_num(napi_env env, napi_value obj, const char* k, double v) {
-  napi_value x;
-  napi_create_double(env, v, &x);
-  napi_set_named_property(env, obj, k, x);
-}
 
 napi_value Info(napi_env env, napi_callback_info info) {
   size_t argc = 1;
@@ -505,6 +577,8 @@ napi_value Init(napi_env env, napi_value exports) {
       {"evalCamPsnr", EvalCamPsnr}, {"setHoldout", SetHoldout}, {"setExcluded", SetExcluded},
       {"setIter", SetIter},         {"info", Info},           {"sync", Sync},
       {"refine", Refine},           {"exportPly", ExportPly},
+      {"evalCamPsnrRefined", EvalCamPsnrRefined}, {"getCamera", GetCamera},
+      {"readCamGrads", ReadCamGrads},
       {"loaded", Loaded},           {"loadError", LoadError},
   };
   for (const auto& f : fns) {

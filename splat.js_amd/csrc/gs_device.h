@@ -138,6 +138,83 @@ __device__ __forceinline__ float conic_lmax(float va, float vb, float vc) {
   return cmid + sqrtf(fmaxf(cmid * cmid - (cva * cvc - vb * vb), 0.0f));
 }
 
+// Screen-space gradients to the 2D-covariance, Jacobian and camera-space
+// position gradients (shaders.js:759-827). gp[2..4] already carry the
+// 1 + lmax factor. Shared by the update pass and the camera-gradient pass.
+struct ChainScreen {
+  float dS00, dS01, dS02, dS11, dS12, dS22;   // dL/dSigma (symmetric)
+  float dT0[3], dT1[3];                       // dL/dT rows
+  float dJ00, dJ02, dJ11, dJ12;               // dL/dJ
+  float dpc[3];                               // dL/d(camera-space position)
+};
+
+__device__ __forceinline__ void chain_screen(const Cam& cam, const Geom& g, const float* gp, float iz,
+                                             ChainScreen& c) {
+  const float f = cam.f;
+  // conic -> dilated 2D covariance (:759-777)
+  const float ad = g.va + 0.3f, cd = g.vc + 0.3f;
+  const float detVd = ad * cd - g.vb * g.vb;
+  const float inv = 1.0f / detVd;
+  const float kA = cd * inv, kB = (-g.vb) * inv, kC = ad * inv;
+  const float g11 = gp[2], g12 = 0.5f * gp[3], g22 = gp[4];
+  const float m100 = g11 * kA + g12 * kB, m101 = g11 * kB + g12 * kC;
+  const float m110 = g12 * kA + g22 * kB, m111 = g12 * kB + g22 * kC;
+  const float p00 = kA * m100 + kB * m110, p01 = kA * m101 + kB * m111;
+  const float p10 = kB * m100 + kC * m110, p11 = kB * m101 + kC * m111;
+  float gva = -p00, gvb = -(p01 + p10), gvc = -p11;
+
+  // comp -> raw 2D covariance (:780-788)
+  const float detV = fmaxf(g.va * g.vc - g.vb * g.vb, 0.0f);
+  const float comp = sqrtf(fmaxf(detV / detVd, 0.0f));
+  if (comp > 1e-4f) {
+    const float gcomp = gp[5];
+    const float denom = 2.0f * comp * detVd;
+    gva = gva + gcomp * (g.vc - comp * comp * cd) / denom;
+    gvc = gvc + gcomp * (g.va - comp * comp * ad) / denom;
+    gvb = gvb + gcomp * (2.0f * g.vb * (comp * comp - 1.0f)) / denom;
+  }
+
+  // V = T Sigma T^T backward (:791-810)
+  const float h11 = gva, h12 = 0.5f * gvb, h22 = gvc;
+  const float* t0 = g.t0;
+  const float* t1 = g.t1;
+  float u0[3], u1[3];
+  for (int k = 0; k < 3; k++) { u0[k] = h11 * t0[k] + h12 * t1[k]; u1[k] = h12 * t0[k] + h22 * t1[k]; }
+  c.dS00 = t0[0] * u0[0] + t1[0] * u1[0];
+  c.dS01 = 0.5f * ((t0[0] * u0[1] + t1[0] * u1[1]) + (t0[1] * u0[0] + t1[1] * u1[0]));
+  c.dS02 = 0.5f * ((t0[0] * u0[2] + t1[0] * u1[2]) + (t0[2] * u0[0] + t1[2] * u1[0]));
+  c.dS11 = t0[1] * u0[1] + t1[1] * u1[1];
+  c.dS12 = 0.5f * ((t0[1] * u0[2] + t1[1] * u1[2]) + (t0[2] * u0[1] + t1[2] * u1[1]));
+  c.dS22 = t0[2] * u0[2] + t1[2] * u1[2];
+  float sT0[3], sT1[3];
+  sT0[0] = (g.s00 * t0[0] + g.s01 * t0[1]) + g.s02 * t0[2];
+  sT0[1] = (g.s01 * t0[0] + g.s11 * t0[1]) + g.s12 * t0[2];
+  sT0[2] = (g.s02 * t0[0] + g.s12 * t0[1]) + g.s22 * t0[2];
+  sT1[0] = (g.s00 * t1[0] + g.s01 * t1[1]) + g.s02 * t1[2];
+  sT1[1] = (g.s01 * t1[0] + g.s11 * t1[1]) + g.s12 * t1[2];
+  sT1[2] = (g.s02 * t1[0] + g.s12 * t1[1]) + g.s22 * t1[2];
+  for (int k = 0; k < 3; k++) {
+    c.dT0[k] = 2.0f * (h11 * sT0[k] + h12 * sT1[k]);
+    c.dT1[k] = 2.0f * (h12 * sT0[k] + h22 * sT1[k]);
+  }
+
+  // T = J W (:813-822) and the mean path (:825-827)
+  c.dJ00 = dot3(c.dT0, cam.R + 0);
+  c.dJ02 = dot3(c.dT0, cam.R + 6);
+  c.dJ11 = dot3(c.dT1, cam.R + 3);
+  c.dJ12 = dot3(c.dT1, cam.R + 6);
+  float* dpc = c.dpc;
+  dpc[0] = 0.0f; dpc[1] = 0.0f; dpc[2] = 0.0f;
+  dpc[0] = dpc[0] + c.dJ02 * ((-f) * iz * iz);
+  dpc[1] = dpc[1] + c.dJ12 * ((-f) * iz * iz);
+  dpc[2] = dpc[2] + ((c.dJ00 + c.dJ11) * ((-f) * iz * iz)
+         + c.dJ02 * (2.0f * f * g.pc[0] * iz * iz * iz))
+         + c.dJ12 * (2.0f * f * g.pc[1] * iz * iz * iz);
+  dpc[0] = dpc[0] + gp[0] * f * iz;
+  dpc[1] = dpc[1] + gp[1] * f * iz;
+  dpc[2] = dpc[2] + (-f) * (gp[0] * g.pc[0] + gp[1] * g.pc[1]) * iz * iz;
+}
+
 // shaders.js:56-111 real SH basis (degrees 1..3) and dY/dv; D is K x 3.
 // Y[14] follows shaders.js:102 (x * (xx - yy)).
 template <int kDeg, bool kGrad>

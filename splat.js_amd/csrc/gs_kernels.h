@@ -97,6 +97,21 @@ void launch_raster_bwd(hipStream_t st, const Cam& cam, const Cut& cut, const Ras
 void launch_update(hipStream_t st, int deg, int accumMode, bool mixed, const Cam& cam, const AdamHP& hp,
                    const UpdateBufs& u);
 
+// ---- camera pose, focal and exposure gradients (camOpt) ----
+struct CamPoseBufs {
+  int64_t n, cap;
+  const float4* P;        // 4 float4 planes
+  void* accum;            // n x 12 screen-space sums (i32 or f32)
+  const uint32_t* cnt;    // tiles covered (0 = culled)
+  uint32_t* row;          // this camera's gradient row (slots 0-5)
+  uint32_t* focal;        // the shared log-focal slot
+};
+
+// row: this camera's gradient row; slots 6 (log gain) and 7 (bias) are added to
+void launch_cam_exposure(hipStream_t st, const Cam& cam, const float4* img, const uint32_t* tgt, uint32_t* row);
+// clear: also zero the screen-space accumulators (evaluation passes)
+void launch_cam_pose(hipStream_t st, int accumMode, bool clear, const Cam& cam, const CamPoseBufs& b);
+
 // ---- refine (MCMC-lite relocation and growth) ----
 
 // One spawn of the refine plan: target row gets the donor's row with its

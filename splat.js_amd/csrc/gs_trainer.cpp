@@ -26,10 +26,14 @@ constexpr int64_t kDefaultEntries = 12000000;   // shaders.js:34 ENTRIES_CAP
 constexpr int64_t kTargetLimitBytes = int64_t(64) << 30;
 
 struct CamMeta {
-  gs_camera c;
+  gs_camera c;      // f32 view of the camera, as the kernels take it (trainer.js:357-372)
   int64_t offset;   // first target pixel
-  float f0;
   int activeDeg;    // SH degree in this camera's uniform (trainer.js:367, :411)
+  // the reference's camMeta entry in JS numbers: pose, focal (f0 the
+  // original) and exposure, which the camera optimizer updates
+  // (trainer.js:306, :518-533)
+  double R[9], t[3];
+  double f0, f, g, b;
 };
 
 struct ProfSlot {
@@ -106,8 +110,13 @@ struct gs_trainer {
   int64_t totalPx = 0;
   double* stats = nullptr;      // training stats [4]
   double* evalStats = nullptr;  // eval/view stats [4]
+  uint32_t* camGrad = nullptr;  // (numCams + 1) x 8 i32 camera gradient rows (trainer.js:173)
 
   std::vector<CamMeta> cams;
+  // camera optimizer state (trainer.js:308-313)
+  std::vector<double> camM, camV;
+  int64_t camStep = 0;
+  double logfScale = 0.0;
   int64_t iter = 0;
   int64_t pixelsSeen = 0;
   float sceneRadius = 1.0f;
@@ -241,6 +250,58 @@ Cam make_cam(const gs_trainer* h, const gs_camera& c, int trainMode, int64_t off
   return k;
 }
 
+// the training uniform of a camera: its f32 fields plus the exposure of
+// its JS-number state (trainer.js:363, :368: gain = exp(g) and bias b, each
+// rounded once to f32)
+Cam make_cam_meta(const gs_trainer* h, const CamMeta& m, int activeDeg) {
+  Cam k = make_cam(h, m.c, 1, m.offset, activeDeg);
+  k.gain = (float)std::exp(m.g);
+  k.bias = (float)m.b;
+  return k;
+}
+
+// the f32 camera fields from the JS-number state (Float32Array stores)
+void refresh_cam(CamMeta& m) {
+  for (int k = 0; k < 9; k++) m.c.R[k] = (float)m.R[k];
+  for (int k = 0; k < 3; k++) m.c.t[k] = (float)m.t[k];
+  m.c.f = (float)m.f;
+  m.c.g = (float)m.g;
+  m.c.b = (float)m.b;
+}
+
+// ---- camera optimizer host math (src/sfm/geometry.js:108-114, :150-161) ----
+void m3mul(const double* A, const double* B, double* C) {
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++)
+      C[i * 3 + j] = (A[i * 3] * B[j] + A[i * 3 + 1] * B[3 + j]) + A[i * 3 + 2] * B[6 + j];
+}
+
+void rodrigues(const double* w, double* R) {
+  const double th = std::sqrt((w[0] * w[0] + w[1] * w[1]) + w[2] * w[2]);
+  if (th < 1e-12) {
+    const double I[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    memcpy(R, I, sizeof I);
+    return;
+  }
+  const double k[3] = {w[0] / th, w[1] / th, w[2] / th};
+  const double c = std::cos(th), s = std::sin(th), v = 1 - c;
+  R[0] = c + k[0] * k[0] * v;        R[1] = k[0] * k[1] * v - k[2] * s; R[2] = k[0] * k[2] * v + k[1] * s;
+  R[3] = k[1] * k[0] * v + k[2] * s; R[4] = c + k[1] * k[1] * v;        R[5] = k[1] * k[2] * v - k[0] * s;
+  R[6] = k[2] * k[0] * v - k[1] * s; R[7] = k[2] * k[1] * v + k[0] * s; R[8] = c + k[2] * k[2] * v;
+}
+
+// R <- rodrigues(-u[0..2]) R, t <- t - u[3..5] (trainer.js:517-523, :604-605)
+void apply_pose_step(const double* u, double* R, double* t) {
+  const double dw[3] = {-u[0], -u[1], -u[2]};
+  double Rw[9], Rn[9];
+  rodrigues(dw, Rw);
+  m3mul(Rw, R, Rn);
+  memcpy(R, Rn, sizeof Rn);
+  for (int k = 0; k < 3; k++) t[k] = t[k] - u[3 + k];
+}
+
+double clampd(double x, double lo, double hi) { return std::max(lo, std::min(hi, x)); }
+
 AdamHP make_hp(const gs_trainer* h) {
   // trainer.js:320-342, :417-424
   AdamHP a{};
@@ -325,6 +386,115 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
   return GS_OK;
 }
 
+CamPoseBufs cam_pose_bufs(gs_trainer* h, int ci) {
+  CamPoseBufs b{};
+  b.n = h->n;
+  b.cap = h->cap;
+  b.P = h->P;
+  b.accum = h->accum;
+  b.cnt = h->cnt;
+  b.row = h->camGrad + 8 * ci;
+  b.focal = h->camGrad + 8 * h->cams.size();
+  return b;
+}
+
+// GSTrainer._applyCamGrads (trainer.js:476-537): reads and clears the
+// gradient rows, then one Adam step (JS numbers) on every camera pose but
+// camera 0 (the gauge anchor) and the holdout, on the exposure when expComp
+// is set, and on the shared log-focal. The reference applies the step when
+// its asynchronous readback resolves, so the next few steps may still use
+// the old uniforms; here it applies before the next step is enqueued.
+int apply_cam_grads(gs_trainer* h) {
+  const int nc = (int)h->cams.size();
+  std::vector<int32_t> g((size_t)(nc + 1) * 8);
+  HIPCK(hipMemcpyAsync(g.data(), h->camGrad, g.size() * 4, hipMemcpyDeviceToHost, h->st));
+  HIPCK(hipMemsetAsync(h->camGrad, 0, g.size() * 4, h->st));
+  HIPCK(hipStreamSynchronize(h->st));
+  h->camStep++;
+  const double t = (double)h->camStep;
+  const double decay = std::pow(0.02, std::min(1.0, (double)h->iter / (0.75 * h->horizon)));
+  const double rotLr = 2e-4 * decay;
+  const double trnLr = 2e-4 * (double)h->sceneRadius * decay;
+  const double focLr = 1e-4 * decay;
+  const double b1 = 0.9, b2 = 0.99, eps = 1e-15;
+  auto step = [&](int row, int slot, double lr) {
+    const size_t j = (size_t)row * 8 + slot;
+    const double grad = g[j] / 64.0;
+    h->camM[j] = b1 * h->camM[j] + (1 - b1) * grad;
+    h->camV[j] = b2 * h->camV[j] + (1 - b2) * grad * grad;
+    const double mh = h->camM[j] / (1 - std::pow(b1, t));
+    const double vh = h->camV[j] / (1 - std::pow(b2, t));
+    return lr * mh / (std::sqrt(vh) + eps);
+  };
+  const double expLr = 5e-3;
+  for (int r = 1; r < nc; r++) {
+    if (r == h->holdout) continue;
+    CamMeta& m = h->cams[r];
+    double u[6];
+    for (int k = 0; k < 3; k++) u[k] = step(r, k, rotLr);
+    for (int k = 3; k < 6; k++) u[k] = step(r, k, trnLr);
+    apply_pose_step(u, m.R, m.t);
+    if (h->o.expComp) {
+      m.g = clampd(m.g - step(r, 6, expLr), -0.1, 0.1);
+      m.b = clampd(m.b - step(r, 7, expLr), -0.05, 0.05);
+    }
+  }
+  h->logfScale = clampd(h->logfScale - step(nc, 0, focLr), -0.3, 0.3);
+  for (CamMeta& m : h->cams) {
+    m.f = m.f0 * std::exp(h->logfScale);
+    refresh_cam(m);
+    m.activeDeg = h->shDeg;   // the rebuilt uniforms carry the full degree (trainer.js:369, :535)
+  }
+  return GS_OK;
+}
+
+// GSTrainer._evalPass (trainer.js:542-573) on camera ci at pose (R, t) and
+// exposure (g, b): the training raster, loss and backward plus the camera
+// terms of the chain, without any parameter update. Returns the view's PSNR
+// and its camera gradient row (everything accumulated in the row, as the
+// reference reads the buffer), then drains all rows and resets the training
+// loss window, as the reference does.
+int eval_pass(gs_trainer* h, int ci, const double* R, const double* t, double g, double b, double* psnr,
+              int32_t* grad8) {
+  CamMeta m = h->cams[ci];
+  memcpy(m.R, R, sizeof m.R);
+  memcpy(m.t, t, sizeof m.t);
+  m.g = g;
+  m.b = b;
+  refresh_cam(m);
+  const Cam cam = make_cam_meta(h, m, h->shDeg);   // _camUniform: full SH degree (trainer.js:369)
+  HIPCK(hipMemsetAsync(h->evalStats, 0, 32, h->st));
+  int rc = run_raster(h, cam, 1, h->evalStats);
+  if (rc) return rc;
+  launch_cam_exposure(h->st, cam, h->img, h->targets, h->camGrad + 8 * ci);
+  RasterBufs rb{};
+  rb.ranges = h->ranges;
+  rb.tileOrder = h->lastBin.tileOrder;
+  rb.ids = h->lastBin.ids;
+  rb.rec = h->rec;
+  rb.T = h->Tfin;
+  rb.end = h->endIdx;
+  rb.gC = h->gC;
+  launch_raster_bwd(h->st, cam, make_cut(h), rb, h->o.gradAccum, h->accum);
+  launch_cam_pose(h->st, h->o.gradAccum, true, cam, cam_pose_bufs(h, ci));
+  HIPCK(hipGetLastError());
+  double s[4];
+  int32_t row[8];
+  const size_t rows = h->cams.size() + 1;
+  HIPCK(hipMemcpyAsync(s, h->evalStats, 32, hipMemcpyDeviceToHost, h->st));
+  HIPCK(hipMemcpyAsync(row, h->camGrad + 8 * ci, 32, hipMemcpyDeviceToHost, h->st));
+  HIPCK(hipMemsetAsync(h->camGrad, 0, rows * 32, h->st));
+  HIPCK(hipMemsetAsync(h->stats, 0, 32, h->st));
+  HIPCK(hipStreamSynchronize(h->st));
+  h->pixelsSeen = 0;
+  if (grad8) memcpy(grad8, row, sizeof row);
+  if (psnr) {
+    const double mse = s[0] / (std::max(1.0, s[2]) * 3.0);
+    *psnr = mse > 0 ? -10.0 * std::log10(mse) : INFINITY;
+  }
+  return GS_OK;
+}
+
 int pick_camera(gs_trainer* h) {
   // trainer.js:400-406 (Math.random replaced by a seeded mulberry32)
   const int nc = (int)h->cams.size();
@@ -346,7 +516,7 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
   h->lastCam = ci;
   // trainer.js:409-412: one SH band per 1000 iterations
   meta.activeDeg = (int)std::min<int64_t>(h->shDeg, h->iter / 1000);
-  const Cam cam = make_cam(h, meta.c, 1, meta.offset, meta.activeDeg);
+  const Cam cam = make_cam_meta(h, meta, meta.activeDeg);
   h->iter++;
   h->pixelsSeen += (int64_t)meta.c.w * meta.c.h;
   const AdamHP hp = make_hp(h);
@@ -362,6 +532,11 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
     lb.stats = h->stats;
     Scope sc(h, "loss_l1ssim", (double)cam.W * cam.H * 116.0);
     launch_loss_l1ssim(h->st, cam, h->o.ssimLambda, lb, h->ssimScratch);
+  }
+  if (h->o.camOpt && !ssim) {
+    // exposure terms of the Charbonnier loss (shaders.js:614-616)
+    Scope sc(h, "cam_exposure", (double)cam.W * cam.H * 20.0);
+    launch_cam_exposure(h->st, cam, h->img, h->targets, h->camGrad + 8 * ci);
   }
   HIPCK(hipGetLastError());
   const int64_t npx = (int64_t)cam.W * cam.H;
@@ -432,6 +607,12 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
       HIPCK(hipMalloc(&dSG, 4 * (h->shC ? h->shC : 1) * h->n));
     }
   }
+  if (h->o.camOpt) {
+    // pose and focal terms of the chain (shaders.js:834-858), read before
+    // the update pass clears the accumulators
+    Scope sc(h, "cam_pose", (double)h->n * (4.0 + 32.0 + 64.0));
+    launch_cam_pose(h->st, h->o.gradAccum, false, cam, cam_pose_bufs(h, ci));
+  }
   UpdateBufs u{};
   u.n = h->n;
   u.cap = h->cap;
@@ -456,6 +637,8 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
     }
     if (dbg->stats) HIPCK(hipMemcpy(dbg->stats, h->stats, 32, hipMemcpyDeviceToHost));
   }
+  // trainer.js:468-470: one camera Adam step every 25 iterations after 1500
+  if (h->o.camOpt && h->iter > 1500 && h->iter % 25 == 0) return apply_cam_grads(h);
   return GS_OK;
 }
 
@@ -585,8 +768,12 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
     m.c.w = im.tw;
     m.c.h = im.th;
     m.offset = total;
-    m.f0 = c.f;
     m.activeDeg = h->shDeg;
+    for (int k = 0; k < 9; k++) m.R[k] = c.R[k];
+    for (int k = 0; k < 3; k++) m.t[k] = c.t[k];
+    m.f0 = m.f = c.f;   // trainer.js:306
+    m.g = c.g;
+    m.b = c.b;
     total += (int64_t)im.tw * im.th;
     h->cams.push_back(m);
   }
@@ -665,6 +852,7 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   rc |= dalloc(h, &h->targets, tgt.size() * 4);
   rc |= dalloc(h, &h->stats, 32);
   rc |= dalloc(h, &h->evalStats, 32);
+  rc |= dalloc(h, &h->camGrad, (size_t)(ncams + 1) * 32);
   if (rc) { free_all(h); return rc < 0 ? rc : GS_E_NOMEM; }
 
   std::vector<float> planes;
@@ -690,6 +878,7 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   HIPCK(hipMemset(h->accum, 0, (size_t)cap * 48));
   HIPCK(hipMemset(h->stats, 0, 32));
   HIPCK(hipMemset(h->evalStats, 0, 32));
+  HIPCK(hipMemset(h->camGrad, 0, (size_t)(ncams + 1) * 32));
   const uint32_t n32 = (uint32_t)n;
   HIPCK(hipMemcpy(h->dN, &n32, 4, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(h->targets, tgt.data(), tgt.size() * 4, hipMemcpyHostToDevice));
@@ -699,6 +888,10 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   h->basePosLr = h->o.posLr > 0 ? (double)h->o.posLr : 3e-4 * (double)sceneRadius;
   h->holdout = -1;
   h->excluded.assign(ncams, 0);
+  h->camM.assign((size_t)(ncams + 1) * 8, 0.0);
+  h->camV.assign((size_t)(ncams + 1) * 8, 0.0);
+  h->camStep = 0;
+  h->logfScale = 0.0;
   h->iter = 0;
   h->pixelsSeen = 0;
   h->ready = true;
@@ -764,18 +957,79 @@ int gs_eval_cam_psnr(gs_trainer_t* h, int32_t ci, double* psnr) {
   if (ci < 0 || ci >= (int)h->cams.size()) return fail(h, GS_E_ARG, "camera index out of range");
   const CamMeta& m = h->cams[ci];
   // the reference evaluates with the camera's training uniform (trainer.js:547)
-  const Cam cam = make_cam(h, m.c, 1, m.offset, m.activeDeg);
+  const Cam cam = make_cam_meta(h, m, m.activeDeg);
   HIPCK(hipMemsetAsync(h->evalStats, 0, 32, h->st));
   int rc = run_raster(h, cam, 1, h->evalStats);
   if (rc) return rc;
   double s[4];
   HIPCK(hipMemcpyAsync(s, h->evalStats, 32, hipMemcpyDeviceToHost, h->st));
-  HIPCK(hipStreamSynchronize(h->st));
-  // trainer.js:570-571: the training stats window is reset too
+  // trainer.js:562, :570-571: the camera gradient rows are drained and the
+  // training stats window is reset too
+  HIPCK(hipMemsetAsync(h->camGrad, 0, (h->cams.size() + 1) * 32, h->st));
   HIPCK(hipMemsetAsync(h->stats, 0, 32, h->st));
+  HIPCK(hipStreamSynchronize(h->st));
   h->pixelsSeen = 0;
   const double mse = s[0] / (std::max(1.0, s[2]) * 3.0);
   *psnr = mse > 0 ? -10.0 * std::log10(mse) : INFINITY;
+  return GS_OK;
+}
+
+int gs_eval_cam_psnr_refined(gs_trainer_t* h, int32_t ci, int32_t steps, double* psnr) {
+  // trainer.js:583-612: test-time refinement of one camera's pose (and
+  // exposure with expComp) against its photo, Gaussians frozen
+  if (!h || !psnr || steps < 0) return GS_E_ARG;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  if (ci < 0 || ci >= (int)h->cams.size()) return fail(h, GS_E_ARG, "camera index out of range");
+  HIPCK(hipSetDevice(h->o.device));
+  double R[9], t[3];
+  memcpy(R, h->cams[ci].R, sizeof R);
+  memcpy(t, h->cams[ci].t, sizeof t);
+  double g = 0.0, b = 0.0;
+  double m[8] = {0}, v[8] = {0};
+  const double b1 = 0.9, b2 = 0.99, eps = 1e-15;
+  const double rotLr = 3e-4, trnLr = 3e-4 * (double)h->sceneRadius, expLr = 5e-3;
+  for (int s = 1; s <= steps; s++) {
+    int32_t cg[8];
+    int rc = eval_pass(h, ci, R, t, g, b, nullptr, cg);
+    if (rc) return rc;
+    double upd[8];
+    for (int k = 0; k < 8; k++) {
+      const double grad = cg[k] / 64.0;
+      m[k] = b1 * m[k] + (1 - b1) * grad;
+      v[k] = b2 * v[k] + (1 - b2) * grad * grad;
+      const double mh = m[k] / (1 - std::pow(b1, (double)s));
+      const double vh = v[k] / (1 - std::pow(b2, (double)s));
+      upd[k] = (k < 3 ? rotLr : (k < 6 ? trnLr : expLr)) * mh / (std::sqrt(vh) + eps);
+    }
+    apply_pose_step(upd, R, t);
+    if (h->o.expComp) {
+      g = clampd(g - upd[6], -0.1, 0.1);
+      b = clampd(b - upd[7], -0.05, 0.05);
+    }
+  }
+  return eval_pass(h, ci, R, t, g, b, psnr, nullptr);
+}
+
+int gs_get_camera(gs_trainer_t* h, int32_t ci, gs_cam_state* out) {
+  if (!h || !out) return GS_E_ARG;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  if (ci < 0 || ci >= (int)h->cams.size()) return fail(h, GS_E_ARG, "camera index out of range");
+  const CamMeta& m = h->cams[ci];
+  memcpy(out->R, m.R, sizeof out->R);
+  memcpy(out->t, m.t, sizeof out->t);
+  out->f = m.f;
+  out->g = m.g;
+  out->b = m.b;
+  return GS_OK;
+}
+
+int gs_read_cam_grads(gs_trainer_t* h, int32_t* out, int32_t count) {
+  if (!h || !out) return GS_E_ARG;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  const int64_t total = (int64_t)(h->cams.size() + 1) * 8;
+  if (count < total) return fail(h, GS_E_ARG, "readCamGrads: buffer smaller than (numCams + 1) * 8");
+  HIPCK(hipMemcpyAsync(out, h->camGrad, total * 4, hipMemcpyDeviceToHost, h->st));
+  HIPCK(hipStreamSynchronize(h->st));
   return GS_OK;
 }
 
@@ -1008,9 +1262,9 @@ int gs_export_ply(gs_trainer_t* h, int32_t bake, void* buf, int64_t cap, int64_t
   const int nc = (int)h->cams.size();
   std::vector<float> cp(3 * (size_t)nc);
   for (int c = 0; c < nc; c++) {
-    const gs_camera& k = h->cams[c].c;
+    const CamMeta& k = h->cams[c];
     for (int a = 0; a < 3; a++)
-      cp[3 * c + a] = (float)(-(((double)k.R[a] * k.t[0] + (double)k.R[3 + a] * k.t[1]) + (double)k.R[6 + a] * k.t[2]));
+      cp[3 * c + a] = (float)(-((k.R[a] * k.t[0] + k.R[3 + a] * k.t[1]) + k.R[6 + a] * k.t[2]));
   }
   float* dCam = nullptr;
   float* dOut = nullptr;
@@ -1018,7 +1272,7 @@ int gs_export_ply(gs_trainer_t* h, int32_t bake, void* buf, int64_t cap, int64_t
   hipError_t e = hipMalloc(&dOut, (size_t)std::max<int64_t>(body, 4));
   if (e == hipSuccess) e = hipMemcpy(dCam, cp.data(), cp.size() * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
-    launch_ply_rows(h->st, h->shDeg, h->mixed, n, h->cap, h->P, h->SH, bake ? 1 : 0, (double)h->cams[0].c.f, dCam,
+    launch_ply_rows(h->st, h->shDeg, h->mixed, n, h->cap, h->P, h->SH, bake ? 1 : 0, h->cams[0].f, dCam,
                     nc, dOut);
     e = hipGetLastError();
   }

@@ -64,71 +64,14 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
     gp[2] = gp[2] * cden;
     gp[3] = gp[3] * cden;
     gp[4] = gp[4] * cden;
-    const float f = cam.f;
     const float iz = 1.0f / g.pc[2];
 
-    // conic -> dilated 2D covariance (:759-777)
-    const float ad = g.va + 0.3f, cd = g.vc + 0.3f;
-    const float detVd = ad * cd - g.vb * g.vb;
-    const float inv = 1.0f / detVd;
-    const float kA = cd * inv, kB = (-g.vb) * inv, kC = ad * inv;
-    const float g11 = gp[2], g12 = 0.5f * gp[3], g22 = gp[4];
-    const float m100 = g11 * kA + g12 * kB, m101 = g11 * kB + g12 * kC;
-    const float m110 = g12 * kA + g22 * kB, m111 = g12 * kB + g22 * kC;
-    const float p00 = kA * m100 + kB * m110, p01 = kA * m101 + kB * m111;
-    const float p10 = kB * m100 + kC * m110, p11 = kB * m101 + kC * m111;
-    float gva = -p00, gvb = -(p01 + p10), gvc = -p11;
-
-    // comp -> raw 2D covariance (:780-788)
-    const float detV = fmaxf(g.va * g.vc - g.vb * g.vb, 0.0f);
-    const float comp = sqrtf(fmaxf(detV / detVd, 0.0f));
-    if (comp > 1e-4f) {
-      const float gcomp = gp[5];
-      const float denom = 2.0f * comp * detVd;
-      gva = gva + gcomp * (g.vc - comp * comp * cd) / denom;
-      gvc = gvc + gcomp * (g.va - comp * comp * ad) / denom;
-      gvb = gvb + gcomp * (2.0f * g.vb * (comp * comp - 1.0f)) / denom;
-    }
-
-    // V = T Sigma T^T backward (:791-810)
-    const float h11 = gva, h12 = 0.5f * gvb, h22 = gvc;
-    const float* t0 = g.t0;
-    const float* t1 = g.t1;
-    float u0[3], u1[3];
-    for (int k = 0; k < 3; k++) { u0[k] = h11 * t0[k] + h12 * t1[k]; u1[k] = h12 * t0[k] + h22 * t1[k]; }
-    const float dS00 = t0[0] * u0[0] + t1[0] * u1[0];
-    const float dS01 = 0.5f * ((t0[0] * u0[1] + t1[0] * u1[1]) + (t0[1] * u0[0] + t1[1] * u1[0]));
-    const float dS02 = 0.5f * ((t0[0] * u0[2] + t1[0] * u1[2]) + (t0[2] * u0[0] + t1[2] * u1[0]));
-    const float dS11 = t0[1] * u0[1] + t1[1] * u1[1];
-    const float dS12 = 0.5f * ((t0[1] * u0[2] + t1[1] * u1[2]) + (t0[2] * u0[1] + t1[2] * u1[1]));
-    const float dS22 = t0[2] * u0[2] + t1[2] * u1[2];
-    float sT0[3], sT1[3];
-    sT0[0] = (g.s00 * t0[0] + g.s01 * t0[1]) + g.s02 * t0[2];
-    sT0[1] = (g.s01 * t0[0] + g.s11 * t0[1]) + g.s12 * t0[2];
-    sT0[2] = (g.s02 * t0[0] + g.s12 * t0[1]) + g.s22 * t0[2];
-    sT1[0] = (g.s00 * t1[0] + g.s01 * t1[1]) + g.s02 * t1[2];
-    sT1[1] = (g.s01 * t1[0] + g.s11 * t1[1]) + g.s12 * t1[2];
-    sT1[2] = (g.s02 * t1[0] + g.s12 * t1[1]) + g.s22 * t1[2];
-    float dT0[3], dT1[3];
-    for (int k = 0; k < 3; k++) {
-      dT0[k] = 2.0f * (h11 * sT0[k] + h12 * sT1[k]);
-      dT1[k] = 2.0f * (h12 * sT0[k] + h22 * sT1[k]);
-    }
-
-    // T = J W (:813-822) and the mean path (:825-827)
-    const float dJ00 = dot3(dT0, cam.R + 0);
-    const float dJ02 = dot3(dT0, cam.R + 6);
-    const float dJ11 = dot3(dT1, cam.R + 3);
-    const float dJ12 = dot3(dT1, cam.R + 6);
-    float dpc[3] = {0.0f, 0.0f, 0.0f};
-    dpc[0] = dpc[0] + dJ02 * ((-f) * iz * iz);
-    dpc[1] = dpc[1] + dJ12 * ((-f) * iz * iz);
-    dpc[2] = dpc[2] + ((dJ00 + dJ11) * ((-f) * iz * iz)
-           + dJ02 * (2.0f * f * g.pc[0] * iz * iz * iz))
-           + dJ12 * (2.0f * f * g.pc[1] * iz * iz * iz);
-    dpc[0] = dpc[0] + gp[0] * f * iz;
-    dpc[1] = dpc[1] + gp[1] * f * iz;
-    dpc[2] = dpc[2] + (-f) * (gp[0] * g.pc[0] + gp[1] * g.pc[1]) * iz * iz;
+    // screen space -> Sigma, T, J and the camera-space position (:759-827)
+    ChainScreen cs;
+    chain_screen(cam, g, gp, iz, cs);
+    const float dS00 = cs.dS00, dS01 = cs.dS01, dS02 = cs.dS02;
+    const float dS11 = cs.dS11, dS12 = cs.dS12, dS22 = cs.dS22;
+    const float* dpc = cs.dpc;
     const float* R = cam.R;
     gF[0] = (R[0] * dpc[0] + R[3] * dpc[1]) + R[6] * dpc[2];
     gF[1] = (R[1] * dpc[0] + R[4] * dpc[1]) + R[7] * dpc[2];

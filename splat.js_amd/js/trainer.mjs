@@ -5,7 +5,8 @@
 //   GSTrainer.create(opts) / setup(gaussians, cams, images, maxViewW,
 //   maxViewH, sceneRadius) / stepOnce() / readLoss() / renderView(cam, out,
 //   trainMode, offset) / renderTrainCam(ci, out) / readGaussians() /
-//   evalCamPsnr(ci), the fields callers read (iter, n, cap, camMeta,
+//   evalCamPsnr(ci) / evalCamPsnrRefined(ci, steps), the fields callers read
+//   (iter, n, cap, camMeta (kept current under opts.camOpt),
 //   holdout, excluded, lastCam, horizon, shK, stride, tileGrad, opts) and
 //   device.queue.onSubmittedWorkDone().
 // Differences: renderView writes into a caller Float32Array (w*h*4) instead
@@ -72,10 +73,11 @@ export class GSTrainer {
     });
     for (const m of this.camMeta) m.f0 = m.f;
     this.horizon = this.opts.maxIters === undefined ? 60000 : this.opts.maxIters;
+    this._iter = 0;   // host-side copy of iter: stepOnce reads it without a device sync
   }
 
   get iter() { return addon.info(this._h).iter; }
-  set iter(v) { addon.setIter(this._h, v); }
+  set iter(v) { addon.setIter(this._h, v); this._iter = v; }
   get n() { return addon.info(this._h).n; }
   get cap() { return addon.info(this._h).cap; }
   get lastCam() { return addon.info(this._h).lastCam; }
@@ -90,6 +92,17 @@ export class GSTrainer {
       this._excludedSent = this.excluded;
     }
     addon.step(this._h, cam);
+    this._iter++;
+    // trainer.js:468-470, :533-535: a camera Adam step just moved the poses,
+    // the shared focal and the exposures; mirror them into camMeta
+    if (this.opts.camOpt && this._iter > 1500 && this._iter % 25 === 0) this._syncCamMeta();
+  }
+
+  _syncCamMeta() {
+    for (let ci = 0; ci < this.camMeta.length; ci++) {
+      const c = addon.getCamera(this._h, ci);
+      Object.assign(this.camMeta[ci], { R: c.R, t: c.t, f: c.f, g: c.g, b: c.b });
+    }
   }
 
   /** trainer.js:615-631: MSE per valid pixel-channel since the last call */
@@ -123,6 +136,20 @@ export class GSTrainer {
   async evalCamPsnr(ci) {
     this.pixelsSeen = 0;
     return addon.evalCamPsnr(this._h, ci);
+  }
+
+  /** trainer.js:583-612: PSNR after test-time refinement of camera ci's
+   *  pose (Gaussians frozen); the stored camera is unchanged */
+  async evalCamPsnrRefined(ci, steps = 200) {
+    this.pixelsSeen = 0;
+    return addon.evalCamPsnrRefined(this._h, ci, steps);
+  }
+
+  /** the camera gradient rows (bufCamGrad, trainer.js:173) as an Int32Array */
+  readCamGrads() {
+    const out = new Int32Array((this.camMeta.length + 1) * 8);
+    addon.readCamGrads(this._h, out);
+    return out;
   }
 
   /** trainer.js:666-794. The plan's random stream is the reference's

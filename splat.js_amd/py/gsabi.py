@@ -27,13 +27,19 @@ class GsOpts(C.Structure):
                 ("maxSplats", C.c_int64), ("capMult", C.c_float), ("entriesCap", C.c_int64),
                 ("tileGrad", C.c_int32), ("posLr", C.c_float), ("loss", C.c_int32),
                 ("ssimLambda", C.c_float), ("gradAccum", C.c_int32), ("sortMode", C.c_int32),
-                ("seed", C.c_uint32), ("device", C.c_int32), ("precision", C.c_int32)]
+                ("seed", C.c_uint32), ("device", C.c_int32), ("precision", C.c_int32),
+                ("camOpt", C.c_int32), ("expComp", C.c_int32)]
 
 
 class GsCamera(C.Structure):
     _fields_ = [("R", C.c_float * 9), ("t", C.c_float * 3), ("f", C.c_float), ("cx", C.c_float),
                 ("cy", C.c_float), ("w", C.c_int32), ("h", C.c_int32), ("imgIdx", C.c_int32),
                 ("g", C.c_float), ("b", C.c_float)]
+
+
+class GsCamState(C.Structure):
+    _fields_ = [("R", C.c_double * 9), ("t", C.c_double * 3), ("f", C.c_double), ("g", C.c_double),
+                ("b", C.c_double)]
 
 
 class GsImage(C.Structure):
@@ -71,6 +77,9 @@ SIGNATURES = {
     "gs_render": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_int64, C.c_void_p], C.c_int),
     "gs_read_gaussians": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "gs_eval_cam_psnr": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
+    "gs_eval_cam_psnr_refined": ([C.c_void_p, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
+    "gs_get_camera": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
+    "gs_read_cam_grads": ([C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
     "gs_set_holdout": ([C.c_void_p, C.c_int32], C.c_int),
     "gs_set_excluded": ([C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
     "gs_set_iter": ([C.c_void_p, C.c_int64], C.c_int),
@@ -235,6 +244,25 @@ class GSTrainer:
         v = C.c_double()
         self._ck(lib().gs_eval_cam_psnr(self._h, int(ci), C.byref(v)))
         return v.value
+
+    def evalCamPsnrRefined(self, ci, steps=200):
+        v = C.c_double()
+        self._ck(lib().gs_eval_cam_psnr_refined(self._h, int(ci), int(steps), C.byref(v)))
+        return v.value
+
+    def camState(self, ci):
+        """The current pose, focal and exposure of camera ci (doubles): the
+        fields of trainer.camMeta[ci] that camOpt updates."""
+        st = GsCamState()
+        self._ck(lib().gs_get_camera(self._h, int(ci), C.byref(st)))
+        return {"R": list(st.R), "t": list(st.t), "f": st.f, "g": st.g, "b": st.b}
+
+    def readCamGrads(self):
+        """(numCams + 1) x 8 int32 camera gradient rows (x64 fixed point)."""
+        rows = self.info().numCams + 1
+        out = np.zeros((rows, 8), np.int32)
+        self._ck(lib().gs_read_cam_grads(self._h, _p(out), out.size))
+        return out
 
     @property
     def holdout(self):

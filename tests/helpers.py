@@ -4,6 +4,7 @@ import math
 
 import numpy as np
 
+import camopt as CO
 import oracle as O
 import scenes
 
@@ -51,6 +52,11 @@ class OracleTrainer:
             im = images[c["imgIdx"]]
             self.tgt.append(O.pack_targets(np.asarray(im["rgb"], np.float32).reshape(-1, 3)))
         self.cam_deg = [shDeg] * len(cams)
+        # camera optimiser (opts camOpt / expComp): double camera state and
+        # the (numCams + 1) x 8 gradient rows, wrapped to i32 when read
+        self.camopt = CO.CamState(cams) if opts.get("camOpt") else None
+        self.camGrad = np.zeros((len(cams) + 1, 8), np.int64)
+        self.holdout = -1
         # mixed precision: SH coefficients and all moments live in bf16 (uint16 bits)
         self.mixed = precision == "mixed"
         if self.mixed:
@@ -61,21 +67,31 @@ class OracleTrainer:
             self.SM16 = np.zeros(self.SH.shape, np.uint16)
             self.SV16 = np.zeros(self.SH.shape, np.uint16)
 
+    def cam_grads(self):
+        return CO.wrap_i32(self.camGrad)
+
     def step(self, ci):
         deg = min(self.shDeg, self.iter // 1000)
         self.cam_deg[ci] = deg
-        cam = O.make_cam(self.cams[ci], shActiveDeg=deg)
+        cd = self.camopt.cam_dict(ci) if self.camopt else self.cams[ci]
+        cam = O.make_cam(cd, shActiveDeg=deg)
         self.iter += 1
         st = self.st
+        co = self.camopt is not None
         proj, rect, cnt = st.project(cam, self.P, self.SH if self.K else None)
         ids, ts, keys = st.bin_sort(cam, proj, rect, cnt)
         img, T, end = st.render_fwd(cam, proj, ids, ts)
-        gC, valid, stats = st.loss_charb(cam, img, self.tgt[ci])
+        lo = st.loss_charb(cam, img, self.tgt[ci], cam_grads=co)
+        gC, valid, stats = lo[:3]
         g = st.render_bwd(cam, proj, ids, ts, T, end, gC, valid, fixed=self.fixed)
-        if self.fixed:
-            gradF, shG = st.chain(cam, self.P, self.SH if self.K else None, proj, gfix=g)
-        else:
-            gradF, shG = st.chain(cam, self.P, self.SH if self.K else None, proj, gsum=g)
+        sh = self.SH if self.K else None
+        ch = (st.chain(cam, self.P, sh, proj, gfix=g, cam_grads=co) if self.fixed
+              else st.chain(cam, self.P, sh, proj, gsum=g, cam_grads=co))
+        gradF, shG = ch[:2]
+        if co:
+            self.camGrad[ci, 6:8] += lo[3]
+            self.camGrad[ci, 0:6] += ch[2][:6]
+            self.camGrad[len(self.cams), 0] += ch[2][6]
         hp = adam_for_iter(self.iter, self.radius, posLr=self.opts.get("posLr", 0.0),
                            horizon=self.opts.get("maxIters", 60000),
                            opacityReg=self.opts.get("opacityReg", 0.01), shLr=self.opts.get("shLr", 1.25e-4))
@@ -86,11 +102,40 @@ class OracleTrainer:
             if self.K:
                 O.sh_adam_mixed(hp, self.SH16, shG, self.SM16, self.SV16, self.iter)
                 self.SH = O.bf2f(self.SH16)
-            return pre
-        O.adam(hp, self.P, gradF, self.M, self.V)
-        if self.K:
-            O.sh_adam(hp, self.SH, shG, self.SM, self.SV)
+        else:
+            O.adam(hp, self.P, gradF, self.M, self.V)
+            if self.K:
+                O.sh_adam(hp, self.SH, shG, self.SM, self.SV)
+        if co:
+            pre["camGrad"] = self.cam_grads()
+            if self.iter > 1500 and self.iter % 25 == 0:      # trainer.js:468-470
+                self.camopt.apply(self.cam_grads(), self.iter, self.opts.get("maxIters", 60000), f32(self.radius),
+                                  holdout=self.holdout, exp_comp=bool(self.opts.get("expComp")))
+                self.camGrad[:] = 0
         return pre
+
+    def eval_pass(self, ci, override=None):
+        """_evalPass (trainer.js:542-573): training-path pass on camera ci at
+        an optional pose override, no update. Returns (psnr, the camera's 8
+        gradient slots); all rows are drained afterwards."""
+        src = self.camopt.cam_dict(ci, override) if self.camopt else dict(self.cams[ci], **(override or {}))
+        cam = O.make_cam(src, shActiveDeg=self.shDeg)
+        st = self.st
+        proj, rect, cnt = st.project(cam, self.P, self.SH if self.K else None)
+        ids, ts, _ = st.bin_sort(cam, proj, rect, cnt)
+        img, T, end = st.render_fwd(cam, proj, ids, ts)
+        gC, valid, stats, efix, _ = st.loss_charb(cam, img, self.tgt[ci], cam_grads=True)
+        g = st.render_bwd(cam, proj, ids, ts, T, end, gC, valid, fixed=self.fixed)
+        sh = self.SH if self.K else None
+        ch = (st.chain(cam, self.P, sh, proj, gfix=g, cam_grads=True) if self.fixed
+              else st.chain(cam, self.P, sh, proj, gsum=g, cam_grads=True))
+        row = self.camGrad[ci].copy()
+        row[6:8] += efix
+        row[0:6] += ch[2][:6]
+        self.camGrad[:] = 0
+        mse = stats[0] / (max(1.0, stats[2]) * 3.0)
+        psnr = -10.0 * math.log10(mse) if mse > 0 else math.inf
+        return psnr, CO.wrap_i32(row)
 
 
 def rig_setup(sh_deg=0):

@@ -8,12 +8,14 @@ const require = createRequire(import.meta.url);
 const addon = require('../../splat.js_amd/addon/gs_napi.node');
 assert.strictEqual(addon.loaded(), true, addon.loadError());
 for (const f of ['create', 'setup', 'step', 'readLoss', 'render', 'readGaussians', 'evalCamPsnr',
-  'setHoldout', 'setExcluded', 'setIter', 'info', 'sync']) {
+  'setHoldout', 'setExcluded', 'setIter', 'info', 'sync', 'refine', 'exportPly', 'evalCamPsnrRefined',
+  'getCamera', 'readCamGrads']) {
   assert.strictEqual(typeof addon[f], 'function', f);
 }
 assert.strictEqual(typeof api.createTrainer, 'function');
 assert.strictEqual(typeof api.GSTrainer.create, 'function');
-for (const m of ['setup', 'stepOnce', 'readLoss', 'renderView', 'renderTrainCam', 'readGaussians', 'evalCamPsnr']) {
+for (const m of ['setup', 'stepOnce', 'readLoss', 'renderView', 'renderTrainCam', 'readGaussians', 'evalCamPsnr',
+  'evalCamPsnrRefined', 'refine', 'exportPly', 'readCamGrads']) {
   assert.strictEqual(typeof api.GSTrainer.prototype[m], 'function', m);
 }
 console.log(JSON.stringify({ ok: true }));
