@@ -58,6 +58,11 @@ __device__ __forceinline__ void load_params(const float4* __restrict__ P, int64_
   p[12] = d.x; p[13] = d.y; p[14] = d.z; p[15] = d.w;
 }
 
+// Wave vote on a lane predicate. The builtin takes the bool itself; HIP's
+// __ballot(int) converts it to an int and back, which costs two VALU
+// instructions per vote.
+__device__ __forceinline__ uint64_t wave_ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+
 __device__ __forceinline__ float clampf(float x, float lo, float hi) {
   return fminf(fmaxf(x, lo), hi);
 }

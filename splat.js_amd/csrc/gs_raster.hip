@@ -55,7 +55,7 @@ __device__ __forceinline__ void build_masks(uint32_t cover, uint64_t (*sMask)[4]
   const int tid = threadIdx.x, c = tid >> 6;
 #pragma unroll
   for (int w = 0; w < 4; w++) {
-    const uint64_t bal = __ballot((cover >> w) & 1u);
+    const uint64_t bal = wave_ballot(((cover >> w) & 1u) != 0u);
     if ((tid & 63) == 0) sMask[c][w] = bal;
   }
 }
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
     build_masks(cover, sMask);
     __syncthreads();
     const uint32_t cnt = min((uint32_t)kBlock, e - base);
-    if (__ballot(!done) != 0ull) {
+    if (wave_ballot(!done) != 0ull) {
       uint32_t term = 0xffffffffu;  // batch index of this pixel's last splat
       for (int c = 0; c < 4; c++) {
         uint64_t m = uniform64(sMask[c][wv]);
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
           const float t2 = fmaf(bb.x * dy, dy, (a.z * dx) * dx);
           const float ee = fmaxf(fmaf(0.5f, t2, (a.w * dx) * dy), 0.0f);
           const bool live = !done && ee <= cut.eCut;
-          if (__ballot(live) != 0ull) {
+          if (wave_ballot(live) != 0ull) {
             const float araw = bb.y * gs_expf_nonpos(-ee);   // == gs_expf(-ee) for ee <= eCut <= 80
             const float alpha = fminf(0.99f, araw);
             if (live && alpha >= cut.aMin) {
@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
               T = T * (1.0f - alpha);
               if (T < 1e-4f) { done = true; term = (uint32_t)j; }
             }
-            if (__ballot(!done) == 0ull) { m = 0ull; c = 4; }
+            if (wave_ballot(!done) == 0ull) { m = 0ull; c = 4; }
           }
         }
       }
@@ -303,6 +303,13 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
   __syncthreads();
   const uint32_t endMax = sEnd;
   const float gC[3] = {g.x, g.y, g.z};
+  // GS_ACCUM_FIXED: the x16384 fixed-point scale rides on the two factors
+  // every term carries (galpha and gC). Scaling by a power of two is exact,
+  // so each scaled term equals the term times 16384 bit for bit; the only
+  // exceptions are terms whose unscaled value is subnormal, and those
+  // truncate to zero either way.
+  constexpr float kS = kAccum == 0 ? kFixed : 1.0f;
+  const float gCs[3] = {gC[0] * kS, gC[1] * kS, gC[2] * kS};
   // dot(gC, S) of the colour-behind sum S, which enters only through it
   float GS = fmaf(gC[2], cam.bg[2], fmaf(gC[1], cam.bg[1], gC[0] * cam.bg[0])) * Ta;
   for (uint32_t hi = endMax; hi > s; ) {
@@ -344,12 +351,12 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
         const float t2 = fmaf(cC * dy, dy, (cA * dx) * dx);
         const float e = fmaxf(fmaf(0.5f, t2, (cB * dx) * dy), 0.0f);
         const bool live = active && k < myEnd && e <= cut.eCut;
-        if (__ballot(live) == 0ull) continue;
+        if (wave_ballot(live) == 0ull) continue;
         const float G = gs_expf_nonpos(-e);   // == gs_expf(-e) for e <= eCut <= 80
         const float araw = bb.y * G;
         const float alpha = fminf(0.99f, araw);
         const bool contrib = live && alpha >= cut.aMin;
-        if (__ballot(contrib) == 0ull) continue;
+        if (wave_ballot(contrib) == 0ull) continue;
         float v[10];
         {
           const float4 c4 = sC[j];
@@ -366,35 +373,32 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
           // other factors are finite), and leaves the taking lanes' bits as is.
           const float galpha = (contrib && !(araw > 0.99f)) ? fmaf(gcol, Tb, -(GS * rom)) : 0.0f;
           const float aT = contrib ? alpha * Tb : 0.0f;
-          const float ga = galpha * araw;
+          const float galphaS = galpha * kS;
+          const float ga = galphaS * araw;
           const float gq = ga * d2.x;
           const float gqdx = gq * dx;
           v[0] = ga * (cA * dx + cB * dy);
           v[1] = ga * (cB * dx + cC * dy);
           v[2] = gqdx * dx;
-          v[3] = gqdx * dy;                 // half of the term: doubled in the fixed-point scale
+          v[3] = 2.0f * (gqdx * dy);        // the middle conic term is 2 gq dx dy
           v[4] = (gq * dy) * dy;
-          v[5] = galpha * c4.w * G;
+          v[5] = galphaS * c4.w * G;
           v[6] = ga * d2.y;
-          v[7] = gC[0] * aT;
-          v[8] = gC[1] * aT;
-          v[9] = gC[2] * aT;
+          v[7] = gCs[0] * aT;
+          v[8] = gCs[1] * aT;
+          v[9] = gCs[2] * aT;
           GS = fmaf(gcol, aT, GS);
           Ta = contrib ? Tb : Ta;
         }
         const uint32_t gid = sId[j];
         if (kAccum == 0) {
+          // i32(clamp(v * 16384, -1e9, 1e9)) per pixel; v already carries the scale
           uint32_t iv[10];
 #pragma unroll
-          for (int q = 0; q < 10; q++) {
-            // 2 * (v * 16384) == v * 32768 exactly (power-of-two scalings)
-            const float cl = fminf(fmaxf(v[q] * (q == 3 ? 2.0f * kFixed : kFixed), -1.0e9f), 1.0e9f);
-            iv[q] = (uint32_t)(int32_t)cl;
-          }
+          for (int q = 0; q < 10; q++) iv[q] = (uint32_t)(int32_t)fminf(fmaxf(v[q], -1.0e9f), 1.0e9f);
           const uint32_t r = wave_reduce10<uint32_t>(iv);
           if (flusher && r != 0u) atomicAdd((uint32_t*)accum + (int64_t)gid * 12 + mySlot, r);
         } else {
-          v[3] = 2.0f * v[3];
           const float r = wave_reduce10<float>(v);
           if (flusher && r != 0.0f) unsafeAtomicAdd((float*)accum + (int64_t)gid * 12 + mySlot, r);
         }

@@ -44,7 +44,7 @@ __device__ __forceinline__ uint64_t match_digit8(uint32_t d, uint64_t active) {
 #pragma unroll
   for (int b = 0; b < 8; b++) {
     const bool bit = (d >> b) & 1u;
-    const uint64_t bal = __ballot(bit);
+    const uint64_t bal = wave_ballot(bit);
     m &= bit ? bal : ~bal;
   }
   return m;
@@ -95,7 +95,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const KeyT* __restrict__ 
 #pragma unroll
     for (int i = 0; i < kItems; i++) {
       const bool ok = d[i] < 0x100u;
-      const uint64_t act = __ballot(ok);
+      const uint64_t act = wave_ballot(ok);
       if (act == 0ull) break;
       const uint64_t m = match_digit8(d[i] & 0xffu, act);
       // one lane per digit group: distinct addresses within an instruction
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const KeyT* __restrict
 #pragma unroll
     for (int i = 0; i < kItems; i++) {
       const bool ok = rank[i] == 0u;
-      mm[i] = match_digit8(digit_of(key[i], shift), __ballot(ok)) ;
+      mm[i] = match_digit8(digit_of(key[i], shift), wave_ballot(ok)) ;
       old[i] = 0u;
       if (ok && (mm[i] & lanemask_lt()) == 0ull)
         old[i] = atomicAdd(&wc[w][digit_of(key[i], shift)], (uint32_t)__popcll(mm[i]));
