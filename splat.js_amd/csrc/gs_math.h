@@ -78,11 +78,11 @@ GS_HD float gs_expf(float x) {
 
 // gs_expf without branches for the rasterizer's inner loops. For x in
 // [-87, 0] the operations are exactly those of gs_expf (k = round(x/ln2) lies
-// in [-126, 0], the normal-result path), so the bits agree. Other inputs
-// (including NaN) are clamped into that range first and give a finite,
-// positive value that callers discard.
-GS_HD float gs_expf_nonpos(float x) {
-  const float xc = fminf(fmaxf(x, -87.0f), 0.0f);
+// in [-126, 0], the normal-result path), so the bits agree.
+// gs_expf_nonpos_in takes x already in [-87, 0] (the caller clamps);
+// gs_expf_nonpos clamps any input (including NaN) into that range first, which
+// gives a finite, positive value that callers discard.
+GS_HD float gs_expf_nonpos_in(float xc) {
   const float kf = rintf(xc * 1.44269502162933349609375f);
   float r = fmaf(kf, -0.693145751953125f, xc);
   r = fmaf(kf, -1.428606765330187045e-06f, r);
@@ -97,6 +97,8 @@ GS_HD float gs_expf_nonpos(float x) {
   const int k = (int)kf;
   return p * gs_u2f((uint32_t)(k + 127) << 23);
 }
+
+GS_HD float gs_expf_nonpos(float x) { return gs_expf_nonpos_in(fminf(fmaxf(x, -87.0f), 0.0f)); }
 
 // natural log; NaN for x < 0, -inf at 0, +inf at +inf.
 GS_HD float gs_logf(float x) {

@@ -123,10 +123,12 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
           const float4 bb = sB[j];
           const float dx = px - a.x, dy = py - a.y;
           const float t2 = fmaf(bb.x * dy, dy, (a.z * dx) * dx);
-          const float ee = fmaxf(fmaf(0.5f, t2, (a.w * dx) * dy), 0.0f);
+          // clamped to [0, 87]: an exponent above 87 > eCut is never taken,
+          // and the exp below needs its argument in [-87, 0]
+          const float ee = fminf(fmaxf(fmaf(0.5f, t2, (a.w * dx) * dy), 0.0f), 87.0f);
           const bool live = !done && ee <= cut.eCut;
           if (wave_ballot(live) != 0ull) {
-            const float araw = bb.y * gs_expf_nonpos(-ee);   // == gs_expf(-ee) for ee <= eCut <= 80
+            const float araw = bb.y * gs_expf_nonpos_in(-ee);   // == gs_expf(-ee) for ee <= eCut <= 80
             const float alpha = fminf(0.99f, araw);
             if (live && alpha >= cut.aMin) {
               const float4 c4 = sC[j];
@@ -349,10 +351,11 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
         const float dx = px - a.x, dy = py - a.y;
         const float cA = a.z, cB = a.w, cC = bb.x;
         const float t2 = fmaf(cC * dy, dy, (cA * dx) * dx);
-        const float e = fmaxf(fmaf(0.5f, t2, (cB * dx) * dy), 0.0f);
+        // clamped to [0, 87] as in the forward pass
+        const float e = fminf(fmaxf(fmaf(0.5f, t2, (cB * dx) * dy), 0.0f), 87.0f);
         const bool live = active && k < myEnd && e <= cut.eCut;
         if (wave_ballot(live) == 0ull) continue;
-        const float G = gs_expf_nonpos(-e);   // == gs_expf(-e) for e <= eCut <= 80
+        const float G = gs_expf_nonpos_in(-e);   // == gs_expf(-e) for e <= eCut <= 80
         const float araw = bb.y * G;
         const float alpha = fminf(0.99f, araw);
         const bool contrib = live && alpha >= cut.aMin;

@@ -108,6 +108,7 @@ int main(void) {
   for (long long mm = 0; mm <= (long long)top; mm++) {
     const float x = gs_u2f(0x80000000u | (uint32_t)mm);
     n++;
+    if (gs_f2u(gs_expf_nonpos_in(x)) != gs_f2u(gs_expf(x))) bad++;
     if (gs_f2u(gs_expf_nonpos(x)) != gs_f2u(gs_expf(x))) bad++;
   }
   /* outside the range: finite and positive */
@@ -124,8 +125,9 @@ int main(void) {
 
 
 def test_branch_free_exp_matches_portable_exp_exhaustively(tmp_path):
-    """gs_expf_nonpos (raster inner loops) == gs_expf bit for bit on every
-    float of [-87, 0], the range eCut <= 80 confines it to."""
+    """gs_expf_nonpos_in (raster inner loops, argument clamped to [-87, 0] by
+    the caller) and gs_expf_nonpos == gs_expf bit for bit on every float of
+    [-87, 0]; the rasterizer takes a splat only at exponents <= eCut <= 80."""
     src = tmp_path / "m.c"
     src.write_text(MATH_PROBE)
     exe = tmp_path / "m"
