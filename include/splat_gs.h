@@ -38,7 +38,7 @@ enum {
 
 enum { GS_LOSS_CHARBONNIER = 0, GS_LOSS_L1_DSSIM = 1 };
 enum { GS_ACCUM_FIXED = 0, GS_ACCUM_F32 = 1 };
-enum { GS_SORT_DEPTH_THEN_TILE = 0, GS_SORT_GLOBAL64 = 1 };
+enum { GS_SORT_DEPTH_THEN_TILE = 0, GS_SORT_GLOBAL64 = 1, GS_SORT_TILE_SEGMENTS = 2 };
 enum { GS_PREC_F32 = 0, GS_PREC_MIXED = 1 };
 
 /* Trainer options: the reference's opts object (trainer.js:18-36, :45-49,
@@ -62,7 +62,10 @@ typedef struct gs_opts {
   int32_t loss;        /* GS_LOSS_CHARBONNIER (the reference's loss) or GS_LOSS_L1_DSSIM */
   float ssimLambda;    /* D-SSIM weight for GS_LOSS_L1_DSSIM, default 0.2 */
   int32_t gradAccum;   /* GS_ACCUM_FIXED (the reference's x16384 i32 sums) or GS_ACCUM_F32 */
-  int32_t sortMode;    /* key ordering strategy; both produce the identical permutation */
+  int32_t sortMode;    /* key ordering strategy, all giving the identical permutation:
+                          GS_SORT_TILE_SEGMENTS (default): per-tile segments sorted in LDS, with
+                          the reference's entry budget (whole tiles dropped, shaders.js:321-370);
+                          the two radix strategies drop the keys past the budget in sort order */
   uint32_t seed;       /* camera-pick RNG seed (the reference uses Math.random, trainer.js:344) */
   int32_t device;      /* HIP device ordinal, default 0 */
   int32_t precision;   /* GS_PREC_F32 (default), or GS_PREC_MIXED: SH coefficients and all Adam

@@ -219,6 +219,54 @@ class Stage:
         return gradF, shGrad[:, :3 * K]
 
 
+SHARED_SORT = 2048     # shaders.js:33
+SCAN_CHUNK = 64        # shaders.js:325
+
+
+def padded_size(c):
+    """shaders.js:329-332: segments over SHARED_SORT take a power-of-two slot."""
+    c = int(c)
+    return c if c <= SHARED_SORT else 1 << (c - 1).bit_length()
+
+
+def tile_budget(tile_cnt, cap):
+    """The reference's budgeted segment scan (shaders.js:321-370): thread li
+    of 256 owns tiles [64 li, 64 li + 64); its chunk starts at the exclusive
+    prefix of the padded sizes of earlier chunks (no drops applied); inside
+    the chunk a tile whose padded size takes the running total past the
+    budget is dropped (counted even when empty). Returns (kept count per
+    tile, dropped tiles)."""
+    cnt = [int(c) for c in tile_cnt]
+    T = len(cnt)
+    nch = (T + SCAN_CHUNK - 1) // SCAN_CHUNK
+    sums = [sum(padded_size(c) for c in cnt[k * SCAN_CHUNK:(k + 1) * SCAN_CHUNK]) for k in range(nch)]
+    kept = [0] * T
+    dropped = 0
+    pre = 0
+    for k in range(nch):
+        acc = pre
+        for t in range(k * SCAN_CHUNK, min(T, (k + 1) * SCAN_CHUNK)):
+            p = padded_size(cnt[t])
+            if acc + p > cap:
+                p = 0
+                dropped += 1
+            acc += p
+            kept[t] = cnt[t] if p else 0
+        pre += sums[k]
+    return np.array(kept, np.int64), dropped
+
+
+def apply_tile_budget(ids, keys, ts, cap):
+    """A bin_sort result with the budget applied: the keys of dropped tiles
+    removed, tile starts recomputed. Returns (ids, keys, ts, dropped)."""
+    cnt = np.diff(ts.astype(np.int64))
+    kept, dropped = tile_budget(cnt, cap)
+    keep = np.repeat(kept == cnt, cnt)
+    ts2 = np.zeros(len(ts), np.uint32)
+    ts2[1:] = np.cumsum(kept)
+    return ids[keep], keys[keep], ts2, dropped
+
+
 def pack_targets(rgb):
     """float RGB (npx x 3, r < 0 = invalid) -> packed RGBA8, as trainer.js:128-139."""
     rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)

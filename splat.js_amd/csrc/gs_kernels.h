@@ -22,10 +22,21 @@ struct SortScratch {
   uint32_t* cntS;         // cap: key counts in sort order
   uint2* rectS;           // cap: tile rectangles in sort order
   uint32_t* dkS;          // cap: depth bits in id order (global64 only)
+  uint32_t* nDepth;       // 1: Gaussians left after the first depth pass drops the culled ones
+  uint32_t* tileCnt;      // maxTiles: keys per tile (tile segments)
+  uint32_t* segTable;     // rows x maxTiles: per-block key counts, then offsets (tile segments)
 };
 
+// Most tiles the tile-segment strategy handles (LDS histograms of u32; the
+// reference's scan covers 256 x 64 tiles, shaders.js:325); larger views use
+// the depth-then-tile radix sort.
+constexpr int kMaxSegTiles = 16384;
+// Gaussians per count/scatter block of the tile-segment strategy (rows of
+// the table = ceil(n / seg_group(n)) <= 1024)
+int64_t seg_group(int64_t n);
+
 struct BinArgs {
-  int mode;               // 0 depth-then-tile, 1 global64
+  int mode;               // 0 depth-then-tile, 1 global64, 2 tile segments
   int tilesX, numTiles;
   int64_t n;              // Gaussians
   uint64_t cap;           // key budget
@@ -47,10 +58,10 @@ struct BinArgs {
 };
 
 struct BinOut {
-  const void* keys;
+  const void* keys;       // null for tile segments (a key's tile is the range holding it)
   const uint32_t* ids;
   const uint32_t* tileOrder;
-  int keyBytes;
+  int keyBytes;           // 8 (tile << 32 | depth), 4 (tile), 0 (none)
 };
 
 struct RasterBufs {

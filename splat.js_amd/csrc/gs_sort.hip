@@ -7,7 +7,10 @@
 //    depth bits (emitted in id order, so ties keep id order), then keys are
 //    emitted in that order and stably sorted by tile id only
 //    (ceil(tileBits/8) passes). Sorting Gaussians instead of keys moves the
-//    32 depth bits off the long key array.
+//    32 depth bits off the long key array. The first depth pass drops the
+//    culled Gaussians (key 0xffffffff, which would sort last and emit no
+//    keys) and counts the rest on the device, so the later passes, the count
+//    and the emission run over the visible Gaussians only.
 //  * global64: keys (tile << 32 | depth bits) emitted in id order and sorted
 //    by a stable LSD radix sort over all 32 + tileBits bits.
 //
@@ -74,7 +77,8 @@ __device__ __forceinline__ uint32_t digit_of(KeyT k, int shift) {
   return (uint32_t)(k >> shift) & 0xffu;
 }
 
-template <typename KeyT>
+// kFilter: keys equal to ~0 (culled Gaussians) are left out
+template <typename KeyT, bool kFilter>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const KeyT* __restrict__ keys,
                                                        const uint32_t* __restrict__ dN, int shift,
                                                        int G, uint32_t* __restrict__ hist) {
@@ -90,13 +94,18 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const KeyT* __restrict__ 
 #pragma unroll
     for (int i = 0; i < kItems; i++) {
       const uint32_t idx = base + (uint32_t)i * 64u + lane;
-      d[i] = idx < hi ? digit_of(keys[idx], shift) : 0x100u;
+      d[i] = 0x100u;
+      if (idx < hi) {
+        const KeyT k = keys[idx];
+        if (!kFilter || k != (KeyT)~(KeyT)0) d[i] = digit_of(k, shift);
+      }
     }
 #pragma unroll
     for (int i = 0; i < kItems; i++) {
       const bool ok = d[i] < 0x100u;
       const uint64_t act = wave_ballot(ok);
-      if (act == 0ull) break;
+      if (!kFilter && act == 0ull) break;   // unfiltered, only the chunk's tail is empty
+      if (act == 0ull) continue;
       const uint64_t m = match_digit8(d[i] & 0xffu, act);
       // one lane per digit group: distinct addresses within an instruction
       if (ok && (m & lanemask_lt()) == 0ull) atomicAdd(&wc[w][d[i]], (uint32_t)__popcll(m));
@@ -133,7 +142,9 @@ __global__ __launch_bounds__(kBlock) void k_radix_scan(uint32_t* __restrict__ hi
   if (tid == 0) totals[d] = carry;
 }
 
-template <typename KeyT, bool kWriteKeys>
+// kFilter: keys equal to ~0 are dropped, and block 0 stores the number of
+// keys kept in *nOut
+template <typename KeyT, bool kWriteKeys, bool kFilter>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const KeyT* __restrict__ keysIn,
                                                           const uint32_t* __restrict__ valsIn,
                                                           const uint32_t* __restrict__ dN,
@@ -141,7 +152,8 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const KeyT* __restrict
                                                           const uint32_t* __restrict__ scan,
                                                           KeyT* __restrict__ keysOut,
                                                           uint32_t* __restrict__ valsOut,
-                                                          const uint32_t* __restrict__ totals) {
+                                                          const uint32_t* __restrict__ totals,
+                                                          uint32_t* __restrict__ nOut) {
   __shared__ uint32_t wc[4][256];
   __shared__ uint32_t wpre[4][256];
   __shared__ uint32_t gbase[256];
@@ -165,6 +177,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const KeyT* __restrict
     uint32_t wp = 0;
     for (int k = 0; k < w; k++) wp += wsum[k];
     gbase[tid] = wp + x - t + scan[(int64_t)tid * G + blockIdx.x];
+    if (kFilter && blockIdx.x == 0 && tid == 0) *nOut = wsum[0] + wsum[1] + wsum[2] + wsum[3];
   }
   wc[0][tid] = wc[1][tid] = wc[2][tid] = wc[3][tid] = 0;
   __syncthreads();
@@ -176,8 +189,9 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const KeyT* __restrict
 #pragma unroll
     for (int i = 0; i < kItems; i++) {
       const uint32_t idx = base + (uint32_t)i * 64u + lane;
-      const bool ok = idx < hi;
+      bool ok = idx < hi;
       key[i] = ok ? keysIn[idx] : (KeyT)0;
+      if (kFilter) ok = ok && key[i] != (KeyT)~(KeyT)0;
       val[i] = ok ? (valsIn ? valsIn[idx] : idx) : 0u;
       rank[i] = ok ? 0u : 0xffffffffu;
     }
@@ -221,6 +235,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const KeyT* __restrict
     for (int k = 0; k < w; k++) wp += wsum[k];
     const uint32_t myStart = wp + x - tot;
     lstart[tid] = myStart;
+    const uint32_t kept = wsum[0] + wsum[1] + wsum[2] + wsum[3];   // keys of this round staged below
     __syncthreads();
     // 3. stage in local (digit, rank) order
 #pragma unroll
@@ -234,8 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const KeyT* __restrict
     }
     __syncthreads();
     // 4. write out: consecutive threads write consecutive slots of a digit run
-    const uint32_t cnt = min((uint32_t)kRound, hi - r);
-    for (uint32_t j = tid; j < cnt; j += kBlock) {
+    for (uint32_t j = tid; j < kept; j += kBlock) {
       const KeyT k = sk[j];
       const uint32_t d = digit_of(k, shift);
       const uint32_t g = gbase[d] + (j - lstart[d]);
@@ -485,36 +499,519 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, int64_t count, uint32_t v) {
   k_fill_u32<<<(unsigned)((count + 255) / 256), 256, 0, st>>>(p, count, v);
 }
 
+// ---- tile segments (default strategy) ----
+//
+// The reference's own structure (shaders.js:318-500): per-tile key counts, a
+// scan that lays out one segment per tile under the entry budget, a scatter
+// of every (Gaussian, tile) key into its tile's segment, and a sort of each
+// segment. Keys are u64 (depth bits << 32 | id << 4 | block mask): within a
+// tile no two keys share an id, so sorting by the whole word orders a
+// segment by (depth, id), the reference's order (shaders.js:454), whatever
+// order the scatter left it in.
+//
+// No global atomics: blocks of kSegGroup Gaussians count their keys per tile
+// in an LDS histogram and write it as one row of a [block][tile] table; a
+// column scan turns the table into each block's offset inside every tile's
+// segment (and the tile totals); the scatter then places keys with LDS
+// counters only. A segment of up to 1024 keys sorts in one block: 256-key
+// runs by single waves (bitonic through lane shuffles), then up to two LDS
+// merge levels; longer segments sort 1024-key runs that way and merge them
+// pairwise in global memory.
+
+constexpr int kSegThreads = 1024;     // threads of the count and scatter blocks
+constexpr uint32_t kShSort = 2048;    // shaders.js:33 SHARED_SORT: longer segments count pow2-padded
+constexpr int kScanChunk = 64;        // tiles per scan thread (shaders.js:325)
+
+__device__ __forceinline__ uint64_t seg_key(uint32_t depth, uint32_t id, uint32_t mask) {
+  return ((uint64_t)depth << 32) | ((uint64_t)id << 4) | (uint64_t)mask;
+}
+__device__ __forceinline__ uint32_t seg_val(uint64_t k) {
+  return (uint32_t)((k >> 4) & kIdMask) | ((uint32_t)(k & 0xfu) << kMaskShift);
+}
+
+// exclusive scan over a block of kT threads (kT / 64 waves)
+template <int kT>
+__device__ __forceinline__ uint32_t block_excl_scan_t(uint32_t v, uint32_t& total, uint32_t* wsum) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  uint32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  __syncthreads();
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  uint32_t pre = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kT / 64; k++) {
+    const uint32_t ws = wsum[k];
+    pre += k < w ? ws : 0u;
+    tot += ws;
+  }
+  total = tot;
+  return pre + x - v;
+}
+
+// Gaussians [lo, hi) of a segment block, one round of kSegThreads at a time:
+// per round the keys are spread over the threads (block scan of the counts,
+// then a binary search per key), and f(g, tile, tx, ty) runs once per key,
+// g the round-local index. `prep` stages per-Gaussian data for the round.
+struct SegRound {
+  uint32_t* sOff;
+  uint2* sRect;
+  uint32_t* wsum;
+};
+
+template <typename Prep, typename F>
+__device__ __forceinline__ void seg_rounds(int64_t lo, int64_t hi, const uint32_t* __restrict__ cnt,
+                                           const uint2* __restrict__ rect, int tilesX, const SegRound& sr,
+                                           Prep prep, F f) {
+  const int tid = threadIdx.x;
+  for (int64_t r0 = lo; r0 < hi; r0 += kSegThreads) {
+    const int64_t i = r0 + tid;
+    const uint32_t c = i < hi ? cnt[i] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan_t<kSegThreads>(c, tot, sr.wsum);
+    sr.sOff[tid] = ex;
+    sr.sRect[tid] = c ? rect[i] : make_uint2(0, 0);
+    prep(tid, i, c);
+    __syncthreads();
+    for (uint32_t q = tid; q < tot; q += kSegThreads) {
+      int g = 0;
+#pragma unroll
+      for (int step = kSegThreads / 2; step > 0; step >>= 1)
+        if (sr.sOff[g + step] <= q) g += step;
+      const uint2 rc = sr.sRect[g];
+      const uint32_t tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu;
+      const uint32_t wdt = tx1 - tx0 + 1u;
+      const uint32_t li = q - sr.sOff[g];
+      const uint32_t dy = li / wdt;
+      const uint32_t tx = tx0 + (li - dy * wdt), ty = ty0 + dy;
+      f(g, r0, ty * (uint32_t)tilesX + tx, tx, ty);
+    }
+    __syncthreads();
+  }
+}
+
+// Row b of the [block][tile] table: the keys of Gaussians [b G, (b+1) G) per
+// tile; also the visible count.
+__global__ __launch_bounds__(kSegThreads) void k_seg_count(int64_t n, int64_t G, const uint32_t* __restrict__ cnt,
+                                                           const uint2* __restrict__ rect, int tilesX, int numTiles,
+                                                           uint32_t* __restrict__ table,
+                                                           uint32_t* __restrict__ numVisible) {
+  __shared__ uint32_t hist[kMaxSegTiles];
+  __shared__ uint32_t sOff[kSegThreads];
+  __shared__ uint2 sRect[kSegThreads];
+  __shared__ uint32_t wsum[kSegThreads / 64];
+  const int tid = threadIdx.x;
+  for (int t = tid; t < numTiles; t += kSegThreads) hist[t] = 0u;
+  const int64_t lo = (int64_t)blockIdx.x * G;
+  const int64_t hi = lo + G < n ? lo + G : n;
+  uint32_t vis = 0;
+  const SegRound sr{sOff, sRect, wsum};
+  seg_rounds(lo, hi, cnt, rect, tilesX, sr, [&](int, int64_t, uint32_t c) { vis += c != 0u; },
+             [&](int, int64_t, uint32_t tile, uint32_t, uint32_t) { atomicAdd(&hist[tile], 1u); });
+  uint32_t tot;
+  block_excl_scan_t<kSegThreads>(vis, tot, wsum);
+  if (tid == 0 && tot) atomicAdd(numVisible, tot);
+  __syncthreads();
+  uint32_t* row = table + (int64_t)blockIdx.x * numTiles;
+  for (int t = tid; t < numTiles; t += kSegThreads) row[t] = hist[t];
+}
+
+// Column scan of the table: table[b][t] becomes the exclusive prefix over
+// blocks, tileCnt[t] the column total. Blocks of 256 threads handle 16
+// tiles, 16 threads per tile (a wave reads 16 consecutive tiles of 4 rows),
+// each thread a sixteenth of the rows.
+__global__ __launch_bounds__(kBlock) void k_seg_colscan(uint32_t* __restrict__ table, int nb, int numTiles,
+                                                        uint32_t* __restrict__ tileCnt) {
+  __shared__ uint32_t part[16][16];
+  const int tl = threadIdx.x & 15, q = threadIdx.x >> 4;
+  const int t = blockIdx.x * 16 + tl;
+  const int per = (nb + 15) / 16;
+  const int b0 = q * per, b1 = b0 + per < nb ? b0 + per : nb;
+  uint32_t sum = 0;
+  if (t < numTiles)
+    for (int b = b0; b < b1; b++) sum += table[(int64_t)b * numTiles + t];
+  part[q][tl] = sum;
+  __syncthreads();
+  if (t >= numTiles) return;
+  uint32_t run = 0;
+  for (int k = 0; k < q; k++) run += part[k][tl];
+  for (int b = b0; b < b1; b++) {
+    uint32_t* p = table + (int64_t)b * numTiles + t;
+    const uint32_t v = *p;
+    *p = run;
+    run += v;
+  }
+  if (q == 15) tileCnt[t] = run;
+}
+
+__device__ __forceinline__ uint32_t seg_padded(uint32_t c) {
+  // shaders.js:329-332: segments over SHARED_SORT take a power-of-two slot
+  return c <= kShSort ? c : 1u << (32 - __clz((int)(c - 1u)));
+}
+
+// Single block of 256 threads: the reference's budgeted segment scan
+// (shaders.js:321-370). Thread li owns tiles [64 li, 64 li + 64). Its chunk
+// starts at the unbudgeted exclusive prefix of the padded sizes of the
+// chunks before it; within the chunk, a tile whose padded size would take
+// the running total past the budget is dropped (and counted, as the
+// reference counts it, even when empty). Kept tiles get compact, unpadded
+// segments: ranges[t] = (start, start + count).
+__global__ __launch_bounds__(kBlock) void k_seg_scan(const uint32_t* __restrict__ tileCnt, int numTiles, uint64_t cap,
+                                                     uint2* __restrict__ ranges, uint32_t* __restrict__ numKeys,
+                                                     uint64_t* __restrict__ totalKeys, double* __restrict__ stats) {
+  // tile t at sc[t + t / 64]: coalesced fill, and each thread's walk over
+  // its 64 tiles (stride 65 between threads) is free of bank conflicts
+  __shared__ uint32_t sc[kMaxSegTiles + kMaxSegTiles / kScanChunk];
+  __shared__ uint64_t ws[4];   // padded chunk sums per wave
+  __shared__ uint64_t wk[4];   // kept keys per wave
+  __shared__ uint64_t wr[4];   // raw keys per wave
+  __shared__ uint32_t wd[4];   // dropped tiles per wave
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  // 16 loads in flight per thread per batch
+  for (int b0 = 0; b0 < kScanChunk; b0 += 16) {
+    uint32_t v[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int t = tid + kBlock * (b0 + k);
+      v[k] = t < numTiles ? tileCnt[t] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      const int t = tid + kBlock * (b0 + k);
+      sc[t + t / kScanChunk] = v[k];
+    }
+  }
+  __syncthreads();
+  const int t0 = tid * kScanChunk;
+  const uint32_t* mine = sc + tid * (kScanChunk + 1);
+  uint64_t pad = 0, raw = 0;
+  for (int k = 0; k < kScanChunk; k++) {
+    const int t = t0 + k;
+    if (t < numTiles) {
+      const uint32_t c = mine[k];
+      pad += seg_padded(c);
+      raw += c;
+    }
+  }
+  uint64_t x = pad;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) ws[w] = x;
+  __syncthreads();
+  uint64_t pre = 0;
+  for (int k = 0; k < w; k++) pre += ws[k];
+  const uint64_t acc0 = pre + x - pad;
+  uint64_t acc = acc0, kept = 0;
+  uint32_t dropped = 0;
+  for (int k = 0; k < kScanChunk; k++) {
+    const int t = t0 + k;
+    if (t >= numTiles) break;
+    const uint32_t c = mine[k];
+    uint64_t p = seg_padded(c);
+    if (acc + p > cap) { p = 0; dropped++; }
+    acc += p;
+    kept += p ? c : 0u;
+  }
+  uint64_t y = kept;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t z = __shfl_up(y, o);
+    if (lane >= o) y += z;
+  }
+  uint64_t rsum = raw;
+  uint32_t dsum = dropped;
+  for (int o = 32; o > 0; o >>= 1) {
+    rsum += __shfl_xor(rsum, o);
+    dsum += __shfl_xor(dsum, o);
+  }
+  if (lane == 63) wk[w] = y;
+  if (lane == 0) { wr[w] = rsum; wd[w] = dsum; }
+  __syncthreads();
+  uint64_t kpre = 0;
+  for (int k = 0; k < w; k++) kpre += wk[k];
+  uint64_t start = kpre + y - kept;
+  acc = acc0;
+  for (int k = 0; k < kScanChunk; k++) {
+    const int t = t0 + k;
+    if (t >= numTiles) break;
+    const uint32_t c = mine[k];
+    uint64_t p = seg_padded(c);
+    if (acc + p > cap) p = 0;
+    acc += p;
+    const uint32_t len = p ? c : 0u;
+    ranges[t] = make_uint2((uint32_t)start, (uint32_t)(start + len));
+    start += len;
+  }
+  if (tid == kBlock - 1) *numKeys = (uint32_t)start;
+  if (tid == 0) {
+    *totalKeys = wr[0] + wr[1] + wr[2] + wr[3];
+    const uint32_t d = wd[0] + wd[1] + wd[2] + wd[3];
+    if (d) stats[3] += (double)d;
+  }
+}
+
+// Every (Gaussian, kept tile) key into its tile's segment: at the tile's
+// start, plus this block's offset in the tile (the column scan), plus an LDS
+// counter; the tile's block mask (gs_cover.h) in the low 4 bits.
+__global__ __launch_bounds__(kSegThreads) void k_seg_scatter(int64_t n, int64_t G, const uint32_t* __restrict__ cnt,
+                                                             const uint2* __restrict__ rect,
+                                                             const uint32_t* __restrict__ depthKey,
+                                                             const float4* __restrict__ rec, int tilesX, int numTiles,
+                                                             const uint32_t* __restrict__ table,
+                                                             const uint2* __restrict__ ranges,
+                                                             uint64_t* __restrict__ keys) {
+  __shared__ uint32_t cur[kMaxSegTiles];
+  __shared__ uint32_t sOff[kSegThreads];
+  __shared__ uint2 sRect[kSegThreads];
+  __shared__ uint32_t sDk[kSegThreads];
+  __shared__ gs_cover_splat sG[kSegThreads];
+  __shared__ uint32_t wsum[kSegThreads / 64];
+  const int tid = threadIdx.x;
+  const uint32_t* row = table + (int64_t)blockIdx.x * numTiles;
+  for (int t = tid; t < numTiles; t += kSegThreads) cur[t] = 0u;
+  const int64_t lo = (int64_t)blockIdx.x * G;
+  const int64_t hi = lo + G < n ? lo + G : n;
+  const SegRound sr{sOff, sRect, wsum};
+  seg_rounds(lo, hi, cnt, rect, tilesX, sr,
+             [&](int k, int64_t i, uint32_t c) {
+               sDk[k] = c ? depthKey[i] : 0u;
+               if (c != 0u) {
+                 const float4 g0 = rec[4 * i + 0];
+                 const float4 g1 = rec[4 * i + 1];
+                 sG[k] = gs_cover_prepare(g0.x, g0.y, g0.z, g0.w, g1.x, g1.z);
+               }
+             },
+             [&](int g, int64_t r0, uint32_t tile, uint32_t tx, uint32_t ty) {
+               const uint2 rg = ranges[tile];
+               if (rg.y == rg.x) return;   // tile dropped by the budget (shaders.js:406-407)
+               const uint32_t pos = rg.x + row[tile] + atomicAdd(&cur[tile], 1u);
+               const gs_cover_splat sg = sG[g];
+               const uint32_t mask = gs_cover_tile_mask(&sg, (int)tx, (int)ty);
+               keys[pos] = seg_key(sDk[g], (uint32_t)(r0 + g), mask);
+             });
+}
+
+// x from lane ^ m (m = 1, 2: DPP quad permutes on the VALU; otherwise a
+// ds_bpermute), for 64-bit x
+template <int kM>
+__device__ __forceinline__ uint64_t lane_xor_u64(uint64_t x) {
+  const int lo = (int)(uint32_t)x, hi = (int)(uint32_t)(x >> 32);
+  int a, b;
+  if constexpr (kM == 1) {
+    a = __builtin_amdgcn_update_dpp(0, lo, 0xB1, 0xf, 0xf, false);   // quad_perm [1,0,3,2]
+    b = __builtin_amdgcn_update_dpp(0, hi, 0xB1, 0xf, 0xf, false);
+  } else if constexpr (kM == 2) {
+    a = __builtin_amdgcn_update_dpp(0, lo, 0x4E, 0xf, 0xf, false);   // quad_perm [2,3,0,1]
+    b = __builtin_amdgcn_update_dpp(0, hi, 0x4E, 0xf, 0xf, false);
+  } else {
+    a = __shfl_xor(lo, kM);
+    b = __shfl_xor(hi, kM);
+  }
+  return ((uint64_t)(uint32_t)b << 32) | (uint32_t)a;
+}
+
+// One compare-exchange stage (k, j) of the bitonic network over 256 keys,
+// element i = 4 lane + r in v[r]: partners at distance j < 4 are in the same
+// lane, others in lane ^ (j / 4).
+template <uint32_t k, uint32_t j>
+__device__ __forceinline__ void bitonic_stage(uint64_t (&v)[4], uint32_t lane) {
+  uint64_t o[4];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    if constexpr (j < 4) o[r] = v[r ^ (int)j];
+    else o[r] = lane_xor_u64<(int)(j / 4)>(v[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const uint32_t i = 4u * lane + (uint32_t)r;
+    const bool keepMin = ((i & j) == 0u) == ((i & k) == 0u);
+    const bool lt = v[r] < o[r];
+    v[r] = keepMin == lt ? v[r] : o[r];
+  }
+}
+
+template <uint32_t k, uint32_t j>
+__device__ __forceinline__ void bitonic_merge_steps(uint64_t (&v)[4], uint32_t lane) {
+  bitonic_stage<k, j>(v, lane);
+  if constexpr (j > 1) bitonic_merge_steps<k, j / 2>(v, lane);
+}
+
+template <uint32_t k>
+__device__ __forceinline__ void bitonic_blocks(uint64_t (&v)[4], uint32_t lane) {
+  bitonic_merge_steps<k, k / 2>(v, lane);
+  if constexpr (k < 256) bitonic_blocks<k * 2>(v, lane);
+}
+
+// Ascending bitonic sort of 256 keys within one wave, keys 4 lane .. 4 lane
+// + 3 in v[0..3]: no LDS traffic except the exchanges at lane distances of
+// 4 and more, no barriers.
+__device__ __forceinline__ void wave_sort256(uint64_t (&v)[4]) {
+  bitonic_blocks<2>(v, __lane_id());
+}
+
+// Sorts m <= kChunk keys: runs of 256 sorted by waves (padded with ~0 to
+// R = 1, 2 or 4 runs), then merged pairwise in LDS (merge path, R outputs
+// per thread per level). Returns the LDS buffer holding the sorted keys
+// (sorted[0, m)).
+constexpr uint32_t kChunk = 4 * 256;
+
+__device__ uint64_t* seg_sort_chunk(const uint64_t* __restrict__ src, uint32_t m, uint64_t* sa, uint64_t* sb) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+  const uint32_t R = m <= 256u ? 1u : (m <= 512u ? 2u : 4u);
+  if (w < R) {
+    uint64_t v[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const uint32_t i = 256u * w + 4u * lane + (uint32_t)r;
+      v[r] = i < m ? src[i] : ~0ull;
+    }
+    wave_sort256(v);
+#pragma unroll
+    for (int r = 0; r < 4; r++) sa[256u * w + 4u * lane + (uint32_t)r] = v[r];
+  }
+  __syncthreads();
+  uint64_t* in = sa;
+  uint64_t* out = sb;
+  const uint32_t N = 256u * R;
+  for (uint32_t W = 256u; W < N; W <<= 1) {
+    // thread tid writes outputs [tid R, tid R + R) of N (R = N / 256)
+    const uint32_t o = tid * R;
+    const uint32_t base = o / (2u * W) * (2u * W);
+    const uint64_t* A = in + base;
+    const uint64_t* B = A + W;
+    const uint32_t d = o - base;
+    uint32_t lo = d > W ? d - W : 0u, hi = d < W ? d : W;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (A[mid] < B[d - mid - 1u]) lo = mid + 1u;
+      else hi = mid;
+    }
+    uint32_t ia = lo, ib = d - lo;
+    for (uint32_t q = 0; q < R; q++) {
+      const bool takeA = ib >= W || (ia < W && A[ia] < B[ib]);
+      out[o + q] = takeA ? A[ia++] : B[ib++];
+    }
+    __syncthreads();
+    uint64_t* t = in;
+    in = out;
+    out = t;
+  }
+  return in;
+}
+
+// out[0, na + nb) = merge of the sorted runs A and B (keys are distinct):
+// each thread finds where its 8 outputs start by a merge-path search.
+__device__ void seg_merge(const uint64_t* A, uint32_t na, const uint64_t* B, uint32_t nb, uint64_t* out) {
+  const uint32_t N = na + nb;
+  for (uint32_t o = threadIdx.x * 8u; o < N; o += kBlock * 8u) {
+    uint32_t lo = o > nb ? o - nb : 0u, hi = o < na ? o : na;
+    while (lo < hi) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (A[mid] < B[o - mid - 1u]) lo = mid + 1u;
+      else hi = mid;
+    }
+    uint32_t ia = lo, ib = o - lo;
+    const uint32_t e = o + 8u < N ? o + 8u : N;
+    for (uint32_t q = o; q < e; q++) {
+      const bool takeA = ib >= nb || (ia < na && A[ia] < B[ib]);
+      out[q] = takeA ? A[ia++] : B[ib++];
+    }
+  }
+}
+
+// One block per tile: sorts the tile's segment and writes the raster values
+// (id | mask << 28) in order. Segments over kChunk keys sort kChunk-key runs
+// and merge them pairwise in global memory (keys <-> tmp).
+__global__ __launch_bounds__(kBlock) void k_seg_sort(const uint2* __restrict__ ranges, uint64_t* __restrict__ keys,
+                                                     uint64_t* __restrict__ tmp, uint32_t* __restrict__ vals) {
+  __shared__ uint64_t sa[kChunk];
+  __shared__ uint64_t sb[kChunk];
+  const uint2 r = ranges[blockIdx.x];
+  const uint32_t s = r.x, len = r.y - r.x;
+  const uint32_t tid = threadIdx.x;
+  if (len == 0u) return;
+  if (len == 1u) {
+    if (tid == 0) vals[s] = seg_val(keys[s]);
+    return;
+  }
+  if (len <= kChunk) {
+    const uint64_t* sorted = seg_sort_chunk(keys + s, len, sa, sb);
+    for (uint32_t i = tid; i < len; i += kBlock) vals[s + i] = seg_val(sorted[i]);
+    return;
+  }
+  for (uint32_t c0 = 0; c0 < len; c0 += kChunk) {
+    const uint32_t m = len - c0 < kChunk ? len - c0 : kChunk;
+    const uint64_t* sorted = seg_sort_chunk(keys + s + c0, m, sa, sb);
+    for (uint32_t i = tid; i < m; i += kBlock) keys[s + c0 + i] = sorted[i];
+    __syncthreads();
+  }
+  uint64_t* src = keys + s;
+  uint64_t* dst = tmp + s;
+  for (uint32_t wdt = kChunk; wdt < len; wdt <<= 1) {
+    for (uint32_t a = 0; a < len; a += 2u * wdt) {
+      const uint32_t na = len - a < wdt ? len - a : wdt;
+      const uint32_t nb = len - a - na < wdt ? len - a - na : wdt;
+      seg_merge(src + a, na, src + a + na, nb, dst + a);
+    }
+    __syncthreads();
+    uint64_t* t = src;
+    src = dst;
+    dst = t;
+  }
+  for (uint32_t i = tid; i < len; i += kBlock) vals[s + i] = seg_val(src[i]);
+}
+
 // one pass: histogram, per-digit scan (with digit totals), stable scatter
+// nOut != nullptr: drop keys equal to ~0 and store the count kept there
 template <typename KeyT>
 static void radix_pass(hipStream_t st, const SortScratch& s, const KeyT* kin, const uint32_t* vin,
-                       KeyT* kout, uint32_t* vout, const uint32_t* dN, int shift) {
-  k_radix_hist<KeyT><<<s.G, kBlock, 0, st>>>(kin, dN, shift, s.G, s.hist);
+                       KeyT* kout, uint32_t* vout, const uint32_t* dN, int shift, uint32_t* nOut) {
+  if (nOut) k_radix_hist<KeyT, true><<<s.G, kBlock, 0, st>>>(kin, dN, shift, s.G, s.hist);
+  else k_radix_hist<KeyT, false><<<s.G, kBlock, 0, st>>>(kin, dN, shift, s.G, s.hist);
   k_radix_scan<<<256, kBlock, 0, st>>>(s.hist, s.G, s.totals);
-  k_radix_scatter<KeyT, true><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout, s.totals);
+  if (nOut)
+    k_radix_scatter<KeyT, true, true><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout, s.totals,
+                                                             nOut);
+  else
+    k_radix_scatter<KeyT, true, false><<<s.G, kBlock, 0, st>>>(kin, vin, dN, shift, s.G, s.hist, kout, vout,
+                                                              s.totals, nullptr);
 }
 
 // Stable LSD sort of (key, val) pairs by key bits [bit0, bit1), 8 per pass.
 // Pass p reads the previous pass's output (the input for p = 0; vals = iota
 // when vin is null) and writes (k1, v1) for even p, (k2, v2) for odd p. The
-// input may alias (k2, v2): it is consumed by the first pass.
+// input may alias (k2, v2): it is consumed by the first pass. nKept !=
+// nullptr: the first pass drops keys equal to ~0, stores the number kept in
+// *nKept, and the later passes sort that many.
 template <typename KeyT>
 static void radix_sort(hipStream_t st, const SortScratch& s, const KeyT* kin, const uint32_t* vin,
                        KeyT* k1, uint32_t* v1, KeyT* k2, uint32_t* v2, const uint32_t* dN,
-                       int bit0, int bit1, const KeyT** kres, const uint32_t** vres) {
+                       int bit0, int bit1, const KeyT** kres, const uint32_t** vres, uint32_t* nKept = nullptr) {
   const KeyT* ki = kin;
   const uint32_t* vi = vin;
   bool odd = false;
   for (int b = bit0; b < bit1; b += 8) {
     KeyT* ko = odd ? k2 : k1;
     uint32_t* vo = odd ? v2 : v1;
-    radix_pass<KeyT>(st, s, ki, vi, ko, vo, dN, b);
+    const bool first = b == bit0;
+    radix_pass<KeyT>(st, s, ki, vi, ko, vo, first || !nKept ? dN : nKept, b, first ? nKept : nullptr);
     ki = ko;
     vi = vo;
     odd = !odd;
   }
   *kres = ki;
   *vres = vi;
+}
+
+int64_t seg_group(int64_t n) {
+  // Gaussians per count/scatter block: 4096, more for huge n so that the
+  // [block][tile] table keeps at most 1024 rows
+  const int64_t g = (n + 1023) / 1024;
+  return g > 4096 ? (g + kSegThreads - 1) / kSegThreads * kSegThreads : 4096;
 }
 
 int tile_bits(int numTiles) {
@@ -528,16 +1025,35 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
   const int tb = tile_bits(a.numTiles);
   const int tpasses = (tb + 7) / 8;
   const int Gc = (int)((a.n + kBlock - 1) / kBlock);
-  if (a.mode == 0) {
+  if (a.mode == 2 && a.numTiles <= kMaxSegTiles) {
+    // tile segments: count table, column scan, budgeted scan, scatter, per-tile sort
+    const int64_t G = seg_group(a.n);
+    const int nb = (int)((a.n + G - 1) / G);
+    launch_fill_u32(st, a.numVisible, 1, 0u);
+    k_seg_count<<<nb, kSegThreads, 0, st>>>(a.n, G, a.cnt, a.rect, a.tilesX, a.numTiles, s.segTable, a.numVisible);
+    k_seg_colscan<<<(a.numTiles + 15) / 16, kBlock, 0, st>>>(s.segTable, nb, a.numTiles, s.tileCnt);
+    k_seg_scan<<<1, kBlock, 0, st>>>(s.tileCnt, a.numTiles, a.cap, a.ranges, a.numKeys, a.totalKeys, a.stats);
+    k_seg_scatter<<<nb, kSegThreads, 0, st>>>(a.n, G, a.cnt, a.rect, a.depthKey, a.rec, a.tilesX, a.numTiles,
+                                              s.segTable, a.ranges, (uint64_t*)a.tkA);
+    k_seg_sort<<<a.numTiles, kBlock, 0, st>>>(a.ranges, (uint64_t*)a.tkA, (uint64_t*)a.tkB, a.tvA);
+    k_tile_order<<<1, 1024, 0, st>>>(a.ranges, a.numTiles, a.tileOrder);
+    out.keys = nullptr;
+    out.ids = a.tvA;
+    out.keyBytes = 0;
+    out.tileOrder = a.tileOrder;
+    return;
+  }
+  if (a.mode == 0 || a.mode == 2) {   // mode 2 lands here above kMaxSegTiles tiles
     // 1) stable sort of the Gaussians by depth bits, values = ids
     const uint32_t* dk;
     const uint32_t* order;
-    radix_sort<uint32_t>(st, s, a.depthKey, nullptr, a.dkA, a.dvA, a.dkB, a.dvB, a.dN, 0, 32, &dk, &order);
+    radix_sort<uint32_t>(st, s, a.depthKey, nullptr, a.dkA, a.dvA, a.dkB, a.dvB, a.dN, 0, 32, &dk, &order,
+                         s.nDepth);
     // 2) per-Gaussian key counts in depth order -> offsets -> one key per tile
-    k_count_partials<<<Gc, kBlock, 0, st>>>(order, a.cnt, a.rect, a.depthKey, a.dN, s.cntS, s.rectS, nullptr,
+    k_count_partials<<<Gc, kBlock, 0, st>>>(order, a.cnt, a.rect, a.depthKey, s.nDepth, s.cntS, s.rectS, nullptr,
                                             s.partials, s.vpartials);
     k_count_top<<<1, 1024, 0, st>>>(s.partials, s.vpartials, Gc, a.cap, a.numKeys, a.totalKeys, a.numVisible, a.stats);
-    k_emit<false><<<Gc, kBlock, 0, st>>>(order, s.cntS, s.rectS, nullptr, a.dN, s.partials, a.tilesX, a.cap,
+    k_emit<false><<<Gc, kBlock, 0, st>>>(order, s.cntS, s.rectS, nullptr, s.nDepth, s.partials, a.tilesX, a.cap,
                                          a.rec, a.tkA, a.tvA);
     // 3) stable sort by tile id
     const uint32_t* tk;

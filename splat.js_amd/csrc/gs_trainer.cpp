@@ -566,6 +566,12 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
       if (dbg->keys && nk) {
         if (h->lastBin.keyBytes == 8) {
           HIPCK(hipMemcpy(dbg->keys, h->lastBin.keys, 8ull * nk, hipMemcpyDeviceToHost));
+        } else if (h->lastBin.keyBytes == 0) {
+          // tile segments: a key's tile is the range that holds it
+          std::vector<uint2> rg(cam.numTiles);
+          HIPCK(hipMemcpy(rg.data(), h->ranges, 8ull * cam.numTiles, hipMemcpyDeviceToHost));
+          for (int t = 0; t < cam.numTiles; t++)
+            for (uint32_t k = rg[t].x; k < rg[t].y && k < nk; k++) dbg->keys[k] = ((uint64_t)t << 32) | dk[ids[k]];
         } else {
           std::vector<uint32_t> tk(nk);
           HIPCK(hipMemcpy(tk.data(), h->lastBin.keys, 4ull * nk, hipMemcpyDeviceToHost));
@@ -680,7 +686,7 @@ int gs_default_opts(gs_opts* o) {
   o->loss = GS_LOSS_CHARBONNIER;
   o->ssimLambda = 0.2f;
   o->gradAccum = GS_ACCUM_FIXED;
-  o->sortMode = GS_SORT_DEPTH_THEN_TILE;
+  o->sortMode = GS_SORT_TILE_SEGMENTS;
   o->seed = 0;
   o->device = 0;
   o->precision = GS_PREC_F32;
@@ -699,7 +705,7 @@ int gs_create(const gs_opts* o, gs_trainer_t** out) {
     }
     h->o = *o;
   }
-  if (h->o.shDeg < 0 || h->o.shDeg > 3 || h->o.sortMode < 0 || h->o.sortMode > 1 ||
+  if (h->o.shDeg < 0 || h->o.shDeg > 3 || h->o.sortMode < 0 || h->o.sortMode > 2 ||
       h->o.gradAccum < 0 || h->o.gradAccum > 1 || h->o.loss < 0 || h->o.loss > 1 ||
       h->o.precision < 0 || h->o.precision > 1 ||
       !(h->o.eCut >= 0.0f && h->o.eCut <= 80.0f)) {
@@ -824,7 +830,7 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   rc |= dalloc(h, &h->dvA, (size_t)cap * 4);
   rc |= dalloc(h, &h->dkB, (size_t)cap * 4);
   rc |= dalloc(h, &h->dvB, (size_t)cap * 4);
-  const size_t kb = h->o.sortMode == GS_SORT_GLOBAL64 ? 8 : 4;
+  const size_t kb = h->o.sortMode == GS_SORT_DEPTH_THEN_TILE ? 4 : 8;
   rc |= dalloc(h, &h->tkA, (size_t)h->entriesCap * kb);
   rc |= dalloc(h, &h->tkB, (size_t)h->entriesCap * kb);
   rc |= dalloc(h, &h->tvA, (size_t)h->entriesCap * 4);
@@ -838,6 +844,13 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   rc |= dalloc(h, &h->sort.cntS, (size_t)cap * 4);
   rc |= dalloc(h, &h->sort.rectS, (size_t)cap * 8);
   rc |= dalloc(h, &h->sort.dkS, (size_t)cap * 4);
+  rc |= dalloc(h, &h->sort.nDepth, 4);
+  rc |= dalloc(h, &h->sort.tileCnt, (size_t)h->maxTiles * 4);
+  if (h->o.sortMode == GS_SORT_TILE_SEGMENTS) {
+    // rows = ceil(n / seg_group(n)) <= min(1024, ceil(cap / 4096)) for every n <= cap
+    const int64_t rows = std::min<int64_t>(1024, (cap + 4095) / 4096);
+    rc |= dalloc(h, &h->sort.segTable, (size_t)rows * h->maxTiles * 4);
+  }
   rc |= dalloc(h, &h->dN, 4);
   rc |= dalloc(h, &h->numKeys, 4);
   rc |= dalloc(h, &h->numVisible, 4);

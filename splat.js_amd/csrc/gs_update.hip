@@ -32,8 +32,14 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
   const int64_t cap = u.cap;
 
   // ---- screen-space gradients (decode + clear), shaders.js:727-744 ----
+  // Only Gaussians with keys in this step can have nonzero sums (the
+  // backward adds to keyed Gaussians only, and every pass leaves the sums
+  // cleared), so a culled Gaussian's sums are zero and stay untouched.
+  const bool keyed = u.cnt[i] != 0u;
   float gp[10];
-  {
+#pragma unroll
+  for (int k = 0; k < 10; k++) gp[k] = 0.0f;
+  if (keyed) {
     int4* acc = (int4*)u.accum + 3 * i;
     const int4 a0 = acc[0], a1 = acc[1], a2 = acc[2];
     const int raw[10] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, a2.x, a2.y};
@@ -58,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
 
   Geom g;
   g.ok = false;
-  if (u.cnt[i] != 0u) compute_geom(cam, p, g);
+  if (keyed) compute_geom(cam, p, g);
   if (g.ok) {
     const float cden = 1.0f + conic_lmax(g.va, g.vb, g.vc);
     gp[2] = gp[2] * cden;

@@ -22,7 +22,7 @@ export const STRIDE = 16;
 export const TILE = 16;
 export const LOSS = { charbonnier: 0, l1_dssim: 1 };
 export const ACCUM = { fixed: 0, f32: 1 };
-export const SORT = { depthThenTile: 0, global64: 1 };
+export const SORT = { depthThenTile: 0, global64: 1, tileSegments: 2 };
 export const PRECISION = { f32: 0, mixed: 1 };
 
 const shRestCoefs = (deg) => (deg + 1) * (deg + 1) - 1;

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "libsplatgs.so")
 
 GS_LOSS_CHARBONNIER, GS_LOSS_L1_DSSIM = 0, 1
 GS_ACCUM_FIXED, GS_ACCUM_F32 = 0, 1
-GS_SORT_DEPTH_THEN_TILE, GS_SORT_GLOBAL64 = 0, 1
+GS_SORT_DEPTH_THEN_TILE, GS_SORT_GLOBAL64, GS_SORT_TILE_SEGMENTS = 0, 1, 2
 GS_PREC_F32, GS_PREC_MIXED = 0, 1
 
 

@@ -148,18 +148,20 @@ def test_mixed_precision_config1_bit_exact():
     _run_pair(p, sh, cams, images, W, H, radius, {}, 2, steps=3, start_iter=2500, precision=gsabi.GS_PREC_MIXED)
 
 
-def test_global64_sort_gives_same_permutation():
+def test_sort_strategies_give_the_same_permutation():
     p, sh, cams, images, W, H, radius = config1_setup(n=8000)
     outs = []
-    for mode in (gsabi.GS_SORT_DEPTH_THEN_TILE, gsabi.GS_SORT_GLOBAL64):
+    for mode in (gsabi.GS_SORT_TILE_SEGMENTS, gsabi.GS_SORT_DEPTH_THEN_TILE, gsabi.GS_SORT_GLOBAL64):
         t = _trainer(p, sh, cams, images, W, H, radius, {}, shDeg=2, sortMode=mode)
         outs.append(t.step_debug(0, want=("ids", "keys", "tileRanges", "img")))
         t.destroy()
-    a, b = outs
-    assert a["numKeys"] == b["numKeys"] > 0
-    np.testing.assert_array_equal(a["ids"], b["ids"])
-    np.testing.assert_array_equal(a["keys"], b["keys"])
-    np.testing.assert_array_equal(a["img"], b["img"])
+    a = outs[0]
+    for b in outs[1:]:
+        assert a["numKeys"] == b["numKeys"] > 0
+        np.testing.assert_array_equal(a["ids"], b["ids"])
+        np.testing.assert_array_equal(a["keys"], b["keys"])
+        np.testing.assert_array_equal(a["tileRanges"], b["tileRanges"])
+        np.testing.assert_array_equal(a["img"], b["img"])
 
 
 def test_f32_accumulation_matches_f64_gradients():
@@ -215,12 +217,13 @@ def test_render_view_and_read_loss():
     t.destroy()
 
 
-def test_dense_tiles_and_ragged_image():
+@pytest.mark.parametrize("n,min_tile", [(3000, 2048), (20000, 8192)])
+def test_dense_tiles_and_ragged_image(n, min_tile):
     """Tiles with > 2048 keys (the reference's global-memory bitonic path,
     shaders.js:471-499), several LDS batches per tile, and a 70x45 view whose
-    edge tiles are partial."""
+    edge tiles are partial. With 20000 Gaussians some tiles hold more than
+    8192 keys: their segments sort as 4096-key LDS runs merged twice."""
     rng = np.random.default_rng(11)
-    n = 3000
     p = np.zeros((n, 16), np.float32)
     p[:, 0] = rng.normal(0, 0.15, n)
     p[:, 1] = rng.normal(0, 0.1, n)
@@ -239,7 +242,7 @@ def test_dense_tiles_and_ragged_image():
     ref = _run_pair(p, None, cams, images, W, H, 1.0, {}, 0, steps=2)
     proj, rect, cnt = ref.st.project(O.make_cam(cams[0]), ref.P, None)
     _, ts, _ = ref.st.bin_sort(O.make_cam(cams[0]), proj, rect, cnt)
-    assert np.diff(ts.astype(np.int64)).max() > 2048
+    assert np.diff(ts.astype(np.int64)).max() > min_tile
 
 
 def test_all_culled_and_single_gaussian():
@@ -288,9 +291,41 @@ def test_multi_camera_seeded_pick_and_holdout():
     assert seen == exp
 
 
-def test_entry_budget_overflow_is_reported():
+def test_entry_budget_drops_whole_tiles_as_the_reference():
+    """Tile segments (the default) under a budget that cannot hold every
+    key: the reference's scan drops whole tiles (shaders.js:321-370). The
+    kept keys, tile ranges, image and dropped-tile count equal the
+    restatement's (oracle.apply_tile_budget)."""
     p, sh, cams, images, W, H, radius = config1_setup(n=4000)
-    t = _trainer(p, sh, cams, images, W, H, radius, {}, shDeg=2, entriesCap=500)
+    st = O.Stage(O.make_opts(shDeg=2), "f32")
+    cam = O.make_cam(cams[0], shActiveDeg=0)
+    proj, rect, cnt = st.project(cam, p, sh)
+    ids, ts, keys = st.bin_sort(cam, proj, rect, cnt)
+    cap = len(ids) // 2
+    ids2, keys2, ts2, dropped = O.apply_tile_budget(ids, keys, ts, cap)
+    assert 0 < dropped and 0 < len(ids2) < len(ids)
+    t = _trainer(p, sh, cams, images, W, H, radius, {}, shDeg=2, entriesCap=cap)
+    g = t.step_debug(0)
+    assert g["numKeys"] == len(ids2)
+    np.testing.assert_array_equal(g["ids"], ids2)
+    np.testing.assert_array_equal(g["keys"], keys2)
+    tr = g["tileRanges"].astype(np.int64)
+    np.testing.assert_array_equal(tr[:, 0], ts2[:-1])
+    np.testing.assert_array_equal(tr[:, 1], ts2[1:])
+    img, T, end = st.render_fwd(cam, proj, ids2, ts2)
+    np.testing.assert_array_equal(g["img"].view(np.uint32), img.astype(np.float32).view(np.uint32))
+    np.testing.assert_array_equal(g["end"], end)
+    info = t.info()
+    assert info.lastKeys == len(ids2) and info.overflowKeys == dropped
+    t.destroy()
+
+
+def test_radix_sort_budget_drops_keys_in_sort_order():
+    """The radix strategies keep the first `entriesCap` keys of the sorted
+    order instead."""
+    p, sh, cams, images, W, H, radius = config1_setup(n=4000)
+    t = _trainer(p, sh, cams, images, W, H, radius, {}, shDeg=2, entriesCap=500,
+                 sortMode=gsabi.GS_SORT_DEPTH_THEN_TILE)
     t.stepOnce(0)
     info = t.info()
     assert info.lastKeys == 500
