@@ -25,14 +25,18 @@ struct SortScratch {
   uint32_t* nDepth;       // 1: Gaussians left after the first depth pass drops the culled ones
   uint32_t* tileCnt;      // maxTiles: keys per tile (tile segments)
   uint32_t* segTable;     // rows x maxTiles: per-block key counts, then offsets (tile segments)
+  uint32_t* visPart;      // rows: Gaussians with keys per count block (tile segments)
 };
 
 // Most tiles the tile-segment strategy handles (LDS histograms of u32; the
 // reference's scan covers 256 x 64 tiles, shaders.js:325); larger views use
 // the depth-then-tile radix sort.
 constexpr int kMaxSegTiles = 16384;
-// Gaussians per count/scatter block of the tile-segment strategy (rows of
-// the table = ceil(n / seg_group(n)) <= 1024)
+// Gaussians per count/scatter block of the tile-segment strategy: at least
+// kSegMinGroup, so that the table has ceil(n / seg_group(n)) <=
+// min(kSegMaxRows, ceil(n / kSegMinGroup)) rows
+constexpr int64_t kSegMinGroup = 4096;
+constexpr int64_t kSegMaxRows = 1024;
 int64_t seg_group(int64_t n);
 
 struct BinArgs {

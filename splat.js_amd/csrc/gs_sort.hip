@@ -519,6 +519,9 @@ void launch_fill_u32(hipStream_t st, uint32_t* p, int64_t count, uint32_t v) {
 // pairwise in global memory.
 
 constexpr int kSegThreads = 1024;     // threads of the count and scatter blocks
+// per-tile LDS arrays (histogram, counters, scan input) are sized to the
+// view's tile count at launch, so that small views fit more blocks per CU
+extern __shared__ __align__(16) uint32_t seg_dyn[];
 constexpr uint32_t kShSort = 2048;    // shaders.js:33 SHARED_SORT: longer segments count pow2-padded
 constexpr int kScanChunk = 64;        // tiles per scan thread (shaders.js:325)
 
@@ -553,13 +556,18 @@ __device__ __forceinline__ uint32_t block_excl_scan_t(uint32_t v, uint32_t& tota
 }
 
 // Gaussians [lo, hi) of a segment block, one round of kSegThreads at a time:
-// per round the keys are spread over the threads (block scan of the counts,
-// then a binary search per key), and f(g, tile, tx, ty) runs once per key,
-// g the round-local index. `prep` stages per-Gaussian data for the round.
+// per round the keys are spread over the threads (block scan of the counts;
+// each Gaussian writes its index over its key slots in an owner table, or,
+// for a round of more than kOwnerCap keys, each key finds its Gaussian by a
+// binary search of the offsets), and f(g, r0, tile, tx, ty) runs once per
+// key, g the round-local index. `prep` stages per-Gaussian data.
+constexpr uint32_t kOwnerCap = 8192;
+
 struct SegRound {
   uint32_t* sOff;
   uint2* sRect;
   uint32_t* wsum;
+  uint16_t* owner;   // kOwnerCap
 };
 
 template <typename Prep, typename F>
@@ -575,12 +583,19 @@ __device__ __forceinline__ void seg_rounds(int64_t lo, int64_t hi, const uint32_
     sr.sOff[tid] = ex;
     sr.sRect[tid] = c ? rect[i] : make_uint2(0, 0);
     prep(tid, i, c);
+    const bool owned = tot <= kOwnerCap;
+    if (owned)
+      for (uint32_t k = 0; k < c; k++) sr.owner[ex + k] = (uint16_t)tid;
     __syncthreads();
     for (uint32_t q = tid; q < tot; q += kSegThreads) {
       int g = 0;
+      if (owned) {
+        g = sr.owner[q];
+      } else {
 #pragma unroll
-      for (int step = kSegThreads / 2; step > 0; step >>= 1)
-        if (sr.sOff[g + step] <= q) g += step;
+        for (int step = kSegThreads / 2; step > 0; step >>= 1)
+          if (sr.sOff[g + step] <= q) g += step;
+      }
       const uint2 rc = sr.sRect[g];
       const uint32_t tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu;
       const uint32_t wdt = tx1 - tx0 + 1u;
@@ -594,26 +609,27 @@ __device__ __forceinline__ void seg_rounds(int64_t lo, int64_t hi, const uint32_
 }
 
 // Row b of the [block][tile] table: the keys of Gaussians [b G, (b+1) G) per
-// tile; also the visible count.
+// tile; visPart[b]: how many of them have keys.
 __global__ __launch_bounds__(kSegThreads) void k_seg_count(int64_t n, int64_t G, const uint32_t* __restrict__ cnt,
                                                            const uint2* __restrict__ rect, int tilesX, int numTiles,
                                                            uint32_t* __restrict__ table,
-                                                           uint32_t* __restrict__ numVisible) {
-  __shared__ uint32_t hist[kMaxSegTiles];
+                                                           uint32_t* __restrict__ visPart) {
+  uint32_t* hist = seg_dyn;   // numTiles
   __shared__ uint32_t sOff[kSegThreads];
   __shared__ uint2 sRect[kSegThreads];
   __shared__ uint32_t wsum[kSegThreads / 64];
+  __shared__ uint16_t owner[kOwnerCap];
   const int tid = threadIdx.x;
   for (int t = tid; t < numTiles; t += kSegThreads) hist[t] = 0u;
   const int64_t lo = (int64_t)blockIdx.x * G;
   const int64_t hi = lo + G < n ? lo + G : n;
   uint32_t vis = 0;
-  const SegRound sr{sOff, sRect, wsum};
+  const SegRound sr{sOff, sRect, wsum, owner};
   seg_rounds(lo, hi, cnt, rect, tilesX, sr, [&](int, int64_t, uint32_t c) { vis += c != 0u; },
              [&](int, int64_t, uint32_t tile, uint32_t, uint32_t) { atomicAdd(&hist[tile], 1u); });
   uint32_t tot;
   block_excl_scan_t<kSegThreads>(vis, tot, wsum);
-  if (tid == 0 && tot) atomicAdd(numVisible, tot);
+  if (tid == 0) visPart[blockIdx.x] = tot;
   __syncthreads();
   uint32_t* row = table + (int64_t)blockIdx.x * numTiles;
   for (int t = tid; t < numTiles; t += kSegThreads) row[t] = hist[t];
@@ -652,71 +668,85 @@ __device__ __forceinline__ uint32_t seg_padded(uint32_t c) {
   return c <= kShSort ? c : 1u << (32 - __clz((int)(c - 1u)));
 }
 
-// Single block of 256 threads: the reference's budgeted segment scan
-// (shaders.js:321-370). Thread li owns tiles [64 li, 64 li + 64). Its chunk
-// starts at the unbudgeted exclusive prefix of the padded sizes of the
-// chunks before it; within the chunk, a tile whose padded size would take
-// the running total past the budget is dropped (and counted, as the
-// reference counts it, even when empty). Kept tiles get compact, unpadded
-// segments: ranges[t] = (start, start + count).
-__global__ __launch_bounds__(kBlock) void k_seg_scan(const uint32_t* __restrict__ tileCnt, int numTiles, uint64_t cap,
-                                                     uint2* __restrict__ ranges, uint32_t* __restrict__ numKeys,
-                                                     uint64_t* __restrict__ totalKeys, double* __restrict__ stats) {
-  // tile t at sc[t + t / 64]: coalesced fill, and each thread's walk over
-  // its 64 tiles (stride 65 between threads) is free of bank conflicts
-  __shared__ uint32_t sc[kMaxSegTiles + kMaxSegTiles / kScanChunk];
-  __shared__ uint64_t ws[4];   // padded chunk sums per wave
-  __shared__ uint64_t wk[4];   // kept keys per wave
-  __shared__ uint64_t wr[4];   // raw keys per wave
-  __shared__ uint32_t wd[4];   // dropped tiles per wave
+// Single block of 1024 threads: the reference's budgeted segment scan
+// (shaders.js:321-370), the raster launch order and the visible count.
+//
+// The reference's thread li owns the chunk of tiles [64 li, 64 li + 64). The
+// chunk starts at the unbudgeted exclusive prefix of the padded sizes of
+// the chunks before it; within the chunk, a tile whose padded size would
+// take the running total past the budget is dropped (and counted, as the
+// reference counts it, even when empty). Here four threads share a chunk,
+// 16 tiles each: a chunk whose unbudgeted end stays within the budget drops
+// nothing, so each quarter starts at its unbudgeted prefix; a chunk that
+// may overflow is walked in order by its first thread, which leaves each
+// tile's kept count in LDS. Kept tiles get compact, unpadded segments:
+// ranges[t] = (start, start + count). Then the tiles are ordered by
+// descending key count for the raster launch (as k_tile_order).
+constexpr int kScanThreads = 1024;
+constexpr int kScanQuarter = kScanChunk / 4;   // tiles per thread
+
+__global__ __launch_bounds__(kScanThreads) void k_seg_scan(const uint32_t* __restrict__ tileCnt, int numTiles,
+                                                           uint64_t cap, const uint32_t* __restrict__ visPart,
+                                                           int nb, uint2* __restrict__ ranges,
+                                                           uint32_t* __restrict__ order,
+                                                           uint32_t* __restrict__ numKeys,
+                                                           uint32_t* __restrict__ numVisible,
+                                                           uint64_t* __restrict__ totalKeys,
+                                                           double* __restrict__ stats) {
+  // tile t at sc[t + t / 16] (numTiles + numTiles / 16 + 1 words): coalesced
+  // fill, conflict-free walks (stride 17 between threads)
+  uint32_t* sc = seg_dyn;
+  __shared__ uint64_t wsA[kScanThreads / 64];
+  __shared__ uint64_t wsB[kScanThreads / 64];
+  __shared__ uint32_t wsC[kScanThreads / 64];
+  __shared__ uint32_t wsV[kScanThreads / 64];
+  __shared__ uint32_t bucket[256];
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  // 16 loads in flight per thread per batch
-  for (int b0 = 0; b0 < kScanChunk; b0 += 16) {
-    uint32_t v[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-      const int t = tid + kBlock * (b0 + k);
-      v[k] = t < numTiles ? tileCnt[t] : 0u;
-    }
-#pragma unroll
-    for (int k = 0; k < 16; k++) {
-      const int t = tid + kBlock * (b0 + k);
-      sc[t + t / kScanChunk] = v[k];
-    }
-  }
+  for (int t = tid; t < numTiles; t += kScanThreads) sc[t + t / kScanQuarter] = tileCnt[t];
+  if (tid < 256) bucket[tid] = 0u;
   __syncthreads();
-  const int t0 = tid * kScanChunk;
-  const uint32_t* mine = sc + tid * (kScanChunk + 1);
+  const int t0 = tid * kScanQuarter;
+  uint32_t* mine = sc + tid * (kScanQuarter + 1);
+  const int nt = numTiles - t0 >= kScanQuarter ? kScanQuarter : (numTiles - t0 > 0 ? numTiles - t0 : 0);
   uint64_t pad = 0, raw = 0;
-  for (int k = 0; k < kScanChunk; k++) {
-    const int t = t0 + k;
-    if (t < numTiles) {
-      const uint32_t c = mine[k];
-      pad += seg_padded(c);
-      raw += c;
-    }
+  for (int k = 0; k < nt; k++) {
+    const uint32_t c = mine[k];
+    pad += seg_padded(c);
+    raw += c;
   }
+  // unbudgeted exclusive prefix (tile order), the chunk's start and end
   uint64_t x = pad;
   for (int o = 1; o < 64; o <<= 1) {
     const uint64_t y = __shfl_up(x, o);
     if (lane >= o) x += y;
   }
-  if (lane == 63) ws[w] = x;
+  if (lane == 63) wsA[w] = x;
   __syncthreads();
   uint64_t pre = 0;
-  for (int k = 0; k < w; k++) pre += ws[k];
-  const uint64_t acc0 = pre + x - pad;
-  uint64_t acc = acc0, kept = 0;
+  for (int k = 0; k < w; k++) pre += wsA[k];
+  const uint64_t acc0 = pre + x - pad;                // this quarter's unbudgeted start
+  const uint64_t q0 = __shfl(acc0, lane & ~3);        // the chunk's start
+  const uint64_t qend = __shfl(acc0 + pad, lane | 3); // the chunk's unbudgeted end
+  // an overflowing chunk: its first thread walks its 64 tiles in order and
+  // rewrites their counts as kept counts (0 = dropped)
   uint32_t dropped = 0;
-  for (int k = 0; k < kScanChunk; k++) {
-    const int t = t0 + k;
-    if (t >= numTiles) break;
-    const uint32_t c = mine[k];
-    uint64_t p = seg_padded(c);
-    if (acc + p > cap) { p = 0; dropped++; }
-    acc += p;
-    kept += p ? c : 0u;
+  if (qend > cap && (tid & 3) == 0) {
+    uint64_t acc = q0;
+    for (int k = 0; k < kScanChunk; k++) {
+      const int t = t0 + k;
+      if (t >= numTiles) break;
+      uint32_t* cp = sc + t + t / kScanQuarter;
+      const uint32_t c = *cp;
+      uint64_t p = seg_padded(c);
+      if (acc + p > cap) { p = 0; dropped++; }
+      acc += p;
+      *cp = p ? c : 0u;
+    }
   }
+  __syncthreads();
+  uint64_t kept = 0;
+  for (int k = 0; k < nt; k++) kept += mine[k];
+  // compact starts
   uint64_t y = kept;
   for (int o = 1; o < 64; o <<= 1) {
     const uint64_t z = __shfl_up(y, o);
@@ -724,39 +754,64 @@ __global__ __launch_bounds__(kBlock) void k_seg_scan(const uint32_t* __restrict_
   }
   uint64_t rsum = raw;
   uint32_t dsum = dropped;
+  uint32_t vsum = tid < nb ? visPart[tid] : 0u;   // nb <= kSegMaxRows = kScanThreads
   for (int o = 32; o > 0; o >>= 1) {
     rsum += __shfl_xor(rsum, o);
     dsum += __shfl_xor(dsum, o);
+    vsum += __shfl_xor(vsum, o);
   }
-  if (lane == 63) wk[w] = y;
-  if (lane == 0) { wr[w] = rsum; wd[w] = dsum; }
+  if (lane == 63) wsA[w] = y;   // wsA's earlier reads all precede the barrier above
+  if (lane == 0) { wsB[w] = rsum; wsC[w] = dsum; wsV[w] = vsum; }
   __syncthreads();
   uint64_t kpre = 0;
-  for (int k = 0; k < w; k++) kpre += wk[k];
+  for (int k = 0; k < w; k++) kpre += wsA[k];
   uint64_t start = kpre + y - kept;
-  acc = acc0;
-  for (int k = 0; k < kScanChunk; k++) {
-    const int t = t0 + k;
-    if (t >= numTiles) break;
-    const uint32_t c = mine[k];
-    uint64_t p = seg_padded(c);
-    if (acc + p > cap) p = 0;
-    acc += p;
-    const uint32_t len = p ? c : 0u;
-    ranges[t] = make_uint2((uint32_t)start, (uint32_t)(start + len));
+  for (int k = 0; k < nt; k++) {
+    const uint32_t len = mine[k];
+    const uint2 rg = make_uint2((uint32_t)start, (uint32_t)(start + len));
+    ranges[t0 + k] = rg;
+    atomicAdd(&bucket[tile_bucket(rg)], 1u);
     start += len;
   }
-  if (tid == kBlock - 1) *numKeys = (uint32_t)start;
+  if (tid == kScanThreads - 1) *numKeys = (uint32_t)start;
   if (tid == 0) {
-    *totalKeys = wr[0] + wr[1] + wr[2] + wr[3];
-    const uint32_t d = wd[0] + wd[1] + wd[2] + wd[3];
+    uint64_t r = 0;
+    uint32_t d = 0, v = 0;
+    for (int k = 0; k < kScanThreads / 64; k++) { r += wsB[k]; d += wsC[k]; v += wsV[k]; }
+    *totalKeys = r;
     if (d) stats[3] += (double)d;
+    *numVisible = v;
+  }
+  __syncthreads();
+  // exclusive bucket offsets, heaviest bucket first: threads 0..255 scan
+  // bucket 255 - tid (four waves)
+  uint32_t bc = 0, bx = 0;
+  if (tid < 256) {
+    bc = bucket[255 - tid];
+    bx = bc;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t z = __shfl_up(bx, o);
+      if (lane >= o) bx += z;
+    }
+    if (lane == 63) wsC[w] = bx;
+  }
+  __syncthreads();
+  if (tid < 256) {
+    uint32_t bp = 0;
+    for (int k = 0; k < w; k++) bp += wsC[k];
+    bucket[255 - tid] = bp + bx - bc;
+  }
+  __syncthreads();
+  for (int t = tid; t < numTiles; t += kScanThreads) {
+    const uint32_t len = sc[t + t / kScanQuarter];
+    order[atomicAdd(&bucket[tile_bucket(make_uint2(0u, len))], 1u)] = (uint32_t)t;
   }
 }
 
-// Every (Gaussian, kept tile) key into its tile's segment: at the tile's
-// start, plus this block's offset in the tile (the column scan), plus an LDS
-// counter; the tile's block mask (gs_cover.h) in the low 4 bits.
+// Every (Gaussian, kept tile) key into its tile's segment: each tile's LDS
+// counter starts at the tile's segment start plus this block's offset in
+// the tile (the column scan), so a key's slot is one LDS atomic; the tile's
+// block mask (gs_cover.h) rides in the low 4 bits.
 __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(int64_t n, int64_t G, const uint32_t* __restrict__ cnt,
                                                              const uint2* __restrict__ rect,
                                                              const uint32_t* __restrict__ depthKey,
@@ -764,18 +819,24 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(int64_t n, int64_t 
                                                              const uint32_t* __restrict__ table,
                                                              const uint2* __restrict__ ranges,
                                                              uint64_t* __restrict__ keys) {
-  __shared__ uint32_t cur[kMaxSegTiles];
+  uint32_t* cur = seg_dyn;                            // numTiles: next slot of each tile
+  uint8_t* dropped = (uint8_t*)(seg_dyn + numTiles);  // numTiles: tile dropped by the budget
   __shared__ uint32_t sOff[kSegThreads];
   __shared__ uint2 sRect[kSegThreads];
   __shared__ uint32_t sDk[kSegThreads];
   __shared__ gs_cover_splat sG[kSegThreads];
   __shared__ uint32_t wsum[kSegThreads / 64];
+  __shared__ uint16_t owner[kOwnerCap];
   const int tid = threadIdx.x;
   const uint32_t* row = table + (int64_t)blockIdx.x * numTiles;
-  for (int t = tid; t < numTiles; t += kSegThreads) cur[t] = 0u;
+  for (int t = tid; t < numTiles; t += kSegThreads) {
+    const uint2 rg = ranges[t];
+    cur[t] = rg.x + row[t];
+    dropped[t] = rg.y == rg.x;
+  }
   const int64_t lo = (int64_t)blockIdx.x * G;
   const int64_t hi = lo + G < n ? lo + G : n;
-  const SegRound sr{sOff, sRect, wsum};
+  const SegRound sr{sOff, sRect, wsum, owner};
   seg_rounds(lo, hi, cnt, rect, tilesX, sr,
              [&](int k, int64_t i, uint32_t c) {
                sDk[k] = c ? depthKey[i] : 0u;
@@ -786,9 +847,8 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(int64_t n, int64_t 
                }
              },
              [&](int g, int64_t r0, uint32_t tile, uint32_t tx, uint32_t ty) {
-               const uint2 rg = ranges[tile];
-               if (rg.y == rg.x) return;   // tile dropped by the budget (shaders.js:406-407)
-               const uint32_t pos = rg.x + row[tile] + atomicAdd(&cur[tile], 1u);
+               const uint32_t pos = atomicAdd(&cur[tile], 1u);
+               if (dropped[tile]) return;   // tile dropped by the budget (shaders.js:406-407)
                const gs_cover_splat sg = sG[g];
                const uint32_t mask = gs_cover_tile_mask(&sg, (int)tx, (int)ty);
                keys[pos] = seg_key(sDk[g], (uint32_t)(r0 + g), mask);
@@ -1008,10 +1068,10 @@ static void radix_sort(hipStream_t st, const SortScratch& s, const KeyT* kin, co
 }
 
 int64_t seg_group(int64_t n) {
-  // Gaussians per count/scatter block: 4096, more for huge n so that the
-  // [block][tile] table keeps at most 1024 rows
-  const int64_t g = (n + 1023) / 1024;
-  return g > 4096 ? (g + kSegThreads - 1) / kSegThreads * kSegThreads : 4096;
+  // Gaussians per count/scatter block: kSegMinGroup, more for huge n so
+  // that the [block][tile] table keeps at most kSegMaxRows rows
+  const int64_t g = (n + kSegMaxRows - 1) / kSegMaxRows;
+  return g > kSegMinGroup ? (g + kSegThreads - 1) / kSegThreads * kSegThreads : kSegMinGroup;
 }
 
 int tile_bits(int numTiles) {
@@ -1029,14 +1089,26 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
     // tile segments: count table, column scan, budgeted scan, scatter, per-tile sort
     const int64_t G = seg_group(a.n);
     const int nb = (int)((a.n + G - 1) / G);
-    launch_fill_u32(st, a.numVisible, 1, 0u);
-    k_seg_count<<<nb, kSegThreads, 0, st>>>(a.n, G, a.cnt, a.rect, a.tilesX, a.numTiles, s.segTable, a.numVisible);
+    static const bool ldsRaised = [] {
+      // the per-tile LDS arrays at kMaxSegTiles: 5 B per tile in the scatter
+      const int dyn = (kMaxSegTiles + kMaxSegTiles / kScanQuarter + 1) * 5;
+      (void)hipFuncSetAttribute((const void*)k_seg_count, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+      (void)hipFuncSetAttribute((const void*)k_seg_scan, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+      (void)hipFuncSetAttribute((const void*)k_seg_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+      return true;
+    }();
+    (void)ldsRaised;
+    const size_t tileLds = (size_t)a.numTiles * 4;
+    k_seg_count<<<nb, kSegThreads, tileLds, st>>>(a.n, G, a.cnt, a.rect, a.tilesX, a.numTiles, s.segTable,
+                                                  s.visPart);
     k_seg_colscan<<<(a.numTiles + 15) / 16, kBlock, 0, st>>>(s.segTable, nb, a.numTiles, s.tileCnt);
-    k_seg_scan<<<1, kBlock, 0, st>>>(s.tileCnt, a.numTiles, a.cap, a.ranges, a.numKeys, a.totalKeys, a.stats);
-    k_seg_scatter<<<nb, kSegThreads, 0, st>>>(a.n, G, a.cnt, a.rect, a.depthKey, a.rec, a.tilesX, a.numTiles,
-                                              s.segTable, a.ranges, (uint64_t*)a.tkA);
+    k_seg_scan<<<1, kScanThreads, (size_t)(a.numTiles + a.numTiles / kScanQuarter + 1) * 4, st>>>(
+        s.tileCnt, a.numTiles, a.cap, s.visPart, nb, a.ranges, a.tileOrder, a.numKeys, a.numVisible, a.totalKeys,
+        a.stats);
+    k_seg_scatter<<<nb, kSegThreads, (size_t)a.numTiles * 5, st>>>(a.n, G, a.cnt, a.rect, a.depthKey, a.rec,
+                                                                   a.tilesX, a.numTiles, s.segTable, a.ranges,
+                                                                   (uint64_t*)a.tkA);
     k_seg_sort<<<a.numTiles, kBlock, 0, st>>>(a.ranges, (uint64_t*)a.tkA, (uint64_t*)a.tkB, a.tvA);
-    k_tile_order<<<1, 1024, 0, st>>>(a.ranges, a.numTiles, a.tileOrder);
     out.keys = nullptr;
     out.ids = a.tvA;
     out.keyBytes = 0;

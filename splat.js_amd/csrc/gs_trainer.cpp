@@ -847,9 +847,10 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   rc |= dalloc(h, &h->sort.nDepth, 4);
   rc |= dalloc(h, &h->sort.tileCnt, (size_t)h->maxTiles * 4);
   if (h->o.sortMode == GS_SORT_TILE_SEGMENTS) {
-    // rows = ceil(n / seg_group(n)) <= min(1024, ceil(cap / 4096)) for every n <= cap
-    const int64_t rows = std::min<int64_t>(1024, (cap + 4095) / 4096);
+    // rows = ceil(n / seg_group(n)) <= min(kSegMaxRows, ceil(cap / kSegMinGroup)) for every n <= cap
+    const int64_t rows = std::min<int64_t>(kSegMaxRows, (cap + kSegMinGroup - 1) / kSegMinGroup);
     rc |= dalloc(h, &h->sort.segTable, (size_t)rows * h->maxTiles * 4);
+    rc |= dalloc(h, &h->sort.visPart, (size_t)rows * 4);
   }
   rc |= dalloc(h, &h->dN, 4);
   rc |= dalloc(h, &h->numKeys, 4);
