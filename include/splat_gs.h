@@ -114,7 +114,9 @@ typedef struct gs_info {
   int32_t lastCam, holdout;
   int64_t lastKeys;          /* (tile, Gaussian) keys of the last raster */
   int64_t lastVisible;       /* Gaussians that passed projection culling in the last raster */
-  int64_t overflowKeys;      /* keys dropped for the entry budget, cumulative */
+  int64_t overflowKeys;      /* entry-budget drops, cumulative: tiles with the default sort (the
+                                reference's entryOverflowTiles, trainer.js:627), keys with the
+                                radix sorts */
   int64_t entriesCap;
   int32_t maxTiles;
   float horizon;
@@ -215,7 +217,7 @@ typedef struct gs_refine_opts {
 typedef struct gs_refine_result {
   int64_t moved, grown, n;
   int64_t maxTile;       /* most keys in one tile of the last raster */
-  int64_t overflow;      /* keys dropped for the entry budget, cumulative */
+  int64_t overflow;      /* entry-budget drops, cumulative (as gs_info.overflowKeys) */
 } gs_refine_result;
 
 int gs_refine(gs_trainer_t* h, const gs_refine_opts* o, gs_refine_result* out);
