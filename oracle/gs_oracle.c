@@ -85,8 +85,8 @@ void orc_adam_f32(const orc_adam* hp, int64_t n, float* P, const float* G,
     const float mm = hp->b1 * m[j] + (1.0f - hp->b1) * g;
     const float vv = hp->b2 * v[j] + (1.0f - hp->b2) * g * g;
     m[j] = mm; v[j] = vv;
-    const float mh = mm / hp->bc1;
-    const float vh = vv / hp->bc2;
+    const float mh = mm * hp->rbc1;
+    const float vh = vv * hp->rbc2;
     float p = P[j] - lr * mh / (sqrtf(vh) + hp->eps);
     if (slot >= 3 && slot <= 5) p = fminf(fmaxf(p, hp->minLS), hp->maxLS);
     if (slot >= 10 && slot <= 13) p = fminf(fmaxf(p, -hp->maxLogit), hp->maxLogit);
@@ -103,8 +103,8 @@ void orc_sh_adam_f32(const orc_adam* hp, int64_t count, float* sh, const float* 
     const float mm = hp->b1 * m[j] + (1.0f - hp->b1) * g;
     const float vv = hp->b2 * v[j] + (1.0f - hp->b2) * g * g;
     m[j] = mm; v[j] = vv;
-    const float mh = mm / hp->bc1;
-    const float vh = vv / hp->bc2;
+    const float mh = mm * hp->rbc1;
+    const float vh = vv * hp->rbc2;
     sh[j] = sh[j] - hp->shLr * mh / (sqrtf(vh) + hp->eps);
   }
 }
@@ -132,8 +132,8 @@ void orc_adam_mixed_f32(const orc_adam* hp, int64_t n, float* P, const float* G,
         const float mm = hp->b1 * mf + (1.0f - hp->b1) * g;
         const float vv = hp->b2 * vf + (1.0f - hp->b2) * g * g;
         mf = mm; vf = vv;
-        const float mh = mm / hp->bc1;
-        const float vh = vv / hp->bc2;
+        const float mh = mm * hp->rbc1;
+        const float vh = vv * hp->rbc2;
         float p = P[j] - lr * mh / (sqrtf(vh) + hp->eps);
         if (slot >= 3 && slot <= 5) p = fminf(fmaxf(p, hp->minLS), hp->maxLS);
         if (slot >= 10 && slot <= 13) p = fminf(fmaxf(p, -hp->maxLogit), hp->maxLogit);
@@ -154,8 +154,8 @@ void orc_sh_adam_mixed_f32(const orc_adam* hp, int64_t n, int nc, uint16_t* sh, 
       if (!(fabsf(g) < 1e18f)) g = 0.0f;
       const float mm = hp->b1 * gs_bf2f(m[j]) + (1.0f - hp->b1) * g;
       const float vv = hp->b2 * gs_bf2f(v[j]) + (1.0f - hp->b2) * g * g;
-      const float mh = mm / hp->bc1;
-      const float vh = vv / hp->bc2;
+      const float mh = mm * hp->rbc1;
+      const float vh = vv * hp->rbc2;
       const float s = gs_bf2f(sh[j]) - hp->shLr * mh / (sqrtf(vh) + hp->eps);
       sh[j] = (uint16_t)gs_f2bf_sr(s, gs_sr_bits(seed, 2u * 64u + (uint32_t)c));
       m[j] = (uint16_t)gs_f2bf_sr(mm, gs_sr_bits(seed, 3u * 64u + (uint32_t)c));

@@ -48,7 +48,7 @@ typedef struct {
 typedef struct {
   float lr[16];
   float b1, b2, eps;
-  float bc1, bc2;      /* 1 - b1^t, 1 - b2^t */
+  float rbc1, rbc2;    /* 1 / (1 - b1^t), 1 / (1 - b2^t): bias corrections as multipliers */
   float minLS, maxLS, maxLogit, reg;
   float shLr;
 } orc_adam;

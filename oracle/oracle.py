@@ -37,7 +37,7 @@ class OrcOpts(C.Structure):
 
 class OrcAdam(C.Structure):
     _fields_ = [("lr", C.c_float * 16), ("b1", C.c_float), ("b2", C.c_float), ("eps", C.c_float),
-                ("bc1", C.c_float), ("bc2", C.c_float), ("minLS", C.c_float), ("maxLS", C.c_float),
+                ("rbc1", C.c_float), ("rbc2", C.c_float), ("minLS", C.c_float), ("maxLS", C.c_float),
                 ("maxLogit", C.c_float), ("reg", C.c_float), ("shLr", C.c_float)]
 
 
@@ -284,8 +284,8 @@ def adam_hp(lr, b1=0.9, b2=0.999, eps=1e-15, t=1, minLS=-20.0, maxLS=2.0, maxLog
         h.lr[k] = float(lr[k])
     b1f, b2f = float(np.float32(b1)), float(np.float32(b2))
     h.b1, h.b2, h.eps = b1f, b2f, eps
-    h.bc1 = float(np.float32(1.0 - b1f ** t))
-    h.bc2 = float(np.float32(1.0 - b2f ** t))
+    h.rbc1 = float(np.float32(1.0 / (1.0 - b1f ** t)))
+    h.rbc2 = float(np.float32(1.0 / (1.0 - b2f ** t)))
     h.minLS, h.maxLS, h.maxLogit, h.reg, h.shLr = minLS, maxLS, maxLogit, reg, shLr
     return h
 

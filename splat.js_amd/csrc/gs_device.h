@@ -40,7 +40,8 @@ struct Cut {
 // shaders.js:952-1036, filled as trainer.js:315-342, :417-429 do).
 struct AdamHP {
   float lr[16];
-  float b1, b2, eps, bc1, bc2;
+  float b1, b2, eps;
+  float rbc1, rbc2;   // 1 / (1 - b1^t), 1 / (1 - b2^t): bias corrections as multipliers
   float minLS, maxLS, maxLogit, reg;
   float shLr;
   float anisoReg;

@@ -152,13 +152,19 @@ GS_HD uint32_t gs_sr_seed(uint64_t gaussian, uint32_t step) {
   return gs_hash32((uint32_t)gaussian * 0x9E3779B1u ^ gs_hash32((uint32_t)(gaussian >> 32) + step * 0xC2B2AE3Du));
 }
 
-// 16 random bits for value k of that seed (k = 64 * stream + slot): one
-// multiply-xorshift round per pair of values, the low half for even k and
-// the high half for odd k.
-GS_HD uint32_t gs_sr_bits(uint32_t seed, uint32_t k) {
-  uint32_t x = seed ^ ((k >> 1) * 0x9E3779B9u);
+// Random bits of that seed for the pair of values k = 2p, 2p + 1 (k = 64 *
+// stream + slot): one multiply-xorshift round; value 2p takes the low 16
+// bits and 2p + 1 the high 16 (gs_sr_bits).
+GS_HD uint32_t gs_sr_pair(uint32_t seed, uint32_t p) {
+  uint32_t x = seed ^ (p * 0x9E3779B9u);
   x *= 0x85ebca6bu; x ^= x >> 13;
   x *= 0xc2b2ae35u; x ^= x >> 16;
+  return x;
+}
+
+// 16 random bits for value k of that seed
+GS_HD uint32_t gs_sr_bits(uint32_t seed, uint32_t k) {
+  const uint32_t x = gs_sr_pair(seed, k >> 1);
   return (k & 1u) ? (x >> 16) : (x & 0xffffu);
 }
 
@@ -173,8 +179,9 @@ GS_HD uint32_t gs_f2bf_rne(float x) {
 
 GS_HD uint32_t gs_f2bf_sr(float x, uint32_t rnd) {
   const uint32_t u = gs_f2u(x);
-  if ((u & 0x7f800000u) == 0x7f800000u) return (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);
-  return (u + (rnd & 0xffffu)) >> 16;
+  const uint32_t special = (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);   // inf stays inf, NaN stays NaN
+  const uint32_t r = (u + (rnd & 0xffffu)) >> 16;
+  return (u & 0x7f800000u) == 0x7f800000u ? special : r;
 }
 
 // 1 / (1 + e^-clamp(x, -9, 9)): the activation for colour and opacity logits.

@@ -313,8 +313,8 @@ AdamHP make_hp(const gs_trainer* h) {
   a.lr[13] = 2.5e-2f;
   a.lr[14] = a.lr[15] = 0.0f;
   a.b1 = 0.9f; a.b2 = 0.999f; a.eps = 1e-15f;
-  a.bc1 = (float)(1.0 - std::pow((double)0.9f, (double)h->iter));
-  a.bc2 = (float)(1.0 - std::pow((double)0.999f, (double)h->iter));
+  a.rbc1 = (float)(1.0 / (1.0 - std::pow((double)0.9f, (double)h->iter)));
+  a.rbc2 = (float)(1.0 / (1.0 - std::pow((double)0.999f, (double)h->iter)));
   a.step = (uint32_t)h->iter;
   const double r = h->sceneRadius;
   a.minLS = (float)std::log(r * (double)h->o.minScale);

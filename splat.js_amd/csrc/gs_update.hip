@@ -168,6 +168,14 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
       float ms[8], vs[8];
       unpack_bf8(SHM16[c * cap + i], ms);
       unpack_bf8(SHV16[c * cap + i], vs);
+      // random bits of values 8c .. 8c + 7 of streams 2, 3, 4: one hash per pair
+      uint32_t hs[4], hm[4], hv[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        hs[q] = gs_sr_pair(seed, 2u * 32u + 4u * (uint32_t)c + (uint32_t)q);
+        hm[q] = gs_sr_pair(seed, 3u * 32u + 4u * (uint32_t)c + (uint32_t)q);
+        hv[q] = gs_sr_pair(seed, 4u * 32u + 4u * (uint32_t)c + (uint32_t)q);
+      }
       uint32_t bs[8], bm[8], bv[8];
 #pragma unroll
       for (int l = 0; l < 8; l++) {
@@ -178,13 +186,14 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
           if (!(fabsf(gr) < 1e18f)) gr = 0.0f;
           mm = hp.b1 * ms[l] + (1.0f - hp.b1) * gr;
           vn = hp.b2 * vs[l] + (1.0f - hp.b2) * gr * gr;
-          const float mh = mm / hp.bc1;
-          const float vh = vn / hp.bc2;
+          const float mh = mm * hp.rbc1;
+          const float vh = vn * hp.rbc2;
           ss = sh[j] - hp.shLr * mh / (sqrtf(vh) + hp.eps);
         }
-        bs[l] = gs_f2bf_sr(ss, gs_sr_bits(seed, 2u * 64u + (uint32_t)j));
-        bm[l] = gs_f2bf_sr(mm, gs_sr_bits(seed, 3u * 64u + (uint32_t)j));
-        bv[l] = gs_f2bf_sr(vn, gs_sr_bits(seed, 4u * 64u + (uint32_t)j));
+        const int sh16 = (l & 1) ? 16 : 0;   // value j = 8c + l: half l & 1 of pair l / 2
+        bs[l] = gs_f2bf_sr(ss, hs[l >> 1] >> sh16);
+        bm[l] = gs_f2bf_sr(mm, hm[l >> 1] >> sh16);
+        bv[l] = gs_f2bf_sr(vn, hv[l >> 1] >> sh16);
       }
       ((uint4*)u.SH)[c * cap + i] = pack_bf8(bs);
       ((uint4*)u.SHM)[c * cap + i] = pack_bf8(bm);
@@ -211,8 +220,8 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
         const float vn = hp.b2 * vs[l] + (1.0f - hp.b2) * gr * gr;
         ms[l] = mm;
         vs[l] = vn;
-        const float mh = mm / hp.bc1;
-        const float vh = vn / hp.bc2;
+        const float mh = mm * hp.rbc1;
+        const float vh = vn * hp.rbc2;
         ss[l] = ss[l] - hp.shLr * mh / (sqrtf(vh) + hp.eps);
       }
       SH[c * cap + i] = make_float4(ss[0], ss[1], ss[2], ss[3]);
@@ -238,8 +247,8 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
     const float vn = hp.b2 * vj + (1.0f - hp.b2) * gr * gr;
     mj = mm;
     vj = vn;
-    const float mh = mm / hp.bc1;
-    const float vh = vn / hp.bc2;
+    const float mh = mm * hp.rbc1;
+    const float vh = vn * hp.rbc2;
     float pn = p[j] - lr * mh / (sqrtf(vh) + hp.eps);
     if (j >= 3 && j <= 5) pn = clampf(pn, hp.minLS, hp.maxLS);
     if (j >= 10 && j <= 13) pn = clampf(pn, -hp.maxLogit, hp.maxLogit);
@@ -256,13 +265,21 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
       float m[8], vv[8];
       unpack_bf8(M16[h * cap + i], m);
       unpack_bf8(V16[h * cap + i], vv);
+      // random bits of slots 8h .. 8h + 7 of streams 0, 1: one hash per pair
+      uint32_t hm[4], hv[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        hm[q] = gs_sr_pair(seed, 0u * 32u + 4u * (uint32_t)h + (uint32_t)q);
+        hv[q] = gs_sr_pair(seed, 1u * 32u + 4u * (uint32_t)h + (uint32_t)q);
+      }
       uint32_t bm[8], bv[8];
 #pragma unroll
       for (int l = 0; l < 8; l++) {
         const int j = 8 * h + l;
         adam_slot(j, m[l], vv[l]);
-        bm[l] = gs_f2bf_sr(m[l], gs_sr_bits(seed, 0u * 64u + (uint32_t)j));
-        bv[l] = gs_f2bf_sr(vv[l], gs_sr_bits(seed, 1u * 64u + (uint32_t)j));
+        const int sh16 = (l & 1) ? 16 : 0;
+        bm[l] = gs_f2bf_sr(m[l], hm[l >> 1] >> sh16);
+        bv[l] = gs_f2bf_sr(vv[l], hv[l >> 1] >> sh16);
       }
       M16[h * cap + i] = pack_bf8(bm);
       V16[h * cap + i] = pack_bf8(bv);
