@@ -177,11 +177,13 @@ GS_HD uint32_t gs_f2bf_rne(float x) {
   return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
 }
 
+// Adds 16 random bits below the kept mantissa and truncates. An infinity
+// has a zero mantissa, so it cannot carry and stays infinite; a NaN keeps
+// its sign and exponent and a set mantissa bit.
 GS_HD uint32_t gs_f2bf_sr(float x, uint32_t rnd) {
   const uint32_t u = gs_f2u(x);
-  const uint32_t special = (u >> 16) | ((u & 0xffffu) ? 0x40u : 0u);   // inf stays inf, NaN stays NaN
   const uint32_t r = (u + (rnd & 0xffffu)) >> 16;
-  return (u & 0x7f800000u) == 0x7f800000u ? special : r;
+  return x != x ? (u >> 16) | 0x40u : r;
 }
 
 // 1 / (1 + e^-clamp(x, -9, 9)): the activation for colour and opacity logits.

@@ -56,7 +56,7 @@ def test_stochastic_rounding_is_exactly_unbiased_over_its_random_bits():
     rnd = np.arange(1 << 16, dtype=np.uint32)
     for x in xs:
         out = O.f2bf_sr(np.full(1 << 16, x, np.float32), rnd).astype(np.uint32)
-        xb = int(_bits(np.float32(x)))
+        xb = int(np.float32(x).view(np.uint32))
         ups = int(np.sum(out != (xb >> 16)))
         assert ups == (xb & 0xFFFF), (x, ups)
         mean = O.bf2f(out.astype(np.uint16)).astype(np.float64).mean()
