@@ -110,7 +110,10 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
     build_masks(cover, sMask);
     __syncthreads();
     const uint32_t cnt = min((uint32_t)kBlock, e - base);
-    if (wave_ballot(!done) != 0ull) {
+    // lanes still compositing; a lane is done once outside the image or
+    // once T < 1e-4, so the vote is kept current with one compare per splat
+    uint64_t run = wave_ballot(!done);
+    if (run != 0ull) {
       uint32_t term = 0xffffffffu;  // batch index of this pixel's last splat
       for (int c = 0; c < 4; c++) {
         uint64_t m = uniform64(sMask[c][wv]);
@@ -127,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
           // and the exp below needs its argument in [-87, 0]
           const float ee = fminf(fmaxf(fmaf(0.5f, t2, (a.w * dx) * dy), 0.0f), 87.0f);
           const bool live = !done && ee <= cut.eCut;
-          if (wave_ballot(live) != 0ull) {
+          if ((run & wave_ballot(ee <= cut.eCut)) != 0ull) {
             const float araw = bb.y * gs_expf_nonpos_in(-ee);   // == gs_expf(-ee) for ee <= eCut <= 80
             const float alpha = fminf(0.99f, araw);
             if (live && alpha >= cut.aMin) {
@@ -139,7 +142,8 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
               T = T * (1.0f - alpha);
               if (T < 1e-4f) { done = true; term = (uint32_t)j; }
             }
-            if (wave_ballot(!done) == 0ull) { m = 0ull; c = 4; }
+            run &= wave_ballot(!(T < 1e-4f));
+            if (run == 0ull) { m = 0ull; c = 4; }
           }
         }
       }
@@ -299,6 +303,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
     }
   }
   const bool active = inside && g.w != 0.0f;
+  const uint64_t activeMask = wave_ballot(active);
   if (tid == 0) sEnd = s;
   __syncthreads();
   if (myEnd > s) atomicMax(&sEnd, myEnd);
@@ -353,13 +358,16 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
         const float t2 = fmaf(cC * dy, dy, (cA * dx) * dx);
         // clamped to [0, 87] as in the forward pass
         const float e = fminf(fmaxf(fmaf(0.5f, t2, (cB * dx) * dy), 0.0f), 87.0f);
+        // votes of single compares, combined in scalar registers (a vote
+        // of a combined condition costs two VALU instructions)
         const bool live = active && k < myEnd && e <= cut.eCut;
-        if (wave_ballot(live) == 0ull) continue;
+        const uint64_t liveMask = activeMask & wave_ballot(k < myEnd) & wave_ballot(e <= cut.eCut);
+        if (liveMask == 0ull) continue;
         const float G = gs_expf_nonpos_in(-e);   // == gs_expf(-e) for e <= eCut <= 80
         const float araw = bb.y * G;
         const float alpha = fminf(0.99f, araw);
         const bool contrib = live && alpha >= cut.aMin;
-        if (wave_ballot(contrib) == 0ull) continue;
+        if ((liveMask & wave_ballot(alpha >= cut.aMin)) == 0ull) continue;
         float v[10];
         {
           const float4 c4 = sC[j];

@@ -164,6 +164,36 @@ def test_sort_strategies_give_the_same_permutation():
         np.testing.assert_array_equal(a["img"], b["img"])
 
 
+def test_views_above_the_segment_tile_limit_fall_back_to_radix():
+    """A 2112x2112 view has 17424 tiles, above the 16384 the tile-segment
+    scan handles: the default sort falls back to depth-then-tile and gives
+    the same keys, ranges and image as that strategy chosen explicitly."""
+    W = H = 2112
+    rng = np.random.default_rng(4)
+    n = 20000
+    p = np.zeros((n, 16), np.float32)
+    p[:, 0:2] = rng.uniform(-1.2, 1.2, (n, 2))
+    p[:, 2] = rng.uniform(2.0, 4.0, n)
+    p[:, 3:6] = rng.normal(-3.5, 0.4, (n, 3))
+    p[:, 6] = 1.0
+    p[:, 10:13] = rng.normal(0, 1, (n, 3))
+    p[:, 13] = rng.normal(0, 1, n)
+    cams = [{"imgIdx": 0, "R": [1, 0, 0, 0, 1, 0, 0, 0, 1], "t": [0, 0, 0], "f": 900.0, "cx": W / 2, "cy": H / 2,
+             "w": W, "h": H}]
+    images = [{"tw": W, "th": H, "rgb": np.full((W * H, 3), 0.5, np.float32)}]
+    outs = []
+    for mode in (gsabi.GS_SORT_TILE_SEGMENTS, gsabi.GS_SORT_DEPTH_THEN_TILE):
+        t = _trainer(p, None, cams, images, W, H, 1.0, {}, shDeg=0, sortMode=mode)
+        outs.append(t.step_debug(0, want=("ids", "keys", "tileRanges", "img")))
+        t.destroy()
+    a, b = outs
+    assert a["numKeys"] == b["numKeys"] > 20000
+    np.testing.assert_array_equal(a["ids"], b["ids"])
+    np.testing.assert_array_equal(a["keys"], b["keys"])
+    np.testing.assert_array_equal(a["tileRanges"], b["tileRanges"])
+    np.testing.assert_array_equal(a["img"].view(np.uint32), b["img"].view(np.uint32))
+
+
 def test_f32_accumulation_matches_f64_gradients():
     """GS_ACCUM_F32: atomic f32 sums, checked against the double-precision
     analytic gradients (which central differences pin, test_oracle.py)."""
