@@ -40,7 +40,16 @@ GS_HD float gs_cover_q(float A, float B, float C, float u, float w) {
   return 0.5f * (A * u * u + C * w * w) + B * u * w;
 }
 
-GS_HD float gs_cover_clamp(float v, float lo, float hi) { return fminf(fmaxf(v, lo), hi); }
+// clamp(v, lo, hi) for lo <= hi and finite v (the only uses below). On the
+// device one v_med3_f32: fminf/fmaxf of values computed in another basic
+// block each cost an extra canonicalizing instruction per operand.
+GS_HD float gs_cover_clamp(float v, float lo, float hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_fmed3f(v, lo, hi);
+#else
+  return fminf(fmaxf(v, lo), hi);
+#endif
+}
 
 // 1 when some pixel centre of [u0,u1] x [w0,w1] (offsets from the mean,
 // u0 <= u1, w0 <= w1) may have e <= L; 0 only when none can.
@@ -81,8 +90,10 @@ GS_HD gs_cover_splat gs_cover_prepare(float mx, float my, float A, float B, floa
 // w[2j..2j+1]. A vertical edge line u is evaluated at the clamped vertex
 // kC*u of each block row, q = hA u^2 + wc (hC wc + B u), and symmetrically
 // for horizontal lines; the f32 error of this form is also a few ulp of S.
+// Branch-free, so the whole test is one basic block on the device (a
+// conic that is not positive definite keeps every block: kA = kC = 0 there
+// and the mask computed for it is discarded).
 GS_HD uint32_t gs_cover_tile_mask(const gs_cover_splat* s, int tx, int ty) {
-  if (!s->pd) return 0xfu;
   const float x0 = (float)(tx * 16) + 0.5f, y0 = (float)(ty * 16) + 0.5f;
   const float u[4] = {x0 - s->mx, (x0 + 7.0f) - s->mx, (x0 + 8.0f) - s->mx, (x0 + 15.0f) - s->mx};
   const float w[4] = {y0 - s->my, (y0 + 7.0f) - s->my, (y0 + 8.0f) - s->my, (y0 + 15.0f) - s->my};
@@ -102,13 +113,14 @@ GS_HD uint32_t gs_cover_tile_mask(const gs_cover_splat* s, int tx, int ty) {
   for (int bj = 0; bj < 2; bj++)
     for (int bi = 0; bi < 2; bi++) {
       const float u0 = u[2 * bi], u1 = u[2 * bi + 1], w0 = w[2 * bj], w1 = w[2 * bj + 1];
-      const int inside = u0 <= 0.0f && u1 >= 0.0f && w0 <= 0.0f && w1 >= 0.0f;
+      const int inside = (u0 <= 0.0f) & (u1 >= 0.0f) & (w0 <= 0.0f) & (w1 >= 0.0f);
       const float q = fminf(fminf(qv[2 * bi][bj], qv[2 * bi + 1][bj]), fminf(qh[2 * bj][bi], qh[2 * bj + 1][bi]));
       const float du = fmaxf(fabsf(u0), fabsf(u1)), dw = fmaxf(fabsf(w0), fabsf(w1));
       const float S = fmaf(s->hA * du, du, fmaf(s->hC * dw, dw, aB * du * dw));
-      if (inside || !(q > s->L + GS_COVER_MARGIN * S)) m |= 1u << (2 * bj + bi);
+      const int near = !(q > s->L + GS_COVER_MARGIN * S);
+      m |= (uint32_t)(inside | near) << (2 * bj + bi);
     }
-  return m;
+  return s->pd ? m : 0xfu;
 }
 
 // Cover level stored with each splat: an upper bound on the largest f32

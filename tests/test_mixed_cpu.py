@@ -84,6 +84,27 @@ def test_sr_bits_are_uniform_and_uncorrelated():
     assert abs(np.mean(flat[1:] * flat[:-1])) / np.var(flat) < 0.02
 
 
+def test_sr_bits_of_deep_sh_slots():
+    """The SH streams draw bits for up to 48 coefficient slots per (Gaussian,
+    step) (degree 3, padded): the 16-bit words stay uniform (mean, variance,
+    16-bin chi-square on the top bits) in every third of the slot range, and
+    neighbouring slots and streams are uncorrelated."""
+    L = O.lib()
+    n = 300
+    v = np.array([[[L.orc_sr_bits(i, c, 11, st) for c in range(48)] for st in (2, 3, 4)] for i in range(n)],
+                 np.float64)                                   # [gaussian, stream, coefficient]
+    u = v / 65536.0
+    for depth in (slice(0, 16), slice(16, 32), slice(32, 48)):
+        w = u[:, :, depth].reshape(-1)
+        assert abs(w.mean() - 0.5) < 0.015 and abs(w.var() - 1.0 / 12.0) < 0.006
+        hist = np.bincount((w * 16).astype(int), minlength=16)
+        e = w.size / 16.0
+        assert np.sum((hist - e) ** 2 / e) < 45.0              # chi-square, 15 dof: p ~ 1e-4
+    d = u - 0.5
+    assert abs(np.mean(d[:, :, 1:] * d[:, :, :-1])) / np.var(d) < 0.02
+    assert abs(np.mean(d[:, 1:] * d[:, :-1])) / np.var(d) < 0.02
+
+
 def test_mixed_adam_is_f32_adam_plus_documented_rounding():
     rng = np.random.default_rng(3)
     n = 257
