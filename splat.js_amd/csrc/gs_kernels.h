@@ -106,8 +106,10 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out);
 void launch_raster_fwd(hipStream_t st, const Cam& cam, const Cut& cut, const RasterBufs& b, int loss);
 void launch_loss_l1ssim(hipStream_t st, const Cam& cam, float lambda, const RasterBufs& b,
                         float* scratch);
+// boundedColourGrad: every |dL/dC| * 16384 is far below 1e9 (the
+// Charbonnier loss), so the colour terms skip the fixed-point clamp
 void launch_raster_bwd(hipStream_t st, const Cam& cam, const Cut& cut, const RasterBufs& b,
-                       int accumMode, void* accum);
+                       int accumMode, void* accum, bool boundedColourGrad);
 
 void launch_update(hipStream_t st, int deg, int accumMode, bool mixed, const Cam& cam, const AdamHP& hp,
                    const UpdateBufs& u);

@@ -272,13 +272,17 @@ __device__ __forceinline__ int reduce10_slot() {
   return s >= 0 && s < 10 ? s : -1;
 }
 
-template <int kAccum>
+// kClampColour false: dL/dC is bounded (the Charbonnier loss: |dL/dC| <=
+// gain * |err| / sqrt(err^2 + d^2) <= gain), so the colour terms
+// 16384 dL/dC * alpha * T (alpha * T <= 1) stay far inside the +-1e9 clamp
+// and convert without it.
+template <int kAccum, bool kClampColour>
 __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterBufs b, void* accum) {
+  // four 16-byte records per staged splat, all at index j: one LDS address
   __shared__ float4 sA[kBlock];   // mx, my, A, B
   __shared__ float4 sB[kBlock];   // C, opa*comp, L, 0
   __shared__ float4 sC[kBlock];   // r, g, b, opa
-  __shared__ float2 sD[kBlock];   // -0.5 cnorm, 1 - opa
-  __shared__ uint32_t sId[kBlock];
+  __shared__ float4 sD[kBlock];   // -0.5 cnorm, 1 - opa, id (bits), 0
   __shared__ uint64_t sMask[4][4];
   __shared__ uint32_t sEnd;
   const int tid = threadIdx.x, wv = tid >> 6;
@@ -329,12 +333,11 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
       cover = v >> kMaskShift;
       if (cover != 0u) {
         const uint32_t id = v & kIdMask;
-        sId[tid] = id;
         sA[tid] = b.rec[4 * id + 0];
         sB[tid] = b.rec[4 * id + 1];
         sC[tid] = b.rec[4 * id + 2];
         const float4 d = b.rec[4 * id + 3];
-        sD[tid] = make_float2(d.y, d.z);
+        sD[tid] = make_float4(d.y, d.z, __uint_as_float(id), 0.0f);
       }
     }
     build_masks(cover, sMask);
@@ -352,7 +355,10 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
         // leaves as soon as no lane is inside the cutoff (before the exp) or
         // no lane takes the splat.
         const float4 a = sA[j];
-        const float4 bb = sB[j];
+        const float2 bb = *(const float2*)&sB[j];   // C, opa*comp
+        // keep both in this one 8-byte read (otherwise the compiler sinks
+        // the second word into the next block with its own address move)
+        asm volatile("" ::"v"(bb.y));
         const float dx = px - a.x, dy = py - a.y;
         const float cA = a.z, cB = a.w, cC = bb.x;
         const float t2 = fmaf(cC * dy, dy, (cA * dx) * dx);
@@ -369,9 +375,12 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
         const bool contrib = live && alpha >= cut.aMin;
         if ((liveMask & wave_ballot(alpha >= cut.aMin)) == 0ull) continue;
         float v[10];
+        uint32_t gid;
         {
           const float4 c4 = sC[j];
-          const float2 d2 = sD[j];          // -0.5 cnorm, 1 - opa
+          const float4 d2 = sD[j];          // -0.5 cnorm, 1 - opa, id
+          asm volatile("" ::"v"(d2.z));
+          gid = __float_as_uint(d2.z);
           // Tb = Ta / (1 - alpha), S / (1 - alpha) through one reciprocal.
           // 1 - alpha lies in [0.0099, 1] (alpha in [0, 0.99]); on [2^-9, 1]
           // v_rcp_f32 refined by one FMA Newton step equals the correctly
@@ -401,12 +410,13 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
           GS = fmaf(gcol, aT, GS);
           Ta = contrib ? Tb : Ta;
         }
-        const uint32_t gid = sId[j];
         if (kAccum == 0) {
           // i32(clamp(v * 16384, -1e9, 1e9)) per pixel; v already carries the scale
           uint32_t iv[10];
 #pragma unroll
-          for (int q = 0; q < 10; q++) iv[q] = (uint32_t)(int32_t)fminf(fmaxf(v[q], -1.0e9f), 1.0e9f);
+          for (int q = 0; q < 10; q++)
+            iv[q] = (q >= 7 && !kClampColour) ? (uint32_t)(int32_t)v[q]
+                                               : (uint32_t)(int32_t)fminf(fmaxf(v[q], -1.0e9f), 1.0e9f);
           const uint32_t r = wave_reduce10<uint32_t>(iv);
           if (flusher && r != 0u) atomicAdd((uint32_t*)accum + (int64_t)gid * 12 + mySlot, r);
         } else {
@@ -427,10 +437,11 @@ void launch_raster_fwd(hipStream_t st, const Cam& cam, const Cut& cut, const Ras
 }
 
 void launch_raster_bwd(hipStream_t st, const Cam& cam, const Cut& cut, const RasterBufs& b,
-                       int accumMode, void* accum) {
+                       int accumMode, void* accum, bool boundedColourGrad) {
   const dim3 grid((unsigned)cam.numTiles);
-  if (accumMode == 0) k_raster_bwd<0><<<grid, kBlock, 0, st>>>(cam, cut, b, accum);
-  else k_raster_bwd<1><<<grid, kBlock, 0, st>>>(cam, cut, b, accum);
+  if (accumMode != 0) k_raster_bwd<1, true><<<grid, kBlock, 0, st>>>(cam, cut, b, accum);
+  else if (boundedColourGrad) k_raster_bwd<0, false><<<grid, kBlock, 0, st>>>(cam, cut, b, accum);
+  else k_raster_bwd<0, true><<<grid, kBlock, 0, st>>>(cam, cut, b, accum);
 }
 
 }  // namespace gs

@@ -329,6 +329,10 @@ AdamHP make_hp(const gs_trainer* h) {
   return a;
 }
 
+// The Charbonnier loss's dL/dC = gain * err / sqrt(err^2 + d^2) is at most
+// the exposure gain in magnitude (shaders.js:608-610).
+bool charb_grad_bounded(const Cam& cam) { return std::fabs(cam.gain) <= 6.0e4f; }
+
 Cut make_cut(const gs_trainer* h) {
   Cut c;
   c.eCut = h->o.eCut;
@@ -478,7 +482,7 @@ int eval_pass(gs_trainer* h, int ci, const double* R, const double* t, double g,
   rb.T = h->Tfin;
   rb.end = h->endIdx;
   rb.gC = h->gC;
-  launch_raster_bwd(h->st, cam, make_cut(h), rb, h->o.gradAccum, h->accum);
+  launch_raster_bwd(h->st, cam, make_cut(h), rb, h->o.gradAccum, h->accum, charb_grad_bounded(cam));
   launch_cam_pose(h->st, h->o.gradAccum, true, cam, cam_pose_bufs(h, ci));
   HIPCK(hipGetLastError());
   double s[4];
@@ -598,7 +602,7 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
   b.gC = h->gC;
   {
     Scope s(h, "raster_bwd", 0.0);
-    launch_raster_bwd(h->st, cam, make_cut(h), b, h->o.gradAccum, h->accum);
+    launch_raster_bwd(h->st, cam, make_cut(h), b, h->o.gradAccum, h->accum, !ssim && charb_grad_bounded(cam));
   }
   HIPCK(hipGetLastError());
   float* dGF = nullptr;
