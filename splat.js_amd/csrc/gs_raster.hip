@@ -66,6 +66,21 @@ __device__ __forceinline__ uint64_t uniform64(uint64_t v) {
   return ((uint64_t)hi << 32) | lo;
 }
 
+// Byte offset j * size of a staged record as a VGPR value the compiler
+// cannot recompute: every LDS read of record j then shares this one address
+// register (the compiler otherwise moves the uniform offset into a fresh
+// VGPR before each group of reads).
+__device__ __forceinline__ uint32_t lds_offset(uint32_t bytes) {
+  uint32_t v;
+  asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(bytes));
+  return v;
+}
+
+template <typename T>
+__device__ __forceinline__ const T& lds_at(const void* base, uint32_t off) {
+  return *(const T*)((const char*)base + off);
+}
+
 // 1/x for x in [2^-9, 1]: hardware reciprocal estimate plus one Newton step
 // with fused residual. Equal to IEEE 1.0f/x for all 75,497,473 floats of that
 // range (exhaustive check on gfx950); callers guarantee the range.
@@ -122,8 +137,10 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
           m &= m - 1ull;
           // Straight-line for all lanes; only lanes that take the splat
           // (not done, inside the cutoff, alpha >= A_MIN) update their state.
-          const float4 a = sA[j];
-          const float4 bb = sB[j];
+          const uint32_t jo = lds_offset((uint32_t)j * 16u);
+          const float4 a = lds_at<float4>(sA, jo);
+          const float2 bb = lds_at<float2>(sB, jo);   // C, opa*comp
+          asm volatile("" ::"v"(bb.y));                // one 8-byte read
           const float dx = px - a.x, dy = py - a.y;
           const float t2 = fmaf(bb.x * dy, dy, (a.z * dx) * dx);
           // clamped to [0, 87]: an exponent above 87 > eCut is never taken,
@@ -134,16 +151,23 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
             const float araw = bb.y * gs_expf_nonpos_in(-ee);   // == gs_expf(-ee) for ee <= eCut <= 80
             const float alpha = fminf(0.99f, araw);
             if (live && alpha >= cut.aMin) {
-              const float4 c4 = sC[j];
+              const float4 c4 = lds_at<float4>(sC, jo);
               const float w = T * alpha;
               C0 = fmaf(w, c4.x, C0);
               C1 = fmaf(w, c4.y, C1);
               C2 = fmaf(w, c4.z, C2);
               T = T * (1.0f - alpha);
-              if (T < 1e-4f) { done = true; term = (uint32_t)j; }
             }
-            run &= wave_ballot(!(T < 1e-4f));
-            if (run == 0ull) { m = 0ull; c = 4; }
+            // T only falls here, so a lane reaches T < 1e-4 on this splat
+            // exactly when it is still running and now below; that happens
+            // once per pixel, so its bookkeeping sits behind a scalar branch
+            const bool fin = T < 1e-4f;
+            const uint64_t left = run & ~wave_ballot(fin);
+            if (left != run) {
+              if (fin && !done) { done = true; term = (uint32_t)j; }
+              run = left;
+              if (run == 0ull) { m = 0ull; c = 4; }
+            }
           }
         }
       }
@@ -354,8 +378,9 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
         // below A_MIN) keep their state and contribute zero terms. The wave
         // leaves as soon as no lane is inside the cutoff (before the exp) or
         // no lane takes the splat.
-        const float4 a = sA[j];
-        const float2 bb = *(const float2*)&sB[j];   // C, opa*comp
+        const uint32_t jo = lds_offset((uint32_t)j * 16u);
+        const float4 a = lds_at<float4>(sA, jo);
+        const float2 bb = lds_at<float2>(sB, jo);   // C, opa*comp
         // keep both in this one 8-byte read (otherwise the compiler sinks
         // the second word into the next block with its own address move)
         asm volatile("" ::"v"(bb.y));
@@ -377,8 +402,8 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
         float v[10];
         uint32_t gid;
         {
-          const float4 c4 = sC[j];
-          const float4 d2 = sD[j];          // -0.5 cnorm, 1 - opa, id
+          const float4 c4 = lds_at<float4>(sC, jo);
+          const float4 d2 = lds_at<float4>(sD, jo);   // -0.5 cnorm, 1 - opa, id
           asm volatile("" ::"v"(d2.z));
           gid = __float_as_uint(d2.z);
           // Tb = Ta / (1 - alpha), S / (1 - alpha) through one reciprocal.
