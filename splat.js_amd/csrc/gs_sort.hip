@@ -560,7 +560,14 @@ __device__ __forceinline__ uint32_t block_excl_scan_t(uint32_t v, uint32_t& tota
 // each Gaussian writes its index over its key slots in an owner table, or,
 // for a round of more than kOwnerCap keys, each key finds its Gaussian by a
 // binary search of the offsets), and f(g, r0, tile, tx, ty) runs once per
-// key, g the round-local index. `prep` stages per-Gaussian data.
+// key, g the round-local index. `prep` stages per-Gaussian data: prep.load
+// issues its global reads, prep.store puts them in LDS.
+//
+// The loads are pipelined one round ahead (a block holds only 16 waves per
+// CU, too few to hide a load's latency otherwise): the next round's counts
+// and rectangles are issued at the start of a round, its per-Gaussian data
+// before this round's keys, and the first round's before `init` (the
+// block's per-tile setup).
 constexpr uint32_t kOwnerCap = 8192;
 
 struct SegRound {
@@ -570,23 +577,30 @@ struct SegRound {
   uint16_t* owner;   // kOwnerCap
 };
 
-template <typename Prep, typename F>
+template <typename Prep, typename Init, typename F>
 __device__ __forceinline__ void seg_rounds(int64_t lo, int64_t hi, const uint32_t* __restrict__ cnt,
                                            const uint2* __restrict__ rect, int tilesX, const SegRound& sr,
-                                           Prep prep, F f) {
+                                           const Prep& prep, Init init, F f) {
   const int tid = threadIdx.x;
+  int64_t i = lo + tid;
+  uint32_t c = i < hi ? cnt[i] : 0u;
+  uint2 rc = i < hi ? rect[i] : make_uint2(0, 0);   // read whatever c is: no dependent load
+  typename Prep::State ps = prep.load(i, c);
+  init();
   for (int64_t r0 = lo; r0 < hi; r0 += kSegThreads) {
-    const int64_t i = r0 + tid;
-    const uint32_t c = i < hi ? cnt[i] : 0u;
+    const int64_t in = i + kSegThreads;
+    const uint32_t cn = in < hi ? cnt[in] : 0u;
+    const uint2 rn = in < hi ? rect[in] : make_uint2(0, 0);
     uint32_t tot;
     const uint32_t ex = block_excl_scan_t<kSegThreads>(c, tot, sr.wsum);
     sr.sOff[tid] = ex;
-    sr.sRect[tid] = c ? rect[i] : make_uint2(0, 0);
-    prep(tid, i, c);
+    sr.sRect[tid] = c ? rc : make_uint2(0, 0);
+    prep.store(tid, c, ps);
     const bool owned = tot <= kOwnerCap;
     if (owned)
       for (uint32_t k = 0; k < c; k++) sr.owner[ex + k] = (uint16_t)tid;
     __syncthreads();
+    ps = prep.load(in, cn);
     for (uint32_t q = tid; q < tot; q += kSegThreads) {
       int g = 0;
       if (owned) {
@@ -596,8 +610,8 @@ __device__ __forceinline__ void seg_rounds(int64_t lo, int64_t hi, const uint32_
         for (int step = kSegThreads / 2; step > 0; step >>= 1)
           if (sr.sOff[g + step] <= q) g += step;
       }
-      const uint2 rc = sr.sRect[g];
-      const uint32_t tx0 = rc.x & 0xffffu, tx1 = rc.x >> 16, ty0 = rc.y & 0xffffu;
+      const uint2 rq = sr.sRect[g];
+      const uint32_t tx0 = rq.x & 0xffffu, tx1 = rq.x >> 16, ty0 = rq.y & 0xffffu;
       const uint32_t wdt = tx1 - tx0 + 1u;
       const uint32_t li = q - sr.sOff[g];
       const uint32_t dy = li / wdt;
@@ -605,8 +619,20 @@ __device__ __forceinline__ void seg_rounds(int64_t lo, int64_t hi, const uint32_
       f(g, r0, ty * (uint32_t)tilesX + tx, tx, ty);
     }
     __syncthreads();
+    i = in;
+    c = cn;
+    rc = rn;
   }
 }
+
+// count pass: no per-Gaussian data beyond the count; tallies the Gaussians
+// with keys
+struct CountPrep {
+  uint32_t* vis;
+  struct State {};
+  __device__ State load(int64_t, uint32_t) const { return {}; }
+  __device__ void store(int, uint32_t c, const State&) const { *vis += c != 0u; }
+};
 
 // Row b of the [block][tile] table: the keys of Gaussians [b G, (b+1) G) per
 // tile; visPart[b]: how many of them have keys.
@@ -620,12 +646,12 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_count(int64_t n, int64_t G,
   __shared__ uint32_t wsum[kSegThreads / 64];
   __shared__ uint16_t owner[kOwnerCap];
   const int tid = threadIdx.x;
-  for (int t = tid; t < numTiles; t += kSegThreads) hist[t] = 0u;
   const int64_t lo = (int64_t)blockIdx.x * G;
   const int64_t hi = lo + G < n ? lo + G : n;
   uint32_t vis = 0;
   const SegRound sr{sOff, sRect, wsum, owner};
-  seg_rounds(lo, hi, cnt, rect, tilesX, sr, [&](int, int64_t, uint32_t c) { vis += c != 0u; },
+  seg_rounds(lo, hi, cnt, rect, tilesX, sr, CountPrep{&vis},
+             [&]() { for (int t = tid; t < numTiles; t += kSegThreads) hist[t] = 0u; },
              [&](int, int64_t, uint32_t tile, uint32_t, uint32_t) { atomicAdd(&hist[tile], 1u); });
   uint32_t tot;
   block_excl_scan_t<kSegThreads>(vis, tot, wsum);
@@ -808,6 +834,31 @@ __global__ __launch_bounds__(kScanThreads) void k_seg_scan(const uint32_t* __res
   }
 }
 
+// scatter pass: depth key and cover-test constants of each Gaussian with keys
+struct ScatterPrep {
+  const uint32_t* __restrict__ depthKey;
+  const float4* __restrict__ rec;
+  uint32_t* sDk;
+  gs_cover_splat* sG;
+  struct State {
+    uint32_t dk;
+    float4 g0, g1;
+  };
+  __device__ State load(int64_t i, uint32_t c) const {
+    State s{0u, make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
+    if (c != 0u) {
+      s.dk = depthKey[i];
+      s.g0 = rec[4 * i + 0];
+      s.g1 = rec[4 * i + 1];
+    }
+    return s;
+  }
+  __device__ void store(int k, uint32_t c, const State& s) const {
+    sDk[k] = s.dk;
+    if (c != 0u) sG[k] = gs_cover_prepare(s.g0.x, s.g0.y, s.g0.z, s.g0.w, s.g1.x, s.g1.z);
+  }
+};
+
 // Every (Gaussian, kept tile) key into its tile's segment: each tile's LDS
 // counter starts at the tile's segment start plus this block's offset in
 // the tile (the column scan), so a key's slot is one LDS atomic; the tile's
@@ -829,21 +880,16 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(int64_t n, int64_t 
   __shared__ uint16_t owner[kOwnerCap];
   const int tid = threadIdx.x;
   const uint32_t* row = table + (int64_t)blockIdx.x * numTiles;
-  for (int t = tid; t < numTiles; t += kSegThreads) {
-    const uint2 rg = ranges[t];
-    cur[t] = rg.x + row[t];
-    dropped[t] = rg.y == rg.x;
-  }
   const int64_t lo = (int64_t)blockIdx.x * G;
   const int64_t hi = lo + G < n ? lo + G : n;
   const SegRound sr{sOff, sRect, wsum, owner};
-  seg_rounds(lo, hi, cnt, rect, tilesX, sr,
-             [&](int k, int64_t i, uint32_t c) {
-               sDk[k] = c ? depthKey[i] : 0u;
-               if (c != 0u) {
-                 const float4 g0 = rec[4 * i + 0];
-                 const float4 g1 = rec[4 * i + 1];
-                 sG[k] = gs_cover_prepare(g0.x, g0.y, g0.z, g0.w, g1.x, g1.z);
+  seg_rounds(lo, hi, cnt, rect, tilesX, sr, ScatterPrep{depthKey, rec, sDk, sG},
+             [&]() {
+#pragma unroll 4
+               for (int t = tid; t < numTiles; t += kSegThreads) {
+                 const uint2 rg = ranges[t];
+                 cur[t] = rg.x + row[t];
+                 dropped[t] = rg.y == rg.x;
                }
              },
              [&](int g, int64_t r0, uint32_t tile, uint32_t tx, uint32_t ty) {
