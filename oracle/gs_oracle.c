@@ -19,6 +19,7 @@
 #define REAL float
 #define SUFFIX _f32
 #define EXP gs_expf
+#define EXP_RASTER gs_expf_raster
 #define LOG gs_logf
 #define SQRT sqrtf
 #define FMA fmaf
@@ -29,6 +30,7 @@
 #undef REAL
 #undef SUFFIX
 #undef EXP
+#undef EXP_RASTER
 #undef LOG
 #undef SQRT
 #undef FMA
@@ -40,6 +42,7 @@
 #define REAL double
 #define SUFFIX _f64
 #define EXP exp
+#define EXP_RASTER exp
 #define LOG log
 #define SQRT sqrt
 #define FMA fma
@@ -49,6 +52,7 @@
 #include "gs_oracle_body.h"
 
 float orc_expf(float x) { return gs_expf(x); }
+float orc_expf_raster(float x) { return gs_expf_raster(x); }
 float orc_logf(float x) { return gs_logf(x); }
 
 void orc_sh_basis_f64(int deg, const double v[3], double* Y, double* dY) {

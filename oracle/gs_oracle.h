@@ -59,6 +59,7 @@ void orc_cam_pos(orc_cam* cam);
 
 /* portable exp/log (gs_math.h), exported so tests can pin them against libm */
 float orc_expf(float x);
+float orc_expf_raster(float x);
 float orc_logf(float x);
 
 /* R3/R4: projection. rect = n x 4 int32 (tx0, tx1, ty0, ty1), cnt = tiles

@@ -286,7 +286,7 @@ void FN(orc_render_fwd)(const orc_opts* o, const orc_cam* cam, const REAL* proj,
         const REAL t2 = FMA(pr[5] * dy, dy, (pr[3] * dx) * dx);
         const REAL e = FMAX(FMA(RL(0.5), t2, (pr[4] * dx) * dy), RL(0.0));
         if (e > E_CUT) continue;
-        const REAL araw = pr[7] * pr[6] * EXP(-e);
+        const REAL araw = pr[7] * pr[6] * EXP_RASTER(-e);
         const REAL alpha = FMIN(RL(0.99), araw);
         if (alpha < A_MIN) continue;
         const REAL w = T * alpha;
@@ -367,7 +367,7 @@ void FN(orc_render_bwd)(const orc_opts* o, const orc_cam* cam, const REAL* proj,
         const REAL e = FMAX(FMA(RL(0.5), t2, (cB * dx) * dy), RL(0.0));
         if (!(e <= E_CUT)) continue;
         const REAL comp = pr[6], opa = pr[7];
-        const REAL G = EXP(-e);
+        const REAL G = EXP_RASTER(-e);
         const REAL araw = opa * comp * G;
         const REAL alpha = FMIN(RL(0.99), araw);
         if (!(alpha >= A_MIN)) continue;

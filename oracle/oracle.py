@@ -54,6 +54,7 @@ def lib():
         for name, args, res in [
             ("orc_cam_pos", [P], None),
             ("orc_expf", [C.c_float], C.c_float),
+            ("orc_expf_raster", [C.c_float], C.c_float),
             ("orc_logf", [C.c_float], C.c_float),
             ("orc_count_keys", [C.c_int64, P], C.c_int64),
             ("orc_sh_basis_f64", [C.c_int, P, P, P], None),

@@ -126,8 +126,9 @@ GS_HD uint32_t gs_cover_tile_mask(const gs_cover_splat* s, int tx, int ty) {
 // Cover level stored with each splat: an upper bound on the largest f32
 // exponent a pixel can take the splat at. eMax = min(eCut, ln(opa*comp /
 // A_MIN)) is computed in f32 (division, gs_logf: a few ulp), and alpha =
-// min(0.99, opa*comp * gs_expf(-e)) >= A_MIN needs e <= ln(opa*comp/A_MIN)
-// up to the ~1 ulp of gs_expf; the relative and absolute 1e-5 cover both.
+// min(0.99, opa*comp * gs_expf_raster(-e)) >= A_MIN needs e <=
+// ln(opa*comp/A_MIN) up to the 2 ulp of gs_expf_raster; the relative and
+// absolute 1e-5 cover both.
 GS_HD float gs_cover_level(float eMax) { return eMax * 1.00001f + 1.0e-5f; }
 
 #endif  // GS_COVER_H

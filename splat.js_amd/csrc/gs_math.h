@@ -76,29 +76,28 @@ GS_HD float gs_expf(float x) {
   return p * gs_u2f((uint32_t)(k + 127) << 23);
 }
 
-// gs_expf without branches for the rasterizer's inner loops. For x in
-// [-87, 0] the operations are exactly those of gs_expf (k = round(x/ln2) lies
-// in [-126, 0], the normal-result path), so the bits agree.
-// gs_expf_nonpos_in takes x already in [-87, 0] (the caller clamps);
-// gs_expf_nonpos clamps any input (including NaN) into that range first, which
-// gives a finite, positive value that callers discard.
-GS_HD float gs_expf_nonpos_in(float xc) {
+// e^x for the rasterizer's inner loops, x in [-87, 0] (the caller clamps;
+// a pixel takes a splat only at x >= -eCut >= -80). Branch-free: the range
+// reduction of gs_expf (k = round(x/ln2) in [-126, 0], a normal result),
+// then a degree-5 polynomial for e^r on [-ln2/2, ln2/2] with the constant and
+// linear coefficients fixed at 1 and the other four fitted for minimal
+// relative error (Lawson's iteratively reweighted least squares on a dense
+// grid, rounded to f32). Two fewer FMAs than gs_expf's degree-7 Taylor
+// polynomial; at most 2 ulp from the correctly rounded e^x on every float of
+// [-87, 0] (gs_expf: at most 1), inside WGSL's 3 + 2|x| ulp for exp.
+GS_HD float gs_expf_raster(float xc) {
   const float kf = rintf(xc * 1.44269502162933349609375f);
   float r = fmaf(kf, -0.693145751953125f, xc);
   r = fmaf(kf, -1.428606765330187045e-06f, r);
-  float p = 1.984126984126984e-04f;
-  p = fmaf(p, r, 1.388888888888889e-03f);
-  p = fmaf(p, r, 8.333333333333333e-03f);
-  p = fmaf(p, r, 4.166666666666667e-02f);
-  p = fmaf(p, r, 1.666666666666667e-01f);
-  p = fmaf(p, r, 0.5f);
+  float p = gs_u2f(0x3c0840d6u);             // 8.316239e-03
+  p = fmaf(p, r, gs_u2f(0x3d2b9b09u));       // 4.1895900e-02
+  p = fmaf(p, r, gs_u2f(0x3e2aabc8u));       // 1.6667092e-01
+  p = fmaf(p, r, gs_u2f(0x3efffeeeu));       // 4.9999183e-01
   p = fmaf(p, r, 1.0f);
   p = fmaf(p, r, 1.0f);
   const int k = (int)kf;
   return p * gs_u2f((uint32_t)(k + 127) << 23);
 }
-
-GS_HD float gs_expf_nonpos(float x) { return gs_expf_nonpos_in(fminf(fmaxf(x, -87.0f), 0.0f)); }
 
 // natural log; NaN for x < 0, -inf at 0, +inf at +inf.
 GS_HD float gs_logf(float x) {

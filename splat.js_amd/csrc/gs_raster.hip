@@ -148,7 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
           const float ee = fminf(fmaxf(fmaf(0.5f, t2, (a.w * dx) * dy), 0.0f), 87.0f);
           const bool live = !done && ee <= cut.eCut;
           if ((run & wave_ballot(ee <= cut.eCut)) != 0ull) {
-            const float araw = bb.y * gs_expf_nonpos_in(-ee);   // == gs_expf(-ee) for ee <= eCut <= 80
+            const float araw = bb.y * gs_expf_raster(-ee);
             const float alpha = fminf(0.99f, araw);
             if (live && alpha >= cut.aMin) {
               const float4 c4 = lds_at<float4>(sC, jo);
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
         const bool live = active && k < myEnd && e <= cut.eCut;
         const uint64_t liveMask = activeMask & wave_ballot(k < myEnd) & wave_ballot(e <= cut.eCut);
         if (liveMask == 0ull) continue;
-        const float G = gs_expf_nonpos_in(-e);   // == gs_expf(-e) for e <= eCut <= 80
+        const float G = gs_expf_raster(-e);
         const float araw = bb.y * G;
         const float alpha = fminf(0.99f, araw);
         const bool contrib = live && alpha >= cut.aMin;

@@ -99,35 +99,46 @@ def test_create_rejects_ecut_outside_supported_range():
 
 MATH_PROBE = r"""
 #include <stdio.h>
+#include <stdlib.h>
+#include <math.h>
 #include "gs_math.h"
 int main(void) {
-  /* every float in [-87, 0] (sign bit set, magnitude 0 .. 87) */
+  /* every float in [-87, 0] (sign bit set, magnitude 0 .. 87): ulp distance
+     of gs_expf and gs_expf_raster from the correctly rounded e^x (libm's
+     double exp rounded once to float; e^-87 is still a normal float) */
   const uint32_t top = gs_f2u(87.0f);
-  unsigned long long bad = 0, n = 0;
-#pragma omp parallel for reduction(+ : bad, n) schedule(static)
-  for (long long mm = 0; mm <= (long long)top; mm++) {
-    const float x = gs_u2f(0x80000000u | (uint32_t)mm);
-    n++;
-    if (gs_f2u(gs_expf_nonpos_in(x)) != gs_f2u(gs_expf(x))) bad++;
-    if (gs_f2u(gs_expf_nonpos(x)) != gs_f2u(gs_expf(x))) bad++;
+  unsigned long long n = 0, hp[4] = {0, 0, 0, 0}, hr[4] = {0, 0, 0, 0};
+#pragma omp parallel
+  {
+    unsigned long long cn = 0, cp[4] = {0, 0, 0, 0}, cr[4] = {0, 0, 0, 0};
+#pragma omp for schedule(static)
+    for (long long mm = 0; mm <= (long long)top; mm++) {
+      const float x = gs_u2f(0x80000000u | (uint32_t)mm);
+      const long long t = (long long)gs_f2u((float)exp((double)x));
+      const long long dp = llabs((long long)gs_f2u(gs_expf(x)) - t);
+      const long long dr = llabs((long long)gs_f2u(gs_expf_raster(x)) - t);
+      cn++;
+      cp[dp > 3 ? 3 : dp]++;
+      cr[dr > 3 ? 3 : dr]++;
+    }
+#pragma omp critical
+    {
+      n += cn;
+      for (int i = 0; i < 4; i++) { hp[i] += cp[i]; hr[i] += cr[i]; }
+    }
   }
-  /* outside the range: finite and positive */
-  const float odd[] = {-87.5f, -1e30f, gs_u2f(0xff800000u), gs_u2f(0x7fc00000u), 1.0f, 1e30f};
-  int oddbad = 0;
-  for (int i = 0; i < 6; i++) {
-    const float r = gs_expf_nonpos(odd[i]);
-    if (!(r > 0.0f && r <= 1.0f)) oddbad++;
-  }
-  printf("%llu %llu %d\n", n, bad, oddbad);
+  printf("%llu %llu %llu %llu %llu %llu %llu %llu %llu\n", n, hp[0], hp[1], hp[2], hp[3], hr[0], hr[1],
+         hr[2], hr[3]);
   return 0;
 }
 """
 
 
-def test_branch_free_exp_matches_portable_exp_exhaustively(tmp_path):
-    """gs_expf_nonpos_in (raster inner loops, argument clamped to [-87, 0] by
-    the caller) and gs_expf_nonpos == gs_expf bit for bit on every float of
-    [-87, 0]; the rasterizer takes a splat only at exponents <= eCut <= 80."""
+def test_exps_against_correctly_rounded_exp_exhaustively(tmp_path):
+    """On every float of [-87, 0]: gs_expf (projection, sigmoid) is within 1
+    ulp of the correctly rounded e^x, and gs_expf_raster (the rasterizers'
+    alpha, argument clamped to [-87, 0] by the caller) within 2 ulp, with the
+    2-ulp cases rare; WGSL allows exp 3 + 2|x| ulp."""
     src = tmp_path / "m.c"
     src.write_text(MATH_PROBE)
     exe = tmp_path / "m"
@@ -137,5 +148,7 @@ def test_branch_free_exp_matches_portable_exp_exhaustively(tmp_path):
         flags += ["-mfma", "-msse4.1"]   # hardware fmaf/rintf: same IEEE results, 20x faster than libm
     subprocess.check_call(["gcc", *flags, "-I", os.path.join(ROOT, "splat.js_amd", "csrc"),
                            "-o", str(exe), str(src), "-lm"])
-    n, bad, oddbad = map(int, subprocess.check_output([str(exe)], text=True).split())
-    assert n == 0x42ae0000 + 1 and bad == 0 and oddbad == 0
+    n, p0, p1, p2, p3, r0, r1, r2, r3 = map(int, subprocess.check_output([str(exe)], text=True).split())
+    assert n == 0x42ae0000 + 1
+    assert p2 == 0 and p3 == 0 and p1 < n // 100
+    assert r3 == 0 and r2 < n // 50 and r0 > n * 9 // 10
