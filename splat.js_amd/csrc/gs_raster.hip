@@ -81,6 +81,12 @@ __device__ __forceinline__ const T& lds_at(const void* base, uint32_t off) {
   return *(const T*)((const char*)base + off);
 }
 
+// The exponent clamped to [0, 87] in one v_med3_f32 (fminf(fmaxf()) takes
+// two): max(e, 0) is the reference's (shaders.js:565); the upper clamp only
+// keeps the exp's argument in range, since e > eCut (<= 80) is never taken.
+// e is finite (bounded conic and offsets), where med3 equals the pair.
+__device__ __forceinline__ float clamp_e(float e) { return __builtin_amdgcn_fmed3f(e, 0.0f, 87.0f); }
+
 // 1/x for x in [2^-9, 1]: hardware reciprocal estimate plus one Newton step
 // with fused residual. Equal to IEEE 1.0f/x for all 75,497,473 floats of that
 // range (exhaustive check on gfx950); callers guarantee the range.
@@ -145,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
           const float t2 = fmaf(bb.x * dy, dy, (a.z * dx) * dx);
           // clamped to [0, 87]: an exponent above 87 > eCut is never taken,
           // and the exp below needs its argument in [-87, 0]
-          const float ee = fminf(fmaxf(fmaf(0.5f, t2, (a.w * dx) * dy), 0.0f), 87.0f);
+          const float ee = clamp_e(fmaf(0.5f, t2, (a.w * dx) * dy));
           const bool live = !done && ee <= cut.eCut;
           if ((run & wave_ballot(ee <= cut.eCut)) != 0ull) {
             const float araw = bb.y * gs_expf_raster(-ee);
@@ -388,7 +394,7 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
         const float cA = a.z, cB = a.w, cC = bb.x;
         const float t2 = fmaf(cC * dy, dy, (cA * dx) * dx);
         // clamped to [0, 87] as in the forward pass
-        const float e = fminf(fmaxf(fmaf(0.5f, t2, (cB * dx) * dy), 0.0f), 87.0f);
+        const float e = clamp_e(fmaf(0.5f, t2, (cB * dx) * dy));
         // votes of single compares, combined in scalar registers (a vote
         // of a combined condition costs two VALU instructions)
         const bool live = active && k < myEnd && e <= cut.eCut;
