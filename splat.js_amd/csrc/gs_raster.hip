@@ -344,15 +344,16 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
   __syncthreads();
   const uint32_t endMax = sEnd;
   const float gC[3] = {g.x, g.y, g.z};
-  // GS_ACCUM_FIXED: the x16384 fixed-point scale rides on the two factors
-  // every term carries (galpha and gC). Scaling by a power of two is exact,
-  // so each scaled term equals the term times 16384 bit for bit; the only
-  // exceptions are terms whose unscaled value is subnormal, and those
-  // truncate to zero either way.
+  // GS_ACCUM_FIXED: the x16384 fixed-point scale rides on dL/dC from the
+  // start, so on every product with it (dot(gC, c), dot(gC, S), galpha) and
+  // on every term, which carries galpha or gC. Scaling by a power of two is
+  // exact, so each scaled value equals the value times 16384 bit for bit;
+  // the only exceptions are values whose unscaled form is subnormal, and
+  // terms that small truncate to zero either way.
   constexpr float kS = kAccum == 0 ? kFixed : 1.0f;
   const float gCs[3] = {gC[0] * kS, gC[1] * kS, gC[2] * kS};
-  // dot(gC, S) of the colour-behind sum S, which enters only through it
-  float GS = fmaf(gC[2], cam.bg[2], fmaf(gC[1], cam.bg[1], gC[0] * cam.bg[0])) * Ta;
+  // 16384 dot(gC, S) of the colour-behind sum S, which enters only through it
+  float GS = fmaf(gCs[2], cam.bg[2], fmaf(gCs[1], cam.bg[1], gCs[0] * cam.bg[0])) * Ta;
   for (uint32_t hi = endMax; hi > s; ) {
     const uint32_t lo = hi - s > (uint32_t)kBlock ? hi - kBlock : s;
     const int n = (int)(hi - lo);
@@ -418,13 +419,12 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
           // rounded 1/x for every float (checked exhaustively on gfx950).
           const float rom = rcp_exact_unit(1.0f - alpha);
           const float Tb = Ta * rom;
-          const float gcol = fmaf(gC[2], c4.z, fmaf(gC[1], c4.y, gC[0] * c4.x));
+          const float gcol = fmaf(gCs[2], c4.z, fmaf(gCs[1], c4.y, gCs[0] * c4.x));   // scaled
           // Every term is a product with galpha or aT; zeroing those two for
           // the lanes that do not take the splat zeroes all ten terms (all
           // other factors are finite), and leaves the taking lanes' bits as is.
-          const float galpha = (contrib && !(araw > 0.99f)) ? fmaf(gcol, Tb, -(GS * rom)) : 0.0f;
+          const float galphaS = (contrib && !(araw > 0.99f)) ? fmaf(gcol, Tb, -(GS * rom)) : 0.0f;
           const float aT = contrib ? alpha * Tb : 0.0f;
-          const float galphaS = galpha * kS;
           const float ga = galphaS * araw;
           const float gq = ga * d2.x;
           const float gqdx = gq * dx;
