@@ -103,7 +103,7 @@ __device__ __forceinline__ void compute_geom(const Cam& cam, const float* P, Geo
   g.r2[1] = 2.0f * (qy * qz + qw * qx);
   g.r2[2] = 1.0f - 2.0f * (qx * qx + qy * qy);
 
-  for (int k = 0; k < 3; k++) g.s[k] = gs_expf(clampf(P[3 + k], -12.0f, 6.0f));
+  for (int k = 0; k < 3; k++) g.s[k] = gs_expf_mid(clampf(P[3 + k], -12.0f, 6.0f));
 
   float m0[3], m1[3], m2[3];
   for (int k = 0; k < 3; k++) {

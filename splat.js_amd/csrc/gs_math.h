@@ -76,6 +76,26 @@ GS_HD float gs_expf(float x) {
   return p * gs_u2f((uint32_t)(k + 127) << 23);
 }
 
+// gs_expf for x in [-87, 87] without its range branches: there k =
+// round(x/ln2) lies in [-126, 126] and gs_expf takes its normal-result path,
+// whose operations these are, so the bits agree. For the clamped arguments
+// of the scales (exp of [-12, 6]) and the sigmoids (exp of [-9, 9]).
+GS_HD float gs_expf_mid(float x) {
+  const float kf = rintf(x * 1.44269502162933349609375f);
+  float r = fmaf(kf, -0.693145751953125f, x);
+  r = fmaf(kf, -1.428606765330187045e-06f, r);
+  float p = 1.984126984126984e-04f;
+  p = fmaf(p, r, 1.388888888888889e-03f);
+  p = fmaf(p, r, 8.333333333333333e-03f);
+  p = fmaf(p, r, 4.166666666666667e-02f);
+  p = fmaf(p, r, 1.666666666666667e-01f);
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.0f);
+  p = fmaf(p, r, 1.0f);
+  const int k = (int)kf;
+  return p * gs_u2f((uint32_t)(k + 127) << 23);
+}
+
 // e^x for the rasterizer's inner loops, x in [-87, 0] (the caller clamps;
 // a pixel takes a splat only at x >= -eCut >= -80). Branch-free: the range
 // reduction of gs_expf (k = round(x/ln2) in [-126, 0], a normal result),
@@ -188,7 +208,7 @@ GS_HD uint32_t gs_f2bf_sr(float x, uint32_t rnd) {
 // 1 / (1 + e^-clamp(x, -9, 9)): the activation for colour and opacity logits.
 GS_HD float gs_sigmoid_clamped(float x) {
   const float c = fminf(fmaxf(x, -9.0f), 9.0f);
-  return 1.0f / (1.0f + gs_expf(-c));
+  return 1.0f / (1.0f + gs_expf_mid(-c));
 }
 
 #endif  // GS_MATH_H

@@ -127,8 +127,16 @@ int main(void) {
       for (int i = 0; i < 4; i++) { hp[i] += cp[i]; hr[i] += cr[i]; }
     }
   }
-  printf("%llu %llu %llu %llu %llu %llu %llu %llu %llu\n", n, hp[0], hp[1], hp[2], hp[3], hr[0], hr[1],
-         hr[2], hr[3]);
+  /* gs_expf_mid == gs_expf bit for bit on every float of [-87, 87] */
+  unsigned long long midbad = 0;
+#pragma omp parallel for reduction(+ : midbad) schedule(static)
+  for (long long mm = 0; mm <= (long long)top; mm++) {
+    const float xn = gs_u2f(0x80000000u | (uint32_t)mm), xp = gs_u2f((uint32_t)mm);
+    midbad += gs_f2u(gs_expf_mid(xn)) != gs_f2u(gs_expf(xn));
+    midbad += gs_f2u(gs_expf_mid(xp)) != gs_f2u(gs_expf(xp));
+  }
+  printf("%llu %llu %llu %llu %llu %llu %llu %llu %llu %llu\n", n, hp[0], hp[1], hp[2], hp[3], hr[0], hr[1],
+         hr[2], hr[3], midbad);
   return 0;
 }
 """
@@ -138,7 +146,9 @@ def test_exps_against_correctly_rounded_exp_exhaustively(tmp_path):
     """On every float of [-87, 0]: gs_expf (projection, sigmoid) is within 1
     ulp of the correctly rounded e^x, and gs_expf_raster (the rasterizers'
     alpha, argument clamped to [-87, 0] by the caller) within 2 ulp, with the
-    2-ulp cases rare; WGSL allows exp 3 + 2|x| ulp."""
+    2-ulp cases rare; WGSL allows exp 3 + 2|x| ulp. gs_expf_mid (scales and
+    sigmoids, branch-free) equals gs_expf bit for bit on every float of
+    [-87, 87]."""
     src = tmp_path / "m.c"
     src.write_text(MATH_PROBE)
     exe = tmp_path / "m"
@@ -148,7 +158,7 @@ def test_exps_against_correctly_rounded_exp_exhaustively(tmp_path):
         flags += ["-mfma", "-msse4.1"]   # hardware fmaf/rintf: same IEEE results, 20x faster than libm
     subprocess.check_call(["gcc", *flags, "-I", os.path.join(ROOT, "splat.js_amd", "csrc"),
                            "-o", str(exe), str(src), "-lm"])
-    n, p0, p1, p2, p3, r0, r1, r2, r3 = map(int, subprocess.check_output([str(exe)], text=True).split())
-    assert n == 0x42ae0000 + 1
+    n, p0, p1, p2, p3, r0, r1, r2, r3, midbad = map(int, subprocess.check_output([str(exe)], text=True).split())
+    assert n == 0x42ae0000 + 1 and midbad == 0
     assert p2 == 0 and p3 == 0 and p1 < n // 100
     assert r3 == 0 and r2 < n // 50 and r0 > n * 9 // 10
