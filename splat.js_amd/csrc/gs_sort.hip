@@ -672,19 +672,42 @@ __global__ __launch_bounds__(kBlock) void k_seg_colscan(uint32_t* __restrict__ t
   const int t = blockIdx.x * 16 + tl;
   const int per = (nb + 15) / 16;
   const int b0 = q * per, b1 = b0 + per < nb ? b0 + per : nb;
+  // up to kRegRows rows per thread (tables of up to 256 rows: 1M Gaussians)
+  // stay in registers: one batch of independent loads, no second read
+  constexpr int kRegRows = 16;
+  uint32_t v[kRegRows];
   uint32_t sum = 0;
-  if (t < numTiles)
+  if (per <= kRegRows) {
+#pragma unroll
+    for (int k = 0; k < kRegRows; k++) {
+      const int b = b0 + k;
+      v[k] = (t < numTiles && b < b1) ? table[(int64_t)b * numTiles + t] : 0u;
+      sum += v[k];
+    }
+  } else if (t < numTiles) {
     for (int b = b0; b < b1; b++) sum += table[(int64_t)b * numTiles + t];
+  }
   part[q][tl] = sum;
   __syncthreads();
   if (t >= numTiles) return;
   uint32_t run = 0;
   for (int k = 0; k < q; k++) run += part[k][tl];
-  for (int b = b0; b < b1; b++) {
-    uint32_t* p = table + (int64_t)b * numTiles + t;
-    const uint32_t v = *p;
-    *p = run;
-    run += v;
+  if (per <= kRegRows) {
+#pragma unroll
+    for (int k = 0; k < kRegRows; k++) {
+      const int b = b0 + k;
+      if (b < b1) {
+        table[(int64_t)b * numTiles + t] = run;
+        run += v[k];
+      }
+    }
+  } else {
+    for (int b = b0; b < b1; b++) {
+      uint32_t* p = table + (int64_t)b * numTiles + t;
+      const uint32_t x = *p;
+      *p = run;
+      run += x;
+    }
   }
   if (q == 15) tileCnt[t] = run;
 }
