@@ -17,7 +17,7 @@ and Adam moments, f32 arithmetic), and `config.fp32` and
 `config.psnr_delta_db` give the FP32 throughput and the measured PSNR
 difference over all views after the same steps.
 
-Prints one JSON line (the driver's contract) with the roofline of the
+Prints one JSON line (metric, value, timing, config) with the roofline of the
 dominant kernel (HIP events on the trainer's stream over the timed region)
 and a CPU baseline (the C restatement under oracle/, one thread, one full
 step of the same workload).
@@ -111,7 +111,7 @@ def main():
                     help="also run one plain 1 GiB device-to-device copy (scales HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (DESIGN.md)",
+        print(json.dumps({"error": "this trainer runs on one GPU; sharding across GPUs is a design only",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
