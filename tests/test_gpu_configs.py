@@ -18,7 +18,7 @@ import pytest
 import gsabi
 import scenes
 from helpers import OracleTrainer
-from test_gpu_parity import _check_step
+from test_gpu_parity import _check_step, _run_pair
 
 
 def _targets(gp, gsh, cams, deg, n, entries_cap=0):
@@ -117,3 +117,21 @@ def test_config5_stress_properties():
     assert np.isfinite(oa["data"]).all() and np.isfinite(oa["sh"]).all()
     a.destroy()
     b.destroy()
+
+
+@pytest.mark.gpu
+def test_config2_full_size_mixed_step_bit_exact():
+    """The benchmark workload itself: config 2 (1M Gaussians, 1600x900, SH
+    degree 2 with all bands active at iteration 2000, position lr 1.6e-4),
+    one mixed-precision training step on view 0. Every intermediate
+    (records, depth keys, rectangles, counts, the sorted keys and ranges,
+    image, T, end, dL/dC, the fixed-point gradient sums, the chained
+    gradients) and the updated params and SH match the restatement's mixed
+    mode bit for bit. The target is a GPU render of a second seeded 1M set."""
+    n = 1_000_000
+    p, sh, cams = scenes.config2(n=n, seed=0)
+    gp, gsh, _ = scenes.config2(n=n, seed=1)
+    cams = [dict(cams[0], imgIdx=0)]
+    images, W, H = _targets(gp, gsh, cams, 2, n)
+    _run_pair(p, sh, cams, images, W, H, 8.0, {"posLr": 1.6e-4}, 2, steps=1, start_iter=2000,
+              precision=gsabi.GS_PREC_MIXED)
