@@ -343,6 +343,11 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
   if (myEnd > s) atomicMax(&sEnd, myEnd);
   __syncthreads();
   const uint32_t endMax = sEnd;
+  // this wave's largest end: keys at or past it are past every lane's end,
+  // so the wave leaves them out of its visit set (they would add nothing)
+  uint32_t we = myEnd;
+  for (int o = 32; o > 0; o >>= 1) we = max(we, (uint32_t)__shfl_xor((int)we, o));
+  const uint32_t waveEnd = __builtin_amdgcn_readfirstlane(we);
   const float gC[3] = {g.x, g.y, g.z};
   // GS_ACCUM_FIXED: the x16384 fixed-point scale rides on dL/dC from the
   // start, so on every product with it (dot(gC, c), dot(gC, S), galpha) and
@@ -375,6 +380,9 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
     __syncthreads();
     for (int c = 3; c >= 0; c--) {
       uint64_t m = uniform64(sMask[c][wv]);
+      const uint32_t g0 = lo + 64u * (uint32_t)c;   // key of bit 0
+      if (waveEnd <= g0) m = 0ull;
+      else if (waveEnd - g0 < 64u) m &= (1ull << (waveEnd - g0)) - 1ull;
       while (m) {
         const int bit = 63 - (int)__builtin_clzll(m);
         m &= ~(1ull << bit);
