@@ -9,7 +9,8 @@
 // binning pass): a wave visits only the splats of its bit, a splat with an
 // empty mask is not even loaded, and a wave leaves a splat early once no
 // lane is inside the cutoff or takes it. The forward stops the whole tile once
-// every pixel has reached T < 1e-4.
+// every pixel has reached T < 1e-4. Both passes evaluate alpha with
+// gs_expf_raster (gs_math.h), as the restatement's raster passes do.
 //
 // Forward (shaders.js:556-585) writes the image, the final transmittance T
 // and `end` (one past the last key the pixel processed); in training it also
@@ -17,7 +18,8 @@
 // dL/dC, so the backward pass needs no second loss read.
 //
 // Backward (shaders.js:620-700) walks each tile from the largest `end` of
-// its pixels down to the first key, recomputing each pixel's transmittance
+// its pixels down to the first key (each wave from the largest `end` of its
+// own pixels), recomputing each pixel's transmittance
 // in front of a splat from the stored final T (Tb = Ta/(1-a), through one
 // reciprocal that also scales the colour-behind sum S).
 // Per key, the 10 screen-space gradient terms of the wave's 64 pixels are
@@ -27,7 +29,9 @@
 //                   i32(clamp(v * 16384, -1e9, 1e9)), exactly as the
 //                   reference's atomicAdd on atomic<i32> (shaders.js:529-541);
 //                   integer sums commute, so the result is deterministic and
-//                   bit-identical to any other summation order.
+//                   bit-identical to any other summation order. (Under the
+//                   Charbonnier loss the three colour terms cannot reach the
+//                   clamp and convert without it.)
 //   GS_ACCUM_F32:   plain f32 sums (atomic add order is not deterministic).
 #include "gs_kernels.h"
 
