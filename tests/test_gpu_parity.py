@@ -275,6 +275,46 @@ def test_dense_tiles_and_ragged_image(n, min_tile):
     assert np.diff(ts.astype(np.int64)).max() > min_tile
 
 
+def test_needle_border_and_near_plane_splats_bit_exact():
+    """Splats that stress the wave-skip test and the culling end to end:
+    needles (one axis e^-1, two axes e^-7 to e^-9, random rotations: the
+    ill-conditioned conics the cover test's rounding margin exists for),
+    splats straddling the image border and the view-frustum edge, and
+    splats just in front of and behind the near plane (z = 0.05). Records,
+    sort keys and ids, images, T, end, the fixed-point gradient sums,
+    gradients and updated params match the restatement bit for bit over two
+    steps, in both precisions; an 8x8 block the cover test skipped wrongly
+    would change the image and the sums."""
+    rng = np.random.default_rng(23)
+    n = 4000
+    p = np.zeros((n, 16), np.float32)
+    W, H, f = 256, 192, 220.0
+    z = rng.uniform(0.5, 4.0, n)
+    z[: n // 20] = rng.uniform(0.02, 0.12, n // 20)             # around the near plane
+    p[:, 2] = z
+    # image-plane positions over 1.4x the view, so many splats straddle the border
+    p[:, 0] = (rng.uniform(-0.7, 0.7, n) * W / f) * z
+    p[:, 1] = (rng.uniform(-0.7, 0.7, n) * H / f) * z
+    ls = np.empty((n, 3))
+    ls[:, 0] = rng.uniform(-1.5, -0.8, n)
+    ls[:, 1:] = rng.uniform(-9.0, -7.0, (n, 2))
+    for r in range(n):                                           # needle along a random axis
+        ls[r] = ls[r][rng.permutation(3)]
+    iso = rng.random(n) < 0.3                                     # some ordinary splats too
+    ls[iso] = rng.normal(-3.0, 0.5, (int(iso.sum()), 3))
+    p[:, 3:6] = ls
+    q = rng.normal(size=(n, 4))
+    p[:, 6:10] = q / np.linalg.norm(q, axis=1, keepdims=True)
+    p[:, 10:13] = rng.normal(0, 1, (n, 3))
+    p[:, 13] = rng.normal(1.0, 1.0, n)
+    cams = [{"imgIdx": 0, "R": [1, 0, 0, 0, 1, 0, 0, 0, 1], "t": [0, 0, 0], "f": f, "cx": W / 2, "cy": H / 2,
+             "w": W, "h": H}]
+    rgb = rng.random((H * W, 3)).astype(np.float32)
+    images = [{"tw": W, "th": H, "rgb": rgb}]
+    _run_pair(p, None, cams, images, W, H, 2.0, {}, 0, steps=2)
+    _run_pair(p, None, cams, images, W, H, 2.0, {}, 0, steps=2, precision=gsabi.GS_PREC_MIXED)
+
+
 def test_all_culled_and_single_gaussian():
     W, H = 48, 32
     cams = [{"imgIdx": 0, "R": [1, 0, 0, 0, 1, 0, 0, 0, 1], "t": [0, 0, 0], "f": 40, "cx": 24, "cy": 16,
