@@ -47,6 +47,7 @@ struct PendingEv {
   int slot;
   hipEvent_t a, b;
   double bytes;
+  bool ownsA;   // false: a is the previous stage's end event (returned with it)
 };
 
 uint32_t mulberry32(uint32_t& a) {
@@ -133,6 +134,7 @@ struct gs_trainer {
   bool prof = false;
   std::vector<ProfSlot> slots;
   std::vector<PendingEv> pending;
+  hipEvent_t profLast = nullptr;   // end event of the last profiled stage of this raster/step
   std::vector<hipEvent_t> evPool;
 };
 
@@ -194,22 +196,32 @@ int prof_slot(gs_trainer* h, const char* name) {
 }
 
 // Brackets one stage with events on the trainer's stream when profiling.
+// Consecutive stages of one step share their boundary: a stage starts at the
+// previous stage's end event (nothing is enqueued between them), so a step
+// records one event per stage plus one, not two per stage.
 struct Scope {
   gs_trainer* h;
   int slot = -1;
   hipEvent_t a = nullptr;
+  bool ownsA = true;
   double bytes;
   Scope(gs_trainer* h_, const char* name, double bytes_) : h(h_), bytes(bytes_) {
     if (!h->prof) return;
     slot = prof_slot(h, name);
-    a = ev_get(h);
-    (void)hipEventRecord(a, h->st);
+    if (h->profLast) {
+      a = h->profLast;
+      ownsA = false;
+    } else {
+      a = ev_get(h);
+      (void)hipEventRecord(a, h->st);
+    }
   }
   ~Scope() {
     if (slot < 0) return;
     hipEvent_t b = ev_get(h);
     (void)hipEventRecord(b, h->st);
-    h->pending.push_back(PendingEv{slot, a, b, bytes});
+    h->pending.push_back(PendingEv{slot, a, b, bytes, ownsA});
+    h->profLast = b;
   }
 };
 
@@ -222,10 +234,11 @@ void prof_collect(gs_trainer* h) {
     h->slots[p.slot].ms += ms;
     h->slots[p.slot].launches += 1;
     h->slots[p.slot].bytes += p.bytes;
-    h->evPool.push_back(p.a);
+    if (p.ownsA) h->evPool.push_back(p.a);
     h->evPool.push_back(p.b);
   }
   h->pending.clear();
+  h->profLast = nullptr;
 }
 
 Cam make_cam(const gs_trainer* h, const gs_camera& c, int trainMode, int64_t offset, int activeDeg) {
@@ -344,6 +357,7 @@ Cut make_cut(const gs_trainer* h) {
 // ---- one raster: project -> bin/sort -> forward (+ loss) ----
 int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
   const int64_t n = h->n;
+  h->profLast = nullptr;   // a step or render starts its own first stage
   {
     Scope s(h, "project", (double)n * (56.0 + 4.0 * h->shC + 48.0 + 16.0));
     launch_project(h->st, h->shDeg, h->mixed, n, h->cap, cam, make_cut(h), h->P, h->SH, h->rec, h->depthKey,
