@@ -173,12 +173,12 @@ GS_HD uint32_t gs_sr_seed(uint64_t gaussian, uint32_t step) {
 
 // Random bits of that seed for the pair of values k = 2p, 2p + 1 (k = 64 *
 // stream + slot): one multiply-xorshift round; value 2p takes the low 16
-// bits and 2p + 1 the high 16 (gs_sr_bits).
+// bits and 2p + 1 the high 16 (gs_sr_bits). Each step is a bijection of the
+// word, so for a uniform seed both halves are exactly uniform.
 GS_HD uint32_t gs_sr_pair(uint32_t seed, uint32_t p) {
   uint32_t x = seed ^ (p * 0x9E3779B9u);
-  x *= 0x85ebca6bu; x ^= x >> 13;
-  x *= 0xc2b2ae35u; x ^= x >> 16;
-  return x;
+  x *= 0x85ebca6bu;
+  return x ^ (x >> 16);
 }
 
 // 16 random bits for value k of that seed
