@@ -5,7 +5,7 @@ Workload (the benchmark's FP32 single-view configuration): 1M Gaussians
 logits N(0,1), DC U(0,1) as logits, SH degree 2 rest N(0,0.05)), a ring of
 64 views at radius 8 looking at the origin, 1600x900, f=1100. Targets are
 renders of a second seeded 1M-Gaussian set. Each step = projection, 16x16
-tile binning + radix sort, raster forward + Charbonnier loss, raster
+tile binning + per-tile depth sort, raster forward + Charbonnier loss, raster
 backward, chain + Adam (position lr 1.6e-4) on one view picked by the
 seeded camera RNG. All three SH bands are active (iteration counter starts
 at 2000, past the one-band-per-1000-iterations ramp).
@@ -18,9 +18,10 @@ and Adam moments, f32 arithmetic), and `config.fp32` and
 difference over all views after the same steps.
 
 Prints one JSON line (metric, value, timing, config) with the roofline of the
-dominant kernel (HIP events on the trainer's stream over the timed region)
-and a CPU baseline (the C restatement under oracle/, one thread, one full
-step of the same workload).
+dominant kernel (HIP events on the trainer's stream over the timed region),
+a measured HBM ceiling (a plain 1 GiB device-to-device copy) and a CPU
+baseline (the C restatement under oracle/, one thread, one full step of the
+same workload).
 """
 import argparse
 import json
@@ -141,6 +142,9 @@ def main():
     if os.path.exists(tf):
         traffic = json.load(open(tf)).get("bytes_per_launch")
 
+    copy_gbps = round(copy_ceiling(), 1)
+    log(f"device-to-device copy: {copy_gbps} GB/s (read + write)")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline (one f32 step of the C restatement, 1 thread)")
@@ -157,7 +161,8 @@ def main():
               "storage": storage,
               "keys_per_step": round(mean_keys), "visible_per_step": round(mean_vis),
               "sorted_keys_per_s": round(mean_keys * its), "kernels": kernels,
-              "train_mse": head["mse"], "psnr_db": round(head["psnr"], 4)}
+              "train_mse": head["mse"], "psnr_db": round(head["psnr"], 4),
+              "hbm_copy_ceiling_GBps": copy_gbps}
     if len(runs) == 2:
         r32 = runs["f32"]
         config["fp32"] = {"value": round(r32["its"], 2), "ms_per_step": round(1e3 * r32["dt"] / args.steps, 4),
@@ -213,6 +218,35 @@ def run_one(precision, p, sh, cams, images, W, H, args):
     t.destroy()
     return {"its": args.steps / dt, "dt": dt, "prof": prof, "keys": float(np.mean(keys)),
             "vis": float(np.mean(vis)), "mse": mse, "psnr": psnr}
+
+
+def copy_ceiling(nbytes=1 << 30, reps=5):
+    """Measured HBM ceiling: a plain device-to-device copy of nbytes (one
+    read and one write of every byte), timed with HIP events, best of reps.
+    Returns GB/s of read + write traffic."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    e0, e1 = C.c_void_p(), C.c_void_p()
+    assert hip.hipEventCreate(C.byref(e0)) == 0 and hip.hipEventCreate(C.byref(e1)) == 0
+    best = None
+    for _ in range(reps + 1):      # the first copy warms up
+        assert hip.hipEventRecord(e0, None) == 0
+        assert hip.hipMemcpyAsync(b, a, C.c_size_t(nbytes), 3, None) == 0  # hipMemcpyDeviceToDevice
+        assert hip.hipEventRecord(e1, None) == 0
+        assert hip.hipEventSynchronize(e1) == 0
+        ms = C.c_float()
+        assert hip.hipEventElapsedTime(C.byref(ms), e0, e1) == 0
+        best = ms.value if best is None else min(best, ms.value)
+    hip.hipEventDestroy(e0)
+    hip.hipEventDestroy(e1)
+    hip.hipFree(a)
+    hip.hipFree(b)
+    return 2.0 * nbytes / (best * 1e-3) / 1e9
 
 
 def copy_calibration(nbytes=1 << 30):
