@@ -1054,12 +1054,15 @@ __device__ void seg_merge(const uint64_t* A, uint32_t na, const uint64_t* B, uin
 
 // One block per tile: sorts the tile's segment and writes the raster values
 // (id | mask << 28) in order. Segments over kChunk keys sort kChunk-key runs
-// and merge them pairwise in global memory (keys <-> tmp).
-__global__ __launch_bounds__(kBlock) void k_seg_sort(const uint2* __restrict__ ranges, uint64_t* __restrict__ keys,
+// and merge them pairwise in global memory (keys <-> tmp). Blocks take the
+// tiles in the raster launch order (most keys first), so the few segments
+// that need global merges start early instead of trailing the launch.
+__global__ __launch_bounds__(kBlock) void k_seg_sort(const uint2* __restrict__ ranges,
+                                                     const uint32_t* __restrict__ order, uint64_t* __restrict__ keys,
                                                      uint64_t* __restrict__ tmp, uint32_t* __restrict__ vals) {
   __shared__ uint64_t sa[kChunk];
   __shared__ uint64_t sb[kChunk];
-  const uint2 r = ranges[blockIdx.x];
+  const uint2 r = ranges[order[blockIdx.x]];
   const uint32_t s = r.x, len = r.y - r.x;
   const uint32_t tid = threadIdx.x;
   if (len == 0u) return;
@@ -1177,7 +1180,7 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
     k_seg_scatter<<<nb, kSegThreads, (size_t)a.numTiles * 5, st>>>(a.n, G, a.cnt, a.rect, a.depthKey, a.rec,
                                                                    a.tilesX, a.numTiles, s.segTable, a.ranges,
                                                                    (uint64_t*)a.tkA);
-    k_seg_sort<<<a.numTiles, kBlock, 0, st>>>(a.ranges, (uint64_t*)a.tkA, (uint64_t*)a.tkB, a.tvA);
+    k_seg_sort<<<a.numTiles, kBlock, 0, st>>>(a.ranges, a.tileOrder, (uint64_t*)a.tkA, (uint64_t*)a.tkB, a.tvA);
     out.keys = nullptr;
     out.ids = a.tvA;
     out.keyBytes = 0;
