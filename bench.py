@@ -68,7 +68,9 @@ def algorithmic_bytes(name, n, vis, keys, npx, shC, mixed=False):
     raster fwd  keys*(4 id + 48 record) + px*(16 img + 4 T + 4 end + 16 dL/dC + 4 target)
     raster bwd  keys*(4 id + 48 record) + px*(16 dL/dC + 4 T + 4 end) + vis*40 gradient sums
     update    n*(40+40 sums + 4 count + 2*56 params + 2*2*56 m/v) + n*3*2*4*shC SH/m/v
-              (mixed: m/v 2*2*28, SH/m/v n*3*2*2*shC)"""
+              (mixed: m/v 2*2*28, SH/m/v n*3*2*2*shC)
+    bin_sort  n*(16 count pass + 40 scatter pass: count, rect, depth key, conic)
+              + keys*(8 scattered key + 8 sort read + 4 sorted id)"""
     sb = 2.0 if mixed else 4.0
     if name == "project":
         return n * 72.0 + vis * (sb * shC + 48.0)
@@ -78,6 +80,8 @@ def algorithmic_bytes(name, n, vis, keys, npx, shC, mixed=False):
         return keys * 52.0 + npx * 24.0 + vis * 40.0
     if name == "update":
         return n * (84.0 + 112.0 + 4.0 * 14.0 * sb) + n * 6.0 * sb * shC
+    if name == "bin_sort":
+        return n * 56.0 + keys * 20.0
     return None
 
 
