@@ -26,6 +26,9 @@ struct SortScratch {
   uint32_t* tileCnt;      // maxTiles: keys per tile (tile segments)
   uint32_t* segTable;     // rows x maxTiles: per-block key counts, then offsets (tile segments)
   uint32_t* visPart;      // rows: Gaussians with keys per count block (tile segments)
+  uint32_t* chunkKeys;    // ceil(cap / kSegChunk): keys per chunk of Gaussians, summed by the
+                          // projection, cleared by the scan (tile segments)
+  uint32_t* blockStart;   // rows + 1: first Gaussian of each count/scatter block (tile segments)
 };
 
 // Most tiles the tile-segment strategy handles (LDS histograms of u32; the
@@ -37,6 +40,9 @@ constexpr int kMaxSegTiles = 16384;
 // min(kSegMaxRows, ceil(n / kSegMinGroup)) rows
 constexpr int64_t kSegMinGroup = 4096;
 constexpr int64_t kSegMaxRows = 1024;
+// Gaussians per chunk of the block balancing: count/scatter blocks take whole
+// chunks, as many as give each block an equal share of the keys
+constexpr int64_t kSegChunk = 256;
 int64_t seg_group(int64_t n);
 
 struct BinArgs {
@@ -94,9 +100,10 @@ struct UpdateBufs {
 
 void launch_fill_u32(hipStream_t st, uint32_t* p, int64_t count, uint32_t v);
 
+// chunkKeys (may be null): += each chunk of kSegChunk Gaussians' key count
 void launch_project(hipStream_t st, int deg, bool mixed, int64_t n, int64_t cap, const Cam& cam,
                     const Cut& cut, const float4* P, const void* SH, float4* rec, uint32_t* depthKey,
-                    uint2* rect, uint32_t* cnt);
+                    uint2* rect, uint32_t* cnt, uint32_t* chunkKeys);
 
 int tile_bits(int numTiles);
 void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out);

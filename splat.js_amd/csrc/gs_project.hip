@@ -20,15 +20,11 @@
 namespace gs {
 
 template <int kDeg, bool kMixed>
-__global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam cam, Cut cut,
-                                                    const float4* __restrict__ P,
-                                                    const void* __restrict__ SH,
-                                                    float4* __restrict__ rec,
-                                                    uint32_t* __restrict__ depthKey,
-                                                    uint2* __restrict__ rect,
-                                                    uint32_t* __restrict__ cnt) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ uint32_t project_one(int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
+                                                const float4* __restrict__ P, const void* __restrict__ SH,
+                                                float4* __restrict__ rec, uint32_t* __restrict__ depthKey,
+                                                uint2* __restrict__ rect, uint32_t* __restrict__ cnt,
+                                                int64_t i) {
   float p[16];
   load_params(P, cap, i, p);
   Geom g;
@@ -94,27 +90,48 @@ __global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam 
   depthKey[i] = key;
   rect[i] = rc;
   cnt[i] = c;
+  return c;
+}
+
+template <int kDeg, bool kMixed>
+__global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam cam, Cut cut,
+                                                    const float4* __restrict__ P,
+                                                    const void* __restrict__ SH,
+                                                    float4* __restrict__ rec,
+                                                    uint32_t* __restrict__ depthKey,
+                                                    uint2* __restrict__ rect,
+                                                    uint32_t* __restrict__ cnt,
+                                                    uint32_t* __restrict__ chunkKeys) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t c = i < n ? project_one<kDeg, kMixed>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, i) : 0u;
+  if (chunkKeys) {
+    // the wave's 64 Gaussians lie in one chunk (kSegChunk is a multiple of 64)
+    uint32_t s = c;
+    for (int o = 32; o > 0; o >>= 1) s += (uint32_t)__shfl_xor((int)s, o);
+    const int64_t i0 = i - (threadIdx.x & 63);
+    if ((threadIdx.x & 63) == 0 && s != 0u) atomicAdd(&chunkKeys[i0 / kSegChunk], s);
+  }
 }
 
 template <bool kMixed>
 static void launch_project_t(hipStream_t st, int deg, int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
                              const float4* P, const void* SH, float4* rec, uint32_t* depthKey, uint2* rect,
-                             uint32_t* cnt) {
+                             uint32_t* cnt, uint32_t* chunkKeys) {
   const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
   switch (deg) {
-    case 0: k_project<0, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
-    case 1: k_project<1, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
-    case 2: k_project<2, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
-    default: k_project<3, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt); break;
+    case 0: k_project<0, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    case 1: k_project<1, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    case 2: k_project<2, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    default: k_project<3, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
   }
 }
 
 void launch_project(hipStream_t st, int deg, bool mixed, int64_t n, int64_t cap, const Cam& cam,
                     const Cut& cut, const float4* P, const void* SH, float4* rec, uint32_t* depthKey,
-                    uint2* rect, uint32_t* cnt) {
+                    uint2* rect, uint32_t* cnt, uint32_t* chunkKeys) {
   if (n <= 0) return;
-  if (mixed) launch_project_t<true>(st, deg, n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt);
-  else launch_project_t<false>(st, deg, n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt);
+  if (mixed) launch_project_t<true>(st, deg, n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys);
+  else launch_project_t<false>(st, deg, n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys);
 }
 
 }  // namespace gs

@@ -636,8 +636,61 @@ struct CountPrep {
 
 // Row b of the [block][tile] table: the keys of Gaussians [b G, (b+1) G) per
 // tile; visPart[b]: how many of them have keys.
-__global__ __launch_bounds__(kSegThreads) void k_seg_count(int64_t n, int64_t G, const uint32_t* __restrict__ cnt,
+// This block's Gaussians: the chunks of kSegChunk Gaussians whose key prefix
+// p (over the chunks before them) satisfies b K <= p nb < (b + 1) K, K the
+// total, so each of the nb blocks takes about K / nb keys. The first block
+// starts at 0 and the last ends at n, and block b's end is block b + 1's
+// start by the same formula: the blocks partition [0, n) whatever the chunk
+// sums hold (they only set the balance). Returns [lo, hi).
+__device__ __forceinline__ int2 seg_block_chunks(int64_t n, const uint32_t* __restrict__ chunkKeys, int nb,
+                                                 uint64_t* ws64, uint32_t (*wsc)[kSegThreads / 64]) {
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const int64_t nChunks = (n + kSegChunk - 1) / kSegChunk;
+  const int64_t m = (nChunks + kSegThreads - 1) / kSegThreads;   // chunks per thread, in order
+  const int64_t c0 = (int64_t)tid * m, c1 = c0 + m < nChunks ? c0 + m : nChunks;
+  uint64_t part = 0;
+  for (int64_t c = c0; c < c1; c++) part += chunkKeys[c];
+  uint64_t x = part;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) ws64[w] = x;
+  __syncthreads();
+  uint64_t pre = 0, total = 0;
+  for (int k = 0; k < kSegThreads / 64; k++) {
+    const uint64_t v = ws64[k];
+    if (k < w) pre += v;
+    total += v;
+  }
+  pre += x - part;
+  const uint64_t b = blockIdx.x;
+  const uint64_t tLo = b * total, tHi = (b + 1) * total;
+  uint32_t nLo = 0, nHi = 0;          // chunks with prefix * nb below each bound
+  for (int64_t c = c0; c < c1; c++) {
+    const uint64_t pn = pre * (uint64_t)nb;
+    nLo += pn < tLo;
+    nHi += pn < tHi;
+    pre += chunkKeys[c];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    nLo += (uint32_t)__shfl_xor((int)nLo, o);
+    nHi += (uint32_t)__shfl_xor((int)nHi, o);
+  }
+  if (lane == 0) { wsc[0][w] = nLo; wsc[1][w] = nHi; }
+  __syncthreads();
+  uint32_t sLo = 0, sHi = 0;
+  for (int k = 0; k < kSegThreads / 64; k++) { sLo += wsc[0][k]; sHi += wsc[1][k]; }
+  __syncthreads();   // ws64 and wsc are free again
+  const int64_t lo = (int64_t)sLo * kSegChunk;
+  const int64_t hi = blockIdx.x == (unsigned)(nb - 1) ? n : (int64_t)sHi * kSegChunk;
+  return make_int2((int)(lo < n ? lo : n), (int)(hi < n ? hi : n));
+}
+
+__global__ __launch_bounds__(kSegThreads) void k_seg_count(int64_t n, int nb, const uint32_t* __restrict__ cnt,
                                                            const uint2* __restrict__ rect, int tilesX, int numTiles,
+                                                           const uint32_t* __restrict__ chunkKeys,
+                                                           uint32_t* __restrict__ blockStart,
                                                            uint32_t* __restrict__ table,
                                                            uint32_t* __restrict__ visPart) {
   uint32_t* hist = seg_dyn;   // numTiles
@@ -645,9 +698,15 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_count(int64_t n, int64_t G,
   __shared__ uint2 sRect[kSegThreads];
   __shared__ uint32_t wsum[kSegThreads / 64];
   __shared__ uint16_t owner[kOwnerCap];
+  __shared__ uint64_t ws64[kSegThreads / 64];
+  __shared__ uint32_t wsc[2][kSegThreads / 64];
   const int tid = threadIdx.x;
-  const int64_t lo = (int64_t)blockIdx.x * G;
-  const int64_t hi = lo + G < n ? lo + G : n;
+  const int2 r = seg_block_chunks(n, chunkKeys, nb, ws64, wsc);
+  const int64_t lo = r.x, hi = r.y;
+  if (tid == 0) {
+    blockStart[blockIdx.x] = (uint32_t)lo;
+    if (blockIdx.x == (unsigned)(nb - 1)) blockStart[nb] = (uint32_t)hi;
+  }
   uint32_t vis = 0;
   const SegRound sr{sOff, sRect, wsum, owner};
   seg_rounds(lo, hi, cnt, rect, tilesX, sr, CountPrep{&vis},
@@ -741,7 +800,11 @@ __global__ __launch_bounds__(kScanThreads) void k_seg_scan(const uint32_t* __res
                                                            uint32_t* __restrict__ numKeys,
                                                            uint32_t* __restrict__ numVisible,
                                                            uint64_t* __restrict__ totalKeys,
-                                                           double* __restrict__ stats) {
+                                                           double* __restrict__ stats,
+                                                           uint32_t* __restrict__ chunkKeys, int64_t nChunks) {
+  // the count pass has read the chunk key sums: clear them for the next
+  // projection to fill
+  for (int64_t c = threadIdx.x; c < nChunks; c += kScanThreads) chunkKeys[c] = 0u;
   // tile t at sc[t + t / 16] (numTiles + numTiles / 16 + 1 words): coalesced
   // fill, conflict-free walks (stride 17 between threads)
   uint32_t* sc = seg_dyn;
@@ -886,7 +949,8 @@ struct ScatterPrep {
 // counter starts at the tile's segment start plus this block's offset in
 // the tile (the column scan), so a key's slot is one LDS atomic; the tile's
 // block mask (gs_cover.h) rides in the low 4 bits.
-__global__ __launch_bounds__(kSegThreads) void k_seg_scatter(int64_t n, int64_t G, const uint32_t* __restrict__ cnt,
+__global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint32_t* __restrict__ blockStart,
+                                                             const uint32_t* __restrict__ cnt,
                                                              const uint2* __restrict__ rect,
                                                              const uint32_t* __restrict__ depthKey,
                                                              const float4* __restrict__ rec, int tilesX, int numTiles,
@@ -903,8 +967,7 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(int64_t n, int64_t 
   __shared__ uint16_t owner[kOwnerCap];
   const int tid = threadIdx.x;
   const uint32_t* row = table + (int64_t)blockIdx.x * numTiles;
-  const int64_t lo = (int64_t)blockIdx.x * G;
-  const int64_t hi = lo + G < n ? lo + G : n;
+  const int64_t lo = blockStart[blockIdx.x], hi = blockStart[blockIdx.x + 1];   // the count pass's blocks
   const SegRound sr{sOff, sRect, wsum, owner};
   seg_rounds(lo, hi, cnt, rect, tilesX, sr, ScatterPrep{depthKey, rec, sDk, sG},
              [&]() {
@@ -1171,13 +1234,13 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
     }();
     (void)ldsRaised;
     const size_t tileLds = (size_t)a.numTiles * 4;
-    k_seg_count<<<nb, kSegThreads, tileLds, st>>>(a.n, G, a.cnt, a.rect, a.tilesX, a.numTiles, s.segTable,
-                                                  s.visPart);
+    k_seg_count<<<nb, kSegThreads, tileLds, st>>>(a.n, nb, a.cnt, a.rect, a.tilesX, a.numTiles, s.chunkKeys,
+                                                  s.blockStart, s.segTable, s.visPart);
     k_seg_colscan<<<(a.numTiles + 15) / 16, kBlock, 0, st>>>(s.segTable, nb, a.numTiles, s.tileCnt);
     k_seg_scan<<<1, kScanThreads, (size_t)(a.numTiles + a.numTiles / kScanQuarter + 1) * 4, st>>>(
         s.tileCnt, a.numTiles, a.cap, s.visPart, nb, a.ranges, a.tileOrder, a.numKeys, a.numVisible, a.totalKeys,
-        a.stats);
-    k_seg_scatter<<<nb, kSegThreads, (size_t)a.numTiles * 5, st>>>(a.n, G, a.cnt, a.rect, a.depthKey, a.rec,
+        a.stats, s.chunkKeys, (a.n + kSegChunk - 1) / kSegChunk);
+    k_seg_scatter<<<nb, kSegThreads, (size_t)a.numTiles * 5, st>>>(s.blockStart, a.cnt, a.rect, a.depthKey, a.rec,
                                                                    a.tilesX, a.numTiles, s.segTable, a.ranges,
                                                                    (uint64_t*)a.tkA);
     k_seg_sort<<<a.numTiles, kBlock, 0, st>>>(a.ranges, a.tileOrder, (uint64_t*)a.tkA, (uint64_t*)a.tkB, a.tvA);

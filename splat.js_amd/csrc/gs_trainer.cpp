@@ -360,8 +360,11 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
   h->profLast = nullptr;   // a step or render starts its own first stage
   {
     Scope s(h, "project", (double)n * (56.0 + 4.0 * h->shC + 48.0 + 16.0));
+    // the tile-segment binning balances its blocks by the keys per chunk of
+    // Gaussians, which the projection sums on the way
+    const bool segPath = h->o.sortMode == GS_SORT_TILE_SEGMENTS && cam.numTiles <= kMaxSegTiles;
     launch_project(h->st, h->shDeg, h->mixed, n, h->cap, cam, make_cut(h), h->P, h->SH, h->rec, h->depthKey,
-                   h->rect, h->cnt);
+                   h->rect, h->cnt, segPath ? h->sort.chunkKeys : nullptr);
   }
   {
     BinArgs a{};
@@ -872,6 +875,11 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
     const int64_t rows = std::min<int64_t>(kSegMaxRows, (cap + kSegMinGroup - 1) / kSegMinGroup);
     rc |= dalloc(h, &h->sort.segTable, (size_t)rows * h->maxTiles * 4);
     rc |= dalloc(h, &h->sort.visPart, (size_t)rows * 4);
+    rc |= dalloc(h, &h->sort.blockStart, (size_t)(rows + 1) * 4);
+    const size_t chunkBytes = (size_t)((cap + kSegChunk - 1) / kSegChunk) * 4;
+    rc |= dalloc(h, &h->sort.chunkKeys, chunkBytes);
+    // the projection adds to these sums and the scan clears them after use
+    if (rc == 0) HIPCK(hipMemset(h->sort.chunkKeys, 0, chunkBytes));
   }
   rc |= dalloc(h, &h->dN, 4);
   rc |= dalloc(h, &h->numKeys, 4);
