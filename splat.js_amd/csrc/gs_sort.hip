@@ -636,12 +636,15 @@ struct CountPrep {
 
 // Row b of the [block][tile] table: the keys of Gaussians [b G, (b+1) G) per
 // tile; visPart[b]: how many of them have keys.
-// This block's Gaussians: the chunks of kSegChunk Gaussians whose key prefix
-// p (over the chunks before them) satisfies b K <= p nb < (b + 1) K, K the
-// total, so each of the nb blocks takes about K / nb keys. The first block
-// starts at 0 and the last ends at n, and block b's end is block b + 1's
-// start by the same formula: the blocks partition [0, n) whatever the chunk
-// sums hold (they only set the balance). Returns [lo, hi).
+// This block's Gaussians: the chunks of kSegChunk Gaussians whose cost
+// prefix p (over the chunks before them) satisfies b K <= p nb < (b + 1) K,
+// K the total, so each of the nb blocks takes about K / nb of the cost. A
+// chunk costs its keys plus one per Gaussian: a round of Gaussians costs
+// its loads, scan and barriers even when they have no keys, so key shares
+// alone would hand a block a long run of off-view Gaussians. The first
+// block starts at 0 and the last ends at n, and block b's end is block
+// b + 1's start by the same formula: the blocks partition [0, n) whatever
+// the chunk sums hold (they only set the balance). Returns [lo, hi).
 __device__ __forceinline__ int2 seg_block_chunks(int64_t n, const uint32_t* __restrict__ chunkKeys, int nb,
                                                  uint64_t* ws64, uint32_t (*wsc)[kSegThreads / 64]) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -649,7 +652,7 @@ __device__ __forceinline__ int2 seg_block_chunks(int64_t n, const uint32_t* __re
   const int64_t m = (nChunks + kSegThreads - 1) / kSegThreads;   // chunks per thread, in order
   const int64_t c0 = (int64_t)tid * m, c1 = c0 + m < nChunks ? c0 + m : nChunks;
   uint64_t part = 0;
-  for (int64_t c = c0; c < c1; c++) part += chunkKeys[c];
+  for (int64_t c = c0; c < c1; c++) part += chunkKeys[c] + (uint64_t)kSegChunk;
   uint64_t x = part;
   for (int o = 1; o < 64; o <<= 1) {
     const uint64_t y = __shfl_up(x, o);
@@ -671,7 +674,7 @@ __device__ __forceinline__ int2 seg_block_chunks(int64_t n, const uint32_t* __re
     const uint64_t pn = pre * (uint64_t)nb;
     nLo += pn < tLo;
     nHi += pn < tHi;
-    pre += chunkKeys[c];
+    pre += chunkKeys[c] + (uint64_t)kSegChunk;
   }
   for (int o = 32; o > 0; o >>= 1) {
     nLo += (uint32_t)__shfl_xor((int)nLo, o);
