@@ -136,7 +136,8 @@ def test_config1_bit_exact():
 
 def test_spatially_ordered_input_bit_exact():
     """Config 1 with its Gaussians in Morton order of their positions: the
-    binning's blocks take equal shares of the keys (chunks of 256 Gaussians),
+    binning's blocks take equal shares of the work (keys plus one per
+    Gaussian, in chunks of 256 Gaussians),
     and in this order a chunk holds 72 to 1356 keys (fixed 4096-Gaussian
     blocks would hold up to 1.31x the mean). Two steps match the restatement
     bit for bit (the block partition changes only the time)."""
