@@ -41,7 +41,8 @@ constexpr int kMaxSegTiles = 16384;
 constexpr int64_t kSegMinGroup = 4096;
 constexpr int64_t kSegMaxRows = 1024;
 // Gaussians per chunk of the block balancing: count/scatter blocks take whole
-// chunks, as many as give each block an equal share of the keys
+// chunks, as many as give each block an equal share of the work (keys plus
+// one per Gaussian)
 constexpr int64_t kSegChunk = 256;
 int64_t seg_group(int64_t n);
 
