@@ -116,7 +116,14 @@ GS_HD float gs_expf_raster(float xc) {
   p = fmaf(p, r, 1.0f);
   p = fmaf(p, r, 1.0f);
   const int k = (int)kf;
+#if defined(__HIP_DEVICE_COMPILE__)
+  // p 2^k in one v_ldexp_f32 instead of building 2^k and multiplying. Both
+  // are exact and equal: k = -126 only for x < -86.99, where r >= 0.337 and
+  // p >= 1.4, and p >= 0.707 otherwise, so p 2^k is always normal.
+  return __builtin_amdgcn_ldexpf(p, k);
+#else
   return p * gs_u2f((uint32_t)(k + 127) << 23);
+#endif
 }
 
 // natural log; NaN for x < 0, -inf at 0, +inf at +inf.
