@@ -127,16 +127,19 @@ int main(void) {
       for (int i = 0; i < 4; i++) { hp[i] += cp[i]; hr[i] += cr[i]; }
     }
   }
-  /* gs_expf_mid == gs_expf bit for bit on every float of [-87, 87] */
-  unsigned long long midbad = 0;
-#pragma omp parallel for reduction(+ : midbad) schedule(static)
+  /* gs_expf_mid == gs_expf bit for bit on every float of [-87, 87];
+     rsub: gs_expf_raster results below FLT_MIN (its final p 2^k is exact,
+     and equal to the device's ldexp, only while the result is normal) */
+  unsigned long long midbad = 0, rsub = 0;
+#pragma omp parallel for reduction(+ : midbad, rsub) schedule(static)
   for (long long mm = 0; mm <= (long long)top; mm++) {
     const float xn = gs_u2f(0x80000000u | (uint32_t)mm), xp = gs_u2f((uint32_t)mm);
     midbad += gs_f2u(gs_expf_mid(xn)) != gs_f2u(gs_expf(xn));
     midbad += gs_f2u(gs_expf_mid(xp)) != gs_f2u(gs_expf(xp));
+    rsub += gs_f2u(gs_expf_raster(xn)) < 0x00800000u;
   }
-  printf("%llu %llu %llu %llu %llu %llu %llu %llu %llu %llu\n", n, hp[0], hp[1], hp[2], hp[3], hr[0], hr[1],
-         hr[2], hr[3], midbad);
+  printf("%llu %llu %llu %llu %llu %llu %llu %llu %llu %llu %llu\n", n, hp[0], hp[1], hp[2], hp[3], hr[0], hr[1],
+         hr[2], hr[3], midbad, rsub);
   return 0;
 }
 """
@@ -148,7 +151,8 @@ def test_exps_against_correctly_rounded_exp_exhaustively(tmp_path):
     alpha, argument clamped to [-87, 0] by the caller) within 2 ulp, with the
     2-ulp cases rare; WGSL allows exp 3 + 2|x| ulp. gs_expf_mid (scales and
     sigmoids, branch-free) equals gs_expf bit for bit on every float of
-    [-87, 87]."""
+    [-87, 87]. Every gs_expf_raster result on [-87, 0] is a normal float, so
+    its final scaling by 2^k is exact (the device does it with ldexp)."""
     src = tmp_path / "m.c"
     src.write_text(MATH_PROBE)
     exe = tmp_path / "m"
@@ -158,7 +162,8 @@ def test_exps_against_correctly_rounded_exp_exhaustively(tmp_path):
         flags += ["-mfma", "-msse4.1"]   # hardware fmaf/rintf: same IEEE results, 20x faster than libm
     subprocess.check_call(["gcc", *flags, "-I", os.path.join(ROOT, "splat.js_amd", "csrc"),
                            "-o", str(exe), str(src), "-lm"])
-    n, p0, p1, p2, p3, r0, r1, r2, r3, midbad = map(int, subprocess.check_output([str(exe)], text=True).split())
+    n, p0, p1, p2, p3, r0, r1, r2, r3, midbad, rsub = map(int, subprocess.check_output([str(exe)], text=True).split())
     assert n == 0x42ae0000 + 1 and midbad == 0
+    assert rsub == 0   # every gs_expf_raster result on [-87, 0] is a normal float
     assert p2 == 0 and p3 == 0 and p1 < n // 100
     assert r3 == 0 and r2 < n // 50 and r0 > n * 9 // 10
