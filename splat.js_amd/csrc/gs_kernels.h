@@ -37,9 +37,11 @@ struct SortScratch {
 constexpr int kMaxSegTiles = 16384;
 // Gaussians per count/scatter block of the tile-segment strategy: at least
 // kSegMinGroup, so that the table has ceil(n / seg_group(n)) <=
-// min(kSegMaxRows, ceil(n / kSegMinGroup)) rows
-constexpr int64_t kSegMinGroup = 4096;
+// min(kSegMaxRows, ceil(n / kSegMinGroup)) rows; from 256 * kSegMinGroup
+// Gaussians up, n / kSegBlocks, one 1024-thread block per CU of MI355X's 256
+constexpr int64_t kSegMinGroup = 2048;
 constexpr int64_t kSegMaxRows = 1024;
+constexpr int64_t kSegBlocks = 256;
 // Gaussians per chunk of the block balancing: count/scatter blocks take whole
 // chunks, as many as give each block an equal share of the work (keys plus
 // one per Gaussian)

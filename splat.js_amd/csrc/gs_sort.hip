@@ -1206,10 +1206,13 @@ static void radix_sort(hipStream_t st, const SortScratch& s, const KeyT* kin, co
 }
 
 int64_t seg_group(int64_t n) {
-  // Gaussians per count/scatter block: kSegMinGroup, more for huge n so
-  // that the [block][tile] table keeps at most kSegMaxRows rows
-  const int64_t g = (n + kSegMaxRows - 1) / kSegMaxRows;
-  return g > kSegMinGroup ? (g + kSegThreads - 1) / kSegThreads * kSegThreads : kSegMinGroup;
+  // Gaussians per count/scatter block: n / kSegBlocks (one block per CU),
+  // at least kSegMinGroup; the blocks take whole chunks by cost, so the
+  // group only sets their number. kSegBlocks <= kSegMaxRows keeps the table
+  // within its rows.
+  static_assert(kSegBlocks <= kSegMaxRows, "block count above the table's rows");
+  const int64_t g = (n + kSegBlocks - 1) / kSegBlocks;
+  return g > kSegMinGroup ? g : kSegMinGroup;
 }
 
 int tile_bits(int numTiles) {
