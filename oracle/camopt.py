@@ -4,8 +4,8 @@ refinement of evalCamPsnrRefined (:583-612) and the rotation helpers it uses
 (rodrigues, m3mul: src/sfm/geometry.js:108-114, :150-161), in JS-number
 (double) arithmetic.
 
-Test infrastructure only: tests/ use it as the checker of the trainer's
-camera optimiser. Math.sqrt / Math.cos / Math.sin / Math.exp / Math.pow
+Test infrastructure only: tests/ use it for the expected results of the
+trainer's camera optimiser. Math.sqrt / Math.cos / Math.sin / Math.exp / Math.pow
 are taken from Python's math module, which calls the same libm as the
 trainer's host code; V8's own implementations may differ from libm in the
 last ulp (parity unpinned there).

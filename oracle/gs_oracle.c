@@ -1,6 +1,6 @@
 /* gs_oracle.c — CPU restatement of the splat.js 3DGS trainer step.
  *
- * TEST INFRASTRUCTURE ONLY: this is the checker. Only tests/,
+ * TEST INFRASTRUCTURE ONLY: it computes the expected results. Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg load it; the
  * product (splat.js_amd/) never links or calls it.
  *

@@ -1,7 +1,7 @@
 /* gs_oracle.h — CPU restatement of the splat.js 3DGS trainer step.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP
- * implementation: only tests/, __graft_entry__.smoke() and bench.py's
+ * TEST INFRASTRUCTURE ONLY. This library computes the expected results for
+ * the HIP implementation: only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg load it. The product never links it, and nothing in the
  * product falls back to it.
  *

@@ -2,7 +2,8 @@
 trainer step.
 
 TEST INFRASTRUCTURE ONLY. Imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker. The product never imports it.
+bench.py's cpu_baseline leg, for expected results. The product never
+imports it.
 
 Each stage follows src/gs/shaders.js (see the C sources for per-function
 line citations). `prec` selects the single-precision twin ("f32", bit-exact

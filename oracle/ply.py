@@ -1,8 +1,8 @@
 """CPU restatement of the reference's PLY export (src/io/ply.js:12-100) and
 of Session.exportPlyBlob's camera centres (src/session.js:32-38, :637-645),
 in the reference's arithmetic: JS numbers (f64) over Float32Array inputs,
-f32 stores. Test infrastructure: only tests/ use it, as the checker of the
-trainer's gs_export_ply.
+f32 stores. Test infrastructure: only tests/ use it, for the expected results
+of the trainer's gs_export_ply.
 
 Math.exp / Math.log / Math.sqrt come from Python's math module; Math.hypot
 is taken as the square root of the sum of squares (V8 scales and

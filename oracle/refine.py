@@ -1,8 +1,8 @@
 """CPU restatement of the reference's MCMC-lite refine (src/gs/trainer.js:
 666-794) on full state arrays, in the reference's arithmetic: JS doubles
 for every intermediate, float32 storage (Float32Array stores round to
-nearest). Test infrastructure: only tests/ use it, as the checker of the
-trainer's gs_refine (host planning on 16 B per Gaussian + device apply).
+nearest). Test infrastructure: only tests/ use it, for the expected results
+of the trainer's gs_refine (host planning on 16 B per Gaussian + device apply).
 
 The random stream is the reference's makeRng (src/sfm/geometry.js:750-758).
 Math.log / Math.exp / Math.cos / Math.sqrt are taken from Python's math

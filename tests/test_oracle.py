@@ -1,4 +1,5 @@
-"""Pins the CPU restatement (oracle/) before it is trusted as the checker.
+"""Pins the CPU restatement (oracle/) before its results are trusted as the
+expected ones.
 
 Each check is chosen so that a plausible mistake in the restatement (a dropped
 term, a wrong sign or index, a transposed operand) fails: closed forms,
