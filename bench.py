@@ -214,7 +214,7 @@ def run_one(precision, p, sh, cams, images, W, H, args):
     # key and visible counts vary by view: sample them over a pass of steps after the timed region
     for _ in range(16):
         t.stepOnce()
-        i = t.info()
+        i = t.step_stats()
         keys.append(i.lastKeys)
         vis.append(i.lastVisible)
     mse = t.readLoss()

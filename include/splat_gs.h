@@ -12,7 +12,9 @@
  *
  * Threading: one trainer is used from one host thread at a time. Work is
  * enqueued on the trainer's own HIP stream; gs_step never waits for the GPU
- * (the reference's queue.submit, trainer.js:458); readbacks synchronise.
+ * (the reference's queue.submit, trainer.js:458). The plain readbacks wait
+ * for the stream; their *_async forms return a pending readback whose wait
+ * may run on another thread.
  */
 #ifndef SPLAT_GS_H
 #define SPLAT_GS_H
@@ -54,9 +56,11 @@ typedef struct gs_opts {
   float shLr;          /* SH learning rate, default 1.25e-4 (trainer.js:302) */
   float minScale;      /* min scale as a fraction of scene radius, default 1e-4 (trainer.js:338) */
   int32_t maxIters;    /* schedule horizon, default 60000 (trainer.js:319) */
-  int64_t maxSplats;   /* capacity ceiling, default 600000 (trainer.js:107) */
+  int64_t maxSplats;   /* capacity ceiling; 0 (default) = unset: capacity at most 600000
+                          (trainer.js:107) */
   float capMult;       /* capacity = n * capMult, default 4 (trainer.js:106) */
-  int64_t entriesCap;  /* (tile, Gaussian) key budget; 0 = max(12M, 24*maxSplats) (trainer.js:48-49) */
+  int64_t entriesCap;  /* (tile, Gaussian) key budget; 0 = 12M, or max(12M, 24*maxSplats) when
+                          maxSplats is set (trainer.js:48-49) */
   int32_t tileGrad;    /* accepted for compatibility (trainer.js:26); both orders give identical sums */
   float posLr;         /* base position lr; 0 = 3e-4 * sceneRadius (trainer.js:322) */
   int32_t loss;        /* GS_LOSS_CHARBONNIER (the reference's loss) or GS_LOSS_L1_DSSIM */
@@ -107,20 +111,26 @@ typedef struct gs_image {
 } gs_image;
 
 /* Fields the reference's callers read off the trainer object
- * (session.js:263-277, :479-497, :572-604; trainer.js:27, :102-107, :319). */
+ * (session.js:263-277, :479-497, :572-604; trainer.js:27, :102-107, :319).
+ * All of them are host state: gs_get_info never waits for the GPU, so a
+ * caller may read them every frame while steps are queued. */
 typedef struct gs_info {
   int64_t n, cap, iter;
   int32_t shDeg, shK, stride, numCams;
   int32_t lastCam, holdout;
-  int64_t lastKeys;          /* (tile, Gaussian) keys of the last raster */
-  int64_t lastVisible;       /* Gaussians that passed projection culling in the last raster */
-  int64_t overflowKeys;      /* entry-budget drops, cumulative: tiles with the default sort (the
-                                reference's entryOverflowTiles, trainer.js:627), keys with the
-                                radix sorts */
   int64_t entriesCap;
   int32_t maxTiles;
   float horizon;
 } gs_info;
+
+/* Device-side figures of the last raster (synchronises). */
+typedef struct gs_step_stats {
+  int64_t lastKeys;          /* (tile, Gaussian) keys of the last raster */
+  int64_t lastVisible;       /* Gaussians with at least one key in the last raster */
+  int64_t overflowKeys;      /* entry-budget drops of the training steps, cumulative since setup:
+                                tiles with the default sort (the reference's entryOverflowTiles,
+                                trainer.js:627), keys with the radix sorts */
+} gs_step_stats;
 
 /* fills defaults (the reference's values) */
 int gs_default_opts(gs_opts* o);
@@ -132,7 +142,8 @@ void gs_destroy(gs_trainer_t* h);
 
 /* GSTrainer.setup(gaussians, cams, images, maxViewW, maxViewH, sceneRadius)
  * (trainer.js:100-355). params: n x 16 f32 in the reference layout
- * (shaders.js:14-16); sh: n x 3K f32 channel-major or NULL (zeros).
+ * (shaders.js:14-16); sh: n x 3K f32 channel-major (K of the trainer's
+ * shDeg; the bindings reject any other length) or NULL (zeros).
  * Copies everything; the caller keeps ownership. */
 int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh,
              const gs_camera* cams, int32_t ncams, const gs_image* images, int32_t nimages,
@@ -150,8 +161,40 @@ int gs_step_batch(gs_trainer_t* h, const int32_t* cams, int32_t count);
 
 /* GSTrainer.readLoss() (trainer.js:615-631): mean squared error per valid
  * pixel-channel since the last call, then resets. *has = 0 is the
- * reference's null (no pixels seen). */
+ * reference's null (no pixels seen). Synchronises; gs_read_loss_async is
+ * the form that does not. */
 int gs_read_loss(gs_trainer_t* h, double* mse, int32_t* has);
+
+/* ---- asynchronous readbacks (the reference's copy-to-staging plus
+ * mapAsync, trainer.js:615-631, :797-817, :542-573) ----
+ * A *_async call enqueues the device work and the copy into pinned host
+ * memory on the trainer's stream, updates the trainer's host state as the
+ * reference does at that point (readLoss resets its pixel window), and
+ * returns at once with a pending readback. gs_pending_wait blocks until the
+ * copy has landed and finishes the result; it touches no trainer state, so
+ * it may run on another thread (the addon's napi_async_work) while the
+ * trainer's own thread keeps enqueueing steps. A pending readback stays
+ * valid after gs_destroy of its trainer. */
+typedef struct gs_pending gs_pending_t;
+
+typedef struct gs_readback_result {
+  double value;       /* readLoss: MSE; evalCamPsnr: PSNR */
+  int32_t has;        /* readLoss: 0 is the reference's null (no pixel seen); otherwise 1 */
+  int64_t overflow;   /* readLoss: entry-budget drops of the training steps, cumulative
+                         (the reference's entryOverflowTiles after this readLoss) */
+} gs_readback_result;
+
+int gs_read_loss_async(gs_trainer_t* h, gs_pending_t** out);
+/* evalCamPsnr(ci); the training loss window is reset at enqueue (trainer.js:570-571) */
+int gs_eval_cam_psnr_async(gs_trainer_t* h, int32_t ci, gs_pending_t** out);
+/* readGaussians(): params n x 16 and, when shDeg > 0, sh n x 3K are written
+ * by gs_pending_wait (either may be NULL); n is the count at enqueue. The
+ * caller keeps both arrays alive until the wait returns. */
+int gs_read_gaussians_async(gs_trainer_t* h, float* params, float* sh, gs_pending_t** out);
+/* waits for the readback; 0 or GS_E_HIP (message in gs_pending_error) */
+int gs_pending_wait(gs_pending_t* p, gs_readback_result* out);
+const char* gs_pending_error(gs_pending_t* p);
+void gs_pending_free(gs_pending_t* p);
 
 /* GSTrainer.renderView(camParams, ctx, trainMode, offset) (trainer.js:634-655):
  * renders into a caller buffer of w*h*4 f32 (C + T*bg, 1 - T) instead of a
@@ -185,7 +228,10 @@ int gs_set_excluded(gs_trainer_t* h, const int32_t* cams, int32_t count);
 /* trainer.iter = iter: the reference's callers assign it (session.js resume) */
 int gs_set_iter(gs_trainer_t* h, int64_t iter);
 
+/* host fields only; never waits for the GPU */
 int gs_get_info(gs_trainer_t* h, gs_info* out);
+/* the last raster's key and visibility counts and the cumulative budget drops */
+int gs_read_step_stats(gs_trainer_t* h, gs_step_stats* out);
 
 /* Standard 3DGS binary PLY of the current Gaussians (the reference's
  * gaussiansToPly, src/io/ply.js:47-100: x y z, zero normals, f_dc, f_rest

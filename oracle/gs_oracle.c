@@ -258,9 +258,12 @@ void orc_loss_l1ssim_f64(const orc_cam* cam, const double* img, const uint32_t* 
     for (int c = 0; c < 3; c++) gC[3 * p + c] = 0;
   }
   for (int c = 0; c < 3; c++) {
+    /* an invalid target pixel (undistortion border) enters every window as
+     * zero in both images, like the padding outside the image, so the loss
+     * does not depend on what is rendered there */
     for (int64_t p = 0; p < np; p++) {
-      X[p] = (double)cam->gain * img[4 * p + c] + (double)cam->bias;
-      Y[p] = (double)((tgt[p] >> (8 * c)) & 0xffu) / 255.0;
+      X[p] = valid[p] ? (double)cam->gain * img[4 * p + c] + (double)cam->bias : 0.0;
+      Y[p] = valid[p] ? (double)((tgt[p] >> (8 * c)) & 0xffu) / 255.0 : 0.0;
     }
     blur(W, H, X, mx, tmp, g);
     blur(W, H, Y, my, tmp, g);
@@ -296,12 +299,10 @@ void orc_loss_l1ssim_f64(const orc_cam* cam, const double* img, const uint32_t* 
     blur(W, H, B, t2, tmp, g);
     blur(W, H, Cm, A, tmp, g);
     for (int64_t p = 0; p < np; p++) {
+      if (!valid[p]) continue;   /* constant zero in the loss: no gradient */
       const double dssim = t1[p] + 2.0 * X[p] * t2[p] + Y[p] * A[p];
-      double d = -lambda * dssim;
-      if (valid[p]) {
-        const double err = X[p] - Y[p];
-        d += (1.0 - lambda) * (err > 0 ? 1.0 : (err < 0 ? -1.0 : 0.0));
-      }
+      const double err = X[p] - Y[p];
+      const double d = -lambda * dssim + (1.0 - lambda) * (err > 0 ? 1.0 : (err < 0 ? -1.0 : 0.0));
       gC[3 * p + c] = (double)cam->gain * d;
     }
   }

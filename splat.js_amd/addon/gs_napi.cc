@@ -5,6 +5,12 @@
 // The library is opened with dlopen from ../libsplatgs.so, so loading the
 // addon does not touch the GPU and a missing library is reported as a JS
 // Error naming the path.
+//
+// Readbacks (readLossAsync, readGaussiansAsync, evalCamPsnrAsync) return
+// Promises: the JS thread enqueues the copy (gs_*_async) and a
+// napi_async_work waits for it on a worker thread, so the event loop keeps
+// running and stepOnce keeps enqueueing meanwhile. Errors carry a `code`
+// ("GS_E_HIP" for device errors, which the shim maps to device loss).
 #include <dlfcn.h>
 #include <node_api.h>
 
@@ -37,6 +43,13 @@ struct Api {
   int (*set_excluded)(gs_trainer_t*, const int32_t*, int32_t);
   int (*set_iter)(gs_trainer_t*, int64_t);
   int (*get_info)(gs_trainer_t*, gs_info*);
+  int (*read_step_stats)(gs_trainer_t*, gs_step_stats*);
+  int (*read_loss_async)(gs_trainer_t*, gs_pending_t**);
+  int (*eval_cam_psnr_async)(gs_trainer_t*, int32_t, gs_pending_t**);
+  int (*read_gaussians_async)(gs_trainer_t*, float*, float*, gs_pending_t**);
+  int (*pending_wait)(gs_pending_t*, gs_readback_result*);
+  const char* (*pending_error)(gs_pending_t*);
+  void (*pending_free)(gs_pending_t*);
   int (*sync)(gs_trainer_t*);
   int (*refine)(gs_trainer_t*, const gs_refine_opts*, gs_refine_result*);
   int (*export_ply)(gs_trainer_t*, int32_t, void*, int64_t, int64_t*);
@@ -85,6 +98,13 @@ bool load_api() {
   SYM(set_excluded, "gs_set_excluded");
   SYM(set_iter, "gs_set_iter");
   SYM(get_info, "gs_get_info");
+  SYM(read_step_stats, "gs_read_step_stats");
+  SYM(read_loss_async, "gs_read_loss_async");
+  SYM(eval_cam_psnr_async, "gs_eval_cam_psnr_async");
+  SYM(read_gaussians_async, "gs_read_gaussians_async");
+  SYM(pending_wait, "gs_pending_wait");
+  SYM(pending_error, "gs_pending_error");
+  SYM(pending_free, "gs_pending_free");
   SYM(sync, "gs_sync");
   SYM(refine, "gs_refine");
   SYM(export_ply, "gs_export_ply");
@@ -99,12 +119,30 @@ napi_value throw_err(napi_env env, const std::string& msg) {
   return nullptr;
 }
 
+const char* code_name(int rc) {
+  switch (rc) {
+    case GS_E_ARG: return "GS_E_ARG";
+    case GS_E_HIP: return "GS_E_HIP";
+    case GS_E_NOMEM: return "GS_E_NOMEM";
+    case GS_E_LIMIT: return "GS_E_LIMIT";
+    default: return "GS_E_UNKNOWN";
+  }
+}
+
 bool check(napi_env env, gs_trainer_t* h, int rc) {
   if (rc == 0) return true;
-  std::string msg = h ? g_api.last_error(h) : "";
+  std::string msg = h ? g_api.last_error(h) : "trainer destroyed";
   if (msg.empty()) msg = "splat_gs error " + std::to_string(rc);
-  napi_throw_error(env, nullptr, msg.c_str());
+  napi_throw_error(env, code_name(rc), msg.c_str());
   return false;
+}
+
+napi_value make_error(napi_env env, int rc, const std::string& msg) {
+  napi_value code, text, err;
+  napi_create_string_utf8(env, code_name(rc), NAPI_AUTO_LENGTH, &code);
+  napi_create_string_utf8(env, msg.c_str(), NAPI_AUTO_LENGTH, &text);
+  napi_create_error(env, code, text, &err);
+  return err;
 }
 
 #define NAPI_CALL(env, call)                          \
@@ -156,10 +194,16 @@ bool get_f32(napi_env env, napi_value v, float** data, size_t* len) {
   return true;
 }
 
+// The JS handle is an external around a box, so destroy() can free the
+// trainer at once and the finalizer then finds nothing left to free.
+struct Box {
+  gs_trainer_t* h;
+};
+
 gs_trainer_t* get_handle(napi_env env, napi_value v) {
   void* p = nullptr;
-  if (napi_get_value_external(env, v, &p) != napi_ok) return nullptr;
-  return (gs_trainer_t*)p;
+  if (napi_get_value_external(env, v, &p) != napi_ok || !p) return nullptr;
+  return ((Box*)p)->h;
 }
 
 bool read_camera(napi_env env, napi_value c, gs_camera* out) {
@@ -191,7 +235,10 @@ bool read_camera(napi_env env, napi_value c, gs_camera* out) {
 }
 
 void finalize_trainer(napi_env, void* data, void*) {
-  if (data && g_api.destroy) g_api.destroy((gs_trainer_t*)data);
+  Box* b = (Box*)data;
+  if (!b) return;
+  if (b->h && g_api.destroy) g_api.destroy(b->h);
+  delete b;
 }
 
 // create(opts) -> external
@@ -236,8 +283,22 @@ napi_value Create(napi_env env, napi_callback_info info) {
   const int rc = g_api.create(&o, &h);
   if (rc != 0) return throw_err(env, "GSTrainer.create: no usable HIP device (gs_create returned " + std::to_string(rc) + ")");
   napi_value ext;
-  NAPI_CALL(env, napi_create_external(env, h, finalize_trainer, nullptr, &ext));
+  Box* box = new Box{h};
+  NAPI_CALL(env, napi_create_external(env, box, finalize_trainer, nullptr, &ext));
   return ext;
+}
+
+// destroy(h): frees the device state now (trainer.js's device.destroy())
+napi_value Destroy(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* p = nullptr;
+  if (argc < 1 || napi_get_value_external(env, argv[0], &p) != napi_ok || !p) return nullptr;
+  Box* b = (Box*)p;
+  if (b->h) g_api.destroy(b->h);
+  b->h = nullptr;
+  return nullptr;
 }
 
 // setup(h, params, n, sh|null, cams[], images[], maxW, maxH, radius)
@@ -247,6 +308,7 @@ napi_value Setup(napi_env env, napi_callback_info info) {
   NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   if (argc < 9) return throw_err(env, "setup: expected 9 arguments");
   gs_trainer_t* h = get_handle(env, argv[0]);
+  if (!h) return throw_err(env, "setup: trainer destroyed");
   float* params = nullptr;
   size_t plen = 0;
   if (!get_f32(env, argv[1], &params, &plen)) return throw_err(env, "setup: gaussians.data must be a Float32Array");
@@ -259,6 +321,17 @@ napi_value Setup(napi_env env, napi_callback_info info) {
   napi_typeof(env, argv[3], &st);
   if (st != napi_null && st != napi_undefined && !get_f32(env, argv[3], &sh, &shlen))
     return throw_err(env, "setup: sh must be a Float32Array or null");
+  if (sh) {
+    // the trainer reads n * 3K coefficients of its own SH degree
+    gs_info in;
+    if (!check(env, h, g_api.get_info(h, &in))) return nullptr;
+    const int64_t want = n * 3 * (int64_t)in.shK;
+    if ((int64_t)shlen != want)
+      return throw_err(env, "setup: gaussians.sh holds " + std::to_string(shlen) + " floats; SH degree " +
+                                std::to_string(in.shDeg) + " needs n * 3 * " + std::to_string(in.shK) + " = " +
+                                std::to_string(want));
+    if (want == 0) sh = nullptr;
+  }
   uint32_t ncam = 0, nimg = 0;
   NAPI_CALL(env, napi_get_array_length(env, argv[4], &ncam));
   NAPI_CALL(env, napi_get_array_length(env, argv[5], &nimg));
@@ -495,13 +568,148 @@ napi_value Info(napi_env env, napi_callback_info info) {
   set_num(env, o, "numCams", in.numCams);
   set_num(env, o, "lastCam", in.lastCam);
   set_num(env, o, "holdout", in.holdout);
-  set_num(env, o, "lastKeys", (double)in.lastKeys);
-  set_num(env, o, "lastVisible", (double)in.lastVisible);
-  set_num(env, o, "overflowKeys", (double)in.overflowKeys);
   set_num(env, o, "entriesCap", (double)in.entriesCap);
   set_num(env, o, "maxTiles", in.maxTiles);
   set_num(env, o, "horizon", in.horizon);
   return o;
+}
+
+// stepStats(h) -> {lastKeys, lastVisible, overflowKeys} (waits for the GPU)
+napi_value StepStats(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  gs_trainer_t* h = get_handle(env, argv[0]);
+  if (!h) return throw_err(env, "stepStats: trainer destroyed");
+  gs_step_stats s;
+  if (!check(env, h, g_api.read_step_stats(h, &s))) return nullptr;
+  napi_value o;
+  NAPI_CALL(env, napi_create_object(env, &o));
+  set_num(env, o, "lastKeys", (double)s.lastKeys);
+  set_num(env, o, "lastVisible", (double)s.lastVisible);
+  set_num(env, o, "overflowKeys", (double)s.overflowKeys);
+  return o;
+}
+
+// ---- Promise readbacks ----
+struct Job {
+  enum Kind { kLoss, kPsnr, kGaussians } kind;
+  napi_async_work work = nullptr;
+  napi_deferred deferred = nullptr;
+  gs_pending_t* pending = nullptr;
+  napi_ref keep[2] = {nullptr, nullptr};   // destination arrays of readGaussians
+  int rc = 0;
+  gs_readback_result r{};
+  std::string err;
+};
+
+void job_execute(napi_env, void* data) {
+  Job* j = (Job*)data;
+  j->rc = g_api.pending_wait(j->pending, &j->r);
+  if (j->rc != 0) j->err = g_api.pending_error(j->pending);
+}
+
+void job_complete(napi_env env, napi_status status, void* data) {
+  Job* j = (Job*)data;
+  if (status != napi_ok && j->rc == 0) {
+    j->rc = GS_E_ARG;
+    j->err = "readback cancelled";
+  }
+  if (j->rc != 0) {
+    napi_reject_deferred(env, j->deferred, make_error(env, j->rc, j->err.empty() ? "readback failed" : j->err));
+  } else {
+    napi_value v;
+    if (j->kind == Job::kLoss) {
+      // {mse: number | null, overflow: number}
+      napi_create_object(env, &v);
+      napi_value m;
+      if (j->r.has) napi_create_double(env, j->r.value, &m);
+      else napi_get_null(env, &m);
+      napi_set_named_property(env, v, "mse", m);
+      set_num(env, v, "overflow", (double)j->r.overflow);
+    } else if (j->kind == Job::kPsnr) {
+      napi_create_double(env, j->r.value, &v);
+    } else {
+      napi_get_undefined(env, &v);
+    }
+    napi_resolve_deferred(env, j->deferred, v);
+  }
+  g_api.pending_free(j->pending);
+  for (napi_ref& r : j->keep)
+    if (r) napi_delete_reference(env, r);
+  napi_delete_async_work(env, j->work);
+  delete j;
+}
+
+// Starts the wait of an enqueued readback; returns the Promise.
+napi_value start_job(napi_env env, Job* j) {
+  napi_value promise, name;
+  if (napi_create_promise(env, &j->deferred, &promise) != napi_ok ||
+      napi_create_string_utf8(env, "splat_gs_readback", NAPI_AUTO_LENGTH, &name) != napi_ok ||
+      napi_create_async_work(env, nullptr, name, job_execute, job_complete, j, &j->work) != napi_ok ||
+      napi_queue_async_work(env, j->work) != napi_ok) {
+    g_api.pending_free(j->pending);
+    for (napi_ref& r : j->keep)
+      if (r) napi_delete_reference(env, r);
+    delete j;
+    return throw_err(env, "readback: cannot queue the async work");
+  }
+  return promise;
+}
+
+// readLossAsync(h) -> Promise<{mse, overflow}>
+napi_value ReadLossAsync(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  gs_trainer_t* h = get_handle(env, argv[0]);
+  if (!h) return throw_err(env, "readLoss: trainer destroyed");
+  Job* j = new Job();
+  j->kind = Job::kLoss;
+  if (!check(env, h, g_api.read_loss_async(h, &j->pending))) { delete j; return nullptr; }
+  return start_job(env, j);
+}
+
+// evalCamPsnrAsync(h, ci) -> Promise<number>
+napi_value EvalCamPsnrAsync(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  gs_trainer_t* h = get_handle(env, argv[0]);
+  if (!h) return throw_err(env, "evalCamPsnr: trainer destroyed");
+  int32_t ci = 0;
+  napi_get_value_int32(env, argv[1], &ci);
+  Job* j = new Job();
+  j->kind = Job::kPsnr;
+  if (!check(env, h, g_api.eval_cam_psnr_async(h, ci, &j->pending))) { delete j; return nullptr; }
+  return start_job(env, j);
+}
+
+// readGaussiansAsync(h, Float32Array params, Float32Array|null sh) -> Promise<undefined>
+napi_value ReadGaussiansAsync(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  gs_trainer_t* h = get_handle(env, argv[0]);
+  if (!h) return throw_err(env, "readGaussians: trainer destroyed");
+  gs_info in;
+  if (!check(env, h, g_api.get_info(h, &in))) return nullptr;
+  float *p = nullptr, *sh = nullptr;
+  size_t lp = 0, ls = 0;
+  if (!get_f32(env, argv[1], &p, &lp) || (int64_t)lp < in.n * 16)
+    return throw_err(env, "readGaussians: params must be a Float32Array of n*16");
+  napi_valuetype t;
+  napi_typeof(env, argv[2], &t);
+  const bool hasSh = t != napi_null && t != napi_undefined;
+  if (hasSh && (!get_f32(env, argv[2], &sh, &ls) || (int64_t)ls < in.n * 3 * in.shK))
+    return throw_err(env, "readGaussians: sh must be a Float32Array of n*3*shK");
+  Job* j = new Job();
+  j->kind = Job::kGaussians;
+  if (!check(env, h, g_api.read_gaussians_async(h, p, sh, &j->pending))) { delete j; return nullptr; }
+  // the wait writes into both arrays: keep them from collection until then
+  napi_create_reference(env, argv[1], 1, &j->keep[0]);
+  if (hasSh) napi_create_reference(env, argv[2], 1, &j->keep[1]);
+  return start_job(env, j);
 }
 
 napi_value Sync(napi_env env, napi_callback_info info) {
@@ -579,6 +787,9 @@ napi_value Init(napi_env env, napi_value exports) {
       {"refine", Refine},           {"exportPly", ExportPly},
       {"evalCamPsnrRefined", EvalCamPsnrRefined}, {"getCamera", GetCamera},
       {"readCamGrads", ReadCamGrads},
+      {"destroy", Destroy},         {"stepStats", StepStats},
+      {"readLossAsync", ReadLossAsync}, {"evalCamPsnrAsync", EvalCamPsnrAsync},
+      {"readGaussiansAsync", ReadGaussiansAsync},
       {"loaded", Loaded},           {"loadError", LoadError},
   };
   for (const auto& f : fns) {

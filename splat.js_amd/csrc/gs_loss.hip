@@ -5,7 +5,11 @@
 //       (1 - lambda) |x - y| + lambda (1 - ssim_c(p))
 //   x = gain * C + bias (rendered, exposure-adjusted), y = target
 //   ssim (Wang et al. 2004): 11x11 Gaussian window, sigma 1.5, C1 = 0.01^2,
-//   C2 = 0.03^2, zero padding outside the image.
+//   C2 = 0.03^2, zero padding outside the image. An invalid target pixel
+//   (undistortion border, alpha byte 0) enters every window as zero in both
+//   images, like the padding, so the loss does not depend on what is
+//   rendered there and its dL/dC is zero, as its valid flag (gC.w = 0) tells
+//   the backward pass.
 //
 // Two LDS-tiled passes over 16x16 pixel tiles with a 5-pixel halo:
 //   k_ssim_fwd  blurs x, y, x^2, y^2, xy (separable: 26 rows x 16 columns
@@ -39,9 +43,13 @@ __device__ __forceinline__ void load_xy(const Cam& cam, const RasterBufs& b, int
     return;
   }
   const int64_t pi = (int64_t)gy * cam.W + gx;
-  const float4 o = b.img[pi];
   const uint32_t pk = b.tgt[cam.tgtOffset + pi];
   valid = (pk >> 24) != 0u;
+  if (!valid) {
+    for (int c = 0; c < 3; c++) { x[c] = 0.0f; y[c] = 0.0f; }
+    return;
+  }
+  const float4 o = b.img[pi];
   const float oc[3] = {o.x, o.y, o.z};
   for (int c = 0; c < 3; c++) {
     x[c] = cam.gain * oc[c] + cam.bias;
@@ -172,18 +180,15 @@ __global__ __launch_bounds__(kBlock) void k_ssim_bwd(Cam cam, SsimWin win, float
   float xs[3], ys[3];
   bool valid;
   load_xy(cam, b, x, y, xs, ys, valid);
-  float g[3];
-  for (int ch = 0; ch < 3; ch++) {
+  float g[3] = {0.f, 0.f, 0.f};
+  for (int ch = 0; ch < 3 && valid; ch++) {
     float bl[3] = {0.f, 0.f, 0.f};
 #pragma unroll
     for (int q = 0; q < 11; q++)
       for (int s = 0; s < 3; s++) bl[s] = fmaf(win.g[q], sH[3 * ch + s][ly + q][lx], bl[s]);
     const float dssim = bl[0] + 2.f * xs[ch] * bl[1] + ys[ch] * bl[2];
-    float d = -lambda * dssim;
-    if (valid) {
-      const float err = xs[ch] - ys[ch];
-      d += (1.f - lambda) * (err > 0.f ? 1.f : (err < 0.f ? -1.f : 0.f));
-    }
+    const float err = xs[ch] - ys[ch];
+    const float d = -lambda * dssim + (1.f - lambda) * (err > 0.f ? 1.f : (err < 0.f ? -1.f : 0.f));
     g[ch] = cam.gain * d;
   }
   b.gC[pi] = make_float4(g[0], g[1], g[2], valid ? 1.0f : 0.0f);

@@ -128,7 +128,7 @@ struct gs_trainer {
   uint32_t rng = 0;
   int lastCam = -1;
   int lastNumTiles = 0;   // tiles of the last raster (refine's maxTile)
-  double overflowTotal = 0.0;
+  bool lastSegPath = false;   // the last raster binned by tile segments (raw tile counts in sort.tileCnt)
   BinOut lastBin{};
 
   bool prof = false;
@@ -136,6 +136,25 @@ struct gs_trainer {
   std::vector<PendingEv> pending;
   hipEvent_t profLast = nullptr;   // end event of the last profiled stage of this raster/step
   std::vector<hipEvent_t> evPool;
+};
+
+// One asynchronous readback: pinned host copies, filled on the trainer's
+// stream, and an event behind them. It owns everything it touches, so it
+// outlives its trainer and its wait can run on any thread.
+struct gs_pending {
+  enum Kind { kLoss, kPsnr, kGaussians } kind = kLoss;
+  int device = 0;
+  hipEvent_t ev = nullptr;
+  double* stats = nullptr;     // pinned [4] (loss, psnr)
+  int64_t px = 0;              // pixels the loss window saw (loss)
+  void* hP = nullptr;          // pinned parameter planes (gaussians)
+  void* hSH = nullptr;         // pinned SH planes (gaussians)
+  float* dstP = nullptr;
+  float* dstSH = nullptr;
+  int64_t n = 0, cap = 0;
+  int shC = 0, shP4 = 0, shQ8 = 0;
+  bool mixed = false;
+  std::string err;
 };
 
 namespace {
@@ -363,6 +382,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
     // the tile-segment binning balances its blocks by the keys per chunk of
     // Gaussians, which the projection sums on the way
     const bool segPath = h->o.sortMode == GS_SORT_TILE_SEGMENTS && cam.numTiles <= kMaxSegTiles;
+    h->lastSegPath = segPath;
     launch_project(h->st, h->shDeg, h->mixed, n, h->cap, cam, make_cut(h), h->P, h->SH, h->rec, h->depthKey,
                    h->rect, h->cnt, segPath ? h->sort.chunkKeys : nullptr);
   }
@@ -508,7 +528,7 @@ int eval_pass(gs_trainer* h, int ci, const double* R, const double* t, double g,
   HIPCK(hipMemcpyAsync(s, h->evalStats, 32, hipMemcpyDeviceToHost, h->st));
   HIPCK(hipMemcpyAsync(row, h->camGrad + 8 * ci, 32, hipMemcpyDeviceToHost, h->st));
   HIPCK(hipMemsetAsync(h->camGrad, 0, rows * 32, h->st));
-  HIPCK(hipMemsetAsync(h->stats, 0, 32, h->st));
+  HIPCK(hipMemsetAsync(h->stats, 0, 24, h->st));   // the loss window; the budget drops keep counting
   HIPCK(hipStreamSynchronize(h->st));
   h->pixelsSeen = 0;
   if (grad8) memcpy(grad8, row, sizeof row);
@@ -680,11 +700,6 @@ void repack_to_planes(const float* src, int64_t n, int width, int planes, int64_
     for (int j = 0; j < width; j++) dst[((size_t)(j / 4) * cap + i) * 4 + (j % 4)] = src[i * width + j];
 }
 
-void planes_to_rows(const std::vector<float>& src, int64_t n, int width, int64_t cap, float* dst) {
-  for (int64_t i = 0; i < n; i++)
-    for (int j = 0; j < width; j++) dst[i * width + j] = src[((size_t)(j / 4) * cap + i) * 4 + (j % 4)];
-}
-
 }  // namespace
 
 extern "C" {
@@ -702,7 +717,7 @@ int gs_default_opts(gs_opts* o) {
   o->shLr = 1.25e-4f;
   o->minScale = 1e-4f;
   o->maxIters = 60000;
-  o->maxSplats = 600000;
+  o->maxSplats = 0;   // unset: capacity at most 600000, budget 12M
   o->capMult = 4.0f;
   o->entriesCap = 0;
   o->tileGrad = 1;
@@ -782,8 +797,10 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   const int64_t maxSplats = h->o.maxSplats > 0 ? h->o.maxSplats : 600000;
   h->cap = std::max<int64_t>(n, std::min<int64_t>(std::max<int64_t>((int64_t)std::floor(n * (double)h->o.capMult), n), maxSplats));
   if (h->cap > (int64_t)kIdMask + 1) return fail(h, GS_E_LIMIT, "setup: more than 2^28 Gaussians");
-  // trainer.js:48-49
-  h->entriesCap = h->o.entriesCap > 0 ? h->o.entriesCap : std::max<int64_t>(kDefaultEntries, 24 * maxSplats);
+  // trainer.js:48-49: the budget follows maxSplats only when the caller set it
+  h->entriesCap = h->o.entriesCap > 0 ? h->o.entriesCap
+                  : h->o.maxSplats > 0 ? std::max<int64_t>(kDefaultEntries, 24 * h->o.maxSplats)
+                                       : kDefaultEntries;
   h->entriesCap = std::min<int64_t>(h->entriesCap, 0xffffffffll);
 
   // targets (trainer.js:115-139)
@@ -960,21 +977,23 @@ int gs_step_debug(gs_trainer_t* h, int32_t cam, gs_debug_out* out) {
   return step_impl(h, cam, out);
 }
 
+// Runs a *_async entry point, waits and frees: the synchronous form.
+static int wait_pending(gs_trainer* h, int rc, gs_pending* p, gs_readback_result* r) {
+  if (rc != GS_OK) return rc;
+  rc = gs_pending_wait(p, r);
+  if (rc != GS_OK) fail(h, rc, p->err);
+  gs_pending_free(p);
+  return rc;
+}
+
 int gs_read_loss(gs_trainer_t* h, double* mse, int32_t* has) {
   if (!h || !mse || !has) return GS_E_ARG;
-  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
-  double s[4];
-  HIPCK(hipMemcpyAsync(s, h->stats, 32, hipMemcpyDeviceToHost, h->st));
-  HIPCK(hipMemsetAsync(h->stats, 0, 32, h->st));
-  HIPCK(hipStreamSynchronize(h->st));
-  const int64_t px = h->pixelsSeen;
-  h->pixelsSeen = 0;
-  h->overflowTotal += s[3];
-  *has = 0;
-  *mse = 0.0;
-  if (px == 0 || s[2] == 0.0) return GS_OK;
-  *has = 1;
-  *mse = s[0] / (s[2] * 3.0);
+  gs_pending* p = nullptr;
+  gs_readback_result r{};
+  const int rc = wait_pending(h, gs_read_loss_async(h, &p), p, &r);
+  if (rc != GS_OK) return rc;
+  *mse = r.value;
+  *has = r.has;
   return GS_OK;
 }
 
@@ -996,24 +1015,11 @@ int gs_render(gs_trainer_t* h, const gs_camera* c, int32_t trainMode, int64_t of
 
 int gs_eval_cam_psnr(gs_trainer_t* h, int32_t ci, double* psnr) {
   if (!h || !psnr) return GS_E_ARG;
-  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
-  if (ci < 0 || ci >= (int)h->cams.size()) return fail(h, GS_E_ARG, "camera index out of range");
-  const CamMeta& m = h->cams[ci];
-  // the reference evaluates with the camera's training uniform (trainer.js:547)
-  const Cam cam = make_cam_meta(h, m, m.activeDeg);
-  HIPCK(hipMemsetAsync(h->evalStats, 0, 32, h->st));
-  int rc = run_raster(h, cam, 1, h->evalStats);
-  if (rc) return rc;
-  double s[4];
-  HIPCK(hipMemcpyAsync(s, h->evalStats, 32, hipMemcpyDeviceToHost, h->st));
-  // trainer.js:562, :570-571: the camera gradient rows are drained and the
-  // training stats window is reset too
-  HIPCK(hipMemsetAsync(h->camGrad, 0, (h->cams.size() + 1) * 32, h->st));
-  HIPCK(hipMemsetAsync(h->stats, 0, 32, h->st));
-  HIPCK(hipStreamSynchronize(h->st));
-  h->pixelsSeen = 0;
-  const double mse = s[0] / (std::max(1.0, s[2]) * 3.0);
-  *psnr = mse > 0 ? -10.0 * std::log10(mse) : INFINITY;
+  gs_pending* p = nullptr;
+  gs_readback_result r{};
+  const int rc = wait_pending(h, gs_eval_cam_psnr_async(h, ci, &p), p, &r);
+  if (rc != GS_OK) return rc;
+  *psnr = r.value;
   return GS_OK;
 }
 
@@ -1078,25 +1084,179 @@ int gs_read_cam_grads(gs_trainer_t* h, int32_t* out, int32_t count) {
 
 int gs_read_gaussians(gs_trainer_t* h, float* params, float* sh) {
   if (!h) return GS_E_ARG;
-  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
-  HIPCK(hipStreamSynchronize(h->st));
-  if (params) {
-    std::vector<float> planes((size_t)4 * h->cap * 4);
-    HIPCK(hipMemcpy(planes.data(), h->P, planes.size() * 4, hipMemcpyDeviceToHost));
-    planes_to_rows(planes, h->n, 16, h->cap, params);
+  gs_pending* p = nullptr;
+  gs_readback_result r{};
+  return wait_pending(h, gs_read_gaussians_async(h, params, sh, &p), p, &r);
+}
+
+// ---- asynchronous readbacks ----
+
+static int pending_new(gs_trainer* h, gs_pending::Kind kind, gs_pending** out) {
+  gs_pending* p = new gs_pending();
+  p->kind = kind;
+  p->device = h->o.device;
+  hipError_t e = hipEventCreateWithFlags(&p->ev, hipEventDisableTiming);
+  if (e == hipSuccess && kind != gs_pending::kGaussians) e = hipHostMalloc((void**)&p->stats, 32, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    gs_pending_free(p);
+    return hipfail(h, "readback setup", e);
   }
-  if (sh && h->shC && !h->mixed) {
-    std::vector<float> planes((size_t)h->shP4 * h->cap * 4);
-    HIPCK(hipMemcpy(planes.data(), h->SH, planes.size() * 4, hipMemcpyDeviceToHost));
-    planes_to_rows(planes, h->n, h->shC, h->cap, sh);
-  } else if (sh && h->shC) {
-    std::vector<uint16_t> b16((size_t)h->shQ8 * h->cap * 8);
-    HIPCK(hipMemcpy(b16.data(), h->SH, b16.size() * 2, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < h->n; i++)
-      for (int j = 0; j < h->shC; j++)
-        sh[i * h->shC + j] = gs_bf2f(b16[((size_t)(j / 8) * h->cap + i) * 8 + (j % 8)]);
+  *out = p;
+  return GS_OK;
+}
+
+int gs_read_loss_async(gs_trainer_t* h, gs_pending_t** out) {
+  if (!h || !out) return GS_E_ARG;
+  *out = nullptr;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  HIPCK(hipSetDevice(h->o.device));
+  gs_pending* p = nullptr;
+  int rc = pending_new(h, gs_pending::kLoss, &p);
+  if (rc != GS_OK) return rc;
+  // copy the window's stats, then reset the window; stats[3] (the budget
+  // drops) keeps counting across windows (trainer.js:621-627)
+  hipError_t e = hipMemcpyAsync(p->stats, h->stats, 32, hipMemcpyDeviceToHost, h->st);
+  if (e == hipSuccess) e = hipMemsetAsync(h->stats, 0, 24, h->st);
+  if (e == hipSuccess) e = hipEventRecord(p->ev, h->st);
+  if (e != hipSuccess) {
+    gs_pending_free(p);
+    return hipfail(h, "readLoss", e);
+  }
+  p->px = h->pixelsSeen;
+  h->pixelsSeen = 0;
+  *out = p;
+  return GS_OK;
+}
+
+int gs_eval_cam_psnr_async(gs_trainer_t* h, int32_t ci, gs_pending_t** out) {
+  if (!h || !out) return GS_E_ARG;
+  *out = nullptr;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  if (ci < 0 || ci >= (int)h->cams.size()) return fail(h, GS_E_ARG, "camera index out of range");
+  HIPCK(hipSetDevice(h->o.device));
+  const CamMeta& m = h->cams[ci];
+  // the reference evaluates with the camera's training uniform (trainer.js:547)
+  const Cam cam = make_cam_meta(h, m, m.activeDeg);
+  HIPCK(hipMemsetAsync(h->evalStats, 0, 32, h->st));
+  int rc = run_raster(h, cam, 1, h->evalStats);
+  if (rc) return rc;
+  gs_pending* p = nullptr;
+  rc = pending_new(h, gs_pending::kPsnr, &p);
+  if (rc != GS_OK) return rc;
+  hipError_t e = hipMemcpyAsync(p->stats, h->evalStats, 32, hipMemcpyDeviceToHost, h->st);
+  // trainer.js:562, :570-571: the camera gradient rows are drained and the
+  // training loss window is reset too
+  if (e == hipSuccess) e = hipMemsetAsync(h->camGrad, 0, (h->cams.size() + 1) * 32, h->st);
+  if (e == hipSuccess) e = hipMemsetAsync(h->stats, 0, 24, h->st);
+  if (e == hipSuccess) e = hipEventRecord(p->ev, h->st);
+  if (e != hipSuccess) {
+    gs_pending_free(p);
+    return hipfail(h, "evalCamPsnr", e);
+  }
+  h->pixelsSeen = 0;
+  *out = p;
+  return GS_OK;
+}
+
+int gs_read_gaussians_async(gs_trainer_t* h, float* params, float* sh, gs_pending_t** out) {
+  if (!h || !out) return GS_E_ARG;
+  *out = nullptr;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  HIPCK(hipSetDevice(h->o.device));
+  gs_pending* p = nullptr;
+  int rc = pending_new(h, gs_pending::kGaussians, &p);
+  if (rc != GS_OK) return rc;
+  p->n = h->n;
+  p->cap = h->cap;
+  p->shC = h->shC;
+  p->shP4 = h->shP4;
+  p->shQ8 = h->shQ8;
+  p->mixed = h->mixed;
+  p->dstP = params;
+  p->dstSH = h->shC ? sh : nullptr;
+  hipError_t e = hipSuccess;
+  if (params) {
+    const size_t bytes = (size_t)h->cap * 64;
+    e = hipHostMalloc(&p->hP, bytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->hP, h->P, bytes, hipMemcpyDeviceToHost, h->st);
+  }
+  if (e == hipSuccess && p->dstSH) {
+    const size_t bytes = (size_t)h->cap * 16 * (h->mixed ? h->shQ8 : h->shP4);
+    e = hipHostMalloc(&p->hSH, bytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->hSH, h->SH, bytes, hipMemcpyDeviceToHost, h->st);
+  }
+  if (e == hipSuccess) e = hipEventRecord(p->ev, h->st);
+  if (e != hipSuccess) {
+    gs_pending_free(p);
+    return hipfail(h, "readGaussians", e);
+  }
+  *out = p;
+  return GS_OK;
+}
+
+int gs_pending_wait(gs_pending_t* p, gs_readback_result* out) {
+  if (!p || !out) return GS_E_ARG;
+  memset(out, 0, sizeof(*out));
+  hipError_t e = hipSetDevice(p->device);
+  if (e == hipSuccess) e = hipEventSynchronize(p->ev);
+  if (e != hipSuccess) {
+    p->err = std::string("readback: ") + hipGetErrorString(e);
+    return GS_E_HIP;
+  }
+  const double* s = p->stats;
+  switch (p->kind) {
+    case gs_pending::kLoss:
+      // trainer.js:621-630
+      out->overflow = (int64_t)s[3];
+      if (p->px != 0 && s[2] != 0.0) {
+        out->has = 1;
+        out->value = s[0] / (s[2] * 3.0);
+      }
+      break;
+    case gs_pending::kPsnr: {
+      // trainer.js:565-572
+      const double mse = s[0] / (std::max(1.0, s[2]) * 3.0);
+      out->has = 1;
+      out->value = mse > 0 ? -10.0 * std::log10(mse) : INFINITY;
+      break;
+    }
+    case gs_pending::kGaussians: {
+      out->has = 1;
+      const int64_t n = p->n, cap = p->cap;
+      if (p->dstP) {
+        const float* pl = (const float*)p->hP;
+        for (int64_t i = 0; i < n; i++)
+          for (int j = 0; j < 16; j++) p->dstP[i * 16 + j] = pl[((size_t)(j / 4) * cap + i) * 4 + (j % 4)];
+      }
+      if (p->dstSH && !p->mixed) {
+        const float* pl = (const float*)p->hSH;
+        for (int64_t i = 0; i < n; i++)
+          for (int j = 0; j < p->shC; j++) p->dstSH[i * p->shC + j] = pl[((size_t)(j / 4) * cap + i) * 4 + (j % 4)];
+      } else if (p->dstSH) {
+        const uint16_t* b16 = (const uint16_t*)p->hSH;
+        for (int64_t i = 0; i < n; i++)
+          for (int j = 0; j < p->shC; j++)
+            p->dstSH[i * p->shC + j] = gs_bf2f(b16[((size_t)(j / 8) * cap + i) * 8 + (j % 8)]);
+      }
+      break;
+    }
   }
   return GS_OK;
+}
+
+const char* gs_pending_error(gs_pending_t* p) { return p ? p->err.c_str() : "null readback"; }
+
+void gs_pending_free(gs_pending_t* p) {
+  if (!p) return;
+  (void)hipSetDevice(p->device);
+  if (p->ev) {
+    (void)hipEventSynchronize(p->ev);   // a copy in flight still targets the pinned buffers
+    (void)hipEventDestroy(p->ev);
+  }
+  if (p->stats) (void)hipHostFree(p->stats);
+  if (p->hP) (void)hipHostFree(p->hP);
+  if (p->hSH) (void)hipHostFree(p->hSH);
+  delete p;
 }
 
 int gs_set_holdout(gs_trainer_t* h, int32_t ci) {
@@ -1134,19 +1294,23 @@ int gs_get_info(gs_trainer_t* h, gs_info* out) {
   out->entriesCap = h->entriesCap;
   out->maxTiles = h->maxTiles;
   out->horizon = (float)h->horizon;
-  if (h->ready) {
-    uint32_t nk = 0, nv = 0;
-    uint64_t tk = 0;
-    double s[4];
-    HIPCK(hipStreamSynchronize(h->st));
-    HIPCK(hipMemcpy(&nk, h->numKeys, 4, hipMemcpyDeviceToHost));
-    HIPCK(hipMemcpy(&nv, h->numVisible, 4, hipMemcpyDeviceToHost));
-    out->lastVisible = nv;
-    HIPCK(hipMemcpy(&tk, h->totalKeys, 8, hipMemcpyDeviceToHost));
-    HIPCK(hipMemcpy(s, h->stats, 32, hipMemcpyDeviceToHost));
-    out->lastKeys = nk;
-    out->overflowKeys = (int64_t)(h->overflowTotal + s[3]);
-  }
+  return GS_OK;
+}
+
+int gs_read_step_stats(gs_trainer_t* h, gs_step_stats* out) {
+  if (!h || !out) return GS_E_ARG;
+  memset(out, 0, sizeof(*out));
+  if (!h->ready) return GS_OK;
+  HIPCK(hipSetDevice(h->o.device));
+  uint32_t nk = 0, nv = 0;
+  double s[4];
+  HIPCK(hipMemcpyAsync(&nk, h->numKeys, 4, hipMemcpyDeviceToHost, h->st));
+  HIPCK(hipMemcpyAsync(&nv, h->numVisible, 4, hipMemcpyDeviceToHost, h->st));
+  HIPCK(hipMemcpyAsync(s, h->stats, 32, hipMemcpyDeviceToHost, h->st));
+  HIPCK(hipStreamSynchronize(h->st));
+  out->lastKeys = nk;
+  out->lastVisible = nv;
+  out->overflowKeys = (int64_t)s[3];
   return GS_OK;
 }
 
@@ -1168,9 +1332,16 @@ int gs_refine(gs_trainer_t* h, const gs_refine_opts* ro, gs_refine_result* out) 
   HIPCK(hipStreamSynchronize(h->st));
   const int64_t n = h->n;
 
-  // telemetry (trainer.js:784-792): most keys in one tile of the last raster,
-  // and the cumulative entry-budget drops
-  if (h->lastNumTiles > 0) {
+  // telemetry (trainer.js:780-792): most keys in one tile of the last
+  // raster, counted before the entry budget (the reference reads the raw
+  // per-tile counts), and the cumulative entry-budget drops
+  if (h->lastNumTiles > 0 && h->lastSegPath) {
+    std::vector<uint32_t> tc((size_t)h->lastNumTiles);
+    HIPCK(hipMemcpy(tc.data(), h->sort.tileCnt, tc.size() * 4, hipMemcpyDeviceToHost));
+    for (uint32_t c : tc) out->maxTile = std::max<int64_t>(out->maxTile, c);
+  } else if (h->lastNumTiles > 0) {
+    // the radix strategies hold no per-tile totals; their ranges lack only
+    // keys past the budget
     std::vector<uint32_t> rg(2 * (size_t)h->lastNumTiles);
     HIPCK(hipMemcpy(rg.data(), h->ranges, rg.size() * 4, hipMemcpyDeviceToHost));
     for (int t = 0; t < h->lastNumTiles; t++) out->maxTile = std::max<int64_t>(out->maxTile, rg[2 * t + 1] - rg[2 * t]);
@@ -1178,7 +1349,7 @@ int gs_refine(gs_trainer_t* h, const gs_refine_opts* ro, gs_refine_result* out) 
   {
     double s[4];
     HIPCK(hipMemcpy(s, h->stats, 32, hipMemcpyDeviceToHost));
-    out->overflow = (int64_t)(h->overflowTotal + s[3]);
+    out->overflow = (int64_t)s[3];
   }
 
   // opacity logit and log-scales of every Gaussian (16 B each)

@@ -13,6 +13,14 @@
 // of a canvas context (Node has no canvas); refine(rng) seeds the reference's
 // mulberry32 from one draw of rng. Syntax stays ES2019 (no ?? or ?.) for
 // Node 12.
+//
+// As in the reference, stepOnce only enqueues, the fields callers read every
+// frame (iter, n, cap, lastCam, holdout, entryOverflowTiles) are host state
+// and never wait for the GPU, and readLoss / readGaussians / evalCamPsnr
+// return Promises that settle once their copy lands (the addon waits on a
+// worker thread). A GPU error (code GS_E_HIP) settles device.lost with
+// reason 'unknown', the signal the reference's watchLost turns into onLost
+// (src/gpu/context.js:43-56); destroy() settles it with 'destroyed'.
 import { createRequire } from 'module';
 
 const require = createRequire(import.meta.url);
@@ -42,6 +50,7 @@ export class GSTrainer {
     if (typeof o.sortMode === 'string') o.sortMode = SORT[o.sortMode];
     if (typeof o.precision === 'string') o.precision = PRECISION[o.precision];
     this._h = addon.create(o);
+    this.entryOverflowTiles = 0;   // trainer.js:627, updated by readLoss
     this.tileGrad = opts.tileGrad === undefined ? true : opts.tileGrad;
     this.shDeg = opts.shDeg === undefined ? 2 : opts.shDeg;
     this.shK = this.shDeg > 0 ? shRestCoefs(this.shDeg) : 0;
@@ -51,17 +60,51 @@ export class GSTrainer {
     this.growLimit = 0;   // trainer.js:777: soft ceiling below cap (0 = cap)
     this.excluded = null;
     const self = this;
+    let settleLost = null;
+    this._lostSettled = false;
+    this._settleLost = (info) => {
+      if (self._lostSettled) return;
+      self._lostSettled = true;
+      settleLost(info);
+    };
     // session.js:520, :542 wait on the queue between frames
     this.device = {
-      queue: { onSubmittedWorkDone() { addon.sync(self._h); return Promise.resolve(); } },
+      queue: { onSubmittedWorkDone() { return self._guard(() => { addon.sync(self._h); }); } },
+      lost: new Promise((resolve) => { settleLost = resolve; }),
       destroy() { self.destroy(); },
     };
+  }
+
+  // Runs an addon call; a device error settles device.lost before it
+  // propagates. Returns a Promise for Promise results.
+  _guard(fn) {
+    const onErr = (e) => {
+      if (e && e.code === 'GS_E_HIP') this._settleLost({ reason: 'unknown', message: e.message });
+      throw e;
+    };
+    let r;
+    try {
+      r = fn();
+    } catch (e) {
+      onErr(e);
+    }
+    return Promise.resolve(r).catch(onErr);
+  }
+
+  _call(fn) {
+    try {
+      return fn();
+    } catch (e) {
+      if (e && e.code === 'GS_E_HIP') this._settleLost({ reason: 'unknown', message: e.message });
+      throw e;
+    }
   }
 
   /** trainer.js:100-355 */
   setup(gaussians, cams, images, maxViewW, maxViewH, sceneRadius) {
     const sh = gaussians.sh || null;
-    addon.setup(this._h, gaussians.data, gaussians.n, sh, cams, images, maxViewW | 0, maxViewH | 0, sceneRadius);
+    this._call(() => addon.setup(this._h, gaussians.data, gaussians.n, sh, cams, images, maxViewW | 0, maxViewH | 0,
+                                 sceneRadius));
     this.cams = cams;
     this.sceneRadius = sceneRadius;
     let total = 0;
@@ -76,6 +119,7 @@ export class GSTrainer {
     this._iter = 0;   // host-side copy of iter: stepOnce reads it without a device sync
   }
 
+  // host fields of the trainer: no GPU wait
   get iter() { return addon.info(this._h).iter; }
   set iter(v) { addon.setIter(this._h, v); this._iter = v; }
   get n() { return addon.info(this._h).n; }
@@ -83,7 +127,6 @@ export class GSTrainer {
   get lastCam() { return addon.info(this._h).lastCam; }
   get holdout() { return addon.info(this._h).holdout; }
   set holdout(ci) { addon.setHoldout(this._h, ci); }
-  get entryOverflowTiles() { return addon.info(this._h).overflowKeys; }
 
   /** trainer.js:398-471; `excluded` is a Set of camera indices as in :403 */
   stepOnce(cam = -1) {
@@ -91,7 +134,7 @@ export class GSTrainer {
       addon.setExcluded(this._h, this.excluded ? Array.from(this.excluded) : []);
       this._excludedSent = this.excluded;
     }
-    addon.step(this._h, cam);
+    this._call(() => addon.step(this._h, cam));
     this._iter++;
     // trainer.js:468-470, :533-535: a camera Adam step just moved the poses,
     // the shared focal and the exposures; mirror them into camMeta
@@ -105,15 +148,19 @@ export class GSTrainer {
     }
   }
 
-  /** trainer.js:615-631: MSE per valid pixel-channel since the last call */
-  async readLoss() {
-    return addon.readLoss(this._h);
+  /** trainer.js:615-631: MSE per valid pixel-channel since the last call
+   *  (null when no pixel was seen); also updates entryOverflowTiles */
+  readLoss() {
+    return this._guard(() => addon.readLossAsync(this._h)).then((r) => {
+      this.entryOverflowTiles = r.overflow;
+      return r.mse;
+    });
   }
 
   /** trainer.js:634-655: renders into `out` (Float32Array w*h*4) */
   renderView(camParams, out, trainMode = 0, offset = 0) {
     const buf = out || new Float32Array(camParams.w * camParams.h * 4);
-    addon.render(this._h, camParams, trainMode, offset, buf);
+    this._call(() => addon.render(this._h, camParams, trainMode, offset, buf));
     return buf;
   }
 
@@ -124,25 +171,24 @@ export class GSTrainer {
   }
 
   /** trainer.js:797-817 */
-  async readGaussians() {
+  readGaussians() {
     const n = this.n;
     const data = new Float32Array(n * STRIDE);
     const sh = this.shK ? new Float32Array(n * this.shK * 3) : null;
-    addon.readGaussians(this._h, data, sh);
-    return { data, n, sh, shK: this.shK };
+    return this._guard(() => addon.readGaussiansAsync(this._h, data, sh)).then(() => ({ data, n, sh, shK: this.shK }));
   }
 
   /** trainer.js:577-579 */
-  async evalCamPsnr(ci) {
+  evalCamPsnr(ci) {
     this.pixelsSeen = 0;
-    return addon.evalCamPsnr(this._h, ci);
+    return this._guard(() => addon.evalCamPsnrAsync(this._h, ci));
   }
 
   /** trainer.js:583-612: PSNR after test-time refinement of camera ci's
    *  pose (Gaussians frozen); the stored camera is unchanged */
-  async evalCamPsnrRefined(ci, steps = 200) {
+  evalCamPsnrRefined(ci, steps = 200) {
     this.pixelsSeen = 0;
-    return addon.evalCamPsnrRefined(this._h, ci, steps);
+    return this._guard(() => addon.evalCamPsnrRefined(this._h, ci, steps));
   }
 
   /** the camera gradient rows (bufCamGrad, trainer.js:173) as an Int32Array */
@@ -159,12 +205,12 @@ export class GSTrainer {
    *  reference; growLimit is the trainer field. */
   async refine(rng = Math.random) {
     const o = this.opts;
-    const r = addon.refine(this._h, {
+    const r = this._call(() => addon.refine(this._h, {
       seed: Math.floor(rng() * 4294967296) >>> 0,
       growRate: o.growRate === undefined ? -1 : o.growRate,
       growUntil: o.growUntil === undefined ? -1 : o.growUntil,
       growLimit: this.growLimit || 0,
-    });
+    }));
     return { moved: r.moved, grown: r.grown, n: r.n, maxTile: r.maxTile, overflow: r.overflow };
   }
 
@@ -175,8 +221,12 @@ export class GSTrainer {
 
   info() { return addon.info(this._h); }
 
+  /** the last raster's key and visible counts, and the budget drops (waits for the GPU) */
+  stepStats() { return this._call(() => addon.stepStats(this._h)); }
+
+  /** frees the device state now (device.destroy(), gradcheck.js:184) */
   destroy() {
-    // the addon frees the trainer when the handle is collected; sync first
-    addon.sync(this._h);
+    addon.destroy(this._h);
+    this._settleLost({ reason: 'destroyed', message: 'GSTrainer.destroy()' });
   }
 }

@@ -49,9 +49,16 @@ class GsImage(C.Structure):
 class GsInfo(C.Structure):
     _fields_ = [("n", C.c_int64), ("cap", C.c_int64), ("iter", C.c_int64), ("shDeg", C.c_int32),
                 ("shK", C.c_int32), ("stride", C.c_int32), ("numCams", C.c_int32),
-                ("lastCam", C.c_int32), ("holdout", C.c_int32), ("lastKeys", C.c_int64), ("lastVisible", C.c_int64),
-                ("overflowKeys", C.c_int64), ("entriesCap", C.c_int64), ("maxTiles", C.c_int32),
+                ("lastCam", C.c_int32), ("holdout", C.c_int32), ("entriesCap", C.c_int64), ("maxTiles", C.c_int32),
                 ("horizon", C.c_float)]
+
+
+class GsStepStats(C.Structure):
+    _fields_ = [("lastKeys", C.c_int64), ("lastVisible", C.c_int64), ("overflowKeys", C.c_int64)]
+
+
+class GsReadbackResult(C.Structure):
+    _fields_ = [("value", C.c_double), ("has", C.c_int32), ("overflow", C.c_int64)]
 
 
 class GsDebugOut(C.Structure):
@@ -84,6 +91,13 @@ SIGNATURES = {
     "gs_set_excluded": ([C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
     "gs_set_iter": ([C.c_void_p, C.c_int64], C.c_int),
     "gs_get_info": ([C.c_void_p, C.c_void_p], C.c_int),
+    "gs_read_step_stats": ([C.c_void_p, C.c_void_p], C.c_int),
+    "gs_read_loss_async": ([C.c_void_p, C.c_void_p], C.c_int),
+    "gs_eval_cam_psnr_async": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
+    "gs_read_gaussians_async": ([C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
+    "gs_pending_wait": ([C.c_void_p, C.c_void_p], C.c_int),
+    "gs_pending_error": ([C.c_void_p], C.c_char_p),
+    "gs_pending_free": ([C.c_void_p], None),
     "gs_sync": ([C.c_void_p], C.c_int),
     "gs_refine": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "gs_step_batch": ([C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
@@ -191,8 +205,15 @@ class GSTrainer:
     def setup(self, gaussians, cams, images, maxViewW, maxViewH, sceneRadius):
         data = np.ascontiguousarray(gaussians["data"], np.float32).reshape(-1, 16)
         n = int(gaussians.get("n", data.shape[0]))
+        if data.shape[0] < n:
+            raise GsError("setup: gaussians.data shorter than n * 16")
         sh = gaussians.get("sh")
-        sh = None if sh is None else np.ascontiguousarray(sh, np.float32).reshape(n, -1)
+        if sh is not None:
+            sh = np.ascontiguousarray(sh, np.float32).reshape(-1)
+            if sh.size != n * 3 * self.shK:
+                raise GsError(f"setup: gaussians.sh holds {sh.size} floats; SH degree {self.shDeg} "
+                              f"needs n * 3 * {self.shK} = {n * 3 * self.shK}")
+            sh = sh.reshape(n, -1) if self.shK else None
         self._cams = (GsCamera * len(cams))(*[make_camera(c) for c in cams])
         self._rgb = [np.ascontiguousarray(im["rgb"], np.float32).reshape(-1) for im in images]
         ims = (GsImage * len(images))()
@@ -289,9 +310,37 @@ class GSTrainer:
         return self.info().n
 
     def info(self):
+        """Host fields (never waits for the GPU)."""
         i = GsInfo()
         self._ck(lib().gs_get_info(self._h, C.byref(i)))
         return i
+
+    def step_stats(self):
+        """The last raster's key and visibility counts and the cumulative
+        entry-budget drops (waits for the GPU)."""
+        s = GsStepStats()
+        self._ck(lib().gs_read_step_stats(self._h, C.byref(s)))
+        return s
+
+    def read_loss_async(self):
+        """Enqueues the loss readback; returns a Pending whose wait() gives
+        (mse or None, cumulative budget drops)."""
+        p = C.c_void_p()
+        self._ck(lib().gs_read_loss_async(self._h, C.byref(p)))
+        return Pending(p, lambda r: (r.value if r.has else None, r.overflow))
+
+    def eval_cam_psnr_async(self, ci):
+        p = C.c_void_p()
+        self._ck(lib().gs_eval_cam_psnr_async(self._h, int(ci), C.byref(p)))
+        return Pending(p, lambda r: r.value)
+
+    def read_gaussians_async(self):
+        info = self.info()
+        data = np.zeros((info.n, 16), np.float32)
+        sh = np.zeros((info.n, 3 * self.shK), np.float32) if self.shK else None
+        p = C.c_void_p()
+        self._ck(lib().gs_read_gaussians_async(self._h, _p(data), _p(sh), C.byref(p)))
+        return Pending(p, lambda r: {"data": data, "n": info.n, "sh": sh, "shK": self.shK}, keep=(data, sh))
 
     def sync(self):
         self._ck(lib().gs_sync(self._h))
@@ -358,6 +407,31 @@ class GSTrainer:
         self._ck(lib().gs_profile_read(self._h, buf, 64, C.byref(cnt)))
         return {buf[i].name.decode(): {"ms": buf[i].total_ms, "launches": buf[i].launches, "bytes": buf[i].bytes}
                 for i in range(min(cnt.value, 64))}
+
+
+class Pending:
+    """An enqueued readback (gs_*_async); wait() blocks until it lands."""
+
+    def __init__(self, handle, finish, keep=()):
+        self._p = handle
+        self._finish = finish
+        self._keep = keep     # the destination arrays stay alive until the wait
+
+    def wait(self):
+        if self._p is None:
+            raise GsError("readback already waited")
+        r = GsReadbackResult()
+        rc = lib().gs_pending_wait(self._p, C.byref(r))
+        msg = lib().gs_pending_error(self._p).decode()
+        lib().gs_pending_free(self._p)
+        self._p = None
+        if rc != 0:
+            raise GsError(msg or f"error {rc}")
+        return self._finish(r)
+
+    def __del__(self):
+        if getattr(self, "_p", None) is not None:
+            lib().gs_pending_free(self._p)
 
 
 def createTrainer(**opts):

@@ -39,7 +39,8 @@ def test_default_opts_are_the_references():
     assert o.radClamp == 1.0 and o.shDeg == 2
     assert o.anisoReg == pytest.approx(0.005) and o.opacityReg == pytest.approx(0.01)
     assert o.shLr == pytest.approx(1.25e-4) and o.minScale == pytest.approx(1e-4)
-    assert o.maxIters == 60000 and o.maxSplats == 600000 and o.capMult == 4.0
+    # maxSplats unset: capacity ceiling 600000 and the 12M key budget (trainer.js:48-49, :105-107)
+    assert o.maxIters == 60000 and o.maxSplats == 0 and o.entriesCap == 0 and o.capMult == 4.0
     assert o.loss == gsabi.GS_LOSS_CHARBONNIER and o.gradAccum == gsabi.GS_ACCUM_FIXED
     assert o.camOpt == 0 and o.expComp == 0          # trainer.js:468, :511: both off by default
 
@@ -57,6 +58,8 @@ def test_struct_layouts_match_the_header(tmp_path):
         "gs_refine_opts": [f[0] for f in gsabi.GsRefineOpts._fields_],
         "gs_refine_result": [f[0] for f in gsabi.GsRefineResult._fields_],
         "gs_cam_state": [f[0] for f in gsabi.GsCamState._fields_],
+        "gs_step_stats": [f[0] for f in gsabi.GsStepStats._fields_],
+        "gs_readback_result": [f[0] for f in gsabi.GsReadbackResult._fields_],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for s, fs in fields.items():
@@ -71,7 +74,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     py = {"gs_opts": gsabi.GsOpts, "gs_camera": gsabi.GsCamera, "gs_image": gsabi.GsImage,
           "gs_info": gsabi.GsInfo, "gs_debug_out": gsabi.GsDebugOut, "gs_kernel_time": gsabi.GsKernelTime,
           "gs_refine_opts": gsabi.GsRefineOpts, "gs_refine_result": gsabi.GsRefineResult,
-          "gs_cam_state": gsabi.GsCamState}
+          "gs_cam_state": gsabi.GsCamState, "gs_step_stats": gsabi.GsStepStats,
+          "gs_readback_result": gsabi.GsReadbackResult}
     for s, cls in py.items():
         assert int(got[s]) == C.sizeof(cls), s
         for f, _ in cls._fields_:

@@ -98,7 +98,7 @@ def test_config5_stress_properties():
     for ci in range(len(cams)):
         d = a.step_debug(ci, want=("cnt", "tileRanges"))
         b.stepOnce(ci)
-        info = a.info()
+        info = a.step_stats()
         total = int(d["cnt"].astype(np.int64).sum())
         print(f"view {ci}: {total} keys, {info.lastVisible} visible")
         # the stress regime: ~281M keys on the initial view (the count the
@@ -110,7 +110,7 @@ def test_config5_stress_properties():
         tr = d["tileRanges"].astype(np.int64)
         assert tr[0, 0] == 0 and tr[-1, 1] == total
         assert np.all(tr[1:, 0] == tr[:-1, 1]) and np.all(tr[:, 1] >= tr[:, 0])
-    assert a.info().overflowKeys == 0
+    assert a.step_stats().overflowKeys == 0
     oa, ob = a.readGaussians(), b.readGaussians()
     np.testing.assert_array_equal(oa["data"].view(np.uint32), ob["data"].view(np.uint32))
     np.testing.assert_array_equal(oa["sh"].view(np.uint32), ob["sh"].view(np.uint32))
@@ -135,3 +135,18 @@ def test_config2_full_size_mixed_step_bit_exact():
     images, W, H = _targets(gp, gsh, cams, 2, n)
     _run_pair(p, sh, cams, images, W, H, 8.0, {"posLr": 1.6e-4}, 2, steps=1, start_iter=2000,
               precision=gsabi.GS_PREC_MIXED)
+
+
+@pytest.mark.gpu
+def test_config2_full_size_f32_step_bit_exact():
+    """Config 2 as the metric's plain-FP32 single-view step: 1M Gaussians,
+    1600x900, SH degree 2 with all bands active, position lr 1.6e-4, one f32
+    training step on view 0. Every intermediate and the updated params and
+    SH match the f32 restatement bit for bit."""
+    n = 1_000_000
+    p, sh, cams = scenes.config2(n=n, seed=0)
+    gp, gsh, _ = scenes.config2(n=n, seed=1)
+    cams = [dict(cams[0], imgIdx=0)]
+    images, W, H = _targets(gp, gsh, cams, 2, n)
+    _run_pair(p, sh, cams, images, W, H, 8.0, {"posLr": 1.6e-4}, 2, steps=1, start_iter=2000,
+              precision=gsabi.GS_PREC_F32)

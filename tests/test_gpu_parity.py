@@ -404,8 +404,12 @@ def test_entry_budget_drops_whole_tiles_as_the_reference():
     img, T, end = st.render_fwd(cam, proj, ids2, ts2)
     np.testing.assert_array_equal(g["img"].view(np.uint32), img.astype(np.float32).view(np.uint32))
     np.testing.assert_array_equal(g["end"], end)
-    info = t.info()
+    info = t.step_stats()
     assert info.lastKeys == len(ids2) and info.overflowKeys == dropped
+    # refine's telemetry reads the per-tile counts before the budget
+    # (trainer.js:780-789), so it reports the largest tile even when dropped
+    r = t.refine(seed=1)
+    assert r["maxTile"] == int(np.diff(ts).max()) and r["overflow"] == dropped
     t.destroy()
 
 
@@ -416,7 +420,7 @@ def test_radix_sort_budget_drops_keys_in_sort_order():
     t = _trainer(p, sh, cams, images, W, H, radius, {}, shDeg=2, entriesCap=500,
                  sortMode=gsabi.GS_SORT_DEPTH_THEN_TILE)
     t.stepOnce(0)
-    info = t.info()
+    info = t.step_stats()
     assert info.lastKeys == 500
     assert info.overflowKeys > 0
     t.destroy()
@@ -439,8 +443,15 @@ def test_l1_dssim_loss_against_f64():
     restatement on the GPU's own forward image. f32 blur sums lose digits
     in var = E[x^2] - mu^2, so dL/dC is held to 3e-4 of its largest
     magnitude (and 1e-5 relative on the loss). The backward pass fed with
-    the GPU's dL/dC stays bit-exact."""
+    the GPU's dL/dC stays bit-exact. The target carries an invalid border
+    (4 rows and 6 columns) and a few invalid pixels inside, which enter the
+    SSIM windows as zeros and take no gradient."""
     p, sh, cams, images, W, H, radius = config1_setup(n=8000)
+    rgb = images[0]["rgb"].reshape(H, W, 3).copy()
+    rgb[:4, :, 0] = -1.0
+    rgb[:, -6:, 0] = -1.0
+    rgb[100:103, 50:52, 0] = -1.0
+    images = [dict(images[0], rgb=rgb.reshape(-1))]
     t = _trainer(p, sh, cams, images, W, H, radius, {}, shDeg=2, loss=gsabi.GS_LOSS_L1_DSSIM, ssimLambda=0.2)
     t.iter = 2500
     g = t.step_debug(0, want=("img", "T", "end", "gC", "accum", "stats", "ids", "tileRanges"))
@@ -454,6 +465,7 @@ def test_l1_dssim_loss_against_f64():
     assert err.max() <= 3e-4 * scale, (err.max(), scale)
     assert np.median(err) <= 1e-6 * scale
     np.testing.assert_array_equal(g["gC"][:, 3] != 0, valid != 0)
+    assert (valid == 0).sum() > 4 * W and not g["gC"][valid == 0].any()
     np.testing.assert_allclose(g["stats"][:3], stats64, rtol=1e-5)
     # backward given the same dL/dC: deterministic fixed-point sums, bit-exact
     st32 = O.Stage(O.make_opts(shDeg=2), "f32")

@@ -57,3 +57,50 @@ def test_node_trainer_matches_ctypes(tmp_path):
     np.testing.assert_array_equal(js_params.view(np.uint32), data.view(np.uint32))
     np.testing.assert_array_equal(js_img.view(np.uint32), img.view(np.uint32))
     assert res["loss"] == loss and res["psnr"] == psnr
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(NODE is None, reason="node is not installed")
+def test_node_async_readback_and_host_fields(tmp_path):
+    """Config 1 (20k Gaussians, 320x240) from Node: 100 stepOnce calls are
+    only enqueued; 100 reads of iter/n/cap do not wait for the GPU;
+    readLoss() returns a pending Promise while those steps run, and the event
+    loop keeps turning until it settles; the results equal the ctypes
+    path's; destroy() frees at once and a new trainer works right after."""
+    from helpers import config1_setup
+    p, sh, cams, images, W, H, radius = config1_setup()
+    steps = 100
+    (tmp_path / "params.f32").write_bytes(p.astype(np.float32).tobytes())
+    (tmp_path / "sh.f32").write_bytes(sh.astype(np.float32).tobytes())
+    for i, im in enumerate(images):
+        (tmp_path / f"img{i}.f32").write_bytes(np.asarray(im["rgb"], np.float32).tobytes())
+    cfg = {"opts": {"shDeg": 2}, "cams": cams, "images": [{"tw": im["tw"], "th": im["th"]} for im in images],
+           "maxW": W, "maxH": H, "radius": radius, "steps": steps, "sh": True}
+    (tmp_path / "cfg.json").write_text(json.dumps(cfg))
+    out = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "async_readback.mjs"), str(tmp_path)],
+                         stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
+    assert out.returncode == 0, out.stdout
+    res = json.loads(out.stdout.strip().splitlines()[-1])
+    print(res)
+    assert res["iter"] == steps + 1
+    assert res["pendingAfterCall"] is True
+    # the enqueue returned long before the GPU finished the queued steps
+    assert res["waitMs"] > res["readLossCallMs"]
+    assert res["iterReadsMs"] < 20.0
+    assert res["lost"] == "destroyed" and "destroyed" in res["afterDestroy"]
+    t = gsabi.GSTrainer(shDeg=2)
+    t.setup({"data": p, "n": p.shape[0], "sh": sh}, cams, images, W, H, radius)
+    t.stepOnce(0)
+    first = t.readLoss()
+    for _ in range(steps):
+        t.stepOnce(0)
+    loss = t.readLoss()
+    data = t.readGaussians()["data"]
+    psnr = t.evalCamPsnr(0)
+    t.destroy()
+    js = np.frombuffer((tmp_path / "out_params.f32").read_bytes(), np.float32).reshape(-1, 16)
+    np.testing.assert_array_equal(js.view(np.uint32), data.view(np.uint32))
+    # the loss statistics are double atomics (summation order varies)
+    for a, b in ((res["first"], first), (res["loss"], loss), (res["loss2"], first)):
+        assert abs(a - b) <= 1e-12 * abs(b)
+    assert abs(res["psnr"] - psnr) <= 1e-9 * abs(psnr)

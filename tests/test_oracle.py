@@ -378,6 +378,41 @@ def test_l1ssim_identity_and_fd(oracle_lib):
         assert abs(fd - gC[pi, c]) <= 1e-6 * max(1, abs(fd))
 
 
+def test_l1ssim_invalid_border_fd(oracle_lib):
+    """With invalid target pixels (an undistortion border of 3 pixels and a
+    scattered few inside), dL/dC at every pixel is the derivative of the
+    reported loss: central differences agree at valid pixels next to the
+    border, and at invalid pixels both are zero (what is rendered there
+    does not enter the loss)."""
+    W, H = 24, 19
+    cam = O.make_cam(_cam(W=W, H=H))
+    st = O.Stage(O.make_opts(shDeg=0), "f64")
+    rng = np.random.default_rng(4)
+    rgb = rng.random((H, W, 3)).astype(np.float32)
+    rgb[:3, :, 0] = -1.0
+    rgb[:, -3:, 0] = -1.0
+    rgb[10, 7, 0] = -1.0
+    rgb[12, 15, 0] = -1.0
+    tgt = O.pack_targets(rgb.reshape(-1, 3))
+    t = np.stack([(tgt >> (8 * c)) & 255 for c in range(3)], 1) / 255.0
+    img = np.zeros((W * H, 4))
+    img[:, :3] = np.clip(t + 0.1 * rng.normal(size=t.shape), 0, 1)
+    gC, valid, stats = st.loss_l1ssim(cam, img, tgt, 0.2)
+    assert valid.sum() == stats[2] < W * H
+    h = 1e-6
+    probes = [(3 * W + 5, 0), (4 * W + 20, 1), (10 * W + 8, 2), (11 * W + 7, 0), (12 * W + 14, 1),
+              (0, 0), (2 * W + 9, 2), (10 * W + 7, 1), (5 * W + 23, 0)]
+    for pi, c in probes:
+        ip = img.copy()
+        ip[pi, c] += h
+        im = img.copy()
+        im[pi, c] -= h
+        fd = (st.loss_l1ssim(cam, ip, tgt, 0.2)[2][1] - st.loss_l1ssim(cam, im, tgt, 0.2)[2][1]) / (2 * h)
+        assert abs(fd - gC[pi, c]) <= 1e-6 * max(1, abs(fd)), (pi, c, fd, gC[pi, c])
+        if not valid[pi]:
+            assert fd == 0.0 and gC[pi, c] == 0.0
+
+
 # ---------------------------------------------------------------- gradients (FD)
 def _rig_stage(sh_deg=0):
     params, cams, images, W, H, radius, opts = scenes.gradcheck_rig()
@@ -517,3 +552,46 @@ def test_adam_closed_form(oracle_lib):
     np.testing.assert_allclose(m[:, live], md[:, live], rtol=1e-5, atol=1e-7)
     assert np.all(P[:, ~live] == P0[:, ~live]) and np.all(m[:, ~live] == m0[:, ~live])
     assert np.isfinite(P).all()
+
+
+def test_sh_adam_closed_form(oracle_lib):
+    """orc_sh_adam_f32 (shaders.js:952-982) against textbook Kingma-Ba Adam
+    in double: the SH learning rate, bias correction through 1 - beta^t, the
+    NaN/Inf guard (a non-finite gradient counts as zero) and no clamps."""
+    rng = np.random.default_rng(9)
+    count = 4096
+    sh = rng.normal(0, 0.05, count).astype(np.float32)
+    G = rng.normal(0, 1e-2, count).astype(np.float32)
+    G[::97] = np.nan
+    G[5::101] = np.inf
+    G[7::103] = -np.inf
+    G[11::89] = np.float32(3e18)       # above the guard's 1e18: dropped
+    m = rng.normal(0, 1e-3, count).astype(np.float32)
+    v = np.abs(rng.normal(0, 1e-4, count)).astype(np.float32)
+    for t, shLr in ((1, 1.25e-4), (13, 3e-3), (2500, 1.25e-4)):
+        hp = O.adam_hp([0.0] * 16, t=t, shLr=shLr)
+        s1, m1, v1 = sh.copy(), m.copy(), v.copy()
+        O.sh_adam(hp, s1, G, m1, v1)
+        # the betas are the uniform's f32 values (0.9f, 0.999f); 1 - beta is
+        # exact in f32 (Sterbenz), so the double textbook uses the same constants
+        b1, b2 = float(np.float32(0.9)), float(np.float32(0.999))
+        Gd = np.where(np.abs(G) < 1e18, G, 0).astype(np.float64)
+        md = b1 * m.astype(np.float64) + (1 - b1) * Gd
+        vd = b2 * v.astype(np.float64) + (1 - b2) * Gd ** 2
+        upd = np.float32(shLr) * (md / (1 - b1 ** t)) / (np.sqrt(vd / (1 - b2 ** t)) + 1e-15)
+        sd = sh - upd
+        # f32 rounding of each product and sum: a few ulp of the terms' size
+        # (the two terms of m can cancel)
+        tm = b1 * np.abs(m.astype(np.float64)) + (1 - b1) * np.abs(Gd)
+        tv = b2 * v.astype(np.float64) + (1 - b2) * Gd ** 2
+        assert np.all(np.abs(m1 - md) <= 4e-7 * tm)
+        assert np.all(np.abs(v1 - vd) <= 4e-7 * tv)
+        # the update itself, to 1e-4 relative plus the rounding of the f32
+        # subtraction (one ulp of the coefficient)
+        du = sh.astype(np.float64) - s1.astype(np.float64)
+        assert np.all(np.abs(du - upd) <= 1e-4 * np.abs(upd) + np.spacing(np.abs(sh)).astype(np.float64))
+        np.testing.assert_allclose(s1, sd, rtol=1e-6, atol=1e-9)
+        assert np.isfinite(s1).all() and np.isfinite(m1).all() and np.isfinite(v1).all()
+        # guarded slots: moments decay as with a zero gradient
+        bad = ~(np.abs(G) < 1e18)
+        np.testing.assert_array_equal(m1[bad], (np.float32(0.9) * m[bad] + np.float32(0.1) * np.float32(0)))

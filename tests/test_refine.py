@@ -110,6 +110,120 @@ def test_refine_needs_sixteen_donors():
     assert R.refine(st, n, cap, 0, 60000, R.make_rng(1)) == (0, 0, n)
 
 
+def _scripted(values):
+    """An rng that returns the given values in order (and records how many
+    were drawn), standing in for makeRng so the test chooses every random
+    decision itself."""
+    vals = list(values)
+    drawn = []
+
+    def rng():
+        assert vals, "the refine drew more random numbers than the script holds"
+        drawn.append(vals.pop(0))
+        return drawn[-1]
+    rng.left = vals
+    return rng
+
+
+def test_refine_hand_worked_donor_choice():
+    """A case worked by hand from trainer.js:687-779, with the random stream
+    scripted so that each choice is known in advance.
+
+    n = 100, cap = 200. Gaussians 0-79 are donors (opacity logit 3, sigma
+    0.95 > 0.4), 80-97 neither (logit -2, sigma 0.12), 98-99 dead (logit -6,
+    sigma 0.0025 < 0.02; moveCap = ceil(5) holds both). Donor i has all three
+    log-scales -3 - 0.01 * ((37 i) mod 80), so its size rank is (37 i) mod 80
+    (0 = largest) and rank r belongs to donor (13 r) mod 80 (13 = 37^-1 mod
+    80). Donor 79 (rank 43) is given donor 0's log-scales: a tie for the
+    largest, which the stable sort keeps in index order. bigDonors is the top
+    max(16, 80 >> 2) = 20, largest first:
+        [0, 79, 13, 26, 39, 52, 65, 78, 11, 24, 37, 50, 63, 76, 9, 22, 35, 48, 61, 74]
+    (rank 19's donor, 7, falls off behind the tie).
+    Relocation (no split draw): dead 98 draws 0.5 -> donors[40] = 40; dead 99
+    draws 0.999 -> donors[79] = 79. Growth: ceil(100 * 0.15) = 15 spawns at
+    rows 100-114.
+      k=0: 0.1 < 0.7 split, 0.0 -> bigDonors[0] = 0; 74 moves into slot 0
+      k=1: split, 0.0 -> 74 (the swap-popped donor); 61 moves into slot 0
+      k=2: 0.9 no split, 0.0 -> donors[0] = 0, a clone of the shrunk donor 0
+      k=3: split, 0.999 -> bigDonors[floor(0.999 * 18)] = bigDonors[17] = 48 (the last)
+      k=4: split, 0.05 -> bigDonors[floor(0.05 * 17)] = bigDonors[0] = 61
+      k=5..14: 16 left, no split draw; 0.3 -> donors[24] = 24
+    Every spawn then draws three Gaussians of two values each: (0.5, 0.0)
+    gives sqrt(-2 ln 0.5) cos(0) = sqrt(2 ln 2).
+    With the big-donor sort reversed, or the swap-pop replaced by a removal
+    in place, the split donors differ."""
+    n, cap = 100, 200
+    P = np.zeros((cap, 16), np.float32)
+    idx = np.arange(n)
+    P[:n, 0] = idx * 0.01
+    P[:n, 1] = -idx * 0.02
+    P[:n, 2] = 4.0 + idx * 0.001
+    rank = (37 * idx) % 80
+    ls = np.where(idx < 80, -3.0 - 0.01 * rank, -4.0).astype(np.float32)
+    ls[79] = ls[0]
+    P[:n, 3:6] = ls[:, None]
+    P[:n, 6] = 1.0 + idx          # unique rotation and colour per Gaussian: they identify the donor
+    P[:n, 7:10] = 0.1
+    P[:n, 10] = idx * 0.03
+    P[:n, 11:13] = -0.5
+    P[:n, 13] = np.where(idx < 80, 3.0, np.where(idx < 98, -2.0, -6.0))
+    M = np.ones((cap, 16), np.float32)
+    V = np.ones((cap, 16), np.float32)
+    st = {"P": P, "M": M, "V": V}
+    P0 = P.copy()
+    gauss3 = [0.5, 0.0] * 3
+    script = ([0.5] + gauss3 + [0.999] + gauss3 +
+              [0.1, 0.0] + gauss3 + [0.1, 0.0] + gauss3 + [0.9, 0.0] + gauss3 +
+              [0.1, 0.999] + gauss3 + [0.1, 0.05] + gauss3 +
+              ([0.3] + gauss3) * 10)
+    rng = _scripted(script)
+    moved, grown, n2 = R.refine(st, n, cap, 100, 60000, rng)
+    assert (moved, grown, n2) == (2, 15, 115)
+    assert rng.left == [], "every scripted value is drawn, no more"
+
+    def donor_of(row):
+        return int(round(P[row, 6] - 1.0))
+
+    assert donor_of(98) == 40 and donor_of(99) == 79
+    spawned = [donor_of(100 + k) for k in range(15)]
+    assert spawned == [0, 74, 0, 48, 61] + [24] * 10
+    splits = [0, 74, 48, 61]
+    shrink, clone = math.log(1 / 1.6), math.log(0.85)
+    g = math.sqrt(-2 * math.log(0.5))
+
+    def f32(x):
+        return float(np.float32(x))
+
+    # split donors: log-scales shrunk once, moments cleared, opacity kept
+    for d in splits:
+        assert np.all(P[d, 3:6] == np.float32(f32(float(P0[d, 3]) + shrink)))
+        assert not M[d].any() and not V[d].any()
+        assert P[d, 13] == P0[d, 13]
+    # untouched Gaussians keep their rows and moments
+    for i in set(range(98)) - set(splits):
+        assert np.array_equal(P[i], P0[i]) and M[i].all()
+    for k, d in enumerate([0, 74, 0, 48, 61] + [24] * 10):
+        row = 100 + k
+        split = k in (0, 1, 3, 4)
+        # a clone of donor 0 at k=2 reads its shrunk log-scales
+        base = float(P[d, 3]) if not split else f32(float(P0[d, 3]))
+        want_ls = f32(base + shrink) if split else f32(base + clone)
+        assert np.all(P[row, 3:6] == np.float32(want_ls)), (k, P[row, 3:6], want_ls)
+        want_opa = P0[d, 13] if split else np.float32(math.log(0.25 / 0.75))
+        assert P[row, 13] == want_opa
+        assert np.array_equal(P[row, 6:13], P0[d, 6:13])
+        assert P[row, 14] == 0 and P[row, 15] == 0 and not M[row].any() and not V[row].any()
+        # jitter: s = exp(mean log-scale of the donor when it spawns), 0.7 g per axis
+        # (before a split shrinks the donor; a clone of donor 0 sees it shrunk)
+        s = math.exp(float(P0[d, 3])) if split else math.exp(float(P[d, 3]))
+        np.testing.assert_allclose(P[row, :3], P0[d, :3] + g * s * 0.7, rtol=1e-6, atol=1e-7)
+    for row, d in ((98, 40), (99, 79)):
+        assert np.all(P[row, 3:6] == np.float32(f32(float(P0[d, 3]) + clone)))
+        assert P[row, 13] == np.float32(math.log(0.25 / 0.75))
+    # rows past the new n are untouched
+    assert not P[n2:].any()
+
+
 # ---------------------------------------------------------------- GPU ---
 
 def _pad(a, cap):
