@@ -243,6 +243,17 @@ int gs_read_step_stats(gs_trainer_t* h, gs_step_stats* out);
  * *size is set; otherwise cap must be at least that size. */
 int gs_export_ply(gs_trainer_t* h, int32_t bake, void* buf, int64_t cap, int64_t* size);
 
+/* gs_export_ply without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Same arguments and file bytes as gs_export_ply; the rows are
+ * built in chunks of at most 16 MiB in a fixed device array of the library
+ * (about 16 MiB per GPU, held until the process ends) and copied into buf
+ * chunk by chunk. Returns GS_E_LIMIT for more than 4096 cameras (the array
+ * holds 4096 camera centres for the opacity bake). Calls that overlap on one
+ * GPU, from any trainers or threads, share that array and race. */
+int gs_export_ply_noalloc(gs_trainer_t* h, int32_t bake, void* buf, int64_t cap, int64_t* size);
+
 /* GSTrainer.refine(rng) (trainer.js:666-794), MCMC-lite: relocate dead
  * splats (opacity < 0.02, at most ceil(5% n)) onto jittered clones of
  * donors (opacity > 0.4), then grow by ceil(growRate * n) (until growUntil,

@@ -161,8 +161,10 @@ void launch_refine_apply(hipStream_t st, int deg, bool mixed, int64_t cap, const
                          float4* P, void* M, void* V, void* SH, void* SHM, void* SHV);
 
 // ---- PLY export rows (gs_export_ply) ----
-// out: n rows of (17 + 3K) f32; camPos: nc x 3 f32 (bake only)
-void launch_ply_rows(hipStream_t st, int deg, bool mixed, int64_t n, int64_t cap, const float4* P, const void* SH,
-                     int bake, double f, const float* camPos, int nc, float* out);
+// out: rows first .. first + count - 1, each (17 + 3K) f32, from out[0];
+// camPos: nc x 3 f32 (bake only)
+void launch_ply_rows(hipStream_t st, int deg, bool mixed, int64_t first, int64_t count, int64_t cap,
+                     const float4* P, const void* SH, int bake, double f, const float* camPos, int nc,
+                     float* out);
 
 }  // namespace gs

@@ -9,15 +9,18 @@
 
 namespace gs {
 
+// Rows first .. first + count - 1; row i goes to out + (i - first) * W.
 template <int kDeg, bool kMixed>
-__global__ __launch_bounds__(kBlock) void k_ply_rows(int64_t n, int64_t cap, const float4* __restrict__ P,
+__global__ __launch_bounds__(kBlock) void k_ply_rows(int64_t first, int64_t count, int64_t cap,
+                                                    const float4* __restrict__ P,
                                                     const void* __restrict__ SH, int bake, double f,
                                                     const float* __restrict__ camPos, int nc,
                                                     float* __restrict__ out) {
   constexpr int NC = ShDims<kDeg>::C;
   constexpr int W = 17 + NC;   // floats per row
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+  const int64_t r = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= count) return;
+  const int64_t i = first + r;
   float p[16];
   load_params(P, cap, i, p);
   float opaLogit = p[13];
@@ -38,7 +41,7 @@ __global__ __launch_bounds__(kBlock) void k_ply_rows(int64_t n, int64_t cap, con
     const double cl = fmin(1.0 - 1e-6, fmax(1e-6, opa));
     opaLogit = (float)log(cl / (1.0 - cl));
   }
-  float* o = out + (int64_t)W * i;
+  float* o = out + (int64_t)W * r;
   o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
   o[3] = 0.0f; o[4] = 0.0f; o[5] = 0.0f;
   const double C0 = 0.28209479177387814;
@@ -59,22 +62,23 @@ __global__ __launch_bounds__(kBlock) void k_ply_rows(int64_t n, int64_t cap, con
 }
 
 template <bool kMixed>
-static void ply_prec(hipStream_t st, int deg, int64_t n, int64_t cap, const float4* P, const void* SH, int bake,
-                     double f, const float* camPos, int nc, float* out) {
-  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+static void ply_prec(hipStream_t st, int deg, int64_t first, int64_t count, int64_t cap, const float4* P,
+                     const void* SH, int bake, double f, const float* camPos, int nc, float* out) {
+  const dim3 grid((unsigned)((count + kBlock - 1) / kBlock));
   switch (deg) {
-    case 0: k_ply_rows<0, kMixed><<<grid, kBlock, 0, st>>>(n, cap, P, SH, bake, f, camPos, nc, out); break;
-    case 1: k_ply_rows<1, kMixed><<<grid, kBlock, 0, st>>>(n, cap, P, SH, bake, f, camPos, nc, out); break;
-    case 2: k_ply_rows<2, kMixed><<<grid, kBlock, 0, st>>>(n, cap, P, SH, bake, f, camPos, nc, out); break;
-    default: k_ply_rows<3, kMixed><<<grid, kBlock, 0, st>>>(n, cap, P, SH, bake, f, camPos, nc, out); break;
+    case 0: k_ply_rows<0, kMixed><<<grid, kBlock, 0, st>>>(first, count, cap, P, SH, bake, f, camPos, nc, out); break;
+    case 1: k_ply_rows<1, kMixed><<<grid, kBlock, 0, st>>>(first, count, cap, P, SH, bake, f, camPos, nc, out); break;
+    case 2: k_ply_rows<2, kMixed><<<grid, kBlock, 0, st>>>(first, count, cap, P, SH, bake, f, camPos, nc, out); break;
+    default: k_ply_rows<3, kMixed><<<grid, kBlock, 0, st>>>(first, count, cap, P, SH, bake, f, camPos, nc, out); break;
   }
 }
 
-void launch_ply_rows(hipStream_t st, int deg, bool mixed, int64_t n, int64_t cap, const float4* P, const void* SH,
-                     int bake, double f, const float* camPos, int nc, float* out) {
-  if (n <= 0) return;
-  if (mixed) ply_prec<true>(st, deg, n, cap, P, SH, bake, f, camPos, nc, out);
-  else ply_prec<false>(st, deg, n, cap, P, SH, bake, f, camPos, nc, out);
+void launch_ply_rows(hipStream_t st, int deg, bool mixed, int64_t first, int64_t count, int64_t cap,
+                     const float4* P, const void* SH, int bake, double f, const float* camPos, int nc,
+                     float* out) {
+  if (count <= 0) return;
+  if (mixed) ply_prec<true>(st, deg, first, count, cap, P, SH, bake, f, camPos, nc, out);
+  else ply_prec<false>(st, deg, first, count, cap, P, SH, bake, f, camPos, nc, out);
 }
 
 }  // namespace gs

@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -700,6 +701,9 @@ void repack_to_planes(const float* src, int64_t n, int width, int planes, int64_
     for (int j = 0; j < width; j++) dst[((size_t)(j / 4) * cap + i) * 4 + (j % 4)] = src[i * width + j];
 }
 
+// the no-allocation export's device scratch, looked up per device (below)
+int scratch_init(int device);
+
 }  // namespace
 
 extern "C" {
@@ -758,6 +762,12 @@ int gs_create(const gs_opts* o, gs_trainer_t** out) {
     return GS_E_HIP;
   }
   if (hipSetDevice(h->o.device) != hipSuccess || hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return GS_E_HIP;
+  }
+  // this device's address of the no-allocation export's scratch array
+  if (scratch_init(h->o.device) != GS_OK) {
+    (void)hipStreamDestroy(h->st);
     delete h;
     return GS_E_HIP;
   }
@@ -1454,48 +1464,165 @@ int gs_refine(gs_trainer_t* h, const gs_refine_opts* ro, gs_refine_result* out) 
 }
 
 // ---- PLY export (src/io/ply.js; session.js:637-645) ----
+}  // extern "C"
+
+namespace {
+
+// The PLY header of gaussiansToPly (src/io/ply.js:47-70) for n rows with shC
+// rest coefficients; written to dst when it is not null. Returns its length.
+// Builds nothing on the heap (the no-allocation export shares it).
+int64_t ply_header(char* dst, int shC, int64_t n) {
+  int64_t len = 0;
+  auto put = [&](const char* text) {
+    const size_t k = strlen(text);
+    if (dst) memcpy(dst + len, text, k);
+    len += (int64_t)k;
+  };
+  char line[64];
+  put("ply\nformat binary_little_endian 1.0\n");
+  snprintf(line, sizeof line, "element vertex %lld\n", (long long)n);
+  put(line);
+  const char* head[] = {"x", "y", "z", "nx", "ny", "nz", "f_dc_0", "f_dc_1", "f_dc_2"};
+  for (const char* p : head) {
+    snprintf(line, sizeof line, "property float %s\n", p);
+    put(line);
+  }
+  for (int k = 0; k < shC; k++) {
+    snprintf(line, sizeof line, "property float f_rest_%d\n", k);
+    put(line);
+  }
+  const char* tail[] = {"opacity", "scale_0", "scale_1", "scale_2", "rot_0", "rot_1", "rot_2", "rot_3"};
+  for (const char* p : tail) {
+    snprintf(line, sizeof line, "property float %s\n", p);
+    put(line);
+  }
+  put("end_header\n");
+  return len;
+}
+
+// camera centre -(R^T t) in JS numbers, stored through a Float32Array
+// (session.js:32-38, :642)
+void cam_centre(const CamMeta& k, float* out) {
+  for (int a = 0; a < 3; a++)
+    out[a] = (float)(-((k.R[a] * k.t[0] + k.R[3 + a] * k.t[1]) + k.R[6 + a] * k.t[2]));
+}
+
+// ---- device scratch of the no-allocation export (gs_export_ply_noalloc) ----
+// A fixed array per GPU, held until the process ends: camera centres for the
+// opacity bake (at most kNoAllocMaxCams), then one chunk of PLY rows. The
+// host never passes `scratch` itself to a kernel or copy (in host code the
+// name is a stand-in, not a device address); it uses the address that
+// hipGetSymbolAddress returned for the current device.
+constexpr int kNoAllocMaxCams = 4096;
+constexpr int64_t kNoAllocCamFloats = 3 * kNoAllocMaxCams;          // 48 KiB
+constexpr int64_t kNoAllocRowFloats = int64_t(4) << 20;             // 16 MiB of rows
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) float scratch[kNoAllocCamFloats + kNoAllocRowFloats];
+
+std::mutex g_scratchMu;
+float* g_scratchPtr[kMaxDevices] = {};
+
+// Looks up this device's copy of `scratch` once per device; the device must
+// be current. Called by gs_create.
+int scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return GS_E_ARG;
+  std::lock_guard<std::mutex> lk(g_scratchMu);
+  if (g_scratchPtr[device]) return GS_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr) {
+    (void)hipGetLastError();
+    return GS_E_HIP;
+  }
+  g_scratchPtr[device] = (float*)ptr;
+  return GS_OK;
+}
+
+float* scratch_ptr(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratchMu);
+  return g_scratchPtr[device];
+}
+
+}  // namespace
+
+extern "C" {
+
 int gs_export_ply(gs_trainer_t* h, int32_t bake, void* buf, int64_t cap, int64_t* size) {
   if (!h || !size) return GS_E_ARG;
   if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
   const int64_t n = h->n;
   const int W = 17 + h->shC;
-  std::string hdr = "ply\nformat binary_little_endian 1.0\nelement vertex " + std::to_string(n) + "\n";
-  const char* head[] = {"x", "y", "z", "nx", "ny", "nz", "f_dc_0", "f_dc_1", "f_dc_2"};
-  for (const char* p : head) hdr += std::string("property float ") + p + "\n";
-  for (int k = 0; k < h->shC; k++) hdr += "property float f_rest_" + std::to_string(k) + "\n";
-  const char* tail[] = {"opacity", "scale_0", "scale_1", "scale_2", "rot_0", "rot_1", "rot_2", "rot_3"};
-  for (const char* p : tail) hdr += std::string("property float ") + p + "\n";
-  hdr += "end_header\n";
+  const int64_t hdrLen = ply_header(nullptr, h->shC, n);
   const int64_t body = n * (int64_t)W * 4;
-  *size = (int64_t)hdr.size() + body;
+  *size = hdrLen + body;
   if (!buf) return GS_OK;
   if (cap < *size) return fail(h, GS_E_ARG, "exportPly: buffer smaller than the file");
   HIPCK(hipSetDevice(h->o.device));
-  // camera centres -(R^T t) in JS numbers, stored through a Float32Array
-  // (session.js:32-38, :642)
   const int nc = (int)h->cams.size();
   std::vector<float> cp(3 * (size_t)nc);
-  for (int c = 0; c < nc; c++) {
-    const CamMeta& k = h->cams[c];
-    for (int a = 0; a < 3; a++)
-      cp[3 * c + a] = (float)(-((k.R[a] * k.t[0] + k.R[3 + a] * k.t[1]) + k.R[6 + a] * k.t[2]));
-  }
+  for (int c = 0; c < nc; c++) cam_centre(h->cams[c], &cp[3 * (size_t)c]);
   float* dCam = nullptr;
   float* dOut = nullptr;
   HIPCK(hipMalloc(&dCam, cp.size() * 4 + 4));
   hipError_t e = hipMalloc(&dOut, (size_t)std::max<int64_t>(body, 4));
   if (e == hipSuccess) e = hipMemcpy(dCam, cp.data(), cp.size() * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
-    launch_ply_rows(h->st, h->shDeg, h->mixed, n, h->cap, h->P, h->SH, bake ? 1 : 0, h->cams[0].f, dCam,
+    launch_ply_rows(h->st, h->shDeg, h->mixed, 0, n, h->cap, h->P, h->SH, bake ? 1 : 0, h->cams[0].f, dCam,
                     nc, dOut);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(h->st);
-  memcpy(buf, hdr.data(), hdr.size());
-  if (e == hipSuccess) e = hipMemcpy((char*)buf + hdr.size(), dOut, (size_t)body, hipMemcpyDeviceToHost);
+  ply_header((char*)buf, h->shC, n);
+  if (e == hipSuccess) e = hipMemcpy((char*)buf + hdrLen, dOut, (size_t)body, hipMemcpyDeviceToHost);
   (void)hipFree(dCam);
   if (dOut) (void)hipFree(dOut);
   if (e != hipSuccess) return hipfail(h, "exportPly", e);
+  return GS_OK;
+}
+
+int gs_export_ply_noalloc(gs_trainer_t* h, int32_t bake, void* buf, int64_t cap, int64_t* size) {
+  if (!h || !size) return GS_E_ARG;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  const int nc = (int)h->cams.size();
+  if (nc > kNoAllocMaxCams)
+    return fail(h, GS_E_LIMIT, "exportPlyNoAlloc: " + std::to_string(nc) + " cameras; the device scratch holds " +
+                                   std::to_string(kNoAllocMaxCams) + " camera centres");
+  const int64_t n = h->n;
+  const int W = 17 + h->shC;
+  const int64_t hdrLen = ply_header(nullptr, h->shC, n);
+  const int64_t body = n * (int64_t)W * 4;
+  *size = hdrLen + body;
+  if (!buf) return GS_OK;
+  if (cap < *size) return fail(h, GS_E_ARG, "exportPlyNoAlloc: buffer smaller than the file");
+  HIPCK(hipSetDevice(h->o.device));
+  float* base = scratch_ptr(h->o.device);
+  if (!base) return fail(h, GS_E_HIP, "exportPlyNoAlloc: no device scratch for this device");
+  float* dCam = base;
+  float* dRows = base + kNoAllocCamFloats;
+  // Every row of a chunk and every centre the bake reads is written before
+  // it is read, so nothing carries over from an earlier call.
+  if (bake) {
+    float cp[3 * 256];
+    for (int c0 = 0; c0 < nc; c0 += 256) {
+      const int m = std::min(256, nc - c0);
+      for (int c = 0; c < m; c++) cam_centre(h->cams[c0 + c], &cp[3 * c]);
+      HIPCK(hipMemcpyAsync(dCam + 3 * c0, cp, (size_t)m * 12, hipMemcpyHostToDevice, h->st));
+      HIPCK(hipStreamSynchronize(h->st));   // cp is reused by the next group
+    }
+  }
+  ply_header((char*)buf, h->shC, n);
+  const int64_t rowsPerChunk = kNoAllocRowFloats / W;
+  for (int64_t first = 0; first < n; first += rowsPerChunk) {
+    const int64_t count = std::min(rowsPerChunk, n - first);
+    launch_ply_rows(h->st, h->shDeg, h->mixed, first, count, h->cap, h->P, h->SH, bake ? 1 : 0, h->cams[0].f,
+                    dCam, nc, dRows);
+    HIPCK(hipGetLastError());
+    HIPCK(hipMemcpyAsync((char*)buf + hdrLen + first * (int64_t)W * 4, dRows, (size_t)(count * W * 4),
+                         hipMemcpyDeviceToHost, h->st));
+    HIPCK(hipStreamSynchronize(h->st));   // the next chunk overwrites the rows
+  }
   return GS_OK;
 }
 

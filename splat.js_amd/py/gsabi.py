@@ -102,6 +102,7 @@ SIGNATURES = {
     "gs_refine": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "gs_step_batch": ([C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
     "gs_export_ply": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
+    "gs_export_ply_noalloc": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
     "gs_last_error": ([C.c_void_p], C.c_char_p),
     "gs_step_debug": ([C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
     "gs_profile": ([C.c_void_p, C.c_int32], C.c_int),
@@ -345,13 +346,15 @@ class GSTrainer:
     def sync(self):
         self._ck(lib().gs_sync(self._h))
 
-    def exportPly(self, bake=True):
+    def exportPly(self, bake=True, noalloc=False):
         """Standard 3DGS binary PLY bytes (ply.js gaussiansToPly), with the
-        Mip opacity bake of Session.exportPlyBlob when bake."""
+        Mip opacity bake of Session.exportPlyBlob when bake. noalloc selects
+        gs_export_ply_noalloc (no device allocation; same bytes)."""
+        fn = lib().gs_export_ply_noalloc if noalloc else lib().gs_export_ply
         size = C.c_int64()
-        self._ck(lib().gs_export_ply(self._h, 1 if bake else 0, None, 0, C.byref(size)))
+        self._ck(fn(self._h, 1 if bake else 0, None, 0, C.byref(size)))
         buf = (C.c_uint8 * size.value)()
-        self._ck(lib().gs_export_ply(self._h, 1 if bake else 0, buf, size.value, C.byref(size)))
+        self._ck(fn(self._h, 1 if bake else 0, buf, size.value, C.byref(size)))
         return bytes(buf)
 
     def refine(self, seed=0, growRate=-1.0, growUntil=-1.0, growLimit=0):
