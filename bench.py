@@ -1,27 +1,36 @@
 """Trainer-step benchmark on one MI355X.
 
-Workload (the benchmark's FP32 single-view configuration): 1M Gaussians
-(means U(-5,5)^3, log-scales N(-4,0.5), random unit quaternions, opacity
-logits N(0,1), DC U(0,1) as logits, SH degree 2 rest N(0,0.05)), a ring of
-64 views at radius 8 looking at the origin, 1600x900, f=1100. Targets are
-renders of a second seeded 1M-Gaussian set. Each step = projection, 16x16
-tile binning + per-tile depth sort, raster forward + Charbonnier loss, raster
-backward, chain + Adam (position lr 1.6e-4) on one view picked by the
-seeded camera RNG. All three SH bands are active (iteration counter starts
-at 2000, past the one-band-per-1000-iterations ramp).
+Workload (the benchmark's single-view configuration): 1M Gaussians (means
+U(-5,5)^3, log-scales N(-4,0.5), random unit quaternions, opacity logits
+N(0,1), DC U(0,1) as logits, SH degree 2 rest N(0,0.05)), a ring of 64 views
+at radius 8 looking at the origin, 1600x900, f=1100. Targets are renders of a
+second seeded 1M-Gaussian set. A training iteration = projection, 16x16 tile
+binning + per-tile depth sort, raster forward + Charbonnier loss, raster
+backward and chain for one view picked by the seeded camera RNG; Adam
+(position lr 1.6e-4) after every view in the single-view step, once per
+batch of 8 views in the batched step. All three SH bands are active
+(iteration counter starts at 2000, past the one-band-per-1000 ramp).
 
-The metric compares mixed precision with FP32 under a 0.05 dB PSNR bound,
-so by default both are timed on the same workload from the same start:
-`value` is the mixed-precision throughput (bf16 storage of SH coefficients
-and Adam moments, f32 arithmetic), and `config.fp32` and
-`config.psnr_delta_db` give the FP32 throughput and the measured PSNR
-difference over all views after the same steps.
+Runs, all from the same start on the same workload:
+  f32          the plain FP32 single-view step (the reference's arithmetic and storage)
+  bf16         single-view, SH coefficients and raster-record colours stored in bf16,
+               Adam moments and all arithmetic f32 (the metric's mixed precision)
+  mixed        single-view, SH coefficients and all Adam moments in bf16 (beyond the
+               metric's rule that moments stay f32; reported, not the headline)
+  bf16 x8      the batched step: 8 views rasterized forward and backward in one
+               persistent launch each, one Adam step, replayed as a hipGraph
+`value` is the batched bf16 step's training iterations per second: views
+trained per second (the trainer's iteration counter advances by one per view,
+as in the reference's stepOnce). `config.runs` holds every run's rate, and
+`config.psnr` the PSNR drift of each 16-bit storage mode against FP32 after
+2000 iterations under the same schedule.
 
-Prints one JSON line (metric, value, timing, config) with the roofline of the
-dominant kernel (HIP events on the trainer's stream over the timed region),
-a measured HBM ceiling (a plain 1 GiB device-to-device copy) and a CPU
-baseline (the C restatement under oracle/, one thread, one full step of the
-same workload).
+Prints one JSON line with the roofline of the headline run's dominant kernel
+(HIP events on the trainer's stream), a measured HBM ceiling (a plain 1 GiB
+device-to-device copy), a CPU baseline (the C restatement under oracle/ on
+the host's cores, one full step of the same workload) and the error of one
+step of the headline storage mode against the FP64 restatement on a
+4096-Gaussian crop of the scene.
 """
 import argparse
 import json
@@ -39,6 +48,9 @@ import scenes  # noqa: E402
 
 METRIC = "train iters/s @1M splats 1600x900 (fwd+bwd+sort), mixed vs FP32, ≤0.05dB PSNR"
 HBM_PEAK_GBS = 8000.0
+PRECS = {"f32": gsabi.GS_PREC_F32, "mixed": gsabi.GS_PREC_MIXED, "bf16": gsabi.GS_PREC_BF16,
+         "fp16": gsabi.GS_PREC_FP16}
+BATCH = 8
 
 
 def log(msg):
@@ -61,142 +73,47 @@ def build_workload(n, views, W, H, f, seed=0, gt_seed=1):
     return p, sh, cams, images
 
 
-def algorithmic_bytes(name, n, vis, keys, npx, shC, mixed=False):
-    """HBM bytes each kernel has to move per launch (f32; mixed stores SH
-    coefficients and the Adam moments in bf16):
-    project   n*(56 params + 16 key/rect/count) + vis*(4*shC SH + 48 record)   (SH 2*shC mixed)
-    raster fwd  keys*(4 id + 48 record) + px*(16 img + 4 T + 4 end + 16 dL/dC + 4 target)
-    raster bwd  keys*(4 id + 48 record) + px*(16 dL/dC + 4 T + 4 end) + vis*40 gradient sums
-    update    n*(40+40 sums + 4 count + 2*56 params + 2*2*56 m/v) + n*3*2*4*shC SH/m/v
-              (mixed: m/v 2*2*28, SH/m/v n*3*2*2*shC)
-    bin_sort  n*(16 count pass + 40 scatter pass: count, rect, depth key, conic)
-              + keys*(8 scattered key + 8 sort read + 4 sorted id)"""
-    sb = 2.0 if mixed else 4.0
+def algorithmic_bytes(name, n, vis, keys, npx, shC, prec, views=1):
+    """HBM bytes each kernel has to move per launch. SH coefficients take sb
+    bytes (4 f32, 2 in every 16-bit mode), Adam moments mb (4, or 2 in mode
+    mixed); the raster record is 48 B of f32 geometry and colour read per key
+    (32 B in the 16-bit-colour modes: geometry and packed colours).
+    project   n*(56 params + 16 key/rect/count) + vis*(sb*shC SH + 48 record write)
+    raster fwd  keys*(4 id + rec) + px*(16 img + 4 T + 4 end + 16 dL/dC + 4 target)
+    raster bwd  keys*(4 id + rec + 16) + px*(16 dL/dC + 4 T + 4 end) + vis*40 gradient sums
+    update    n*(48 sums + 4 count + 2*56 params + 2*2*14*mb m/v) + n*(2*sb + 2*2*mb)*shC SH and its m/v
+              (the batched update reads every view's sums and counts)
+    bin_sort  n*(16 count pass + 40 scatter pass) + keys*(8 scattered key + 8 sort read + 4 sorted id)
+    keys, vis and npx are per view; views > 1 for the batched launches."""
+    sb = 4.0 if prec == "f32" else 2.0
+    mb = 2.0 if prec == "mixed" else 4.0
+    rec = 32.0 if prec in ("bf16", "fp16") else 48.0
     if name == "project":
         return n * 72.0 + vis * (sb * shC + 48.0)
-    if name == "raster_fwd_loss":
-        return keys * 52.0 + npx * 44.0
-    if name == "raster_bwd":
-        return keys * 52.0 + npx * 24.0 + vis * 40.0
-    if name == "update":
-        return n * (84.0 + 112.0 + 4.0 * 14.0 * sb) + n * 6.0 * sb * shC
+    if name in ("raster_fwd_loss", "raster_fwd_views"):
+        return views * (keys * (4.0 + rec) + npx * 44.0)
+    if name in ("raster_bwd", "raster_bwd_views"):
+        return views * (keys * (20.0 + rec) + npx * 24.0 + vis * 40.0)
+    if name in ("update", "update_views"):
+        return n * (52.0 * views + 112.0 + 4.0 * 14.0 * mb) + n * (2.0 * sb + 4.0 * mb) * shC
     if name == "bin_sort":
         return n * 56.0 + keys * 20.0
     return None
 
 
-def cpu_baseline(p, sh, cam, image, radius):
-    """One full f32 step of the C restatement (single thread) on the same
-    inputs: projection, brute-force binning sort, forward, loss, backward,
-    chain, Adam."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from helpers import OracleTrainer  # CPU restatement, used here only as the baseline
-    ref = OracleTrainer(p, sh, [dict(cam, imgIdx=0)], [image], radius, opts={"posLr": 1.6e-4}, shDeg=2)
-    ref.iter = 2000
-    t0 = time.perf_counter()
-    ref.step(0)
-    return time.perf_counter() - t0
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--n", type=int, default=1_000_000)
-    ap.add_argument("--views", type=int, default=64)
-    ap.add_argument("--width", type=int, default=1600)
-    ap.add_argument("--height", type=int, default=900)
-    ap.add_argument("--precision", choices=["both", "f32", "mixed"], default="both",
-                    help="mixed: SH coefficients and Adam moments stored in bf16 (f32 arithmetic); "
-                         "both (default): time f32 and mixed, report mixed with the f32 figures and PSNR delta")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run one plain 1 GiB device-to-device copy (scales HBM byte counters)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this trainer runs on one GPU; sharding across GPUs is a design only",
-                          "n_gpus": args.gpus}))
-        sys.exit(2)
-
-    W, H, f = args.width, args.height, 1100.0 * args.width / 1600.0
-    p, sh, cams, images = build_workload(args.n, args.views, W, H, f)
-    precs = ["f32", "mixed"] if args.precision == "both" else [args.precision]
-    runs = {pr: run_one(pr, p, sh, cams, images, W, H, args) for pr in precs}
-    head = runs[precs[-1]]
-    its, dt, prof = head["its"], head["dt"], head["prof"]
-    mean_keys, mean_vis = head["keys"], head["vis"]
-    mixed = precs[-1] == "mixed"
-
-    npx = W * H
-    shC = 24
-    kernels = {}
-    for name, v in prof.items():
-        avg_ms = v["ms"] / max(v["launches"], 1)
-        b = algorithmic_bytes(name, args.n, mean_vis, mean_keys, npx, shC, mixed)
-        kernels[name] = {"avg_ms": round(avg_ms, 4),
-                         "GBps": None if b is None else round(b / (avg_ms * 1e-3) / 1e9, 1)}
-    single = {k: v for k, v in kernels.items() if v["GBps"] is not None}
-    dom = max(single, key=lambda k: single[k]["avg_ms"])
-    dom_b = algorithmic_bytes(dom, args.n, mean_vis, mean_keys, npx, shC, mixed)
-    achieved = dom_b / (single[dom]["avg_ms"] * 1e-3) / 1e9
-    traffic = None
-    tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_1M_1600x900.json")
-    if os.path.exists(tf):
-        traffic = json.load(open(tf)).get("bytes_per_launch")
-
-    copy_gbps = round(copy_ceiling(), 1)
-    log(f"device-to-device copy: {copy_gbps} GB/s (read + write)")
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        log("cpu baseline (one f32 step of the C restatement, 1 thread)")
-        sec = cpu_baseline(p, sh, cams[0], images[0], 8.0)
-        cpu = {"value": round(1.0 / sec, 5), "unit": "iters/s", "cores": 1, "kind": "port",
-               "sample": f"1 full training step (project, sort, fwd, loss, bwd, chain, Adam) of the same "
-                         f"{args.n}-Gaussian {W}x{H} workload on view 0, f32 C restatement, {sec:.1f}s"}
-
-    storage = ("bf16 SH coefficients and Adam moments (stochastic rounding), f32 params and arithmetic"
-               if mixed else "f32")
-    config = {"workload": f"single-view trainer step, {args.n} Gaussians SH deg 2, "
-                          f"{args.views}-view ring r=8, {W}x{H} f={f:g}, {'mixed' if mixed else 'FP32'} storage",
-              "gaussians": args.n, "width": W, "height": H, "views": args.views, "sh_degree": 2,
-              "storage": storage,
-              "keys_per_step": round(mean_keys), "visible_per_step": round(mean_vis),
-              "sorted_keys_per_s": round(mean_keys * its), "kernels": kernels,
-              "train_mse": head["mse"], "psnr_db": round(head["psnr"], 4),
-              "hbm_copy_ceiling_GBps": copy_gbps}
-    if len(runs) == 2:
-        r32 = runs["f32"]
-        config["fp32"] = {"value": round(r32["its"], 2), "ms_per_step": round(1e3 * r32["dt"] / args.steps, 4),
-                          "psnr_db": round(r32["psnr"], 4)}
-        config["psnr_delta_db"] = round(head["psnr"] - r32["psnr"], 4)
-        config["psnr_protocol"] = (f"mean PSNR over all {args.views} views after the same {args.warmup}+"
-                                   f"{args.steps}+16 seeded steps from the same start, each precision")
-    line = {
-        "metric": METRIC, "value": round(its, 2), "unit": "iters/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(1e3 * dt / args.steps, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": config,
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(line), flush=True)
-    if args.copy_calibration:
-        copy_calibration()
-
-
-def run_one(precision, p, sh, cams, images, W, H, args):
-    """Warm up, time args.steps steps (per-stage HIP events on the trainer's
-    stream), sample key/visible counts over 16 more steps, then the mean
-    PSNR over all views."""
-    prec = gsabi.GS_PREC_MIXED if precision == "mixed" else gsabi.GS_PREC_F32
-    t = gsabi.GSTrainer(shDeg=2, maxSplats=args.n, posLr=1.6e-4, seed=7, precision=prec)
-    t.setup({"data": p, "n": args.n, "sh": sh}, cams, images, W, H, 8.0)
+def _trainer(p, sh, cams, images, W, H, n, prec, views=1, graph=1):
+    t = gsabi.GSTrainer(shDeg=2, maxSplats=n, posLr=1.6e-4, seed=7, precision=PRECS[prec], viewsPerStep=views,
+                        graph=graph)
+    t.setup({"data": p, "n": n, "sh": sh}, cams, images, W, H, 8.0)
     t.iter = 2000
-    log(f"{precision}: warmup")
+    return t
+
+
+def run_single(prec, p, sh, cams, images, W, H, args):
+    """Warm up, time args.steps single-view steps with per-stage HIP events
+    on the trainer's stream, then sample key and visible counts over 16 more
+    steps (they vary by view)."""
+    t = _trainer(p, sh, cams, images, W, H, args.n, prec)
     for _ in range(args.warmup):
         t.stepOnce()
     t.sync()
@@ -209,19 +126,227 @@ def run_one(precision, p, sh, cams, images, W, H, args):
     dt = time.perf_counter() - t0
     prof = t.profile_read()
     t.profile(False)
-    log(f"{precision}: {args.steps / dt:.1f} iters/s")
     keys, vis = [], []
-    # key and visible counts vary by view: sample them over a pass of steps after the timed region
     for _ in range(16):
         t.stepOnce()
-        i = t.step_stats()
-        keys.append(i.lastKeys)
-        vis.append(i.lastVisible)
-    mse = t.readLoss()
-    psnr = float(np.mean([t.evalCamPsnr(ci) for ci in range(len(cams))]))
+        s = t.step_stats()
+        keys.append(s.lastKeys)
+        vis.append(s.lastVisible)
     t.destroy()
-    return {"its": args.steps / dt, "dt": dt, "prof": prof, "keys": float(np.mean(keys)),
-            "vis": float(np.mean(vis)), "mse": mse, "psnr": psnr}
+    log(f"{prec}: {args.steps / dt:.1f} iters/s")
+    return {"its": args.steps / dt, "ms_per_iter": 1e3 * dt / args.steps, "prof": prof,
+            "keys": float(np.mean(keys)), "vis": float(np.mean(vis))}
+
+
+def run_batched(prec, p, sh, cams, images, W, H, args):
+    """Batched steps of BATCH views: warm up (the second step captures the
+    graph), time args.steps batched steps replayed from the graph, then a
+    profiled pass (eager launches with per-stage HIP events) and key counts."""
+    t = _trainer(p, sh, cams, images, W, H, args.n, prec, views=BATCH)
+    picks = [-1] * BATCH
+    for _ in range(max(args.warmup, 2)):
+        t.stepViews(picks)
+    t.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t.stepViews(picks)
+    t.sync()
+    dt = time.perf_counter() - t0
+    t.profile(True)
+    for _ in range(16):
+        t.stepViews(picks)
+    t.sync()
+    prof = t.profile_read()
+    t.profile(False)
+    keys, vis = [], []
+    for _ in range(4):
+        t.stepViews(picks)
+        s = t.step_stats()
+        keys.append(s.lastKeys / BATCH)
+        vis.append(s.lastVisible / BATCH)
+    t.destroy()
+    its = args.steps * BATCH / dt
+    log(f"{prec} x{BATCH}: {its:.1f} iters/s ({args.steps / dt:.1f} batched steps/s)")
+    return {"its": its, "ms_per_iter": 1e3 * dt / (args.steps * BATCH), "steps_per_s": args.steps / dt,
+            "prof": prof, "keys": float(np.mean(keys)), "vis": float(np.mean(vis))}
+
+
+def psnr_after(prec, p, sh, cams, images, W, H, n, iters, batched):
+    """Mean PSNR over all views after `iters` training iterations from the
+    same start (single-view steps, or batched steps of BATCH views)."""
+    t = _trainer(p, sh, cams, images, W, H, n, prec, views=BATCH if batched else 1)
+    if batched:
+        for _ in range(iters // BATCH):
+            t.stepViews([-1] * BATCH)
+    else:
+        for _ in range(iters):
+            t.stepOnce()
+    v = float(np.mean([t.evalCamPsnr(ci) for ci in range(len(cams))]))
+    t.destroy()
+    return v
+
+
+def cpu_leg(p, sh, cam, image, W, H, radius, n, threads, headline_prec):
+    """The CPU side (runs on the host only): one full f32 step of the C
+    restatement on the same inputs and view, timed over `threads` threads;
+    then the error of one GPU step of the headline storage mode against the
+    FP64 restatement on a 4096-Gaussian crop (the Gaussians inside [-1, 1]^3)
+    of the same scene, with the same view."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle as O  # CPU restatement: the baseline and the FP64 checker, never the product
+    from helpers import OracleTrainer
+    O.set_threads(threads)
+    ref = OracleTrainer(p, sh, [dict(cam, imgIdx=0)], [image], radius, opts={"posLr": 1.6e-4}, shDeg=2)
+    ref.iter = 2000
+    t0 = time.perf_counter()
+    ref.step(0)
+    sec = time.perf_counter() - t0
+    # FP64 error on a crop
+    inside = np.all(np.abs(p[:, :3]) <= 1.0, axis=1)
+    idx = np.nonzero(inside)[0][:4096]
+    cp, csh = np.ascontiguousarray(p[idx]), np.ascontiguousarray(sh[idx])
+    st = O.Stage(O.make_opts(shDeg=2), "f64")
+    ocam = O.make_cam(cam, shActiveDeg=2)
+    tgt = O.pack_targets(np.asarray(image["rgb"], np.float32).reshape(-1, 3))
+    proj, rect, cnt = st.project(ocam, cp, csh)
+    ids, ts, _ = st.bin_sort(ocam, proj, rect, cnt)
+    img64, _, _ = st.render_fwd(ocam, proj, ids, ts)
+    g64 = O.analytic_grads(st, ocam, cp, csh, tgt, fixed=False)[0]
+    O.set_threads(1)
+    t = gsabi.GSTrainer(shDeg=2, maxSplats=cp.shape[0], precision=PRECS[headline_prec])
+    t.setup({"data": cp, "n": cp.shape[0], "sh": csh}, [dict(cam, imgIdx=0)], [image], W, H, radius)
+    t.iter = 2000
+    g = t.step_debug(0, want=("img", "gradF"))
+    t.destroy()
+    err = np.abs(g["img"][:, :3].astype(np.float64) - img64[:, :3]).reshape(-1)
+    meds = []
+    for s in range(14):
+        a, b = g["gradF"][:, s].astype(np.float64), g64[:, s]
+        big = np.abs(b) > 1e-2 * max(np.abs(b).max(), 1e-30)
+        meds.append(float(np.median(np.abs(a[big] - b[big]) / np.abs(b[big]))))
+    fp64 = {"storage": headline_prec, "sample": f"one step on a {cp.shape[0]}-Gaussian crop of the scene, "
+                                                 f"view 0, {W}x{H}",
+            "image_abs_mean": float(f"{err.mean():.3g}"), "image_abs_p999": float(f"{np.quantile(err, 0.999):.3g}"),
+            "image_abs_max": float(f"{err.max():.3g}"),
+            "grad_median_rel_worst_slot": float(f"{max(meds):.3g}")}
+    return sec, fp64
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200, help="timed steps per run (batched: steps of 8 views)")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--views", type=int, default=64)
+    ap.add_argument("--width", type=int, default=1600)
+    ap.add_argument("--height", type=int, default=900)
+    ap.add_argument("--runs", default="f32,bf16,mixed,bf16x8",
+                    help="comma list of f32, bf16, fp16, mixed (single-view) and <prec>x8 (batched)")
+    ap.add_argument("--psnr-iters", type=int, default=2000, help="0 skips the PSNR drift runs")
+    ap.add_argument("--threads", type=int, default=0, help="CPU baseline threads (0: all, at most 16)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one plain 1 GiB device-to-device copy (scales HBM byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this trainer runs on one GPU; sharding across GPUs is a design only",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    W, H, f = args.width, args.height, 1100.0 * args.width / 1600.0
+    p, sh, cams, images = build_workload(args.n, args.views, W, H, f)
+    runs = {}
+    for r in args.runs.split(","):
+        if r.endswith("x8"):
+            runs[r] = run_batched(r[:-2], p, sh, cams, images, W, H, args)
+        else:
+            runs[r] = run_single(r, p, sh, cams, images, W, H, args)
+    head_name = "bf16x8" if "bf16x8" in runs else list(runs)[-1]
+    head = runs[head_name]
+    head_prec = head_name[:-2] if head_name.endswith("x8") else head_name
+    views = BATCH if head_name.endswith("x8") else 1
+
+    npx = W * H
+    shC = 24
+    kernels = {}
+    for name, v in head["prof"].items():
+        avg_ms = v["ms"] / max(v["launches"], 1)
+        nv = views if name.endswith("_views") else 1
+        b = algorithmic_bytes(name, args.n, head["vis"], head["keys"], npx, shC, head_prec, nv)
+        kernels[name] = {"avg_ms": round(avg_ms, 4), "launches_per_step": round(v["launches"] / 16, 2),
+                         "GBps": None if b is None else round(b / (avg_ms * 1e-3) / 1e9, 1)}
+    single = {k: v for k, v in kernels.items() if v["GBps"] is not None}
+    dom = max(single, key=lambda k: single[k]["avg_ms"] * head["prof"][k]["launches"])
+    nv = views if dom.endswith("_views") else 1
+    dom_b = algorithmic_bytes(dom, args.n, head["vis"], head["keys"], npx, shC, head_prec, nv)
+    achieved = dom_b / (single[dom]["avg_ms"] * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", f"traffic_{dom}_1M_1600x900.json")
+    if os.path.exists(tf):
+        traffic = json.load(open(tf)).get("bytes_per_launch")
+
+    copy_gbps = round(copy_ceiling(), 1)
+    log(f"device-to-device copy: {copy_gbps} GB/s (read + write)")
+
+    psnr = {}
+    if args.psnr_iters > 0:
+        for batched in (False, True):
+            base = psnr_after("f32", p, sh, cams, images, W, H, args.n, args.psnr_iters, batched)
+            tag = f"x{BATCH}" if batched else "single"
+            psnr[tag] = {"f32_db": round(base, 4)}
+            for prec in ("bf16", "fp16", "mixed"):
+                v = psnr_after(prec, p, sh, cams, images, W, H, args.n, args.psnr_iters, batched)
+                psnr[tag][f"{prec}_delta_db"] = round(v - base, 4)
+            log(f"psnr after {args.psnr_iters} iterations ({tag}): {psnr[tag]}")
+
+    cpu, fp64 = None, None
+    if not args.no_cpu_baseline:
+        threads = args.threads or min(16, os.cpu_count() or 1)
+        log(f"cpu leg: one f32 step of the C restatement on {threads} threads; FP64 crop check")
+        sec, fp64 = cpu_leg(p, sh, cams[0], images[0], W, H, 8.0, args.n, threads, head_prec)
+        cpu = {"value": round(1.0 / sec, 5), "unit": "iters/s", "cores": threads, "kind": "port",
+               "sample": f"1 full training step (project, sort, fwd, loss, bwd, chain, Adam) of the same "
+                         f"{args.n}-Gaussian {W}x{H} workload on view 0, f32 C restatement with OpenMP, "
+                         f"{sec:.1f}s"}
+
+    storage = {"f32": "f32", "bf16": "bf16 SH coefficients and raster-record colours, f32 Adam moments",
+               "fp16": "fp16 SH coefficients and raster-record colours, f32 Adam moments",
+               "mixed": "bf16 SH coefficients and bf16 Adam moments"}
+    config = {"workload": f"{'batched ' + str(BATCH) + '-view' if views > 1 else 'single-view'} trainer step, "
+                          f"{args.n} Gaussians SH deg 2, {args.views}-view ring r=8, {W}x{H} f={f:g}, "
+                          f"{head_prec} storage",
+              "gaussians": args.n, "width": W, "height": H, "views": args.views, "sh_degree": 2,
+              "storage": storage[head_prec], "arithmetic": "f32 (f64 nowhere on the GPU)",
+              "iteration": "one view's projection, binning + sort, raster fwd + loss, raster bwd and chain; "
+                           "Adam after each view (single) or once per batch (batched)",
+              "views_per_step": views,
+              "keys_per_view": round(head["keys"]), "visible_per_view": round(head["vis"]),
+              "sorted_keys_per_s": round(head["keys"] * head["its"]),
+              "kernels": kernels, "hbm_copy_ceiling_GBps": copy_gbps,
+              "runs": {k: {"iters_per_s": round(v["its"], 2), "ms_per_iter": round(v["ms_per_iter"], 4),
+                           **({"batched_steps_per_s": round(v["steps_per_s"], 2)} if "steps_per_s" in v else {}),
+                           "storage": storage[k[:-2] if k.endswith("x8") else k]}
+                       for k, v in runs.items()},
+              "psnr": psnr, "psnr_protocol": f"mean PSNR over all {args.views} views after {args.psnr_iters} "
+                                            f"iterations from the same start and seed; each mode against f32 "
+                                            f"with the same schedule (single-view, or batches of {BATCH})",
+              "fp64_error": fp64}
+    if "f32" in runs:
+        config["ratio_vs_f32_single"] = round(head["its"] / runs["f32"]["its"], 3)
+    line = {
+        "metric": METRIC, "value": round(head["its"], 2), "unit": "iters/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(head["ms_per_iter"] * views, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": config,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    if args.copy_calibration:
+        copy_calibration()
 
 
 def copy_ceiling(nbytes=1 << 30, reps=5):

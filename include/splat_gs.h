@@ -41,7 +41,16 @@ enum {
 enum { GS_LOSS_CHARBONNIER = 0, GS_LOSS_L1_DSSIM = 1 };
 enum { GS_ACCUM_FIXED = 0, GS_ACCUM_F32 = 1 };
 enum { GS_SORT_DEPTH_THEN_TILE = 0, GS_SORT_GLOBAL64 = 1, GS_SORT_TILE_SEGMENTS = 2 };
-enum { GS_PREC_F32 = 0, GS_PREC_MIXED = 1 };
+/* Storage precision. Blending, transmittance, gradients and all arithmetic
+ * are f32 in every mode; stored values are decoded exactly.
+ *   GS_PREC_F32:   everything f32 (the reference's layout, shaders.js:14-29)
+ *   GS_PREC_MIXED: SH coefficients and all Adam moments in bf16
+ *   GS_PREC_BF16:  SH coefficients and the raster record's colours in bf16,
+ *                  Adam moments f32
+ *   GS_PREC_FP16:  the same in IEEE fp16
+ * Coefficient and moment stores round stochastically; record colours and
+ * the setup upload round to nearest even. */
+enum { GS_PREC_F32 = 0, GS_PREC_MIXED = 1, GS_PREC_BF16 = 2, GS_PREC_FP16 = 3 };
 
 /* Trainer options: the reference's opts object (trainer.js:18-36, :45-49,
  * :74, :105-107, :302, :319, :338-341), plus the MI355X build's own knobs. */
@@ -72,14 +81,18 @@ typedef struct gs_opts {
                           the two radix strategies drop the keys past the budget in sort order */
   uint32_t seed;       /* camera-pick RNG seed (the reference uses Math.random, trainer.js:344) */
   int32_t device;      /* HIP device ordinal, default 0 */
-  int32_t precision;   /* GS_PREC_F32 (default), or GS_PREC_MIXED: SH coefficients and all Adam
-                          moments stored in bf16 with stochastic rounding, f32 arithmetic */
+  int32_t precision;   /* GS_PREC_F32 (default), GS_PREC_MIXED, GS_PREC_BF16 or GS_PREC_FP16
+                          (storage formats above) */
   int32_t camOpt;      /* camera pose + shared focal optimisation, default 0 (trainer.js:468):
                           per-camera gradients each step (shaders.js:614-616, :834-858), one
                           camera Adam step every 25 iterations after 1500 (trainer.js:476-537) */
   int32_t expComp;     /* with camOpt, also per-camera exposure gain/bias, default 0
                           (trainer.js:511); the exposure gradients come from the Charbonnier loss,
                           so with GS_LOSS_L1_DSSIM they stay zero */
+  int32_t viewsPerStep;/* most views one gs_step_views call may batch, 1..16, default 1; setup
+                          sizes one set of per-view buffers for each */
+  int32_t graph;       /* gs_step_views replays a captured hipGraph when its shape repeats
+                          (default 1) */
 } gs_opts;
 
 /* One training camera (trainer.js:95-100 cams[], :357-372): R row-major
@@ -125,8 +138,10 @@ typedef struct gs_info {
 
 /* Device-side figures of the last raster (synchronises). */
 typedef struct gs_step_stats {
-  int64_t lastKeys;          /* (tile, Gaussian) keys of the last raster */
-  int64_t lastVisible;       /* Gaussians with at least one key in the last raster */
+  int64_t lastKeys;          /* (tile, Gaussian) keys of the last raster (of all its views when it
+                                was a gs_step_views step) */
+  int64_t lastVisible;       /* Gaussians with at least one key in the last raster (summed over
+                                the views of a gs_step_views step) */
   int64_t overflowKeys;      /* entry-budget drops of the training steps, cumulative since setup:
                                 tiles with the default sort (the reference's entryOverflowTiles,
                                 trainer.js:627), keys with the radix sorts */
@@ -153,6 +168,21 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh,
  * seeded RNG, skipping the holdout and excluded ones (trainer.js:400-406);
  * cam >= 0 trains on that camera. Enqueues only. */
 int gs_step(gs_trainer_t* h, int32_t cam);
+
+/* Batched step (no counterpart in the reference, whose stepOnce trains one
+ * view): `count` views (cams[k] < 0 picks with the seeded RNG, as gs_step
+ * does) are projected and binned one by one, then rasterized forward (with
+ * the Charbonnier loss) in one persistent launch over all their tiles and
+ * backward in another; each view's screen-space sums are chained through its
+ * own camera, the views' gradients are added in view order, and one Adam
+ * step follows. The iteration counter (band ramp, position-lr schedule)
+ * advances by `count`; the Adam step counter (bias correction) by one. With
+ * opts.graph the whole step is captured once per shape (view sizes, n) and
+ * replayed as a hipGraph; the per-step values (cameras, Adam constants) come
+ * from device memory that a copy before each step rewrites. count <=
+ * opts.viewsPerStep; needs the Charbonnier loss, the tile-segment sort,
+ * views of at most 16384 tiles and camOpt off. Enqueues only. */
+int gs_step_views(gs_trainer_t* h, const int32_t* cams, int32_t count);
 
 /* `count` training steps enqueued in one call, exactly as `count` gs_step
  * calls with cams[k] (k-th step; < 0 picks with the seeded RNG). No
@@ -311,6 +341,11 @@ typedef struct gs_debug_out {
 /* One gs_step on camera `cam` that also copies the intermediates out
  * (synchronises). */
 int gs_step_debug(gs_trainer_t* h, int32_t cam, gs_debug_out* out);
+
+/* One gs_step_views without the graph, copying each view's intermediates
+ * into outs[k] (img, T, end, gC, accum, tileRanges, numKeys); outs[0].gradF
+ * and .shGrad receive the summed gradients before Adam. Synchronises. */
+int gs_step_views_debug(gs_trainer_t* h, const int32_t* cams, int32_t count, gs_debug_out* outs);
 
 /* The camera gradient rows accumulated since the last camera Adam step or
  * evaluation pass (trainer.js:173, bufCamGrad): (numCams + 1) x 8 i32 in

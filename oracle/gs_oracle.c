@@ -10,10 +10,22 @@
 #include "gs_oracle.h"
 
 #include <math.h>
+#include <omp.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "../splat.js_amd/csrc/gs_math.h"
+
+/* Threads of the per-Gaussian and per-pixel loops (orc_set_threads). 1, the
+ * default, keeps every loop serial: the tests' expected values come from
+ * that. Above 1 (the CPU baseline), the loops run under OpenMP; fixed-point
+ * sums stay exact (integer atomics), double sums change order. */
+static int orc_threads = 1;
+
+void orc_set_threads(int n) {
+  orc_threads = n < 1 ? 1 : n;
+  omp_set_num_threads(orc_threads);
+}
 
 /* ---------------- single precision: the pinned-order twin ---------------- */
 #define REAL float
@@ -76,6 +88,7 @@ int64_t orc_count_keys(int64_t n, const uint32_t* cnt) {
 /* shaders.js:984-1036 ADAM_SRC */
 void orc_adam_f32(const orc_adam* hp, int64_t n, float* P, const float* G,
                   float* m, float* v) {
+#pragma omp parallel for schedule(static) if (orc_threads > 1)
   for (int64_t j = 0; j < n * 16; j++) {
     const int slot = (int)(j % 16);
     const float lr = hp->lr[slot];
@@ -101,6 +114,7 @@ void orc_adam_f32(const orc_adam* hp, int64_t n, float* P, const float* G,
 /* shaders.js:952-982 SH_ADAM_SRC */
 void orc_sh_adam_f32(const orc_adam* hp, int64_t count, float* sh, const float* G,
                      float* m, float* v) {
+#pragma omp parallel for schedule(static) if (orc_threads > 1)
   for (int64_t j = 0; j < count; j++) {
     float g = G[j];
     if (!(fabsf(g) < 1e18f)) g = 0.0f;
@@ -165,6 +179,46 @@ void orc_sh_adam_mixed_f32(const orc_adam* hp, int64_t n, int nc, uint16_t* sh, 
       m[j] = (uint16_t)gs_f2bf_sr(mm, gs_sr_bits(seed, 3u * 64u + (uint32_t)c));
       v[j] = (uint16_t)gs_f2bf_sr(vv, gs_sr_bits(seed, 4u * 64u + (uint32_t)c));
     }
+}
+
+/* Precision modes bf16 / fp16 (fmt 1 / 2): the SH coefficients held in 16
+ * bits, the moments in f32. Plain SH Adam (orc_sh_adam_f32's arithmetic);
+ * the new coefficient is stored with stochastic rounding, bits
+ * gs_sr_bits(gs_sr_seed(Gaussian, step), 2 * 64 + slot) as in mode mixed. */
+void orc_sh_adam_h16_f32(const orc_adam* hp, int64_t n, int nc, uint16_t* sh, const float* G,
+                         float* m, float* v, uint32_t step, int fmt) {
+  for (int64_t i = 0; i < n; i++)
+    for (int c = 0; c < nc; c++) {
+      const int64_t j = (int64_t)nc * i + c;
+      const uint32_t seed = gs_sr_seed((uint64_t)i, step);
+      float g = G[j];
+      if (!(fabsf(g) < 1e18f)) g = 0.0f;
+      const float mm = hp->b1 * m[j] + (1.0f - hp->b1) * g;
+      const float vv = hp->b2 * v[j] + (1.0f - hp->b2) * g * g;
+      m[j] = mm; v[j] = vv;
+      const float mh = mm * hp->rbc1;
+      const float vh = vv * hp->rbc2;
+      const float cur = fmt == 2 ? gs_h2f(sh[j]) : gs_bf2f(sh[j]);
+      const float s = cur - hp->shLr * mh / (sqrtf(vh) + hp->eps);
+      const uint32_t r = gs_sr_bits(seed, 2u * 64u + (uint32_t)c);
+      sh[j] = (uint16_t)(fmt == 2 ? gs_f2h_sr(s, r) : gs_f2bf_sr(s, r));
+    }
+}
+
+void orc_f2h_rne(int64_t count, const float* src, uint16_t* dst) {
+  for (int64_t j = 0; j < count; j++) dst[j] = (uint16_t)gs_f2h_rne(src[j]);
+}
+
+void orc_f2h_sr(int64_t count, const float* src, const uint32_t* rnd, uint16_t* dst) {
+  for (int64_t j = 0; j < count; j++) dst[j] = (uint16_t)gs_f2h_sr(src[j], rnd[j]);
+}
+
+void orc_f2h_up(int64_t count, const float* src, uint16_t* dst) {
+  for (int64_t j = 0; j < count; j++) dst[j] = (uint16_t)gs_f2h_up(src[j]);
+}
+
+void orc_h2f(int64_t count, const uint16_t* src, float* dst) {
+  for (int64_t j = 0; j < count; j++) dst[j] = gs_h2f(src[j]);
 }
 
 void orc_f2bf_rne(int64_t count, const float* src, uint16_t* dst) {

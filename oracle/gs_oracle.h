@@ -146,6 +146,16 @@ void orc_adam_f32(const orc_adam* hp, int64_t n, float* params, const float* gra
                   float* m, float* v);
 void orc_sh_adam_f32(const orc_adam* hp, int64_t count, float* sh, const float* g,
                      float* m, float* v);
+/* threads of the per-Gaussian and per-pixel loops; 1 (default) is serial */
+void orc_set_threads(int n);
+
+/* precision modes bf16 / fp16 (fmt 1 / 2): 16-bit SH coefficients, f32 moments */
+void orc_sh_adam_h16_f32(const orc_adam* hp, int64_t n, int nc, uint16_t* sh, const float* g,
+                         float* m, float* v, uint32_t step, int fmt);
+void orc_f2h_rne(int64_t count, const float* src, uint16_t* dst);
+void orc_f2h_sr(int64_t count, const float* src, const uint32_t* rnd, uint16_t* dst);
+void orc_f2h_up(int64_t count, const float* src, uint16_t* dst);
+void orc_h2f(int64_t count, const uint16_t* src, float* dst);
 /* mixed precision: bf16 moments and SH coefficients, stochastic rounding */
 void orc_adam_mixed_f32(const orc_adam* hp, int64_t n, float* params, const float* gradF,
                         uint16_t* m, uint16_t* v, uint32_t step);

@@ -162,6 +162,7 @@ void FN(orc_project)(const orc_opts* o, const orc_cam* cam, int64_t n,
   const int tilesX = (cam->W + 15) / 16;
   const int tilesY = (cam->H + 15) / 16;
   const REAL W = (REAL)cam->W, H = (REAL)cam->H;
+#pragma omp parallel for schedule(static) if (orc_threads > 1)
   for (int64_t i = 0; i < n; i++) {
     const float* P = params + 16 * i;
     REAL* pr = proj + 16 * i;
@@ -272,6 +273,7 @@ void FN(orc_render_fwd)(const orc_opts* o, const orc_cam* cam, const REAL* proj,
                         REAL* img, REAL* Tfin, uint32_t* endOut) {
   const REAL E_CUT = o->eCut, A_MIN = o->aMin;
   const int W = cam->W, H = cam->H, tilesX = (W + 15) / 16;
+#pragma omp parallel for collapse(2) schedule(dynamic, 64) if (orc_threads > 1)
   for (int y = 0; y < H; y++)
     for (int x = 0; x < W; x++) {
       const int tile = (y / 16) * tilesX + (x / 16);
@@ -344,6 +346,8 @@ void FN(orc_render_bwd)(const orc_opts* o, const orc_cam* cam, const REAL* proj,
                         int32_t* gfix, double* gsum) {
   const REAL E_CUT = o->eCut, A_MIN = o->aMin;
   const int W = cam->W, H = cam->H, tilesX = (W + 15) / 16;
+  const int par = orc_threads > 1;
+#pragma omp parallel for collapse(2) schedule(dynamic, 64) if (par)
   for (int y = 0; y < H; y++)
     for (int x = 0; x < W; x++) {
       const int64_t pi = (int64_t)y * W + x;
@@ -407,10 +411,24 @@ void FN(orc_render_bwd)(const orc_opts* o, const orc_cam* cam, const REAL* proj,
             const float fv = (float)v[s] * 16384.0f;
             const float cl = fminf(fmaxf(fv, -1.0e9f), 1.0e9f);
             const int32_t iv = (int32_t)cl;
-            gfix[10 * (int64_t)gid + s] = (int32_t)((uint32_t)gfix[10 * (int64_t)gid + s] + (uint32_t)iv);
+            uint32_t* dst = (uint32_t*)&gfix[10 * (int64_t)gid + s];
+            if (par) {
+#pragma omp atomic
+              *dst += (uint32_t)iv;
+            } else {
+              *dst += (uint32_t)iv;
+            }
           }
         } else {
-          for (int s = 0; s < 10; s++) gsum[10 * (int64_t)gid + s] += (double)v[s];
+          for (int s = 0; s < 10; s++) {
+            double* dst = &gsum[10 * (int64_t)gid + s];
+            if (par) {
+#pragma omp atomic
+              *dst += (double)v[s];
+            } else {
+              *dst += (double)v[s];
+            }
+          }
         }
         GS = FMA(gcol, aT, GS);
         Ta = Tb;

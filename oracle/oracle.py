@@ -65,6 +65,12 @@ def lib():
             ("orc_loss_l1ssim_f64", [P, P, P, C.c_double, P, P, P], None),
             ("orc_adam_mixed_f32", [P, C.c_int64, P, P, P, P, C.c_uint32], None),
             ("orc_sh_adam_mixed_f32", [P, C.c_int64, C.c_int, P, P, P, P, C.c_uint32], None),
+            ("orc_set_threads", [C.c_int], None),
+            ("orc_sh_adam_h16_f32", [P, C.c_int64, C.c_int, P, P, P, P, C.c_uint32, C.c_int], None),
+            ("orc_f2h_rne", [C.c_int64, P, P], None),
+            ("orc_f2h_sr", [C.c_int64, P, P, P], None),
+            ("orc_f2h_up", [C.c_int64, P, P], None),
+            ("orc_h2f", [C.c_int64, P, P], None),
             ("orc_f2bf_rne", [C.c_int64, P, P], None),
             ("orc_f2bf_sr", [C.c_int64, P, P, P], None),
             ("orc_bf2f", [C.c_int64, P, P], None),
@@ -315,6 +321,50 @@ def sh_adam_mixed(hp, sh16, grad, m16, v16, step):
     n, nc = sh16.shape
     lib().orc_sh_adam_mixed_f32(C.byref(hp), n, nc, _p(sh16), _p(np.ascontiguousarray(grad, np.float32)),
                                 _p(m16), _p(v16), step)
+
+
+def set_threads(n):
+    """Threads of the restatement's per-Gaussian and per-pixel loops (OpenMP).
+    1, the default, keeps them serial, as the tests expect; above 1 the
+    fixed-point sums stay exact and double sums change order."""
+    lib().orc_set_threads(int(n))
+
+
+def sh_adam_h16(hp, sh16, grad, m, v, step, fmt):
+    """SH Adam with 16-bit coefficients (uint16, n x 3K; fmt 1 bf16, 2 fp16)
+    and f32 moments, stochastic rounding of the coefficient."""
+    n, nc = sh16.shape
+    lib().orc_sh_adam_h16_f32(C.byref(hp), n, nc, _p(sh16), _p(np.ascontiguousarray(grad, np.float32)),
+                              _p(m), _p(v), step, fmt)
+
+
+def f2h_rne(x):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(x.shape, np.uint16)
+    lib().orc_f2h_rne(x.size, _p(x), _p(out))
+    return out
+
+
+def f2h_sr(x, rnd):
+    x = np.ascontiguousarray(x, np.float32)
+    rnd = np.ascontiguousarray(rnd, np.uint32)
+    out = np.zeros(x.shape, np.uint16)
+    lib().orc_f2h_sr(x.size, _p(x), _p(rnd), _p(out))
+    return out
+
+
+def f2h_up(x):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros(x.shape, np.uint16)
+    lib().orc_f2h_up(x.size, _p(x), _p(out))
+    return out
+
+
+def h2f(b):
+    b = np.ascontiguousarray(b, np.uint16)
+    out = np.zeros(b.shape, np.float32)
+    lib().orc_h2f(b.size, _p(b), _p(out))
+    return out
 
 
 def f2bf_rne(x):

@@ -32,6 +32,7 @@ struct Api {
   int (*setup)(gs_trainer_t*, const float*, int64_t, const float*, const gs_camera*, int32_t,
                const gs_image*, int32_t, int32_t, int32_t, float);
   int (*step)(gs_trainer_t*, int32_t);
+  int (*step_views)(gs_trainer_t*, const int32_t*, int32_t);
   int (*read_loss)(gs_trainer_t*, double*, int32_t*);
   int (*render)(gs_trainer_t*, const gs_camera*, int32_t, int64_t, float*);
   int (*read_gaussians)(gs_trainer_t*, float*, float*);
@@ -87,6 +88,7 @@ bool load_api() {
   SYM(destroy, "gs_destroy");
   SYM(setup, "gs_setup");
   SYM(step, "gs_step");
+  SYM(step_views, "gs_step_views");
   SYM(read_loss, "gs_read_loss");
   SYM(render, "gs_render");
   SYM(read_gaussians, "gs_read_gaussians");
@@ -277,6 +279,8 @@ napi_value Create(napi_env env, napi_callback_info info) {
       o.precision = (int32_t)get_num(env, a, "precision", o.precision);
       o.camOpt = get_num(env, a, "camOpt", o.camOpt) != 0.0 ? 1 : 0;
       o.expComp = get_num(env, a, "expComp", o.expComp) != 0.0 ? 1 : 0;
+      o.viewsPerStep = (int32_t)get_num(env, a, "viewsPerStep", o.viewsPerStep);
+      o.graph = get_num(env, a, "graph", o.graph) != 0.0 ? 1 : 0;
     }
   }
   gs_trainer_t* h = nullptr;
@@ -373,6 +377,25 @@ napi_value Step(napi_env env, napi_callback_info info) {
   int32_t cam = -1;
   if (argc >= 2) napi_get_value_int32(env, argv[1], &cam);
   check(env, h, g_api.step(h, cam));
+  return nullptr;
+}
+
+// stepViews(h, cams[]): one batched step over those views (-1 picks)
+napi_value StepViews(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  gs_trainer_t* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  uint32_t len = 0;
+  if (argc < 2 || napi_get_array_length(env, argv[1], &len) != napi_ok) return throw_err(env, "stepViews: cams array");
+  std::vector<int32_t> cams(len);
+  for (uint32_t k = 0; k < len; k++) {
+    napi_value e;
+    napi_get_element(env, argv[1], k, &e);
+    napi_get_value_int32(env, e, &cams[k]);
+  }
+  check(env, h, g_api.step_views(h, cams.data(), (int32_t)len));
   return nullptr;
 }
 
@@ -781,6 +804,7 @@ napi_value Init(napi_env env, napi_value exports) {
     napi_callback cb;
   } fns[] = {
       {"create", Create},           {"setup", Setup},         {"step", Step},
+      {"stepViews", StepViews},
       {"readLoss", ReadLoss},       {"render", Render},       {"readGaussians", ReadGaussians},
       {"evalCamPsnr", EvalCamPsnr}, {"setHoldout", SetHoldout}, {"setExcluded", SetExcluded},
       {"setIter", SetIter},         {"info", Info},           {"sync", Sync},

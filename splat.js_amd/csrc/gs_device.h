@@ -301,29 +301,56 @@ __device__ __forceinline__ void load_sh(const float4* __restrict__ SH, int64_t c
   }
 }
 
-// bf16 planes: value c of a Gaussian lives in uint4 plane c/8, half c%8
-// (even halves in the low 16 bits of each 32-bit word).
-__device__ __forceinline__ void unpack_bf8(uint4 w, float* o) {
-  o[0] = gs_bf2f(w.x); o[1] = gs_bf2f(w.x >> 16);
-  o[2] = gs_bf2f(w.y); o[3] = gs_bf2f(w.y >> 16);
-  o[4] = gs_bf2f(w.z); o[5] = gs_bf2f(w.z >> 16);
-  o[6] = gs_bf2f(w.w); o[7] = gs_bf2f(w.w >> 16);
+// 16-bit planes (bf16 or fp16): value c of a Gaussian lives in uint4 plane
+// c/8, half c%8 (even halves in the low 16 bits of each 32-bit word).
+// kFmt 1 bf16, 2 fp16; both decode exactly.
+template <int kFmt>
+__device__ __forceinline__ float dec16(uint32_t b) {
+  if constexpr (kFmt == 2) return (float)__builtin_bit_cast(_Float16, (unsigned short)(b & 0xffffu));
+  else return gs_bf2f(b);
 }
+
+template <int kFmt>
+__device__ __forceinline__ void unpack16x8(uint4 w, float* o) {
+  o[0] = dec16<kFmt>(w.x); o[1] = dec16<kFmt>(w.x >> 16);
+  o[2] = dec16<kFmt>(w.y); o[3] = dec16<kFmt>(w.y >> 16);
+  o[4] = dec16<kFmt>(w.z); o[5] = dec16<kFmt>(w.z >> 16);
+  o[6] = dec16<kFmt>(w.w); o[7] = dec16<kFmt>(w.w >> 16);
+}
+
+__device__ __forceinline__ void unpack_bf8(uint4 w, float* o) { unpack16x8<1>(w, o); }
 
 __device__ __forceinline__ uint4 pack_bf8(const uint32_t* b) {
   return make_uint4(b[0] | (b[1] << 16), b[2] | (b[3] << 16), b[4] | (b[5] << 16), b[6] | (b[7] << 16));
 }
 
-template <int kDeg>
-__device__ __forceinline__ void load_sh_bf(const uint4* __restrict__ SH, int64_t cap, int64_t i, float* sh) {
+// Storage of each precision mode (gs_opts.precision):
+//   0 f32   everything f32
+//   1 mixed bf16 SH coefficients and bf16 Adam moments (parameters and SH)
+//   2 bf16  bf16 SH coefficients and raster-record colours; f32 moments
+//   3 fp16  fp16 SH coefficients and raster-record colours; f32 moments
+// Parameters, transmittance, gradients and all arithmetic are f32 in every
+// mode; stored values are decoded exactly.
+template <int kPrec>
+struct Prec {
+  static constexpr int sh = kPrec == 0 ? 0 : (kPrec == 3 ? 2 : 1);   // 0 f32 planes, 1 bf16, 2 fp16
+  static constexpr bool mom16 = kPrec == 1;                           // bf16 moments
+  static constexpr int rec = kPrec >= 2 ? kPrec - 1 : 0;              // record colours: 0 f32, 1 bf16, 2 fp16
+};
+
+// float4s per raster record: 4 (f32 colours) or 3 (16-bit colours)
+constexpr int rec_stride(int recFmt) { return recFmt == 0 ? 4 : 3; }
+
+template <int kDeg, int kFmt>
+__device__ __forceinline__ void load_sh16(const uint4* __restrict__ SH, int64_t cap, int64_t i, float* sh) {
 #pragma unroll
-  for (int c = 0; c < ShDims<kDeg>::Q8; c++) unpack_bf8(SH[c * cap + i], sh + 8 * c);
+  for (int c = 0; c < ShDims<kDeg>::Q8; c++) unpack16x8<kFmt>(SH[c * cap + i], sh + 8 * c);
 }
 
-// SH coefficients from either storage (kMixed: bf16 planes)
-template <int kDeg, bool kMixed>
+// SH coefficients from any storage (kSh 0: f32 planes, 1: bf16, 2: fp16)
+template <int kDeg, int kSh>
 __device__ __forceinline__ void load_sh_any(const void* __restrict__ SH, int64_t cap, int64_t i, float* sh) {
-  if constexpr (kMixed) load_sh_bf<kDeg>((const uint4*)SH, cap, i, sh);
+  if constexpr (kSh != 0) load_sh16<kDeg, kSh>((const uint4*)SH, cap, i, sh);
   else load_sh<kDeg>((const float4*)SH, cap, i, sh);
 }
 

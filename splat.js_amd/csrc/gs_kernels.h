@@ -58,6 +58,7 @@ struct BinArgs {
   const uint2* rect;
   const uint32_t* depthKey;
   const float4* rec;      // raster records: the emit pass derives each key's block mask
+  int recFmt;             // record colours: 0 f32 (64 B records), 1 bf16, 2 fp16 (48 B)
   uint32_t *dkA, *dvA, *dkB, *dvB;   // n-sized depth-sort buffers
   void *tkA, *tkB;                    // key buffers (4 or 8 B per key)
   uint32_t *tvA, *tvB;                // id buffers
@@ -78,6 +79,7 @@ struct BinOut {
 };
 
 struct RasterBufs {
+  int recFmt;             // record colours: 0 f32 (64 B records), 1 bf16, 2 fp16 (48 B)
   const uint2* ranges;
   const uint32_t* tileOrder;   // block b rasterizes tile tileOrder[b]
   const uint32_t* ids;
@@ -93,18 +95,36 @@ struct RasterBufs {
 struct UpdateBufs {
   int64_t n, cap;
   float4* P;              // 4 float4 planes
-  void *M, *V;            // 4 float4 planes, or 2 bf16 uint4 planes (mixed)
-  void *SH, *SHM, *SHV;   // ShDims::P4 float4 planes, or ShDims::Q8 bf16 planes (mixed)
+  void *M, *V;            // 4 float4 planes, or 2 bf16 uint4 planes (precision mode 1)
+  void *SH, *SHM, *SHV;   // ShDims::P4 float4 planes, or ShDims::Q8 16-bit planes (SH in modes 1-3,
+                          // SH moments in mode 1)
   void* accum;            // n x 12 (10 used) i32 or f32
   const uint32_t* cnt;
   float* dbgGradF;        // optional n x 16
   float* dbgShGrad;       // optional n x 3K
 };
 
+// ---- batched views (several training views per step) ----
+constexpr int kMaxViews = 16;
+
+// One batched step's per-view data, in device memory, rewritten before each
+// step: the views' camera constants, raster buffers and gradient sums, the
+// work-item offsets of the persistent raster launches and the step's Adam
+// constants.
+struct ViewBlock {
+  int count;                       // views V <= kMaxViews
+  uint32_t off[kMaxViews + 1];     // first work item (tile) of each view; off[V] = all
+  Cam cams[kMaxViews];
+  RasterBufs bufs[kMaxViews];
+  void* accum[kMaxViews];          // screen-space gradient sums of each view
+  const uint32_t* cnt[kMaxViews];  // tiles covered per Gaussian in each view (0 = culled)
+  AdamHP hp;
+};
+
 void launch_fill_u32(hipStream_t st, uint32_t* p, int64_t count, uint32_t v);
 
 // chunkKeys (may be null): += each chunk of kSegChunk Gaussians' key count
-void launch_project(hipStream_t st, int deg, bool mixed, int64_t n, int64_t cap, const Cam& cam,
+void launch_project(hipStream_t st, int deg, int prec, int64_t n, int64_t cap, const Cam& cam,
                     const Cut& cut, const float4* P, const void* SH, float4* rec, uint32_t* depthKey,
                     uint2* rect, uint32_t* cnt, uint32_t* chunkKeys);
 
@@ -121,8 +141,24 @@ void launch_loss_l1ssim(hipStream_t st, const Cam& cam, float lambda, const Rast
 void launch_raster_bwd(hipStream_t st, const Cam& cam, const Cut& cut, const RasterBufs& b,
                        int accumMode, void* accum, bool boundedColourGrad);
 
-void launch_update(hipStream_t st, int deg, int accumMode, bool mixed, const Cam& cam, const AdamHP& hp,
+void launch_update(hipStream_t st, int deg, int accumMode, int prec, const Cam& cam, const AdamHP& hp,
                    const UpdateBufs& u);
+
+// Batched views: the projection of view v with its camera from vb->cams[v];
+// one persistent forward and one persistent backward launch over every
+// view's tiles (queue: a counter zeroed before each launch; blocks from
+// raster_views_blocks); one update that chains each view's sums through
+// its camera, adds the views' gradients and takes one Adam step (vb->hp).
+void launch_project_view(hipStream_t st, int deg, int prec, int64_t n, int64_t cap, const ViewBlock* vb, int v,
+                         const Cut& cut, const float4* P, const void* SH, float4* rec, uint32_t* depthKey,
+                         uint2* rect, uint32_t* cnt, uint32_t* chunkKeys);
+void launch_raster_fwd_views(hipStream_t st, const ViewBlock* vb, const Cut& cut, int loss, uint32_t* queue,
+                             int blocks);
+void launch_raster_bwd_views(hipStream_t st, const ViewBlock* vb, const Cut& cut, int accumMode,
+                             bool boundedColourGrad, uint32_t* queue, int blocks);
+int raster_views_blocks(int device);
+void launch_update_views(hipStream_t st, int deg, int accumMode, int prec, const ViewBlock* vb,
+                         const UpdateBufs& u);
 
 // ---- camera pose, focal and exposure gradients (camOpt) ----
 struct CamPoseBufs {
@@ -157,13 +193,13 @@ struct RefineOp {
 
 // q[i] = (opacity logit, log-scale x, y, z) for i < n
 void launch_refine_gather(hipStream_t st, int64_t n, int64_t cap, const float4* P, float4* q);
-void launch_refine_apply(hipStream_t st, int deg, bool mixed, int64_t cap, const RefineOp* ops, int64_t count,
+void launch_refine_apply(hipStream_t st, int deg, int prec, int64_t cap, const RefineOp* ops, int64_t count,
                          float4* P, void* M, void* V, void* SH, void* SHM, void* SHV);
 
 // ---- PLY export rows (gs_export_ply) ----
 // out: rows first .. first + count - 1, each (17 + 3K) f32, from out[0];
 // camPos: nc x 3 f32 (bake only)
-void launch_ply_rows(hipStream_t st, int deg, bool mixed, int64_t first, int64_t count, int64_t cap,
+void launch_ply_rows(hipStream_t st, int deg, int prec, int64_t first, int64_t count, int64_t cap,
                      const float4* P, const void* SH, int bake, double f, const float* camPos, int nc,
                      float* out);
 

@@ -212,6 +212,81 @@ GS_HD uint32_t gs_f2bf_sr(float x, uint32_t rnd) {
   return x != x ? (u >> 16) | 0x40u : r;
 }
 
+// ---- IEEE binary16 (fp16) storage ----
+// The same roles as bf16 above with 10 mantissa bits and 5 exponent bits
+// (normal range [2^-14, 65504], subnormal spacing 2^-24). Integer and exact
+// f32 operations only, so host and device round identically.
+
+// exact decode
+GS_HD float gs_h2f(uint32_t h) {
+  const uint32_t s = (h & 0x8000u) << 16, e = (h >> 10) & 0x1fu, m = h & 0x3ffu;
+  if (e == 0u) {
+    const float v = (float)m * 5.9604644775390625e-08f;   // m * 2^-24, exact
+    return s ? -v : v;
+  }
+  if (e == 31u) return gs_u2f(s | 0x7f800000u | (m << 13));
+  return gs_u2f(s | ((e + 112u) << 23) | (m << 13));
+}
+
+// Round to nearest even. Normal results: rebias the exponent and round the
+// 13 dropped mantissa bits as gs_f2bf_rne does; ties at 65520 and above go
+// to infinity. Subnormal results: |x| * 2^24 (exact) rounded to an integer
+// by the 2^23 shift (IEEE round to nearest even).
+GS_HD uint32_t gs_f2h_rne(float x) {
+  const uint32_t u = gs_f2u(x);
+  const uint32_t s = (u >> 16) & 0x8000u, a = u & 0x7fffffffu;
+  if (a > 0x7f800000u) return s | 0x7e00u;
+  if (a >= 0x477ff000u) return s | 0x7c00u;
+  if (a >= 0x38800000u) {
+    const uint32_t r = a - 0x38000000u;
+    return s | ((r + 0x0fffu + ((r >> 13) & 1u)) >> 13);
+  }
+  const float v = gs_u2f(a) * 16777216.0f;
+  const float q = (v + 8388608.0f) - 8388608.0f;
+  return s | (uint32_t)q;
+}
+
+// Stochastic rounding with 16 random bits: a value rounds up with
+// probability equal to its distance from the lower neighbour (to 2^-13 for
+// normal results, 2^-16 for subnormal ones). Infinity and NaN are kept;
+// results past 65504 may round to infinity.
+GS_HD uint32_t gs_f2h_sr(float x, uint32_t rnd) {
+  const uint32_t u = gs_f2u(x);
+  const uint32_t s = (u >> 16) & 0x8000u, a = u & 0x7fffffffu;
+  if (a > 0x7f800000u) return s | 0x7e00u;
+  if (a >= 0x47800000u) return s | 0x7c00u;
+  if (a >= 0x38800000u) {
+    const uint32_t h = (a - 0x38000000u + ((rnd >> 3) & 0x1fffu)) >> 13;
+    return s | (h > 0x7c00u ? 0x7c00u : h);
+  }
+  const float v = gs_u2f(a) * 16777216.0f;   // < 1024, exact
+  const uint32_t fl = (uint32_t)v;           // floor
+  const uint32_t frac16 = (uint32_t)((v - (float)fl) * 65536.0f);
+  return s | (fl + ((rnd & 0xffffu) < frac16 ? 1u : 0u));
+}
+
+// smallest fp16 at or above x (x >= 0 finite): a conservative bound
+GS_HD uint32_t gs_f2h_up(float x) {
+  const uint32_t h = gs_f2h_rne(x);
+  return gs_h2f(h) < x ? h + 1u : h;
+}
+
+#if defined(__cplusplus)
+// 16-bit storage formats: 1 bf16, 2 fp16
+template <int kFmt>
+GS_HD float gs_16_to_f(uint32_t b) {
+  return kFmt == 2 ? gs_h2f(b & 0xffffu) : gs_bf2f(b);
+}
+template <int kFmt>
+GS_HD uint32_t gs_f_to_16_rne(float x) {
+  return kFmt == 2 ? gs_f2h_rne(x) : gs_f2bf_rne(x);
+}
+template <int kFmt>
+GS_HD uint32_t gs_f_to_16_sr(float x, uint32_t rnd) {
+  return kFmt == 2 ? gs_f2h_sr(x, rnd) : gs_f2bf_sr(x, rnd);
+}
+#endif
+
 // 1 / (1 + e^-clamp(x, -9, 9)): the activation for colour and opacity logits.
 GS_HD float gs_sigmoid_clamped(float x) {
   const float c = fminf(fmaxf(x, -9.0f), 9.0f);

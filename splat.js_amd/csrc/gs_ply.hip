@@ -10,7 +10,7 @@
 namespace gs {
 
 // Rows first .. first + count - 1; row i goes to out + (i - first) * W.
-template <int kDeg, bool kMixed>
+template <int kDeg, int kPrec>
 __global__ __launch_bounds__(kBlock) void k_ply_rows(int64_t first, int64_t count, int64_t cap,
                                                     const float4* __restrict__ P,
                                                     const void* __restrict__ SH, int bake, double f,
@@ -48,7 +48,7 @@ __global__ __launch_bounds__(kBlock) void k_ply_rows(int64_t first, int64_t coun
   for (int c = 0; c < 3; c++) o[6 + c] = (float)((1.0 / (1.0 + exp(-(double)p[10 + c])) - 0.5) / C0);
   if constexpr (kDeg > 0) {
     float sh[ShDims<kDeg>::S];
-    load_sh_any<kDeg, kMixed>(SH, cap, i, sh);
+    load_sh_any<kDeg, Prec<kPrec>::sh>(SH, cap, i, sh);
     for (int k = 0; k < NC; k++) o[9 + k] = sh[k];
   }
   float* q = o + 9 + NC;
@@ -61,24 +61,26 @@ __global__ __launch_bounds__(kBlock) void k_ply_rows(int64_t first, int64_t coun
   q[4] = (float)qw; q[5] = (float)qx; q[6] = (float)qy; q[7] = (float)qz;
 }
 
-template <bool kMixed>
+template <int kPrec>
 static void ply_prec(hipStream_t st, int deg, int64_t first, int64_t count, int64_t cap, const float4* P,
                      const void* SH, int bake, double f, const float* camPos, int nc, float* out) {
   const dim3 grid((unsigned)((count + kBlock - 1) / kBlock));
   switch (deg) {
-    case 0: k_ply_rows<0, kMixed><<<grid, kBlock, 0, st>>>(first, count, cap, P, SH, bake, f, camPos, nc, out); break;
-    case 1: k_ply_rows<1, kMixed><<<grid, kBlock, 0, st>>>(first, count, cap, P, SH, bake, f, camPos, nc, out); break;
-    case 2: k_ply_rows<2, kMixed><<<grid, kBlock, 0, st>>>(first, count, cap, P, SH, bake, f, camPos, nc, out); break;
-    default: k_ply_rows<3, kMixed><<<grid, kBlock, 0, st>>>(first, count, cap, P, SH, bake, f, camPos, nc, out); break;
+    case 0: k_ply_rows<0, kPrec><<<grid, kBlock, 0, st>>>(first, count, cap, P, SH, bake, f, camPos, nc, out); break;
+    case 1: k_ply_rows<1, kPrec><<<grid, kBlock, 0, st>>>(first, count, cap, P, SH, bake, f, camPos, nc, out); break;
+    case 2: k_ply_rows<2, kPrec><<<grid, kBlock, 0, st>>>(first, count, cap, P, SH, bake, f, camPos, nc, out); break;
+    default: k_ply_rows<3, kPrec><<<grid, kBlock, 0, st>>>(first, count, cap, P, SH, bake, f, camPos, nc, out); break;
   }
 }
 
-void launch_ply_rows(hipStream_t st, int deg, bool mixed, int64_t first, int64_t count, int64_t cap,
+void launch_ply_rows(hipStream_t st, int deg, int prec, int64_t first, int64_t count, int64_t cap,
                      const float4* P, const void* SH, int bake, double f, const float* camPos, int nc,
                      float* out) {
   if (count <= 0) return;
-  if (mixed) ply_prec<true>(st, deg, first, count, cap, P, SH, bake, f, camPos, nc, out);
-  else ply_prec<false>(st, deg, first, count, cap, P, SH, bake, f, camPos, nc, out);
+  // only the SH storage matters here: f32, bf16 (modes 1 and 2) or fp16
+  if (prec == 0) ply_prec<0>(st, deg, first, count, cap, P, SH, bake, f, camPos, nc, out);
+  else if (prec == 3) ply_prec<3>(st, deg, first, count, cap, P, SH, bake, f, camPos, nc, out);
+  else ply_prec<1>(st, deg, first, count, cap, P, SH, bake, f, camPos, nc, out);
 }
 
 }  // namespace gs

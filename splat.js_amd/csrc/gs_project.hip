@@ -3,23 +3,28 @@
 // compensation, SH colour, opacity-aware radius, tile rectangle.
 //
 // Output per visible Gaussian:
-//   rec[4i..4i+3]  raster record, 64 B: (mx, my, A, B), (C, opa*comp, L, 0),
-//                  (r, g, b, opa), (comp, -0.5 cnorm, 1 - opa, 0). L (gs_cover.h) bounds
-//                  the exponent at which a pixel can still take the splat
-//                  (min(eCut, ln(opa*comp / A_MIN)) plus rounding slack); the
-//                  raster passes use it only to skip whole waves, never to
-//                  change a pixel's value.
+//   rec            raster record, 64 B with f32 colours: (mx, my, A, B),
+//                  (C, opa*comp, L, 0), (r, g, b, opa), (comp, -0.5 cnorm,
+//                  1 - opa, 0); 48 B with 16-bit colours (precision modes
+//                  bf16 and fp16, rounded to nearest even): (mx, my, A, B),
+//                  (C, opa*comp, r | g << 16, b | L16 << 16), (comp,
+//                  -0.5 cnorm, 1 - opa, opa), with L16 the smallest fp16 at or
+//                  above L. L (gs_cover.h) bounds the exponent at which a pixel
+//                  can still take the splat (min(eCut, ln(opa*comp / A_MIN))
+//                  plus rounding slack); the raster passes use it only to
+//                  skip whole waves, never to change a pixel's value, so any
+//                  larger bound is as correct.
 //   depthKey[i]    f32 depth bits (positive, so they order like the depths);
 //                  0xffffffff when culled, which sorts culled Gaussians last
 //   rect[i]        (tx0 | tx1 << 16, ty0 | ty1 << 16)
 //   cnt[i]         tiles covered, 0 when culled
-// HBM-bound: 56-64 B of params + 4*3K B of SH (2*3K B in mixed precision)
-// read, 64 B of record and 16 B of key/rect/count written per visible Gaussian.
+// Reads 56-64 B of params + 4*3K B of SH (2*3K B when stored in 16 bits),
+// writes 64 B (48 B) of record and 16 B of key/rect/count per visible Gaussian.
 #include "gs_kernels.h"
 
 namespace gs {
 
-template <int kDeg, bool kMixed>
+template <int kDeg, int kPrec>
 __device__ __forceinline__ uint32_t project_one(int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
                                                 const float4* __restrict__ P, const void* __restrict__ SH,
                                                 float4* __restrict__ rec, uint32_t* __restrict__ depthKey,
@@ -58,7 +63,7 @@ __device__ __forceinline__ uint32_t project_one(int64_t n, int64_t cap, const Ca
       float Y[15];
       sh_basis<kDeg, false>(v, Y, nullptr);
       float sh[ShDims<kDeg>::S];
-      load_sh_any<kDeg, kMixed>(SH, cap, i, sh);
+      load_sh_any<kDeg, Prec<kPrec>::sh>(SH, cap, i, sh);
       const int aK = cam.shActiveK;
 #pragma unroll
       for (int k = 0; k < K; k++) {
@@ -73,10 +78,19 @@ __device__ __forceinline__ uint32_t project_one(int64_t n, int64_t cap, const Ca
       col[2] = fmaxf(col[2], 0.0f);
     }
     const float cnorm = 1.0f / (1.0f + conic_lmax(g.va, g.vb, g.vc));
-    rec[4 * i + 0] = make_float4(mx, my, cd * inv, (-g.vb) * inv);
-    rec[4 * i + 1] = make_float4(ad * inv, opa * comp, gs_cover_level(eMax), 0.0f);
-    rec[4 * i + 2] = make_float4(col[0], col[1], col[2], opa);
-    rec[4 * i + 3] = make_float4(comp, -0.5f * cnorm, 1.0f - opa, 0.0f);
+    if constexpr (Prec<kPrec>::rec == 0) {
+      rec[4 * i + 0] = make_float4(mx, my, cd * inv, (-g.vb) * inv);
+      rec[4 * i + 1] = make_float4(ad * inv, opa * comp, gs_cover_level(eMax), 0.0f);
+      rec[4 * i + 2] = make_float4(col[0], col[1], col[2], opa);
+      rec[4 * i + 3] = make_float4(comp, -0.5f * cnorm, 1.0f - opa, 0.0f);
+    } else {
+      constexpr int F = Prec<kPrec>::rec;
+      const uint32_t rg = gs_f_to_16_rne<F>(col[0]) | (gs_f_to_16_rne<F>(col[1]) << 16);
+      const uint32_t bl = gs_f_to_16_rne<F>(col[2]) | (gs_f2h_up(gs_cover_level(eMax)) << 16);
+      rec[3 * i + 0] = make_float4(mx, my, cd * inv, (-g.vb) * inv);
+      rec[3 * i + 1] = make_float4(ad * inv, opa * comp, __uint_as_float(rg), __uint_as_float(bl));
+      rec[3 * i + 2] = make_float4(comp, -0.5f * cnorm, 1.0f - opa, opa);
+    }
     key = gs_f2u(g.pc[2]);
     const float TF = 16.0f;
     const int tx0 = (int)clampf(floorf((mx - rad) / TF), 0.0f, (float)(cam.tilesX - 1));
@@ -93,17 +107,14 @@ __device__ __forceinline__ uint32_t project_one(int64_t n, int64_t cap, const Ca
   return c;
 }
 
-template <int kDeg, bool kMixed>
-__global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam cam, Cut cut,
-                                                    const float4* __restrict__ P,
-                                                    const void* __restrict__ SH,
-                                                    float4* __restrict__ rec,
-                                                    uint32_t* __restrict__ depthKey,
-                                                    uint2* __restrict__ rect,
-                                                    uint32_t* __restrict__ cnt,
-                                                    uint32_t* __restrict__ chunkKeys) {
+template <int kDeg, int kPrec>
+__device__ __forceinline__ void project_thread(int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
+                                               const float4* __restrict__ P, const void* __restrict__ SH,
+                                               float4* __restrict__ rec, uint32_t* __restrict__ depthKey,
+                                               uint2* __restrict__ rect, uint32_t* __restrict__ cnt,
+                                               uint32_t* __restrict__ chunkKeys) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t c = i < n ? project_one<kDeg, kMixed>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, i) : 0u;
+  const uint32_t c = i < n ? project_one<kDeg, kPrec>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, i) : 0u;
   if (chunkKeys) {
     // the wave's 64 Gaussians lie in one chunk (kSegChunk is a multiple of 64)
     uint32_t s = c;
@@ -113,25 +124,78 @@ __global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam 
   }
 }
 
-template <bool kMixed>
+template <int kDeg, int kPrec>
+__global__ __launch_bounds__(kBlock) void k_project(int64_t n, int64_t cap, Cam cam, Cut cut,
+                                                    const float4* __restrict__ P,
+                                                    const void* __restrict__ SH,
+                                                    float4* __restrict__ rec,
+                                                    uint32_t* __restrict__ depthKey,
+                                                    uint2* __restrict__ rect,
+                                                    uint32_t* __restrict__ cnt,
+                                                    uint32_t* __restrict__ chunkKeys) {
+  project_thread<kDeg, kPrec>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys);
+}
+
+// view v of a batched step: the camera constants come from device memory
+template <int kDeg, int kPrec>
+__global__ __launch_bounds__(kBlock) void k_project_view(int64_t n, int64_t cap, const ViewBlock* __restrict__ vb,
+                                                         int v, Cut cut, const float4* __restrict__ P,
+                                                         const void* __restrict__ SH, float4* __restrict__ rec,
+                                                         uint32_t* __restrict__ depthKey, uint2* __restrict__ rect,
+                                                         uint32_t* __restrict__ cnt,
+                                                         uint32_t* __restrict__ chunkKeys) {
+  const Cam cam = vb->cams[v];
+  project_thread<kDeg, kPrec>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys);
+}
+
+template <int kPrec>
 static void launch_project_t(hipStream_t st, int deg, int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
                              const float4* P, const void* SH, float4* rec, uint32_t* depthKey, uint2* rect,
                              uint32_t* cnt, uint32_t* chunkKeys) {
   const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
   switch (deg) {
-    case 0: k_project<0, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
-    case 1: k_project<1, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
-    case 2: k_project<2, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
-    default: k_project<3, kMixed><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    case 0: k_project<0, kPrec><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    case 1: k_project<1, kPrec><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    case 2: k_project<2, kPrec><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    default: k_project<3, kPrec><<<grid, kBlock, 0, st>>>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
   }
 }
 
-void launch_project(hipStream_t st, int deg, bool mixed, int64_t n, int64_t cap, const Cam& cam,
+template <int kPrec>
+static void launch_project_view_t(hipStream_t st, int deg, int64_t n, int64_t cap, const ViewBlock* vb, int v,
+                                  const Cut& cut, const float4* P, const void* SH, float4* rec, uint32_t* depthKey,
+                                  uint2* rect, uint32_t* cnt, uint32_t* chunkKeys) {
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
+  switch (deg) {
+    case 0: k_project_view<0, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, v, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    case 1: k_project_view<1, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, v, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    case 2: k_project_view<2, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, v, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    default: k_project_view<3, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, v, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+  }
+}
+
+void launch_project_view(hipStream_t st, int deg, int prec, int64_t n, int64_t cap, const ViewBlock* vb, int v,
+                         const Cut& cut, const float4* P, const void* SH, float4* rec, uint32_t* depthKey,
+                         uint2* rect, uint32_t* cnt, uint32_t* chunkKeys) {
+  if (n <= 0) return;
+  switch (prec) {
+    case 0: launch_project_view_t<0>(st, deg, n, cap, vb, v, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    case 1: launch_project_view_t<1>(st, deg, n, cap, vb, v, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    case 2: launch_project_view_t<2>(st, deg, n, cap, vb, v, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    default: launch_project_view_t<3>(st, deg, n, cap, vb, v, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+  }
+}
+
+void launch_project(hipStream_t st, int deg, int prec, int64_t n, int64_t cap, const Cam& cam,
                     const Cut& cut, const float4* P, const void* SH, float4* rec, uint32_t* depthKey,
                     uint2* rect, uint32_t* cnt, uint32_t* chunkKeys) {
   if (n <= 0) return;
-  if (mixed) launch_project_t<true>(st, deg, n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys);
-  else launch_project_t<false>(st, deg, n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys);
+  switch (prec) {
+    case 0: launch_project_t<0>(st, deg, n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    case 1: launch_project_t<1>(st, deg, n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    case 2: launch_project_t<2>(st, deg, n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+    default: launch_project_t<3>(st, deg, n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys); break;
+  }
 }
 
 }  // namespace gs

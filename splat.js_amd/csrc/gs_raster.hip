@@ -100,15 +100,27 @@ __device__ __forceinline__ float rcp_exact_unit(float x) {
   return fmaf(r, e, r);
 }
 
+// The record's colours as f32: stored as f32 (recFmt 0: (r, g, b, opa) is
+// the third float4) or as bf16 / fp16 pairs in the second float4's z, w
+// (recFmt 1, 2), which decode exactly.
+__device__ __forceinline__ float4 record_colour(int recFmt, const float4& r1, float w) {
+  const uint32_t rg = __float_as_uint(r1.z), bl = __float_as_uint(r1.w);
+  if (recFmt == 2)
+    return make_float4(dec16<2>(rg), dec16<2>(rg >> 16), dec16<2>(bl), w);
+  return make_float4(dec16<1>(rg), dec16<1>(rg >> 16), dec16<1>(bl), w);
+}
+
+// One tile of the forward pass, by the whole 256-thread block. The block's
+// threads must all call it (it synchronises), and nothing else may use the
+// block's LDS until every thread has returned.
 template <int kLoss>
-__global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterBufs b) {
+__device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, const RasterBufs& b, int tile) {
   __shared__ float4 sA[kBlock];   // mx, my, A, B
   __shared__ float4 sB[kBlock];   // C, opa*comp, L, 0
   __shared__ float4 sC[kBlock];   // r, g, b, opa
   __shared__ uint64_t sMask[4][4];
   __shared__ double sStat[3][kBlock / 64];
   const int tid = threadIdx.x, wv = tid >> 6;
-  const int tile = tile_of_block(b);
   const PixelMap pm = pixel_of_thread(cam, tile, tid);
   const int x = pm.x, y = pm.y;
   const bool inside = x < cam.W && y < cam.H;
@@ -127,9 +139,16 @@ __global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterB
       cover = v >> kMaskShift;
       if (cover != 0u) {           // splats no block of the tile can take stay unloaded
         const uint32_t id = v & kIdMask;
-        sA[tid] = b.rec[4 * id + 0];
-        sB[tid] = b.rec[4 * id + 1];
-        sC[tid] = b.rec[4 * id + 2];
+        if (b.recFmt == 0) {
+          sA[tid] = b.rec[4 * id + 0];
+          sB[tid] = b.rec[4 * id + 1];
+          sC[tid] = b.rec[4 * id + 2];
+        } else {
+          const float4 r1 = b.rec[3 * id + 1];
+          sA[tid] = b.rec[3 * id + 0];
+          sB[tid] = r1;
+          sC[tid] = record_colour(b.recFmt, r1, 0.0f);
+        }
       }
     }
     build_masks(cover, sMask);
@@ -310,8 +329,10 @@ __device__ __forceinline__ int reduce10_slot() {
 // gain * |err| / sqrt(err^2 + d^2) <= gain), so the colour terms
 // 16384 dL/dC * alpha * T (alpha * T <= 1) stay far inside the +-1e9 clamp
 // and convert without it.
+// One tile of the backward pass, by the whole block (as raster_fwd_tile).
 template <int kAccum, bool kClampColour>
-__global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterBufs b, void* accum) {
+__device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, const RasterBufs& b, void* accum,
+                                                int tile) {
   // four 16-byte records per staged splat, all at index j: one LDS address
   __shared__ float4 sA[kBlock];   // mx, my, A, B
   __shared__ float4 sB[kBlock];   // C, opa*comp, L, 0
@@ -320,7 +341,6 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
   __shared__ uint64_t sMask[4][4];
   __shared__ uint32_t sEnd;
   const int tid = threadIdx.x, wv = tid >> 6;
-  const int tile = tile_of_block(b);
   const PixelMap pm = pixel_of_thread(cam, tile, tid);
   const int x = pm.x, y = pm.y;
   const bool inside = x < cam.W && y < cam.H;
@@ -373,11 +393,20 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
       cover = v >> kMaskShift;
       if (cover != 0u) {
         const uint32_t id = v & kIdMask;
-        sA[tid] = b.rec[4 * id + 0];
-        sB[tid] = b.rec[4 * id + 1];
-        sC[tid] = b.rec[4 * id + 2];
-        const float4 d = b.rec[4 * id + 3];
-        sD[tid] = make_float4(d.y, d.z, __uint_as_float(id), 0.0f);
+        if (b.recFmt == 0) {
+          sA[tid] = b.rec[4 * id + 0];
+          sB[tid] = b.rec[4 * id + 1];
+          sC[tid] = b.rec[4 * id + 2];
+          const float4 d = b.rec[4 * id + 3];
+          sD[tid] = make_float4(d.y, d.z, __uint_as_float(id), 0.0f);
+        } else {
+          const float4 r1 = b.rec[3 * id + 1];
+          const float4 d = b.rec[3 * id + 2];   // comp, -0.5 cnorm, 1 - opa, opa
+          sA[tid] = b.rec[3 * id + 0];
+          sB[tid] = r1;
+          sC[tid] = record_colour(b.recFmt, r1, d.w);
+          sD[tid] = make_float4(d.y, d.z, __uint_as_float(id), 0.0f);
+        }
       }
     }
     build_masks(cover, sMask);
@@ -472,6 +501,64 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterB
   }
 }
 
+template <int kLoss>
+__global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterBufs b) {
+  raster_fwd_tile<kLoss>(cam, cut, b, tile_of_block(b));
+}
+
+template <int kAccum, bool kClampColour>
+__global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterBufs b, void* accum) {
+  raster_bwd_tile<kAccum, kClampColour>(cam, cut, b, accum, tile_of_block(b));
+}
+
+// ---- several views in one persistent launch ----
+// Work item w in [0, off[V]) is tile tileOrder_v[w - off[v]] of the view v
+// with off[v] <= w < off[v + 1]: each view's tiles in its own heaviest-first
+// order, the views one after another. Each block takes the next item from
+// the queue counter (zeroed before the launch) until the items run out, so
+// every block leaves once the counter passes off[V].
+struct ViewItem {
+  int v;
+  int tile;
+};
+
+__device__ __forceinline__ bool next_item(const ViewBlock* __restrict__ vb, uint32_t* queue, ViewItem& it) {
+  __shared__ uint32_t sWork;
+  __syncthreads();   // the previous item's LDS reads are done
+  if (threadIdx.x == 0) sWork = atomicAdd(queue, 1u);
+  __syncthreads();
+  const uint32_t w = __builtin_amdgcn_readfirstlane(sWork);
+  const int V = vb->count;
+  if (w >= vb->off[V]) return false;
+  int v = 0;
+  while (w >= vb->off[v + 1]) v++;
+  it.v = v;
+  it.tile = (int)vb->bufs[v].tileOrder[w - vb->off[v]];
+  return true;
+}
+
+template <int kLoss>
+__global__ __launch_bounds__(kBlock) void k_raster_fwd_views(const ViewBlock* __restrict__ vb, Cut cut,
+                                                            uint32_t* queue) {
+  ViewItem it;
+  while (next_item(vb, queue, it)) {
+    const Cam cam = vb->cams[it.v];
+    const RasterBufs b = vb->bufs[it.v];
+    raster_fwd_tile<kLoss>(cam, cut, b, it.tile);
+  }
+}
+
+template <int kAccum, bool kClampColour>
+__global__ __launch_bounds__(kBlock) void k_raster_bwd_views(const ViewBlock* __restrict__ vb, Cut cut,
+                                                            uint32_t* queue) {
+  ViewItem it;
+  while (next_item(vb, queue, it)) {
+    const Cam cam = vb->cams[it.v];
+    const RasterBufs b = vb->bufs[it.v];
+    raster_bwd_tile<kAccum, kClampColour>(cam, cut, b, vb->accum[it.v], it.tile);
+  }
+}
+
 void launch_raster_fwd(hipStream_t st, const Cam& cam, const Cut& cut, const RasterBufs& b, int loss) {
   const dim3 grid((unsigned)cam.numTiles);
   if (loss == 1) k_raster_fwd<1><<<grid, kBlock, 0, st>>>(cam, cut, b);
@@ -485,6 +572,32 @@ void launch_raster_bwd(hipStream_t st, const Cam& cam, const Cut& cut, const Ras
   if (accumMode != 0) k_raster_bwd<1, true><<<grid, kBlock, 0, st>>>(cam, cut, b, accum);
   else if (boundedColourGrad) k_raster_bwd<0, false><<<grid, kBlock, 0, st>>>(cam, cut, b, accum);
   else k_raster_bwd<0, true><<<grid, kBlock, 0, st>>>(cam, cut, b, accum);
+}
+
+void launch_raster_fwd_views(hipStream_t st, const ViewBlock* vb, const Cut& cut, int loss, uint32_t* queue,
+                             int blocks) {
+  if (loss == 1) k_raster_fwd_views<1><<<blocks, kBlock, 0, st>>>(vb, cut, queue);
+  else if (loss == 2) k_raster_fwd_views<2><<<blocks, kBlock, 0, st>>>(vb, cut, queue);
+  else k_raster_fwd_views<0><<<blocks, kBlock, 0, st>>>(vb, cut, queue);
+}
+
+void launch_raster_bwd_views(hipStream_t st, const ViewBlock* vb, const Cut& cut, int accumMode,
+                             bool boundedColourGrad, uint32_t* queue, int blocks) {
+  if (accumMode != 0) k_raster_bwd_views<1, true><<<blocks, kBlock, 0, st>>>(vb, cut, queue);
+  else if (boundedColourGrad) k_raster_bwd_views<0, false><<<blocks, kBlock, 0, st>>>(vb, cut, queue);
+  else k_raster_bwd_views<0, true><<<blocks, kBlock, 0, st>>>(vb, cut, queue);
+}
+
+// blocks of the persistent launches: as many as fit on the device at once
+int raster_views_blocks(int device) {
+  int cus = 0, per = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
+  int a = 0, b = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_raster_fwd_views<1>, kBlock, 0) != hipSuccess) a = 1;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_raster_bwd_views<0, false>, kBlock, 0) != hipSuccess) b = 1;
+  per = a < b ? a : b;
+  if (per < 1) per = 1;
+  return cus * per;
 }
 
 }  // namespace gs

@@ -24,7 +24,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_gather(int64_t n, int64_t cap
   q[i] = make_float4(d.y, a.w, b.x, b.y);   // slot 13, slots 3-5
 }
 
-template <int kDeg, bool kMixed>
+template <int kDeg, int kPrec>
 __global__ __launch_bounds__(kBlock) void k_refine_apply(int64_t cap, const RefineOp* __restrict__ ops,
                                                         int64_t count, float4* __restrict__ P, void* M, void* V,
                                                         void* SH, void* SHM, void* SHV) {
@@ -51,7 +51,7 @@ __global__ __launch_bounds__(kBlock) void k_refine_apply(int64_t cap, const Refi
   const int64_t who[2] = {t, d};
   for (int r = 0; r < (op.split ? 2 : 1); r++) {
     const int64_t g = who[r];
-    if constexpr (kMixed) {
+    if constexpr (Prec<kPrec>::mom16) {
       const uint4 z = make_uint4(0u, 0u, 0u, 0u);
       ((uint4*)M)[g] = z; ((uint4*)M)[cap + g] = z;
       ((uint4*)V)[g] = z; ((uint4*)V)[cap + g] = z;
@@ -62,20 +62,23 @@ __global__ __launch_bounds__(kBlock) void k_refine_apply(int64_t cap, const Refi
   }
   if constexpr (kDeg > 0) {
     // the clone inherits the donor's view dependence, with fresh moments
-    if constexpr (kMixed) {
-      constexpr int Q8 = ShDims<kDeg>::Q8;
+    constexpr int Q8 = ShDims<kDeg>::Q8;
+    constexpr int P4 = ShDims<kDeg>::P4;
+    if constexpr (Prec<kPrec>::sh != 0) {
+      for (int c = 0; c < Q8; c++) ((uint4*)SH)[c * cap + t] = ((const uint4*)SH)[c * cap + d];
+    } else {
+      for (int c = 0; c < P4; c++) ((float4*)SH)[c * cap + t] = ((const float4*)SH)[c * cap + d];
+    }
+    if constexpr (Prec<kPrec>::mom16) {
       const uint4 z = make_uint4(0u, 0u, 0u, 0u);
       for (int c = 0; c < Q8; c++) {
-        ((uint4*)SH)[c * cap + t] = ((const uint4*)SH)[c * cap + d];
         ((uint4*)SHM)[c * cap + t] = z;
         ((uint4*)SHV)[c * cap + t] = z;
         if (op.split) { ((uint4*)SHM)[c * cap + d] = z; ((uint4*)SHV)[c * cap + d] = z; }
       }
     } else {
-      constexpr int P4 = ShDims<kDeg>::P4;
       const float4 z = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
       for (int c = 0; c < P4; c++) {
-        ((float4*)SH)[c * cap + t] = ((const float4*)SH)[c * cap + d];
         ((float4*)SHM)[c * cap + t] = z;
         ((float4*)SHV)[c * cap + t] = z;
         if (op.split) { ((float4*)SHM)[c * cap + d] = z; ((float4*)SHV)[c * cap + d] = z; }
@@ -89,23 +92,25 @@ void launch_refine_gather(hipStream_t st, int64_t n, int64_t cap, const float4* 
   k_refine_gather<<<(unsigned)((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(n, cap, P, q);
 }
 
-template <bool kMixed>
+template <int kPrec>
 static void apply_prec(hipStream_t st, int deg, int64_t cap, const RefineOp* ops, int64_t count, float4* P,
                        void* M, void* V, void* SH, void* SHM, void* SHV) {
   const dim3 grid((unsigned)((count + kBlock - 1) / kBlock));
   switch (deg) {
-    case 0: k_refine_apply<0, kMixed><<<grid, kBlock, 0, st>>>(cap, ops, count, P, M, V, SH, SHM, SHV); break;
-    case 1: k_refine_apply<1, kMixed><<<grid, kBlock, 0, st>>>(cap, ops, count, P, M, V, SH, SHM, SHV); break;
-    case 2: k_refine_apply<2, kMixed><<<grid, kBlock, 0, st>>>(cap, ops, count, P, M, V, SH, SHM, SHV); break;
-    default: k_refine_apply<3, kMixed><<<grid, kBlock, 0, st>>>(cap, ops, count, P, M, V, SH, SHM, SHV); break;
+    case 0: k_refine_apply<0, kPrec><<<grid, kBlock, 0, st>>>(cap, ops, count, P, M, V, SH, SHM, SHV); break;
+    case 1: k_refine_apply<1, kPrec><<<grid, kBlock, 0, st>>>(cap, ops, count, P, M, V, SH, SHM, SHV); break;
+    case 2: k_refine_apply<2, kPrec><<<grid, kBlock, 0, st>>>(cap, ops, count, P, M, V, SH, SHM, SHV); break;
+    default: k_refine_apply<3, kPrec><<<grid, kBlock, 0, st>>>(cap, ops, count, P, M, V, SH, SHM, SHV); break;
   }
 }
 
-void launch_refine_apply(hipStream_t st, int deg, bool mixed, int64_t cap, const RefineOp* ops, int64_t count,
+void launch_refine_apply(hipStream_t st, int deg, int prec, int64_t cap, const RefineOp* ops, int64_t count,
                          float4* P, void* M, void* V, void* SH, void* SHM, void* SHV) {
   if (count <= 0) return;
-  if (mixed) apply_prec<true>(st, deg, cap, ops, count, P, M, V, SH, SHM, SHV);
-  else apply_prec<false>(st, deg, cap, ops, count, P, M, V, SH, SHM, SHV);
+  // SH and moment storage: modes 2 and 3 move the same planes
+  if (prec == 0) apply_prec<0>(st, deg, cap, ops, count, P, M, V, SH, SHM, SHV);
+  else if (prec == 1) apply_prec<1>(st, deg, cap, ops, count, P, M, V, SH, SHM, SHV);
+  else apply_prec<2>(st, deg, cap, ops, count, P, M, V, SH, SHM, SHV);
 }
 
 }  // namespace gs

@@ -377,6 +377,19 @@ __global__ __launch_bounds__(1024) void k_count_top(uint64_t* __restrict__ parti
 // kGlobal64: Gaussians in id order, key = (tile << 32) | depth bits.
 // Otherwise Gaussians in depth order, key = tile. Keys past the budget `cap`
 // are dropped (counted by k_count_top).
+
+// Cover-test constants of Gaussian i's raster record (gs_project.hip): the
+// conic and mean from its first float4, C and the cover level L from its
+// second (L in the high half of w as an fp16 upper bound when the record
+// holds 16-bit colours).
+__device__ __forceinline__ gs_cover_splat record_cover(const float4* __restrict__ rec, int recFmt, size_t i) {
+  const size_t rs = (size_t)rec_stride(recFmt);
+  const float4 g0 = rec[rs * i + 0];
+  const float4 g1 = rec[rs * i + 1];
+  const float L = recFmt == 0 ? g1.z : gs_h2f(__float_as_uint(g1.w) >> 16);
+  return gs_cover_prepare(g0.x, g0.y, g0.z, g0.w, g1.x, L);
+}
+
 template <bool kGlobal64>
 __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ order,
                                                  const uint32_t* __restrict__ cntS,
@@ -384,7 +397,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ or
                                                  const uint32_t* __restrict__ dkS,
                                                  const uint32_t* __restrict__ dN,
                                                  const uint64_t* __restrict__ partials, int tilesX,
-                                                 uint64_t cap, const float4* __restrict__ rec,
+                                                 uint64_t cap, const float4* __restrict__ rec, int recFmt,
                                                  void* __restrict__ keysOut,
                                                  uint32_t* __restrict__ valsOut) {
   __shared__ uint32_t wsum[4];
@@ -405,11 +418,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ or
   const uint32_t gidr = ok ? (kGlobal64 ? r : order[r]) : 0u;
   sId[tid] = gidr;
   if (kGlobal64) sDk[tid] = ok ? dkS[r] : 0u;
-  if (c != 0u) {
-    const float4 g0 = rec[4 * (size_t)gidr + 0];
-    const float4 g1 = rec[4 * (size_t)gidr + 1];
-    sG[tid] = gs_cover_prepare(g0.x, g0.y, g0.z, g0.w, g1.x, g1.z);
-  }
+  if (c != 0u) sG[tid] = record_cover(rec, recFmt, gidr);
   __syncthreads();
   const uint64_t run = partials[blockIdx.x];
   for (uint32_t q = tid; q < tot; q += kBlock) {
@@ -927,6 +936,7 @@ __global__ __launch_bounds__(kScanThreads) void k_seg_scan(const uint32_t* __res
 struct ScatterPrep {
   const uint32_t* __restrict__ depthKey;
   const float4* __restrict__ rec;
+  int recFmt;
   uint32_t* sDk;
   gs_cover_splat* sG;
   struct State {
@@ -936,15 +946,19 @@ struct ScatterPrep {
   __device__ State load(int64_t i, uint32_t c) const {
     State s{0u, make_float4(0.f, 0.f, 0.f, 0.f), make_float4(0.f, 0.f, 0.f, 0.f)};
     if (c != 0u) {
+      const int64_t rs = rec_stride(recFmt);
       s.dk = depthKey[i];
-      s.g0 = rec[4 * i + 0];
-      s.g1 = rec[4 * i + 1];
+      s.g0 = rec[rs * i + 0];
+      s.g1 = rec[rs * i + 1];
     }
     return s;
   }
   __device__ void store(int k, uint32_t c, const State& s) const {
     sDk[k] = s.dk;
-    if (c != 0u) sG[k] = gs_cover_prepare(s.g0.x, s.g0.y, s.g0.z, s.g0.w, s.g1.x, s.g1.z);
+    if (c != 0u) {
+      const float L = recFmt == 0 ? s.g1.z : gs_h2f(__float_as_uint(s.g1.w) >> 16);
+      sG[k] = gs_cover_prepare(s.g0.x, s.g0.y, s.g0.z, s.g0.w, s.g1.x, L);
+    }
   }
 };
 
@@ -956,7 +970,8 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint32_t* __r
                                                              const uint32_t* __restrict__ cnt,
                                                              const uint2* __restrict__ rect,
                                                              const uint32_t* __restrict__ depthKey,
-                                                             const float4* __restrict__ rec, int tilesX, int numTiles,
+                                                             const float4* __restrict__ rec, int recFmt,
+                                                             int tilesX, int numTiles,
                                                              const uint32_t* __restrict__ table,
                                                              const uint2* __restrict__ ranges,
                                                              uint64_t* __restrict__ keys) {
@@ -972,7 +987,7 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint32_t* __r
   const uint32_t* row = table + (int64_t)blockIdx.x * numTiles;
   const int64_t lo = blockStart[blockIdx.x], hi = blockStart[blockIdx.x + 1];   // the count pass's blocks
   const SegRound sr{sOff, sRect, wsum, owner};
-  seg_rounds(lo, hi, cnt, rect, tilesX, sr, ScatterPrep{depthKey, rec, sDk, sG},
+  seg_rounds(lo, hi, cnt, rect, tilesX, sr, ScatterPrep{depthKey, rec, recFmt, sDk, sG},
              [&]() {
 #pragma unroll 4
                for (int t = tid; t < numTiles; t += kSegThreads) {
@@ -1247,7 +1262,7 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
         s.tileCnt, a.numTiles, a.cap, s.visPart, nb, a.ranges, a.tileOrder, a.numKeys, a.numVisible, a.totalKeys,
         a.stats, s.chunkKeys, (a.n + kSegChunk - 1) / kSegChunk);
     k_seg_scatter<<<nb, kSegThreads, (size_t)a.numTiles * 5, st>>>(s.blockStart, a.cnt, a.rect, a.depthKey, a.rec,
-                                                                   a.tilesX, a.numTiles, s.segTable, a.ranges,
+                                                                   a.recFmt, a.tilesX, a.numTiles, s.segTable, a.ranges,
                                                                    (uint64_t*)a.tkA);
     k_seg_sort<<<a.numTiles, kBlock, 0, st>>>(a.ranges, a.tileOrder, (uint64_t*)a.tkA, (uint64_t*)a.tkB, a.tvA);
     out.keys = nullptr;
@@ -1267,7 +1282,7 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
                                             s.partials, s.vpartials);
     k_count_top<<<1, 1024, 0, st>>>(s.partials, s.vpartials, Gc, a.cap, a.numKeys, a.totalKeys, a.numVisible, a.stats);
     k_emit<false><<<Gc, kBlock, 0, st>>>(order, s.cntS, s.rectS, nullptr, s.nDepth, s.partials, a.tilesX, a.cap,
-                                         a.rec, a.tkA, a.tvA);
+                                         a.rec, a.recFmt, a.tkA, a.tvA);
     // 3) stable sort by tile id
     const uint32_t* tk;
     const uint32_t* tv;
@@ -1284,7 +1299,7 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
                                             s.partials, s.vpartials);
     k_count_top<<<1, 1024, 0, st>>>(s.partials, s.vpartials, Gc, a.cap, a.numKeys, a.totalKeys, a.numVisible, a.stats);
     k_emit<true><<<Gc, kBlock, 0, st>>>(nullptr, s.cntS, s.rectS, s.dkS, a.dN, s.partials, a.tilesX, a.cap,
-                                        a.rec, a.tkA, a.tvA);
+                                        a.rec, a.recFmt, a.tkA, a.tvA);
     const uint64_t* tk;
     const uint32_t* tv;
     radix_sort<uint64_t>(st, s, (const uint64_t*)a.tkA, a.tvA, (uint64_t*)a.tkB, a.tvB,

@@ -25,6 +25,7 @@ namespace {
 constexpr int kSortBlocks = 512;    // radix grid (measured best of 256/512/1024/2048 at 1M Gaussians, 3M keys)
 constexpr int64_t kDefaultEntries = 12000000;   // shaders.js:34 ENTRIES_CAP
 constexpr int64_t kTargetLimitBytes = int64_t(64) << 30;
+constexpr int kRing = 64;           // pinned ViewBlocks: the host may run this many batched steps ahead
 
 struct CamMeta {
   gs_camera c;      // f32 view of the camera, as the kernels take it (trainer.js:357-372)
@@ -81,7 +82,10 @@ struct gs_trainer {
 
   int64_t n = 0, cap = 0;
   int shDeg = 0, shK = 0, shC = 0, shP4 = 0, shQ8 = 0;
-  bool mixed = false;   // GS_PREC_MIXED: bf16 SH and moments
+  int prec = 0;         // gs_opts.precision (storage formats: Prec in gs_device.h)
+  int shFmt = 0;        // SH coefficients: 0 f32 planes, 1 bf16, 2 fp16 (16-bit planes)
+  bool mom16 = false;   // bf16 Adam moments (GS_PREC_MIXED)
+  int recFmt = 0;       // raster record colours: 0 f32 (64 B records), 1 bf16, 2 fp16 (48 B)
   int64_t entriesCap = 0;
   int maxTiles = 0;
   int64_t maxPix = 0;
@@ -120,6 +124,7 @@ struct gs_trainer {
   int64_t camStep = 0;
   double logfScale = 0.0;
   int64_t iter = 0;
+  int64_t adamT = 0;      // Adam steps (bias correction); equals iter unless batched steps ran
   int64_t pixelsSeen = 0;
   float sceneRadius = 1.0f;
   double basePosLr = 0.0;
@@ -131,6 +136,26 @@ struct gs_trainer {
   int lastNumTiles = 0;   // tiles of the last raster (refine's maxTile)
   bool lastSegPath = false;   // the last raster binned by tile segments (raw tile counts in sort.tileCnt)
   BinOut lastBin{};
+
+  // batched views (gs_step_views): buffer slot k holds view k's records,
+  // counts, gradient sums, sorted ids, tile ranges and order, and pixel
+  // buffers; slot 0 is the single-view set above
+  struct ViewSlot {
+    float4* rec; uint32_t* cnt; void* accum; uint32_t* ids; uint2* ranges; uint32_t* tileOrder;
+    float4* img; float* T; uint32_t* end; float4* gC;
+  };
+  std::vector<ViewSlot> vslots;
+  ViewBlock* dVB = nullptr;        // device copy of the step's ViewBlock
+  ViewBlock* hVB = nullptr;        // pinned ring of kRing host ViewBlocks
+  std::vector<hipEvent_t> ringEv;  // copy-done event of each ring entry
+  int ringPos = 0;
+  uint32_t* dQueue = nullptr;      // work counters of the two persistent launches
+  uint32_t* viewStats = nullptr;   // kMaxViews key counts, then kMaxViews visible counts
+  int lastViews = 0;               // views of the last raster when it was a batched step (else 0)
+  int viewBlocks = 0;              // blocks of the persistent launches
+  hipGraphExec_t gExec = nullptr;  // captured batched step and the shape it was captured for
+  std::vector<int64_t> gKey;
+  std::vector<int64_t> lastEagerKey;   // shape of the last eager batched step
 
   bool prof = false;
   std::vector<ProfSlot> slots;
@@ -154,7 +179,7 @@ struct gs_pending {
   float* dstSH = nullptr;
   int64_t n = 0, cap = 0;
   int shC = 0, shP4 = 0, shQ8 = 0;
-  bool mixed = false;
+  int shFmt = 0;
   std::string err;
 };
 
@@ -189,8 +214,24 @@ int dalloc(gs_trainer* h, T** p, size_t bytes) {
 }
 
 void free_all(gs_trainer* h) {
+  if (h->gExec) {
+    (void)hipGraphExecDestroy(h->gExec);
+    h->gExec = nullptr;
+  }
+  h->gKey.clear();
+  h->lastEagerKey.clear();
+  for (hipEvent_t e : h->ringEv) {
+    (void)hipEventSynchronize(e);
+    (void)hipEventDestroy(e);
+  }
+  h->ringEv.clear();
+  if (h->hVB) {
+    (void)hipHostFree(h->hVB);
+    h->hVB = nullptr;
+  }
   for (void* p : h->allocs) (void)hipFree(p);
   h->allocs.clear();
+  h->vslots.clear();
   h->ready = false;
 }
 
@@ -349,9 +390,9 @@ AdamHP make_hp(const gs_trainer* h) {
   a.lr[13] = 2.5e-2f;
   a.lr[14] = a.lr[15] = 0.0f;
   a.b1 = 0.9f; a.b2 = 0.999f; a.eps = 1e-15f;
-  a.rbc1 = (float)(1.0 / (1.0 - std::pow((double)0.9f, (double)h->iter)));
-  a.rbc2 = (float)(1.0 / (1.0 - std::pow((double)0.999f, (double)h->iter)));
-  a.step = (uint32_t)h->iter;
+  a.rbc1 = (float)(1.0 / (1.0 - std::pow((double)0.9f, (double)h->adamT)));
+  a.rbc2 = (float)(1.0 / (1.0 - std::pow((double)0.999f, (double)h->adamT)));
+  a.step = (uint32_t)h->adamT;
   const double r = h->sceneRadius;
   a.minLS = (float)std::log(r * (double)h->o.minScale);
   a.maxLS = (float)std::log(r * 0.05);
@@ -377,6 +418,7 @@ Cut make_cut(const gs_trainer* h) {
 // ---- one raster: project -> bin/sort -> forward (+ loss) ----
 int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
   const int64_t n = h->n;
+  h->lastViews = 0;
   h->profLast = nullptr;   // a step or render starts its own first stage
   {
     Scope s(h, "project", (double)n * (56.0 + 4.0 * h->shC + 48.0 + 16.0));
@@ -384,7 +426,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
     // Gaussians, which the projection sums on the way
     const bool segPath = h->o.sortMode == GS_SORT_TILE_SEGMENTS && cam.numTiles <= kMaxSegTiles;
     h->lastSegPath = segPath;
-    launch_project(h->st, h->shDeg, h->mixed, n, h->cap, cam, make_cut(h), h->P, h->SH, h->rec, h->depthKey,
+    launch_project(h->st, h->shDeg, h->prec, n, h->cap, cam, make_cut(h), h->P, h->SH, h->rec, h->depthKey,
                    h->rect, h->cnt, segPath ? h->sort.chunkKeys : nullptr);
   }
   {
@@ -400,6 +442,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
     a.rect = h->rect;
     a.depthKey = h->depthKey;
     a.rec = h->rec;
+    a.recFmt = h->recFmt;
     a.dkA = h->dkA; a.dvA = h->dvA; a.dkB = h->dkB; a.dvB = h->dvB;
     a.tkA = h->tkA; a.tkB = h->tkB; a.tvA = h->tvA; a.tvB = h->tvB;
     a.numKeys = h->numKeys;
@@ -417,6 +460,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
   b.tileOrder = h->lastBin.tileOrder;
   b.ids = h->lastBin.ids;
   b.rec = h->rec;
+  b.recFmt = h->recFmt;
   b.img = h->img;
   b.T = h->Tfin;
   b.end = h->endIdx;
@@ -517,6 +561,7 @@ int eval_pass(gs_trainer* h, int ci, const double* R, const double* t, double g,
   rb.tileOrder = h->lastBin.tileOrder;
   rb.ids = h->lastBin.ids;
   rb.rec = h->rec;
+  rb.recFmt = h->recFmt;
   rb.T = h->Tfin;
   rb.end = h->endIdx;
   rb.gC = h->gC;
@@ -563,6 +608,7 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
   meta.activeDeg = (int)std::min<int64_t>(h->shDeg, h->iter / 1000);
   const Cam cam = make_cam_meta(h, meta, meta.activeDeg);
   h->iter++;
+  h->adamT++;
   h->pixelsSeen += (int64_t)meta.c.w * meta.c.h;
   const AdamHP hp = make_hp(h);
 
@@ -589,7 +635,26 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
     HIPCK(hipStreamSynchronize(h->st));
     uint32_t nk = 0;
     HIPCK(hipMemcpy(&nk, h->numKeys, 4, hipMemcpyDeviceToHost));
-    if (dbg->rec) HIPCK(hipMemcpy(dbg->rec, h->rec, 64 * h->n, hipMemcpyDeviceToHost));
+    if (dbg->rec && h->recFmt == 0) {
+      HIPCK(hipMemcpy(dbg->rec, h->rec, 64 * h->n, hipMemcpyDeviceToHost));
+    } else if (dbg->rec) {
+      // 48 B records with 16-bit colours, written out in the 16-float layout:
+      // (mx, my, A, B, C, opa*comp, L, 0, r, g, b, opa, comp, -0.5 cnorm, 1 - opa, 0)
+      // with the colours and L decoded
+      std::vector<float> r((size_t)12 * h->n);
+      HIPCK(hipMemcpy(r.data(), h->rec, 48 * h->n, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < h->n; i++) {
+        const float* q = &r[12 * (size_t)i];
+        float* o = dbg->rec + 16 * i;
+        uint32_t rg, bl;
+        memcpy(&rg, &q[6], 4);
+        memcpy(&bl, &q[7], 4);
+        auto dec = [&](uint32_t b) { return h->recFmt == 2 ? gs_h2f(b & 0xffffu) : gs_bf2f(b); };
+        const float v[16] = {q[0], q[1], q[2], q[3], q[4], q[5], gs_h2f(bl >> 16), 0.0f,
+                             dec(rg), dec(rg >> 16), dec(bl), q[11], q[8], q[9], q[10], 0.0f};
+        memcpy(o, v, sizeof v);
+      }
+    }
     std::vector<uint32_t> dk(h->n);
     HIPCK(hipMemcpy(dk.data(), h->depthKey, 4 * h->n, hipMemcpyDeviceToHost));
     if (dbg->depthKey) memcpy(dbg->depthKey, dk.data(), 4 * h->n);
@@ -635,6 +700,7 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
   b.tileOrder = h->lastBin.tileOrder;
   b.ids = h->lastBin.ids;
   b.rec = h->rec;
+  b.recFmt = h->recFmt;
   b.T = h->Tfin;
   b.end = h->endIdx;
   b.gC = h->gC;
@@ -675,7 +741,7 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
   u.dbgShGrad = dSG;
   {
     Scope s(h, "update", (double)h->n * (100.0 + 6 * 64.0 + 6 * 4.0 * h->shC));
-    launch_update(h->st, h->shDeg, h->o.gradAccum, h->mixed, cam, hp, u);
+    launch_update(h->st, h->shDeg, h->o.gradAccum, h->prec, cam, hp, u);
   }
   HIPCK(hipGetLastError());
   if (dbg) {
@@ -691,6 +757,216 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
   // trainer.js:468-470: one camera Adam step every 25 iterations after 1500
   if (h->o.camOpt && h->iter > 1500 && h->iter % 25 == 0) return apply_cam_grads(h);
   return GS_OK;
+}
+
+// ---- batched views (gs_step_views) ----
+
+// The launches of one batched step. Every per-step value (cameras, Adam
+// constants, the work offsets) comes from the device ViewBlock, so the same
+// launches serve eager steps and a captured graph; what the host fixes here
+// is the shape: n, each view's tile grid and the colour-clamp choice.
+// dbg (eager only): copy the views' intermediates between the stages.
+int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* dGF, float* dSG) {
+  const int V = vb.count;
+  const int64_t n = h->n;
+  const Cut cut = make_cut(h);
+  launch_fill_u32(h->st, h->dQueue, 2, 0u);
+  bool bounded = true;
+  for (int k = 0; k < V; k++) {
+    const gs_trainer::ViewSlot& sl = h->vslots[k];
+    const Cam& cam = vb.cams[k];
+    bounded = bounded && charb_grad_bounded(cam);
+    {
+      Scope sc(h, "project", (double)n * (56.0 + 4.0 * h->shC + 48.0 + 16.0));
+      launch_project_view(h->st, h->shDeg, h->prec, n, h->cap, h->dVB, k, cut, h->P, h->SH, sl.rec, h->depthKey,
+                          h->rect, sl.cnt, h->sort.chunkKeys);
+    }
+    BinArgs a{};
+    a.mode = GS_SORT_TILE_SEGMENTS;
+    a.tilesX = cam.tilesX;
+    a.numTiles = cam.numTiles;
+    a.n = n;
+    a.cap = (uint64_t)h->entriesCap;
+    a.dN = h->dN;
+    a.cnt = sl.cnt;
+    a.rect = h->rect;
+    a.depthKey = h->depthKey;
+    a.rec = sl.rec;
+    a.recFmt = h->recFmt;
+    a.dkA = h->dkA; a.dvA = h->dvA; a.dkB = h->dkB; a.dvB = h->dvB;
+    a.tkA = h->tkA; a.tkB = h->tkB; a.tvA = sl.ids; a.tvB = h->tvB;
+    a.numKeys = h->viewStats + k;
+    a.numVisible = h->viewStats + kMaxViews + k;
+    a.totalKeys = h->totalKeys;
+    a.ranges = sl.ranges;
+    a.tileOrder = sl.tileOrder;
+    a.stats = h->stats;
+    a.scratch = h->sort;
+    {
+      Scope sc(h, "bin_sort", 0.0);
+      BinOut out{};
+      launch_bin_sort(h->st, a, out);
+    }
+    if (dbg) {
+      HIPCK(hipStreamSynchronize(h->st));
+      uint32_t nk = 0;
+      HIPCK(hipMemcpy(&nk, h->viewStats + k, 4, hipMemcpyDeviceToHost));
+      if (dbg[k].numKeys) *dbg[k].numKeys = nk;
+      if (dbg[k].tileRanges)
+        HIPCK(hipMemcpy(dbg[k].tileRanges, sl.ranges, 8ull * cam.numTiles, hipMemcpyDeviceToHost));
+    }
+  }
+  h->lastNumTiles = vb.cams[V - 1].numTiles;
+  h->lastSegPath = true;
+  h->lastViews = V;
+  {
+    Scope sc(h, "raster_fwd_views", 0.0);
+    launch_raster_fwd_views(h->st, h->dVB, cut, 1, h->dQueue, h->viewBlocks);
+  }
+  if (dbg) {
+    HIPCK(hipStreamSynchronize(h->st));
+    for (int k = 0; k < V; k++) {
+      const gs_trainer::ViewSlot& sl = h->vslots[k];
+      const int64_t npx = (int64_t)vb.cams[k].W * vb.cams[k].H;
+      if (dbg[k].img) HIPCK(hipMemcpy(dbg[k].img, sl.img, 16 * npx, hipMemcpyDeviceToHost));
+      if (dbg[k].T) HIPCK(hipMemcpy(dbg[k].T, sl.T, 4 * npx, hipMemcpyDeviceToHost));
+      if (dbg[k].end) HIPCK(hipMemcpy(dbg[k].end, sl.end, 4 * npx, hipMemcpyDeviceToHost));
+      if (dbg[k].gC) HIPCK(hipMemcpy(dbg[k].gC, sl.gC, 16 * npx, hipMemcpyDeviceToHost));
+    }
+  }
+  {
+    Scope sc(h, "raster_bwd_views", 0.0);
+    launch_raster_bwd_views(h->st, h->dVB, cut, h->o.gradAccum, bounded, h->dQueue + 1, h->viewBlocks);
+  }
+  if (dbg) {
+    HIPCK(hipStreamSynchronize(h->st));
+    for (int k = 0; k < V; k++) {
+      if (!dbg[k].accum) continue;
+      std::vector<int32_t> acc(12 * h->n);
+      HIPCK(hipMemcpy(acc.data(), h->vslots[k].accum, 48 * h->n, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < h->n; i++)
+        for (int q = 0; q < 10; q++) dbg[k].accum[10 * i + q] = acc[12 * i + q];
+    }
+  }
+  UpdateBufs u{};
+  u.n = n;
+  u.cap = h->cap;
+  u.P = h->P; u.M = h->M; u.V = h->V;
+  u.SH = h->SH; u.SHM = h->SHM; u.SHV = h->SHV;
+  u.dbgGradF = dGF;
+  u.dbgShGrad = dSG;
+  {
+    Scope sc(h, "update_views", (double)n * (100.0 + 6 * 64.0 + 6 * 4.0 * h->shC));
+    launch_update_views(h->st, h->shDeg, h->o.gradAccum, h->prec, h->dVB, u);
+  }
+  HIPCK(hipGetLastError());
+  return GS_OK;
+}
+
+int step_views_impl(gs_trainer* h, const int32_t* cams, int count, gs_debug_out* dbg) {
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  if (count < 1 || count > (int)h->vslots.size())
+    return fail(h, GS_E_ARG, "stepViews: " + std::to_string(count) + " views; opts.viewsPerStep allows 1.." +
+                                 std::to_string(h->vslots.size()));
+  if (h->vslots.size() < 2 || !h->dVB)
+    return fail(h, GS_E_ARG, "stepViews needs opts.viewsPerStep > 1 at create");
+  if (h->o.loss != GS_LOSS_CHARBONNIER || h->o.sortMode != GS_SORT_TILE_SEGMENTS || h->o.camOpt)
+    return fail(h, GS_E_ARG, "stepViews needs the Charbonnier loss, the tile-segment sort and camOpt off");
+  // the view list, picked as `count` stepOnce calls would pick it
+  int ci[kMaxViews];
+  for (int k = 0; k < count; k++) {
+    ci[k] = cams && cams[k] >= 0 ? cams[k] : pick_camera(h);
+    if (ci[k] >= (int)h->cams.size()) return fail(h, GS_E_ARG, "stepViews: camera index out of range");
+  }
+  // the ring entry this step fills: its previous copy must have landed
+  ViewBlock& vb = h->hVB[h->ringPos];
+  hipEvent_t ev = h->ringEv[(size_t)h->ringPos];
+  HIPCK(hipEventSynchronize(ev));
+  memset(&vb, 0, sizeof vb);
+  vb.count = count;
+  const int activeDeg = (int)std::min<int64_t>(h->shDeg, h->iter / 1000);   // trainer.js:409-412
+  uint32_t off = 0;
+  std::vector<int64_t> key = {count, h->n, 0};
+  for (int k = 0; k < count; k++) {
+    CamMeta& meta = h->cams[ci[k]];
+    meta.activeDeg = activeDeg;
+    vb.cams[k] = make_cam_meta(h, meta, activeDeg);
+    if (vb.cams[k].numTiles > kMaxSegTiles)
+      return fail(h, GS_E_LIMIT, "stepViews: a view of more than 16384 tiles (the tile-segment sort's limit)");
+    const gs_trainer::ViewSlot& sl = h->vslots[k];
+    RasterBufs& b = vb.bufs[k];
+    b.recFmt = h->recFmt;
+    b.ranges = sl.ranges;
+    b.tileOrder = sl.tileOrder;
+    b.ids = sl.ids;
+    b.rec = sl.rec;
+    b.img = sl.img;
+    b.T = sl.T;
+    b.end = sl.end;
+    b.gC = sl.gC;
+    b.tgt = h->targets;
+    b.stats = h->stats;
+    vb.accum[k] = sl.accum;
+    vb.cnt[k] = sl.cnt;
+    vb.off[k] = off;
+    off += (uint32_t)vb.cams[k].numTiles;
+    key.push_back(vb.cams[k].W);
+    key.push_back(vb.cams[k].H);
+    if (!charb_grad_bounded(vb.cams[k])) key[2] = 1;
+  }
+  vb.off[count] = off;
+  h->iter += count;
+  h->adamT += 1;
+  for (int k = 0; k < count; k++) h->pixelsSeen += (int64_t)vb.cams[k].W * vb.cams[k].H;
+  h->lastCam = ci[count - 1];
+  vb.hp = make_hp(h);
+  HIPCK(hipMemcpyAsync(h->dVB, &vb, sizeof vb, hipMemcpyHostToDevice, h->st));
+  HIPCK(hipEventRecord(ev, h->st));
+  h->ringPos = (h->ringPos + 1) % kRing;
+  h->profLast = nullptr;
+
+  float* dGF = nullptr;
+  float* dSG = nullptr;
+  if (dbg && (dbg[0].gradF || dbg[0].shGrad)) {
+    HIPCK(hipMalloc(&dGF, 64 * h->n));
+    HIPCK(hipMalloc(&dSG, 4 * (h->shC ? h->shC : 1) * h->n));
+  }
+  int rc = GS_OK;
+  // A shape is captured on its second step in a row: the first runs eagerly
+  // (one-time host set-up of the launches happens outside any capture).
+  const bool useGraph = h->o.graph && !dbg && !h->prof;
+  if (useGraph && h->gExec && key == h->gKey) {
+    HIPCK(hipGraphLaunch(h->gExec, h->st));
+  } else if (useGraph && key == h->lastEagerKey) {
+    if (h->gExec) {
+      (void)hipGraphExecDestroy(h->gExec);
+      h->gExec = nullptr;
+    }
+    HIPCK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
+    rc = enqueue_views(h, vb, nullptr, nullptr, nullptr);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(h->st, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) return hipfail(h, "stepViews: graph capture", e);
+    const hipError_t ei = hipGraphInstantiate(&h->gExec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) { h->gExec = nullptr; return hipfail(h, "stepViews: graph instantiate", ei); }
+    h->gKey = key;
+    HIPCK(hipGraphLaunch(h->gExec, h->st));
+  } else {
+    rc = enqueue_views(h, vb, dbg, dGF, dSG);
+    h->lastEagerKey = key;
+  }
+  if (dbg && rc == GS_OK) {
+    HIPCK(hipStreamSynchronize(h->st));
+    if (dGF && dbg[0].gradF) HIPCK(hipMemcpy(dbg[0].gradF, dGF, 64 * h->n, hipMemcpyDeviceToHost));
+    if (dSG && dbg[0].shGrad && h->shC)
+      HIPCK(hipMemcpy(dbg[0].shGrad, dSG, 4ull * h->shC * h->n, hipMemcpyDeviceToHost));
+    if (dbg[0].stats) HIPCK(hipMemcpy(dbg[0].stats, h->stats, 32, hipMemcpyDeviceToHost));
+  }
+  if (dGF) (void)hipFree(dGF);
+  if (dSG) (void)hipFree(dSG);
+  return rc;
 }
 
 void repack_to_planes(const float* src, int64_t n, int width, int planes, int64_t cap,
@@ -725,6 +1001,8 @@ int gs_default_opts(gs_opts* o) {
   o->capMult = 4.0f;
   o->entriesCap = 0;
   o->tileGrad = 1;
+  o->viewsPerStep = 1;
+  o->graph = 1;
   o->posLr = 0.0f;
   o->loss = GS_LOSS_CHARBONNIER;
   o->ssimLambda = 0.2f;
@@ -750,7 +1028,8 @@ int gs_create(const gs_opts* o, gs_trainer_t** out) {
   }
   if (h->o.shDeg < 0 || h->o.shDeg > 3 || h->o.sortMode < 0 || h->o.sortMode > 2 ||
       h->o.gradAccum < 0 || h->o.gradAccum > 1 || h->o.loss < 0 || h->o.loss > 1 ||
-      h->o.precision < 0 || h->o.precision > 1 ||
+      h->o.precision < 0 || h->o.precision > 3 ||
+      h->o.viewsPerStep < 1 || h->o.viewsPerStep > kMaxViews ||
       !(h->o.eCut >= 0.0f && h->o.eCut <= 80.0f)) {
     delete h;
     return GS_E_ARG;
@@ -776,7 +1055,10 @@ int gs_create(const gs_opts* o, gs_trainer_t** out) {
   h->shC = 3 * h->shK;
   h->shP4 = (h->shC + 3) / 4;
   h->shQ8 = (h->shC + 7) / 8;
-  h->mixed = h->o.precision == GS_PREC_MIXED;
+  h->prec = h->o.precision;
+  h->shFmt = h->prec == GS_PREC_F32 ? 0 : (h->prec == GS_PREC_FP16 ? 2 : 1);
+  h->mom16 = h->prec == GS_PREC_MIXED;
+  h->recFmt = h->prec == GS_PREC_BF16 ? 1 : (h->prec == GS_PREC_FP16 ? 2 : 0);
   h->rng = h->o.seed;
   *out = h;
   return GS_OK;
@@ -865,14 +1147,16 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   const size_t pb = (size_t)cap * 64;
   int rc = 0;
   rc |= dalloc(h, &h->P, pb);
-  const size_t mb = h->mixed ? (size_t)cap * 32 : pb;   // 16 moments: 2 bf16 planes or 4 f32 planes
+  const size_t mb = h->mom16 ? (size_t)cap * 32 : pb;   // 16 moments: 2 bf16 planes or 4 f32 planes
   rc |= dalloc(h, &h->M, mb);
   rc |= dalloc(h, &h->V, mb);
-  const size_t sb = (size_t)cap * 16 * std::max(h->mixed ? h->shQ8 : h->shP4, 1);
+  // SH coefficients in f32 or 16-bit planes; SH moments in 16-bit planes only with mom16
+  const size_t sb = (size_t)cap * 16 * std::max(h->shFmt ? h->shQ8 : h->shP4, 1);
+  const size_t smb = (size_t)cap * 16 * std::max(h->mom16 ? h->shQ8 : h->shP4, 1);
   rc |= dalloc(h, &h->SH, sb);
-  rc |= dalloc(h, &h->SHM, sb);
-  rc |= dalloc(h, &h->SHV, sb);
-  rc |= dalloc(h, &h->rec, (size_t)cap * 64);
+  rc |= dalloc(h, &h->SHM, smb);
+  rc |= dalloc(h, &h->SHV, smb);
+  rc |= dalloc(h, &h->rec, (size_t)cap * 16 * rec_stride(h->recFmt));
   rc |= dalloc(h, &h->depthKey, (size_t)cap * 4);
   rc |= dalloc(h, &h->cnt, (size_t)cap * 4);
   rc |= dalloc(h, &h->rect, (size_t)cap * 8);
@@ -923,32 +1207,71 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   rc |= dalloc(h, &h->stats, 32);
   rc |= dalloc(h, &h->evalStats, 32);
   rc |= dalloc(h, &h->camGrad, (size_t)(ncams + 1) * 32);
+  // batched views: slot 0 is the single-view set, slots 1.. their own copies
+  h->vslots.assign((size_t)h->o.viewsPerStep, gs_trainer::ViewSlot{});
+  h->vslots[0] = {h->rec, h->cnt, h->accum, h->tvA, h->ranges, h->tileOrder, h->img, h->Tfin, h->endIdx, h->gC};
+  for (size_t k = 1; k < h->vslots.size(); k++) {
+    gs_trainer::ViewSlot& v = h->vslots[k];
+    rc |= dalloc(h, &v.rec, (size_t)cap * 16 * rec_stride(h->recFmt));
+    rc |= dalloc(h, &v.cnt, (size_t)cap * 4);
+    rc |= dalloc(h, &v.accum, (size_t)cap * 48);
+    rc |= dalloc(h, &v.ids, (size_t)h->entriesCap * 4);
+    rc |= dalloc(h, &v.ranges, (size_t)h->maxTiles * 8);
+    rc |= dalloc(h, &v.tileOrder, (size_t)h->maxTiles * 4);
+    rc |= dalloc(h, &v.img, (size_t)h->maxPix * 16);
+    rc |= dalloc(h, &v.T, (size_t)h->maxPix * 4);
+    rc |= dalloc(h, &v.end, (size_t)h->maxPix * 4);
+    rc |= dalloc(h, &v.gC, (size_t)h->maxPix * 16);
+  }
+  if (h->o.viewsPerStep > 1) {
+    rc |= dalloc(h, &h->dVB, sizeof(ViewBlock));
+    rc |= dalloc(h, &h->dQueue, 8);
+    rc |= dalloc(h, &h->viewStats, 8 * kMaxViews);
+  }
   if (rc) { free_all(h); return rc < 0 ? rc : GS_E_NOMEM; }
+  if (h->o.viewsPerStep > 1) {
+    hipError_t e = hipHostMalloc((void**)&h->hVB, sizeof(ViewBlock) * kRing, hipHostMallocDefault);
+    for (int k = 0; k < kRing && e == hipSuccess; k++) {
+      hipEvent_t ev = nullptr;
+      e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e == hipSuccess) h->ringEv.push_back(ev);
+    }
+    if (e != hipSuccess) {
+      free_all(h);
+      return hipfail(h, "setup: batched-view staging", e);
+    }
+    h->ringPos = 0;
+    h->viewBlocks = raster_views_blocks(h->o.device);
+  }
 
   std::vector<float> planes;
   repack_to_planes(params, n, 16, 4, cap, planes);
   HIPCK(hipMemcpy(h->P, planes.data(), pb, hipMemcpyHostToDevice));
-  if (h->shC && !h->mixed) {
+  if (h->shC && !h->shFmt) {
     if (sh) repack_to_planes(sh, n, h->shC, h->shP4, cap, planes);
     else planes.assign((size_t)h->shP4 * cap * 4, 0.0f);
     HIPCK(hipMemcpy(h->SH, planes.data(), sb, hipMemcpyHostToDevice));
   } else if (h->shC) {
-    // bf16 planes: coefficient j in plane j/8, half j%8; f32 rounded to nearest even
+    // 16-bit planes: coefficient j in plane j/8, half j%8; f32 rounded to nearest even
     std::vector<uint16_t> b16((size_t)h->shQ8 * cap * 8, 0);
     if (sh)
       for (int64_t i = 0; i < n; i++)
-        for (int j = 0; j < h->shC; j++)
-          b16[((size_t)(j / 8) * cap + i) * 8 + (j % 8)] = (uint16_t)gs_f2bf_rne(sh[i * h->shC + j]);
+        for (int j = 0; j < h->shC; j++) {
+          const float x = sh[i * h->shC + j];
+          b16[((size_t)(j / 8) * cap + i) * 8 + (j % 8)] =
+              (uint16_t)(h->shFmt == 2 ? gs_f2h_rne(x) : gs_f2bf_rne(x));
+        }
     HIPCK(hipMemcpy(h->SH, b16.data(), sb, hipMemcpyHostToDevice));
   }
   HIPCK(hipMemset(h->M, 0, mb));
   HIPCK(hipMemset(h->V, 0, mb));
-  HIPCK(hipMemset(h->SHM, 0, sb));
-  HIPCK(hipMemset(h->SHV, 0, sb));
+  HIPCK(hipMemset(h->SHM, 0, smb));
+  HIPCK(hipMemset(h->SHV, 0, smb));
   HIPCK(hipMemset(h->accum, 0, (size_t)cap * 48));
   HIPCK(hipMemset(h->stats, 0, 32));
   HIPCK(hipMemset(h->evalStats, 0, 32));
   HIPCK(hipMemset(h->camGrad, 0, (size_t)(ncams + 1) * 32));
+  for (size_t k = 1; k < h->vslots.size(); k++) HIPCK(hipMemset(h->vslots[k].accum, 0, (size_t)cap * 48));
   const uint32_t n32 = (uint32_t)n;
   HIPCK(hipMemcpy(h->dN, &n32, 4, hipMemcpyHostToDevice));
   HIPCK(hipMemcpy(h->targets, tgt.data(), tgt.size() * 4, hipMemcpyHostToDevice));
@@ -963,6 +1286,7 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   h->camStep = 0;
   h->logfScale = 0.0;
   h->iter = 0;
+  h->adamT = 0;
   h->pixelsSeen = 0;
   h->ready = true;
   return GS_OK;
@@ -971,6 +1295,18 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
 int gs_step(gs_trainer_t* h, int32_t cam) {
   if (!h) return GS_E_ARG;
   return step_impl(h, cam, nullptr);
+}
+
+int gs_step_views(gs_trainer_t* h, const int32_t* cams, int32_t count) {
+  if (!h) return GS_E_ARG;
+  HIPCK(hipSetDevice(h->o.device));
+  return step_views_impl(h, cams, count, nullptr);
+}
+
+int gs_step_views_debug(gs_trainer_t* h, const int32_t* cams, int32_t count, gs_debug_out* outs) {
+  if (!h || !outs) return GS_E_ARG;
+  HIPCK(hipSetDevice(h->o.device));
+  return step_views_impl(h, cams, count, outs);
 }
 
 int gs_step_batch(gs_trainer_t* h, const int32_t* cams, int32_t count) {
@@ -1181,7 +1517,7 @@ int gs_read_gaussians_async(gs_trainer_t* h, float* params, float* sh, gs_pendin
   p->shC = h->shC;
   p->shP4 = h->shP4;
   p->shQ8 = h->shQ8;
-  p->mixed = h->mixed;
+  p->shFmt = h->shFmt;
   p->dstP = params;
   p->dstSH = h->shC ? sh : nullptr;
   hipError_t e = hipSuccess;
@@ -1191,7 +1527,7 @@ int gs_read_gaussians_async(gs_trainer_t* h, float* params, float* sh, gs_pendin
     if (e == hipSuccess) e = hipMemcpyAsync(p->hP, h->P, bytes, hipMemcpyDeviceToHost, h->st);
   }
   if (e == hipSuccess && p->dstSH) {
-    const size_t bytes = (size_t)h->cap * 16 * (h->mixed ? h->shQ8 : h->shP4);
+    const size_t bytes = (size_t)h->cap * 16 * (h->shFmt ? h->shQ8 : h->shP4);
     e = hipHostMalloc(&p->hSH, bytes, hipHostMallocDefault);
     if (e == hipSuccess) e = hipMemcpyAsync(p->hSH, h->SH, bytes, hipMemcpyDeviceToHost, h->st);
   }
@@ -1238,7 +1574,7 @@ int gs_pending_wait(gs_pending_t* p, gs_readback_result* out) {
         for (int64_t i = 0; i < n; i++)
           for (int j = 0; j < 16; j++) p->dstP[i * 16 + j] = pl[((size_t)(j / 4) * cap + i) * 4 + (j % 4)];
       }
-      if (p->dstSH && !p->mixed) {
+      if (p->dstSH && !p->shFmt) {
         const float* pl = (const float*)p->hSH;
         for (int64_t i = 0; i < n; i++)
           for (int j = 0; j < p->shC; j++) p->dstSH[i * p->shC + j] = pl[((size_t)(j / 4) * cap + i) * 4 + (j % 4)];
@@ -1246,7 +1582,8 @@ int gs_pending_wait(gs_pending_t* p, gs_readback_result* out) {
         const uint16_t* b16 = (const uint16_t*)p->hSH;
         for (int64_t i = 0; i < n; i++)
           for (int j = 0; j < p->shC; j++)
-            p->dstSH[i * p->shC + j] = gs_bf2f(b16[((size_t)(j / 8) * cap + i) * 8 + (j % 8)]);
+            p->dstSH[i * p->shC + j] = p->shFmt == 2 ? gs_h2f(b16[((size_t)(j / 8) * cap + i) * 8 + (j % 8)])
+                                                      : gs_bf2f(b16[((size_t)(j / 8) * cap + i) * 8 + (j % 8)]);
       }
       break;
     }
@@ -1286,6 +1623,7 @@ int gs_set_excluded(gs_trainer_t* h, const int32_t* c, int32_t count) {
 int gs_set_iter(gs_trainer_t* h, int64_t iter) {
   if (!h || iter < 0) return GS_E_ARG;
   h->iter = iter;
+  h->adamT = iter;   // the reference's Adam t is its iteration counter
   return GS_OK;
 }
 
@@ -1313,13 +1651,22 @@ int gs_read_step_stats(gs_trainer_t* h, gs_step_stats* out) {
   if (!h->ready) return GS_OK;
   HIPCK(hipSetDevice(h->o.device));
   uint32_t nk = 0, nv = 0;
+  uint32_t vs[2 * kMaxViews];
   double s[4];
   HIPCK(hipMemcpyAsync(&nk, h->numKeys, 4, hipMemcpyDeviceToHost, h->st));
   HIPCK(hipMemcpyAsync(&nv, h->numVisible, 4, hipMemcpyDeviceToHost, h->st));
+  if (h->lastViews > 0) HIPCK(hipMemcpyAsync(vs, h->viewStats, sizeof vs, hipMemcpyDeviceToHost, h->st));
   HIPCK(hipMemcpyAsync(s, h->stats, 32, hipMemcpyDeviceToHost, h->st));
   HIPCK(hipStreamSynchronize(h->st));
   out->lastKeys = nk;
   out->lastVisible = nv;
+  if (h->lastViews > 0) {   // a batched step: all its views
+    out->lastKeys = out->lastVisible = 0;
+    for (int k = 0; k < h->lastViews; k++) {
+      out->lastKeys += vs[k];
+      out->lastVisible += vs[kMaxViews + k];
+    }
+  }
   out->overflowKeys = (int64_t)s[3];
   return GS_OK;
 }
@@ -1444,7 +1791,7 @@ int gs_refine(gs_trainer_t* h, const gs_refine_opts* ro, gs_refine_result* out) 
     HIPCK(hipMalloc(&dops, ops.size() * sizeof(RefineOp)));
     e = hipMemcpy(dops, ops.data(), ops.size() * sizeof(RefineOp), hipMemcpyHostToDevice);
     if (e == hipSuccess) {
-      launch_refine_apply(h->st, h->shDeg, h->mixed, h->cap, dops, (int64_t)ops.size(), h->P, h->M, h->V, h->SH,
+      launch_refine_apply(h->st, h->shDeg, h->prec, h->cap, dops, (int64_t)ops.size(), h->P, h->M, h->V, h->SH,
                           h->SHM, h->SHV);
       e = hipGetLastError();
       if (e == hipSuccess) e = hipStreamSynchronize(h->st);
@@ -1569,7 +1916,7 @@ int gs_export_ply(gs_trainer_t* h, int32_t bake, void* buf, int64_t cap, int64_t
   hipError_t e = hipMalloc(&dOut, (size_t)std::max<int64_t>(body, 4));
   if (e == hipSuccess) e = hipMemcpy(dCam, cp.data(), cp.size() * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) {
-    launch_ply_rows(h->st, h->shDeg, h->mixed, 0, n, h->cap, h->P, h->SH, bake ? 1 : 0, h->cams[0].f, dCam,
+    launch_ply_rows(h->st, h->shDeg, h->prec, 0, n, h->cap, h->P, h->SH, bake ? 1 : 0, h->cams[0].f, dCam,
                     nc, dOut);
     e = hipGetLastError();
   }
@@ -1616,7 +1963,7 @@ int gs_export_ply_noalloc(gs_trainer_t* h, int32_t bake, void* buf, int64_t cap,
   const int64_t rowsPerChunk = kNoAllocRowFloats / W;
   for (int64_t first = 0; first < n; first += rowsPerChunk) {
     const int64_t count = std::min(rowsPerChunk, n - first);
-    launch_ply_rows(h->st, h->shDeg, h->mixed, first, count, h->cap, h->P, h->SH, bake ? 1 : 0, h->cams[0].f,
+    launch_ply_rows(h->st, h->shDeg, h->prec, first, count, h->cap, h->P, h->SH, bake ? 1 : 0, h->cams[0].f,
                     dCam, nc, dRows);
     HIPCK(hipGetLastError());
     HIPCK(hipMemcpyAsync((char*)buf + hdrLen + first * (int64_t)W * 4, dRows, (size_t)(count * W * 4),

@@ -13,58 +13,39 @@
 // Invisible Gaussians still take the Adam step with a zero gradient (plus the
 // opacity regulariser), like the reference, whose Adam runs over all n.
 //
-// Mixed precision (kMixed): the SH coefficients and all Adam moments are
-// stored in bf16 (gs_math.h: exact decode, stochastic rounding on store),
-// which takes the pass from ~1.06 KB to ~0.64 KB per Gaussian at SH degree 2;
-// parameters, gradients and all arithmetic stay f32.
+// Precision modes (Prec in gs_device.h): in mode 1 the SH coefficients and
+// all Adam moments are stored in bf16 (exact decode, stochastic rounding on
+// store), which takes the pass from ~1.06 KB to ~0.64 KB per Gaussian at SH
+// degree 2; in modes 2 and 3 only the SH coefficients are stored in 16 bits
+// (bf16 or fp16, stochastic rounding) and the moments stay f32 (~0.92 KB).
+// Parameters, gradients and all arithmetic stay f32.
 #include "gs_kernels.h"
 
 namespace gs {
 
-template <int kDeg, int kAccum, bool kDebug, bool kMixed>
-__global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBufs u) {
+// Screen-space gradient sums of Gaussian i (decoded), cleared in place as
+// the reference's chain clears them (shaders.js:727-732).
+template <int kAccum>
+__device__ __forceinline__ void take_sums(void* accum, int64_t i, float* gp) {
+  int4* acc = (int4*)accum + 3 * i;
+  const int4 a0 = acc[0], a1 = acc[1], a2 = acc[2];
+  const int raw[10] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, a2.x, a2.y};
+  const int4 z = make_int4(0, 0, 0, 0);
+  acc[0] = z; acc[1] = z; acc[2] = z;
+#pragma unroll
+  for (int k = 0; k < 10; k++) gp[k] = kAccum == 0 ? (float)raw[k] / kFixed : __int_as_float(raw[k]);
+}
+
+// The reference's chain (shaders.js:735-947) for one view: the decoded
+// screen-space sums gp[10] of a Gaussian with keys in that view, chained
+// through the view's camera to the 14 parameter gradients gF and the SH
+// coefficient gradients shG (both zero on entry).
+template <int kDeg>
+__device__ __forceinline__ void chain_one(const Cam& cam, const AdamHP& hp, const float* p, const float* sh,
+                                          float* gp, float* gF, float* shG) {
   constexpr int K = ShDims<kDeg>::K;
-  constexpr int NC = ShDims<kDeg>::C;
-  constexpr int P4 = ShDims<kDeg>::P4;
-  constexpr int Q8 = ShDims<kDeg>::Q8;
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= u.n) return;
-  const int64_t cap = u.cap;
-
-  // ---- screen-space gradients (decode + clear), shaders.js:727-744 ----
-  // Only Gaussians with keys in this step can have nonzero sums (the
-  // backward adds to keyed Gaussians only, and every pass leaves the sums
-  // cleared), so a culled Gaussian's sums are zero and stay untouched.
-  const bool keyed = u.cnt[i] != 0u;
-  float gp[10];
-#pragma unroll
-  for (int k = 0; k < 10; k++) gp[k] = 0.0f;
-  if (keyed) {
-    int4* acc = (int4*)u.accum + 3 * i;
-    const int4 a0 = acc[0], a1 = acc[1], a2 = acc[2];
-    const int raw[10] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, a2.x, a2.y};
-    const int4 z = make_int4(0, 0, 0, 0);
-    acc[0] = z; acc[1] = z; acc[2] = z;
-#pragma unroll
-    for (int k = 0; k < 10; k++)
-      gp[k] = kAccum == 0 ? (float)raw[k] / kFixed : __int_as_float(raw[k]);
-  }
-
-  float p[16];
-  load_params(u.P, cap, i, p);
-  float sh[ShDims<kDeg>::S > 0 ? ShDims<kDeg>::S : 1];
-  if constexpr (kDeg > 0) load_sh_any<kDeg, kMixed>(u.SH, cap, i, sh);
-
-  float gF[14];
-#pragma unroll
-  for (int k = 0; k < 14; k++) gF[k] = 0.0f;
-  float shG[NC > 0 ? NC : 1];
-#pragma unroll
-  for (int k = 0; k < NC; k++) shG[k] = 0.0f;
-
   Geom g;
-  g.ok = false;
-  if (keyed) compute_geom(cam, p, g);
+  compute_geom(cam, p, g);
   if (g.ok) {
     const float cden = 1.0f + conic_lmax(g.va, g.vb, g.vc);
     gp[2] = gp[2] * cden;
@@ -149,15 +130,20 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
     gF[13] = gp[6];
   }
 
-  if (kDebug) {
-    for (int k = 0; k < 14; k++) u.dbgGradF[16 * i + k] = gF[k];
-    u.dbgGradF[16 * i + 14] = 0.0f;
-    u.dbgGradF[16 * i + 15] = 0.0f;
-    for (int k = 0; k < NC; k++) u.dbgShGrad[(int64_t)NC * i + k] = shG[k];
-  }
+}
 
+// Adam over one Gaussian's SH coefficients and parameters with gradients gF
+// and shG (the reference's sh-adam and adam passes), in the storage formats
+// of precision mode kPrec; p and sh hold the current values.
+template <int kDeg, int kPrec>
+__device__ __forceinline__ void adam_one(const AdamHP& hp, const UpdateBufs& u, int64_t i, float* p,
+                                         const float* sh, const float* gF, const float* shG) {
+  constexpr int NC = ShDims<kDeg>::C;
+  constexpr int P4 = ShDims<kDeg>::P4;
+  constexpr int Q8 = ShDims<kDeg>::Q8;
+  const int64_t cap = u.cap;
   // ---- Adam over the SH coefficients (shaders.js:963-981) ----
-  if constexpr (kDeg > 0 && kMixed) {
+  if constexpr (kDeg > 0 && Prec<kPrec>::mom16) {
     // bf16 coefficients and moments (streams 2: coefficient, 3: m, 4: v);
     // the padding halves of the last plane stay zero
     const uint4* SHM16 = (const uint4*)u.SHM;
@@ -198,6 +184,49 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
       ((uint4*)u.SH)[c * cap + i] = pack_bf8(bs);
       ((uint4*)u.SHM)[c * cap + i] = pack_bf8(bm);
       ((uint4*)u.SHV)[c * cap + i] = pack_bf8(bv);
+    }
+  } else if constexpr (kDeg > 0 && Prec<kPrec>::sh != 0) {
+    // 16-bit coefficients (stream 2, as above), f32 moments
+    constexpr int F = Prec<kPrec>::sh;
+    float4* SHM = (float4*)u.SHM;
+    float4* SHV = (float4*)u.SHV;
+    float ss[8 * Q8];
+#pragma unroll
+    for (int j = 0; j < 8 * Q8; j++) ss[j] = 0.0f;
+#pragma unroll
+    for (int c = 0; c < P4; c++) {
+      const float4 m4 = SHM[c * cap + i];
+      const float4 v4 = SHV[c * cap + i];
+      float ms[4] = {m4.x, m4.y, m4.z, m4.w};
+      float vs[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+      for (int l = 0; l < 4; l++) {
+        const int j = 4 * c + l;
+        if (j >= NC) break;
+        float gr = shG[j];
+        if (!(fabsf(gr) < 1e18f)) gr = 0.0f;
+        const float mm = hp.b1 * ms[l] + (1.0f - hp.b1) * gr;
+        const float vn = hp.b2 * vs[l] + (1.0f - hp.b2) * gr * gr;
+        ms[l] = mm;
+        vs[l] = vn;
+        const float mh = mm * hp.rbc1;
+        const float vh = vn * hp.rbc2;
+        ss[j] = sh[j] - hp.shLr * mh / (sqrtf(vh) + hp.eps);
+      }
+      SHM[c * cap + i] = make_float4(ms[0], ms[1], ms[2], ms[3]);
+      SHV[c * cap + i] = make_float4(vs[0], vs[1], vs[2], vs[3]);
+    }
+    const uint32_t seed = gs_sr_seed((uint64_t)i, hp.step);
+#pragma unroll
+    for (int c = 0; c < Q8; c++) {
+      uint32_t b[8];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint32_t h = gs_sr_pair(seed, 2u * 32u + 4u * (uint32_t)c + (uint32_t)q);
+        b[2 * q] = gs_f_to_16_sr<F>(ss[8 * c + 2 * q], h);
+        b[2 * q + 1] = gs_f_to_16_sr<F>(ss[8 * c + 2 * q + 1], h >> 16);
+      }
+      ((uint4*)u.SH)[c * cap + i] = pack_bf8(b);
     }
   } else if constexpr (kDeg > 0) {
     float4* SH = (float4*)u.SH;
@@ -254,7 +283,7 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
     if (j >= 10 && j <= 13) pn = clampf(pn, -hp.maxLogit, hp.maxLogit);
     p[j] = pn;
   };
-  if constexpr (kMixed) {
+  if constexpr (Prec<kPrec>::mom16) {
     // moments stored in bf16 with stochastic rounding (streams 0: m, 1: v);
     // slots Adam skips re-encode to their own bits
     uint4* M16 = (uint4*)u.M;
@@ -304,36 +333,162 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
   }
 }
 
-template <int kDeg, bool kMixed>
+template <int kDeg, int kAccum, bool kDebug, int kPrec>
+__global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBufs u) {
+  constexpr int NC = ShDims<kDeg>::C;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= u.n) return;
+  const int64_t cap = u.cap;
+
+  // Only Gaussians with keys in this step can have nonzero sums (the
+  // backward adds to keyed Gaussians only, and every pass leaves the sums
+  // cleared), so a culled Gaussian's sums are zero and stay untouched.
+  const bool keyed = u.cnt[i] != 0u;
+  float p[16];
+  load_params(u.P, cap, i, p);
+  float sh[ShDims<kDeg>::S > 0 ? ShDims<kDeg>::S : 1];
+  if constexpr (kDeg > 0) load_sh_any<kDeg, Prec<kPrec>::sh>(u.SH, cap, i, sh);
+
+  float gF[14];
+#pragma unroll
+  for (int k = 0; k < 14; k++) gF[k] = 0.0f;
+  float shG[NC > 0 ? NC : 1];
+#pragma unroll
+  for (int k = 0; k < NC; k++) shG[k] = 0.0f;
+  if (keyed) {
+    float gp[10];
+    take_sums<kAccum>(u.accum, i, gp);
+    chain_one<kDeg>(cam, hp, p, sh, gp, gF, shG);
+  }
+
+  if (kDebug) {
+    for (int k = 0; k < 14; k++) u.dbgGradF[16 * i + k] = gF[k];
+    u.dbgGradF[16 * i + 14] = 0.0f;
+    u.dbgGradF[16 * i + 15] = 0.0f;
+    for (int k = 0; k < NC; k++) u.dbgShGrad[(int64_t)NC * i + k] = shG[k];
+  }
+  adam_one<kDeg, kPrec>(hp, u, i, p, sh, gF, shG);
+}
+
+// A batched step's update: each view's sums chained through that view's
+// camera, the views' gradients added in view order (the first view with
+// keys gives the value, later ones add to it), then one Adam step with the
+// step's constants (vb->hp). Gaussians without keys in a view skip it.
+template <int kDeg, int kAccum, bool kDebug, int kPrec>
+__global__ __launch_bounds__(kBlock) void k_update_views(const ViewBlock* __restrict__ vb, UpdateBufs u) {
+  constexpr int NC = ShDims<kDeg>::C;
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= u.n) return;
+  const int64_t cap = u.cap;
+  const AdamHP hp = vb->hp;
+  float p[16];
+  load_params(u.P, cap, i, p);
+  float sh[ShDims<kDeg>::S > 0 ? ShDims<kDeg>::S : 1];
+  if constexpr (kDeg > 0) load_sh_any<kDeg, Prec<kPrec>::sh>(u.SH, cap, i, sh);
+  float gF[14];
+#pragma unroll
+  for (int k = 0; k < 14; k++) gF[k] = 0.0f;
+  float shG[NC > 0 ? NC : 1];
+#pragma unroll
+  for (int k = 0; k < NC; k++) shG[k] = 0.0f;
+  bool first = true;
+  const int V = vb->count;
+  for (int v = 0; v < V; v++) {
+    if (vb->cnt[v][i] == 0u) continue;
+    float gp[10];
+    take_sums<kAccum>(vb->accum[v], i, gp);
+    const Cam cam = vb->cams[v];
+    float gFv[14];
+#pragma unroll
+    for (int k = 0; k < 14; k++) gFv[k] = 0.0f;
+    float shGv[NC > 0 ? NC : 1];
+#pragma unroll
+    for (int k = 0; k < NC; k++) shGv[k] = 0.0f;
+    chain_one<kDeg>(cam, hp, p, sh, gp, gFv, shGv);
+#pragma unroll
+    for (int k = 0; k < 14; k++) gF[k] = first ? gFv[k] : gF[k] + gFv[k];
+#pragma unroll
+    for (int k = 0; k < NC; k++) shG[k] = first ? shGv[k] : shG[k] + shGv[k];
+    first = false;
+  }
+  if (kDebug) {
+    for (int k = 0; k < 14; k++) u.dbgGradF[16 * i + k] = gF[k];
+    u.dbgGradF[16 * i + 14] = 0.0f;
+    u.dbgGradF[16 * i + 15] = 0.0f;
+    for (int k = 0; k < NC; k++) u.dbgShGrad[(int64_t)NC * i + k] = shG[k];
+  }
+  adam_one<kDeg, kPrec>(hp, u, i, p, sh, gF, shG);
+}
+
+template <int kDeg, int kPrec>
+static void launch_views_deg(hipStream_t st, int accumMode, bool dbg, const ViewBlock* vb, const UpdateBufs& u) {
+  const dim3 grid((unsigned)((u.n + kBlock - 1) / kBlock));
+  if (accumMode == 0) {
+    if (dbg) k_update_views<kDeg, 0, true, kPrec><<<grid, kBlock, 0, st>>>(vb, u);
+    else k_update_views<kDeg, 0, false, kPrec><<<grid, kBlock, 0, st>>>(vb, u);
+  } else {
+    if (dbg) k_update_views<kDeg, 1, true, kPrec><<<grid, kBlock, 0, st>>>(vb, u);
+    else k_update_views<kDeg, 1, false, kPrec><<<grid, kBlock, 0, st>>>(vb, u);
+  }
+}
+
+template <int kPrec>
+static void launch_views_prec(hipStream_t st, int deg, int accumMode, bool dbg, const ViewBlock* vb,
+                              const UpdateBufs& u) {
+  switch (deg) {
+    case 0: launch_views_deg<0, kPrec>(st, accumMode, dbg, vb, u); break;
+    case 1: launch_views_deg<1, kPrec>(st, accumMode, dbg, vb, u); break;
+    case 2: launch_views_deg<2, kPrec>(st, accumMode, dbg, vb, u); break;
+    default: launch_views_deg<3, kPrec>(st, accumMode, dbg, vb, u); break;
+  }
+}
+
+void launch_update_views(hipStream_t st, int deg, int accumMode, int prec, const ViewBlock* vb,
+                         const UpdateBufs& u) {
+  if (u.n <= 0) return;
+  const bool dbg = u.dbgGradF != nullptr;
+  switch (prec) {
+    case 0: launch_views_prec<0>(st, deg, accumMode, dbg, vb, u); break;
+    case 1: launch_views_prec<1>(st, deg, accumMode, dbg, vb, u); break;
+    case 2: launch_views_prec<2>(st, deg, accumMode, dbg, vb, u); break;
+    default: launch_views_prec<3>(st, deg, accumMode, dbg, vb, u); break;
+  }
+}
+
+template <int kDeg, int kPrec>
 static void launch_deg(hipStream_t st, int accumMode, bool dbg, const Cam& cam, const AdamHP& hp,
                        const UpdateBufs& u) {
   const dim3 grid((unsigned)((u.n + kBlock - 1) / kBlock));
   if (accumMode == 0) {
-    if (dbg) k_update<kDeg, 0, true, kMixed><<<grid, kBlock, 0, st>>>(cam, hp, u);
-    else k_update<kDeg, 0, false, kMixed><<<grid, kBlock, 0, st>>>(cam, hp, u);
+    if (dbg) k_update<kDeg, 0, true, kPrec><<<grid, kBlock, 0, st>>>(cam, hp, u);
+    else k_update<kDeg, 0, false, kPrec><<<grid, kBlock, 0, st>>>(cam, hp, u);
   } else {
-    if (dbg) k_update<kDeg, 1, true, kMixed><<<grid, kBlock, 0, st>>>(cam, hp, u);
-    else k_update<kDeg, 1, false, kMixed><<<grid, kBlock, 0, st>>>(cam, hp, u);
+    if (dbg) k_update<kDeg, 1, true, kPrec><<<grid, kBlock, 0, st>>>(cam, hp, u);
+    else k_update<kDeg, 1, false, kPrec><<<grid, kBlock, 0, st>>>(cam, hp, u);
   }
 }
 
-template <bool kMixed>
+template <int kPrec>
 static void launch_prec(hipStream_t st, int deg, int accumMode, bool dbg, const Cam& cam, const AdamHP& hp,
                         const UpdateBufs& u) {
   switch (deg) {
-    case 0: launch_deg<0, kMixed>(st, accumMode, dbg, cam, hp, u); break;
-    case 1: launch_deg<1, kMixed>(st, accumMode, dbg, cam, hp, u); break;
-    case 2: launch_deg<2, kMixed>(st, accumMode, dbg, cam, hp, u); break;
-    default: launch_deg<3, kMixed>(st, accumMode, dbg, cam, hp, u); break;
+    case 0: launch_deg<0, kPrec>(st, accumMode, dbg, cam, hp, u); break;
+    case 1: launch_deg<1, kPrec>(st, accumMode, dbg, cam, hp, u); break;
+    case 2: launch_deg<2, kPrec>(st, accumMode, dbg, cam, hp, u); break;
+    default: launch_deg<3, kPrec>(st, accumMode, dbg, cam, hp, u); break;
   }
 }
 
-void launch_update(hipStream_t st, int deg, int accumMode, bool mixed, const Cam& cam, const AdamHP& hp,
+void launch_update(hipStream_t st, int deg, int accumMode, int prec, const Cam& cam, const AdamHP& hp,
                    const UpdateBufs& u) {
   if (u.n <= 0) return;
   const bool dbg = u.dbgGradF != nullptr;
-  if (mixed) launch_prec<true>(st, deg, accumMode, dbg, cam, hp, u);
-  else launch_prec<false>(st, deg, accumMode, dbg, cam, hp, u);
+  switch (prec) {
+    case 0: launch_prec<0>(st, deg, accumMode, dbg, cam, hp, u); break;
+    case 1: launch_prec<1>(st, deg, accumMode, dbg, cam, hp, u); break;
+    case 2: launch_prec<2>(st, deg, accumMode, dbg, cam, hp, u); break;
+    default: launch_prec<3>(st, deg, accumMode, dbg, cam, hp, u); break;
+  }
 }
 
 }  // namespace gs

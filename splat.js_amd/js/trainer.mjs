@@ -31,7 +31,7 @@ export const TILE = 16;
 export const LOSS = { charbonnier: 0, l1_dssim: 1 };
 export const ACCUM = { fixed: 0, f32: 1 };
 export const SORT = { depthThenTile: 0, global64: 1, tileSegments: 2 };
-export const PRECISION = { f32: 0, mixed: 1 };
+export const PRECISION = { f32: 0, mixed: 1, bf16: 2, fp16: 3 };
 
 const shRestCoefs = (deg) => (deg + 1) * (deg + 1) - 1;
 
@@ -139,6 +139,19 @@ export class GSTrainer {
     // trainer.js:468-470, :533-535: a camera Adam step just moved the poses,
     // the shared focal and the exposures; mirror them into camMeta
     if (this.opts.camOpt && this._iter > 1500 && this._iter % 25 === 0) this._syncCamMeta();
+  }
+
+  /** Batched step (no reference counterpart): the views `cams` (-1 picks
+   *  with the seeded RNG) rasterized forward and backward in one launch
+   *  each, their gradients added, one Adam step; iter advances by the view
+   *  count. Needs opts.viewsPerStep >= cams.length. Enqueues only. */
+  stepViews(cams) {
+    if (this.excluded !== this._excludedSent) {
+      addon.setExcluded(this._h, this.excluded ? Array.from(this.excluded) : []);
+      this._excludedSent = this.excluded;
+    }
+    this._call(() => addon.stepViews(this._h, cams));
+    this._iter += cams.length;
   }
 
   _syncCamMeta() {

@@ -17,7 +17,7 @@ LIB_PATH = os.path.join(os.path.dirname(HERE), "libsplatgs.so")
 GS_LOSS_CHARBONNIER, GS_LOSS_L1_DSSIM = 0, 1
 GS_ACCUM_FIXED, GS_ACCUM_F32 = 0, 1
 GS_SORT_DEPTH_THEN_TILE, GS_SORT_GLOBAL64, GS_SORT_TILE_SEGMENTS = 0, 1, 2
-GS_PREC_F32, GS_PREC_MIXED = 0, 1
+GS_PREC_F32, GS_PREC_MIXED, GS_PREC_BF16, GS_PREC_FP16 = 0, 1, 2, 3
 
 
 class GsOpts(C.Structure):
@@ -28,7 +28,8 @@ class GsOpts(C.Structure):
                 ("tileGrad", C.c_int32), ("posLr", C.c_float), ("loss", C.c_int32),
                 ("ssimLambda", C.c_float), ("gradAccum", C.c_int32), ("sortMode", C.c_int32),
                 ("seed", C.c_uint32), ("device", C.c_int32), ("precision", C.c_int32),
-                ("camOpt", C.c_int32), ("expComp", C.c_int32)]
+                ("camOpt", C.c_int32), ("expComp", C.c_int32), ("viewsPerStep", C.c_int32),
+                ("graph", C.c_int32)]
 
 
 class GsCamera(C.Structure):
@@ -101,6 +102,8 @@ SIGNATURES = {
     "gs_sync": ([C.c_void_p], C.c_int),
     "gs_refine": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "gs_step_batch": ([C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
+    "gs_step_views": ([C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
+    "gs_step_views_debug": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
     "gs_export_ply": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
     "gs_export_ply_noalloc": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
     "gs_last_error": ([C.c_void_p], C.c_char_p),
@@ -238,6 +241,46 @@ class GSTrainer:
         """len(cams) steps in one call, as that many stepOnce(cams[k]) calls."""
         a = np.ascontiguousarray(cams, np.int32)
         self._ck(lib().gs_step_batch(self._h, _p(a), int(a.size)))
+
+    def stepViews(self, cams):
+        """One batched step over len(cams) views (negative entries pick with
+        the seeded RNG): summed gradients, one Adam step."""
+        a = np.ascontiguousarray(cams, np.int32)
+        self._ck(lib().gs_step_views(self._h, _p(a), int(a.size)))
+
+    def step_views_debug(self, cams, want=("img", "T", "end", "gC", "accum", "tileRanges")):
+        """stepViews without the graph, returning each view's intermediates
+        and (in the first entry) the summed gradients before Adam."""
+        info = self.info()
+        n = info.n
+        a = np.ascontiguousarray(cams, np.int32)
+        outs, keep = [], []
+        arr = (GsDebugOut * a.size)()
+        for k, ci in enumerate(a):
+            m = self.camMeta[int(ci)] if ci >= 0 else None
+            npx = m["w"] * m["h"] if m else 0
+            nt = ((m["w"] + 15) // 16) * ((m["h"] + 15) // 16) if m else 0
+            shapes = {"img": ((npx, 4), np.float32), "T": ((npx,), np.float32), "end": ((npx,), np.uint32),
+                      "gC": ((npx, 4), np.float32), "accum": ((n, 10), np.int32),
+                      "tileRanges": ((nt, 2), np.uint32)}
+            o = {key: np.zeros(*shapes[key]) for key in want}
+            if k == 0:
+                o["gradF"] = np.zeros((n, 16), np.float32)
+                o["shGrad"] = np.zeros((n, max(3 * self.shK, 1)), np.float32)
+                o["stats"] = np.zeros(4, np.float64)
+            nk = np.zeros(1, np.int64)
+            for key, val in o.items():
+                setattr(arr[k], key, val.ctypes.data)
+            arr[k].numKeys = nk.ctypes.data
+            outs.append((o, nk))
+        self._ck(lib().gs_step_views_debug(self._h, _p(a), int(a.size), arr))
+        res = []
+        for o, nk in outs:
+            o["numKeys"] = int(nk[0])
+            if "shGrad" in o:
+                o["shGrad"] = o["shGrad"][:, :3 * self.shK]
+            res.append(o)
+        return res
 
     def readLoss(self):
         mse = C.c_double()

@@ -13,14 +13,16 @@ def f32(x):
     return float(np.float32(x))
 
 
-def adam_for_iter(it, radius, posLr=0.0, horizon=60000, minScale=1e-4, opacityReg=0.01, shLr=1.25e-4):
-    """AdamHP for the step whose Adam t = it (after iter++)."""
+def adam_for_iter(it, radius, posLr=0.0, horizon=60000, minScale=1e-4, opacityReg=0.01, shLr=1.25e-4, t=None):
+    """AdamHP for the step at iteration `it` (after iter++; the position-lr
+    schedule) whose Adam step is t (bias correction; t = it unless batched
+    steps ran)."""
     r = f32(radius)
     base = f32(posLr) if posLr > 0 else 3e-4 * r    # gs_opts.posLr is an f32 field
     pos = f32(base * 0.01 ** min(1.0, it / (0.75 * horizon)))
     lr = [pos] * 3 + [5e-3] * 3 + [1e-3] * 4 + [1.5e-2] * 3 + [2.5e-2, 0, 0]
-    return O.adam_hp(lr, t=it, minLS=f32(math.log(r * f32(minScale))), maxLS=f32(math.log(r * 0.05)),
-                     maxLogit=8.0, reg=f32(opacityReg), shLr=f32(shLr))
+    return O.adam_hp(lr, t=it if t is None else t, minLS=f32(math.log(r * f32(minScale))),
+                     maxLS=f32(math.log(r * 0.05)), maxLogit=8.0, reg=f32(opacityReg), shLr=f32(shLr))
 
 
 class OracleTrainer:
@@ -46,7 +48,7 @@ class OracleTrainer:
         self.radius = radius
         self.shDeg = shDeg
         self.fixed = fixed
-        self.iter = 0
+        self.iter = 0       # also sets the Adam step counter, as gs_set_iter does
         self.tgt = []
         for c in cams:
             im = images[c["imgIdx"]]
@@ -57,8 +59,14 @@ class OracleTrainer:
         self.camopt = CO.CamState(cams) if opts.get("camOpt") else None
         self.camGrad = np.zeros((len(cams) + 1, 8), np.int64)
         self.holdout = -1
-        # mixed precision: SH coefficients and all moments live in bf16 (uint16 bits)
+        # precision modes: "mixed" keeps the SH coefficients and all moments in
+        # bf16 (uint16 bits); "bf16" / "fp16" keep the SH coefficients in 16
+        # bits and round the raster record's colours to nearest even
         self.mixed = precision == "mixed"
+        self.fmt16 = {"bf16": 1, "fp16": 2}.get(precision, 0)
+        if self.fmt16:
+            self.SH16 = self._enc_rne(self.SH)
+            self.SH = self._dec(self.SH16)
         if self.mixed:
             self.SH16 = O.f2bf_rne(self.SH)
             self.SH = O.bf2f(self.SH16)
@@ -66,6 +74,89 @@ class OracleTrainer:
             self.V16 = np.zeros(self.P.shape, np.uint16)
             self.SM16 = np.zeros(self.SH.shape, np.uint16)
             self.SV16 = np.zeros(self.SH.shape, np.uint16)
+
+    @property
+    def iter(self):
+        return self._iter
+
+    @iter.setter
+    def iter(self, v):
+        self._iter = v
+        self.adamT = v
+
+    def _hp(self):
+        return adam_for_iter(self._iter, self.radius, posLr=self.opts.get("posLr", 0.0),
+                             horizon=self.opts.get("maxIters", 60000), opacityReg=self.opts.get("opacityReg", 0.01),
+                             shLr=self.opts.get("shLr", 1.25e-4), t=self.adamT)
+
+    def _adam(self, gradF, shG):
+        hp = self._hp()
+        if self.mixed:
+            O.adam_mixed(hp, self.P, gradF, self.M16, self.V16, self.adamT)
+            if self.K:
+                O.sh_adam_mixed(hp, self.SH16, shG, self.SM16, self.SV16, self.adamT)
+                self.SH = O.bf2f(self.SH16)
+        elif self.fmt16:
+            O.adam(hp, self.P, gradF, self.M, self.V)
+            if self.K:
+                O.sh_adam_h16(hp, self.SH16, shG, self.SM, self.SV, self.adamT, self.fmt16)
+                self.SH = self._dec(self.SH16)
+        else:
+            O.adam(hp, self.P, gradF, self.M, self.V)
+            if self.K:
+                O.sh_adam(hp, self.SH, shG, self.SM, self.SV)
+
+    def step_views(self, cis):
+        """gs_step_views: every view from the same parameters, each view's
+        sums chained through its camera, the gradients added in view order
+        (the first view in which a Gaussian has keys gives the value, later
+        ones add to it in f32), one Adam step; iter advances by the view
+        count and the Adam step counter by one."""
+        deg = min(self.shDeg, self._iter // 1000)
+        st = self.st
+        n = self.P.shape[0]
+        gradF = np.zeros((n, 16), np.float32)
+        shG = np.zeros((n, 3 * self.K), np.float32)
+        seen = np.zeros(n, bool)
+        per = []
+        for ci in cis:
+            cam = O.make_cam(self.cams[ci], shActiveDeg=deg)
+            proj, rect, cnt = st.project(cam, self.P, self.SH if self.K else None)
+            proj = self._round_record(proj)
+            ids, ts, keys = st.bin_sort(cam, proj, rect, cnt)
+            img, T, end = st.render_fwd(cam, proj, ids, ts)
+            gC, valid, stats = st.loss_charb(cam, img, self.tgt[ci])[:3]
+            g = st.render_bwd(cam, proj, ids, ts, T, end, gC, valid, fixed=self.fixed)
+            sh = self.SH if self.K else None
+            ch = (st.chain(cam, self.P, sh, proj, gfix=g) if self.fixed
+                  else st.chain(cam, self.P, sh, proj, gsum=g))
+            gv, sv = ch[0].astype(np.float32), ch[1].astype(np.float32)
+            mask = cnt > 0
+            first, later = mask & ~seen, mask & seen
+            gradF[first] = gv[first]
+            gradF[later] = gradF[later] + gv[later]
+            if self.K:
+                shG[first] = sv[first]
+                shG[later] = shG[later] + sv[later]
+            seen |= mask
+            per.append({"ids": ids, "ts": ts, "img": img, "T": T, "end": end, "gC": gC, "valid": valid,
+                        "stats": stats, "accum": g, "cnt": cnt})
+        self._iter += len(cis)
+        self.adamT += 1
+        self._adam(gradF, shG)
+        return per, gradF, shG
+
+    def _enc_rne(self, x):
+        return O.f2h_rne(x) if self.fmt16 == 2 else O.f2bf_rne(x)
+
+    def _dec(self, b):
+        return O.h2f(b) if self.fmt16 == 2 else O.bf2f(b)
+
+    def _round_record(self, proj):
+        """Record colours stored in 16 bits (rounded to nearest even)."""
+        if self.fmt16:
+            proj[:, 8:11] = self._dec(self._enc_rne(proj[:, 8:11].astype(np.float32)))
+        return proj
 
     def cam_grads(self):
         return CO.wrap_i32(self.camGrad)
@@ -75,10 +166,12 @@ class OracleTrainer:
         self.cam_deg[ci] = deg
         cd = self.camopt.cam_dict(ci) if self.camopt else self.cams[ci]
         cam = O.make_cam(cd, shActiveDeg=deg)
-        self.iter += 1
+        self._iter += 1
+        self.adamT += 1
         st = self.st
         co = self.camopt is not None
         proj, rect, cnt = st.project(cam, self.P, self.SH if self.K else None)
+        proj = self._round_record(proj)
         ids, ts, keys = st.bin_sort(cam, proj, rect, cnt)
         img, T, end = st.render_fwd(cam, proj, ids, ts)
         lo = st.loss_charb(cam, img, self.tgt[ci], cam_grads=co)
@@ -92,20 +185,9 @@ class OracleTrainer:
             self.camGrad[ci, 6:8] += lo[3]
             self.camGrad[ci, 0:6] += ch[2][:6]
             self.camGrad[len(self.cams), 0] += ch[2][6]
-        hp = adam_for_iter(self.iter, self.radius, posLr=self.opts.get("posLr", 0.0),
-                           horizon=self.opts.get("maxIters", 60000),
-                           opacityReg=self.opts.get("opacityReg", 0.01), shLr=self.opts.get("shLr", 1.25e-4))
         pre = {"proj": proj, "rect": rect, "cnt": cnt, "ids": ids, "ts": ts, "keys": keys, "img": img, "T": T,
                "end": end, "gC": gC, "valid": valid, "stats": stats, "accum": g, "gradF": gradF, "shGrad": shG}
-        if self.mixed:
-            O.adam_mixed(hp, self.P, gradF, self.M16, self.V16, self.iter)
-            if self.K:
-                O.sh_adam_mixed(hp, self.SH16, shG, self.SM16, self.SV16, self.iter)
-                self.SH = O.bf2f(self.SH16)
-        else:
-            O.adam(hp, self.P, gradF, self.M, self.V)
-            if self.K:
-                O.sh_adam(hp, self.SH, shG, self.SM, self.SV)
+        self._adam(gradF, shG)
         if co:
             pre["camGrad"] = self.cam_grads()
             if self.iter > 1500 and self.iter % 25 == 0:      # trainer.js:468-470
@@ -122,6 +204,7 @@ class OracleTrainer:
         cam = O.make_cam(src, shActiveDeg=self.shDeg)
         st = self.st
         proj, rect, cnt = st.project(cam, self.P, self.SH if self.K else None)
+        proj = self._round_record(proj)
         ids, ts, _ = st.bin_sort(cam, proj, rect, cnt)
         img, T, end = st.render_fwd(cam, proj, ids, ts)
         gC, valid, stats, efix, _ = st.loss_charb(cam, img, self.tgt[ci], cam_grads=True)

@@ -94,7 +94,8 @@ def _check_step(gpu, ref, n_tiles, label, stats_acc=None):
 
 def _run_pair(params, sh, cams, images, W, H, radius, opts, shDeg, steps, start_iter=0, cams_seq=None, **kw):
     gpu = _trainer(params, sh, cams, images, W, H, radius, opts, shDeg=shDeg, **kw)
-    prec = "mixed" if kw.get("precision", gsabi.GS_PREC_F32) == gsabi.GS_PREC_MIXED else "f32"
+    prec = {gsabi.GS_PREC_MIXED: "mixed", gsabi.GS_PREC_BF16: "bf16",
+            gsabi.GS_PREC_FP16: "fp16"}.get(kw.get("precision", gsabi.GS_PREC_F32), "f32")
     ref = OracleTrainer(params, sh, cams, images, radius, opts=opts, shDeg=shDeg, precision=prec)
     gpu.iter = start_iter
     ref.iter = start_iter
@@ -153,18 +154,24 @@ def test_spatially_ordered_input_bit_exact():
               steps=2, start_iter=2500)
 
 
-def test_mixed_precision_rig_deg3_bit_exact():
-    """Mixed precision (bf16 SH and moments, stochastic rounding) on the
-    degree-3 rig: every intermediate and the updated params/SH match the
-    restatement's mixed mode bit for bit over four steps."""
+PRECISIONS = [gsabi.GS_PREC_MIXED, gsabi.GS_PREC_BF16, gsabi.GS_PREC_FP16]
+
+
+@pytest.mark.parametrize("precision", PRECISIONS, ids=["mixed", "bf16", "fp16"])
+def test_mixed_precision_rig_deg3_bit_exact(precision):
+    """16-bit storage modes on the degree-3 rig: bf16 SH and moments (mixed),
+    or bf16 / fp16 SH and record colours with f32 moments, all with
+    stochastic rounding of the stored coefficients. Every intermediate and
+    the updated params/SH match the restatement's same mode bit for bit over
+    four steps."""
     params, sh, cams, images, W, H, radius, opts = rig_setup(3)
-    _run_pair(params, sh, cams, images, W, H, radius, opts, 3, steps=4, start_iter=5000,
-              precision=gsabi.GS_PREC_MIXED)
+    _run_pair(params, sh, cams, images, W, H, radius, opts, 3, steps=4, start_iter=5000, precision=precision)
 
 
-def test_mixed_precision_config1_bit_exact():
+@pytest.mark.parametrize("precision", PRECISIONS, ids=["mixed", "bf16", "fp16"])
+def test_mixed_precision_config1_bit_exact(precision):
     p, sh, cams, images, W, H, radius = config1_setup()
-    _run_pair(p, sh, cams, images, W, H, radius, {}, 2, steps=3, start_iter=2500, precision=gsabi.GS_PREC_MIXED)
+    _run_pair(p, sh, cams, images, W, H, radius, {}, 2, steps=3, start_iter=2500, precision=precision)
 
 
 def test_sort_strategies_give_the_same_permutation():

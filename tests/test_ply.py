@@ -71,13 +71,13 @@ def test_bake_closed_form():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", ["f32", "mixed"])
+@pytest.mark.parametrize("precision", ["f32", "mixed", "fp16"])
 def test_gpu_export_ply_matches_restatement(precision):
     n, W, H = 20000, 320, 240
     p, sh, _ = scenes.config1(0)
     cams = scenes.ring_cameras(6, 4.0, W, H, 300.0)
     images = [{"tw": W, "th": H, "rgb": np.full(W * H * 3, 0.5, np.float32)} for _ in cams]
-    prec = gsabi.GS_PREC_MIXED if precision == "mixed" else gsabi.GS_PREC_F32
+    prec = {"mixed": gsabi.GS_PREC_MIXED, "fp16": gsabi.GS_PREC_FP16}.get(precision, gsabi.GS_PREC_F32)
     t = gsabi.GSTrainer(shDeg=2, maxSplats=n, precision=prec)
     t.setup({"data": p, "n": n, "sh": sh}, cams, images, W, H, 4.0)
     t.iter = 2500

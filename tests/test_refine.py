@@ -233,14 +233,15 @@ def _pad(a, cap):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("precision", ["f32", "mixed"])
+@pytest.mark.parametrize("precision", ["f32", "mixed", "bf16", "fp16"])
 def test_gpu_refine_matches_restatement(precision):
     p, sh, cams, images, W, H, radius = config1_setup(n=6000)
     p = p.copy()
     rng = np.random.default_rng(11)
     p[rng.random(p.shape[0]) < 0.1, 13] = -5.0          # some dead splats
     n = p.shape[0]
-    prec = gsabi.GS_PREC_MIXED if precision == "mixed" else gsabi.GS_PREC_F32
+    prec = {"mixed": gsabi.GS_PREC_MIXED, "bf16": gsabi.GS_PREC_BF16,
+            "fp16": gsabi.GS_PREC_FP16}.get(precision, gsabi.GS_PREC_F32)
     t = gsabi.GSTrainer(shDeg=2, maxSplats=4 * n, precision=prec)
     t.setup({"data": p, "n": n, "sh": sh}, cams, images, W, H, radius)
     ref = OracleTrainer(p, sh, cams, images, radius, shDeg=2, precision=precision)
@@ -254,6 +255,9 @@ def test_gpu_refine_matches_restatement(precision):
     if precision == "mixed":
         st = {"P": _pad(ref.P, cap), "M": _pad(ref.M16, cap), "V": _pad(ref.V16, cap),
               "SH": _pad(ref.SH16, cap), "SM": _pad(ref.SM16, cap), "SV": _pad(ref.SV16, cap)}
+    elif precision in ("bf16", "fp16"):
+        st = {"P": _pad(ref.P, cap), "M": _pad(ref.M, cap), "V": _pad(ref.V, cap),
+              "SH": _pad(ref.SH16, cap), "SM": _pad(ref.SM, cap), "SV": _pad(ref.SV, cap)}
     else:
         st = {"P": _pad(ref.P, cap), "M": _pad(ref.M, cap), "V": _pad(ref.V, cap),
               "SH": _pad(ref.SH, cap), "SM": _pad(ref.SM, cap), "SV": _pad(ref.SV, cap)}
@@ -266,6 +270,10 @@ def test_gpu_refine_matches_restatement(precision):
         ref.SH16, ref.SM16, ref.SV16 = st["SH"][:n2].copy(), st["SM"][:n2].copy(), st["SV"][:n2].copy()
         import oracle as O
         ref.SH = O.bf2f(ref.SH16)
+    elif precision in ("bf16", "fp16"):
+        ref.M, ref.V = st["M"][:n2].copy(), st["V"][:n2].copy()
+        ref.SH16, ref.SM, ref.SV = st["SH"][:n2].copy(), st["SM"][:n2].copy(), st["SV"][:n2].copy()
+        ref.SH = ref._dec(ref.SH16)
     else:
         ref.M, ref.V = st["M"][:n2].copy(), st["V"][:n2].copy()
         ref.SH, ref.SM, ref.SV = st["SH"][:n2].copy(), st["SM"][:n2].copy(), st["SV"][:n2].copy()
