@@ -19,6 +19,8 @@ Runs, all from the same start on the same workload:
                metric's rule that moments stay f32; reported, not the headline)
   bf16 x8      the batched step: 8 views rasterized forward and backward in one
                persistent launch each, one Adam step, replayed as a hipGraph
+               (x8o: each view's raster on a second stream beside the binning of
+               the next views instead)
 `value` is the batched bf16 step's training iterations per second: views
 trained per second (the trainer's iteration counter advances by one per view,
 as in the reference's stepOnce). `config.runs` holds every run's rate, and
@@ -101,9 +103,9 @@ def algorithmic_bytes(name, n, vis, keys, npx, shC, prec, views=1):
     return None
 
 
-def _trainer(p, sh, cams, images, W, H, n, prec, views=1, graph=1):
+def _trainer(p, sh, cams, images, W, H, n, prec, views=1, graph=1, launch=gsabi.GS_VIEWS_PERSISTENT):
     t = gsabi.GSTrainer(shDeg=2, maxSplats=n, posLr=1.6e-4, seed=7, precision=PRECS[prec], viewsPerStep=views,
-                        graph=graph)
+                        graph=graph, viewsLaunch=launch)
     t.setup({"data": p, "n": n, "sh": sh}, cams, images, W, H, 8.0)
     t.iter = 2000
     return t
@@ -138,11 +140,12 @@ def run_single(prec, p, sh, cams, images, W, H, args):
             "keys": float(np.mean(keys)), "vis": float(np.mean(vis))}
 
 
-def run_batched(prec, p, sh, cams, images, W, H, args):
+def run_batched(prec, p, sh, cams, images, W, H, args, launch=gsabi.GS_VIEWS_PERSISTENT):
     """Batched steps of BATCH views: warm up (the second step captures the
     graph), time args.steps batched steps replayed from the graph, then a
-    profiled pass (eager launches with per-stage HIP events) and key counts."""
-    t = _trainer(p, sh, cams, images, W, H, args.n, prec, views=BATCH)
+    profiled pass (eager persistent launches with per-stage HIP events, one
+    stream) and key counts."""
+    t = _trainer(p, sh, cams, images, W, H, args.n, prec, views=BATCH, launch=launch)
     picks = [-1] * BATCH
     for _ in range(max(args.warmup, 2)):
         t.stepViews(picks)
@@ -166,7 +169,8 @@ def run_batched(prec, p, sh, cams, images, W, H, args):
         vis.append(s.lastVisible / BATCH)
     t.destroy()
     its = args.steps * BATCH / dt
-    log(f"{prec} x{BATCH}: {its:.1f} iters/s ({args.steps / dt:.1f} batched steps/s)")
+    form = "overlapped" if launch == gsabi.GS_VIEWS_OVERLAP else "persistent"
+    log(f"{prec} x{BATCH} {form}: {its:.1f} iters/s ({args.steps / dt:.1f} batched steps/s)")
     return {"its": its, "ms_per_iter": 1e3 * dt / (args.steps * BATCH), "steps_per_s": args.steps / dt,
             "prof": prof, "keys": float(np.mean(keys)), "vis": float(np.mean(vis))}
 
@@ -243,7 +247,8 @@ def main():
     ap.add_argument("--width", type=int, default=1600)
     ap.add_argument("--height", type=int, default=900)
     ap.add_argument("--runs", default="f32,bf16,mixed,bf16x8",
-                    help="comma list of f32, bf16, fp16, mixed (single-view) and <prec>x8 (batched)")
+                    help="comma list of f32, bf16, fp16, mixed (single-view), <prec>x8 (batched, persistent "
+                         "launches) and <prec>x8o (batched, overlapped launches)")
     ap.add_argument("--psnr-iters", type=int, default=2000, help="0 skips the PSNR drift runs")
     ap.add_argument("--threads", type=int, default=0, help="CPU baseline threads (0: all, at most 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -261,12 +266,14 @@ def main():
     for r in args.runs.split(","):
         if r.endswith("x8"):
             runs[r] = run_batched(r[:-2], p, sh, cams, images, W, H, args)
+        elif r.endswith("x8o"):
+            runs[r] = run_batched(r[:-3], p, sh, cams, images, W, H, args, launch=gsabi.GS_VIEWS_OVERLAP)
         else:
             runs[r] = run_single(r, p, sh, cams, images, W, H, args)
     head_name = "bf16x8" if "bf16x8" in runs else list(runs)[-1]
     head = runs[head_name]
-    head_prec = head_name[:-2] if head_name.endswith("x8") else head_name
-    views = BATCH if head_name.endswith("x8") else 1
+    head_prec = head_name.split("x8")[0]
+    views = BATCH if "x8" in head_name else 1
 
     npx = W * H
     shC = 24
@@ -327,7 +334,8 @@ def main():
               "kernels": kernels, "hbm_copy_ceiling_GBps": copy_gbps,
               "runs": {k: {"iters_per_s": round(v["its"], 2), "ms_per_iter": round(v["ms_per_iter"], 4),
                            **({"batched_steps_per_s": round(v["steps_per_s"], 2)} if "steps_per_s" in v else {}),
-                           "storage": storage[k[:-2] if k.endswith("x8") else k]}
+                           "storage": storage[k.split("x8")[0]],
+                           **({"launch": "overlapped" if k.endswith("x8o") else "persistent"} if "x8" in k else {})}
                        for k, v in runs.items()},
               "psnr": psnr, "psnr_protocol": f"mean PSNR over all {args.views} views after {args.psnr_iters} "
                                             f"iterations from the same start and seed; each mode against f32 "

@@ -51,6 +51,7 @@ enum { GS_SORT_DEPTH_THEN_TILE = 0, GS_SORT_GLOBAL64 = 1, GS_SORT_TILE_SEGMENTS 
  * Coefficient and moment stores round stochastically; record colours and
  * the setup upload round to nearest even. */
 enum { GS_PREC_F32 = 0, GS_PREC_MIXED = 1, GS_PREC_BF16 = 2, GS_PREC_FP16 = 3 };
+enum { GS_VIEWS_PERSISTENT = 0, GS_VIEWS_OVERLAP = 1 };
 
 /* Trainer options: the reference's opts object (trainer.js:18-36, :45-49,
  * :74, :105-107, :302, :319, :338-341), plus the MI355X build's own knobs. */
@@ -93,6 +94,11 @@ typedef struct gs_opts {
                           sizes one set of per-view buffers for each */
   int32_t graph;       /* gs_step_views replays a captured hipGraph when its shape repeats
                           (default 1) */
+  int32_t viewsLaunch; /* rasterization of a gs_step_views step: GS_VIEWS_PERSISTENT (the
+                          default: one persistent forward and one persistent backward launch over
+                          all views' tiles) or GS_VIEWS_OVERLAP (each view's forward and backward on
+                          a second stream as soon as its binning is done, beside the binning of the
+                          next views; measured slower on MI355X); identical results */
 } gs_opts;
 
 /* One training camera (trainer.js:95-100 cams[], :357-372): R row-major

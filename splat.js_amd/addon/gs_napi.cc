@@ -281,6 +281,7 @@ napi_value Create(napi_env env, napi_callback_info info) {
       o.expComp = get_num(env, a, "expComp", o.expComp) != 0.0 ? 1 : 0;
       o.viewsPerStep = (int32_t)get_num(env, a, "viewsPerStep", o.viewsPerStep);
       o.graph = get_num(env, a, "graph", o.graph) != 0.0 ? 1 : 0;
+      o.viewsLaunch = (int32_t)get_num(env, a, "viewsLaunch", o.viewsLaunch);
     }
   }
   gs_trainer_t* h = nullptr;

@@ -157,6 +157,11 @@ void launch_raster_fwd_views(hipStream_t st, const ViewBlock* vb, const Cut& cut
 void launch_raster_bwd_views(hipStream_t st, const ViewBlock* vb, const Cut& cut, int accumMode,
                              bool boundedColourGrad, uint32_t* queue, int blocks);
 int raster_views_blocks(int device);
+// view v alone (numTiles blocks, camera and buffers from vb), for the
+// overlapped form of the batched step
+void launch_raster_fwd_view(hipStream_t st, const ViewBlock* vb, int v, int numTiles, const Cut& cut, int loss);
+void launch_raster_bwd_view(hipStream_t st, const ViewBlock* vb, int v, int numTiles, const Cut& cut, int accumMode,
+                            bool boundedColourGrad);
 void launch_update_views(hipStream_t st, int deg, int accumMode, int prec, const ViewBlock* vb,
                          const UpdateBufs& u);
 

@@ -559,6 +559,35 @@ __global__ __launch_bounds__(kBlock) void k_raster_bwd_views(const ViewBlock* __
   }
 }
 
+// one view of a batched step, one block per tile (camera and buffers from
+// device memory)
+template <int kLoss>
+__global__ __launch_bounds__(kBlock) void k_raster_fwd_view(const ViewBlock* __restrict__ vb, int v, Cut cut) {
+  const Cam cam = vb->cams[v];
+  const RasterBufs b = vb->bufs[v];
+  raster_fwd_tile<kLoss>(cam, cut, b, tile_of_block(b));
+}
+
+template <int kAccum, bool kClampColour>
+__global__ __launch_bounds__(kBlock) void k_raster_bwd_view(const ViewBlock* __restrict__ vb, int v, Cut cut) {
+  const Cam cam = vb->cams[v];
+  const RasterBufs b = vb->bufs[v];
+  raster_bwd_tile<kAccum, kClampColour>(cam, cut, b, vb->accum[v], tile_of_block(b));
+}
+
+void launch_raster_fwd_view(hipStream_t st, const ViewBlock* vb, int v, int numTiles, const Cut& cut, int loss) {
+  if (loss == 1) k_raster_fwd_view<1><<<numTiles, kBlock, 0, st>>>(vb, v, cut);
+  else if (loss == 2) k_raster_fwd_view<2><<<numTiles, kBlock, 0, st>>>(vb, v, cut);
+  else k_raster_fwd_view<0><<<numTiles, kBlock, 0, st>>>(vb, v, cut);
+}
+
+void launch_raster_bwd_view(hipStream_t st, const ViewBlock* vb, int v, int numTiles, const Cut& cut, int accumMode,
+                            bool boundedColourGrad) {
+  if (accumMode != 0) k_raster_bwd_view<1, true><<<numTiles, kBlock, 0, st>>>(vb, v, cut);
+  else if (boundedColourGrad) k_raster_bwd_view<0, false><<<numTiles, kBlock, 0, st>>>(vb, v, cut);
+  else k_raster_bwd_view<0, true><<<numTiles, kBlock, 0, st>>>(vb, v, cut);
+}
+
 void launch_raster_fwd(hipStream_t st, const Cam& cam, const Cut& cut, const RasterBufs& b, int loss) {
   const dim3 grid((unsigned)cam.numTiles);
   if (loss == 1) k_raster_fwd<1><<<grid, kBlock, 0, st>>>(cam, cut, b);

@@ -151,6 +151,9 @@ struct gs_trainer {
   int ringPos = 0;
   uint32_t* dQueue = nullptr;      // work counters of the two persistent launches
   uint32_t* viewStats = nullptr;   // kMaxViews key counts, then kMaxViews visible counts
+  hipStream_t st2 = nullptr;       // the overlapped form's raster stream
+  hipEvent_t evFork = nullptr, evJoin = nullptr;
+  hipEvent_t evBin[kMaxViews] = {};   // view k binned
   int lastViews = 0;               // views of the last raster when it was a batched step (else 0)
   int viewBlocks = 0;              // blocks of the persistent launches
   hipGraphExec_t gExec = nullptr;  // captured batched step and the shape it was captured for
@@ -770,12 +773,22 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
   const int V = vb.count;
   const int64_t n = h->n;
   const Cut cut = make_cut(h);
-  launch_fill_u32(h->st, h->dQueue, 2, 0u);
+  // Overlapped form: each view's forward and backward run on the second
+  // stream once its binning is done, beside the next views' binning; the
+  // debug and profiling passes use the persistent form (one stream, so the
+  // stages can be read and timed one by one).
+  const bool overlap = h->o.viewsLaunch == GS_VIEWS_OVERLAP && !dbg && !h->prof;
   bool bounded = true;
+  for (int k = 0; k < V; k++) bounded = bounded && charb_grad_bounded(vb.cams[k]);
+  if (overlap) {
+    HIPCK(hipEventRecord(h->evFork, h->st));
+    HIPCK(hipStreamWaitEvent(h->st2, h->evFork, 0));
+  } else {
+    launch_fill_u32(h->st, h->dQueue, 2, 0u);
+  }
   for (int k = 0; k < V; k++) {
     const gs_trainer::ViewSlot& sl = h->vslots[k];
     const Cam& cam = vb.cams[k];
-    bounded = bounded && charb_grad_bounded(cam);
     {
       Scope sc(h, "project", (double)n * (56.0 + 4.0 * h->shC + 48.0 + 16.0));
       launch_project_view(h->st, h->shDeg, h->prec, n, h->cap, h->dVB, k, cut, h->P, h->SH, sl.rec, h->depthKey,
@@ -815,11 +828,20 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
       if (dbg[k].tileRanges)
         HIPCK(hipMemcpy(dbg[k].tileRanges, sl.ranges, 8ull * cam.numTiles, hipMemcpyDeviceToHost));
     }
+    if (overlap) {
+      HIPCK(hipEventRecord(h->evBin[k], h->st));
+      HIPCK(hipStreamWaitEvent(h->st2, h->evBin[k], 0));
+      launch_raster_fwd_view(h->st2, h->dVB, k, cam.numTiles, cut, 1);
+      launch_raster_bwd_view(h->st2, h->dVB, k, cam.numTiles, cut, h->o.gradAccum, bounded);
+    }
   }
   h->lastNumTiles = vb.cams[V - 1].numTiles;
   h->lastSegPath = true;
   h->lastViews = V;
-  {
+  if (overlap) {
+    HIPCK(hipEventRecord(h->evJoin, h->st2));
+    HIPCK(hipStreamWaitEvent(h->st, h->evJoin, 0));
+  } else {
     Scope sc(h, "raster_fwd_views", 0.0);
     launch_raster_fwd_views(h->st, h->dVB, cut, 1, h->dQueue, h->viewBlocks);
   }
@@ -834,7 +856,7 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
       if (dbg[k].gC) HIPCK(hipMemcpy(dbg[k].gC, sl.gC, 16 * npx, hipMemcpyDeviceToHost));
     }
   }
-  {
+  if (!overlap) {
     Scope sc(h, "raster_bwd_views", 0.0);
     launch_raster_bwd_views(h->st, h->dVB, cut, h->o.gradAccum, bounded, h->dQueue + 1, h->viewBlocks);
   }
@@ -1003,6 +1025,7 @@ int gs_default_opts(gs_opts* o) {
   o->tileGrad = 1;
   o->viewsPerStep = 1;
   o->graph = 1;
+  o->viewsLaunch = GS_VIEWS_PERSISTENT;
   o->posLr = 0.0f;
   o->loss = GS_LOSS_CHARBONNIER;
   o->ssimLambda = 0.2f;
@@ -1030,6 +1053,7 @@ int gs_create(const gs_opts* o, gs_trainer_t** out) {
       h->o.gradAccum < 0 || h->o.gradAccum > 1 || h->o.loss < 0 || h->o.loss > 1 ||
       h->o.precision < 0 || h->o.precision > 3 ||
       h->o.viewsPerStep < 1 || h->o.viewsPerStep > kMaxViews ||
+      h->o.viewsLaunch < 0 || h->o.viewsLaunch > 1 ||
       !(h->o.eCut >= 0.0f && h->o.eCut <= 80.0f)) {
     delete h;
     return GS_E_ARG;
@@ -1050,6 +1074,18 @@ int gs_create(const gs_opts* o, gs_trainer_t** out) {
     delete h;
     return GS_E_HIP;
   }
+  // batched views: the second stream and the events that order it
+  if (h->o.viewsPerStep > 1) {
+    bool ok = hipStreamCreateWithFlags(&h->st2, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&h->evFork, hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&h->evJoin, hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; k < kMaxViews && ok; k++)
+      ok = hipEventCreateWithFlags(&h->evBin[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+      gs_destroy(h);
+      return GS_E_HIP;
+    }
+  }
   h->shDeg = h->o.shDeg;
   h->shK = h->shDeg > 0 ? (h->shDeg + 1) * (h->shDeg + 1) - 1 : 0;
   h->shC = 3 * h->shK;
@@ -1067,10 +1103,16 @@ int gs_create(const gs_opts* o, gs_trainer_t** out) {
 void gs_destroy(gs_trainer_t* h) {
   if (!h) return;
   (void)hipSetDevice(h->o.device);
-  (void)hipStreamSynchronize(h->st);
+  if (h->st) (void)hipStreamSynchronize(h->st);
+  if (h->st2) (void)hipStreamSynchronize(h->st2);
   free_all(h);
   for (auto& p : h->pending) { (void)hipEventDestroy(p.a); (void)hipEventDestroy(p.b); }
   for (auto e : h->evPool) (void)hipEventDestroy(e);
+  if (h->st2) (void)hipStreamDestroy(h->st2);
+  if (h->evFork) (void)hipEventDestroy(h->evFork);
+  if (h->evJoin) (void)hipEventDestroy(h->evJoin);
+  for (hipEvent_t e : h->evBin)
+    if (e) (void)hipEventDestroy(e);
   if (h->st) (void)hipStreamDestroy(h->st);
   delete h;
 }

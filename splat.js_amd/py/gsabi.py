@@ -18,6 +18,7 @@ GS_LOSS_CHARBONNIER, GS_LOSS_L1_DSSIM = 0, 1
 GS_ACCUM_FIXED, GS_ACCUM_F32 = 0, 1
 GS_SORT_DEPTH_THEN_TILE, GS_SORT_GLOBAL64, GS_SORT_TILE_SEGMENTS = 0, 1, 2
 GS_PREC_F32, GS_PREC_MIXED, GS_PREC_BF16, GS_PREC_FP16 = 0, 1, 2, 3
+GS_VIEWS_PERSISTENT, GS_VIEWS_OVERLAP = 0, 1
 
 
 class GsOpts(C.Structure):
@@ -29,7 +30,7 @@ class GsOpts(C.Structure):
                 ("ssimLambda", C.c_float), ("gradAccum", C.c_int32), ("sortMode", C.c_int32),
                 ("seed", C.c_uint32), ("device", C.c_int32), ("precision", C.c_int32),
                 ("camOpt", C.c_int32), ("expComp", C.c_int32), ("viewsPerStep", C.c_int32),
-                ("graph", C.c_int32)]
+                ("graph", C.c_int32), ("viewsLaunch", C.c_int32)]
 
 
 class GsCamera(C.Structure):

@@ -74,15 +74,19 @@ def test_views_bit_exact_against_restatement(precision):
     t.destroy()
 
 
-def test_graph_replay_equals_eager():
-    """Two trainers from the same start and seed, one replaying a captured
-    graph (first step eager, second captured, later ones replayed), one
-    eager; between batched steps, single-view steps, a render and readbacks
-    run on the graph trainer's buffers. State bit-identical after every step."""
+def test_graph_replay_and_launch_forms_equal_eager():
+    """Trainers from the same start and seed: graph replay (first step eager,
+    second captured, later ones replayed) with the overlapped launches (each
+    view's raster on a second stream beside the next views' binning), graph
+    replay with the persistent launches, and eager persistent launches.
+    Between batched steps, single-view steps, a render and readbacks run on
+    the same buffers. State bit-identical after every step."""
     p, sh, cams, images = _scene(n=5000, seed=3, sizes=[(128, 96)] * 4)
     tr = []
-    for graph in (1, 0):
-        t = gsabi.GSTrainer(shDeg=2, maxSplats=p.shape[0], viewsPerStep=3, graph=graph, seed=11)
+    for graph, launch in ((1, gsabi.GS_VIEWS_OVERLAP), (1, gsabi.GS_VIEWS_PERSISTENT),
+                          (0, gsabi.GS_VIEWS_PERSISTENT), (0, gsabi.GS_VIEWS_OVERLAP)):
+        t = gsabi.GSTrainer(shDeg=2, maxSplats=p.shape[0], viewsPerStep=3, graph=graph, viewsLaunch=launch,
+                            seed=11)
         t.setup({"data": p, "n": p.shape[0], "sh": sh}, cams, images, 128, 96, 2.0)
         t.iter = 2500
         tr.append(t)
@@ -93,10 +97,13 @@ def test_graph_replay_equals_eager():
             if s % 3 == 2:
                 t.stepOnce()
                 t.renderView(cams[1])
-        a, b = tr[0].readGaussians(), tr[1].readGaussians()
-        np.testing.assert_array_equal(_bits(a["data"]), _bits(b["data"]), err_msg=f"step {s}")
-        np.testing.assert_array_equal(_bits(a["sh"]), _bits(b["sh"]), err_msg=f"step {s}")
-        assert tr[0].readLoss() == pytest.approx(tr[1].readLoss(), rel=1e-12)   # double atomics
+        ref = tr[2].readGaussians()
+        loss = tr[2].readLoss()
+        for k in (0, 1, 3):
+            a = tr[k].readGaussians()
+            np.testing.assert_array_equal(_bits(a["data"]), _bits(ref["data"]), err_msg=f"trainer {k} step {s}")
+            np.testing.assert_array_equal(_bits(a["sh"]), _bits(ref["sh"]), err_msg=f"trainer {k} step {s}")
+            assert tr[k].readLoss() == pytest.approx(loss, rel=1e-12)   # double atomics
     for t in tr:
         t.destroy()
 
