@@ -359,6 +359,20 @@ int gs_step_views_debug(gs_trainer_t* h, const int32_t* cams, int32_t count, gs_
  * d log f in slot 0 of the last row. count >= (numCams + 1) * 8. */
 int gs_read_cam_grads(gs_trainer_t* h, int32_t* out, int32_t count);
 
+/* Gradient check (the reference's gradCheck, gradcheck.js:266-356): central
+ * finite differences in FP64 of the training loss of camera `cam` (the
+ * Charbonnier sum over its valid pixels, at the current parameters and SH
+ * band ramp) with respect to `count` parameters: entry k moves slot
+ * slots[k] (0..13) of Gaussian gauss[k] by +-steps[k], and out[k] =
+ * (L(+h) - L(-h)) / (2h). Runs on the GPU: the current raster's sorted keys
+ * and records for every other Gaussian, the perturbed Gaussian's record in
+ * FP64 at its key's place in the current sort order (as the analytic backward
+ * assumes), the tiles of its rectangle plus one tile of margin. A Gaussian
+ * without keys in the view (culled) gives 0. The parameters are not changed. No counterpart in the reference (whose gradCheck reruns
+ * whole forward passes); synchronises. */
+int gs_fd_grad64(gs_trainer_t* h, int32_t cam, const int64_t* gauss, const int32_t* slots, const double* steps,
+                 int32_t count, double* out);
+
 /* Per-kernel timing over subsequent steps, measured with HIP events on the
  * trainer's stream. enable = 0 stops and keeps the totals. */
 typedef struct gs_kernel_time {

@@ -35,7 +35,7 @@ HIP_FLAGS = [
 ]
 
 SOURCES = ["gs_project.hip", "gs_sort.hip", "gs_raster.hip", "gs_update.hip", "gs_loss.hip",
-           "gs_refine.hip", "gs_ply.hip", "gs_cam.hip", "gs_trainer.cpp"]
+           "gs_refine.hip", "gs_ply.hip", "gs_cam.hip", "gs_fd.hip", "gs_trainer.cpp"]
 
 
 def _run(cmd):

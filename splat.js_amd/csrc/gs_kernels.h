@@ -201,6 +201,34 @@ void launch_refine_gather(hipStream_t st, int64_t n, int64_t cap, const float4* 
 void launch_refine_apply(hipStream_t st, int deg, int prec, int64_t cap, const RefineOp* ops, int64_t count,
                          float4* P, void* M, void* V, void* SH, void* SHM, void* SHV);
 
+// ---- FP64 finite-difference gradient check (gs_fd_grad64) ----
+struct FdSample {
+  int64_t g;      // Gaussian
+  int32_t slot;   // parameter slot 0..13
+  int32_t pad;
+  double h;       // step
+  uint2 rect;     // its tile rectangle in the current raster (packed as the projection's)
+};
+
+struct FdArgs {
+  int64_t cap;
+  const float4* P;          // parameter planes
+  const void* SH;           // SH planes (storage of the trainer's precision mode)
+  const float4* rec;        // the current raster's records
+  int recFmt;
+  const uint32_t* depthKey; // f32 depth bits per Gaussian
+  const uint32_t* ids;      // sorted key values of the current raster
+  const uint2* ranges;      // per tile
+  const uint32_t* tgt;      // packed targets
+  const FdSample* samples;
+  const uint2* items;       // (sample, tile) per block
+  double* out;              // per sample: sum over pixels of L(+h) - L(-h) (zeroed by the caller)
+};
+
+// One block per item; cam is the view of the current raster.
+void launch_fd64(hipStream_t st, int deg, int shFmt, int64_t items, const Cam& cam, const Cut& cut,
+                 const FdArgs& a);
+
 // ---- PLY export rows (gs_export_ply) ----
 // out: rows first .. first + count - 1, each (17 + 3K) f32, from out[0];
 // camPos: nc x 3 f32 (bake only)

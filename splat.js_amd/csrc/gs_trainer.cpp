@@ -2015,6 +2015,83 @@ int gs_export_ply_noalloc(gs_trainer_t* h, int32_t bake, void* buf, int64_t cap,
   return GS_OK;
 }
 
+int gs_fd_grad64(gs_trainer_t* h, int32_t ci, const int64_t* gauss, const int32_t* slots, const double* steps,
+                 int32_t count, double* out) {
+  if (!h || count < 0 || (count > 0 && (!gauss || !slots || !steps || !out))) return GS_E_ARG;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  if (ci < 0 || ci >= (int)h->cams.size()) return fail(h, GS_E_ARG, "fdGrad: camera index out of range");
+  if (count == 0) return GS_OK;
+  HIPCK(hipSetDevice(h->o.device));
+  for (int k = 0; k < count; k++) {
+    if (gauss[k] < 0 || gauss[k] >= h->n || slots[k] < 0 || slots[k] > 13 || !(steps[k] > 0.0))
+      return fail(h, GS_E_ARG, "fdGrad: sample " + std::to_string(k) + " out of range");
+  }
+  // the raster of the current parameters, with the training uniform
+  CamMeta& meta = h->cams[ci];
+  const int activeDeg = (int)std::min<int64_t>(h->shDeg, h->iter / 1000);
+  const Cam cam = make_cam_meta(h, meta, activeDeg);
+  HIPCK(hipMemsetAsync(h->evalStats, 0, 32, h->st));
+  int rc = run_raster(h, cam, 1, h->evalStats);
+  if (rc) return rc;
+  // the samples' tile rectangles, one tile of margin each side
+  std::vector<uint2> rects((size_t)count);
+  for (int k = 0; k < count; k++)
+    HIPCK(hipMemcpyAsync(&rects[k], h->rect + gauss[k], 8, hipMemcpyDeviceToHost, h->st));
+  HIPCK(hipStreamSynchronize(h->st));
+  std::vector<uint2> items;
+  std::vector<FdSample> smp((size_t)count);
+  for (int k = 0; k < count; k++) {
+    smp[k] = FdSample{gauss[k], slots[k], 0, steps[k], rects[k]};
+    const int tx0 = std::max<int>(0, (int)(rects[k].x & 0xffff) - 1);
+    const int tx1 = std::min<int>(cam.tilesX - 1, (int)(rects[k].x >> 16) + 1);
+    const int ty0 = std::max<int>(0, (int)(rects[k].y & 0xffff) - 1);
+    const int ty1 = std::min<int>(cam.tilesY - 1, (int)(rects[k].y >> 16) + 1);
+    for (int ty = ty0; ty <= ty1; ty++)
+      for (int tx = tx0; tx <= tx1; tx++) items.push_back(make_uint2((uint32_t)k, (uint32_t)(ty * cam.tilesX + tx)));
+  }
+  FdSample* dS = nullptr;
+  uint2* dI = nullptr;
+  double* dO = nullptr;
+  hipError_t e = hipMalloc(&dS, smp.size() * sizeof(FdSample));
+  if (e == hipSuccess) e = hipMalloc(&dI, std::max<size_t>(items.size(), 1) * sizeof(uint2));
+  if (e == hipSuccess) e = hipMalloc(&dO, (size_t)count * 8);
+  if (e == hipSuccess) e = hipMemcpy(dS, smp.data(), smp.size() * sizeof(FdSample), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !items.empty())
+    e = hipMemcpy(dI, items.data(), items.size() * sizeof(uint2), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemsetAsync(dO, 0, (size_t)count * 8, h->st);
+  if (e == hipSuccess) {
+    FdArgs a{};
+    a.cap = h->cap;
+    a.P = h->P;
+    a.SH = h->SH;
+    a.rec = h->rec;
+    a.recFmt = h->recFmt;
+    a.depthKey = h->depthKey;
+    a.ids = h->lastBin.ids;
+    a.ranges = h->ranges;
+    a.tgt = h->targets;
+    a.samples = dS;
+    a.items = dI;
+    a.out = dO;
+    // at most 2^31 - 1 blocks per launch
+    const int64_t kChunk = int64_t(1) << 30;
+    for (int64_t i0 = 0; i0 < (int64_t)items.size() && e == hipSuccess; i0 += kChunk) {
+      a.items = dI + i0;
+      launch_fd64(h->st, h->shDeg, h->shFmt, std::min<int64_t>(kChunk, (int64_t)items.size() - i0), cam, make_cut(h), a);
+      e = hipGetLastError();
+    }
+  }
+  std::vector<double> sums((size_t)count, 0.0);
+  if (e == hipSuccess) e = hipMemcpyAsync(sums.data(), dO, (size_t)count * 8, hipMemcpyDeviceToHost, h->st);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->st);
+  if (dS) (void)hipFree(dS);
+  if (dI) (void)hipFree(dI);
+  if (dO) (void)hipFree(dO);
+  if (e != hipSuccess) return hipfail(h, "fdGrad", e);
+  for (int k = 0; k < count; k++) out[k] = sums[k] / (2.0 * steps[k]);
+  return GS_OK;
+}
+
 int gs_sync(gs_trainer_t* h) {
   if (!h) return GS_E_ARG;
   HIPCK(hipStreamSynchronize(h->st));

@@ -104,6 +104,7 @@ SIGNATURES = {
     "gs_refine": ([C.c_void_p, C.c_void_p, C.c_void_p], C.c_int),
     "gs_step_batch": ([C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
     "gs_step_views": ([C.c_void_p, C.c_void_p, C.c_int32], C.c_int),
+    "gs_fd_grad64": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
     "gs_step_views_debug": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
     "gs_export_ply": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
     "gs_export_ply_noalloc": ([C.c_void_p, C.c_int32, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
@@ -282,6 +283,16 @@ class GSTrainer:
                 o["shGrad"] = o["shGrad"][:, :3 * self.shK]
             res.append(o)
         return res
+
+    def fdGrad64(self, cam, gauss, slots, steps):
+        """Central FP64 finite differences of camera `cam`'s training loss
+        with respect to (Gaussian, slot) pairs with steps h, on the GPU."""
+        g = np.ascontiguousarray(gauss, np.int64)
+        sl = np.ascontiguousarray(slots, np.int32)
+        h = np.ascontiguousarray(steps, np.float64)
+        out = np.zeros(g.size, np.float64)
+        self._ck(lib().gs_fd_grad64(self._h, int(cam), _p(g), _p(sl), _p(h), int(g.size), _p(out)))
+        return out
 
     def readLoss(self):
         mse = C.c_double()

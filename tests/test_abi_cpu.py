@@ -17,7 +17,7 @@ HEADER = os.path.join(ROOT, "include", "splat_gs.h")
 def _declared():
     src = open(HEADER).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(gs_[a-z_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(gs_[a-z0-9_]+)\s*\(", src)))
 
 
 def test_library_exports_every_declared_symbol():

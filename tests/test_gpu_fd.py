@@ -1,0 +1,122 @@
+"""Gradient check with FP64 finite differences on the GPU (gs_fd_grad64),
+following the reference's gradCheck (gradcheck.js:266-356): relative error
+|a - fd| / max(|a|, |fd|) (0 when both are below 1e-3), median per parameter
+slot.
+
+* On the reference's gradcheck rig (160 splats, 64x64, strict cutoffs,
+  gradcheck.js:93-131), the GPU's finite differences against the
+  restatement's FP64 analytic gradients (which CPU central differences pin,
+  test_oracle.py), for every parameter of every splat: this pins the FD
+  kernel itself.
+* Config 5 at full size (3M Gaussians, SH degree 3, log-scales N(-3, 1),
+  1920x1080): the HIP analytic gradients of 256 random visible Gaussians,
+  all 14 slots, against the FP64 finite differences, with the f32-atomic
+  accumulation and with the reference's x16384 fixed-point sums.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import gsabi
+import oracle as O
+import scenes
+from helpers import rig_setup
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ["pos.x", "pos.y", "pos.z", "logSx", "logSy", "logSz", "qw", "qx", "qy", "qz", "r", "g", "b", "opacity"]
+
+
+def _hby_slot(r):
+    # trainer.js:347-350
+    return np.array([r * 2e-3] * 3 + [0.02] * 3 + [0.02] * 4 + [0.05] * 3 + [0.05], np.float64)
+
+
+def _rel(a, fd):
+    den = np.maximum(np.abs(a), np.abs(fd))
+    return np.where(den < 1e-3, 0.0, np.abs(a - fd) / np.maximum(den, 1e-300))
+
+
+def _medians(slots, rel):
+    return {NAMES[s]: float(np.median(rel[slots == s])) for s in range(14) if np.any(slots == s)}
+
+
+@pytest.mark.parametrize("deg", [0, 3])
+def test_fd_kernel_against_fp64_analytic_gradients_on_the_rig(deg):
+    params, sh, cams, images, W, H, radius, opts = rig_setup(deg)
+    t = gsabi.GSTrainer(shDeg=deg, **opts)
+    g = {"data": params, "n": params.shape[0]}
+    if sh is not None:
+        g["sh"] = sh
+    t.setup(g, cams, images, W, H, radius)
+    t.iter = 5000                       # every SH band active
+    n = params.shape[0]
+    gi = np.repeat(np.arange(n), 14)
+    sl = np.tile(np.arange(14), n)
+    h = _hby_slot(float(np.float32(radius)))[sl] / 8.0
+    fd = t.fdGrad64(0, gi, sl, h)
+    t.destroy()
+    o = O.make_opts(eCut=opts["eCut"], aMin=opts["aMin"], radClamp=opts["radClamp"], anisoReg=0.0, shDeg=deg)
+    tgt = O.pack_targets(images[0]["rgb"])
+    g64 = O.analytic_grads(O.Stage(o, "f64"), O.make_cam(cams[0], shActiveDeg=deg), params, sh, tgt)[0]
+    a = g64[gi, sl]
+    rel = _rel(a, fd)
+    med = _medians(sl, rel)
+    close = float(np.mean(rel < 1e-2))
+    print(f"rig deg {deg}: median relative error per slot {med}; {close:.3f} of {rel.size} samples within 1e-2")
+    # a step that moves a splat across a pixel's alpha or exponent cutoff
+    # (strict here, gradcheck.js:155) leaves a jump in the loss: such samples
+    # differ from the smooth gradient, so the bound is on the medians
+    assert max(med.values()) < 2e-3, med
+    assert close > 0.8
+
+
+def test_config5_gradients_against_fp64_finite_differences():
+    """As the reference's gradCheck, the samples favour informative
+    parameters: 256 random Gaussians with keys in the view and some |dL/dp|
+    >= 0.03 (gradcheck.js:315), all 14 slots each; the medians are over the
+    samples whose |a| or |fd| reaches 1e-3 (the others count as 0 there).
+    The cutoffs are the reference's gradient-check ones (eCut 9, A_MIN 1e-4,
+    radius clamp 10, gradcheck.js:155): with the training cutoffs a step of a
+    large splat moves its alpha-cutoff boundary across many pixels at once,
+    and those jumps, not the gradient, dominate the differences."""
+    p, sh, cams = scenes.config5(n=3_000_000, seed=0)
+    W, H = cams[0]["w"], cams[0]["h"]
+    rng = np.random.default_rng(0)
+    images = [{"tw": W, "th": H, "rgb": rng.uniform(0, 1, W * H * 3).astype(np.float32)}]
+    cams = [dict(cams[0], imgIdx=0)]
+    kw = dict(shDeg=3, maxSplats=p.shape[0], entriesCap=1_500_000_000, anisoReg=0.0, eCut=9.0, aMin=1e-4,
+              radClamp=10.0)
+    # f32-atomic sums: one step, then the analytic gradients at the state it left
+    t = gsabi.GSTrainer(gradAccum=gsabi.GS_ACCUM_F32, **kw)
+    t.setup({"data": p, "n": p.shape[0], "sh": sh}, cams, images, W, H, 8.0)
+    t.iter = 5000
+    cnt = t.step_debug(0, want=("cnt",))["cnt"]
+    state = t.readGaussians()
+    ga = t.step_debug(0, want=("gradF",))["gradF"][:, :14].astype(np.float64)
+    t.destroy()
+    # the same state with the reference's fixed-point sums: FP64 differences, then its gradients
+    t = gsabi.GSTrainer(gradAccum=gsabi.GS_ACCUM_FIXED, **kw)
+    t.setup({"data": state["data"], "n": p.shape[0], "sh": state["sh"]}, cams, images, W, H, 8.0)
+    t.iter = 5001
+    cand = np.nonzero((cnt > 0) & (np.abs(ga).max(axis=1) >= 0.03))[0]
+    gsel = np.sort(rng.choice(cand, 256, replace=False))
+    gi = np.repeat(gsel, 14)
+    sl = np.tile(np.arange(14), 256)
+    t0 = time.perf_counter()
+    fd = t.fdGrad64(0, gi, sl, _hby_slot(8.0)[sl])
+    dt = time.perf_counter() - t0
+    gf = t.step_debug(0, want=("gradF",))["gradF"][:, :14].astype(np.float64)
+    t.destroy()
+    print(f"config 5: {int((cnt > 0).sum())} Gaussians with keys, {cand.size} informative; {gi.size} FP64 finite "
+          f"differences in {dt:.2f}s", flush=True)
+    res = {}
+    for name, g in (("f32 atomics", ga), ("x16384 fixed point", gf)):
+        a = g[gi, sl]
+        rel = _rel(a, fd)
+        inf = np.maximum(np.abs(a), np.abs(fd)) >= 1e-3
+        res[name] = {NAMES[s]: float(np.median(rel[(sl == s) & inf])) for s in range(14) if np.any((sl == s) & inf)}
+        print(f"config 5, {name}: median relative error per slot {res[name]} "
+              f"({int(inf.sum())} informative samples)", flush=True)
+    assert max(res["f32 atomics"].values()) < 0.1, res
