@@ -252,6 +252,8 @@ def main():
     ap.add_argument("--psnr-iters", type=int, default=2000, help="0 skips the PSNR drift runs")
     ap.add_argument("--threads", type=int, default=0, help="CPU baseline threads (0: all, at most 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--config5", action="store_true",
+                    help="instead: the stress configuration (3M Gaussians, SH 3, 1920x1080, 4-view batches)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device-to-device copy (scales HBM byte counters)")
     args = ap.parse_args()
@@ -260,6 +262,9 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    if args.config5:
+        run_config5(args)
+        return
     W, H, f = args.width, args.height, 1100.0 * args.width / 1600.0
     p, sh, cams, images = build_workload(args.n, args.views, W, H, f)
     runs = {}
@@ -355,6 +360,67 @@ def main():
     print(json.dumps(line), flush=True)
     if args.copy_calibration:
         copy_calibration()
+
+
+def device_used_bytes():
+    """Bytes of device memory in use on the current GPU (hipMemGetInfo)."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    free, total = C.c_size_t(), C.c_size_t()
+    assert hip.hipMemGetInfo(C.byref(free), C.byref(total)) == 0
+    return total.value - free.value
+
+
+def run_config5(args):
+    """The stress configuration: 3M Gaussians, SH degree 3, log-scales
+    N(-3, 1), 1920x1080, batches of 4 views. The key budget holds every key
+    (the reference's default 12M would drop most tiles of these views).
+    Reports device memory in use (peak over the run, minus what was in use
+    before the trainer), keys per view, sorted keys/s and iterations (views)/s."""
+    n, W, H, V = 3_000_000, 1920, 1080, 4
+    p, sh, cams = scenes.config5(n=n, seed=0, views=V, W=W, H=H)
+    gp, gsh, _ = scenes.config5(n=n, seed=1, views=V, W=W, H=H)
+    base = device_used_bytes()
+    gt = gsabi.GSTrainer(shDeg=3, maxSplats=n, entriesCap=400_000_000)
+    blank = [{"tw": W, "th": H, "rgb": np.zeros(W * H * 3, np.float32)}]
+    gt.setup({"data": gp, "n": n, "sh": gsh}, [dict(cams[0], imgIdx=0)], blank, W, H, 8.0)
+    images = [{"tw": W, "th": H, "rgb": np.clip(gt.renderView(c)[..., :3], 0, 1).reshape(-1).copy()} for c in cams]
+    gt.destroy()
+    out = {}
+    for prec in ("f32", "bf16"):
+        t = gsabi.GSTrainer(shDeg=3, maxSplats=n, entriesCap=400_000_000, posLr=1.6e-4, seed=7,
+                            precision=PRECS[prec], viewsPerStep=V)
+        t.setup({"data": p, "n": n, "sh": sh}, cams, images, W, H, 8.0)
+        t.iter = 5000
+        peak = device_used_bytes()
+        for _ in range(2):
+            t.stepViews(list(range(V)))
+        t.sync()
+        peak = max(peak, device_used_bytes())
+        steps = max(2, args.steps // 20)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t.stepViews(list(range(V)))
+        t.sync()
+        dt = time.perf_counter() - t0
+        t.profile(True)
+        t.stepViews(list(range(V)))
+        t.sync()
+        prof = t.profile_read()
+        t.profile(False)
+        st = t.step_stats()
+        peak = max(peak, device_used_bytes())
+        t.destroy()
+        keys = st.lastKeys / V
+        bin_ms = prof["bin_sort"]["ms"] / max(prof["bin_sort"]["launches"], 1)
+        out[prec] = {"iters_per_s": round(steps * V / dt, 2), "batched_steps_per_s": round(steps / dt, 3),
+                     "keys_per_view": int(keys), "visible_per_view": int(st.lastVisible / V),
+                     "sorted_keys_per_s": round(keys / (bin_ms * 1e-3)), "bin_sort_ms": round(bin_ms, 3),
+                     "peak_device_GB": round((peak - base) / 1e9, 2),
+                     "stage_ms": {k: round(v["ms"] / max(v["launches"], 1), 3) for k, v in prof.items()}}
+        log(f"config 5 {prec}: {out[prec]}")
+    print(json.dumps({"config5": {"workload": f"{n} Gaussians SH deg 3, log-scales N(-3,1), {W}x{H}, "
+                                              f"{V}-view batches, key budget 400M", "runs": out}}), flush=True)
 
 
 def copy_ceiling(nbytes=1 << 30, reps=5):

@@ -2,8 +2,10 @@
 // ordered by (tile, depth, Gaussian id), the total order the reference's
 // per-tile bitonic sort produces (shaders.js:418-500, tie-break :454).
 //
-// Two strategies, identical output:
-//  * depth-then-tile (default): stable LSD radix sort of the n Gaussians by
+// Three strategies, identical output: tile segments (the default, below the
+// radix passes in this file), and two radix strategies:
+//  * depth-then-tile (also the fallback for views above kMaxSegTiles tiles):
+//    stable LSD radix sort of the n Gaussians by
 //    depth bits (emitted in id order, so ties keep id order), then keys are
 //    emitted in that order and stably sorted by tile id only
 //    (ceil(tileBits/8) passes). Sorting Gaussians instead of keys moves the
