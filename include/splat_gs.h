@@ -99,6 +99,12 @@ typedef struct gs_opts {
                           all views' tiles) or GS_VIEWS_OVERLAP (each view's forward and backward on
                           a second stream as soon as its binning is done, beside the binning of the
                           next views; measured slower on MI355X); identical results */
+  int32_t tile;        /* raster tile side in pixels: 16 (default; 0 means 16; the reference's
+                          TILE, shaders.js:32), 8 or 32. The tile grid decides which (tile,
+                          Gaussian) keys exist and which whole tiles the entry budget drops; the
+                          restatement at the same tile size gives the same results bit for bit */
+  int32_t batch;       /* keys staged in LDS per raster round: 256 (default; 0 means 256), 128 or
+                          64; changes no result */
 } gs_opts;
 
 /* One training camera (trainer.js:95-100 cams[], :357-372): R row-major
@@ -140,6 +146,7 @@ typedef struct gs_info {
   int64_t entriesCap;
   int32_t maxTiles;
   float horizon;
+  int32_t tile, batch; /* raster shape in use (gs_opts.tile / .batch, gs_set_raster_shape) */
 } gs_info;
 
 /* Device-side figures of the last raster (synchronises). */
@@ -372,6 +379,37 @@ int gs_read_cam_grads(gs_trainer_t* h, int32_t* out, int32_t count);
  * whole forward passes); synchronises. */
 int gs_fd_grad64(gs_trainer_t* h, int32_t cam, const int64_t* gauss, const int32_t* slots, const double* steps,
                  int32_t count, double* out);
+
+/* Raster shape: tile side 8, 16 or 32 pixels and 64, 128 or 256 keys staged
+ * in LDS per round (as gs_opts.tile and .batch), from the next step or
+ * render on. No counterpart in the reference (16-pixel tiles, shaders.js:32,
+ * and no LDS staging). */
+int gs_set_raster_shape(gs_trainer_t* h, int32_t tile, int32_t batch);
+
+/* One shape of an autotuning sweep. */
+typedef struct gs_tune_entry {
+  int32_t tile, batch;
+  int32_t status;       /* GS_TUNE_MEASURED, GS_TUNE_TOO_MANY_TILES (a view has more tiles than
+                           the batched step's tile-segment binning takes: 16384) or
+                           GS_TUNE_MORE_DROPS (the entry budget drops more keys than at
+                           16-pixel tiles: faster only by rendering less) */
+  int32_t pad;
+  double ms;            /* mean time of one pass (status GS_TUNE_MEASURED) */
+  int64_t keys;         /* sorted keys of one pass, all views */
+  int64_t dropped;      /* keys the entry budget dropped in one pass, all views */
+} gs_tune_entry;
+enum { GS_TUNE_MEASURED = 0, GS_TUNE_TOO_MANY_TILES = 1, GS_TUNE_MORE_DROPS = 2 };
+
+/* Autotuner: for each of the 9 shapes (tile-major: tiles 8, 16, 32, batches
+ * 64, 128, 256 within each), times `reps` passes (after one warm-up pass) of
+ * the training raster of cameras cams[0..count): projection, binning, the
+ * forward pass with the loss and the backward pass, as gs_step (count 1) or
+ * gs_step_views (count > 1, at most opts.viewsPerStep) run them, without the
+ * update. Parameters, optimizer state, iteration count and loss window are
+ * left as they were. Writes the 9 entries to out and sets the fastest
+ * measured shape (the current one if none is). No counterpart in the
+ * reference; synchronises. */
+int gs_autotune(gs_trainer_t* h, const int32_t* cams, int32_t count, int32_t reps, gs_tune_entry* out);
 
 /* Per-kernel timing over subsequent steps, measured with HIP events on the
  * trainer's stream. enable = 0 stops and keeps the totals. */

@@ -38,7 +38,12 @@ typedef struct {
   float bg[3];
   int shActiveDeg;     /* runtime SH degree (ramps one band per 1000 iters) */
   float camPos[3];     /* -(R^T t), see orc_cam_pos */
+  int tile;            /* tile side in pixels (8, 16 or 32); 0 means the reference's 16
+                          (shaders.js:32) */
 } orc_cam;
+
+/* the camera's tile side */
+static inline int orc_tile(const orc_cam* cam) { return cam->tile > 0 ? cam->tile : 16; }
 
 typedef struct {
   float eCut, aMin, radClamp, anisoReg;

@@ -159,8 +159,9 @@ void FN(orc_project)(const orc_opts* o, const orc_cam* cam, int64_t n,
                      REAL* proj, int32_t* rect, uint32_t* cnt) {
   const int K = o->shDeg > 0 ? (o->shDeg + 1) * (o->shDeg + 1) - 1 : 0;
   const REAL E_CUT = o->eCut, A_MIN = o->aMin, RADCL = o->radClamp;
-  const int tilesX = (cam->W + 15) / 16;
-  const int tilesY = (cam->H + 15) / 16;
+  const int TS = orc_tile(cam);
+  const int tilesX = (cam->W + TS - 1) / TS;
+  const int tilesY = (cam->H + TS - 1) / TS;
   const REAL W = (REAL)cam->W, H = (REAL)cam->H;
 #pragma omp parallel for schedule(static) if (orc_threads > 1)
   for (int64_t i = 0; i < n; i++) {
@@ -212,7 +213,7 @@ void FN(orc_project)(const orc_opts* o, const orc_cam* cam, int64_t n,
     pr[8] = col[0]; pr[9] = col[1]; pr[10] = col[2];
     pr[11] = 1;
     pr[12] = g.va; pr[13] = g.vb; pr[14] = g.vc; pr[15] = rad;
-    const REAL TF = 16;
+    const REAL TF = (REAL)TS;   /* a power of two: the divisions are exact */
     const int tx0 = (int)FN(clampr)(FLOOR((mx - rad) / TF), RL(0.0), (REAL)(tilesX - 1));
     const int tx1 = (int)FN(clampr)(FLOOR((mx + rad) / TF), RL(0.0), (REAL)(tilesX - 1));
     const int ty0 = (int)FN(clampr)(FLOOR((my - rad) / TF), RL(0.0), (REAL)(tilesY - 1));
@@ -272,11 +273,12 @@ void FN(orc_render_fwd)(const orc_opts* o, const orc_cam* cam, const REAL* proj,
                         const uint32_t* ids, const uint32_t* tile_start,
                         REAL* img, REAL* Tfin, uint32_t* endOut) {
   const REAL E_CUT = o->eCut, A_MIN = o->aMin;
-  const int W = cam->W, H = cam->H, tilesX = (W + 15) / 16;
+  const int TS = orc_tile(cam);
+  const int W = cam->W, H = cam->H, tilesX = (W + TS - 1) / TS;
 #pragma omp parallel for collapse(2) schedule(dynamic, 64) if (orc_threads > 1)
   for (int y = 0; y < H; y++)
     for (int x = 0; x < W; x++) {
-      const int tile = (y / 16) * tilesX + (x / 16);
+      const int tile = (y / TS) * tilesX + (x / TS);
       const uint32_t segS = tile_start[tile], segE = tile_start[tile + 1];
       const REAL px = (REAL)x + RL(0.5), py = (REAL)y + RL(0.5);
       REAL T = 1, C[3] = {0, 0, 0};
@@ -345,14 +347,15 @@ void FN(orc_render_bwd)(const orc_opts* o, const orc_cam* cam, const REAL* proj,
                         const REAL* gCin, const uint8_t* valid,
                         int32_t* gfix, double* gsum) {
   const REAL E_CUT = o->eCut, A_MIN = o->aMin;
-  const int W = cam->W, H = cam->H, tilesX = (W + 15) / 16;
+  const int TS = orc_tile(cam);
+  const int W = cam->W, H = cam->H, tilesX = (W + TS - 1) / TS;
   const int par = orc_threads > 1;
 #pragma omp parallel for collapse(2) schedule(dynamic, 64) if (par)
   for (int y = 0; y < H; y++)
     for (int x = 0; x < W; x++) {
       const int64_t pi = (int64_t)y * W + x;
       if (!valid[pi]) continue;
-      const int tile = (y / 16) * tilesX + (x / 16);
+      const int tile = (y / TS) * tilesX + (x / TS);
       const uint32_t segS = tile_start[tile];
       const REAL px = (REAL)x + RL(0.5), py = (REAL)y + RL(0.5);
       const REAL gC[3] = {gCin[3 * pi], gCin[3 * pi + 1], gCin[3 * pi + 2]};

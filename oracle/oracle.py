@@ -28,7 +28,8 @@ class OrcCam(C.Structure):
     _fields_ = [("R", C.c_float * 9), ("t", C.c_float * 3), ("nearp", C.c_float),
                 ("f", C.c_float), ("cx", C.c_float), ("cy", C.c_float),
                 ("gain", C.c_float), ("bias", C.c_float), ("W", C.c_int), ("H", C.c_int),
-                ("bg", C.c_float * 3), ("shActiveDeg", C.c_int), ("camPos", C.c_float * 3)]
+                ("bg", C.c_float * 3), ("shActiveDeg", C.c_int), ("camPos", C.c_float * 3),
+                ("tile", C.c_int)]
 
 
 class OrcOpts(C.Structure):
@@ -99,8 +100,9 @@ def _p(a):
     return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
-def make_cam(cam, shActiveDeg=0, bg=(0.0, 0.0, 0.0)):
-    """cam: dict with R (9), t (3), f, cx, cy, w, h, optional g (log gain), b."""
+def make_cam(cam, shActiveDeg=0, bg=(0.0, 0.0, 0.0), tile=16):
+    """cam: dict with R (9), t (3), f, cx, cy, w, h, optional g (log gain), b.
+    tile: tile side in pixels (the reference's 16, or 8 or 32)."""
     c = OrcCam()
     for k in range(9):
         c.R[k] = float(cam["R"][k])
@@ -113,6 +115,7 @@ def make_cam(cam, shActiveDeg=0, bg=(0.0, 0.0, 0.0)):
     c.bias = float(cam.get("b", 0.0))
     c.W, c.H = int(cam["w"]), int(cam["h"])
     c.shActiveDeg = int(shActiveDeg)
+    c.tile = int(tile)
     lib().orc_cam_pos(C.byref(c))
     return c
 
@@ -152,8 +155,9 @@ class Stage:
 
     def bin_sort(self, cam, proj, rect, cnt):
         n = proj.shape[0]
-        tilesX = (cam.W + 15) // 16
-        numTiles = tilesX * ((cam.H + 15) // 16)
+        T = cam.tile or 16
+        tilesX = (cam.W + T - 1) // T
+        numTiles = tilesX * ((cam.H + T - 1) // T)
         K = int(self.L.orc_count_keys(n, _p(cnt)))
         ids = np.zeros(max(K, 1), np.uint32)
         keys = np.zeros(max(K, 1), np.uint64)

@@ -55,6 +55,8 @@ struct Api {
   int (*refine)(gs_trainer_t*, const gs_refine_opts*, gs_refine_result*);
   int (*export_ply)(gs_trainer_t*, int32_t, void*, int64_t, int64_t*);
   const char* (*last_error)(gs_trainer_t*);
+  int (*set_raster_shape)(gs_trainer_t*, int32_t, int32_t);
+  int (*autotune)(gs_trainer_t*, const int32_t*, int32_t, int32_t, gs_tune_entry*);
 };
 
 Api g_api;
@@ -89,6 +91,8 @@ bool load_api() {
   SYM(setup, "gs_setup");
   SYM(step, "gs_step");
   SYM(step_views, "gs_step_views");
+  SYM(set_raster_shape, "gs_set_raster_shape");
+  SYM(autotune, "gs_autotune");
   SYM(read_loss, "gs_read_loss");
   SYM(render, "gs_render");
   SYM(read_gaussians, "gs_read_gaussians");
@@ -282,6 +286,8 @@ napi_value Create(napi_env env, napi_callback_info info) {
       o.viewsPerStep = (int32_t)get_num(env, a, "viewsPerStep", o.viewsPerStep);
       o.graph = get_num(env, a, "graph", o.graph) != 0.0 ? 1 : 0;
       o.viewsLaunch = (int32_t)get_num(env, a, "viewsLaunch", o.viewsLaunch);
+      o.tile = (int32_t)get_num(env, a, "tile", o.tile);
+      o.batch = (int32_t)get_num(env, a, "batch", o.batch);
     }
   }
   gs_trainer_t* h = nullptr;
@@ -398,6 +404,62 @@ napi_value StepViews(napi_env env, napi_callback_info info) {
   }
   check(env, h, g_api.step_views(h, cams.data(), (int32_t)len));
   return nullptr;
+}
+
+// setRasterShape(h, tile, batch)
+napi_value SetRasterShape(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  gs_trainer_t* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  int32_t tile = 0, batch = 0;
+  if (argc < 3 || napi_get_value_int32(env, argv[1], &tile) != napi_ok ||
+      napi_get_value_int32(env, argv[2], &batch) != napi_ok)
+    return throw_err(env, "setRasterShape: tile, batch");
+  check(env, h, g_api.set_raster_shape(h, tile, batch));
+  return nullptr;
+}
+
+// autotune(h, cams[], reps) -> [{tile, batch, status, ms, keys, dropped}] (9 shapes)
+napi_value Autotune(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  gs_trainer_t* h = get_handle(env, argv[0]);
+  if (!h) return nullptr;
+  uint32_t len = 0;
+  int32_t reps = 5;
+  if (argc < 2 || napi_get_array_length(env, argv[1], &len) != napi_ok) return throw_err(env, "autotune: cams array");
+  if (argc >= 3) napi_get_value_int32(env, argv[2], &reps);
+  std::vector<int32_t> cams(len);
+  for (uint32_t k = 0; k < len; k++) {
+    napi_value e;
+    napi_get_element(env, argv[1], k, &e);
+    napi_get_value_int32(env, e, &cams[k]);
+  }
+  gs_tune_entry out[9];
+  if (!check(env, h, g_api.autotune(h, cams.data(), (int32_t)len, reps, out))) return nullptr;
+  napi_value arr;
+  napi_create_array_with_length(env, 9, &arr);
+  for (uint32_t i = 0; i < 9; i++) {
+    napi_value o, v;
+    napi_create_object(env, &o);
+    napi_create_int32(env, out[i].tile, &v);
+    napi_set_named_property(env, o, "tile", v);
+    napi_create_int32(env, out[i].batch, &v);
+    napi_set_named_property(env, o, "batch", v);
+    napi_create_int32(env, out[i].status, &v);
+    napi_set_named_property(env, o, "status", v);
+    napi_create_double(env, out[i].ms, &v);
+    napi_set_named_property(env, o, "ms", v);
+    napi_create_double(env, (double)out[i].keys, &v);
+    napi_set_named_property(env, o, "keys", v);
+    napi_create_double(env, (double)out[i].dropped, &v);
+    napi_set_named_property(env, o, "dropped", v);
+    napi_set_element(env, arr, i, o);
+  }
+  return arr;
 }
 
 // readLoss(h) -> number | null
@@ -805,7 +867,7 @@ napi_value Init(napi_env env, napi_value exports) {
     napi_callback cb;
   } fns[] = {
       {"create", Create},           {"setup", Setup},         {"step", Step},
-      {"stepViews", StepViews},
+      {"stepViews", StepViews},     {"setRasterShape", SetRasterShape}, {"autotune", Autotune},
       {"readLoss", ReadLoss},       {"render", Render},       {"readGaussians", ReadGaussians},
       {"evalCamPsnr", EvalCamPsnr}, {"setHoldout", SetHoldout}, {"setExcluded", SetExcluded},
       {"setIter", SetIter},         {"info", Info},           {"sync", Sync},

@@ -82,21 +82,24 @@ GS_HD gs_cover_splat gs_cover_prepare(float mx, float my, float A, float B, floa
   return s;
 }
 
-// Bit w (w = 0..3: column w & 1, row w >> 1 of 8x8 blocks) set when some
-// pixel centre of that block of 16x16 tile (tx, ty) may take the splat.
-// The same test as gs_cover_rect on each block, arranged so the four blocks
-// share their edge lines: u[0..3] = x offsets of pixel-centre columns 0, 7,
-// 8, 15, w[0..3] likewise for rows; block (i, j) spans u[2i..2i+1] x
+// Bit w (w = 0..3: column w & 1, row w >> 1 of the tile's 2x2 quadrants)
+// set when some pixel centre of that quadrant of the T x T tile (tx, ty)
+// may take the splat (T = 8, 16 or 32; quadrants of T/2 pixels; at T = 16
+// they are the 8x8 blocks the rasterizer's waves own). The same test as
+// gs_cover_rect on each quadrant, arranged so the four share their edge
+// lines: u[0..3] = x offsets of pixel-centre columns 0, h - 1, h, 2h - 1
+// (h = T/2), w[0..3] likewise for rows; quadrant (i, j) spans u[2i..2i+1] x
 // w[2j..2j+1]. A vertical edge line u is evaluated at the clamped vertex
 // kC*u of each block row, q = hA u^2 + wc (hC wc + B u), and symmetrically
 // for horizontal lines; the f32 error of this form is also a few ulp of S.
 // Branch-free, so the whole test is one basic block on the device (a
 // conic that is not positive definite keeps every block: kA = kC = 0 there
 // and the mask computed for it is discarded).
-GS_HD uint32_t gs_cover_tile_mask(const gs_cover_splat* s, int tx, int ty) {
-  const float x0 = (float)(tx * 16) + 0.5f, y0 = (float)(ty * 16) + 0.5f;
-  const float u[4] = {x0 - s->mx, (x0 + 7.0f) - s->mx, (x0 + 8.0f) - s->mx, (x0 + 15.0f) - s->mx};
-  const float w[4] = {y0 - s->my, (y0 + 7.0f) - s->my, (y0 + 8.0f) - s->my, (y0 + 15.0f) - s->my};
+GS_HD uint32_t gs_cover_tile_mask(const gs_cover_splat* s, int tx, int ty, int T) {
+  const float x0 = (float)(tx * T) + 0.5f, y0 = (float)(ty * T) + 0.5f;
+  const float h1 = (float)(T / 2 - 1), h2 = (float)(T / 2), h3 = (float)(T - 1);
+  const float u[4] = {x0 - s->mx, (x0 + h1) - s->mx, (x0 + h2) - s->mx, (x0 + h3) - s->mx};
+  const float w[4] = {y0 - s->my, (y0 + h1) - s->my, (y0 + h2) - s->my, (y0 + h3) - s->my};
   float qv[4][2], qh[4][2];   // [line][block row / column]
   for (int a = 0; a < 4; a++) {
     const float au = s->hA * u[a] * u[a], bu = s->B * u[a], tu = s->kC * u[a];

@@ -14,7 +14,13 @@
 
 namespace gs {
 
-constexpr int kTile = 16;
+// Raster tile: kDefaultTile x kDefaultTile pixels (the reference's TILE,
+// shaders.js:32, and its 16 x 16 raster workgroups, :544) and kDefaultBatch
+// keys staged in LDS per round (the reference reads each splat from global
+// memory); a trainer may choose tiles of 8, 16 or 32 and batches of 64, 128
+// or 256 instead.
+constexpr int kDefaultTile = 16;
+constexpr int kDefaultBatch = 256;
 constexpr int kBlock = 256;
 constexpr float kFixed = 16384.0f;   // shaders.js:35-39 fixed-point scale
 
@@ -28,6 +34,7 @@ struct Cam {
   float bg[3];
   float camPos[3];
   int W, H, tilesX, tilesY, numTiles;
+  int tile;                 // tile side in pixels (8, 16 or 32)
   int shActiveK;            // active SH rest coefficients (<= K)
   int64_t tgtOffset;        // first target pixel of this view
 };

@@ -161,8 +161,10 @@ __device__ __forceinline__ double charb64(const Cam& cam, const double* Cc, doub
   return L;
 }
 
-// Work item: (sample, tile). Block = the tile's 256 pixels; it adds the
-// sum over its pixels of L(+h) - L(-h) to out[sample].
+// Work item: (sample, tile). Block = 256 threads over the tile's pixels (at
+// 8-pixel tiles 64 of them have a pixel; at 32-pixel tiles each thread takes
+// four in turn); it adds the sum over the tile's pixels of L(+h) - L(-h) to
+// out[sample].
 template <int kDeg, int kSh>
 __global__ __launch_bounds__(kBlock) void k_fd64(Cam cam, Cut cut, FdArgs a) {
   constexpr int NC = ShDims<kDeg>::C;
@@ -192,60 +194,67 @@ __global__ __launch_bounds__(kBlock) void k_fd64(Cam cam, Cut cut, FdArgs a) {
   }
   __syncthreads();
   const Rec64 gp = sG[0], gm = sG[1];
+  const int T = cam.tile;
   const int tx = tile % cam.tilesX, ty = tile / cam.tilesX;
-  const int x = tx * kTile + (tid & 15), y = ty * kTile + (tid >> 4);
-  // (uses smp.rect: the Gaussian's tile rectangle in the current raster)
-  const bool inside = x < cam.W && y < cam.H;
-  const int64_t pi = (int64_t)y * cam.W + x;
-  const uint32_t pk = inside ? a.tgt[cam.tgtOffset + pi] : 0u;
-  const bool valid = inside && (pk >> 24) != 0u;
-  const double px = (double)x + 0.5, py = (double)y + 0.5;
-  double Tp = 1.0, Tm = 1.0, Cp[3] = {0.0, 0.0, 0.0}, Cm[3] = {0.0, 0.0, 0.0};
-  bool dp = !valid, dm = !valid, ip = !gp.ok, im = !gm.ok;   // done, g inserted
   const uint2 rg = a.ranges[tile];
   const uint32_t gid = (uint32_t)smp.g;
+  // (uses smp.rect: the Gaussian's tile rectangle in the current raster)
   const bool keyed = tx >= (int)(smp.rect.x & 0xffffu) && tx <= (int)(smp.rect.x >> 16) &&
                      ty >= (int)(smp.rect.y & 0xffffu) && ty <= (int)(smp.rect.y >> 16);
-  for (uint32_t base = rg.x; base < rg.y; base += kBlock) {
-    if (__syncthreads_count(!(dp && dm)) == 0) break;
-    const uint32_t k = base + tid;
-    if (k < rg.y) {
-      const uint32_t id = a.ids[k] & kIdMask;
-      const float4* rr = a.rec + (int64_t)rec_stride(a.recFmt) * id;
-      const float4 r0 = rr[0], r1 = rr[1];
-      float4 c;
-      if (a.recFmt == 0) {
-        c = rr[2];
-      } else {
-        const uint32_t cg = __float_as_uint(r1.z), cb = __float_as_uint(r1.w);
-        c = a.recFmt == 2 ? make_float4(dec16<2>(cg), dec16<2>(cg >> 16), dec16<2>(cb), 0.0f)
-                          : make_float4(dec16<1>(cg), dec16<1>(cg >> 16), dec16<1>(cb), 0.0f);
+  double d = 0.0;
+  // the tile's T x T pixels, kBlock at a time (pixel q: column q % T, row q / T)
+  for (int q0 = 0; q0 < T * T; q0 += kBlock) {
+    const int q = q0 + tid;
+    const int x = tx * T + q % T, y = ty * T + q / T;
+    const bool inside = q < T * T && x < cam.W && y < cam.H;
+    const int64_t pi = (int64_t)y * cam.W + x;
+    const uint32_t pk = inside ? a.tgt[cam.tgtOffset + pi] : 0u;
+    const bool valid = inside && (pk >> 24) != 0u;
+    const double px = (double)x + 0.5, py = (double)y + 0.5;
+    double Tp = 1.0, Tm = 1.0, Cp[3] = {0.0, 0.0, 0.0}, Cm[3] = {0.0, 0.0, 0.0};
+    bool dp = !valid, dm = !valid, ip = !gp.ok, im = !gm.ok;   // done, g inserted
+    for (uint32_t base = rg.x; base < rg.y; base += kBlock) {
+      if (__syncthreads_count(!(dp && dm)) == 0) break;
+      const uint32_t k = base + tid;
+      if (k < rg.y) {
+        const uint32_t id = a.ids[k] & kIdMask;
+        const float4* rr = a.rec + (int64_t)rec_stride(a.recFmt) * id;
+        const float4 r0 = rr[0], r1 = rr[1];
+        float4 c;
+        if (a.recFmt == 0) {
+          c = rr[2];
+        } else {
+          const uint32_t cg = __float_as_uint(r1.z), cb = __float_as_uint(r1.w);
+          c = a.recFmt == 2 ? make_float4(dec16<2>(cg), dec16<2>(cg >> 16), dec16<2>(cb), 0.0f)
+                            : make_float4(dec16<1>(cg), dec16<1>(cg >> 16), dec16<1>(cb), 0.0f);
+        }
+        sA[tid] = r0;
+        sB[tid] = make_float4(r1.x, r1.y, __uint_as_float(a.depthKey[id]), __uint_as_float(id));
+        sC[tid] = c;
       }
-      sA[tid] = r0;
-      sB[tid] = make_float4(r1.x, r1.y, __uint_as_float(a.depthKey[id]), __uint_as_float(id));
-      sC[tid] = c;
-    }
-    __syncthreads();
-    const int cnt = (int)min((uint32_t)kBlock, rg.y - base);
-    for (int j = 0; j < cnt && !(dp && dm); j++) {
-      const float4 r0 = sA[j], r1 = sB[j], c = sC[j];
-      const uint32_t id = __float_as_uint(r1.w);
-      const double dk = (double)r1.z;
-      if (id == gid) {   // the perturbed Gaussian at its key's place
-        if (!ip) { blend_rec(cut, px, py, gp, Tp, Cp, dp); ip = true; }
-        if (!im) { blend_rec(cut, px, py, gm, Tm, Cm, dm); im = true; }
-        continue;
+      __syncthreads();
+      const int cnt = (int)min((uint32_t)kBlock, rg.y - base);
+      for (int j = 0; j < cnt && !(dp && dm); j++) {
+        const float4 r0 = sA[j], r1 = sB[j], c = sC[j];
+        const uint32_t id = __float_as_uint(r1.w);
+        const double dk = (double)r1.z;
+        if (id == gid) {   // the perturbed Gaussian at its key's place
+          if (!ip) { blend_rec(cut, px, py, gp, Tp, Cp, dp); ip = true; }
+          if (!im) { blend_rec(cut, px, py, gm, Tm, Cm, dm); im = true; }
+          continue;
+        }
+        // in a margin tile: in front of the first key behind it in (depth, id) order
+        if (!keyed && !ip && (gp.depth < dk || (gp.depth == dk && gid < id))) { blend_rec(cut, px, py, gp, Tp, Cp, dp); ip = true; }
+        if (!keyed && !im && (gm.depth < dk || (gm.depth == dk && gid < id))) { blend_rec(cut, px, py, gm, Tm, Cm, dm); im = true; }
+        blend64(cut, px, py, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, c.x, c.y, c.z, Tp, Cp, dp);
+        blend64(cut, px, py, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, c.x, c.y, c.z, Tm, Cm, dm);
       }
-      // in a margin tile: in front of the first key behind it in (depth, id) order
-      if (!keyed && !ip && (gp.depth < dk || (gp.depth == dk && gid < id))) { blend_rec(cut, px, py, gp, Tp, Cp, dp); ip = true; }
-      if (!keyed && !im && (gm.depth < dk || (gm.depth == dk && gid < id))) { blend_rec(cut, px, py, gm, Tm, Cm, dm); im = true; }
-      blend64(cut, px, py, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, c.x, c.y, c.z, Tp, Cp, dp);
-      blend64(cut, px, py, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, c.x, c.y, c.z, Tm, Cm, dm);
     }
+    if (!ip) blend_rec(cut, px, py, gp, Tp, Cp, dp);
+    if (!im) blend_rec(cut, px, py, gm, Tm, Cm, dm);
+    if (valid) d += charb64(cam, Cp, Tp, pk) - charb64(cam, Cm, Tm, pk);
+    __syncthreads();   // the next pixel group restages the records
   }
-  if (!ip) blend_rec(cut, px, py, gp, Tp, Cp, dp);
-  if (!im) blend_rec(cut, px, py, gm, Tm, Cm, dm);
-  double d = valid ? charb64(cam, Cp, Tp, pk) - charb64(cam, Cm, Tm, pk) : 0.0;
   for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o);
   if ((tid & 63) == 0) sSum[tid >> 6] = d;
   __syncthreads();

@@ -51,6 +51,7 @@ int64_t seg_group(int64_t n);
 struct BinArgs {
   int mode;               // 0 depth-then-tile, 1 global64, 2 tile segments
   int tilesX, numTiles;
+  int tile;               // tile size in pixels (8, 16 or 32)
   int64_t n;              // Gaussians
   uint64_t cap;           // key budget
   const uint32_t* dN;     // device: number of Gaussians
@@ -131,14 +132,23 @@ void launch_project(hipStream_t st, int deg, int prec, int64_t n, int64_t cap, c
 int tile_bits(int numTiles);
 void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out);
 
+// Shape of the raster launches: tile side in pixels (8, 16 or 32; the
+// camera's tile grid, Cam::tile, must use the same) and keys staged in LDS
+// per round (64, 128 or 256).
+struct RasterShape {
+  int tile, batch;
+};
+bool raster_shape_ok(int tile, int batch);
+
 // loss: 0 none (view render), 1 Charbonnier fused into the forward pass,
 // 2 forward only, loss computed by a separate kernel
-void launch_raster_fwd(hipStream_t st, const Cam& cam, const Cut& cut, const RasterBufs& b, int loss);
+void launch_raster_fwd(hipStream_t st, const RasterShape& sh, const Cam& cam, const Cut& cut, const RasterBufs& b,
+                       int loss);
 void launch_loss_l1ssim(hipStream_t st, const Cam& cam, float lambda, const RasterBufs& b,
                         float* scratch);
 // boundedColourGrad: every |dL/dC| * 16384 is far below 1e9 (the
 // Charbonnier loss), so the colour terms skip the fixed-point clamp
-void launch_raster_bwd(hipStream_t st, const Cam& cam, const Cut& cut, const RasterBufs& b,
+void launch_raster_bwd(hipStream_t st, const RasterShape& sh, const Cam& cam, const Cut& cut, const RasterBufs& b,
                        int accumMode, void* accum, bool boundedColourGrad);
 
 void launch_update(hipStream_t st, int deg, int accumMode, int prec, const Cam& cam, const AdamHP& hp,
@@ -152,16 +162,17 @@ void launch_update(hipStream_t st, int deg, int accumMode, int prec, const Cam& 
 void launch_project_view(hipStream_t st, int deg, int prec, int64_t n, int64_t cap, const ViewBlock* vb, int v,
                          const Cut& cut, const float4* P, const void* SH, float4* rec, uint32_t* depthKey,
                          uint2* rect, uint32_t* cnt, uint32_t* chunkKeys);
-void launch_raster_fwd_views(hipStream_t st, const ViewBlock* vb, const Cut& cut, int loss, uint32_t* queue,
-                             int blocks);
-void launch_raster_bwd_views(hipStream_t st, const ViewBlock* vb, const Cut& cut, int accumMode,
-                             bool boundedColourGrad, uint32_t* queue, int blocks);
-int raster_views_blocks(int device);
+void launch_raster_fwd_views(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, const Cut& cut, int loss,
+                             uint32_t* queue, int blocks);
+void launch_raster_bwd_views(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, const Cut& cut,
+                             int accumMode, bool boundedColourGrad, uint32_t* queue, int blocks);
+int raster_views_blocks(int device, const RasterShape& sh);
 // view v alone (numTiles blocks, camera and buffers from vb), for the
 // overlapped form of the batched step
-void launch_raster_fwd_view(hipStream_t st, const ViewBlock* vb, int v, int numTiles, const Cut& cut, int loss);
-void launch_raster_bwd_view(hipStream_t st, const ViewBlock* vb, int v, int numTiles, const Cut& cut, int accumMode,
-                            bool boundedColourGrad);
+void launch_raster_fwd_view(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, int v, int numTiles,
+                            const Cut& cut, int loss);
+void launch_raster_bwd_view(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, int v, int numTiles,
+                            const Cut& cut, int accumMode, bool boundedColourGrad);
 void launch_update_views(hipStream_t st, int deg, int accumMode, int prec, const ViewBlock* vb,
                          const UpdateBufs& u);
 

@@ -28,8 +28,13 @@
 
 namespace gs {
 
+// The loss kernels' own image tiling (independent of the raster's tiles):
+// 16 x 16 pixels per block, with a 5-pixel halo for the 11-tap windows.
+constexpr int kTile = 16;
 constexpr int kHalo = 5;
 constexpr int kReg = kTile + 2 * kHalo;   // 26
+
+__device__ __forceinline__ int ssim_tiles_x(const Cam& cam) { return (cam.W + kTile - 1) / kTile; }
 
 struct SsimWin {
   float g[11];
@@ -65,7 +70,7 @@ __global__ __launch_bounds__(kBlock) void k_ssim_fwd(Cam cam, SsimWin win, float
   __shared__ double sStat[3][kBlock / 64];
   const int tid = threadIdx.x;
   const int tile = blockIdx.x;
-  const int tx = tile % cam.tilesX, ty = tile / cam.tilesX;
+  const int tx = tile % ssim_tiles_x(cam), ty = tile / ssim_tiles_x(cam);
   const int ox = tx * kTile - kHalo, oy = ty * kTile - kHalo;
   for (int k = tid; k < kReg * kReg; k += kBlock) {
     const int r = k / kReg, c = k % kReg;
@@ -152,7 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_ssim_bwd(Cam cam, SsimWin win, float
   __shared__ float sH[9][kReg][kTile];
   const int tid = threadIdx.x;
   const int tile = blockIdx.x;
-  const int tx = tile % cam.tilesX, ty = tile / cam.tilesX;
+  const int tx = tile % ssim_tiles_x(cam), ty = tile / ssim_tiles_x(cam);
   const int ox = tx * kTile - kHalo, oy = ty * kTile - kHalo;
   const int64_t npx = (int64_t)cam.W * cam.H;
   for (int k = tid; k < kReg * kReg; k += kBlock) {
@@ -199,7 +204,7 @@ void launch_loss_l1ssim(hipStream_t st, const Cam& cam, float lambda, const Rast
   double g[11], s = 0.0;
   for (int i = 0; i < 11; i++) { g[i] = std::exp(-((i - 5) * (i - 5)) / (2.0 * 1.5 * 1.5)); s += g[i]; }
   for (int i = 0; i < 11; i++) w.g[i] = (float)(g[i] / s);
-  const dim3 grid((unsigned)cam.numTiles);
+  const dim3 grid((unsigned)(((cam.W + kTile - 1) / kTile) * ((cam.H + kTile - 1) / kTile)));
   k_ssim_fwd<<<grid, kBlock, 0, st>>>(cam, w, lambda, b, scratch);
   k_ssim_bwd<<<grid, kBlock, 0, st>>>(cam, w, lambda, b, scratch);
 }

@@ -92,7 +92,7 @@ __device__ __forceinline__ uint32_t project_one(int64_t n, int64_t cap, const Ca
       rec[3 * i + 2] = make_float4(comp, -0.5f * cnorm, 1.0f - opa, opa);
     }
     key = gs_f2u(g.pc[2]);
-    const float TF = 16.0f;
+    const float TF = (float)cam.tile;   // a power of two: the divisions are exact
     const int tx0 = (int)clampf(floorf((mx - rad) / TF), 0.0f, (float)(cam.tilesX - 1));
     const int tx1 = (int)clampf(floorf((mx + rad) / TF), 0.0f, (float)(cam.tilesX - 1));
     const int ty0 = (int)clampf(floorf((my - rad) / TF), 0.0f, (float)(cam.tilesY - 1));

@@ -1,14 +1,20 @@
 // gs_raster.hip — per-tile alpha compositing, forward and backward
 // (the reference's `render` pass, shaders.js:511-702).
 //
-// One 256-thread workgroup (4 waves) per 16x16 tile, one thread per pixel,
-// each wave an 8x8 block. The tile's sorted keys are consumed in batches of
-// 256: every thread loads one raster record into LDS, then every pixel walks
-// the batch from LDS (broadcast reads). Each key carries the mask of the
-// tile's 8x8 blocks whose pixels may take the splat (gs_cover.h, set by the
-// binning pass): a wave visits only the splats of its bit, a splat with an
-// empty mask is not even loaded, and a wave leaves a splat early once no
-// lane is inside the cutoff or takes it. The forward stops the whole tile once
+// One workgroup per tile, one thread per pixel, each wave an 8x8 block: by
+// default 16x16 tiles (256 threads, 4 waves) and batches of 256 keys; tiles
+// of 8 or 32 pixels and batches of 64 or 128 are the other shapes (Shape
+// below). The tile's sorted keys are consumed in batches: the waves load
+// the batch's raster records into LDS (lane l of a wave one record of each
+// 64-key chunk it stages), then every pixel walks the batch from LDS
+// (broadcast reads). Each key carries the mask of the tile's quadrants whose
+// pixels may take the splat (gs_cover.h, set by the binning pass): a wave
+// visits only the splats of its quadrant's bit, a splat with an empty mask
+// is not even loaded, and a wave leaves a splat early once no lane is inside
+// the cutoff or takes it. The batch size changes no result (each pixel still
+// walks its keys in order); the tile size changes which keys exist (the
+// tile rectangles and the entry budget are per tile), as in the restatement
+// run at the same tile size. The forward stops the whole tile once
 // every pixel has reached T < 1e-4. Both passes evaluate alpha with
 // gs_expf_raster (gs_math.h), as the restatement's raster passes do.
 //
@@ -39,28 +45,61 @@ namespace gs {
 
 __device__ __forceinline__ int tile_of_block(const RasterBufs& b) { return (int)b.tileOrder[blockIdx.x]; }
 
-// Each wave owns an 8x8 pixel block of the tile (wave w: columns 8*(w&1),
-// rows 8*(w>>1)); lane l is pixel (l & 7, l >> 3) of that block.
+// Shape of a raster launch: kT x kT pixel tiles, one thread per pixel, in
+// waves of 8x8 pixel blocks (wave w: block column w % (kT/8), block row
+// w / (kT/8); lane l is pixel (l & 7, l >> 3) of its block), and kB keys
+// staged in LDS per round, in chunks of 64 keys (one ballot word each).
+// Each key carries 4 cover bits, one per quadrant of the tile (kT/2 x kT/2
+// pixels, bit 2 row + column): a wave tests its quadrant's bit (at kT = 16
+// the quadrant is the wave's own block; at kT = 32 four waves share one; at
+// kT = 8 the one wave spans all four and tests whether any bit is set).
+template <int kT, int kB>
+struct Shape {
+  static constexpr int T = kT, B = kB;
+  static constexpr int kThreads = kT * kT;
+  static constexpr int kWaves = kThreads / 64;
+  static constexpr int kChunks = kB / 64;
+  static constexpr int kBW = kT / 8;            // 8x8 blocks per tile row
+  static constexpr int kMasks = kT == 8 ? 1 : 4;
+  static_assert((kT == 8 || kT == 16 || kT == 32) && (kB == 64 || kB == 128 || kB == 256), "raster shape");
+  __device__ static int mask_slot(int w) {
+    if constexpr (kT == 8) {
+      return 0;
+    } else {
+      constexpr int h = kBW / 2;   // blocks per quadrant row
+      return 2 * ((w / kBW) / h) + (w % kBW) / h;
+    }
+  }
+};
+
 struct PixelMap {
   int x, y;        // pixel
 };
+template <class S>
 __device__ __forceinline__ PixelMap pixel_of_thread(const Cam& cam, int tile, int tid) {
   const int tx = tile % cam.tilesX, ty = tile / cam.tilesX;
   const int w = tid >> 6, l = tid & 63;
   PixelMap m;
-  m.x = tx * kTile + (w & 1) * 8 + (l & 7);
-  m.y = ty * kTile + (w >> 1) * 8 + (l >> 3);
+  m.x = tx * S::T + (w % S::kBW) * 8 + (l & 7);
+  m.y = ty * S::T + (w / S::kBW) * 8 + (l >> 3);
   return m;
 }
 
-// Stores, for the staged batch, wave c's ballot of "entry touches block w"
-// in sMask[c][w]; entries past the batch end contribute no bits.
-__device__ __forceinline__ void build_masks(uint32_t cover, uint64_t (*sMask)[4]) {
-  const int tid = threadIdx.x, c = tid >> 6;
+// Stores the staged chunk's ballots of the cover bits the waves test (lane
+// l of the calling wave holds key 64 c + l of the batch; keys past the
+// segment end have cover 0 and contribute no bits).
+template <class S>
+__device__ __forceinline__ void store_masks(uint32_t cover, uint64_t* m) {
+  const bool lead = (threadIdx.x & 63) == 0;
+  if constexpr (S::kMasks == 1) {
+    const uint64_t bal = wave_ballot(cover != 0u);
+    if (lead) m[0] = bal;
+  } else {
 #pragma unroll
-  for (int w = 0; w < 4; w++) {
-    const uint64_t bal = wave_ballot(((cover >> w) & 1u) != 0u);
-    if ((tid & 63) == 0) sMask[c][w] = bal;
+    for (int q = 0; q < 4; q++) {
+      const uint64_t bal = wave_ballot(((cover >> q) & 1u) != 0u);
+      if (lead) m[q] = bal;
+    }
   }
 }
 
@@ -113,15 +152,16 @@ __device__ __forceinline__ float4 record_colour(int recFmt, const float4& r1, fl
 // One tile of the forward pass, by the whole 256-thread block. The block's
 // threads must all call it (it synchronises), and nothing else may use the
 // block's LDS until every thread has returned.
-template <int kLoss>
+template <class S, int kLoss>
 __device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, const RasterBufs& b, int tile) {
-  __shared__ float4 sA[kBlock];   // mx, my, A, B
-  __shared__ float4 sB[kBlock];   // C, opa*comp, L, 0
-  __shared__ float4 sC[kBlock];   // r, g, b, opa
-  __shared__ uint64_t sMask[4][4];
-  __shared__ double sStat[3][kBlock / 64];
-  const int tid = threadIdx.x, wv = tid >> 6;
-  const PixelMap pm = pixel_of_thread(cam, tile, tid);
+  __shared__ float4 sA[S::B];   // mx, my, A, B
+  __shared__ float4 sB[S::B];   // C, opa*comp, L, 0
+  __shared__ float4 sC[S::B];   // r, g, b, opa
+  __shared__ uint64_t sMask[S::kChunks][S::kMasks];
+  __shared__ double sStat[3][S::kWaves];
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int mslot = S::mask_slot(wv);
+  const PixelMap pm = pixel_of_thread<S>(cam, tile, tid);
   const int x = pm.x, y = pm.y;
   const bool inside = x < cam.W && y < cam.H;
   const float px = (float)x + 0.5f, py = (float)y + 0.5f;
@@ -130,37 +170,41 @@ __device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, 
   float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
   uint32_t end = s;
   bool done = !inside;
-  for (uint32_t base = s; base < e; base += kBlock) {
+  for (uint32_t base = s; base < e; base += S::B) {
     if (__syncthreads_count(!done) == 0) break;
-    const uint32_t k = base + tid;
-    uint32_t cover = 0;
-    if (k < e) {
-      const uint32_t v = b.ids[k];
-      cover = v >> kMaskShift;
-      if (cover != 0u) {           // splats no block of the tile can take stay unloaded
-        const uint32_t id = v & kIdMask;
-        if (b.recFmt == 0) {
-          sA[tid] = b.rec[4 * id + 0];
-          sB[tid] = b.rec[4 * id + 1];
-          sC[tid] = b.rec[4 * id + 2];
-        } else {
-          const float4 r1 = b.rec[3 * id + 1];
-          sA[tid] = b.rec[3 * id + 0];
-          sB[tid] = r1;
-          sC[tid] = record_colour(b.recFmt, r1, 0.0f);
+    // wave w stages chunks w, w + kWaves, ... (lane l: key 64 c + l)
+    for (int c = wv; c < S::kChunks; c += S::kWaves) {
+      const int j = 64 * c + lane;
+      const uint32_t k = base + (uint32_t)j;
+      uint32_t cover = 0;
+      if (k < e) {
+        const uint32_t v = b.ids[k];
+        cover = v >> kMaskShift;
+        if (cover != 0u) {           // splats no block of the tile can take stay unloaded
+          const uint32_t id = v & kIdMask;
+          if (b.recFmt == 0) {
+            sA[j] = b.rec[4 * id + 0];
+            sB[j] = b.rec[4 * id + 1];
+            sC[j] = b.rec[4 * id + 2];
+          } else {
+            const float4 r1 = b.rec[3 * id + 1];
+            sA[j] = b.rec[3 * id + 0];
+            sB[j] = r1;
+            sC[j] = record_colour(b.recFmt, r1, 0.0f);
+          }
         }
       }
+      store_masks<S>(cover, sMask[c]);
     }
-    build_masks(cover, sMask);
     __syncthreads();
-    const uint32_t cnt = min((uint32_t)kBlock, e - base);
+    const uint32_t cnt = min((uint32_t)S::B, e - base);
     // lanes still compositing; a lane is done once outside the image or
     // once T < 1e-4, so the vote is kept current with one compare per splat
     uint64_t run = wave_ballot(!done);
     if (run != 0ull) {
       uint32_t term = 0xffffffffu;  // batch index of this pixel's last splat
-      for (int c = 0; c < 4; c++) {
-        uint64_t m = uniform64(sMask[c][wv]);
+      for (int c = 0; c < S::kChunks; c++) {
+        uint64_t m = uniform64(sMask[c][mslot]);
         while (m) {
           const int j = c * 64 + (int)__builtin_ctzll(m);
           m &= m - 1ull;
@@ -195,7 +239,7 @@ __device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, 
             if (left != run) {
               if (fin && !done) { done = true; term = (uint32_t)j; }
               run = left;
-              if (run == 0ull) { m = 0ull; c = 4; }
+              if (run == 0ull) { m = 0ull; c = S::kChunks; }
             }
           }
         }
@@ -242,7 +286,8 @@ __device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, 
     if ((tid & 63) == 0) { sStat[0][wv] = st0; sStat[1][wv] = st1; sStat[2][wv] = st2; }
     __syncthreads();
     if (tid < 3) {
-      const double v = sStat[tid][0] + sStat[tid][1] + sStat[tid][2] + sStat[tid][3];
+      double v = sStat[tid][0];
+      for (int w = 1; w < S::kWaves; w++) v += sStat[tid][w];
       if (v != 0.0) atomicAdd(&b.stats[tid], v);
     }
   }
@@ -330,18 +375,19 @@ __device__ __forceinline__ int reduce10_slot() {
 // 16384 dL/dC * alpha * T (alpha * T <= 1) stay far inside the +-1e9 clamp
 // and convert without it.
 // One tile of the backward pass, by the whole block (as raster_fwd_tile).
-template <int kAccum, bool kClampColour>
+template <class S, int kAccum, bool kClampColour>
 __device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, const RasterBufs& b, void* accum,
                                                 int tile) {
   // four 16-byte records per staged splat, all at index j: one LDS address
-  __shared__ float4 sA[kBlock];   // mx, my, A, B
-  __shared__ float4 sB[kBlock];   // C, opa*comp, L, 0
-  __shared__ float4 sC[kBlock];   // r, g, b, opa
-  __shared__ float4 sD[kBlock];   // -0.5 cnorm, 1 - opa, id (bits), 0
-  __shared__ uint64_t sMask[4][4];
+  __shared__ float4 sA[S::B];   // mx, my, A, B
+  __shared__ float4 sB[S::B];   // C, opa*comp, L, 0
+  __shared__ float4 sC[S::B];   // r, g, b, opa
+  __shared__ float4 sD[S::B];   // -0.5 cnorm, 1 - opa, id (bits), 0
+  __shared__ uint64_t sMask[S::kChunks][S::kMasks];
   __shared__ uint32_t sEnd;
-  const int tid = threadIdx.x, wv = tid >> 6;
-  const PixelMap pm = pixel_of_thread(cam, tile, tid);
+  const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+  const int mslot = S::mask_slot(wv);
+  const PixelMap pm = pixel_of_thread<S>(cam, tile, tid);
   const int x = pm.x, y = pm.y;
   const bool inside = x < cam.W && y < cam.H;
   const float px = (float)x + 0.5f, py = (float)y + 0.5f;
@@ -384,35 +430,38 @@ __device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, 
   // 16384 dot(gC, S) of the colour-behind sum S, which enters only through it
   float GS = fmaf(gCs[2], cam.bg[2], fmaf(gCs[1], cam.bg[1], gCs[0] * cam.bg[0])) * Ta;
   for (uint32_t hi = endMax; hi > s; ) {
-    const uint32_t lo = hi - s > (uint32_t)kBlock ? hi - kBlock : s;
+    const uint32_t lo = hi - s > (uint32_t)S::B ? hi - S::B : s;
     const int n = (int)(hi - lo);
     __syncthreads();
-    uint32_t cover = 0;
-    if (tid < n) {
-      const uint32_t v = b.ids[lo + tid];
-      cover = v >> kMaskShift;
-      if (cover != 0u) {
-        const uint32_t id = v & kIdMask;
-        if (b.recFmt == 0) {
-          sA[tid] = b.rec[4 * id + 0];
-          sB[tid] = b.rec[4 * id + 1];
-          sC[tid] = b.rec[4 * id + 2];
-          const float4 d = b.rec[4 * id + 3];
-          sD[tid] = make_float4(d.y, d.z, __uint_as_float(id), 0.0f);
-        } else {
-          const float4 r1 = b.rec[3 * id + 1];
-          const float4 d = b.rec[3 * id + 2];   // comp, -0.5 cnorm, 1 - opa, opa
-          sA[tid] = b.rec[3 * id + 0];
-          sB[tid] = r1;
-          sC[tid] = record_colour(b.recFmt, r1, d.w);
-          sD[tid] = make_float4(d.y, d.z, __uint_as_float(id), 0.0f);
+    for (int c = wv; c < S::kChunks; c += S::kWaves) {
+      const int j = 64 * c + lane;
+      uint32_t cover = 0;
+      if (j < n) {
+        const uint32_t v = b.ids[lo + (uint32_t)j];
+        cover = v >> kMaskShift;
+        if (cover != 0u) {
+          const uint32_t id = v & kIdMask;
+          if (b.recFmt == 0) {
+            sA[j] = b.rec[4 * id + 0];
+            sB[j] = b.rec[4 * id + 1];
+            sC[j] = b.rec[4 * id + 2];
+            const float4 d = b.rec[4 * id + 3];
+            sD[j] = make_float4(d.y, d.z, __uint_as_float(id), 0.0f);
+          } else {
+            const float4 r1 = b.rec[3 * id + 1];
+            const float4 d = b.rec[3 * id + 2];   // comp, -0.5 cnorm, 1 - opa, opa
+            sA[j] = b.rec[3 * id + 0];
+            sB[j] = r1;
+            sC[j] = record_colour(b.recFmt, r1, d.w);
+            sD[j] = make_float4(d.y, d.z, __uint_as_float(id), 0.0f);
+          }
         }
       }
+      store_masks<S>(cover, sMask[c]);
     }
-    build_masks(cover, sMask);
     __syncthreads();
-    for (int c = 3; c >= 0; c--) {
-      uint64_t m = uniform64(sMask[c][wv]);
+    for (int c = S::kChunks - 1; c >= 0; c--) {
+      uint64_t m = uniform64(sMask[c][mslot]);
       const uint32_t g0 = lo + 64u * (uint32_t)c;   // key of bit 0
       if (waveEnd <= g0) m = 0ull;
       else if (waveEnd - g0 < 64u) m &= (1ull << (waveEnd - g0)) - 1ull;
@@ -501,14 +550,14 @@ __device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, 
   }
 }
 
-template <int kLoss>
-__global__ __launch_bounds__(kBlock) void k_raster_fwd(Cam cam, Cut cut, RasterBufs b) {
-  raster_fwd_tile<kLoss>(cam, cut, b, tile_of_block(b));
+template <class S, int kLoss>
+__global__ __launch_bounds__(S::kThreads) void k_raster_fwd(Cam cam, Cut cut, RasterBufs b) {
+  raster_fwd_tile<S, kLoss>(cam, cut, b, tile_of_block(b));
 }
 
-template <int kAccum, bool kClampColour>
-__global__ __launch_bounds__(kBlock) void k_raster_bwd(Cam cam, Cut cut, RasterBufs b, void* accum) {
-  raster_bwd_tile<kAccum, kClampColour>(cam, cut, b, accum, tile_of_block(b));
+template <class S, int kAccum, bool kClampColour>
+__global__ __launch_bounds__(S::kThreads) void k_raster_bwd(Cam cam, Cut cut, RasterBufs b, void* accum) {
+  raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, accum, tile_of_block(b));
 }
 
 // ---- several views in one persistent launch ----
@@ -537,94 +586,139 @@ __device__ __forceinline__ bool next_item(const ViewBlock* __restrict__ vb, uint
   return true;
 }
 
-template <int kLoss>
-__global__ __launch_bounds__(kBlock) void k_raster_fwd_views(const ViewBlock* __restrict__ vb, Cut cut,
-                                                            uint32_t* queue) {
+template <class S, int kLoss>
+__global__ __launch_bounds__(S::kThreads) void k_raster_fwd_views(const ViewBlock* __restrict__ vb, Cut cut,
+                                                                  uint32_t* queue) {
   ViewItem it;
   while (next_item(vb, queue, it)) {
     const Cam cam = vb->cams[it.v];
     const RasterBufs b = vb->bufs[it.v];
-    raster_fwd_tile<kLoss>(cam, cut, b, it.tile);
+    raster_fwd_tile<S, kLoss>(cam, cut, b, it.tile);
   }
 }
 
-template <int kAccum, bool kClampColour>
-__global__ __launch_bounds__(kBlock) void k_raster_bwd_views(const ViewBlock* __restrict__ vb, Cut cut,
-                                                            uint32_t* queue) {
+template <class S, int kAccum, bool kClampColour>
+__global__ __launch_bounds__(S::kThreads) void k_raster_bwd_views(const ViewBlock* __restrict__ vb, Cut cut,
+                                                                  uint32_t* queue) {
   ViewItem it;
   while (next_item(vb, queue, it)) {
     const Cam cam = vb->cams[it.v];
     const RasterBufs b = vb->bufs[it.v];
-    raster_bwd_tile<kAccum, kClampColour>(cam, cut, b, vb->accum[it.v], it.tile);
+    raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, vb->accum[it.v], it.tile);
   }
 }
 
 // one view of a batched step, one block per tile (camera and buffers from
 // device memory)
-template <int kLoss>
-__global__ __launch_bounds__(kBlock) void k_raster_fwd_view(const ViewBlock* __restrict__ vb, int v, Cut cut) {
+template <class S, int kLoss>
+__global__ __launch_bounds__(S::kThreads) void k_raster_fwd_view(const ViewBlock* __restrict__ vb, int v, Cut cut) {
   const Cam cam = vb->cams[v];
   const RasterBufs b = vb->bufs[v];
-  raster_fwd_tile<kLoss>(cam, cut, b, tile_of_block(b));
+  raster_fwd_tile<S, kLoss>(cam, cut, b, tile_of_block(b));
 }
 
-template <int kAccum, bool kClampColour>
-__global__ __launch_bounds__(kBlock) void k_raster_bwd_view(const ViewBlock* __restrict__ vb, int v, Cut cut) {
+template <class S, int kAccum, bool kClampColour>
+__global__ __launch_bounds__(S::kThreads) void k_raster_bwd_view(const ViewBlock* __restrict__ vb, int v, Cut cut) {
   const Cam cam = vb->cams[v];
   const RasterBufs b = vb->bufs[v];
-  raster_bwd_tile<kAccum, kClampColour>(cam, cut, b, vb->accum[v], tile_of_block(b));
+  raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, vb->accum[v], tile_of_block(b));
 }
 
-void launch_raster_fwd_view(hipStream_t st, const ViewBlock* vb, int v, int numTiles, const Cut& cut, int loss) {
-  if (loss == 1) k_raster_fwd_view<1><<<numTiles, kBlock, 0, st>>>(vb, v, cut);
-  else if (loss == 2) k_raster_fwd_view<2><<<numTiles, kBlock, 0, st>>>(vb, v, cut);
-  else k_raster_fwd_view<0><<<numTiles, kBlock, 0, st>>>(vb, v, cut);
+// f(Shape<tile, batch>{}) for a supported shape (checked by the callers)
+template <int kT, typename F>
+static void with_batch(int batch, F&& f) {
+  if (batch == 64) f(Shape<kT, 64>{});
+  else if (batch == 128) f(Shape<kT, 128>{});
+  else f(Shape<kT, 256>{});
 }
 
-void launch_raster_bwd_view(hipStream_t st, const ViewBlock* vb, int v, int numTiles, const Cut& cut, int accumMode,
-                            bool boundedColourGrad) {
-  if (accumMode != 0) k_raster_bwd_view<1, true><<<numTiles, kBlock, 0, st>>>(vb, v, cut);
-  else if (boundedColourGrad) k_raster_bwd_view<0, false><<<numTiles, kBlock, 0, st>>>(vb, v, cut);
-  else k_raster_bwd_view<0, true><<<numTiles, kBlock, 0, st>>>(vb, v, cut);
+template <typename F>
+static void with_shape(const RasterShape& sh, F&& f) {
+  if (sh.tile == 8) with_batch<8>(sh.batch, f);
+  else if (sh.tile == 32) with_batch<32>(sh.batch, f);
+  else with_batch<16>(sh.batch, f);
 }
 
-void launch_raster_fwd(hipStream_t st, const Cam& cam, const Cut& cut, const RasterBufs& b, int loss) {
+bool raster_shape_ok(int tile, int batch) {
+  return (tile == 8 || tile == 16 || tile == 32) && (batch == 64 || batch == 128 || batch == 256);
+}
+
+void launch_raster_fwd_view(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, int v, int numTiles,
+                            const Cut& cut, int loss) {
+  with_shape(sh, [&](auto shape) {
+    using S = decltype(shape);
+    if (loss == 1) k_raster_fwd_view<S, 1><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
+    else if (loss == 2) k_raster_fwd_view<S, 2><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
+    else k_raster_fwd_view<S, 0><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
+  });
+}
+
+void launch_raster_bwd_view(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, int v, int numTiles,
+                            const Cut& cut, int accumMode, bool boundedColourGrad) {
+  with_shape(sh, [&](auto shape) {
+    using S = decltype(shape);
+    if (accumMode != 0) k_raster_bwd_view<S, 1, true><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
+    else if (boundedColourGrad) k_raster_bwd_view<S, 0, false><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
+    else k_raster_bwd_view<S, 0, true><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
+  });
+}
+
+void launch_raster_fwd(hipStream_t st, const RasterShape& sh, const Cam& cam, const Cut& cut, const RasterBufs& b,
+                       int loss) {
   const dim3 grid((unsigned)cam.numTiles);
-  if (loss == 1) k_raster_fwd<1><<<grid, kBlock, 0, st>>>(cam, cut, b);
-  else if (loss == 2) k_raster_fwd<2><<<grid, kBlock, 0, st>>>(cam, cut, b);
-  else k_raster_fwd<0><<<grid, kBlock, 0, st>>>(cam, cut, b);
+  with_shape(sh, [&](auto shape) {
+    using S = decltype(shape);
+    if (loss == 1) k_raster_fwd<S, 1><<<grid, S::kThreads, 0, st>>>(cam, cut, b);
+    else if (loss == 2) k_raster_fwd<S, 2><<<grid, S::kThreads, 0, st>>>(cam, cut, b);
+    else k_raster_fwd<S, 0><<<grid, S::kThreads, 0, st>>>(cam, cut, b);
+  });
 }
 
-void launch_raster_bwd(hipStream_t st, const Cam& cam, const Cut& cut, const RasterBufs& b,
+void launch_raster_bwd(hipStream_t st, const RasterShape& sh, const Cam& cam, const Cut& cut, const RasterBufs& b,
                        int accumMode, void* accum, bool boundedColourGrad) {
   const dim3 grid((unsigned)cam.numTiles);
-  if (accumMode != 0) k_raster_bwd<1, true><<<grid, kBlock, 0, st>>>(cam, cut, b, accum);
-  else if (boundedColourGrad) k_raster_bwd<0, false><<<grid, kBlock, 0, st>>>(cam, cut, b, accum);
-  else k_raster_bwd<0, true><<<grid, kBlock, 0, st>>>(cam, cut, b, accum);
+  with_shape(sh, [&](auto shape) {
+    using S = decltype(shape);
+    if (accumMode != 0) k_raster_bwd<S, 1, true><<<grid, S::kThreads, 0, st>>>(cam, cut, b, accum);
+    else if (boundedColourGrad) k_raster_bwd<S, 0, false><<<grid, S::kThreads, 0, st>>>(cam, cut, b, accum);
+    else k_raster_bwd<S, 0, true><<<grid, S::kThreads, 0, st>>>(cam, cut, b, accum);
+  });
 }
 
-void launch_raster_fwd_views(hipStream_t st, const ViewBlock* vb, const Cut& cut, int loss, uint32_t* queue,
-                             int blocks) {
-  if (loss == 1) k_raster_fwd_views<1><<<blocks, kBlock, 0, st>>>(vb, cut, queue);
-  else if (loss == 2) k_raster_fwd_views<2><<<blocks, kBlock, 0, st>>>(vb, cut, queue);
-  else k_raster_fwd_views<0><<<blocks, kBlock, 0, st>>>(vb, cut, queue);
+void launch_raster_fwd_views(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, const Cut& cut, int loss,
+                             uint32_t* queue, int blocks) {
+  with_shape(sh, [&](auto shape) {
+    using S = decltype(shape);
+    if (loss == 1) k_raster_fwd_views<S, 1><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
+    else if (loss == 2) k_raster_fwd_views<S, 2><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
+    else k_raster_fwd_views<S, 0><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
+  });
 }
 
-void launch_raster_bwd_views(hipStream_t st, const ViewBlock* vb, const Cut& cut, int accumMode,
-                             bool boundedColourGrad, uint32_t* queue, int blocks) {
-  if (accumMode != 0) k_raster_bwd_views<1, true><<<blocks, kBlock, 0, st>>>(vb, cut, queue);
-  else if (boundedColourGrad) k_raster_bwd_views<0, false><<<blocks, kBlock, 0, st>>>(vb, cut, queue);
-  else k_raster_bwd_views<0, true><<<blocks, kBlock, 0, st>>>(vb, cut, queue);
+void launch_raster_bwd_views(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, const Cut& cut,
+                             int accumMode, bool boundedColourGrad, uint32_t* queue, int blocks) {
+  with_shape(sh, [&](auto shape) {
+    using S = decltype(shape);
+    if (accumMode != 0) k_raster_bwd_views<S, 1, true><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
+    else if (boundedColourGrad) k_raster_bwd_views<S, 0, false><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
+    else k_raster_bwd_views<S, 0, true><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
+  });
 }
 
-// blocks of the persistent launches: as many as fit on the device at once
-int raster_views_blocks(int device) {
-  int cus = 0, per = 0;
+// blocks of the persistent launches of one shape: as many as fit on the
+// device at once
+int raster_views_blocks(int device, const RasterShape& sh) {
+  int cus = 0, per = 1;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
-  int a = 0, b = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_raster_fwd_views<1>, kBlock, 0) != hipSuccess) a = 1;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_raster_bwd_views<0, false>, kBlock, 0) != hipSuccess) b = 1;
-  per = a < b ? a : b;
+  with_shape(sh, [&](auto shape) {
+    using S = decltype(shape);
+    int a = 0, b = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_raster_fwd_views<S, 1>, S::kThreads, 0) != hipSuccess) a = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_raster_bwd_views<S, 0, false>, S::kThreads, 0) !=
+        hipSuccess)
+      b = 1;
+    per = a < b ? a : b;
+  });
   if (per < 1) per = 1;
   return cus * per;
 }

@@ -398,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ or
                                                  const uint2* __restrict__ rectS,
                                                  const uint32_t* __restrict__ dkS,
                                                  const uint32_t* __restrict__ dN,
-                                                 const uint64_t* __restrict__ partials, int tilesX,
+                                                 const uint64_t* __restrict__ partials, int tilesX, int tileSize,
                                                  uint64_t cap, const float4* __restrict__ rec, int recFmt,
                                                  void* __restrict__ keysOut,
                                                  uint32_t* __restrict__ valsOut) {
@@ -440,7 +440,7 @@ __global__ __launch_bounds__(kBlock) void k_emit(const uint32_t* __restrict__ or
       if (kGlobal64) ((uint64_t*)keysOut)[pos] = ((uint64_t)tile << 32) | sDk[g];
       else ((uint32_t*)keysOut)[pos] = tile;
       const gs_cover_splat sg = sG[g];
-      const uint32_t mask = gs_cover_tile_mask(&sg, (int)tx, (int)ty);
+      const uint32_t mask = gs_cover_tile_mask(&sg, (int)tx, (int)ty, tileSize);
       valsOut[pos] = sId[g] | (mask << kMaskShift);
     }
   }
@@ -973,7 +973,7 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint32_t* __r
                                                              const uint2* __restrict__ rect,
                                                              const uint32_t* __restrict__ depthKey,
                                                              const float4* __restrict__ rec, int recFmt,
-                                                             int tilesX, int numTiles,
+                                                             int tilesX, int tileSize, int numTiles,
                                                              const uint32_t* __restrict__ table,
                                                              const uint2* __restrict__ ranges,
                                                              uint64_t* __restrict__ keys) {
@@ -1002,7 +1002,7 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint32_t* __r
                const uint32_t pos = atomicAdd(&cur[tile], 1u);
                if (dropped[tile]) return;   // tile dropped by the budget (shaders.js:406-407)
                const gs_cover_splat sg = sG[g];
-               const uint32_t mask = gs_cover_tile_mask(&sg, (int)tx, (int)ty);
+               const uint32_t mask = gs_cover_tile_mask(&sg, (int)tx, (int)ty, tileSize);
                keys[pos] = seg_key(sDk[g], (uint32_t)(r0 + g), mask);
              });
 }
@@ -1264,7 +1264,7 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
         s.tileCnt, a.numTiles, a.cap, s.visPart, nb, a.ranges, a.tileOrder, a.numKeys, a.numVisible, a.totalKeys,
         a.stats, s.chunkKeys, (a.n + kSegChunk - 1) / kSegChunk);
     k_seg_scatter<<<nb, kSegThreads, (size_t)a.numTiles * 5, st>>>(s.blockStart, a.cnt, a.rect, a.depthKey, a.rec,
-                                                                   a.recFmt, a.tilesX, a.numTiles, s.segTable, a.ranges,
+                                                                   a.recFmt, a.tilesX, a.tile, a.numTiles, s.segTable, a.ranges,
                                                                    (uint64_t*)a.tkA);
     k_seg_sort<<<a.numTiles, kBlock, 0, st>>>(a.ranges, a.tileOrder, (uint64_t*)a.tkA, (uint64_t*)a.tkB, a.tvA);
     out.keys = nullptr;
@@ -1283,7 +1283,7 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
     k_count_partials<<<Gc, kBlock, 0, st>>>(order, a.cnt, a.rect, a.depthKey, s.nDepth, s.cntS, s.rectS, nullptr,
                                             s.partials, s.vpartials);
     k_count_top<<<1, 1024, 0, st>>>(s.partials, s.vpartials, Gc, a.cap, a.numKeys, a.totalKeys, a.numVisible, a.stats);
-    k_emit<false><<<Gc, kBlock, 0, st>>>(order, s.cntS, s.rectS, nullptr, s.nDepth, s.partials, a.tilesX, a.cap,
+    k_emit<false><<<Gc, kBlock, 0, st>>>(order, s.cntS, s.rectS, nullptr, s.nDepth, s.partials, a.tilesX, a.tile, a.cap,
                                          a.rec, a.recFmt, a.tkA, a.tvA);
     // 3) stable sort by tile id
     const uint32_t* tk;
@@ -1300,7 +1300,7 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
     k_count_partials<<<Gc, kBlock, 0, st>>>(nullptr, a.cnt, a.rect, a.depthKey, a.dN, s.cntS, s.rectS, s.dkS,
                                             s.partials, s.vpartials);
     k_count_top<<<1, 1024, 0, st>>>(s.partials, s.vpartials, Gc, a.cap, a.numKeys, a.totalKeys, a.numVisible, a.stats);
-    k_emit<true><<<Gc, kBlock, 0, st>>>(nullptr, s.cntS, s.rectS, s.dkS, a.dN, s.partials, a.tilesX, a.cap,
+    k_emit<true><<<Gc, kBlock, 0, st>>>(nullptr, s.cntS, s.rectS, s.dkS, a.dN, s.partials, a.tilesX, a.tile, a.cap,
                                         a.rec, a.recFmt, a.tkA, a.tvA);
     const uint64_t* tk;
     const uint32_t* tv;

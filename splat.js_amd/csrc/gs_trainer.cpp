@@ -87,8 +87,9 @@ struct gs_trainer {
   bool mom16 = false;   // bf16 Adam moments (GS_PREC_MIXED)
   int recFmt = 0;       // raster record colours: 0 f32 (64 B records), 1 bf16, 2 fp16 (48 B)
   int64_t entriesCap = 0;
-  int maxTiles = 0;
+  int maxTiles = 0;      // tiles of the largest view at the smallest tile size (8): every shape fits
   int64_t maxPix = 0;
+  RasterShape shape{kDefaultTile, kDefaultBatch};   // raster tile side and LDS batch
 
   std::vector<void*> allocs;
   float4* P = nullptr;
@@ -305,6 +306,10 @@ void prof_collect(gs_trainer* h) {
   h->profLast = nullptr;
 }
 
+constexpr int kMinTile = 8;   // smallest raster tile side
+
+int tiles_of(int w, int hgt, int tile) { return ((w + tile - 1) / tile) * ((hgt + tile - 1) / tile); }
+
 Cam make_cam(const gs_trainer* h, const gs_camera& c, int trainMode, int64_t offset, int activeDeg) {
   Cam k{};
   for (int i = 0; i < 9; i++) k.R[i] = c.R[i];
@@ -320,8 +325,9 @@ Cam make_cam(const gs_trainer* h, const gs_camera& c, int trainMode, int64_t off
   k.camPos[1] = -((R[1] * t[0] + R[4] * t[1]) + R[7] * t[2]);
   k.camPos[2] = -((R[2] * t[0] + R[5] * t[1]) + R[8] * t[2]);
   k.W = c.w; k.H = c.h;
-  k.tilesX = (c.w + kTile - 1) / kTile;
-  k.tilesY = (c.h + kTile - 1) / kTile;
+  k.tile = h->shape.tile;
+  k.tilesX = (c.w + k.tile - 1) / k.tile;
+  k.tilesY = (c.h + k.tile - 1) / k.tile;
   k.numTiles = k.tilesX * k.tilesY;
   const int a = (activeDeg + 1) * (activeDeg + 1) - 1;
   k.shActiveK = std::min(a, h->shK);
@@ -435,6 +441,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
   {
     BinArgs a{};
     a.mode = h->o.sortMode;
+    a.tile = cam.tile;
     a.tilesX = cam.tilesX;
     a.numTiles = cam.numTiles;
     h->lastNumTiles = cam.numTiles;
@@ -472,7 +479,7 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
   b.stats = statsBuf;
   {
     Scope s(h, loss ? "raster_fwd_loss" : "raster_fwd", 0.0);
-    launch_raster_fwd(h->st, cam, make_cut(h), b, loss);
+    launch_raster_fwd(h->st, h->shape, cam, make_cut(h), b, loss);
   }
   HIPCK(hipGetLastError());
   return GS_OK;
@@ -568,7 +575,7 @@ int eval_pass(gs_trainer* h, int ci, const double* R, const double* t, double g,
   rb.T = h->Tfin;
   rb.end = h->endIdx;
   rb.gC = h->gC;
-  launch_raster_bwd(h->st, cam, make_cut(h), rb, h->o.gradAccum, h->accum, charb_grad_bounded(cam));
+  launch_raster_bwd(h->st, h->shape, cam, make_cut(h), rb, h->o.gradAccum, h->accum, charb_grad_bounded(cam));
   launch_cam_pose(h->st, h->o.gradAccum, true, cam, cam_pose_bufs(h, ci));
   HIPCK(hipGetLastError());
   double s[4];
@@ -709,7 +716,8 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
   b.gC = h->gC;
   {
     Scope s(h, "raster_bwd", 0.0);
-    launch_raster_bwd(h->st, cam, make_cut(h), b, h->o.gradAccum, h->accum, !ssim && charb_grad_bounded(cam));
+    launch_raster_bwd(h->st, h->shape, cam, make_cut(h), b, h->o.gradAccum, h->accum,
+                      !ssim && charb_grad_bounded(cam));
   }
   HIPCK(hipGetLastError());
   float* dGF = nullptr;
@@ -769,7 +777,11 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
 // launches serve eager steps and a captured graph; what the host fixes here
 // is the shape: n, each view's tile grid and the colour-clamp choice.
 // dbg (eager only): copy the views' intermediates between the stages.
-int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* dGF, float* dSG) {
+// statsBuf: the loss window and budget drops (the views' RasterBufs.stats
+// must point there too); update false: no chain and Adam (the gradient sums
+// stay in the views' accumulators).
+int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* dGF, float* dSG, double* statsBuf,
+                  bool update) {
   const int V = vb.count;
   const int64_t n = h->n;
   const Cut cut = make_cut(h);
@@ -796,6 +808,7 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
     }
     BinArgs a{};
     a.mode = GS_SORT_TILE_SEGMENTS;
+    a.tile = cam.tile;
     a.tilesX = cam.tilesX;
     a.numTiles = cam.numTiles;
     a.n = n;
@@ -813,7 +826,7 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
     a.totalKeys = h->totalKeys;
     a.ranges = sl.ranges;
     a.tileOrder = sl.tileOrder;
-    a.stats = h->stats;
+    a.stats = statsBuf;
     a.scratch = h->sort;
     {
       Scope sc(h, "bin_sort", 0.0);
@@ -831,8 +844,8 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
     if (overlap) {
       HIPCK(hipEventRecord(h->evBin[k], h->st));
       HIPCK(hipStreamWaitEvent(h->st2, h->evBin[k], 0));
-      launch_raster_fwd_view(h->st2, h->dVB, k, cam.numTiles, cut, 1);
-      launch_raster_bwd_view(h->st2, h->dVB, k, cam.numTiles, cut, h->o.gradAccum, bounded);
+      launch_raster_fwd_view(h->st2, h->shape, h->dVB, k, cam.numTiles, cut, 1);
+      launch_raster_bwd_view(h->st2, h->shape, h->dVB, k, cam.numTiles, cut, h->o.gradAccum, bounded);
     }
   }
   h->lastNumTiles = vb.cams[V - 1].numTiles;
@@ -843,7 +856,7 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
     HIPCK(hipStreamWaitEvent(h->st, h->evJoin, 0));
   } else {
     Scope sc(h, "raster_fwd_views", 0.0);
-    launch_raster_fwd_views(h->st, h->dVB, cut, 1, h->dQueue, h->viewBlocks);
+    launch_raster_fwd_views(h->st, h->shape, h->dVB, cut, 1, h->dQueue, h->viewBlocks);
   }
   if (dbg) {
     HIPCK(hipStreamSynchronize(h->st));
@@ -858,7 +871,8 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
   }
   if (!overlap) {
     Scope sc(h, "raster_bwd_views", 0.0);
-    launch_raster_bwd_views(h->st, h->dVB, cut, h->o.gradAccum, bounded, h->dQueue + 1, h->viewBlocks);
+    launch_raster_bwd_views(h->st, h->shape, h->dVB, cut, h->o.gradAccum, bounded, h->dQueue + 1,
+                            h->viewBlocks);
   }
   if (dbg) {
     HIPCK(hipStreamSynchronize(h->st));
@@ -870,18 +884,57 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
         for (int q = 0; q < 10; q++) dbg[k].accum[10 * i + q] = acc[12 * i + q];
     }
   }
-  UpdateBufs u{};
-  u.n = n;
-  u.cap = h->cap;
-  u.P = h->P; u.M = h->M; u.V = h->V;
-  u.SH = h->SH; u.SHM = h->SHM; u.SHV = h->SHV;
-  u.dbgGradF = dGF;
-  u.dbgShGrad = dSG;
-  {
+  if (update) {
+    UpdateBufs u{};
+    u.n = n;
+    u.cap = h->cap;
+    u.P = h->P; u.M = h->M; u.V = h->V;
+    u.SH = h->SH; u.SHM = h->SHM; u.SHV = h->SHV;
+    u.dbgGradF = dGF;
+    u.dbgShGrad = dSG;
     Scope sc(h, "update_views", (double)n * (100.0 + 6 * 64.0 + 6 * 4.0 * h->shC));
     launch_update_views(h->st, h->shDeg, h->o.gradAccum, h->prec, h->dVB, u);
   }
   HIPCK(hipGetLastError());
+  return GS_OK;
+}
+
+// Fills vb's views from cameras ci[0..count) at SH degree activeDeg (the
+// raster statistics to statsBuf) and appends the step's shape to key.
+int fill_views(gs_trainer* h, const int* ci, int count, int activeDeg, double* statsBuf, ViewBlock& vb,
+               std::vector<int64_t>& key) {
+  memset(&vb, 0, sizeof vb);
+  vb.count = count;
+  uint32_t off = 0;
+  for (int k = 0; k < count; k++) {
+    CamMeta& meta = h->cams[ci[k]];
+    meta.activeDeg = activeDeg;
+    vb.cams[k] = make_cam_meta(h, meta, activeDeg);
+    if (vb.cams[k].numTiles > kMaxSegTiles)
+      return fail(h, GS_E_LIMIT, "stepViews: a view of more than " + std::to_string(kMaxSegTiles) +
+                                     " tiles (the tile-segment sort's limit)");
+    const gs_trainer::ViewSlot& sl = h->vslots[k];
+    RasterBufs& b = vb.bufs[k];
+    b.recFmt = h->recFmt;
+    b.ranges = sl.ranges;
+    b.tileOrder = sl.tileOrder;
+    b.ids = sl.ids;
+    b.rec = sl.rec;
+    b.img = sl.img;
+    b.T = sl.T;
+    b.end = sl.end;
+    b.gC = sl.gC;
+    b.tgt = h->targets;
+    b.stats = statsBuf;
+    vb.accum[k] = sl.accum;
+    vb.cnt[k] = sl.cnt;
+    vb.off[k] = off;
+    off += (uint32_t)vb.cams[k].numTiles;
+    key.push_back(vb.cams[k].W);
+    key.push_back(vb.cams[k].H);
+    if (!charb_grad_bounded(vb.cams[k])) key[2] = 1;
+  }
+  vb.off[count] = off;
   return GS_OK;
 }
 
@@ -904,39 +957,10 @@ int step_views_impl(gs_trainer* h, const int32_t* cams, int count, gs_debug_out*
   ViewBlock& vb = h->hVB[h->ringPos];
   hipEvent_t ev = h->ringEv[(size_t)h->ringPos];
   HIPCK(hipEventSynchronize(ev));
-  memset(&vb, 0, sizeof vb);
-  vb.count = count;
   const int activeDeg = (int)std::min<int64_t>(h->shDeg, h->iter / 1000);   // trainer.js:409-412
-  uint32_t off = 0;
-  std::vector<int64_t> key = {count, h->n, 0};
-  for (int k = 0; k < count; k++) {
-    CamMeta& meta = h->cams[ci[k]];
-    meta.activeDeg = activeDeg;
-    vb.cams[k] = make_cam_meta(h, meta, activeDeg);
-    if (vb.cams[k].numTiles > kMaxSegTiles)
-      return fail(h, GS_E_LIMIT, "stepViews: a view of more than 16384 tiles (the tile-segment sort's limit)");
-    const gs_trainer::ViewSlot& sl = h->vslots[k];
-    RasterBufs& b = vb.bufs[k];
-    b.recFmt = h->recFmt;
-    b.ranges = sl.ranges;
-    b.tileOrder = sl.tileOrder;
-    b.ids = sl.ids;
-    b.rec = sl.rec;
-    b.img = sl.img;
-    b.T = sl.T;
-    b.end = sl.end;
-    b.gC = sl.gC;
-    b.tgt = h->targets;
-    b.stats = h->stats;
-    vb.accum[k] = sl.accum;
-    vb.cnt[k] = sl.cnt;
-    vb.off[k] = off;
-    off += (uint32_t)vb.cams[k].numTiles;
-    key.push_back(vb.cams[k].W);
-    key.push_back(vb.cams[k].H);
-    if (!charb_grad_bounded(vb.cams[k])) key[2] = 1;
-  }
-  vb.off[count] = off;
+  std::vector<int64_t> key = {count, h->n, 0, h->shape.tile, h->shape.batch};
+  int frc = fill_views(h, ci, count, activeDeg, h->stats, vb, key);
+  if (frc) return frc;
   h->iter += count;
   h->adamT += 1;
   for (int k = 0; k < count; k++) h->pixelsSeen += (int64_t)vb.cams[k].W * vb.cams[k].H;
@@ -965,7 +989,7 @@ int step_views_impl(gs_trainer* h, const int32_t* cams, int count, gs_debug_out*
       h->gExec = nullptr;
     }
     HIPCK(hipStreamBeginCapture(h->st, hipStreamCaptureModeThreadLocal));
-    rc = enqueue_views(h, vb, nullptr, nullptr, nullptr);
+    rc = enqueue_views(h, vb, nullptr, nullptr, nullptr, h->stats, true);
     hipGraph_t g = nullptr;
     const hipError_t e = hipStreamEndCapture(h->st, &g);
     if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
@@ -976,7 +1000,7 @@ int step_views_impl(gs_trainer* h, const int32_t* cams, int count, gs_debug_out*
     h->gKey = key;
     HIPCK(hipGraphLaunch(h->gExec, h->st));
   } else {
-    rc = enqueue_views(h, vb, dbg, dGF, dSG);
+    rc = enqueue_views(h, vb, dbg, dGF, dSG, h->stats, true);
     h->lastEagerKey = key;
   }
   if (dbg && rc == GS_OK) {
@@ -1026,6 +1050,8 @@ int gs_default_opts(gs_opts* o) {
   o->viewsPerStep = 1;
   o->graph = 1;
   o->viewsLaunch = GS_VIEWS_PERSISTENT;
+  o->tile = kDefaultTile;
+  o->batch = kDefaultBatch;
   o->posLr = 0.0f;
   o->loss = GS_LOSS_CHARBONNIER;
   o->ssimLambda = 0.2f;
@@ -1054,10 +1080,12 @@ int gs_create(const gs_opts* o, gs_trainer_t** out) {
       h->o.precision < 0 || h->o.precision > 3 ||
       h->o.viewsPerStep < 1 || h->o.viewsPerStep > kMaxViews ||
       h->o.viewsLaunch < 0 || h->o.viewsLaunch > 1 ||
+      !raster_shape_ok(h->o.tile ? h->o.tile : kDefaultTile, h->o.batch ? h->o.batch : kDefaultBatch) ||
       !(h->o.eCut >= 0.0f && h->o.eCut <= 80.0f)) {
     delete h;
     return GS_E_ARG;
   }
+  h->shape = RasterShape{h->o.tile ? h->o.tile : kDefaultTile, h->o.batch ? h->o.batch : kDefaultBatch};
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || h->o.device >= ndev) {
     (void)hipGetLastError();
@@ -1179,10 +1207,10 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
     }
   }
   h->maxPix = (int64_t)maxViewW * maxViewH;
-  h->maxTiles = ((maxViewW + 15) / 16) * ((maxViewH + 15) / 16);
+  h->maxTiles = tiles_of(maxViewW, maxViewH, kMinTile);
   for (const CamMeta& m : h->cams) {
     h->maxPix = std::max<int64_t>(h->maxPix, (int64_t)m.c.w * m.c.h);
-    h->maxTiles = std::max(h->maxTiles, ((m.c.w + 15) / 16) * ((m.c.h + 15) / 16));
+    h->maxTiles = std::max(h->maxTiles, tiles_of(m.c.w, m.c.h, kMinTile));
   }
 
   const int64_t cap = h->cap;
@@ -1226,7 +1254,8 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   if (h->o.sortMode == GS_SORT_TILE_SEGMENTS) {
     // rows = ceil(n / seg_group(n)) <= min(kSegMaxRows, ceil(cap / kSegMinGroup)) for every n <= cap
     const int64_t rows = std::min<int64_t>(kSegMaxRows, (cap + kSegMinGroup - 1) / kSegMinGroup);
-    rc |= dalloc(h, &h->sort.segTable, (size_t)rows * h->maxTiles * 4);
+    // (the segment path takes views of at most kMaxSegTiles tiles)
+    rc |= dalloc(h, &h->sort.segTable, (size_t)rows * std::min(h->maxTiles, kMaxSegTiles) * 4);
     rc |= dalloc(h, &h->sort.visPart, (size_t)rows * 4);
     rc |= dalloc(h, &h->sort.blockStart, (size_t)(rows + 1) * 4);
     const size_t chunkBytes = (size_t)((cap + kSegChunk - 1) / kSegChunk) * 4;
@@ -1283,7 +1312,7 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
       return hipfail(h, "setup: batched-view staging", e);
     }
     h->ringPos = 0;
-    h->viewBlocks = raster_views_blocks(h->o.device);
+    h->viewBlocks = raster_views_blocks(h->o.device, h->shape);
   }
 
   std::vector<float> planes;
@@ -1351,6 +1380,155 @@ int gs_step_views_debug(gs_trainer_t* h, const int32_t* cams, int32_t count, gs_
   return step_views_impl(h, cams, count, outs);
 }
 
+namespace {
+
+void set_shape(gs_trainer* h, const RasterShape& sh) {
+  if (sh.tile == h->shape.tile && sh.batch == h->shape.batch) return;
+  h->shape = sh;
+  if (h->dVB) h->viewBlocks = raster_views_blocks(h->o.device, sh);
+}
+
+// One autotuning pass over cameras ci[0..count) at the current shape: the
+// training raster and backward of gs_step (count 1) or gs_step_views
+// (count > 1) without the update; statistics go to evalStats, the gradient
+// sums stay in the accumulators.
+int tune_pass(gs_trainer* h, const int* ci, int count, int activeDeg) {
+  if (count == 1) {
+    const Cam cam = make_cam_meta(h, h->cams[ci[0]], activeDeg);
+    const bool ssim = h->o.loss == GS_LOSS_L1_DSSIM;
+    int rc = run_raster(h, cam, ssim ? 2 : 1, h->evalStats);
+    if (rc) return rc;
+    if (ssim) {
+      RasterBufs lb{};
+      lb.img = h->img;
+      lb.gC = h->gC;
+      lb.tgt = h->targets;
+      lb.stats = h->evalStats;
+      launch_loss_l1ssim(h->st, cam, h->o.ssimLambda, lb, h->ssimScratch);
+    }
+    RasterBufs b{};
+    b.ranges = h->ranges;
+    b.tileOrder = h->lastBin.tileOrder;
+    b.ids = h->lastBin.ids;
+    b.rec = h->rec;
+    b.recFmt = h->recFmt;
+    b.T = h->Tfin;
+    b.end = h->endIdx;
+    b.gC = h->gC;
+    launch_raster_bwd(h->st, h->shape, cam, make_cut(h), b, h->o.gradAccum, h->accum,
+                      !ssim && charb_grad_bounded(cam));
+    HIPCK(hipGetLastError());
+    return GS_OK;
+  }
+  ViewBlock& vb = h->hVB[h->ringPos];
+  hipEvent_t ev = h->ringEv[(size_t)h->ringPos];
+  HIPCK(hipEventSynchronize(ev));
+  std::vector<int64_t> key = {count, h->n, 0, h->shape.tile, h->shape.batch};
+  int rc = fill_views(h, ci, count, activeDeg, h->evalStats, vb, key);
+  if (rc) return rc;
+  vb.hp = make_hp(h);
+  HIPCK(hipMemcpyAsync(h->dVB, &vb, sizeof vb, hipMemcpyHostToDevice, h->st));
+  HIPCK(hipEventRecord(ev, h->st));
+  h->ringPos = (h->ringPos + 1) % kRing;
+  return enqueue_views(h, vb, nullptr, nullptr, nullptr, h->evalStats, false);
+}
+
+}  // namespace
+
+int gs_set_raster_shape(gs_trainer_t* h, int32_t tile, int32_t batch) {
+  if (!h) return GS_E_ARG;
+  if (!raster_shape_ok(tile, batch))
+    return fail(h, GS_E_ARG, "raster shape: tile 8, 16 or 32 and batch 64, 128 or 256");
+  HIPCK(hipSetDevice(h->o.device));
+  set_shape(h, RasterShape{tile, batch});
+  return GS_OK;
+}
+
+int gs_autotune(gs_trainer_t* h, const int32_t* cams, int32_t count, int32_t reps, gs_tune_entry* out) {
+  if (!h || !cams || !out || count < 1 || reps < 1) return GS_E_ARG;
+  if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
+  HIPCK(hipSetDevice(h->o.device));
+  int ci[kMaxViews];
+  if (count > 1) {
+    if (count > (int)h->vslots.size() || !h->dVB)
+      return fail(h, GS_E_ARG, "autotune: " + std::to_string(count) + " views; opts.viewsPerStep allows 1.." +
+                                   std::to_string(std::max<size_t>(1, h->vslots.size())));
+    if (h->o.loss != GS_LOSS_CHARBONNIER || h->o.sortMode != GS_SORT_TILE_SEGMENTS || h->o.camOpt)
+      return fail(h, GS_E_ARG, "autotune of batched views needs the Charbonnier loss, the tile-segment sort "
+                               "and camOpt off");
+  }
+  for (int k = 0; k < count; k++) {
+    if (cams[k] < 0 || cams[k] >= (int)h->cams.size()) return fail(h, GS_E_ARG, "autotune: camera index out of range");
+    ci[k] = cams[k];
+  }
+  const RasterShape start = h->shape;
+  const int activeDeg = (int)std::min<int64_t>(h->shDeg, h->iter / 1000);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  HIPCK(hipEventCreate(&e0));
+  HIPCK(hipEventCreate(&e1));
+  int rc = GS_OK;
+  const int tiles[3] = {8, 16, 32}, batches[3] = {64, 128, 256};
+  for (int i = 0; i < 9 && rc == GS_OK; i++) {
+    gs_tune_entry& en = out[i];
+    memset(&en, 0, sizeof en);
+    en.tile = tiles[i / 3];
+    en.batch = batches[i % 3];
+    bool fits = true;
+    for (int k = 0; k < count && count > 1; k++)
+      fits = fits && tiles_of(h->cams[ci[k]].c.w, h->cams[ci[k]].c.h, en.tile) <= kMaxSegTiles;
+    if (!fits) {
+      en.status = GS_TUNE_TOO_MANY_TILES;
+      continue;
+    }
+    set_shape(h, RasterShape{en.tile, en.batch});
+    // warm-up pass: also gives the pass's keys and budget drops
+    if (hipMemsetAsync(h->evalStats, 0, 32, h->st) != hipSuccess) { rc = hipfail(h, "autotune", hipGetLastError()); break; }
+    rc = tune_pass(h, ci, count, activeDeg);
+    if (rc) break;
+    double st4[4];
+    uint32_t nk[kMaxViews] = {};
+    HIPCK(hipMemcpyAsync(st4, h->evalStats, 32, hipMemcpyDeviceToHost, h->st));
+    if (count == 1) HIPCK(hipMemcpyAsync(nk, h->numKeys, 4, hipMemcpyDeviceToHost, h->st));
+    else HIPCK(hipMemcpyAsync(nk, h->viewStats, 4ull * count, hipMemcpyDeviceToHost, h->st));
+    HIPCK(hipStreamSynchronize(h->st));
+    en.dropped = (int64_t)st4[3];
+    for (int k = 0; k < count; k++) en.keys += nk[k];
+    HIPCK(hipEventRecord(e0, h->st));
+    for (int r = 0; r < reps && rc == GS_OK; r++) rc = tune_pass(h, ci, count, activeDeg);
+    if (rc) break;
+    HIPCK(hipEventRecord(e1, h->st));
+    HIPCK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCK(hipEventElapsedTime(&ms, e0, e1));
+    en.ms = (double)ms / reps;
+    en.status = GS_TUNE_MEASURED;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  // the gradient sums of the passes and their statistics are discarded
+  launch_fill_u32(h->st, (uint32_t*)h->accum, 12 * h->cap, 0u);
+  for (const gs_trainer::ViewSlot& sl : h->vslots) launch_fill_u32(h->st, (uint32_t*)sl.accum, 12 * h->cap, 0u);
+  launch_fill_u32(h->st, (uint32_t*)h->evalStats, 8, 0u);
+  HIPCK(hipStreamSynchronize(h->st));
+  if (rc) {
+    set_shape(h, start);
+    return rc;
+  }
+  // eligible: measured, and dropping no more keys than the reference's 16-pixel tiles
+  const int64_t drops16 = out[5].status == GS_TUNE_MEASURED ? out[5].dropped : INT64_MAX;
+  int best = -1;
+  for (int i = 0; i < 9; i++) {
+    if (out[i].status != GS_TUNE_MEASURED) continue;
+    if (out[i].dropped > drops16) {
+      out[i].status = GS_TUNE_MORE_DROPS;
+      continue;
+    }
+    if (best < 0 || out[i].ms < out[best].ms) best = i;
+  }
+  set_shape(h, best >= 0 ? RasterShape{out[best].tile, out[best].batch} : start);
+  return GS_OK;
+}
+
 int gs_step_batch(gs_trainer_t* h, const int32_t* cams, int32_t count) {
   if (!h || count < 0 || (count > 0 && !cams)) return GS_E_ARG;
   for (int32_t k = 0; k < count; k++) {
@@ -1388,7 +1566,7 @@ int gs_read_loss(gs_trainer_t* h, double* mse, int32_t* has) {
 int gs_render(gs_trainer_t* h, const gs_camera* c, int32_t trainMode, int64_t offset, float* rgba) {
   if (!h || !c || !rgba) return GS_E_ARG;
   if (!h->ready) return fail(h, GS_E_ARG, "setup() has not been called");
-  if ((int64_t)c->w * c->h > h->maxPix || ((c->w + 15) / 16) * ((c->h + 15) / 16) > h->maxTiles)
+  if ((int64_t)c->w * c->h > h->maxPix || tiles_of(c->w, c->h, h->shape.tile) > h->maxTiles)
     return fail(h, GS_E_LIMIT, "renderView: view larger than maxViewW x maxViewH");
   if (c->w <= 0 || c->h <= 0) return fail(h, GS_E_ARG, "renderView: empty view");
   if (trainMode && (offset < 0 || offset + (int64_t)c->w * c->h > h->totalPx))
@@ -1684,6 +1862,8 @@ int gs_get_info(gs_trainer_t* h, gs_info* out) {
   out->entriesCap = h->entriesCap;
   out->maxTiles = h->maxTiles;
   out->horizon = (float)h->horizon;
+  out->tile = h->shape.tile;
+  out->batch = h->shape.batch;
   return GS_OK;
 }
 

@@ -154,6 +154,20 @@ export class GSTrainer {
     this._iter += cams.length;
   }
 
+  /** Raster shape (no reference counterpart): tile side 8, 16 or 32 pixels
+   *  and 64, 128 or 256 keys staged in LDS per round (opts.tile, opts.batch) */
+  setRasterShape(tile, batch) {
+    this._call(() => addon.setRasterShape(this._h, tile, batch));
+  }
+
+  /** Autotuner (no reference counterpart): times the 9 raster shapes on
+   *  the views `cams` (one batched pass when there are several) and keeps
+   *  the fastest eligible one; returns one entry per shape
+   *  {tile, batch, status, ms, keys, dropped}. Synchronises. */
+  autotune(cams, reps = 5) {
+    return this._call(() => addon.autotune(this._h, cams, reps));
+  }
+
   _syncCamMeta() {
     for (let ci = 0; ci < this.camMeta.length; ci++) {
       const c = addon.getCamera(this._h, ci);

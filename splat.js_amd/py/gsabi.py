@@ -30,7 +30,7 @@ class GsOpts(C.Structure):
                 ("ssimLambda", C.c_float), ("gradAccum", C.c_int32), ("sortMode", C.c_int32),
                 ("seed", C.c_uint32), ("device", C.c_int32), ("precision", C.c_int32),
                 ("camOpt", C.c_int32), ("expComp", C.c_int32), ("viewsPerStep", C.c_int32),
-                ("graph", C.c_int32), ("viewsLaunch", C.c_int32)]
+                ("graph", C.c_int32), ("viewsLaunch", C.c_int32), ("tile", C.c_int32), ("batch", C.c_int32)]
 
 
 class GsCamera(C.Structure):
@@ -52,7 +52,7 @@ class GsInfo(C.Structure):
     _fields_ = [("n", C.c_int64), ("cap", C.c_int64), ("iter", C.c_int64), ("shDeg", C.c_int32),
                 ("shK", C.c_int32), ("stride", C.c_int32), ("numCams", C.c_int32),
                 ("lastCam", C.c_int32), ("holdout", C.c_int32), ("entriesCap", C.c_int64), ("maxTiles", C.c_int32),
-                ("horizon", C.c_float)]
+                ("horizon", C.c_float), ("tile", C.c_int32), ("batch", C.c_int32)]
 
 
 class GsStepStats(C.Structure):
@@ -67,6 +67,14 @@ class GsDebugOut(C.Structure):
     _fields_ = [(name, C.c_void_p) for name in
                 ("rec", "depthKey", "rect", "cnt", "ids", "keys", "numKeys", "tileRanges", "img", "T",
                  "end", "gC", "accum", "gradF", "shGrad", "stats")]
+
+
+GS_TUNE_MEASURED, GS_TUNE_TOO_MANY_TILES, GS_TUNE_MORE_DROPS = 0, 1, 2
+
+
+class GsTuneEntry(C.Structure):
+    _fields_ = [("tile", C.c_int32), ("batch", C.c_int32), ("status", C.c_int32), ("pad", C.c_int32),
+                ("ms", C.c_double), ("keys", C.c_int64), ("dropped", C.c_int64)]
 
 
 class GsKernelTime(C.Structure):
@@ -113,6 +121,8 @@ SIGNATURES = {
     "gs_profile": ([C.c_void_p, C.c_int32], C.c_int),
     "gs_profile_read": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p], C.c_int),
     "gs_stream": ([C.c_void_p], C.c_void_p),
+    "gs_set_raster_shape": ([C.c_void_p, C.c_int32, C.c_int32], C.c_int),
+    "gs_autotune": ([C.c_void_p, C.c_void_p, C.c_int32, C.c_int32, C.c_void_p], C.c_int),
 }
 
 _lib = None
@@ -261,7 +271,8 @@ class GSTrainer:
         for k, ci in enumerate(a):
             m = self.camMeta[int(ci)] if ci >= 0 else None
             npx = m["w"] * m["h"] if m else 0
-            nt = ((m["w"] + 15) // 16) * ((m["h"] + 15) // 16) if m else 0
+            T = info.tile
+            nt = ((m["w"] + T - 1) // T) * ((m["h"] + T - 1) // T) if m else 0
             shapes = {"img": ((npx, 4), np.float32), "T": ((npx,), np.float32), "end": ((npx,), np.uint32),
                       "gC": ((npx, 4), np.float32), "accum": ((n, 10), np.int32),
                       "tileRanges": ((nt, 2), np.uint32)}
@@ -293,6 +304,20 @@ class GSTrainer:
         out = np.zeros(g.size, np.float64)
         self._ck(lib().gs_fd_grad64(self._h, int(cam), _p(g), _p(sl), _p(h), int(g.size), _p(out)))
         return out
+
+    def setRasterShape(self, tile, batch):
+        """Raster tile side (8, 16, 32) and LDS batch (64, 128, 256)."""
+        self._ck(lib().gs_set_raster_shape(self._h, int(tile), int(batch)))
+
+    def autotune(self, cams, reps=5):
+        """Times the 9 raster shapes on cameras `cams` (one batched pass
+        when there are several) and sets the fastest eligible one. Returns
+        one dict per shape: tile, batch, status, ms, keys, dropped."""
+        a = np.ascontiguousarray(cams, np.int32)
+        out = (GsTuneEntry * 9)()
+        self._ck(lib().gs_autotune(self._h, _p(a), int(a.size), int(reps), out))
+        return [{"tile": e.tile, "batch": e.batch, "status": e.status, "ms": e.ms, "keys": e.keys,
+                 "dropped": e.dropped} for e in out]
 
     def readLoss(self):
         mse = C.c_double()
@@ -432,7 +457,8 @@ class GSTrainer:
         n = info.n
         m = self.camMeta[cam]
         npx = m["w"] * m["h"]
-        nt = ((m["w"] + 15) // 16) * ((m["h"] + 15) // 16)
+        T = info.tile
+        nt = ((m["w"] + T - 1) // T) * ((m["h"] + T - 1) // T)
         cap = info.entriesCap
         shapes = {"rec": ((n, 16), np.float32), "depthKey": ((n,), np.uint32), "rect": ((n, 4), np.uint32),
                   "cnt": ((n,), np.uint32), "ids": ((cap,), np.uint32), "keys": ((cap,), np.uint64),

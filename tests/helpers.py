@@ -28,12 +28,14 @@ def adam_for_iter(it, radius, posLr=0.0, horizon=60000, minScale=1e-4, opacityRe
 class OracleTrainer:
     """The f32 restatement driven like GSTrainer.stepOnce on a fixed camera."""
 
-    def __init__(self, params, sh, cams, images, radius, opts=None, shDeg=2, fixed=True, precision="f32"):
+    def __init__(self, params, sh, cams, images, radius, opts=None, shDeg=2, fixed=True, precision="f32",
+                 tile=16):
         opts = dict(opts or {})
         self.o = O.make_opts(eCut=opts.get("eCut", 4.5), aMin=opts.get("aMin", 0.0039),
                              radClamp=opts.get("radClamp", 1.0), anisoReg=opts.get("anisoReg", 0.005),
                              shDeg=shDeg)
         self.opts = opts
+        self.tile = tile      # raster tile side in pixels
         self.st = O.Stage(self.o, "f32")
         self.P = np.array(params, np.float32).reshape(-1, 16)
         n = self.P.shape[0]
@@ -120,7 +122,7 @@ class OracleTrainer:
         seen = np.zeros(n, bool)
         per = []
         for ci in cis:
-            cam = O.make_cam(self.cams[ci], shActiveDeg=deg)
+            cam = O.make_cam(self.cams[ci], shActiveDeg=deg, tile=self.tile)
             proj, rect, cnt = st.project(cam, self.P, self.SH if self.K else None)
             proj = self._round_record(proj)
             ids, ts, keys = st.bin_sort(cam, proj, rect, cnt)
@@ -165,7 +167,7 @@ class OracleTrainer:
         deg = min(self.shDeg, self.iter // 1000)
         self.cam_deg[ci] = deg
         cd = self.camopt.cam_dict(ci) if self.camopt else self.cams[ci]
-        cam = O.make_cam(cd, shActiveDeg=deg)
+        cam = O.make_cam(cd, shActiveDeg=deg, tile=self.tile)
         self._iter += 1
         self.adamT += 1
         st = self.st
@@ -201,7 +203,7 @@ class OracleTrainer:
         an optional pose override, no update. Returns (psnr, the camera's 8
         gradient slots); all rows are drained afterwards."""
         src = self.camopt.cam_dict(ci, override) if self.camopt else dict(self.cams[ci], **(override or {}))
-        cam = O.make_cam(src, shActiveDeg=self.shDeg)
+        cam = O.make_cam(src, shActiveDeg=self.shDeg, tile=self.tile)
         st = self.st
         proj, rect, cnt = st.project(cam, self.P, self.SH if self.K else None)
         proj = self._round_record(proj)

@@ -43,6 +43,7 @@ def test_default_opts_are_the_references():
     assert o.maxIters == 60000 and o.maxSplats == 0 and o.entriesCap == 0 and o.capMult == 4.0
     assert o.loss == gsabi.GS_LOSS_CHARBONNIER and o.gradAccum == gsabi.GS_ACCUM_FIXED
     assert o.camOpt == 0 and o.expComp == 0          # trainer.js:468, :511: both off by default
+    assert o.tile == 16 and o.batch == 256             # shaders.js:32 TILE
 
 
 def test_struct_layouts_match_the_header(tmp_path):
@@ -60,6 +61,7 @@ def test_struct_layouts_match_the_header(tmp_path):
         "gs_cam_state": [f[0] for f in gsabi.GsCamState._fields_],
         "gs_step_stats": [f[0] for f in gsabi.GsStepStats._fields_],
         "gs_readback_result": [f[0] for f in gsabi.GsReadbackResult._fields_],
+        "gs_tune_entry": [f[0] for f in gsabi.GsTuneEntry._fields_],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for s, fs in fields.items():
@@ -75,7 +77,7 @@ def test_struct_layouts_match_the_header(tmp_path):
           "gs_info": gsabi.GsInfo, "gs_debug_out": gsabi.GsDebugOut, "gs_kernel_time": gsabi.GsKernelTime,
           "gs_refine_opts": gsabi.GsRefineOpts, "gs_refine_result": gsabi.GsRefineResult,
           "gs_cam_state": gsabi.GsCamState, "gs_step_stats": gsabi.GsStepStats,
-          "gs_readback_result": gsabi.GsReadbackResult}
+          "gs_readback_result": gsabi.GsReadbackResult, "gs_tune_entry": gsabi.GsTuneEntry}
     for s, cls in py.items():
         assert int(got[s]) == C.sizeof(cls), s
         for f, _ in cls._fields_:

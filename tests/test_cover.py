@@ -66,9 +66,11 @@ int main(int argc, char** argv) {
       else { empty++; if (!hit) empty_rejected++; }
     }
   }
-  /* the tile form used by the binning pass: 16x16 tiles, 4 block bits */
+  /* the tile form used by the binning pass: T x T tiles (T = 8, 16, 32 in
+     turn), 4 quadrant bits */
   long tblocks = 0, tunsafe = 0, ttakes = 0, tempty = 0, tempty_rejected = 0;
   for (long it = 0; it < cases / 4; it++) {
+    const int T = 8 << (it % 3), hq = T / 2;
     const float va = (float)pow(10.0, -4.0 + 10.0 * urand());
     const float vc = (float)pow(10.0, -4.0 + 10.0 * urand());
     const double rho = (urand() < 0.5 ? -1.0 : 1.0) * (1.0 - pow(10.0, -8.0 * urand()));
@@ -82,20 +84,20 @@ int main(int argc, char** argv) {
     const float mx = (float)(-200.0 + 2400.0 * urand()), my = (float)(-200.0 + 1400.0 * urand());
     const double exx = sqrt(2.0 * L0 * (double)ad) * 1.3 + 24.0, eyy = sqrt(2.0 * L0 * (double)cd) * 1.3 + 24.0;
     for (int b = 0; b < 4; b++) {
-      const int tx = (int)floor((mx + (2.0 * urand() - 1.0) * exx) / 16.0);
-      const int ty = (int)floor((my + (2.0 * urand() - 1.0) * eyy) / 16.0);
+      const int tx = (int)floor((mx + (2.0 * urand() - 1.0) * exx) / (double)T);
+      const int ty = (int)floor((my + (2.0 * urand() - 1.0) * eyy) / (double)T);
       float emin[4] = {INFINITY, INFINITY, INFINITY, INFINITY};
-      for (int j = 0; j < 16; j++)
-        for (int i = 0; i < 16; i++) {
-          const float e = pixel_e((float)(tx * 16 + i) + 0.5f, (float)(ty * 16 + j) + 0.5f, mx, my, A, B, C);
-          const int blk = (i >> 3) + 2 * (j >> 3);
+      for (int j = 0; j < T; j++)
+        for (int i = 0; i < T; i++) {
+          const float e = pixel_e((float)(tx * T + i) + 0.5f, (float)(ty * T + j) + 0.5f, mx, my, A, B, C);
+          const int blk = (i / hq) + 2 * (j / hq);
           emin[blk] = fminf(emin[blk], e);
         }
       /* every other tile is borderline: L = smallest exponent of one block */
       const float L = (b & 1) ? L0 : emin[(int)(urand() * 4.0) & 3];
       if (!(L > 0.0f) || L == INFINITY) continue;
       const gs_cover_splat sp = gs_cover_prepare(mx, my, A, B, C, L);
-      const uint32_t m = gs_cover_tile_mask(&sp, tx, ty);
+      const uint32_t m = gs_cover_tile_mask(&sp, tx, ty, T);
       for (int blk = 0; blk < 4; blk++) {
         tblocks++;
         const int any = emin[blk] <= L, hit = (m >> blk) & 1u;

@@ -150,3 +150,20 @@ def test_config2_full_size_f32_step_bit_exact():
     images, W, H = _targets(gp, gsh, cams, 2, n)
     _run_pair(p, sh, cams, images, W, H, 8.0, {"posLr": 1.6e-4}, 2, steps=1, start_iter=2000,
               precision=gsabi.GS_PREC_F32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile,batch", [(8, 128), (32, 64)])
+def test_config2_full_size_step_bit_exact_at_other_tiles(tile, batch):
+    """The config-2 f32 step at 8- and 32-pixel tiles: bit-exact against the
+    restatement at the same tile size. At 8-pixel tiles the 1600x900 view has
+    22,600 tiles, past the tile-segment binning's 16,384, so this step also
+    covers the depth-then-tile radix path the trainer falls back to."""
+    n = 1_000_000
+    p, sh, cams = scenes.config2(n=n, seed=0)
+    gp, gsh, _ = scenes.config2(n=n, seed=1)
+    cams = [dict(cams[0], imgIdx=0)]
+    images, W, H = _targets(gp, gsh, cams, 2, n)
+    print(f"config 2 at tile {tile}, batch {batch}", flush=True)
+    _run_pair(p, sh, cams, images, W, H, 8.0, {"posLr": 1.6e-4}, 2, steps=1, start_iter=2000,
+              precision=gsabi.GS_PREC_F32, tile=tile, batch=batch)

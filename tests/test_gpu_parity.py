@@ -96,7 +96,8 @@ def _run_pair(params, sh, cams, images, W, H, radius, opts, shDeg, steps, start_
     gpu = _trainer(params, sh, cams, images, W, H, radius, opts, shDeg=shDeg, **kw)
     prec = {gsabi.GS_PREC_MIXED: "mixed", gsabi.GS_PREC_BF16: "bf16",
             gsabi.GS_PREC_FP16: "fp16"}.get(kw.get("precision", gsabi.GS_PREC_F32), "f32")
-    ref = OracleTrainer(params, sh, cams, images, radius, opts=opts, shDeg=shDeg, precision=prec)
+    ref = OracleTrainer(params, sh, cams, images, radius, opts=opts, shDeg=shDeg, precision=prec,
+                        tile=kw.get("tile", 16))
     gpu.iter = start_iter
     ref.iter = start_iter
     n_tiles = ((W + 15) // 16) * ((H + 15) // 16)
@@ -133,6 +134,19 @@ def test_config1_bit_exact():
     p, sh, cams, images, W, H, radius = config1_setup()
     _run_pair(p, sh, cams, images, W, H, radius, {}, 2, steps=1, start_iter=0)
     _run_pair(p, sh, cams, images, W, H, radius, {}, 2, steps=2, start_iter=2500)
+
+
+@pytest.mark.parametrize("tile,batch", [(8, 64), (8, 128), (8, 256), (16, 64), (16, 128), (32, 64), (32, 128),
+                                        (32, 256)])
+def test_config1_bit_exact_at_every_raster_shape(tile, batch):
+    """Config 1 at the other raster shapes (tiles of 8 and 32 pixels, LDS
+    batches of 64 and 128 keys) against the restatement run at the same tile
+    size: the tile grid changes the rectangles, keys and ranges, the batch
+    changes nothing; every intermediate and the updated params stay
+    bit-exact."""
+    print(f"config 1 at tile {tile}, batch {batch}", flush=True)
+    p, sh, cams, images, W, H, radius = config1_setup()
+    _run_pair(p, sh, cams, images, W, H, radius, {}, 2, steps=2, start_iter=2500, tile=tile, batch=batch)
 
 
 def test_spatially_ordered_input_bit_exact():

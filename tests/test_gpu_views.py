@@ -5,7 +5,8 @@ views, one Adam step; replayed as a hipGraph when the step's shape repeats.
 * Bit-exact against the restatement's batched step (helpers.OracleTrainer.
   step_views) on uneven view sizes: every view's image, T, end, dL/dC and
   fixed-point sums, the summed gradients, and the params and SH after Adam,
-  over several steps, in f32 and 16-bit storage.
+  over several steps, in f32 and 16-bit storage, at the default raster shape
+  and at 8- and 32-pixel tiles with smaller LDS batches.
 * Graph replay equals eager execution, step after step, with single-view
   steps, renders and readbacks in between.
 * At full size (config 4: 1M Gaussians, 8 views from 800x450 to 1920x1080,
@@ -44,13 +45,15 @@ def _bits(a):
     return np.ascontiguousarray(a, np.float32).view(np.uint32)
 
 
-@pytest.mark.parametrize("precision", ["f32", "bf16"])
-def test_views_bit_exact_against_restatement(precision):
+@pytest.mark.parametrize("precision,tile,batch", [("f32", 16, 256), ("bf16", 16, 256), ("f32", 8, 128),
+                                                  ("bf16", 32, 64)])
+def test_views_bit_exact_against_restatement(precision, tile, batch):
+    print(f"batched views: {precision}, tile {tile}, batch {batch}", flush=True)
     p, sh, cams, images = _scene()
     prec = {"f32": gsabi.GS_PREC_F32, "bf16": gsabi.GS_PREC_BF16}[precision]
-    t = gsabi.GSTrainer(shDeg=2, maxSplats=p.shape[0], viewsPerStep=4, precision=prec)
+    t = gsabi.GSTrainer(shDeg=2, maxSplats=p.shape[0], viewsPerStep=4, precision=prec, tile=tile, batch=batch)
     t.setup({"data": p, "n": p.shape[0], "sh": sh}, cams, images, 200, 120, 2.0)
-    ref = OracleTrainer(p, sh, cams, images, 2.0, shDeg=2, precision=precision)
+    ref = OracleTrainer(p, sh, cams, images, 2.0, shDeg=2, precision=precision, tile=tile)
     t.iter = ref.iter = 1996      # the second step starts at 2000: all SH bands from there
     for s, order in enumerate(([0, 1, 2, 3], [2, 0, 3], [3, 1, 0, 2])):
         print(f"batched step {s}: views {order}", flush=True)
