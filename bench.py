@@ -4,7 +4,7 @@ Workload (the benchmark's single-view configuration): 1M Gaussians (means
 U(-5,5)^3, log-scales N(-4,0.5), random unit quaternions, opacity logits
 N(0,1), DC U(0,1) as logits, SH degree 2 rest N(0,0.05)), a ring of 64 views
 at radius 8 looking at the origin, 1600x900, f=1100. Targets are renders of a
-second seeded 1M-Gaussian set. A training iteration = projection, 16x16 tile
+second seeded 1M-Gaussian set. A training iteration = projection, tile
 binning + per-tile depth sort, raster forward + Charbonnier loss, raster
 backward and chain for one view picked by the seeded camera RNG; Adam
 (position lr 1.6e-4) after every view in the single-view step, once per
@@ -21,6 +21,11 @@ Runs, all from the same start on the same workload:
                persistent launch each, one Adam step, replayed as a hipGraph
                (x8o: each view's raster on a second stream beside the binning of
                the next views instead)
+Before timing, each run's trainer picks its raster shape (tile side 8, 16 or
+32 pixels x LDS batch of 64, 128 or 256 keys) with its autotuner on views
+0.. of the ring (--no-autotune keeps 16 x 256); config.runs records the
+choice and the sweep's pass times. --sweep runs the autotuner on config 4
+instead (each of its 8 view sizes alone, then one 8-view batch).
 `value` is the batched bf16 step's training iterations per second: views
 trained per second (the trainer's iteration counter advances by one per view,
 as in the reference's stepOnce). `config.runs` holds every run's rate, and
@@ -111,11 +116,24 @@ def _trainer(p, sh, cams, images, W, H, n, prec, views=1, graph=1, launch=gsabi.
     return t
 
 
+def autotune(t, views, args):
+    """The trainer's raster shape from its autotuner (gs_autotune) on
+    `views`, or the default 16 x 256 with --no-autotune."""
+    if args.no_autotune:
+        return {"tile": 16, "batch": 256, "autotuned": False}
+    res = t.autotune(views, reps=3)
+    info = t.info()
+    ms = {f"{e['tile']}x{e['batch']}": round(e["ms"], 3) for e in res if e["status"] == gsabi.GS_TUNE_MEASURED}
+    log(f"autotune over views {views}: {ms} -> tile {info.tile}, batch {info.batch}")
+    return {"tile": info.tile, "batch": info.batch, "autotuned": True, "pass_ms": ms}
+
+
 def run_single(prec, p, sh, cams, images, W, H, args):
     """Warm up, time args.steps single-view steps with per-stage HIP events
     on the trainer's stream, then sample key and visible counts over 16 more
     steps (they vary by view)."""
     t = _trainer(p, sh, cams, images, W, H, args.n, prec)
+    shape = autotune(t, [0], args)
     for _ in range(args.warmup):
         t.stepOnce()
     t.sync()
@@ -137,7 +155,7 @@ def run_single(prec, p, sh, cams, images, W, H, args):
     t.destroy()
     log(f"{prec}: {args.steps / dt:.1f} iters/s")
     return {"its": args.steps / dt, "ms_per_iter": 1e3 * dt / args.steps, "prof": prof,
-            "keys": float(np.mean(keys)), "vis": float(np.mean(vis))}
+            "keys": float(np.mean(keys)), "vis": float(np.mean(vis)), "shape": shape}
 
 
 def run_batched(prec, p, sh, cams, images, W, H, args, launch=gsabi.GS_VIEWS_PERSISTENT):
@@ -146,6 +164,7 @@ def run_batched(prec, p, sh, cams, images, W, H, args, launch=gsabi.GS_VIEWS_PER
     profiled pass (eager persistent launches with per-stage HIP events, one
     stream) and key counts."""
     t = _trainer(p, sh, cams, images, W, H, args.n, prec, views=BATCH, launch=launch)
+    shape = autotune(t, list(range(BATCH)), args)
     picks = [-1] * BATCH
     for _ in range(max(args.warmup, 2)):
         t.stepViews(picks)
@@ -172,7 +191,7 @@ def run_batched(prec, p, sh, cams, images, W, H, args, launch=gsabi.GS_VIEWS_PER
     form = "overlapped" if launch == gsabi.GS_VIEWS_OVERLAP else "persistent"
     log(f"{prec} x{BATCH} {form}: {its:.1f} iters/s ({args.steps / dt:.1f} batched steps/s)")
     return {"its": its, "ms_per_iter": 1e3 * dt / (args.steps * BATCH), "steps_per_s": args.steps / dt,
-            "prof": prof, "keys": float(np.mean(keys)), "vis": float(np.mean(vis))}
+            "prof": prof, "keys": float(np.mean(keys)), "vis": float(np.mean(vis)), "shape": shape}
 
 
 def psnr_after(prec, p, sh, cams, images, W, H, n, iters, batched):
@@ -254,6 +273,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--config5", action="store_true",
                     help="instead: the stress configuration (3M Gaussians, SH 3, 1920x1080, 4-view batches)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default raster shape (16-pixel tiles, 256-key LDS batches)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="instead: autotuner sweeps of config 4 (each view size alone, then one 8-view batch)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one plain 1 GiB device-to-device copy (scales HBM byte counters)")
     args = ap.parse_args()
@@ -264,6 +287,9 @@ def main():
 
     if args.config5:
         run_config5(args)
+        return
+    if args.sweep:
+        run_sweep(args)
         return
     W, H, f = args.width, args.height, 1100.0 * args.width / 1600.0
     p, sh, cams, images = build_workload(args.n, args.views, W, H, f)
@@ -339,7 +365,7 @@ def main():
               "kernels": kernels, "hbm_copy_ceiling_GBps": copy_gbps,
               "runs": {k: {"iters_per_s": round(v["its"], 2), "ms_per_iter": round(v["ms_per_iter"], 4),
                            **({"batched_steps_per_s": round(v["steps_per_s"], 2)} if "steps_per_s" in v else {}),
-                           "storage": storage[k.split("x8")[0]],
+                           "storage": storage[k.split("x8")[0]], "raster_shape": v["shape"],
                            **({"launch": "overlapped" if k.endswith("x8o") else "persistent"} if "x8" in k else {})}
                        for k, v in runs.items()},
               "psnr": psnr, "psnr_protocol": f"mean PSNR over all {args.views} views after {args.psnr_iters} "
@@ -360,6 +386,30 @@ def main():
     print(json.dumps(line), flush=True)
     if args.copy_calibration:
         copy_calibration()
+
+
+def run_sweep(args):
+    """Autotuner sweeps on config 4 (1M Gaussians, 8 views from 800x450 to
+    1920x1080, bf16 storage): each view size alone (single-view passes),
+    then the 8 views as one batched pass. One JSON line per sweep."""
+    p, sh, cams = scenes.config4(n=args.n, seed=0)
+    images = [{"tw": c["w"], "th": c["h"], "rgb": np.full(c["w"] * c["h"] * 3, 0.4, np.float32)} for c in cams]
+    maxW, maxH = max(c["w"] for c in cams), max(c["h"] for c in cams)
+    t = gsabi.GSTrainer(shDeg=2, maxSplats=args.n, precision=gsabi.GS_PREC_BF16, viewsPerStep=BATCH)
+    t.setup({"data": p, "n": args.n, "sh": sh}, cams, images, maxW, maxH, 8.0)
+    t.iter = 2500
+    status = {gsabi.GS_TUNE_MEASURED: "measured", gsabi.GS_TUNE_TOO_MANY_TILES: "too many tiles",
+              gsabi.GS_TUNE_MORE_DROPS: "more budget drops"}
+    for views in [[k] for k in range(len(cams))] + [list(range(len(cams)))]:
+        res = t.autotune(views, reps=5)
+        info = t.info()
+        label = (f"{cams[views[0]]['w']}x{cams[views[0]]['h']} single view" if len(views) == 1
+                 else f"{len(views)}-view batch")
+        print(json.dumps({"sweep": label, "best": f"{info.tile}x{info.batch}",
+                          "shapes": {f"{e['tile']}x{e['batch']}": {"ms": round(e["ms"], 4), "keys": e["keys"],
+                                                                    "status": status[e["status"]]}
+                                     for e in res}}), flush=True)
+    t.destroy()
 
 
 def device_used_bytes():
