@@ -166,7 +166,7 @@ __device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, 
   const bool inside = x < cam.W && y < cam.H;
   const float px = (float)x + 0.5f, py = (float)y + 0.5f;
   const uint2 rg = b.ranges[tile];
-  const uint32_t s = rg.x, e = rg.y;
+  const uint32_t s = __builtin_amdgcn_readfirstlane(rg.x), e = __builtin_amdgcn_readfirstlane(rg.y);
   float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
   uint32_t end = s;
   bool done = !inside;
@@ -391,8 +391,13 @@ __device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, 
   const int x = pm.x, y = pm.y;
   const bool inside = x < cam.W && y < cam.H;
   const float px = (float)x + 0.5f, py = (float)y + 0.5f;
+  // the tile's range and the block's largest end are the same in every
+  // lane; read through readfirstlane so the compiler keeps the key indices
+  // and the visit masks derived from them in scalar registers (as LDS and
+  // global loads they would count as per-lane values, and every mask
+  // operation of the visit loop would run on the VALU)
   const uint2 rg = b.ranges[tile];
-  const uint32_t s = rg.x;
+  const uint32_t s = __builtin_amdgcn_readfirstlane(rg.x);
   const int64_t pi = (int64_t)y * cam.W + x;
   const int mySlot = reduce10_slot();
   const bool flusher = (__lane_id() & 3) == 0 && mySlot >= 0;
@@ -412,7 +417,7 @@ __device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, 
   __syncthreads();
   if (myEnd > s) atomicMax(&sEnd, myEnd);
   __syncthreads();
-  const uint32_t endMax = sEnd;
+  const uint32_t endMax = __builtin_amdgcn_readfirstlane(sEnd);
   // this wave's largest end: keys at or past it are past every lane's end,
   // so the wave leaves them out of its visit set (they would add nothing)
   uint32_t we = myEnd;
