@@ -91,20 +91,21 @@ def algorithmic_bytes(name, n, vis, keys, npx, shC, prec, views=1):
     update    n*(48 sums + 4 count + 2*56 params + 2*2*14*mb m/v) + n*(2*sb + 2*2*mb)*shC SH and its m/v
               (the batched update reads every view's sums and counts)
     bin_sort  n*(16 count pass + 40 scatter pass) + keys*(8 scattered key + 8 sort read + 4 sorted id)
-    keys, vis and npx are per view; views > 1 for the batched launches."""
+    keys, vis and npx are per view; views > 1 for the batched launches (*_views:
+    one launch covers every view of the step)."""
     sb = 4.0 if prec == "f32" else 2.0
     mb = 2.0 if prec == "mixed" else 4.0
     rec = 32.0 if prec in ("bf16", "fp16") else 48.0
-    if name == "project":
-        return n * 72.0 + vis * (sb * shC + 48.0)
+    if name in ("project", "project_views"):
+        return views * (n * 72.0 + vis * (sb * shC + 48.0))
     if name in ("raster_fwd_loss", "raster_fwd_views"):
         return views * (keys * (4.0 + rec) + npx * 44.0)
     if name in ("raster_bwd", "raster_bwd_views"):
         return views * (keys * (20.0 + rec) + npx * 24.0 + vis * 40.0)
     if name in ("update", "update_views"):
         return n * (52.0 * views + 112.0 + 4.0 * 14.0 * mb) + n * (2.0 * sb + 4.0 * mb) * shC
-    if name == "bin_sort":
-        return n * 56.0 + keys * 20.0
+    if name in ("bin_sort", "bin_sort_views"):
+        return views * (n * 56.0 + keys * 20.0)
     return None
 
 

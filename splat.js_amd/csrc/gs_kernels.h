@@ -47,6 +47,7 @@ constexpr int64_t kSegBlocks = 256;
 // one per Gaussian)
 constexpr int64_t kSegChunk = 256;
 int64_t seg_group(int64_t n);
+int seg_blocks(int64_t n);   // count/scatter blocks of the tile-segment binning
 
 struct BinArgs {
   int mode;               // 0 depth-then-tile, 1 global64, 2 tile segments
@@ -71,6 +72,40 @@ struct BinArgs {
   double* stats;          // [3] += dropped keys
   SortScratch scratch;
 };
+
+// One view's tile-segment binning: its inputs (the projection's outputs),
+// its scratch and its outputs. The single-view path passes one by value;
+// the batched step keeps one per view in device memory and runs every
+// binning stage of all views in one launch (the view from the block index).
+struct SegJob {
+  int64_t n;              // Gaussians
+  uint64_t cap;           // key budget
+  int nb;                 // count/scatter blocks (ceil(n / seg_group(n)))
+  int tilesX, tile, numTiles;
+  int recFmt;
+  float4* rec;            // raster records (the projection's output)
+  uint32_t* depthKey;
+  uint2* rect;
+  uint32_t* cnt;
+  uint32_t* chunkKeys;    // keys per chunk of Gaussians (the projection adds, the scan clears)
+  uint32_t* blockStart;   // nb + 1
+  uint32_t* table;        // nb x numTiles
+  uint32_t* visPart;      // nb
+  uint32_t* tileCnt;      // numTiles
+  uint2* ranges;          // numTiles
+  uint32_t* tileOrder;    // numTiles
+  uint32_t* numKeys;
+  uint32_t* numVisible;
+  uint64_t* totalKeys;
+  double* stats;          // [3] += dropped keys (atomically: views run together)
+  uint64_t* keys;         // cap: the scattered keys
+  uint64_t* tmp;          // cap: merge scratch of segments over 1024 keys
+  uint32_t* ids;          // cap: the raster values in sorted order
+};
+
+// All views' binning stages, one launch each; hJobs: the host copy of the
+// device array dJobs (for the launch shapes).
+void launch_bin_sort_views(hipStream_t st, const SegJob* dJobs, const SegJob* hJobs, int V);
 
 struct BinOut {
   const void* keys;       // null for tile segments (a key's tile is the range holding it)
@@ -114,6 +149,7 @@ constexpr int kMaxViews = 16;
 // constants.
 struct ViewBlock {
   int count;                       // views V <= kMaxViews
+  SegJob seg[kMaxViews];           // each view's binning (batched binning only)
   uint32_t off[kMaxViews + 1];     // first work item (tile) of each view; off[V] = all
   Cam cams[kMaxViews];
   RasterBufs bufs[kMaxViews];
@@ -159,6 +195,9 @@ void launch_update(hipStream_t st, int deg, int accumMode, int prec, const Cam& 
 // view's tiles (queue: a counter zeroed before each launch; blocks from
 // raster_views_blocks); one update that chains each view's sums through
 // its camera, adds the views' gradients and takes one Adam step (vb->hp).
+// every view's projection in one launch, into vb->seg[v]'s buffers
+void launch_project_views(hipStream_t st, int deg, int prec, int64_t n, int64_t cap, const ViewBlock* vb, int V,
+                          const Cut& cut, const float4* P, const void* SH);
 void launch_project_view(hipStream_t st, int deg, int prec, int64_t n, int64_t cap, const ViewBlock* vb, int v,
                          const Cut& cut, const float4* P, const void* SH, float4* rec, uint32_t* depthKey,
                          uint2* rect, uint32_t* cnt, uint32_t* chunkKeys);

@@ -148,6 +148,41 @@ __global__ __launch_bounds__(kBlock) void k_project_view(int64_t n, int64_t cap,
   project_thread<kDeg, kPrec>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys);
 }
 
+// every view of a batched step in one launch: view blockIdx.y, its camera
+// from vb->cams and its buffers from vb->seg
+template <int kDeg, int kPrec>
+__global__ __launch_bounds__(kBlock) void k_project_views(int64_t n, int64_t cap, const ViewBlock* __restrict__ vb,
+                                                          Cut cut, const float4* __restrict__ P,
+                                                          const void* __restrict__ SH) {
+  const int v = blockIdx.y;
+  const Cam cam = vb->cams[v];
+  const SegJob& j = vb->seg[v];
+  project_thread<kDeg, kPrec>(n, cap, cam, cut, P, SH, j.rec, j.depthKey, j.rect, j.cnt, j.chunkKeys);
+}
+
+template <int kPrec>
+static void launch_project_views_t(hipStream_t st, int deg, int64_t n, int64_t cap, const ViewBlock* vb, int V,
+                                   const Cut& cut, const float4* P, const void* SH) {
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock), (unsigned)V);
+  switch (deg) {
+    case 0: k_project_views<0, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, cut, P, SH); break;
+    case 1: k_project_views<1, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, cut, P, SH); break;
+    case 2: k_project_views<2, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, cut, P, SH); break;
+    default: k_project_views<3, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, cut, P, SH); break;
+  }
+}
+
+void launch_project_views(hipStream_t st, int deg, int prec, int64_t n, int64_t cap, const ViewBlock* vb, int V,
+                          const Cut& cut, const float4* P, const void* SH) {
+  if (n <= 0 || V <= 0) return;
+  switch (prec) {
+    case 0: launch_project_views_t<0>(st, deg, n, cap, vb, V, cut, P, SH); break;
+    case 1: launch_project_views_t<1>(st, deg, n, cap, vb, V, cut, P, SH); break;
+    case 2: launch_project_views_t<2>(st, deg, n, cap, vb, V, cut, P, SH); break;
+    default: launch_project_views_t<3>(st, deg, n, cap, vb, V, cut, P, SH); break;
+  }
+}
+
 template <int kPrec>
 static void launch_project_t(hipStream_t st, int deg, int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
                              const float4* P, const void* SH, float4* rec, uint32_t* depthKey, uint2* rect,

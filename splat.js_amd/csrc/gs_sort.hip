@@ -543,6 +543,14 @@ __device__ __forceinline__ uint32_t seg_val(uint64_t k) {
   return (uint32_t)((k >> 4) & kIdMask) | ((uint32_t)(k & 0xfu) << kMaskShift);
 }
 
+// the binning job of this block: the single view's (by value), or view v
+// of the batched array
+template <bool kJobs>
+__device__ __forceinline__ SegJob seg_job(const SegJob& arg, const SegJob* __restrict__ jobs, int v) {
+  if constexpr (kJobs) return jobs[v];
+  else return arg;
+}
+
 // exclusive scan over a block of kT threads (kT / 64 waves)
 template <int kT>
 __device__ __forceinline__ uint32_t block_excl_scan_t(uint32_t v, uint32_t& total, uint32_t* wsum) {
@@ -701,12 +709,17 @@ __device__ __forceinline__ int2 seg_block_chunks(int64_t n, const uint32_t* __re
   return make_int2((int)(lo < n ? lo : n), (int)(hi < n ? hi : n));
 }
 
-__global__ __launch_bounds__(kSegThreads) void k_seg_count(int64_t n, int nb, const uint32_t* __restrict__ cnt,
-                                                           const uint2* __restrict__ rect, int tilesX, int numTiles,
-                                                           const uint32_t* __restrict__ chunkKeys,
-                                                           uint32_t* __restrict__ blockStart,
-                                                           uint32_t* __restrict__ table,
-                                                           uint32_t* __restrict__ visPart) {
+template <bool kJobs>
+__global__ __launch_bounds__(kSegThreads) void k_seg_count(SegJob arg, const SegJob* __restrict__ jobs) {
+  const SegJob j = seg_job<kJobs>(arg, jobs, blockIdx.y);
+  const int64_t n = j.n;
+  const int nb = j.nb, tilesX = j.tilesX, numTiles = j.numTiles;
+  const uint32_t* __restrict__ cnt = j.cnt;
+  const uint2* __restrict__ rect = j.rect;
+  const uint32_t* __restrict__ chunkKeys = j.chunkKeys;
+  uint32_t* __restrict__ blockStart = j.blockStart;
+  uint32_t* __restrict__ table = j.table;
+  uint32_t* __restrict__ visPart = j.visPart;
   uint32_t* hist = seg_dyn;   // numTiles
   __shared__ uint32_t sOff[kSegThreads];
   __shared__ uint2 sRect[kSegThreads];
@@ -738,8 +751,12 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_count(int64_t n, int nb, co
 // blocks, tileCnt[t] the column total. Blocks of 256 threads handle 16
 // tiles, 16 threads per tile (a wave reads 16 consecutive tiles of 4 rows),
 // each thread a sixteenth of the rows.
-__global__ __launch_bounds__(kBlock) void k_seg_colscan(uint32_t* __restrict__ table, int nb, int numTiles,
-                                                        uint32_t* __restrict__ tileCnt) {
+template <bool kJobs>
+__global__ __launch_bounds__(kBlock) void k_seg_colscan(SegJob arg, const SegJob* __restrict__ jobs) {
+  const SegJob j = seg_job<kJobs>(arg, jobs, blockIdx.y);
+  uint32_t* __restrict__ table = j.table;
+  const int nb = j.nb, numTiles = j.numTiles;
+  uint32_t* __restrict__ tileCnt = j.tileCnt;
   __shared__ uint32_t part[16][16];
   const int tl = threadIdx.x & 15, q = threadIdx.x >> 4;
   const int t = blockIdx.x * 16 + tl;
@@ -807,15 +824,17 @@ __device__ __forceinline__ uint32_t seg_padded(uint32_t c) {
 constexpr int kScanThreads = 1024;
 constexpr int kScanQuarter = kScanChunk / 4;   // tiles per thread
 
-__global__ __launch_bounds__(kScanThreads) void k_seg_scan(const uint32_t* __restrict__ tileCnt, int numTiles,
-                                                           uint64_t cap, const uint32_t* __restrict__ visPart,
-                                                           int nb, uint2* __restrict__ ranges,
-                                                           uint32_t* __restrict__ order,
-                                                           uint32_t* __restrict__ numKeys,
-                                                           uint32_t* __restrict__ numVisible,
-                                                           uint64_t* __restrict__ totalKeys,
-                                                           double* __restrict__ stats,
-                                                           uint32_t* __restrict__ chunkKeys, int64_t nChunks) {
+template <bool kJobs>
+__global__ __launch_bounds__(kScanThreads) void k_seg_scan(SegJob arg, const SegJob* __restrict__ jobs) {
+  const SegJob j = seg_job<kJobs>(arg, jobs, blockIdx.y);
+  const uint32_t* __restrict__ tileCnt = j.tileCnt;
+  const int numTiles = j.numTiles, nb = j.nb;
+  const uint64_t cap = j.cap;
+  const uint32_t* __restrict__ visPart = j.visPart;
+  uint2* __restrict__ ranges = j.ranges;
+  uint32_t* __restrict__ order = j.tileOrder;
+  uint32_t* __restrict__ chunkKeys = j.chunkKeys;
+  const int64_t nChunks = (j.n + kSegChunk - 1) / kSegChunk;
   // the count pass has read the chunk key sums: clear them for the next
   // projection to fill
   for (int64_t c = threadIdx.x; c < nChunks; c += kScanThreads) chunkKeys[c] = 0u;
@@ -899,14 +918,14 @@ __global__ __launch_bounds__(kScanThreads) void k_seg_scan(const uint32_t* __res
     atomicAdd(&bucket[tile_bucket(rg)], 1u);
     start += len;
   }
-  if (tid == kScanThreads - 1) *numKeys = (uint32_t)start;
+  if (tid == kScanThreads - 1) *j.numKeys = (uint32_t)start;
   if (tid == 0) {
     uint64_t r = 0;
     uint32_t d = 0, v = 0;
     for (int k = 0; k < kScanThreads / 64; k++) { r += wsB[k]; d += wsC[k]; v += wsV[k]; }
-    *totalKeys = r;
-    if (d) stats[3] += (double)d;
-    *numVisible = v;
+    *j.totalKeys = r;
+    if (d) atomicAdd(&j.stats[3], (double)d);   // the views' scans run together
+    *j.numVisible = v;
   }
   __syncthreads();
   // exclusive bucket offsets, heaviest bucket first: threads 0..255 scan
@@ -968,15 +987,18 @@ struct ScatterPrep {
 // counter starts at the tile's segment start plus this block's offset in
 // the tile (the column scan), so a key's slot is one LDS atomic; the tile's
 // block mask (gs_cover.h) rides in the low 4 bits.
-__global__ __launch_bounds__(kSegThreads) void k_seg_scatter(const uint32_t* __restrict__ blockStart,
-                                                             const uint32_t* __restrict__ cnt,
-                                                             const uint2* __restrict__ rect,
-                                                             const uint32_t* __restrict__ depthKey,
-                                                             const float4* __restrict__ rec, int recFmt,
-                                                             int tilesX, int tileSize, int numTiles,
-                                                             const uint32_t* __restrict__ table,
-                                                             const uint2* __restrict__ ranges,
-                                                             uint64_t* __restrict__ keys) {
+template <bool kJobs>
+__global__ __launch_bounds__(kSegThreads) void k_seg_scatter(SegJob arg, const SegJob* __restrict__ jobs) {
+  const SegJob j = seg_job<kJobs>(arg, jobs, blockIdx.y);
+  const uint32_t* __restrict__ blockStart = j.blockStart;
+  const uint32_t* __restrict__ cnt = j.cnt;
+  const uint2* __restrict__ rect = j.rect;
+  const uint32_t* __restrict__ depthKey = j.depthKey;
+  const float4* __restrict__ rec = j.rec;
+  const int recFmt = j.recFmt, tilesX = j.tilesX, tileSize = j.tile, numTiles = j.numTiles;
+  const uint32_t* __restrict__ table = j.table;
+  const uint2* __restrict__ ranges = j.ranges;
+  uint64_t* __restrict__ keys = j.keys;
   uint32_t* cur = seg_dyn;                            // numTiles: next slot of each tile
   uint8_t* dropped = (uint8_t*)(seg_dyn + numTiles);  // numTiles: tile dropped by the budget
   __shared__ uint32_t sOff[kSegThreads];
@@ -1140,12 +1162,19 @@ __device__ void seg_merge(const uint64_t* A, uint32_t na, const uint64_t* B, uin
 // and merge them pairwise in global memory (keys <-> tmp). Blocks take the
 // tiles in the raster launch order (most keys first), so the few segments
 // that need global merges start early instead of trailing the launch.
-__global__ __launch_bounds__(kBlock) void k_seg_sort(const uint2* __restrict__ ranges,
-                                                     const uint32_t* __restrict__ order, uint64_t* __restrict__ keys,
-                                                     uint64_t* __restrict__ tmp, uint32_t* __restrict__ vals) {
+// Batched: view blockIdx.x, tile rank blockIdx.y, so the launch starts every
+// view's heaviest tiles first.
+template <bool kJobs>
+__global__ __launch_bounds__(kBlock) void k_seg_sort(SegJob arg, const SegJob* __restrict__ jobs) {
+  const SegJob j = seg_job<kJobs>(arg, jobs, kJobs ? blockIdx.x : 0);
+  const uint32_t rank = kJobs ? blockIdx.y : blockIdx.x;
+  if (kJobs && rank >= (uint32_t)j.numTiles) return;
+  uint64_t* __restrict__ keys = j.keys;
+  uint64_t* __restrict__ tmp = j.tmp;
+  uint32_t* __restrict__ vals = j.ids;
   __shared__ uint64_t sa[kChunk];
   __shared__ uint64_t sb[kChunk];
-  const uint2 r = ranges[order[blockIdx.x]];
+  const uint2 r = j.ranges[j.tileOrder[rank]];
   const uint32_t s = r.x, len = r.y - r.x;
   const uint32_t tid = threadIdx.x;
   if (len == 0u) return;
@@ -1238,6 +1267,40 @@ int tile_bits(int numTiles) {
   return b < 1 ? 1 : b;
 }
 
+int seg_blocks(int64_t n) {
+  const int64_t G = seg_group(n);
+  return (int)((n + G - 1) / G);
+}
+
+// The five tile-segment stages for V jobs (kJobs: from the device array
+// jobs, view from the block index; else the one job j by value); maxTiles:
+// the most tiles of any job. Every job has the same Gaussian count (nb).
+template <bool kJobs>
+static void seg_launches(hipStream_t st, const SegJob& j, const SegJob* jobs, int V, int maxTiles) {
+  static const bool ldsRaised = [] {
+    // the per-tile LDS arrays at kMaxSegTiles: 5 B per tile in the scatter
+    const int dyn = (kMaxSegTiles + kMaxSegTiles / kScanQuarter + 1) * 5;
+    (void)hipFuncSetAttribute((const void*)k_seg_count<kJobs>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    (void)hipFuncSetAttribute((const void*)k_seg_scan<kJobs>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    (void)hipFuncSetAttribute((const void*)k_seg_scatter<kJobs>, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
+    return true;
+  }();
+  (void)ldsRaised;
+  const unsigned v = (unsigned)V;
+  k_seg_count<kJobs><<<dim3((unsigned)j.nb, v), kSegThreads, (size_t)maxTiles * 4, st>>>(j, jobs);
+  k_seg_colscan<kJobs><<<dim3((unsigned)((maxTiles + 15) / 16), v), kBlock, 0, st>>>(j, jobs);
+  k_seg_scan<kJobs><<<dim3(1u, v), kScanThreads, (size_t)(maxTiles + maxTiles / kScanQuarter + 1) * 4, st>>>(j, jobs);
+  k_seg_scatter<kJobs><<<dim3((unsigned)j.nb, v), kSegThreads, (size_t)maxTiles * 5, st>>>(j, jobs);
+  if (kJobs) k_seg_sort<kJobs><<<dim3(v, (unsigned)maxTiles), kBlock, 0, st>>>(j, jobs);
+  else k_seg_sort<kJobs><<<dim3((unsigned)maxTiles), kBlock, 0, st>>>(j, jobs);
+}
+
+void launch_bin_sort_views(hipStream_t st, const SegJob* dJobs, const SegJob* hJobs, int V) {
+  int maxTiles = 0;
+  for (int k = 0; k < V; k++) maxTiles = maxTiles > hJobs[k].numTiles ? maxTiles : hJobs[k].numTiles;
+  seg_launches<true>(st, hJobs[0], dJobs, V, maxTiles);
+}
+
 void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
   const SortScratch& s = a.scratch;
   const int tb = tile_bits(a.numTiles);
@@ -1245,28 +1308,33 @@ void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {
   const int Gc = (int)((a.n + kBlock - 1) / kBlock);
   if (a.mode == 2 && a.numTiles <= kMaxSegTiles) {
     // tile segments: count table, column scan, budgeted scan, scatter, per-tile sort
-    const int64_t G = seg_group(a.n);
-    const int nb = (int)((a.n + G - 1) / G);
-    static const bool ldsRaised = [] {
-      // the per-tile LDS arrays at kMaxSegTiles: 5 B per tile in the scatter
-      const int dyn = (kMaxSegTiles + kMaxSegTiles / kScanQuarter + 1) * 5;
-      (void)hipFuncSetAttribute((const void*)k_seg_count, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-      (void)hipFuncSetAttribute((const void*)k_seg_scan, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-      (void)hipFuncSetAttribute((const void*)k_seg_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, dyn);
-      return true;
-    }();
-    (void)ldsRaised;
-    const size_t tileLds = (size_t)a.numTiles * 4;
-    k_seg_count<<<nb, kSegThreads, tileLds, st>>>(a.n, nb, a.cnt, a.rect, a.tilesX, a.numTiles, s.chunkKeys,
-                                                  s.blockStart, s.segTable, s.visPart);
-    k_seg_colscan<<<(a.numTiles + 15) / 16, kBlock, 0, st>>>(s.segTable, nb, a.numTiles, s.tileCnt);
-    k_seg_scan<<<1, kScanThreads, (size_t)(a.numTiles + a.numTiles / kScanQuarter + 1) * 4, st>>>(
-        s.tileCnt, a.numTiles, a.cap, s.visPart, nb, a.ranges, a.tileOrder, a.numKeys, a.numVisible, a.totalKeys,
-        a.stats, s.chunkKeys, (a.n + kSegChunk - 1) / kSegChunk);
-    k_seg_scatter<<<nb, kSegThreads, (size_t)a.numTiles * 5, st>>>(s.blockStart, a.cnt, a.rect, a.depthKey, a.rec,
-                                                                   a.recFmt, a.tilesX, a.tile, a.numTiles, s.segTable, a.ranges,
-                                                                   (uint64_t*)a.tkA);
-    k_seg_sort<<<a.numTiles, kBlock, 0, st>>>(a.ranges, a.tileOrder, (uint64_t*)a.tkA, (uint64_t*)a.tkB, a.tvA);
+    SegJob j{};
+    j.n = a.n;
+    j.cap = a.cap;
+    j.nb = seg_blocks(a.n);
+    j.tilesX = a.tilesX;
+    j.tile = a.tile;
+    j.numTiles = a.numTiles;
+    j.recFmt = a.recFmt;
+    j.rec = const_cast<float4*>(a.rec);
+    j.depthKey = const_cast<uint32_t*>(a.depthKey);
+    j.rect = const_cast<uint2*>(a.rect);
+    j.cnt = const_cast<uint32_t*>(a.cnt);
+    j.chunkKeys = s.chunkKeys;
+    j.blockStart = s.blockStart;
+    j.table = s.segTable;
+    j.visPart = s.visPart;
+    j.tileCnt = s.tileCnt;
+    j.ranges = a.ranges;
+    j.tileOrder = a.tileOrder;
+    j.numKeys = a.numKeys;
+    j.numVisible = a.numVisible;
+    j.totalKeys = a.totalKeys;
+    j.stats = a.stats;
+    j.keys = (uint64_t*)a.tkA;
+    j.tmp = (uint64_t*)a.tkB;
+    j.ids = a.tvA;
+    seg_launches<false>(st, j, nullptr, 1, j.numTiles);
     out.keys = nullptr;
     out.ids = a.tvA;
     out.keyBytes = 0;

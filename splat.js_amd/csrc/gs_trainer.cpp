@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -24,6 +25,8 @@ namespace {
 
 constexpr int kSortBlocks = 512;    // radix grid (measured best of 256/512/1024/2048 at 1M Gaussians, 3M keys)
 constexpr int64_t kDefaultEntries = 12000000;   // shaders.js:34 ENTRIES_CAP
+// most device memory the batched binning's per-view copies may take
+constexpr size_t kBatchedBinBytes = size_t(24) << 30;
 constexpr int64_t kTargetLimitBytes = int64_t(64) << 30;
 constexpr int kRing = 64;           // pinned ViewBlocks: the host may run this many batched steps ahead
 
@@ -144,7 +147,11 @@ struct gs_trainer {
   struct ViewSlot {
     float4* rec; uint32_t* cnt; void* accum; uint32_t* ids; uint2* ranges; uint32_t* tileOrder;
     float4* img; float* T; uint32_t* end; float4* gC;
+    // the batched binning's own copies (slot 0: the single-view buffers)
+    uint32_t* depthKey; uint2* rect; uint32_t* chunkKeys; uint32_t* blockStart; uint32_t* table;
+    uint32_t* visPart; uint32_t* tileCnt; uint64_t* keys; uint64_t* tmp; uint64_t* totalKeys;
   };
+  bool batchedBin = false;         // every view has its own binning buffers: one launch per stage
   std::vector<ViewSlot> vslots;
   ViewBlock* dVB = nullptr;        // device copy of the step's ViewBlock
   ViewBlock* hVB = nullptr;        // pinned ring of kRing host ViewBlocks
@@ -798,7 +805,28 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
   } else {
     launch_fill_u32(h->st, h->dQueue, 2, 0u);
   }
-  for (int k = 0; k < V; k++) {
+  if (h->batchedBin && !overlap) {
+    // every view's projection, then each binning stage of all views, one launch each
+    {
+      Scope sc(h, "project_views", (double)V * n * (56.0 + 4.0 * h->shC + 48.0 + 16.0));
+      launch_project_views(h->st, h->shDeg, h->prec, n, h->cap, h->dVB, V, cut, h->P, h->SH);
+    }
+    {
+      Scope sc(h, "bin_sort_views", 0.0);
+      launch_bin_sort_views(h->st, (const SegJob*)((const char*)h->dVB + offsetof(ViewBlock, seg)), vb.seg, V);
+    }
+    if (dbg) {
+      HIPCK(hipStreamSynchronize(h->st));
+      for (int k = 0; k < V; k++) {
+        uint32_t nk = 0;
+        HIPCK(hipMemcpy(&nk, h->viewStats + k, 4, hipMemcpyDeviceToHost));
+        if (dbg[k].numKeys) *dbg[k].numKeys = nk;
+        if (dbg[k].tileRanges)
+          HIPCK(hipMemcpy(dbg[k].tileRanges, h->vslots[k].ranges, 8ull * vb.cams[k].numTiles, hipMemcpyDeviceToHost));
+      }
+    }
+  }
+  for (int k = 0; k < V && !(h->batchedBin && !overlap); k++) {
     const gs_trainer::ViewSlot& sl = h->vslots[k];
     const Cam& cam = vb.cams[k];
     {
@@ -928,6 +956,34 @@ int fill_views(gs_trainer* h, const int* ci, int count, int activeDeg, double* s
     b.stats = statsBuf;
     vb.accum[k] = sl.accum;
     vb.cnt[k] = sl.cnt;
+    if (h->batchedBin) {
+      SegJob& j = vb.seg[k];
+      j.n = h->n;
+      j.cap = (uint64_t)h->entriesCap;
+      j.nb = seg_blocks(h->n);
+      j.tilesX = vb.cams[k].tilesX;
+      j.tile = vb.cams[k].tile;
+      j.numTiles = vb.cams[k].numTiles;
+      j.recFmt = h->recFmt;
+      j.rec = sl.rec;
+      j.depthKey = sl.depthKey;
+      j.rect = sl.rect;
+      j.cnt = sl.cnt;
+      j.chunkKeys = sl.chunkKeys;
+      j.blockStart = sl.blockStart;
+      j.table = sl.table;
+      j.visPart = sl.visPart;
+      j.tileCnt = sl.tileCnt;
+      j.ranges = sl.ranges;
+      j.tileOrder = sl.tileOrder;
+      j.numKeys = h->viewStats + k;
+      j.numVisible = h->viewStats + kMaxViews + k;
+      j.totalKeys = sl.totalKeys;
+      j.stats = statsBuf;
+      j.keys = sl.keys;
+      j.tmp = sl.tmp;
+      j.ids = sl.ids;
+    }
     vb.off[k] = off;
     off += (uint32_t)vb.cams[k].numTiles;
     key.push_back(vb.cams[k].W);
@@ -1280,7 +1336,9 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   rc |= dalloc(h, &h->camGrad, (size_t)(ncams + 1) * 32);
   // batched views: slot 0 is the single-view set, slots 1.. their own copies
   h->vslots.assign((size_t)h->o.viewsPerStep, gs_trainer::ViewSlot{});
-  h->vslots[0] = {h->rec, h->cnt, h->accum, h->tvA, h->ranges, h->tileOrder, h->img, h->Tfin, h->endIdx, h->gC};
+  h->vslots[0] = {h->rec, h->cnt, h->accum, h->tvA, h->ranges, h->tileOrder, h->img, h->Tfin, h->endIdx, h->gC,
+                  h->depthKey, h->rect, h->sort.chunkKeys, h->sort.blockStart, h->sort.segTable, h->sort.visPart,
+                  h->sort.tileCnt, (uint64_t*)h->tkA, (uint64_t*)h->tkB, h->totalKeys};
   for (size_t k = 1; k < h->vslots.size(); k++) {
     gs_trainer::ViewSlot& v = h->vslots[k];
     rc |= dalloc(h, &v.rec, (size_t)cap * 16 * rec_stride(h->recFmt));
@@ -1298,6 +1356,35 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
     rc |= dalloc(h, &h->dVB, sizeof(ViewBlock));
     rc |= dalloc(h, &h->dQueue, 8);
     rc |= dalloc(h, &h->viewStats, 8 * kMaxViews);
+  }
+  // Batched binning: each further view gets its own binning buffers, so
+  // all views' binning stages run as one launch each. Skipped when those
+  // copies would pass kBatchedBinBytes (then the views bin one after
+  // another through the shared buffers).
+  if (rc == 0 && h->o.viewsPerStep > 1 && h->o.sortMode == GS_SORT_TILE_SEGMENTS) {
+    const int64_t rows = std::min<int64_t>(kSegMaxRows, (cap + kSegMinGroup - 1) / kSegMinGroup);
+    const size_t segTiles = (size_t)std::min(h->maxTiles, kMaxSegTiles);
+    const size_t chunkBytes = (size_t)((cap + kSegChunk - 1) / kSegChunk) * 4;
+    const size_t per = (size_t)cap * 12 + chunkBytes + (size_t)(2 * rows + 1) * 4 + (size_t)rows * segTiles * 4 +
+                       (size_t)h->maxTiles * 4 + (size_t)h->entriesCap * 16 + 8;
+    if (per * (h->vslots.size() - 1) <= kBatchedBinBytes) {
+      for (size_t k = 1; k < h->vslots.size() && rc == 0; k++) {
+        gs_trainer::ViewSlot& v = h->vslots[k];
+        rc |= dalloc(h, &v.depthKey, (size_t)cap * 4);
+        rc |= dalloc(h, &v.rect, (size_t)cap * 8);
+        rc |= dalloc(h, &v.chunkKeys, chunkBytes);
+        rc |= dalloc(h, &v.blockStart, (size_t)(rows + 1) * 4);
+        rc |= dalloc(h, &v.table, (size_t)rows * segTiles * 4);
+        rc |= dalloc(h, &v.visPart, (size_t)rows * 4);
+        rc |= dalloc(h, &v.tileCnt, (size_t)h->maxTiles * 4);
+        rc |= dalloc(h, &v.keys, (size_t)h->entriesCap * 8);
+        rc |= dalloc(h, &v.tmp, (size_t)h->entriesCap * 8);
+        rc |= dalloc(h, &v.totalKeys, 8);
+        // the projection adds to these sums and the scan clears them after use
+        if (rc == 0) HIPCK(hipMemset(v.chunkKeys, 0, chunkBytes));
+      }
+      h->batchedBin = rc == 0;
+    }
   }
   if (rc) { free_all(h); return rc < 0 ? rc : GS_E_NOMEM; }
   if (h->o.viewsPerStep > 1) {
