@@ -664,6 +664,17 @@ struct CountPrep {
 // block starts at 0 and the last ends at n, and block b's end is block
 // b + 1's start by the same formula: the blocks partition [0, n) whatever
 // the chunk sums hold (they only set the balance). Returns [lo, hi).
+// Logical index of this count/scatter block. Workgroups are dispatched
+// round-robin over MI355X's 8 XCDs, so logical blocks 0..nb/8-1 run on the
+// first XCD, the next nb/8 on the second, and so on: each tile segment's
+// runs from neighbouring logical blocks (adjacent in memory) are then
+// written through one XCD's L2, which merges their partial lines before
+// they reach HBM. Any bijection partitions the work the same way.
+__device__ __forceinline__ int seg_logical_block(int nb) {
+  const int i = (int)blockIdx.x;
+  return nb % 8 == 0 ? (i % 8) * (nb / 8) + i / 8 : i;
+}
+
 __device__ __forceinline__ int2 seg_block_chunks(int64_t n, const uint32_t* __restrict__ chunkKeys, int nb,
                                                  uint64_t* ws64, uint32_t (*wsc)[kSegThreads / 64]) {
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -686,7 +697,7 @@ __device__ __forceinline__ int2 seg_block_chunks(int64_t n, const uint32_t* __re
     total += v;
   }
   pre += x - part;
-  const uint64_t b = blockIdx.x;
+  const uint64_t b = (uint64_t)seg_logical_block(nb);
   const uint64_t tLo = b * total, tHi = (b + 1) * total;
   uint32_t nLo = 0, nHi = 0;          // chunks with prefix * nb below each bound
   for (int64_t c = c0; c < c1; c++) {
@@ -705,7 +716,7 @@ __device__ __forceinline__ int2 seg_block_chunks(int64_t n, const uint32_t* __re
   for (int k = 0; k < kSegThreads / 64; k++) { sLo += wsc[0][k]; sHi += wsc[1][k]; }
   __syncthreads();   // ws64 and wsc are free again
   const int64_t lo = (int64_t)sLo * kSegChunk;
-  const int64_t hi = blockIdx.x == (unsigned)(nb - 1) ? n : (int64_t)sHi * kSegChunk;
+  const int64_t hi = b == (uint64_t)(nb - 1) ? n : (int64_t)sHi * kSegChunk;
   return make_int2((int)(lo < n ? lo : n), (int)(hi < n ? hi : n));
 }
 
@@ -729,10 +740,11 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_count(SegJob arg, const Seg
   __shared__ uint32_t wsc[2][kSegThreads / 64];
   const int tid = threadIdx.x;
   const int2 r = seg_block_chunks(n, chunkKeys, nb, ws64, wsc);
+  const int lb = seg_logical_block(nb);
   const int64_t lo = r.x, hi = r.y;
   if (tid == 0) {
-    blockStart[blockIdx.x] = (uint32_t)lo;
-    if (blockIdx.x == (unsigned)(nb - 1)) blockStart[nb] = (uint32_t)hi;
+    blockStart[lb] = (uint32_t)lo;
+    if (lb == nb - 1) blockStart[nb] = (uint32_t)hi;
   }
   uint32_t vis = 0;
   const SegRound sr{sOff, sRect, wsum, owner};
@@ -741,9 +753,9 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_count(SegJob arg, const Seg
              [&](int, int64_t, uint32_t tile, uint32_t, uint32_t) { atomicAdd(&hist[tile], 1u); });
   uint32_t tot;
   block_excl_scan_t<kSegThreads>(vis, tot, wsum);
-  if (tid == 0) visPart[blockIdx.x] = tot;
+  if (tid == 0) visPart[lb] = tot;
   __syncthreads();
-  uint32_t* row = table + (int64_t)blockIdx.x * numTiles;
+  uint32_t* row = table + (int64_t)lb * numTiles;
   for (int t = tid; t < numTiles; t += kSegThreads) row[t] = hist[t];
 }
 
@@ -1008,8 +1020,9 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(SegJob arg, const S
   __shared__ uint32_t wsum[kSegThreads / 64];
   __shared__ uint16_t owner[kOwnerCap];
   const int tid = threadIdx.x;
-  const uint32_t* row = table + (int64_t)blockIdx.x * numTiles;
-  const int64_t lo = blockStart[blockIdx.x], hi = blockStart[blockIdx.x + 1];   // the count pass's blocks
+  const int lb = seg_logical_block(j.nb);
+  const uint32_t* row = table + (int64_t)lb * numTiles;
+  const int64_t lo = blockStart[lb], hi = blockStart[lb + 1];   // the count pass's blocks
   const SegRound sr{sOff, sRect, wsum, owner};
   seg_rounds(lo, hi, cnt, rect, tilesX, sr, ScatterPrep{depthKey, rec, recFmt, sDk, sG},
              [&]() {
