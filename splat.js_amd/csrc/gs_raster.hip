@@ -39,6 +39,8 @@
 //                   Charbonnier loss the three colour terms cannot reach the
 //                   clamp and convert without it.)
 //   GS_ACCUM_F32:   plain f32 sums (atomic add order is not deterministic).
+#include <type_traits>
+
 #include "gs_kernels.h"
 
 namespace gs {
@@ -361,6 +363,33 @@ __device__ __forceinline__ T wave_reduce10(const T* v) {
   return r;
 }
 
+// The same network on 20 values (two visits' terms): one more halving step
+// (l ^ 2) before the last addition (l ^ 1). Its five halving steps carry
+// twice the independent work of the 10-value network's, and it serves two
+// visits, which halves the passes through the lane-exchange chain.
+template <typename T>
+__device__ __forceinline__ T wave_reduce20(const T* v) {
+  const uint32_t lane = __lane_id();
+  T a[10], b5[5], c[3], d[2], e[1];
+  swap_halve<T, 20, true>(v, a);
+  swap_halve<T, 10, false>(a, b5);
+  dpp_halve<T, 5, 0x140>(b5, c, (lane & 8) != 0);   // row_mirror: l ^ 15
+  dpp_halve<T, 3, 0x141>(c, d, (lane & 4) != 0);    // row_half_mirror: l ^ 7
+  dpp_halve<T, 2, 0x4E>(d, e, (lane & 2) != 0);     // quad_perm [2,3,0,1]: l ^ 2
+  T r = e[0];
+  r = r + from_u<T>(dpp<0xB1>(as_u(r)));            // quad_perm [1,0,3,2]: l ^ 1
+  return r;
+}
+
+// slot whose 64-lane total this lane holds after wave_reduce20 (-1: none)
+__device__ __forceinline__ int reduce20_slot() {
+  uint32_t v[20];
+#pragma unroll
+  for (int i = 0; i < 20; i++) v[i] = (uint32_t)(i + 1);
+  const int s = (int)(wave_reduce20<uint32_t>(v) / 64u) - 1;
+  return s >= 0 && s < 20 ? s : -1;
+}
+
 // slot whose 64-lane total this lane holds after wave_reduce10 (-1: none)
 __device__ __forceinline__ int reduce10_slot() {
   uint32_t v[10];
@@ -368,6 +397,17 @@ __device__ __forceinline__ int reduce10_slot() {
   for (int i = 0; i < 10; i++) v[i] = (uint32_t)(i + 1);
   const int s = (int)(wave_reduce10<uint32_t>(v) / 64u) - 1;
   return s >= 0 && s < 10 ? s : -1;
+}
+
+// Adds one wave total to slot q of Gaussian g's accumulator row (the lanes
+// holding a total; zeros skipped).
+template <int kAccum, typename T>
+__device__ __forceinline__ void flush_sum(void* accum, uint32_t g, int q, T r, bool flusher) {
+  if constexpr (kAccum == 0) {
+    if (flusher && r != 0u) atomicAdd((uint32_t*)accum + (int64_t)g * 12 + q, r);
+  } else {
+    if (flusher && r != 0.0f) unsafeAtomicAdd((float*)accum + (int64_t)g * 12 + q, r);
+  }
 }
 
 // kClampColour false: dL/dC is bounded (the Charbonnier loss: |dL/dC| <=
@@ -401,6 +441,17 @@ __device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, 
   const int64_t pi = (int64_t)y * cam.W + x;
   const int mySlot = reduce10_slot();
   const bool flusher = (__lane_id() & 3) == 0 && mySlot >= 0;
+  const int mySlot20 = reduce20_slot();
+  const bool flusher20 = (__lane_id() & 1) == 0 && mySlot20 >= 0;
+  // A visit's per-lane terms wait for the next visit's, and the two are
+  // summed over the wave in one wave_reduce20 (slots 0-9 the waiting
+  // visit's, 10-19 the new one's); the sums are added to the accumulators
+  // in a different order than visited, which changes nothing (integer
+  // sums; f32 atomics have no fixed order anyway).
+  using AccT = typename std::conditional<kAccum == 0, uint32_t, float>::type;
+  AccT pend[10];
+  uint32_t pendGid = 0;
+  bool hasPend = false;
   float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float Ta = 0.0f;
   uint32_t myEnd = s;
@@ -536,23 +587,39 @@ __device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, 
           GS = fmaf(gcol, aT, GS);
           Ta = contrib ? Tb : Ta;
         }
-        if (kAccum == 0) {
+        AccT cur[10];
+        if constexpr (kAccum == 0) {
           // i32(clamp(v * 16384, -1e9, 1e9)) per pixel; v already carries the scale
-          uint32_t iv[10];
 #pragma unroll
           for (int q = 0; q < 10; q++)
-            iv[q] = (q >= 7 && !kClampColour) ? (uint32_t)(int32_t)v[q]
-                                               : (uint32_t)(int32_t)fminf(fmaxf(v[q], -1.0e9f), 1.0e9f);
-          const uint32_t r = wave_reduce10<uint32_t>(iv);
-          if (flusher && r != 0u) atomicAdd((uint32_t*)accum + (int64_t)gid * 12 + mySlot, r);
+            cur[q] = (q >= 7 && !kClampColour) ? (uint32_t)(int32_t)v[q]
+                                                : (uint32_t)(int32_t)fminf(fmaxf(v[q], -1.0e9f), 1.0e9f);
         } else {
-          const float r = wave_reduce10<float>(v);
-          if (flusher && r != 0.0f) unsafeAtomicAdd((float*)accum + (int64_t)gid * 12 + mySlot, r);
+#pragma unroll
+          for (int q = 0; q < 10; q++) cur[q] = v[q];
+        }
+        if (hasPend) {
+          AccT both[20];
+#pragma unroll
+          for (int q = 0; q < 10; q++) {
+            both[q] = pend[q];
+            both[10 + q] = cur[q];
+          }
+          const AccT r = wave_reduce20<AccT>(both);
+          const uint32_t g2 = mySlot20 < 10 ? pendGid : gid;
+          flush_sum<kAccum>(accum, g2, mySlot20 % 10, r, flusher20);
+          hasPend = false;
+        } else {
+#pragma unroll
+          for (int q = 0; q < 10; q++) pend[q] = cur[q];
+          pendGid = gid;
+          hasPend = true;
         }
       }
     }
     hi = lo;
   }
+  if (hasPend) flush_sum<kAccum>(accum, pendGid, mySlot, wave_reduce10<AccT>(pend), flusher);
 }
 
 template <class S, int kLoss>
