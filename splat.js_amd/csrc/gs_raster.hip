@@ -296,15 +296,19 @@ __device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, 
 }
 
 // ---- wave transpose reduction of 10 values (VALU lane exchanges) ----
-// Halving steps: 10 -> 5 (lanes l, l^32: v_permlane32_swap), 5 -> 3
-// (l, l^16: v_permlane16_swap), 3 -> 2 (l, l^15: DPP row_mirror), 2 -> 1
-// (l, l^7: DPP row_half_mirror); then the quad sums (DPP quad_perm xor 2,
-// xor 1). In each halving step a lane keeps one half of its values and adds
-// its partner's copy of that half, so after the last step each lane holds
-// the 64-lane total of one slot (or of padding). The swap builtins return
-// both operands after exchanging one half of the lanes, so lo' + hi' is that
-// sum whichever half a lane ends up with; the slot each lane ends with is
-// measured once per kernel by running the network on the constants 1..10.
+// Six halving steps, the lane exchanges inside a row of 16 first while the
+// values are many (DPP: 10 -> 5 with l ^ 15, row_mirror; 5 -> 3 with l ^ 7,
+// row_half_mirror; 3 -> 2 with l ^ 2 and 2 -> 1 with l ^ 1, quad_perm),
+// then the exchanges across rows on one value (l ^ 16: v_permlane16_swap;
+// l ^ 32: v_permlane32_swap), which measured ~1% faster over the backward
+// than the reverse order. In each halving step a lane keeps one half of its
+// values and adds its partner's copy of that half (an odd count is padded
+// with a zero), so after the last step each slot's 64-lane total sits in
+// exactly one lane and every other lane holds a padding zero. The swap
+// builtins return both operands after exchanging one half of the lanes, so
+// lo' + hi' is that sum whichever half a lane ends up with; the slot each
+// lane ends with is measured once per kernel by running the network on the
+// constants 1..10.
 template <int kCtrl>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, 0xf, 0xf, false);
@@ -352,33 +356,30 @@ __device__ __forceinline__ void dpp_halve(const T* v, T* out, bool upper) {
 template <typename T>
 __device__ __forceinline__ T wave_reduce10(const T* v) {
   const uint32_t lane = __lane_id();
-  T a[5], b3[3], c[2], d[1];
-  swap_halve<T, 10, true>(v, a);
-  swap_halve<T, 5, false>(a, b3);
-  dpp_halve<T, 3, 0x140>(b3, c, (lane & 8) != 0);   // row_mirror: l ^ 15
-  dpp_halve<T, 2, 0x141>(c, d, (lane & 4) != 0);    // row_half_mirror: l ^ 7
-  T r = d[0];
-  r = r + from_u<T>(dpp<0x4E>(as_u(r)));            // quad_perm [2,3,0,1]: l ^ 2
-  r = r + from_u<T>(dpp<0xB1>(as_u(r)));            // quad_perm [1,0,3,2]: l ^ 1
-  return r;
+  T a[5], b3[3], c[2], d[1], e[1], f[1];
+  dpp_halve<T, 10, 0x140>(v, a, (lane & 8) != 0);
+  dpp_halve<T, 5, 0x141>(a, b3, (lane & 4) != 0);
+  dpp_halve<T, 3, 0x4E>(b3, c, (lane & 2) != 0);
+  dpp_halve<T, 2, 0xB1>(c, d, (lane & 1) != 0);
+  swap_halve<T, 1, false>(d, e);
+  swap_halve<T, 1, true>(e, f);
+  return f[0];
 }
 
-// The same network on 20 values (two visits' terms): one more halving step
-// (l ^ 2) before the last addition (l ^ 1). Its five halving steps carry
-// twice the independent work of the 10-value network's, and it serves two
-// visits, which halves the passes through the lane-exchange chain.
+// The same network on 20 values (two visits' terms): the same six steps,
+// with twice the values in the first ones; it serves two visits, so the
+// single-value cross-row exchanges run once per two visits.
 template <typename T>
 __device__ __forceinline__ T wave_reduce20(const T* v) {
   const uint32_t lane = __lane_id();
-  T a[10], b5[5], c[3], d[2], e[1];
-  swap_halve<T, 20, true>(v, a);
-  swap_halve<T, 10, false>(a, b5);
-  dpp_halve<T, 5, 0x140>(b5, c, (lane & 8) != 0);   // row_mirror: l ^ 15
-  dpp_halve<T, 3, 0x141>(c, d, (lane & 4) != 0);    // row_half_mirror: l ^ 7
-  dpp_halve<T, 2, 0x4E>(d, e, (lane & 2) != 0);     // quad_perm [2,3,0,1]: l ^ 2
-  T r = e[0];
-  r = r + from_u<T>(dpp<0xB1>(as_u(r)));            // quad_perm [1,0,3,2]: l ^ 1
-  return r;
+  T a[10], b5[5], c[3], d[2], e[1], g[1];
+  dpp_halve<T, 20, 0x140>(v, a, (lane & 8) != 0);
+  dpp_halve<T, 10, 0x141>(a, b5, (lane & 4) != 0);
+  dpp_halve<T, 5, 0x4E>(b5, c, (lane & 2) != 0);
+  dpp_halve<T, 3, 0xB1>(c, d, (lane & 1) != 0);
+  swap_halve<T, 2, false>(d, e);
+  swap_halve<T, 1, true>(e, g);
+  return g[0];
 }
 
 // slot whose 64-lane total this lane holds after wave_reduce20 (-1: none)
@@ -440,9 +441,9 @@ __device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, 
   const uint32_t s = __builtin_amdgcn_readfirstlane(rg.x);
   const int64_t pi = (int64_t)y * cam.W + x;
   const int mySlot = reduce10_slot();
-  const bool flusher = (__lane_id() & 3) == 0 && mySlot >= 0;
+  const bool flusher = mySlot >= 0;
   const int mySlot20 = reduce20_slot();
-  const bool flusher20 = (__lane_id() & 1) == 0 && mySlot20 >= 0;
+  const bool flusher20 = mySlot20 >= 0;
   // A visit's per-lane terms wait for the next visit's, and the two are
   // summed over the wave in one wave_reduce20 (slots 0-9 the waiting
   // visit's, 10-19 the new one's); the sums are added to the accumulators
