@@ -71,12 +71,13 @@ def test_logf_matches_libm(oracle_lib):
 
 
 # ---------------------------------------------------------------- projection
-@pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_project_on_axis_closed_form(oracle_lib, prec):
+@pytest.mark.parametrize("prec,tile", [("f32", 16), ("f64", 16), ("f32", 8), ("f32", 32)])
+def test_project_on_axis_closed_form(oracle_lib, prec, tile):
     """Identity q, isotropic s: V = (f s / z)^2 I, conic = 1/(v + 0.3),
-    comp = v/(v+0.3) (ply.js:24-25 bakes the same factor), mean = (cx, cy)."""
+    comp = v/(v+0.3) (ply.js:24-25 bakes the same factor), mean = (cx, cy);
+    the tile rectangle at the camera's tile size."""
     o = O.make_opts(shDeg=0)
-    cam = O.make_cam(_cam(W=64, H=48, f=50.0))
+    cam = O.make_cam(_cam(W=64, H=48, f=50.0), tile=tile)
     st = O.Stage(o, prec)
     z, s = 3.0, 0.05
     p = _gauss((0, 0, z), [math.log(s)] * 3, op=2.0)[None]
@@ -97,10 +98,12 @@ def test_project_on_axis_closed_form(oracle_lib, prec):
     np.testing.assert_allclose(pr[7], opa, rtol=tol * 10)
     emax = min(4.5, math.log(opa * (v / (v + 0.3)) / 0.0039))
     np.testing.assert_allclose(pr[15], math.sqrt(2 * emax) * math.sqrt(v + 0.3), rtol=tol * 10)
-    # tile rect = floor((m -+ rad)/16) clamped to the grid
+    # tile rect = floor((m -+ rad)/T) clamped to the grid of ceil(64/T) x ceil(48/T) tiles
     r = pr[15]
-    assert list(rect[0]) == [max(0, math.floor((32 - r) / 16)), min(3, math.floor((32 + r) / 16)),
-                             max(0, math.floor((24 - r) / 16)), min(2, math.floor((24 + r) / 16))]
+    T = tile
+    gx, gy = (64 + T - 1) // T - 1, (48 + T - 1) // T - 1
+    assert list(rect[0]) == [max(0, math.floor((32 - r) / T)), min(gx, math.floor((32 + r) / T)),
+                             max(0, math.floor((24 - r) / T)), min(gy, math.floor((24 + r) / T))]
     assert cnt[0] == (rect[0][1] - rect[0][0] + 1) * (rect[0][3] - rect[0][2] + 1)
 
 
@@ -198,15 +201,23 @@ def test_sh_basis_constants_and_gradient(oracle_lib):
 
 
 # ---------------------------------------------------------------- binning + sort
-def test_bin_sort_order_and_ranges(oracle_lib):
+@pytest.mark.parametrize("tile", [8, 16, 32])
+def test_bin_sort_order_and_ranges(oracle_lib, tile):
     o = O.make_opts(shDeg=2)
     p, sh, cams = scenes.config1(seed=0)
     p, sh = p[:3000], sh[:3000]
-    cam = O.make_cam(cams[0], shActiveDeg=2)
+    cam = O.make_cam(cams[0], shActiveDeg=2, tile=tile)
     st = O.Stage(o, "f32")
     proj, rect, cnt = st.project(cam, p, sh)
     ids, ts, keys = st.bin_sort(cam, proj, rect, cnt)
-    tilesX = (cam.W + 15) // 16
+    tilesX = (cam.W + tile - 1) // tile
+    assert len(ts) - 1 == tilesX * ((cam.H + tile - 1) // tile)
+    # every rectangle is the radius box in tile units at this tile size
+    vis = cnt > 0
+    r = proj[vis, 15].astype(np.float32)
+    mx, my = proj[vis, 0].astype(np.float32), proj[vis, 1].astype(np.float32)
+    np.testing.assert_array_equal(rect[vis, 0], np.clip(np.floor((mx - r) / np.float32(tile)), 0, tilesX - 1))
+    np.testing.assert_array_equal(rect[vis, 1], np.clip(np.floor((mx + r) / np.float32(tile)), 0, tilesX - 1))
     # brute force with numpy lexsort over (tile, depth bits, id)
     tl, dp, gi = [], [], []
     for i in np.nonzero(cnt)[0]:
@@ -224,10 +235,10 @@ def test_bin_sort_order_and_ranges(oracle_lib):
 
 
 # ---------------------------------------------------------------- raster forward
-@pytest.mark.parametrize("prec", ["f32", "f64"])
-def test_render_single_gaussian_closed_form(oracle_lib, prec):
+@pytest.mark.parametrize("prec,tile", [("f32", 16), ("f64", 16), ("f64", 8), ("f64", 32)])
+def test_render_single_gaussian_closed_form(oracle_lib, prec, tile):
     o = O.make_opts(shDeg=0)
-    cam = O.make_cam(_cam(W=48, H=40, f=60.0))
+    cam = O.make_cam(_cam(W=48, H=40, f=60.0), tile=tile)
     st = O.Stage(o, prec)
     p = _gauss((0.02, -0.01, 3.0), [math.log(0.08), math.log(0.05), math.log(0.06)],
                q=(0.9, 0.1, -0.2, 0.3), col=(0.3, -0.4, 1.0), op=1.0)[None]
@@ -242,7 +253,7 @@ def test_render_single_gaussian_closed_form(oracle_lib, prec):
     a = np.minimum(0.99, pr[7] * pr[6] * np.exp(-e))
     a = np.where((e > 4.5) | (a < 0.0039), 0, a)
     # pixels outside the binned tiles get nothing
-    tx, ty = xs // 16, ys // 16
+    tx, ty = xs // tile, ys // tile
     inrect = (tx >= rect[0, 0]) & (tx <= rect[0, 1]) & (ty >= rect[0, 2]) & (ty <= rect[0, 3])
     a = np.where(inrect, a, 0).reshape(-1)
     tol = 2e-6 if prec == "f32" else 1e-12
