@@ -2,7 +2,8 @@
 
 CPU: the addon loads under this image's Node and every binding resolves.
 GPU: a Node process trains the reference's gradcheck rig through
-createTrainer()/GSTrainer and its results equal the ctypes path bit for bit.
+createTrainer()/GSTrainer (after an autotuning sweep, at 8-pixel tiles with
+128-key batches) and its results equal the ctypes path bit for bit.
 """
 import json
 import os
@@ -36,7 +37,7 @@ def test_node_trainer_matches_ctypes(tmp_path):
     for i, im in enumerate(images):
         (tmp_path / f"img{i}.f32").write_bytes(np.asarray(im["rgb"], np.float32).tobytes())
     cfg = {"opts": dict(opts, shDeg=0), "cams": cams, "images": [{"tw": im["tw"], "th": im["th"]} for im in images],
-           "maxW": W, "maxH": H, "radius": radius, "steps": steps}
+           "maxW": W, "maxH": H, "radius": radius, "steps": steps, "tile": 8, "batch": 128}
     (tmp_path / "cfg.json").write_text(json.dumps(cfg))
     out = subprocess.run([NODE, os.path.join(ROOT, "tests", "node", "run_trainer.mjs"), str(tmp_path)],
                          stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True, timeout=120)
@@ -44,6 +45,7 @@ def test_node_trainer_matches_ctypes(tmp_path):
     res = json.loads(out.stdout.strip().splitlines()[-1])
     t = gsabi.GSTrainer(shDeg=0, **opts)
     t.setup({"data": params, "n": params.shape[0]}, cams, images, W, H, radius)
+    t.setRasterShape(8, 128)
     for _ in range(steps):
         t.stepOnce(0)
     loss = t.readLoss()
@@ -53,7 +55,7 @@ def test_node_trainer_matches_ctypes(tmp_path):
     t.destroy()
     js_params = np.frombuffer((tmp_path / "out_params.f32").read_bytes(), np.float32).reshape(-1, 16)
     js_img = np.frombuffer((tmp_path / "out_img.f32").read_bytes(), np.float32).reshape(img.shape)
-    assert res["iter"] == steps and res["n"] == params.shape[0]
+    assert res["iter"] == steps and res["n"] == params.shape[0] and res["tune"] == 9
     np.testing.assert_array_equal(js_params.view(np.uint32), data.view(np.uint32))
     np.testing.assert_array_equal(js_img.view(np.uint32), img.view(np.uint32))
     assert res["loss"] == loss and res["psnr"] == psnr
