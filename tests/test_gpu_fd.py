@@ -42,10 +42,12 @@ def _medians(slots, rel):
     return {NAMES[s]: float(np.median(rel[slots == s])) for s in range(14) if np.any(slots == s)}
 
 
-@pytest.mark.parametrize("deg", [0, 3])
-def test_fd_kernel_against_fp64_analytic_gradients_on_the_rig(deg):
+@pytest.mark.parametrize("deg,tile", [(0, 16), (3, 16), (0, 8), (0, 32)])
+def test_fd_kernel_against_fp64_analytic_gradients_on_the_rig(deg, tile):
+    """At tiles of 8 and 32 pixels the FD kernel walks the tile's pixels in
+    groups of 256 (or one group of 64): the same bound holds."""
     params, sh, cams, images, W, H, radius, opts = rig_setup(deg)
-    t = gsabi.GSTrainer(shDeg=deg, **opts)
+    t = gsabi.GSTrainer(shDeg=deg, tile=tile, **opts)
     g = {"data": params, "n": params.shape[0]}
     if sh is not None:
         g["sh"] = sh
@@ -59,12 +61,13 @@ def test_fd_kernel_against_fp64_analytic_gradients_on_the_rig(deg):
     t.destroy()
     o = O.make_opts(eCut=opts["eCut"], aMin=opts["aMin"], radClamp=opts["radClamp"], anisoReg=0.0, shDeg=deg)
     tgt = O.pack_targets(images[0]["rgb"])
-    g64 = O.analytic_grads(O.Stage(o, "f64"), O.make_cam(cams[0], shActiveDeg=deg), params, sh, tgt)[0]
+    g64 = O.analytic_grads(O.Stage(o, "f64"), O.make_cam(cams[0], shActiveDeg=deg, tile=tile), params, sh, tgt)[0]
     a = g64[gi, sl]
     rel = _rel(a, fd)
     med = _medians(sl, rel)
     close = float(np.mean(rel < 1e-2))
-    print(f"rig deg {deg}: median relative error per slot {med}; {close:.3f} of {rel.size} samples within 1e-2")
+    print(f"rig deg {deg}, tile {tile}: median relative error per slot {med}; {close:.3f} of {rel.size} samples "
+          f"within 1e-2")
     # a step that moves a splat across a pixel's alpha or exponent cutoff
     # (strict here, gradcheck.js:155) leaves a jump in the loss: such samples
     # differ from the smooth gradient, so the bound is on the medians

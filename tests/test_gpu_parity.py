@@ -458,7 +458,8 @@ def test_setup_rejects_bad_input():
     t.destroy()
 
 
-def test_l1_dssim_loss_against_f64():
+@pytest.mark.parametrize("tile,batch", [(16, 256), (32, 128), (8, 64)])
+def test_l1_dssim_loss_against_f64(tile, batch):
     """L1 + 0.2 D-SSIM (the textbook loss; the reference has only
     Charbonnier): the two SSIM kernels against the double-precision
     restatement on the GPU's own forward image. f32 blur sums lose digits
@@ -466,18 +467,21 @@ def test_l1_dssim_loss_against_f64():
     magnitude (and 1e-5 relative on the loss). The backward pass fed with
     the GPU's dL/dC stays bit-exact. The target carries an invalid border
     (4 rows and 6 columns) and a few invalid pixels inside, which enter the
-    SSIM windows as zeros and take no gradient."""
+    SSIM windows as zeros and take no gradient. The loss kernels keep their
+    own 16x16 image tiling at every raster shape."""
+    print(f"L1 + D-SSIM at tile {tile}, batch {batch}", flush=True)
     p, sh, cams, images, W, H, radius = config1_setup(n=8000)
     rgb = images[0]["rgb"].reshape(H, W, 3).copy()
     rgb[:4, :, 0] = -1.0
     rgb[:, -6:, 0] = -1.0
     rgb[100:103, 50:52, 0] = -1.0
     images = [dict(images[0], rgb=rgb.reshape(-1))]
-    t = _trainer(p, sh, cams, images, W, H, radius, {}, shDeg=2, loss=gsabi.GS_LOSS_L1_DSSIM, ssimLambda=0.2)
+    t = _trainer(p, sh, cams, images, W, H, radius, {}, shDeg=2, loss=gsabi.GS_LOSS_L1_DSSIM, ssimLambda=0.2,
+                 tile=tile, batch=batch)
     t.iter = 2500
     g = t.step_debug(0, want=("img", "T", "end", "gC", "accum", "stats", "ids", "tileRanges"))
     t.destroy()
-    cam = O.make_cam(cams[0], shActiveDeg=2)
+    cam = O.make_cam(cams[0], shActiveDeg=2, tile=tile)
     tgt = O.pack_targets(images[0]["rgb"])
     st64 = O.Stage(O.make_opts(shDeg=2), "f64")
     gC64, valid, stats64 = st64.loss_l1ssim(cam, g["img"].astype(np.float64), tgt, 0.2)
