@@ -17,10 +17,12 @@ Runs, all from the same start on the same workload:
                Adam moments and all arithmetic f32 (the metric's mixed precision)
   mixed        single-view, SH coefficients and all Adam moments in bf16 (beyond the
                metric's rule that moments stay f32; reported, not the headline)
-  bf16 x8      the batched step: 8 views rasterized forward and backward in one
-               persistent launch each, one Adam step, replayed as a hipGraph
-               (x8o: each view's raster on a second stream beside the binning of
-               the next views instead)
+  bf16 x8      the batched step: 8 views projected and binned in one launch per
+               stage, rasterized forward and backward in one persistent launch
+               each, one Adam step, replayed as a hipGraph (x8o: each view's
+               raster on a second stream beside the binning of the next views
+               instead)
+  bf16 x16     the same with 16 views per step (reported beside the headline)
 Before timing, each run's trainer picks its raster shape (tile side 8, 16 or
 32 pixels x LDS batch of 64, 128 or 256 keys) with its autotuner on views
 0.. of the ring (--no-autotune keeps 16 x 256); config.runs records the
@@ -42,6 +44,7 @@ step of the headline storage mode against the FP64 restatement on a
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -129,6 +132,15 @@ def autotune(t, views, args):
     return {"tile": info.tile, "batch": info.batch, "autotuned": True, "pass_ms": ms}
 
 
+def parse_run(name):
+    """'bf16' -> ('bf16', 0, False); 'bf16x8' -> ('bf16', 8, False); 'bf16x8o'
+    (overlapped launches) -> ('bf16', 8, True)."""
+    m = re.fullmatch(r"(f32|bf16|fp16|mixed)(?:x(\d+)(o?))?", name)
+    if not m:
+        raise SystemExit(f"bench.py: unknown run {name!r}")
+    return m.group(1), int(m.group(2) or 0), m.group(3) == "o"
+
+
 def run_single(prec, p, sh, cams, images, W, H, args):
     """Warm up, time args.steps single-view steps with per-stage HIP events
     on the trainer's stream, then sample key and visible counts over 16 more
@@ -159,14 +171,14 @@ def run_single(prec, p, sh, cams, images, W, H, args):
             "keys": float(np.mean(keys)), "vis": float(np.mean(vis)), "shape": shape}
 
 
-def run_batched(prec, p, sh, cams, images, W, H, args, launch=gsabi.GS_VIEWS_PERSISTENT):
-    """Batched steps of BATCH views: warm up (the second step captures the
+def run_batched(prec, p, sh, cams, images, W, H, args, launch=gsabi.GS_VIEWS_PERSISTENT, batch=BATCH):
+    """Batched steps of `batch` views: warm up (the second step captures the
     graph), time args.steps batched steps replayed from the graph, then a
     profiled pass (eager persistent launches with per-stage HIP events, one
     stream) and key counts."""
-    t = _trainer(p, sh, cams, images, W, H, args.n, prec, views=BATCH, launch=launch)
-    shape = autotune(t, list(range(BATCH)), args)
-    picks = [-1] * BATCH
+    t = _trainer(p, sh, cams, images, W, H, args.n, prec, views=batch, launch=launch)
+    shape = autotune(t, list(range(batch)), args)
+    picks = [-1] * batch
     for _ in range(max(args.warmup, 2)):
         t.stepViews(picks)
     t.sync()
@@ -185,13 +197,13 @@ def run_batched(prec, p, sh, cams, images, W, H, args, launch=gsabi.GS_VIEWS_PER
     for _ in range(4):
         t.stepViews(picks)
         s = t.step_stats()
-        keys.append(s.lastKeys / BATCH)
-        vis.append(s.lastVisible / BATCH)
+        keys.append(s.lastKeys / batch)
+        vis.append(s.lastVisible / batch)
     t.destroy()
-    its = args.steps * BATCH / dt
+    its = args.steps * batch / dt
     form = "overlapped" if launch == gsabi.GS_VIEWS_OVERLAP else "persistent"
-    log(f"{prec} x{BATCH} {form}: {its:.1f} iters/s ({args.steps / dt:.1f} batched steps/s)")
-    return {"its": its, "ms_per_iter": 1e3 * dt / (args.steps * BATCH), "steps_per_s": args.steps / dt,
+    log(f"{prec} x{batch} {form}: {its:.1f} iters/s ({args.steps / dt:.1f} batched steps/s)")
+    return {"its": its, "ms_per_iter": 1e3 * dt / (args.steps * batch), "steps_per_s": args.steps / dt, "views": batch,
             "prof": prof, "keys": float(np.mean(keys)), "vis": float(np.mean(vis)), "shape": shape}
 
 
@@ -266,9 +278,9 @@ def main():
     ap.add_argument("--views", type=int, default=64)
     ap.add_argument("--width", type=int, default=1600)
     ap.add_argument("--height", type=int, default=900)
-    ap.add_argument("--runs", default="f32,bf16,mixed,bf16x8",
-                    help="comma list of f32, bf16, fp16, mixed (single-view), <prec>x8 (batched, persistent "
-                         "launches) and <prec>x8o (batched, overlapped launches)")
+    ap.add_argument("--runs", default="f32,bf16,mixed,bf16x8,bf16x16",
+                    help="comma list of f32, bf16, fp16, mixed (single-view), <prec>x<V> (batched steps of V "
+                         "views, persistent launches; bf16x8 is the headline) and <prec>x<V>o (overlapped)")
     ap.add_argument("--psnr-iters", type=int, default=2000, help="0 skips the PSNR drift runs")
     ap.add_argument("--threads", type=int, default=0, help="CPU baseline threads (0: all, at most 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -296,16 +308,16 @@ def main():
     p, sh, cams, images = build_workload(args.n, args.views, W, H, f)
     runs = {}
     for r in args.runs.split(","):
-        if r.endswith("x8"):
-            runs[r] = run_batched(r[:-2], p, sh, cams, images, W, H, args)
-        elif r.endswith("x8o"):
-            runs[r] = run_batched(r[:-3], p, sh, cams, images, W, H, args, launch=gsabi.GS_VIEWS_OVERLAP)
+        prec, nv, over = parse_run(r)
+        if nv:
+            runs[r] = run_batched(prec, p, sh, cams, images, W, H, args, batch=nv,
+                                  launch=gsabi.GS_VIEWS_OVERLAP if over else gsabi.GS_VIEWS_PERSISTENT)
         else:
             runs[r] = run_single(r, p, sh, cams, images, W, H, args)
     head_name = "bf16x8" if "bf16x8" in runs else list(runs)[-1]
     head = runs[head_name]
-    head_prec = head_name.split("x8")[0]
-    views = BATCH if "x8" in head_name else 1
+    head_prec = parse_run(head_name)[0]
+    views = head.get("views", 1)
 
     npx = W * H
     shC = 24
@@ -366,8 +378,9 @@ def main():
               "kernels": kernels, "hbm_copy_ceiling_GBps": copy_gbps,
               "runs": {k: {"iters_per_s": round(v["its"], 2), "ms_per_iter": round(v["ms_per_iter"], 4),
                            **({"batched_steps_per_s": round(v["steps_per_s"], 2)} if "steps_per_s" in v else {}),
-                           "storage": storage[k.split("x8")[0]], "raster_shape": v["shape"],
-                           **({"launch": "overlapped" if k.endswith("x8o") else "persistent"} if "x8" in k else {})}
+                           "storage": storage[parse_run(k)[0]], "raster_shape": v["shape"],
+                           **({"views_per_step": parse_run(k)[1],
+                               "launch": "overlapped" if parse_run(k)[2] else "persistent"} if parse_run(k)[1] else {})}
                        for k, v in runs.items()},
               "psnr": psnr, "psnr_protocol": f"mean PSNR over all {args.views} views after {args.psnr_iters} "
                                             f"iterations from the same start and seed; each mode against f32 "
