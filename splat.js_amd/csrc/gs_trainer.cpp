@@ -1555,11 +1555,14 @@ int gs_autotune(gs_trainer_t* h, const int32_t* cams, int32_t count, int32_t rep
   HIPCK(hipEventCreate(&e1));
   int rc = GS_OK;
   const int tiles[3] = {8, 16, 32}, batches[3] = {64, 128, 256};
+  for (int i = 0; i < 9; i++) {
+    memset(&out[i], 0, sizeof out[i]);
+    out[i].tile = tiles[i / 3];
+    out[i].batch = batches[i % 3];
+    out[i].status = GS_TUNE_TOO_MANY_TILES;   // until measured
+  }
   for (int i = 0; i < 9 && rc == GS_OK; i++) {
     gs_tune_entry& en = out[i];
-    memset(&en, 0, sizeof en);
-    en.tile = tiles[i / 3];
-    en.batch = batches[i % 3];
     bool fits = true;
     for (int k = 0; k < count && count > 1; k++)
       fits = fits && tiles_of(h->cams[ci[k]].c.w, h->cams[ci[k]].c.h, en.tile) <= kMaxSegTiles;
