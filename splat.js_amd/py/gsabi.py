@@ -191,6 +191,8 @@ class GSTrainer:
         self.opts = default_opts(**opts)
         h = C.c_void_p()
         rc = lib().gs_create(C.byref(self.opts), C.byref(h))
+        if rc == -1:
+            raise GsError("gs_create: an option is out of range (GS_E_ARG)")
         if rc != 0:
             raise GsError(f"gs_create failed ({rc}): no usable HIP device?")
         self._h = h

@@ -128,3 +128,28 @@ def test_config4_batched_sweep():
     t.stepViews(list(range(8)))
     assert np.isfinite(t.readGaussians()["data"]).all()
     t.destroy()
+
+
+def test_shape_and_autotune_arguments_are_checked():
+    p, sh, cams, images = _small(n=2000)
+    with pytest.raises(gsabi.GsError):
+        gsabi.GSTrainer(shDeg=2, maxSplats=p.shape[0], tile=12)
+    with pytest.raises(gsabi.GsError):
+        gsabi.GSTrainer(shDeg=2, maxSplats=p.shape[0], batch=96)
+    t = gsabi.GSTrainer(shDeg=2, maxSplats=p.shape[0], viewsPerStep=2)
+    t.setup({"data": p, "n": p.shape[0], "sh": sh}, cams, images, 200, 128, 2.0)
+    for tile, batch in ((16, 32), (4, 256), (64, 64)):
+        with pytest.raises(gsabi.GsError):
+            t.setRasterShape(tile, batch)
+    assert (t.info().tile, t.info().batch) == (16, 256)
+    with pytest.raises(gsabi.GsError):
+        t.autotune([0, 1, 2], reps=1)          # more views than viewsPerStep
+    with pytest.raises(gsabi.GsError):
+        t.autotune([-1], reps=1)               # the sweep takes explicit cameras
+    with pytest.raises(gsabi.GsError):
+        t.autotune([len(cams)], reps=1)
+    t.setRasterShape(32, 64)
+    assert (t.info().tile, t.info().batch) == (32, 64)
+    t.stepViews([0, 1])
+    assert np.isfinite(t.readGaussians()["data"]).all()
+    t.destroy()
