@@ -667,9 +667,9 @@ struct CountPrep {
 // Logical index of this count/scatter block. Workgroups are dispatched
 // round-robin over MI355X's 8 XCDs, so logical blocks 0..nb/8-1 run on the
 // first XCD, the next nb/8 on the second, and so on: each tile segment's
-// runs from neighbouring logical blocks (adjacent in memory) are then
-// written through one XCD's L2, which merges their partial lines before
-// they reach HBM. Any bijection partitions the work the same way.
+// runs from neighbouring logical blocks (adjacent in memory) are written
+// from one XCD. Measured ~0.5-1% faster than the identity order at the same
+// state. Any bijection partitions the work the same way.
 __device__ __forceinline__ int seg_logical_block(int nb) {
   const int i = (int)blockIdx.x;
   return nb % 8 == 0 ? (i % 8) * (nb / 8) + i / 8 : i;
