@@ -476,7 +476,10 @@ def run_config5(args):
         peak = max(peak, device_used_bytes())
         t.destroy()
         keys = st.lastKeys / V
-        bin_ms = prof["bin_sort"]["ms"] / max(prof["bin_sort"]["launches"], 1)
+        if "bin_sort_views" in prof:   # every view's binning in one launch per stage
+            bin_ms = prof["bin_sort_views"]["ms"] / max(prof["bin_sort_views"]["launches"], 1) / V
+        else:
+            bin_ms = prof["bin_sort"]["ms"] / max(prof["bin_sort"]["launches"], 1)
         out[prec] = {"iters_per_s": round(steps * V / dt, 2), "batched_steps_per_s": round(steps / dt, 3),
                      "keys_per_view": int(keys), "visible_per_view": int(st.lastVisible / V),
                      "sorted_keys_per_s": round(keys / (bin_ms * 1e-3)), "bin_sort_ms": round(bin_ms, 3),
