@@ -63,6 +63,11 @@ struct Shape {
   static constexpr int kChunks = kB / 64;
   static constexpr int kBW = kT / 8;            // 8x8 blocks per tile row
   static constexpr int kMasks = kT == 8 ? 1 : 4;
+  // occupancy the persistent backward is compiled for: 8 waves per SIMD (at
+  // most 64 VGPRs; left free, the compiler takes up to 80 there and the
+  // grid, sized by occupancy, shrinks); 8-pixel tiles with 128- or 256-key
+  // batches are held below that by their LDS anyway
+  static constexpr int kBwdWavesPerEU = (kT == 8 && kB > 64) ? 1 : 8;
   static_assert((kT == 8 || kT == 16 || kT == 32) && (kB == 64 || kB == 128 || kB == 256), "raster shape");
   __device__ static int mask_slot(int w) {
     if constexpr (kT == 8) {
@@ -295,20 +300,25 @@ __device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, 
   }
 }
 
-// ---- wave transpose reduction of 10 values (VALU lane exchanges) ----
-// Six halving steps, the lane exchanges inside a row of 16 first while the
-// values are many (DPP: 10 -> 5 with l ^ 15, row_mirror; 5 -> 3 with l ^ 7,
-// row_half_mirror; 3 -> 2 with l ^ 2 and 2 -> 1 with l ^ 1, quad_perm),
-// then the exchanges across rows on one value (l ^ 16: v_permlane16_swap;
-// l ^ 32: v_permlane32_swap), which measured ~1% faster over the backward
-// than the reverse order. In each halving step a lane keeps one half of its
-// values and adds its partner's copy of that half (an odd count is padded
-// with a zero), so after the last step each slot's 64-lane total sits in
-// exactly one lane and every other lane holds a padding zero. The swap
-// builtins return both operands after exchanging one half of the lanes, so
-// lo' + hi' is that sum whichever half a lane ends up with; the slot each
-// lane ends with is measured once per kernel by running the network on the
-// constants 1..10.
+// ---- wave transpose reduction of a visit's 10 values (VALU lane exchanges) ----
+// Six halving steps, one per lane-index bit, in this order: l ^ 15 (row_mirror,
+// across bit 3), l ^ 7 (row_half_mirror, bit 2), l ^ 2 and l ^ 1 (quad_perm),
+// l ^ 16 (v_permlane16_swap), l ^ 32 (v_permlane32_swap). In each a lane
+// keeps one half of its values and adds its partner's copy of that half (an
+// odd count is padded with a zero), so after the last step each slot's
+// 64-lane total sits in exactly one lane and every other lane holds a
+// padding zero. The slot each lane ends with is measured once per kernel by
+// running the network on constants.
+//
+// The first two steps separate the lower and upper half by 4-lane bank
+// (lanes 0-7 / 8-15 of a row, then 0-3, 8-11 / 4-7, 12-15), so they run as
+// two DPP adds per output, each writing one half through the DPP bank mask:
+// no selects. They run on each visit's terms as they are made (10 -> 5 -> 3
+// values per lane); the last four steps run once per two visits on their
+// 3 + 3 values (keep/send selects and one DPP add per output for the quad
+// steps; the cross-row swaps on one value each). The asm blocks start with
+// two wait states for the DPP read of a value the previous VALU instruction
+// wrote, and end with two for a DPP read of their results that follows.
 template <int kCtrl>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, 0xf, 0xf, false);
@@ -353,50 +363,105 @@ __device__ __forceinline__ void dpp_halve(const T* v, T* out, bool upper) {
   }
 }
 
-template <typename T>
-__device__ __forceinline__ T wave_reduce10(const T* v) {
-  const uint32_t lane = __lane_id();
-  T a[5], b3[3], c[2], d[1], e[1], f[1];
-  dpp_halve<T, 10, 0x140>(v, a, (lane & 8) != 0);
-  dpp_halve<T, 5, 0x141>(a, b3, (lane & 4) != 0);
-  dpp_halve<T, 3, 0x4E>(b3, c, (lane & 2) != 0);
-  dpp_halve<T, 2, 0xB1>(c, d, (lane & 1) != 0);
-  swap_halve<T, 1, false>(d, e);
-  swap_halve<T, 1, true>(e, f);
-  return f[0];
-}
 
-// The same network on 20 values (two visits' terms): the same six steps,
-// with twice the values in the first ones; it serves two visits, so the
-// single-value cross-row exchanges run once per two visits.
+// Row step 1 (l ^ 15) on one visit's 10 values: lanes 0-7 of each row keep
+// values 0-4, lanes 8-15 values 5-9.
+#define GS_DPP_M(OP, O, LO, HI, CTRL, MLO, MHI) \
+  OP " " O ", " LO ", " LO " " CTRL " row_mask:0xf bank_mask:" MLO "\n\t" \
+  OP " " O ", " HI ", " HI " " CTRL " row_mask:0xf bank_mask:" MHI "\n\t"
+#define GS_ROW_STEP1(OP)                                              \
+  "s_nop 1\n\t"                                                      \
+  GS_DPP_M(OP, "%0", "%5", "%10", "row_mirror", "0x3", "0xc")         \
+  GS_DPP_M(OP, "%1", "%6", "%11", "row_mirror", "0x3", "0xc")         \
+  GS_DPP_M(OP, "%2", "%7", "%12", "row_mirror", "0x3", "0xc")         \
+  GS_DPP_M(OP, "%3", "%8", "%13", "row_mirror", "0x3", "0xc")         \
+  GS_DPP_M(OP, "%4", "%9", "%14", "row_mirror", "0x3", "0xc")         \
+  "s_nop 1"
+// Row step 2 (l ^ 7) on 5 values: lanes 0-3 and 8-11 keep values 0-2,
+// lanes 4-7 and 12-15 values 3, 4 and a zero.
+#define GS_ROW_STEP2(OP)                                              \
+  "s_nop 1\n\t"                                                      \
+  GS_DPP_M(OP, "%0", "%3", "%6", "row_half_mirror", "0x5", "0xa")     \
+  GS_DPP_M(OP, "%1", "%4", "%7", "row_half_mirror", "0x5", "0xa")     \
+  "v_mov_b32 %2, 0\n\t"                                              \
+  OP " %2, %5, %5 row_half_mirror row_mask:0xf bank_mask:0x5\n\t"    \
+  "s_nop 1"
+
 template <typename T>
-__device__ __forceinline__ T wave_reduce20(const T* v) {
+__device__ __forceinline__ void visit_halve(const T* v, T* h3) {
+  static_assert(sizeof(T) == 4, "32-bit terms");
+  T h5[5];
+  if constexpr (std::is_same<T, float>::value) {
+    asm volatile(GS_ROW_STEP1("v_add_f32_dpp")
+                 : "=&v"(h5[0]), "=&v"(h5[1]), "=&v"(h5[2]), "=&v"(h5[3]), "=&v"(h5[4])
+                 : "v"(v[0]), "v"(v[1]), "v"(v[2]), "v"(v[3]), "v"(v[4]),
+                   "v"(v[5]), "v"(v[6]), "v"(v[7]), "v"(v[8]), "v"(v[9]));
+    asm volatile(GS_ROW_STEP2("v_add_f32_dpp")
+                 : "=&v"(h3[0]), "=&v"(h3[1]), "=&v"(h3[2])
+                 : "v"(h5[0]), "v"(h5[1]), "v"(h5[2]), "v"(h5[3]), "v"(h5[4]));
+  } else {
+    asm volatile(GS_ROW_STEP1("v_add_u32_dpp")
+                 : "=&v"(h5[0]), "=&v"(h5[1]), "=&v"(h5[2]), "=&v"(h5[3]), "=&v"(h5[4])
+                 : "v"(v[0]), "v"(v[1]), "v"(v[2]), "v"(v[3]), "v"(v[4]),
+                   "v"(v[5]), "v"(v[6]), "v"(v[7]), "v"(v[8]), "v"(v[9]));
+    asm volatile(GS_ROW_STEP2("v_add_u32_dpp")
+                 : "=&v"(h3[0]), "=&v"(h3[1]), "=&v"(h3[2])
+                 : "v"(h5[0]), "v"(h5[1]), "v"(h5[2]), "v"(h5[3]), "v"(h5[4]));
+  }
+}
+#undef GS_ROW_STEP2
+#undef GS_ROW_STEP1
+#undef GS_DPP_M
+
+// The last four steps on two visits' row-halved values (the waiting one's
+// in p, the new one's in h): 20 wave totals, slots 0-9 the waiting visit's.
+template <typename T>
+__device__ __forceinline__ T pair_reduce(const T* p, const T* h) {
   const uint32_t lane = __lane_id();
-  T a[10], b5[5], c[3], d[2], e[1], g[1];
-  dpp_halve<T, 20, 0x140>(v, a, (lane & 8) != 0);
-  dpp_halve<T, 10, 0x141>(a, b5, (lane & 4) != 0);
-  dpp_halve<T, 5, 0x4E>(b5, c, (lane & 2) != 0);
+  const T v[6] = {p[0], p[1], p[2], h[0], h[1], h[2]};
+  T c[3], d[2], e[1], g[1];
+  dpp_halve<T, 6, 0x4E>(v, c, (lane & 2) != 0);
   dpp_halve<T, 3, 0xB1>(c, d, (lane & 1) != 0);
   swap_halve<T, 2, false>(d, e);
   swap_halve<T, 1, true>(e, g);
   return g[0];
 }
 
-// slot whose 64-lane total this lane holds after wave_reduce20 (-1: none)
-__device__ __forceinline__ int reduce20_slot() {
-  uint32_t v[20];
+// The last four steps on one visit's row-halved values (a visit left
+// without a partner at the end of the tile).
+template <typename T>
+__device__ __forceinline__ T single_reduce(const T* h) {
+  const uint32_t lane = __lane_id();
+  T d[2], e[1], f[1], g[1];
+  dpp_halve<T, 3, 0x4E>(h, d, (lane & 2) != 0);
+  dpp_halve<T, 2, 0xB1>(d, e, (lane & 1) != 0);
+  swap_halve<T, 1, false>(e, f);
+  swap_halve<T, 1, true>(f, g);
+  return g[0];
+}
+
+// slot whose 64-lane total this lane holds after pair_reduce (0-9: the
+// waiting visit's terms, 10-19: the new visit's; -1: none)
+__device__ __forceinline__ int pair_slot() {
+  uint32_t va[10], vb[10], ha[3], hb[3];
 #pragma unroll
-  for (int i = 0; i < 20; i++) v[i] = (uint32_t)(i + 1);
-  const int s = (int)(wave_reduce20<uint32_t>(v) / 64u) - 1;
+  for (int i = 0; i < 10; i++) {
+    va[i] = (uint32_t)(i + 1);
+    vb[i] = (uint32_t)(i + 11);
+  }
+  visit_halve<uint32_t>(va, ha);
+  visit_halve<uint32_t>(vb, hb);
+  const int s = (int)(pair_reduce<uint32_t>(ha, hb) / 64u) - 1;
   return s >= 0 && s < 20 ? s : -1;
 }
 
-// slot whose 64-lane total this lane holds after wave_reduce10 (-1: none)
-__device__ __forceinline__ int reduce10_slot() {
-  uint32_t v[10];
+// slot whose 64-lane total this lane holds after single_reduce (-1: none)
+__device__ __forceinline__ int single_slot() {
+  uint32_t v[10], h[3];
 #pragma unroll
   for (int i = 0; i < 10; i++) v[i] = (uint32_t)(i + 1);
-  const int s = (int)(wave_reduce10<uint32_t>(v) / 64u) - 1;
+  visit_halve<uint32_t>(v, h);
+  const int s = (int)(single_reduce<uint32_t>(h) / 64u) - 1;
   return s >= 0 && s < 10 ? s : -1;
 }
 
@@ -440,17 +505,18 @@ __device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, 
   const uint2 rg = b.ranges[tile];
   const uint32_t s = __builtin_amdgcn_readfirstlane(rg.x);
   const int64_t pi = (int64_t)y * cam.W + x;
-  const int mySlot = reduce10_slot();
+  const int mySlot = single_slot();
   const bool flusher = mySlot >= 0;
-  const int mySlot20 = reduce20_slot();
+  const int mySlot20 = pair_slot();
   const bool flusher20 = mySlot20 >= 0;
-  // A visit's per-lane terms wait for the next visit's, and the two are
-  // summed over the wave in one wave_reduce20 (slots 0-9 the waiting
-  // visit's, 10-19 the new one's); the sums are added to the accumulators
-  // in a different order than visited, which changes nothing (integer
-  // sums; f32 atomics have no fixed order anyway).
+  // A visit's terms, halved twice inside each row as they are made (3
+  // values per lane), wait for the next visit's, and the two finish in one
+  // pair_reduce (slots 0-9 the waiting visit's, 10-19 the new one's); the
+  // sums are added to the accumulators in a different order than visited,
+  // which changes nothing (integer sums; f32 atomics have no fixed order
+  // anyway).
   using AccT = typename std::conditional<kAccum == 0, uint32_t, float>::type;
-  AccT pend[10];
+  AccT pend[3];
   uint32_t pendGid = 0;
   bool hasPend = false;
   float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -599,28 +665,24 @@ __device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, 
 #pragma unroll
           for (int q = 0; q < 10; q++) cur[q] = v[q];
         }
+        AccT h[3];
+        visit_halve<AccT>(cur, h);
         if (hasPend) {
-          AccT both[20];
-#pragma unroll
-          for (int q = 0; q < 10; q++) {
-            both[q] = pend[q];
-            both[10 + q] = cur[q];
-          }
-          const AccT r = wave_reduce20<AccT>(both);
+          const AccT r = pair_reduce<AccT>(pend, h);
           const uint32_t g2 = mySlot20 < 10 ? pendGid : gid;
           flush_sum<kAccum>(accum, g2, mySlot20 % 10, r, flusher20);
-          hasPend = false;
-        } else {
-#pragma unroll
-          for (int q = 0; q < 10; q++) pend[q] = cur[q];
-          pendGid = gid;
-          hasPend = true;
         }
+        // copied on both paths (after a pair it is never read), so the
+        // waiting values need no merge of two register sets at the join
+#pragma unroll
+        for (int q = 0; q < 3; q++) pend[q] = h[q];
+        pendGid = gid;
+        hasPend = !hasPend;
       }
     }
     hi = lo;
   }
-  if (hasPend) flush_sum<kAccum>(accum, pendGid, mySlot, wave_reduce10<AccT>(pend), flusher);
+  if (hasPend) flush_sum<kAccum>(accum, pendGid, mySlot, single_reduce<AccT>(pend), flusher);
 }
 
 template <class S, int kLoss>
@@ -671,8 +733,8 @@ __global__ __launch_bounds__(S::kThreads) void k_raster_fwd_views(const ViewBloc
 }
 
 template <class S, int kAccum, bool kClampColour>
-__global__ __launch_bounds__(S::kThreads) void k_raster_bwd_views(const ViewBlock* __restrict__ vb, Cut cut,
-                                                                  uint32_t* queue) {
+__global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::kBwdWavesPerEU))) void k_raster_bwd_views(
+    const ViewBlock* __restrict__ vb, Cut cut, uint32_t* queue) {
   ViewItem it;
   while (next_item(vb, queue, it)) {
     const Cam cam = vb->cams[it.v];
