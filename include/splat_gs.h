@@ -193,8 +193,10 @@ int gs_step(gs_trainer_t* h, int32_t cam);
  * opts.graph the whole step is captured once per shape (view sizes, n) and
  * replayed as a hipGraph; the per-step values (cameras, Adam constants) come
  * from device memory that a copy before each step rewrites. count <=
- * opts.viewsPerStep; needs the Charbonnier loss, the tile-segment sort,
- * views of at most 16384 tiles and camOpt off. Enqueues only. */
+ * opts.viewsPerStep; needs the Charbonnier loss, the tile-segment sort and
+ * camOpt off (a view of more than 16384 tiles, past the tile-segment
+ * binning's LDS tables, is binned by the depth-then-tile radix path inside
+ * the same step). Enqueues only. */
 int gs_step_views(gs_trainer_t* h, const int32_t* cams, int32_t count);
 
 /* `count` training steps enqueued in one call, exactly as `count` gs_step
@@ -389,10 +391,10 @@ int gs_set_raster_shape(gs_trainer_t* h, int32_t tile, int32_t batch);
 /* One shape of an autotuning sweep. */
 typedef struct gs_tune_entry {
   int32_t tile, batch;
-  int32_t status;       /* GS_TUNE_MEASURED, GS_TUNE_TOO_MANY_TILES (a view has more tiles than
-                           the batched step's tile-segment binning takes: 16384) or
-                           GS_TUNE_MORE_DROPS (the entry budget drops more keys than at
-                           16-pixel tiles: faster only by rendering less) */
+  int32_t status;       /* GS_TUNE_MEASURED or GS_TUNE_MORE_DROPS (the entry budget drops
+                           more keys than at 16-pixel tiles: faster only by rendering
+                           less). GS_TUNE_TOO_MANY_TILES keeps its value but is no longer
+                           returned: every shape is measured */
   int32_t pad;
   double ms;            /* mean time of one pass (status GS_TUNE_MEASURED) */
   int64_t keys;         /* sorted keys of one pass, all views */

@@ -83,6 +83,8 @@ struct SegJob {
   int nb;                 // count/scatter blocks (ceil(n / seg_group(n)))
   int tilesX, tile, numTiles;
   int recFmt;
+  int skip;               // batched only: a view of more than kMaxSegTiles tiles, binned by the
+                          // depth-then-tile radix path instead (every stage's blocks leave at once)
   float4* rec;            // raster records (the projection's output)
   uint32_t* depthKey;
   uint2* rect;
@@ -104,7 +106,8 @@ struct SegJob {
 };
 
 // All views' binning stages, one launch each; hJobs: the host copy of the
-// device array dJobs (for the launch shapes).
+// device array dJobs (for the launch shapes). Jobs with `skip` set are left
+// out (their blocks return at once).
 void launch_bin_sort_views(hipStream_t st, const SegJob* dJobs, const SegJob* hJobs, int V);
 
 struct BinOut {

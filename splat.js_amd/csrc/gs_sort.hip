@@ -723,6 +723,7 @@ __device__ __forceinline__ int2 seg_block_chunks(int64_t n, const uint32_t* __re
 template <bool kJobs>
 __global__ __launch_bounds__(kSegThreads) void k_seg_count(SegJob arg, const SegJob* __restrict__ jobs) {
   const SegJob j = seg_job<kJobs>(arg, jobs, blockIdx.y);
+  if (kJobs && j.skip) return;
   const int64_t n = j.n;
   const int nb = j.nb, tilesX = j.tilesX, numTiles = j.numTiles;
   const uint32_t* __restrict__ cnt = j.cnt;
@@ -766,6 +767,7 @@ __global__ __launch_bounds__(kSegThreads) void k_seg_count(SegJob arg, const Seg
 template <bool kJobs>
 __global__ __launch_bounds__(kBlock) void k_seg_colscan(SegJob arg, const SegJob* __restrict__ jobs) {
   const SegJob j = seg_job<kJobs>(arg, jobs, blockIdx.y);
+  if (kJobs && j.skip) return;
   uint32_t* __restrict__ table = j.table;
   const int nb = j.nb, numTiles = j.numTiles;
   uint32_t* __restrict__ tileCnt = j.tileCnt;
@@ -839,6 +841,7 @@ constexpr int kScanQuarter = kScanChunk / 4;   // tiles per thread
 template <bool kJobs>
 __global__ __launch_bounds__(kScanThreads) void k_seg_scan(SegJob arg, const SegJob* __restrict__ jobs) {
   const SegJob j = seg_job<kJobs>(arg, jobs, blockIdx.y);
+  if (kJobs && j.skip) return;
   const uint32_t* __restrict__ tileCnt = j.tileCnt;
   const int numTiles = j.numTiles, nb = j.nb;
   const uint64_t cap = j.cap;
@@ -1002,6 +1005,7 @@ struct ScatterPrep {
 template <bool kJobs>
 __global__ __launch_bounds__(kSegThreads) void k_seg_scatter(SegJob arg, const SegJob* __restrict__ jobs) {
   const SegJob j = seg_job<kJobs>(arg, jobs, blockIdx.y);
+  if (kJobs && j.skip) return;
   const uint32_t* __restrict__ blockStart = j.blockStart;
   const uint32_t* __restrict__ cnt = j.cnt;
   const uint2* __restrict__ rect = j.rect;
@@ -1180,6 +1184,7 @@ __device__ void seg_merge(const uint64_t* A, uint32_t na, const uint64_t* B, uin
 template <bool kJobs>
 __global__ __launch_bounds__(kBlock) void k_seg_sort(SegJob arg, const SegJob* __restrict__ jobs) {
   const SegJob j = seg_job<kJobs>(arg, jobs, kJobs ? blockIdx.x : 0);
+  if (kJobs && j.skip) return;
   const uint32_t rank = kJobs ? blockIdx.y : blockIdx.x;
   if (kJobs && rank >= (uint32_t)j.numTiles) return;
   uint64_t* __restrict__ keys = j.keys;
@@ -1310,8 +1315,9 @@ static void seg_launches(hipStream_t st, const SegJob& j, const SegJob* jobs, in
 
 void launch_bin_sort_views(hipStream_t st, const SegJob* dJobs, const SegJob* hJobs, int V) {
   int maxTiles = 0;
-  for (int k = 0; k < V; k++) maxTiles = maxTiles > hJobs[k].numTiles ? maxTiles : hJobs[k].numTiles;
-  seg_launches<true>(st, hJobs[0], dJobs, V, maxTiles);
+  for (int k = 0; k < V; k++)
+    if (!hJobs[k].skip) maxTiles = maxTiles > hJobs[k].numTiles ? maxTiles : hJobs[k].numTiles;
+  if (maxTiles > 0) seg_launches<true>(st, hJobs[0], dJobs, V, maxTiles);
 }
 
 void launch_bin_sort(hipStream_t st, const BinArgs& a, BinOut& out) {

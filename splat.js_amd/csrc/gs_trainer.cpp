@@ -138,7 +138,9 @@ struct gs_trainer {
   uint32_t rng = 0;
   int lastCam = -1;
   int lastNumTiles = 0;   // tiles of the last raster (refine's maxTile)
-  bool lastSegPath = false;   // the last raster binned by tile segments (raw tile counts in sort.tileCnt)
+  bool lastSegPath = false;   // the last raster binned by tile segments (raw tile counts in lastTileCnt)
+  const uint32_t* lastTileCnt = nullptr;   // its per-tile counts before the budget (tile segments)
+  const uint2* lastRanges = nullptr;       // its tile ranges
   BinOut lastBin{};
 
   // batched views (gs_step_views): buffer slot k holds view k's records,
@@ -431,6 +433,16 @@ Cut make_cut(const gs_trainer* h) {
   return c;
 }
 
+// The depth-then-tile radix path (views of more than kMaxSegTiles tiles)
+// ends its tile sort in a.tvA after an even number of passes and in a.tvB
+// after an odd one; these set the two so that the sorted values land in
+// `ids`, with `scratch` the other buffer.
+void radix_values_into(BinArgs& a, uint32_t* ids, uint32_t* scratch) {
+  const int passes = (tile_bits(a.numTiles) + 7) / 8;
+  a.tvA = passes % 2 ? scratch : ids;
+  a.tvB = passes % 2 ? ids : scratch;
+}
+
 // ---- one raster: project -> bin/sort -> forward (+ loss) ----
 int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
   const int64_t n = h->n;
@@ -442,6 +454,8 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
     // Gaussians, which the projection sums on the way
     const bool segPath = h->o.sortMode == GS_SORT_TILE_SEGMENTS && cam.numTiles <= kMaxSegTiles;
     h->lastSegPath = segPath;
+    h->lastTileCnt = h->sort.tileCnt;
+    h->lastRanges = h->ranges;
     launch_project(h->st, h->shDeg, h->prec, n, h->cap, cam, make_cut(h), h->P, h->SH, h->rec, h->depthKey,
                    h->rect, h->cnt, segPath ? h->sort.chunkKeys : nullptr);
   }
@@ -814,6 +828,38 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
     {
       Scope sc(h, "bin_sort_views", 0.0);
       launch_bin_sort_views(h->st, (const SegJob*)((const char*)h->dVB + offsetof(ViewBlock, seg)), vb.seg, V);
+      // views above the segment path's tiles, one after another by the
+      // depth-then-tile radix path into their own slots (its key buffers are
+      // slot 0's segment keys, free again once the launches above are done)
+      for (int k = 0; k < V; k++) {
+        if (!vb.seg[k].skip) continue;
+        const gs_trainer::ViewSlot& sl = h->vslots[k];
+        BinArgs a{};
+        a.mode = GS_SORT_TILE_SEGMENTS;
+        a.tile = vb.cams[k].tile;
+        a.tilesX = vb.cams[k].tilesX;
+        a.numTiles = vb.cams[k].numTiles;
+        a.n = n;
+        a.cap = (uint64_t)h->entriesCap;
+        a.dN = h->dN;
+        a.cnt = sl.cnt;
+        a.rect = sl.rect;
+        a.depthKey = sl.depthKey;
+        a.rec = sl.rec;
+        a.recFmt = h->recFmt;
+        a.dkA = h->dkA; a.dvA = h->dvA; a.dkB = h->dkB; a.dvB = h->dvB;
+        a.tkA = h->tkA; a.tkB = h->tkB;
+        radix_values_into(a, sl.ids, h->tvB);
+        a.numKeys = h->viewStats + k;
+        a.numVisible = h->viewStats + kMaxViews + k;
+        a.totalKeys = sl.totalKeys;
+        a.ranges = sl.ranges;
+        a.tileOrder = sl.tileOrder;
+        a.stats = statsBuf;
+        a.scratch = h->sort;
+        BinOut out{};
+        launch_bin_sort(h->st, a, out);
+      }
     }
     if (dbg) {
       HIPCK(hipStreamSynchronize(h->st));
@@ -832,7 +878,7 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
     {
       Scope sc(h, "project", (double)n * (56.0 + 4.0 * h->shC + 48.0 + 16.0));
       launch_project_view(h->st, h->shDeg, h->prec, n, h->cap, h->dVB, k, cut, h->P, h->SH, sl.rec, h->depthKey,
-                          h->rect, sl.cnt, h->sort.chunkKeys);
+                          h->rect, sl.cnt, cam.numTiles <= kMaxSegTiles ? h->sort.chunkKeys : nullptr);
     }
     BinArgs a{};
     a.mode = GS_SORT_TILE_SEGMENTS;
@@ -849,6 +895,8 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
     a.recFmt = h->recFmt;
     a.dkA = h->dkA; a.dvA = h->dvA; a.dkB = h->dkB; a.dvB = h->dvB;
     a.tkA = h->tkA; a.tkB = h->tkB; a.tvA = sl.ids; a.tvB = h->tvB;
+    // above the segment path's tiles launch_bin_sort takes the radix path
+    if (cam.numTiles > kMaxSegTiles) radix_values_into(a, sl.ids, h->tvB);
     a.numKeys = h->viewStats + k;
     a.numVisible = h->viewStats + kMaxViews + k;
     a.totalKeys = h->totalKeys;
@@ -877,7 +925,11 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
     }
   }
   h->lastNumTiles = vb.cams[V - 1].numTiles;
-  h->lastSegPath = true;
+  h->lastSegPath = vb.cams[V - 1].numTiles <= kMaxSegTiles;
+  // the last view's raw tile counts: its own slot's with batched binning,
+  // the shared buffer (which it binned last) otherwise
+  h->lastTileCnt = h->batchedBin && !overlap ? h->vslots[V - 1].tileCnt : h->sort.tileCnt;
+  h->lastRanges = h->vslots[V - 1].ranges;
   h->lastViews = V;
   if (overlap) {
     HIPCK(hipEventRecord(h->evJoin, h->st2));
@@ -938,9 +990,6 @@ int fill_views(gs_trainer* h, const int* ci, int count, int activeDeg, double* s
     CamMeta& meta = h->cams[ci[k]];
     meta.activeDeg = activeDeg;
     vb.cams[k] = make_cam_meta(h, meta, activeDeg);
-    if (vb.cams[k].numTiles > kMaxSegTiles)
-      return fail(h, GS_E_LIMIT, "stepViews: a view of more than " + std::to_string(kMaxSegTiles) +
-                                     " tiles (the tile-segment sort's limit)");
     const gs_trainer::ViewSlot& sl = h->vslots[k];
     RasterBufs& b = vb.bufs[k];
     b.recFmt = h->recFmt;
@@ -964,12 +1013,15 @@ int fill_views(gs_trainer* h, const int* ci, int count, int activeDeg, double* s
       j.tilesX = vb.cams[k].tilesX;
       j.tile = vb.cams[k].tile;
       j.numTiles = vb.cams[k].numTiles;
+      // above the segment path's tiles the view bins by the radix path
+      // (enqueue_views), which needs no chunk sums from the projection
+      j.skip = vb.cams[k].numTiles > kMaxSegTiles;
       j.recFmt = h->recFmt;
       j.rec = sl.rec;
       j.depthKey = sl.depthKey;
       j.rect = sl.rect;
       j.cnt = sl.cnt;
-      j.chunkKeys = sl.chunkKeys;
+      j.chunkKeys = j.skip ? nullptr : sl.chunkKeys;
       j.blockStart = sl.blockStart;
       j.table = sl.table;
       j.visPart = sl.visPart;
@@ -1559,17 +1611,9 @@ int gs_autotune(gs_trainer_t* h, const int32_t* cams, int32_t count, int32_t rep
     memset(&out[i], 0, sizeof out[i]);
     out[i].tile = tiles[i / 3];
     out[i].batch = batches[i % 3];
-    out[i].status = GS_TUNE_TOO_MANY_TILES;   // until measured
   }
   for (int i = 0; i < 9 && rc == GS_OK; i++) {
     gs_tune_entry& en = out[i];
-    bool fits = true;
-    for (int k = 0; k < count && count > 1; k++)
-      fits = fits && tiles_of(h->cams[ci[k]].c.w, h->cams[ci[k]].c.h, en.tile) <= kMaxSegTiles;
-    if (!fits) {
-      en.status = GS_TUNE_TOO_MANY_TILES;
-      continue;
-    }
     set_shape(h, RasterShape{en.tile, en.batch});
     // warm-up pass: also gives the pass's keys and budget drops
     if (hipMemsetAsync(h->evalStats, 0, 32, h->st) != hipSuccess) { rc = hipfail(h, "autotune", hipGetLastError()); break; }
@@ -2006,13 +2050,13 @@ int gs_refine(gs_trainer_t* h, const gs_refine_opts* ro, gs_refine_result* out) 
   // per-tile counts), and the cumulative entry-budget drops
   if (h->lastNumTiles > 0 && h->lastSegPath) {
     std::vector<uint32_t> tc((size_t)h->lastNumTiles);
-    HIPCK(hipMemcpy(tc.data(), h->sort.tileCnt, tc.size() * 4, hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpy(tc.data(), h->lastTileCnt, tc.size() * 4, hipMemcpyDeviceToHost));
     for (uint32_t c : tc) out->maxTile = std::max<int64_t>(out->maxTile, c);
   } else if (h->lastNumTiles > 0) {
     // the radix strategies hold no per-tile totals; their ranges lack only
     // keys past the budget
     std::vector<uint32_t> rg(2 * (size_t)h->lastNumTiles);
-    HIPCK(hipMemcpy(rg.data(), h->ranges, rg.size() * 4, hipMemcpyDeviceToHost));
+    HIPCK(hipMemcpy(rg.data(), h->lastRanges, rg.size() * 4, hipMemcpyDeviceToHost));
     for (int t = 0; t < h->lastNumTiles; t++) out->maxTile = std::max<int64_t>(out->maxTile, rg[2 * t + 1] - rg[2 * t]);
   }
   {

@@ -9,8 +9,9 @@ fastest eligible one.
   measured shape that drops no more keys to the entry budget than 16-pixel
   tiles do; a shape dropping more is flagged, not chosen.
 * Config 4 (1M Gaussians, 8 views from 800x450 to 1920x1080, bf16 storage,
-  one batched pass): views above the batched binning's 16,384 tiles make
-  the 8-pixel shapes unavailable; the others are measured and printed.
+  one batched pass): every shape is measured and printed, the 8-pixel ones
+  too (their views above 16,384 tiles bin by the radix path inside the
+  batched step).
 """
 import numpy as np
 import pytest
@@ -44,8 +45,6 @@ def _check_rule(entries):
     d16 = entries[5]["dropped"]
     assert entries[5]["status"] == gsabi.GS_TUNE_MEASURED
     for e in entries:
-        if e["status"] == gsabi.GS_TUNE_TOO_MANY_TILES:
-            continue
         assert e["keys"] > 0
         want = gsabi.GS_TUNE_MORE_DROPS if e["dropped"] > d16 else gsabi.GS_TUNE_MEASURED
         assert e["status"] == want, e
@@ -123,7 +122,7 @@ def test_config4_batched_sweep():
               f"keys {e['keys']} dropped {e['dropped']}", flush=True)
     best = _check_rule(res)
     big = any(((c["w"] + 7) // 8) * ((c["h"] + 7) // 8) > 16384 for c in cams)
-    assert big and all(e["status"] == gsabi.GS_TUNE_TOO_MANY_TILES for e in res[:3])
+    assert big and all(e["status"] != gsabi.GS_TUNE_TOO_MANY_TILES for e in res)
     assert (t.info().tile, t.info().batch) == (best["tile"], best["batch"])
     t.stepViews(list(range(8)))
     assert np.isfinite(t.readGaussians()["data"]).all()

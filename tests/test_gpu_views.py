@@ -6,7 +6,9 @@ views, one Adam step; replayed as a hipGraph when the step's shape repeats.
   step_views) on uneven view sizes: every view's image, T, end, dL/dC and
   fixed-point sums, the summed gradients, and the params and SH after Adam,
   over several steps, in f32 and 16-bit storage, at the default raster shape
-  and at 8- and 32-pixel tiles with smaller LDS batches.
+  and at 8- and 32-pixel tiles with smaller LDS batches; with 8-pixel tiles
+  on a view too large for the tile-segment binning, which then bins by the
+  radix path inside the batched step.
 * Graph replay equals eager execution, step after step, with single-view
   steps, renders and readbacks in between.
 * At full size (config 4: 1M Gaussians, 8 views from 800x450 to 1920x1080,
@@ -75,6 +77,44 @@ def test_views_bit_exact_against_restatement(precision, tile, batch):
         np.testing.assert_array_equal(_bits(g["sh"]), _bits(ref.SH), err_msg=f"sh step {s}")
         assert t.info().iter == ref.iter
     t.destroy()
+
+
+def test_view_above_segment_tiles_bit_exact():
+    """8-pixel tiles on a 1040x1040 view: 16,900 tiles, more than the
+    tile-segment binning's 16,384, so that view bins by the depth-then-tile
+    radix path into its own slot while the others take the batched segment
+    launches. Bit-exact against the restatement (eager debug steps), and a
+    trainer replaying the step from a hipGraph ends in the same state."""
+    sizes = [(1040, 1040), (160, 96), (200, 120)]
+    p, sh, cams, images = _scene(n=6000, seed=4, sizes=sizes)
+    kw = dict(shDeg=2, maxSplats=p.shape[0], viewsPerStep=3, tile=8, batch=128)
+    t = gsabi.GSTrainer(graph=0, **kw)
+    g = gsabi.GSTrainer(graph=1, **kw)
+    for x in (t, g):
+        x.setup({"data": p, "n": p.shape[0], "sh": sh}, cams, images, 1040, 1040, 2.0)
+    ref = OracleTrainer(p, sh, cams, images, 2.0, shDeg=2, precision="f32", tile=8)
+    t.iter = g.iter = ref.iter = 1996
+    for s, order in enumerate(([0, 1, 2], [1, 0, 2], [2, 1, 0])):
+        print(f"batched step {s}: views {order}", flush=True)
+        outs = t.step_views_debug(order)
+        g.stepViews(order)
+        per, gF, sG = ref.step_views(order)
+        for k, (o, r) in enumerate(zip(outs, per)):
+            lab = f"step {s} view {order[k]}"
+            np.testing.assert_array_equal(_bits(o["img"]), _bits(r["img"]), err_msg=lab)
+            np.testing.assert_array_equal(_bits(o["T"]), _bits(r["T"]), err_msg=lab)
+            np.testing.assert_array_equal(o["end"], r["end"], err_msg=lab)
+            np.testing.assert_array_equal(o["accum"], r["accum"], err_msg=lab)
+            tr = o["tileRanges"].astype(np.int64)
+            np.testing.assert_array_equal(tr[:, 1] - tr[:, 0], np.diff(r["ts"].astype(np.int64)), err_msg=lab)
+        np.testing.assert_array_equal(_bits(outs[0]["gradF"][:, :14]), _bits(gF[:, :14]), err_msg=f"step {s}")
+        a, b = t.readGaussians(), g.readGaussians()
+        np.testing.assert_array_equal(_bits(a["data"]), _bits(ref.P), err_msg=f"params step {s}")
+        np.testing.assert_array_equal(_bits(a["sh"]), _bits(ref.SH), err_msg=f"sh step {s}")
+        np.testing.assert_array_equal(_bits(b["data"]), _bits(ref.P), err_msg=f"graph params step {s}")
+        np.testing.assert_array_equal(_bits(b["sh"]), _bits(ref.SH), err_msg=f"graph sh step {s}")
+    t.destroy()
+    g.destroy()
 
 
 def test_graph_replay_and_launch_forms_equal_eager():
