@@ -319,6 +319,9 @@ __device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, 
 // steps; the cross-row swaps on one value each). The asm blocks start with
 // two wait states for the DPP read of a value the previous VALU instruction
 // wrote, and end with two for a DPP read of their results that follows.
+// Every step runs with all 64 lanes active (the visit loop branches only on
+// wave-uniform votes, and every wave of a block is full): a DPP add whose
+// source lane were inactive would leave its output unwritten.
 template <int kCtrl>
 __device__ __forceinline__ uint32_t dpp(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, kCtrl, 0xf, 0xf, false);
