@@ -374,7 +374,7 @@ def main():
                            "Adam after each view (single) or once per batch (batched)",
               "views_per_step": views,
               "keys_per_view": round(head["keys"]), "visible_per_view": round(head["vis"]),
-              "sorted_keys_per_s": round(head["keys"] * head["its"]),
+              "sorted_keys_per_s": round(head["keys"] * head["its"]),   # over the whole step
               "kernels": kernels, "hbm_copy_ceiling_GBps": copy_gbps,
               "runs": {k: {"iters_per_s": round(v["its"], 2), "ms_per_iter": round(v["ms_per_iter"], 4),
                            **({"batched_steps_per_s": round(v["steps_per_s"], 2)} if "steps_per_s" in v else {}),
@@ -386,6 +386,11 @@ def main():
                                             f"iterations from the same start and seed; each mode against f32 "
                                             f"with the same schedule (single-view, or batches of {BATCH})",
               "fp64_error": fp64}
+    # keys per second of the binning launch itself (K / t_sort): the keys one
+    # launch bins over its event-timed duration
+    kb = kernels.get("bin_sort_views" if views > 1 else "bin_sort")
+    if kb and kb.get("avg_ms"):
+        config["binning_keys_per_s"] = round(head["keys"] * views / (kb["avg_ms"] * 1e-3))
     if "f32" in runs:
         config["ratio_vs_f32_single"] = round(head["its"] / runs["f32"]["its"], 3)
     line = {
