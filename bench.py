@@ -91,6 +91,9 @@ def algorithmic_bytes(name, n, vis, keys, npx, shC, prec, views=1):
     project   n*(56 params + 16 key/rect/count) + vis*(sb*shC SH + 48 record write)
     raster fwd  keys*(4 id + rec) + px*(16 img + 4 T + 4 end + 16 dL/dC + 4 target)
     raster bwd  keys*(4 id + rec + 16) + px*(16 dL/dC + 4 T + 4 end) + vis*40 gradient sums
+    raster fwd+bwd (one launch, each tile's backward right after its forward)
+                keys*(4 id + rec + 16) + px*44 + vis*40: the records and ids once, the
+                forward's per-pixel traffic, the gradient sums
     update    n*(48 sums + 4 count + 2*56 params + 2*2*14*mb m/v) + n*(2*sb + 2*2*mb)*shC SH and its m/v
               (the batched update reads every view's sums and counts)
     bin_sort  n*(16 count pass + 40 scatter pass) + keys*(8 scattered key + 8 sort read + 4 sorted id)
@@ -105,6 +108,8 @@ def algorithmic_bytes(name, n, vis, keys, npx, shC, prec, views=1):
         return views * (keys * (4.0 + rec) + npx * 44.0)
     if name in ("raster_bwd", "raster_bwd_views"):
         return views * (keys * (20.0 + rec) + npx * 24.0 + vis * 40.0)
+    if name in ("raster_fwd_bwd", "raster_fwd_bwd_views"):
+        return views * (keys * (20.0 + rec) + npx * 44.0 + vis * 40.0)
     if name in ("update", "update_views"):
         return n * (52.0 * views + 112.0 + 4.0 * 14.0 * mb) + n * (2.0 * sb + 4.0 * mb) * shC
     if name in ("bin_sort", "bin_sort_views"):

@@ -189,6 +189,11 @@ void launch_loss_l1ssim(hipStream_t st, const Cam& cam, float lambda, const Rast
 // Charbonnier loss), so the colour terms skip the fixed-point clamp
 void launch_raster_bwd(hipStream_t st, const RasterShape& sh, const Cam& cam, const Cut& cut, const RasterBufs& b,
                        int accumMode, void* accum, bool boundedColourGrad);
+// forward with the Charbonnier loss and backward of a training step in one
+// launch (the same results as launch_raster_fwd with loss 1, then
+// launch_raster_bwd)
+void launch_raster_fwd_bwd(hipStream_t st, const RasterShape& sh, const Cam& cam, const Cut& cut, const RasterBufs& b,
+                           int accumMode, void* accum, bool boundedColourGrad);
 
 void launch_update(hipStream_t st, int deg, int accumMode, int prec, const Cam& cam, const AdamHP& hp,
                    const UpdateBufs& u);
@@ -204,17 +209,15 @@ void launch_project_views(hipStream_t st, int deg, int prec, int64_t n, int64_t 
 void launch_project_view(hipStream_t st, int deg, int prec, int64_t n, int64_t cap, const ViewBlock* vb, int v,
                          const Cut& cut, const float4* P, const void* SH, float4* rec, uint32_t* depthKey,
                          uint2* rect, uint32_t* cnt, uint32_t* chunkKeys);
-void launch_raster_fwd_views(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, const Cut& cut, int loss,
-                             uint32_t* queue, int blocks);
-void launch_raster_bwd_views(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, const Cut& cut,
-                             int accumMode, bool boundedColourGrad, uint32_t* queue, int blocks);
+// forward with the Charbonnier loss and backward of every view's tiles in one
+// persistent launch (blocks: raster_views_blocks)
+void launch_raster_fwd_bwd_views(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, const Cut& cut,
+                                 int accumMode, bool boundedColourGrad, uint32_t* queue, int blocks);
 int raster_views_blocks(int device, const RasterShape& sh);
 // view v alone (numTiles blocks, camera and buffers from vb), for the
-// overlapped form of the batched step
-void launch_raster_fwd_view(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, int v, int numTiles,
-                            const Cut& cut, int loss);
-void launch_raster_bwd_view(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, int v, int numTiles,
-                            const Cut& cut, int accumMode, bool boundedColourGrad);
+// overlapped form of the batched step: forward with the loss and backward
+void launch_raster_fwd_bwd_view(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, int v, int numTiles,
+                                const Cut& cut, int accumMode, bool boundedColourGrad);
 void launch_update_views(hipStream_t st, int deg, int accumMode, int prec, const ViewBlock* vb,
                          const UpdateBufs& u);
 

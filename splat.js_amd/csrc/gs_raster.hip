@@ -29,8 +29,8 @@
 // in front of a splat from the stored final T (Tb = Ta/(1-a), through one
 // reciprocal that also scales the colour-behind sum S).
 // Per key, the 10 screen-space gradient terms of the wave's 64 pixels are
-// summed with a transpose reduction (13 lane shuffles instead of 60) and
-// added to the Gaussian's accumulator with one atomic per term per wave:
+// summed with a transpose reduction of lane exchanges (below) and added to
+// the Gaussian's accumulator with one atomic per term per wave:
 //   GS_ACCUM_FIXED: each pixel's term is converted to i32 first,
 //                   i32(clamp(v * 16384, -1e9, 1e9)), exactly as the
 //                   reference's atomicAdd on atomic<i32> (shaders.js:529-541);
@@ -39,6 +39,14 @@
 //                   Charbonnier loss the three colour terms cannot reach the
 //                   clamp and convert without it.)
 //   GS_ACCUM_F32:   plain f32 sums (atomic add order is not deterministic).
+//
+// Training under the Charbonnier loss runs both passes in one launch: the
+// loss gradient of a pixel needs only that pixel, so a block runs its tile's
+// backward right after the forward (k_raster_fwd_bwd, and the persistent
+// k_raster_fwd_bwd_views over a batched step's views). The two passes share
+// one LDS array, and the backward stages the tile's records again while
+// they are still in L2 from the forward. Renders, and the L1 + D-SSIM loss
+// (whose gradient needs neighbouring pixels), run the passes apart.
 #include <type_traits>
 
 #include "gs_kernels.h"
@@ -63,10 +71,10 @@ struct Shape {
   static constexpr int kChunks = kB / 64;
   static constexpr int kBW = kT / 8;            // 8x8 blocks per tile row
   static constexpr int kMasks = kT == 8 ? 1 : 4;
-  // occupancy the persistent backward is compiled for: 8 waves per SIMD (at
-  // most 64 VGPRs; left free, the compiler takes up to 80 there and the
-  // grid, sized by occupancy, shrinks); 8-pixel tiles with 128- or 256-key
-  // batches are held below that by their LDS anyway
+  // occupancy the persistent forward-and-backward launch is compiled for: 8
+  // waves per SIMD (at most 64 VGPRs; left free, the compiler takes up to 80
+  // there and the grid, sized by occupancy, shrinks); 8-pixel tiles with
+  // 128- or 256-key batches are held below that by their LDS anyway
   static constexpr int kBwdWavesPerEU = (kT == 8 && kB > 64) ? 1 : 8;
   static_assert((kT == 8 || kT == 16 || kT == 32) && (kB == 64 || kB == 128 || kB == 256), "raster shape");
   __device__ static int mask_slot(int w) {
@@ -77,6 +85,19 @@ struct Shape {
       return 2 * ((w / kBW) / h) + (w % kBW) / h;
     }
   }
+};
+
+// LDS of the raster passes, carved from one array that the kernel declares,
+// so a launch that runs both passes holds the larger layout, not both.
+// Forward: records sA, sB, sC (16 B each per key), the chunk masks, the loss
+// partials; backward: records sA-sD, the chunk masks, the block's largest
+// end. Sizes in float4.
+template <class S>
+struct RasterLds {
+  static constexpr int kMaskBytes = S::kChunks * S::kMasks * 8;
+  static constexpr int kFwd = 3 * S::B + (kMaskBytes + 3 * S::kWaves * 8 + 15) / 16;
+  static constexpr int kBwd = 4 * S::B + (kMaskBytes + 4 + 15) / 16;
+  static constexpr int kBoth = kFwd > kBwd ? kFwd : kBwd;
 };
 
 struct PixelMap {
@@ -160,12 +181,13 @@ __device__ __forceinline__ float4 record_colour(int recFmt, const float4& r1, fl
 // threads must all call it (it synchronises), and nothing else may use the
 // block's LDS until every thread has returned.
 template <class S, int kLoss>
-__device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, const RasterBufs& b, int tile) {
-  __shared__ float4 sA[S::B];   // mx, my, A, B
-  __shared__ float4 sB[S::B];   // C, opa*comp, L, 0
-  __shared__ float4 sC[S::B];   // r, g, b, opa
-  __shared__ uint64_t sMask[S::kChunks][S::kMasks];
-  __shared__ double sStat[3][S::kWaves];
+__device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, const RasterBufs& b, int tile,
+                                                float4* lds) {
+  float4* const sA = lds;              // mx, my, A, B
+  float4* const sB = lds + S::B;       // C, opa*comp, L, 0
+  float4* const sC = lds + 2 * S::B;   // r, g, b, opa
+  uint64_t (*const sMask)[S::kMasks] = reinterpret_cast<uint64_t (*)[S::kMasks]>(lds + 3 * S::B);
+  double (*const sStat)[S::kWaves] = reinterpret_cast<double (*)[S::kWaves]>(&sMask[S::kChunks][0]);
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int mslot = S::mask_slot(wv);
   const PixelMap pm = pixel_of_thread<S>(cam, tile, tid);
@@ -486,14 +508,14 @@ __device__ __forceinline__ void flush_sum(void* accum, uint32_t g, int q, T r, b
 // One tile of the backward pass, by the whole block (as raster_fwd_tile).
 template <class S, int kAccum, bool kClampColour>
 __device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, const RasterBufs& b, void* accum,
-                                                int tile) {
+                                                int tile, float4* lds) {
   // four 16-byte records per staged splat, all at index j: one LDS address
-  __shared__ float4 sA[S::B];   // mx, my, A, B
-  __shared__ float4 sB[S::B];   // C, opa*comp, L, 0
-  __shared__ float4 sC[S::B];   // r, g, b, opa
-  __shared__ float4 sD[S::B];   // -0.5 cnorm, 1 - opa, id (bits), 0
-  __shared__ uint64_t sMask[S::kChunks][S::kMasks];
-  __shared__ uint32_t sEnd;
+  float4* const sA = lds;              // mx, my, A, B
+  float4* const sB = lds + S::B;       // C, opa*comp, L, 0
+  float4* const sC = lds + 2 * S::B;   // r, g, b, opa
+  float4* const sD = lds + 3 * S::B;   // -0.5 cnorm, 1 - opa, id (bits), 0
+  uint64_t (*const sMask)[S::kMasks] = reinterpret_cast<uint64_t (*)[S::kMasks]>(lds + 4 * S::B);
+  uint32_t& sEnd = *reinterpret_cast<uint32_t*>(&sMask[S::kChunks][0]);
   const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
   const int mslot = S::mask_slot(wv);
   const PixelMap pm = pixel_of_thread<S>(cam, tile, tid);
@@ -690,12 +712,27 @@ __device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, 
 
 template <class S, int kLoss>
 __global__ __launch_bounds__(S::kThreads) void k_raster_fwd(Cam cam, Cut cut, RasterBufs b) {
-  raster_fwd_tile<S, kLoss>(cam, cut, b, tile_of_block(b));
+  __shared__ float4 lds[RasterLds<S>::kFwd];
+  raster_fwd_tile<S, kLoss>(cam, cut, b, tile_of_block(b), lds);
 }
 
 template <class S, int kAccum, bool kClampColour>
 __global__ __launch_bounds__(S::kThreads) void k_raster_bwd(Cam cam, Cut cut, RasterBufs b, void* accum) {
-  raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, accum, tile_of_block(b));
+  __shared__ float4 lds[RasterLds<S>::kBwd];
+  raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, accum, tile_of_block(b), lds);
+}
+
+// A training step's forward with the Charbonnier loss and its backward in
+// one launch: the loss gradient of a pixel needs only that pixel, so each
+// block runs the backward of its tile right after the forward (each thread
+// reads back the T, end and dL/dC it has just stored).
+template <class S, int kAccum, bool kClampColour>
+__global__ __launch_bounds__(S::kThreads) void k_raster_fwd_bwd(Cam cam, Cut cut, RasterBufs b, void* accum) {
+  __shared__ float4 lds[RasterLds<S>::kBoth];   // the two passes' layouts share it
+  const int tile = tile_of_block(b);
+  raster_fwd_tile<S, 1>(cam, cut, b, tile, lds);
+  __syncthreads();   // the forward's last LDS reads are done
+  raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, accum, tile, lds);
 }
 
 // ---- several views in one persistent launch ----
@@ -724,42 +761,35 @@ __device__ __forceinline__ bool next_item(const ViewBlock* __restrict__ vb, uint
   return true;
 }
 
-template <class S, int kLoss>
-__global__ __launch_bounds__(S::kThreads) void k_raster_fwd_views(const ViewBlock* __restrict__ vb, Cut cut,
-                                                                  uint32_t* queue) {
-  ViewItem it;
-  while (next_item(vb, queue, it)) {
-    const Cam cam = vb->cams[it.v];
-    const RasterBufs b = vb->bufs[it.v];
-    raster_fwd_tile<S, kLoss>(cam, cut, b, it.tile);
-  }
-}
-
+// Forward with the Charbonnier loss and backward of every view's tiles in
+// one persistent launch: each work item's tile runs both passes back to
+// back, as k_raster_fwd_bwd does for one view.
 template <class S, int kAccum, bool kClampColour>
-__global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::kBwdWavesPerEU))) void k_raster_bwd_views(
-    const ViewBlock* __restrict__ vb, Cut cut, uint32_t* queue) {
+__global__ __launch_bounds__(S::kThreads) __attribute__((amdgpu_waves_per_eu(S::kBwdWavesPerEU))) void
+k_raster_fwd_bwd_views(const ViewBlock* __restrict__ vb, Cut cut, uint32_t* queue) {
+  __shared__ float4 lds[RasterLds<S>::kBoth];
   ViewItem it;
   while (next_item(vb, queue, it)) {
     const Cam cam = vb->cams[it.v];
     const RasterBufs b = vb->bufs[it.v];
-    raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, vb->accum[it.v], it.tile);
+    raster_fwd_tile<S, 1>(cam, cut, b, it.tile, lds);
+    __syncthreads();   // the forward's last LDS reads are done
+    raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, vb->accum[it.v], it.tile, lds);
   }
 }
 
 // one view of a batched step, one block per tile (camera and buffers from
-// device memory)
-template <class S, int kLoss>
-__global__ __launch_bounds__(S::kThreads) void k_raster_fwd_view(const ViewBlock* __restrict__ vb, int v, Cut cut) {
-  const Cam cam = vb->cams[v];
-  const RasterBufs b = vb->bufs[v];
-  raster_fwd_tile<S, kLoss>(cam, cut, b, tile_of_block(b));
-}
-
+// device memory), forward with the loss and backward in one launch
 template <class S, int kAccum, bool kClampColour>
-__global__ __launch_bounds__(S::kThreads) void k_raster_bwd_view(const ViewBlock* __restrict__ vb, int v, Cut cut) {
+__global__ __launch_bounds__(S::kThreads) void k_raster_fwd_bwd_view(const ViewBlock* __restrict__ vb, int v,
+                                                                     Cut cut) {
   const Cam cam = vb->cams[v];
   const RasterBufs b = vb->bufs[v];
-  raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, vb->accum[v], tile_of_block(b));
+  __shared__ float4 lds[RasterLds<S>::kBoth];
+  const int tile = tile_of_block(b);
+  raster_fwd_tile<S, 1>(cam, cut, b, tile, lds);
+  __syncthreads();
+  raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, vb->accum[v], tile, lds);
 }
 
 // f(Shape<tile, batch>{}) for a supported shape (checked by the callers)
@@ -781,23 +811,13 @@ bool raster_shape_ok(int tile, int batch) {
   return (tile == 8 || tile == 16 || tile == 32) && (batch == 64 || batch == 128 || batch == 256);
 }
 
-void launch_raster_fwd_view(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, int v, int numTiles,
-                            const Cut& cut, int loss) {
+void launch_raster_fwd_bwd_view(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, int v, int numTiles,
+                                const Cut& cut, int accumMode, bool boundedColourGrad) {
   with_shape(sh, [&](auto shape) {
     using S = decltype(shape);
-    if (loss == 1) k_raster_fwd_view<S, 1><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
-    else if (loss == 2) k_raster_fwd_view<S, 2><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
-    else k_raster_fwd_view<S, 0><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
-  });
-}
-
-void launch_raster_bwd_view(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, int v, int numTiles,
-                            const Cut& cut, int accumMode, bool boundedColourGrad) {
-  with_shape(sh, [&](auto shape) {
-    using S = decltype(shape);
-    if (accumMode != 0) k_raster_bwd_view<S, 1, true><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
-    else if (boundedColourGrad) k_raster_bwd_view<S, 0, false><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
-    else k_raster_bwd_view<S, 0, true><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
+    if (accumMode != 0) k_raster_fwd_bwd_view<S, 1, true><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
+    else if (boundedColourGrad) k_raster_fwd_bwd_view<S, 0, false><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
+    else k_raster_fwd_bwd_view<S, 0, true><<<numTiles, S::kThreads, 0, st>>>(vb, v, cut);
   });
 }
 
@@ -823,23 +843,24 @@ void launch_raster_bwd(hipStream_t st, const RasterShape& sh, const Cam& cam, co
   });
 }
 
-void launch_raster_fwd_views(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, const Cut& cut, int loss,
-                             uint32_t* queue, int blocks) {
+void launch_raster_fwd_bwd(hipStream_t st, const RasterShape& sh, const Cam& cam, const Cut& cut, const RasterBufs& b,
+                           int accumMode, void* accum, bool boundedColourGrad) {
+  const dim3 grid((unsigned)cam.numTiles);
   with_shape(sh, [&](auto shape) {
     using S = decltype(shape);
-    if (loss == 1) k_raster_fwd_views<S, 1><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
-    else if (loss == 2) k_raster_fwd_views<S, 2><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
-    else k_raster_fwd_views<S, 0><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
+    if (accumMode != 0) k_raster_fwd_bwd<S, 1, true><<<grid, S::kThreads, 0, st>>>(cam, cut, b, accum);
+    else if (boundedColourGrad) k_raster_fwd_bwd<S, 0, false><<<grid, S::kThreads, 0, st>>>(cam, cut, b, accum);
+    else k_raster_fwd_bwd<S, 0, true><<<grid, S::kThreads, 0, st>>>(cam, cut, b, accum);
   });
 }
 
-void launch_raster_bwd_views(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, const Cut& cut,
-                             int accumMode, bool boundedColourGrad, uint32_t* queue, int blocks) {
+void launch_raster_fwd_bwd_views(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, const Cut& cut,
+                                 int accumMode, bool boundedColourGrad, uint32_t* queue, int blocks) {
   with_shape(sh, [&](auto shape) {
     using S = decltype(shape);
-    if (accumMode != 0) k_raster_bwd_views<S, 1, true><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
-    else if (boundedColourGrad) k_raster_bwd_views<S, 0, false><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
-    else k_raster_bwd_views<S, 0, true><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
+    if (accumMode != 0) k_raster_fwd_bwd_views<S, 1, true><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
+    else if (boundedColourGrad) k_raster_fwd_bwd_views<S, 0, false><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
+    else k_raster_fwd_bwd_views<S, 0, true><<<blocks, S::kThreads, 0, st>>>(vb, cut, queue);
   });
 }
 
@@ -850,12 +871,9 @@ int raster_views_blocks(int device, const RasterShape& sh) {
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus <= 0) cus = 256;
   with_shape(sh, [&](auto shape) {
     using S = decltype(shape);
-    int a = 0, b = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&a, k_raster_fwd_views<S, 1>, S::kThreads, 0) != hipSuccess) a = 1;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k_raster_bwd_views<S, 0, false>, S::kThreads, 0) !=
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_raster_fwd_bwd_views<S, 0, false>, S::kThreads, 0) !=
         hipSuccess)
-      b = 1;
-    per = a < b ? a : b;
+      per = 1;
   });
   if (per < 1) per = 1;
   return cus * per;

@@ -159,7 +159,7 @@ struct gs_trainer {
   ViewBlock* hVB = nullptr;        // pinned ring of kRing host ViewBlocks
   std::vector<hipEvent_t> ringEv;  // copy-done event of each ring entry
   int ringPos = 0;
-  uint32_t* dQueue = nullptr;      // work counters of the two persistent launches
+  uint32_t* dQueue = nullptr;      // work counter of the persistent raster launch
   uint32_t* viewStats = nullptr;   // kMaxViews key counts, then kMaxViews visible counts
   hipStream_t st2 = nullptr;       // the overlapped form's raster stream
   hipEvent_t evFork = nullptr, evJoin = nullptr;
@@ -444,7 +444,7 @@ void radix_values_into(BinArgs& a, uint32_t* ids, uint32_t* scratch) {
 }
 
 // ---- one raster: project -> bin/sort -> forward (+ loss) ----
-int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
+int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf, bool withBwd = false) {
   const int64_t n = h->n;
   h->lastViews = 0;
   h->profLast = nullptr;   // a step or render starts its own first stage
@@ -498,7 +498,10 @@ int run_raster(gs_trainer* h, const Cam& cam, int loss, double* statsBuf) {
   b.gC = h->gC;
   b.tgt = h->targets;
   b.stats = statsBuf;
-  {
+  if (withBwd) {
+    Scope s(h, "raster_fwd_bwd", 0.0);
+    launch_raster_fwd_bwd(h->st, h->shape, cam, make_cut(h), b, h->o.gradAccum, h->accum, charb_grad_bounded(cam));
+  } else {
     Scope s(h, loss ? "raster_fwd_loss" : "raster_fwd", 0.0);
     launch_raster_fwd(h->st, h->shape, cam, make_cut(h), b, loss);
   }
@@ -584,19 +587,9 @@ int eval_pass(gs_trainer* h, int ci, const double* R, const double* t, double g,
   refresh_cam(m);
   const Cam cam = make_cam_meta(h, m, h->shDeg);   // _camUniform: full SH degree (trainer.js:369)
   HIPCK(hipMemsetAsync(h->evalStats, 0, 32, h->st));
-  int rc = run_raster(h, cam, 1, h->evalStats);
+  int rc = run_raster(h, cam, 1, h->evalStats, true);   // forward, loss and backward
   if (rc) return rc;
   launch_cam_exposure(h->st, cam, h->img, h->targets, h->camGrad + 8 * ci);
-  RasterBufs rb{};
-  rb.ranges = h->ranges;
-  rb.tileOrder = h->lastBin.tileOrder;
-  rb.ids = h->lastBin.ids;
-  rb.rec = h->rec;
-  rb.recFmt = h->recFmt;
-  rb.T = h->Tfin;
-  rb.end = h->endIdx;
-  rb.gC = h->gC;
-  launch_raster_bwd(h->st, h->shape, cam, make_cut(h), rb, h->o.gradAccum, h->accum, charb_grad_bounded(cam));
   launch_cam_pose(h->st, h->o.gradAccum, true, cam, cam_pose_bufs(h, ci));
   HIPCK(hipGetLastError());
   double s[4];
@@ -644,7 +637,8 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
   const AdamHP hp = make_hp(h);
 
   const bool ssim = h->o.loss == GS_LOSS_L1_DSSIM;
-  int rc = run_raster(h, cam, ssim ? 2 : 1, h->stats);
+  // the Charbonnier loss is per pixel, so its backward runs in the forward's launch
+  int rc = run_raster(h, cam, ssim ? 2 : 1, h->stats, !ssim);
   if (rc) return rc;
   if (ssim) {
     RasterBufs lb{};
@@ -735,10 +729,9 @@ int step_impl(gs_trainer* h, int ci, gs_debug_out* dbg) {
   b.T = h->Tfin;
   b.end = h->endIdx;
   b.gC = h->gC;
-  {
+  if (ssim) {
     Scope s(h, "raster_bwd", 0.0);
-    launch_raster_bwd(h->st, h->shape, cam, make_cut(h), b, h->o.gradAccum, h->accum,
-                      !ssim && charb_grad_bounded(cam));
+    launch_raster_bwd(h->st, h->shape, cam, make_cut(h), b, h->o.gradAccum, h->accum, false);
   }
   HIPCK(hipGetLastError());
   float* dGF = nullptr;
@@ -817,7 +810,7 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
     HIPCK(hipEventRecord(h->evFork, h->st));
     HIPCK(hipStreamWaitEvent(h->st2, h->evFork, 0));
   } else {
-    launch_fill_u32(h->st, h->dQueue, 2, 0u);
+    launch_fill_u32(h->st, h->dQueue, 1, 0u);
   }
   if (h->batchedBin && !overlap) {
     // every view's projection, then each binning stage of all views, one launch each
@@ -920,8 +913,7 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
     if (overlap) {
       HIPCK(hipEventRecord(h->evBin[k], h->st));
       HIPCK(hipStreamWaitEvent(h->st2, h->evBin[k], 0));
-      launch_raster_fwd_view(h->st2, h->shape, h->dVB, k, cam.numTiles, cut, 1);
-      launch_raster_bwd_view(h->st2, h->shape, h->dVB, k, cam.numTiles, cut, h->o.gradAccum, bounded);
+      launch_raster_fwd_bwd_view(h->st2, h->shape, h->dVB, k, cam.numTiles, cut, h->o.gradAccum, bounded);
     }
   }
   h->lastNumTiles = vb.cams[V - 1].numTiles;
@@ -935,8 +927,9 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
     HIPCK(hipEventRecord(h->evJoin, h->st2));
     HIPCK(hipStreamWaitEvent(h->st, h->evJoin, 0));
   } else {
-    Scope sc(h, "raster_fwd_views", 0.0);
-    launch_raster_fwd_views(h->st, h->shape, h->dVB, cut, 1, h->dQueue, h->viewBlocks);
+    // each tile's backward right after its forward, in one persistent launch
+    Scope sc(h, "raster_fwd_bwd_views", 0.0);
+    launch_raster_fwd_bwd_views(h->st, h->shape, h->dVB, cut, h->o.gradAccum, bounded, h->dQueue, h->viewBlocks);
   }
   if (dbg) {
     HIPCK(hipStreamSynchronize(h->st));
@@ -948,11 +941,6 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
       if (dbg[k].end) HIPCK(hipMemcpy(dbg[k].end, sl.end, 4 * npx, hipMemcpyDeviceToHost));
       if (dbg[k].gC) HIPCK(hipMemcpy(dbg[k].gC, sl.gC, 16 * npx, hipMemcpyDeviceToHost));
     }
-  }
-  if (!overlap) {
-    Scope sc(h, "raster_bwd_views", 0.0);
-    launch_raster_bwd_views(h->st, h->shape, h->dVB, cut, h->o.gradAccum, bounded, h->dQueue + 1,
-                            h->viewBlocks);
   }
   if (dbg) {
     HIPCK(hipStreamSynchronize(h->st));
@@ -1406,7 +1394,7 @@ int gs_setup(gs_trainer_t* h, const float* params, int64_t n, const float* sh, c
   }
   if (h->o.viewsPerStep > 1) {
     rc |= dalloc(h, &h->dVB, sizeof(ViewBlock));
-    rc |= dalloc(h, &h->dQueue, 8);
+    rc |= dalloc(h, &h->dQueue, 4);
     rc |= dalloc(h, &h->viewStats, 8 * kMaxViews);
   }
   // Batched binning: each further view gets its own binning buffers, so
@@ -1535,9 +1523,11 @@ int tune_pass(gs_trainer* h, const int* ci, int count, int activeDeg) {
   if (count == 1) {
     const Cam cam = make_cam_meta(h, h->cams[ci[0]], activeDeg);
     const bool ssim = h->o.loss == GS_LOSS_L1_DSSIM;
-    int rc = run_raster(h, cam, ssim ? 2 : 1, h->evalStats);
+    // as the step runs it: Charbonnier's backward inside the forward's launch
+    int rc = run_raster(h, cam, ssim ? 2 : 1, h->evalStats, !ssim);
     if (rc) return rc;
-    if (ssim) {
+    if (!ssim) return GS_OK;
+    {
       RasterBufs lb{};
       lb.img = h->img;
       lb.gC = h->gC;
@@ -1554,8 +1544,7 @@ int tune_pass(gs_trainer* h, const int* ci, int count, int activeDeg) {
     b.T = h->Tfin;
     b.end = h->endIdx;
     b.gC = h->gC;
-    launch_raster_bwd(h->st, h->shape, cam, make_cut(h), b, h->o.gradAccum, h->accum,
-                      !ssim && charb_grad_bounded(cam));
+    launch_raster_bwd(h->st, h->shape, cam, make_cut(h), b, h->o.gradAccum, h->accum, false);
     HIPCK(hipGetLastError());
     return GS_OK;
   }
