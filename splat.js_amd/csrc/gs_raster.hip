@@ -103,6 +103,16 @@ struct RasterLds {
 struct PixelMap {
   int x, y;        // pixel
 };
+
+// What the forward with the loss leaves for the backward at a thread's
+// pixel: dL/dC (w = 1 for a valid target pixel, 0 otherwise, and 0 outside
+// the image), the final T and `end`. Handed over in registers when both
+// passes run in one launch.
+struct PixelState {
+  float4 g;
+  float T;
+  uint32_t end;
+};
 template <class S>
 __device__ __forceinline__ PixelMap pixel_of_thread(const Cam& cam, int tile, int tid) {
   const int tx = tile % cam.tilesX, ty = tile / cam.tilesX;
@@ -182,7 +192,7 @@ __device__ __forceinline__ float4 record_colour(int recFmt, const float4& r1, fl
 // block's LDS until every thread has returned.
 template <class S, int kLoss>
 __device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, const RasterBufs& b, int tile,
-                                                float4* lds) {
+                                                float4* lds, PixelState* out = nullptr) {
   float4* const sA = lds;              // mx, my, A, B
   float4* const sB = lds + S::B;       // C, opa*comp, L, 0
   float4* const sC = lds + 2 * S::B;   // r, g, b, opa
@@ -196,6 +206,7 @@ __device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, 
   const float px = (float)x + 0.5f, py = (float)y + 0.5f;
   const uint2 rg = b.ranges[tile];
   const uint32_t s = __builtin_amdgcn_readfirstlane(rg.x), e = __builtin_amdgcn_readfirstlane(rg.y);
+  if (out) *out = PixelState{make_float4(0.0f, 0.0f, 0.0f, 0.0f), 0.0f, s};
   float T = 1.0f, C0 = 0.0f, C1 = 0.0f, C2 = 0.0f;
   uint32_t end = s;
   bool done = !inside;
@@ -285,6 +296,10 @@ __device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, 
     if (kLoss) {
       b.T[pi] = T;
       b.end[pi] = end;
+      if (out) {
+        out->T = T;
+        out->end = end;
+      }
     }
     if (kLoss == 1) {
       const uint32_t pk = b.tgt[cam.tgtOffset + pi];
@@ -304,6 +319,7 @@ __device__ __forceinline__ void raster_fwd_tile(const Cam& cam, const Cut& cut, 
         st2 = 1.0;
       }
       b.gC[pi] = g;
+      if (out) out->g = g;
     }
   }
   if (kLoss == 1) {
@@ -508,7 +524,7 @@ __device__ __forceinline__ void flush_sum(void* accum, uint32_t g, int q, T r, b
 // One tile of the backward pass, by the whole block (as raster_fwd_tile).
 template <class S, int kAccum, bool kClampColour>
 __device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, const RasterBufs& b, void* accum,
-                                                int tile, float4* lds) {
+                                                int tile, float4* lds, const PixelState* in = nullptr) {
   // four 16-byte records per staged splat, all at index j: one LDS address
   float4* const sA = lds;              // mx, my, A, B
   float4* const sB = lds + S::B;       // C, opa*comp, L, 0
@@ -547,7 +563,13 @@ __device__ __forceinline__ void raster_bwd_tile(const Cam& cam, const Cut& cut, 
   float4 g = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
   float Ta = 0.0f;
   uint32_t myEnd = s;
-  if (inside) {
+  if (in) {   // the forward of this launch left them in registers
+    g = in->g;
+    if (g.w != 0.0f) {
+      Ta = in->T;
+      myEnd = in->end;
+    }
+  } else if (inside) {
     g = b.gC[pi];
     if (g.w != 0.0f) {
       Ta = b.T[pi];
@@ -725,14 +747,16 @@ __global__ __launch_bounds__(S::kThreads) void k_raster_bwd(Cam cam, Cut cut, Ra
 // A training step's forward with the Charbonnier loss and its backward in
 // one launch: the loss gradient of a pixel needs only that pixel, so each
 // block runs the backward of its tile right after the forward (each thread
-// reads back the T, end and dL/dC it has just stored).
+// keeps its pixel's T, end and dL/dC in registers; they are stored too,
+// for the debug readbacks).
 template <class S, int kAccum, bool kClampColour>
 __global__ __launch_bounds__(S::kThreads) void k_raster_fwd_bwd(Cam cam, Cut cut, RasterBufs b, void* accum) {
   __shared__ float4 lds[RasterLds<S>::kBoth];   // the two passes' layouts share it
   const int tile = tile_of_block(b);
-  raster_fwd_tile<S, 1>(cam, cut, b, tile, lds);
+  PixelState ps;
+  raster_fwd_tile<S, 1>(cam, cut, b, tile, lds, &ps);
   __syncthreads();   // the forward's last LDS reads are done
-  raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, accum, tile, lds);
+  raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, accum, tile, lds, &ps);
 }
 
 // ---- several views in one persistent launch ----
@@ -772,9 +796,10 @@ k_raster_fwd_bwd_views(const ViewBlock* __restrict__ vb, Cut cut, uint32_t* queu
   while (next_item(vb, queue, it)) {
     const Cam cam = vb->cams[it.v];
     const RasterBufs b = vb->bufs[it.v];
-    raster_fwd_tile<S, 1>(cam, cut, b, it.tile, lds);
+    PixelState ps;
+    raster_fwd_tile<S, 1>(cam, cut, b, it.tile, lds, &ps);
     __syncthreads();   // the forward's last LDS reads are done
-    raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, vb->accum[it.v], it.tile, lds);
+    raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, vb->accum[it.v], it.tile, lds, &ps);
   }
 }
 
@@ -787,9 +812,10 @@ __global__ __launch_bounds__(S::kThreads) void k_raster_fwd_bwd_view(const ViewB
   const RasterBufs b = vb->bufs[v];
   __shared__ float4 lds[RasterLds<S>::kBoth];
   const int tile = tile_of_block(b);
-  raster_fwd_tile<S, 1>(cam, cut, b, tile, lds);
+  PixelState ps;
+  raster_fwd_tile<S, 1>(cam, cut, b, tile, lds, &ps);
   __syncthreads();
-  raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, vb->accum[v], tile, lds);
+  raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, vb->accum[v], tile, lds, &ps);
 }
 
 // f(Shape<tile, batch>{}) for a supported shape (checked by the callers)
