@@ -1104,15 +1104,18 @@ __device__ __forceinline__ void wave_sort256(uint64_t (&v)[4]) {
   bitonic_blocks<2>(v, __lane_id());
 }
 
-// Sorts m <= kChunk keys: runs of 256 sorted by waves (padded with ~0 to
-// R = 1, 2 or 4 runs), then merged pairwise in LDS (merge path, R outputs
-// per thread per level). Returns the LDS buffer holding the sorted keys
-// (sorted[0, m)).
+// Sorts m <= kChunk keys: R = ceil(m / 256) runs of 256 sorted by waves
+// (the last padded with ~0), then merged pairwise in LDS (merge path, P
+// outputs per thread per level; with three runs the last pair of the first
+// level has no partner and is copied). Returns the LDS buffer holding the
+// sorted keys (sorted[0, m)).
 constexpr uint32_t kChunk = 4 * 256;
 
 __device__ uint64_t* seg_sort_chunk(const uint64_t* __restrict__ src, uint32_t m, uint64_t* sa, uint64_t* sb) {
   const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-  const uint32_t R = m <= 256u ? 1u : (m <= 512u ? 2u : 4u);
+  // runs of 256 keys: as many as m needs (a segment of 513-768 keys sorts
+  // three runs, not four padded ones)
+  const uint32_t R = (m + 255u) / 256u;
   if (w < R) {
     uint64_t v[4];
 #pragma unroll
@@ -1128,23 +1131,32 @@ __device__ uint64_t* seg_sort_chunk(const uint64_t* __restrict__ src, uint32_t m
   uint64_t* in = sa;
   uint64_t* out = sb;
   const uint32_t N = 256u * R;
+  // outputs per thread: a power of two, so a thread's outputs never straddle
+  // two merged pairs (2W >= 512); with three runs 192 threads write 4 each
+  const uint32_t P = R == 3u ? 4u : R;
   for (uint32_t W = 256u; W < N; W <<= 1) {
-    // thread tid writes outputs [tid R, tid R + R) of N (R = N / 256)
-    const uint32_t o = tid * R;
-    const uint32_t base = o / (2u * W) * (2u * W);
-    const uint64_t* A = in + base;
-    const uint64_t* B = A + W;
-    const uint32_t d = o - base;
-    uint32_t lo = d > W ? d - W : 0u, hi = d < W ? d : W;
-    while (lo < hi) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (A[mid] < B[d - mid - 1u]) lo = mid + 1u;
-      else hi = mid;
-    }
-    uint32_t ia = lo, ib = d - lo;
-    for (uint32_t q = 0; q < R; q++) {
-      const bool takeA = ib >= W || (ia < W && A[ia] < B[ib]);
-      out[o + q] = takeA ? A[ia++] : B[ib++];
+    // thread tid writes outputs [tid P, tid P + P) of N; the pair at base
+    // merges A = [base, base + na) and B = [base + na, base + na + nb), the
+    // last pair possibly shorter (nb = 0: A is copied)
+    const uint32_t o = tid * P;
+    if (o < N) {
+      const uint32_t base = o / (2u * W) * (2u * W);
+      const uint32_t na = N - base < W ? N - base : W;
+      const uint32_t nb = N - base - na < W ? N - base - na : W;
+      const uint64_t* A = in + base;
+      const uint64_t* B = A + na;
+      const uint32_t d = o - base;
+      uint32_t lo = d > nb ? d - nb : 0u, hi = d < na ? d : na;
+      while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (A[mid] < B[d - mid - 1u]) lo = mid + 1u;
+        else hi = mid;
+      }
+      uint32_t ia = lo, ib = d - lo;
+      for (uint32_t q = 0; q < P; q++) {
+        const bool takeA = ib >= nb || (ia < na && A[ia] < B[ib]);
+        out[o + q] = takeA ? A[ia++] : B[ib++];
+      }
     }
     __syncthreads();
     uint64_t* t = in;
