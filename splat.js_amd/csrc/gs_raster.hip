@@ -404,7 +404,6 @@ __device__ __forceinline__ void dpp_halve(const T* v, T* out, bool upper) {
   }
 }
 
-
 // Row step 1 (l ^ 15) on one visit's 10 values: lanes 0-7 of each row keep
 // values 0-4, lanes 8-15 values 5-9.
 #define GS_DPP_M(OP, O, LO, HI, CTRL, MLO, MHI) \
