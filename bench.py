@@ -89,6 +89,8 @@ def algorithmic_bytes(name, n, vis, keys, npx, shC, prec, views=1):
     mixed); the raster record is 48 B of f32 geometry and colour read per key
     (32 B in the 16-bit-colour modes: geometry and packed colours).
     project   n*(56 params + 16 key/rect/count) + vis*(sb*shC SH + 48 record write)
+    project_views (one thread per Gaussian for all views) n*(56 params + sb*shC SH)
+              + views*(n*16 key/rect/count + vis*48 record write)
     raster fwd  keys*(4 id + rec) + px*(16 img + 4 T + 4 end + 16 dL/dC + 4 target)
     raster bwd  keys*(4 id + rec + 16) + px*(16 dL/dC + 4 T + 4 end) + vis*40 gradient sums
     raster fwd+bwd (one launch, each tile's backward right after its forward)
@@ -102,8 +104,10 @@ def algorithmic_bytes(name, n, vis, keys, npx, shC, prec, views=1):
     sb = 4.0 if prec == "f32" else 2.0
     mb = 2.0 if prec == "mixed" else 4.0
     rec = 32.0 if prec in ("bf16", "fp16") else 48.0
-    if name in ("project", "project_views"):
+    if name == "project":
         return views * (n * 72.0 + vis * (sb * shC + 48.0))
+    if name == "project_views":
+        return n * (56.0 + sb * shC) + views * (n * 16.0 + vis * 48.0)
     if name in ("raster_fwd_loss", "raster_fwd_views"):
         return views * (keys * (4.0 + rec) + npx * 44.0)
     if name in ("raster_bwd", "raster_bwd_views"):

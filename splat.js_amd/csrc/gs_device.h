@@ -66,6 +66,19 @@ __device__ __forceinline__ void load_params(const float4* __restrict__ P, int64_
   p[12] = d.x; p[13] = d.y; p[14] = d.z; p[15] = d.w;
 }
 
+// A pointer field that a kernel reads from device memory (a batched step's
+// per-view block or binning jobs), loaded as a pointer to global memory.
+// Loaded as a plain pointer, the compiler cannot tell global memory from LDS
+// or scratch and emits flat loads and stores, which count against the LDS
+// and scalar-load wait counter as well: every LDS or scalar-load wait then
+// also waits for the global loads in flight. Every such field holds device
+// (or pinned host) memory.
+template <typename T>
+__device__ __forceinline__ T* gs_global(T* const& field) {
+  const void* at = &field;
+  return (T*)*static_cast<__attribute__((address_space(1))) T* const*>(at);
+}
+
 // Wave vote on a lane predicate. The builtin takes the bool itself; HIP's
 // __ballot(int) converts it to an int and back, which costs two VALU
 // instructions per vote.
@@ -79,21 +92,21 @@ __device__ __forceinline__ float dot3(const float* a, const float* b) {
   return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
 }
 
-struct Geom {
-  bool ok;
-  float pc[3], q[4], r0[3], r1[3], r2[3], s[3], t0[3], t1[3];
-  float va, vb, vc, s00, s01, s02, s11, s12, s22;
+// The camera-independent part of a Gaussian's geometry: unit quaternion,
+// rotation rows, scales and the 3D covariance Sigma = (R S)(R S)^T.
+struct GeomShape {
+  float q[4], r0[3], r1[3], r2[3], s[3];
+  float s00, s01, s02, s11, s12, s22;
 };
 
-// shaders.js:176-224 computeGeom
-__device__ __forceinline__ void compute_geom(const Cam& cam, const float* P, Geom& g) {
-  g.ok = false;
-  const float p[3] = {P[0], P[1], P[2]};
-  g.pc[0] = dot3(cam.R + 0, p) + cam.t[0];
-  g.pc[1] = dot3(cam.R + 3, p) + cam.t[1];
-  g.pc[2] = dot3(cam.R + 6, p) + cam.t[2];
-  if (g.pc[2] < cam.nearp) return;
+struct Geom : GeomShape {
+  bool ok;
+  float pc[3], t0[3], t1[3];
+  float va, vb, vc;
+};
 
+// shaders.js:183-201 (the part of computeGeom no camera enters)
+__device__ __forceinline__ void compute_shape(const float* P, GeomShape& g) {
   float q[4] = {P[6], P[7], P[8], P[9]};
   const float ql = sqrtf(((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3]);
   if (ql < 1e-6f) { q[0] = 1.0f; q[1] = 0.0f; q[2] = 0.0f; q[3] = 0.0f; }
@@ -120,7 +133,21 @@ __device__ __forceinline__ void compute_geom(const Cam& cam, const float* P, Geo
   }
   g.s00 = dot3(m0, m0); g.s01 = dot3(m0, m1); g.s02 = dot3(m0, m2);
   g.s11 = dot3(m1, m1); g.s12 = dot3(m1, m2); g.s22 = dot3(m2, m2);
+}
 
+// camera-space position (shaders.js:179-181); false when behind the near plane
+__device__ __forceinline__ bool geom_position(const Cam& cam, const float* P, Geom& g) {
+  g.ok = false;
+  const float p[3] = {P[0], P[1], P[2]};
+  g.pc[0] = dot3(cam.R + 0, p) + cam.t[0];
+  g.pc[1] = dot3(cam.R + 3, p) + cam.t[1];
+  g.pc[2] = dot3(cam.R + 6, p) + cam.t[2];
+  return !(g.pc[2] < cam.nearp);
+}
+
+// projection Jacobian rows T = J W and the 2D covariance V = T Sigma T^T
+// (shaders.js:203-222), from g.pc and the shape already in g
+__device__ __forceinline__ void geom_project(const Cam& cam, Geom& g) {
   const float f = cam.f;
   const float iz = 1.0f / g.pc[2];
   const float a = f * iz;
@@ -141,6 +168,20 @@ __device__ __forceinline__ void compute_geom(const Cam& cam, const float* P, Geo
   g.vb = dot3(g.t1, st0);
   g.vc = fmaxf(dot3(g.t1, st1), 0.0f);
   g.ok = true;
+}
+
+// shaders.js:176-224 computeGeom
+__device__ __forceinline__ void compute_geom(const Cam& cam, const float* P, Geom& g) {
+  if (!geom_position(cam, P, g)) return;
+  compute_shape(P, g);
+  geom_project(cam, g);
+}
+
+// the same from a shape computed once for several cameras (identical values)
+__device__ __forceinline__ void compute_geom(const Cam& cam, const float* P, const GeomShape& sp, Geom& g) {
+  if (!geom_position(cam, P, g)) return;
+  static_cast<GeomShape&>(g) = sp;
+  geom_project(cam, g);
 }
 
 // shaders.js:674-678 (render) / :736-740 (chain): largest eigenvalue of the

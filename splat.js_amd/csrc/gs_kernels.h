@@ -105,6 +105,30 @@ struct SegJob {
   uint32_t* ids;          // cap: the raster values in sorted order
 };
 
+// a SegJob read from device memory, every pointer loaded by gs_global
+__device__ __forceinline__ SegJob global_job(const SegJob& s) {
+  SegJob r = s;
+  r.rec = gs_global(s.rec);
+  r.depthKey = gs_global(s.depthKey);
+  r.rect = gs_global(s.rect);
+  r.cnt = gs_global(s.cnt);
+  r.chunkKeys = gs_global(s.chunkKeys);
+  r.blockStart = gs_global(s.blockStart);
+  r.table = gs_global(s.table);
+  r.visPart = gs_global(s.visPart);
+  r.tileCnt = gs_global(s.tileCnt);
+  r.ranges = gs_global(s.ranges);
+  r.tileOrder = gs_global(s.tileOrder);
+  r.numKeys = gs_global(s.numKeys);
+  r.numVisible = gs_global(s.numVisible);
+  r.totalKeys = gs_global(s.totalKeys);
+  r.stats = gs_global(s.stats);
+  r.keys = gs_global(s.keys);
+  r.tmp = gs_global(s.tmp);
+  r.ids = gs_global(s.ids);
+  return r;
+}
+
 // All views' binning stages, one launch each; hJobs: the host copy of the
 // device array dJobs (for the launch shapes). Jobs with `skip` set are left
 // out (their blocks return at once).
@@ -130,6 +154,23 @@ struct RasterBufs {
   const uint32_t* tgt;    // packed RGBA8 targets (all views)
   double* stats;          // [0] sum |err|^2, [1] loss, [2] valid pixels
 };
+
+// a RasterBufs read from device memory, every pointer loaded by gs_global
+__device__ __forceinline__ RasterBufs global_bufs(const RasterBufs& s) {
+  RasterBufs r;
+  r.recFmt = s.recFmt;
+  r.ranges = gs_global(s.ranges);
+  r.tileOrder = gs_global(s.tileOrder);
+  r.ids = gs_global(s.ids);
+  r.rec = gs_global(s.rec);
+  r.img = gs_global(s.img);
+  r.T = gs_global(s.T);
+  r.end = gs_global(s.end);
+  r.gC = gs_global(s.gC);
+  r.tgt = gs_global(s.tgt);
+  r.stats = gs_global(s.stats);
+  return r;
+}
 
 struct UpdateBufs {
   int64_t n, cap;

@@ -24,16 +24,43 @@
 
 namespace gs {
 
-template <int kDeg, int kPrec>
-__device__ __forceinline__ uint32_t project_one(int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
-                                                const float4* __restrict__ P, const void* __restrict__ SH,
-                                                float4* __restrict__ rec, uint32_t* __restrict__ depthKey,
-                                                uint2* __restrict__ rect, uint32_t* __restrict__ cnt,
-                                                int64_t i) {
+// A Gaussian's projection inputs that no camera changes: parameters,
+// shape, opacity and colour sigmoids (shaders.js:252, :277-280) and the SH
+// coefficients. One view at a time they are computed inside project_one for
+// the Gaussians that pass the culls; the batched launch computes them once
+// per Gaussian for all of its views.
+template <int kDeg>
+struct ProjFixed {
   float p[16];
-  load_params(P, cap, i, p);
+  GeomShape sp;
+  float opa, sig[3];
+  float sh[ShDims<kDeg>::S > 0 ? ShDims<kDeg>::S : 1];
+};
+
+template <int kDeg, int kPrec>
+__device__ __forceinline__ void proj_fixed(int64_t cap, const float4* __restrict__ P, const void* __restrict__ SH,
+                                           int64_t i, ProjFixed<kDeg>& pf) {
+  load_params(P, cap, i, pf.p);
+  compute_shape(pf.p, pf.sp);
+  pf.opa = gs_sigmoid_clamped(pf.p[13]);
+  for (int c = 0; c < 3; c++) pf.sig[c] = gs_sigmoid_clamped(pf.p[10 + c]);
+  if constexpr (kDeg > 0) load_sh_any<kDeg, Prec<kPrec>::sh>(SH, cap, i, pf.sh);
+}
+
+// kPre: pf holds proj_fixed's values; otherwise project_one loads and
+// computes them itself, the parameters first and the rest only for a
+// Gaussian that passes the culls. Same values either way.
+template <int kDeg, int kPrec, bool kPre>
+__device__ __forceinline__ uint32_t project_one(int64_t cap, const Cam& cam, const Cut& cut,
+                                                const float4* __restrict__ P, const void* __restrict__ SH,
+                                                ProjFixed<kDeg>& pf, float4* __restrict__ rec,
+                                                uint32_t* __restrict__ depthKey, uint2* __restrict__ rect,
+                                                uint32_t* __restrict__ cnt, int64_t i) {
+  if constexpr (!kPre) load_params(P, cap, i, pf.p);
+  const float* p = pf.p;
   Geom g;
-  compute_geom(cam, p, g);
+  if constexpr (kPre) compute_geom(cam, p, pf.sp, g);
+  else compute_geom(cam, p, g);
   uint32_t key = 0xffffffffu, c = 0;
   uint2 rc = make_uint2(0, 0);
   while (g.ok) {  // single pass; `break` = culled (shaders.js:243-266)
@@ -43,7 +70,7 @@ __device__ __forceinline__ uint32_t project_one(int64_t n, int64_t cap, const Ca
     const float detVd = ad * cd - g.vb * g.vb;
     if (detVd < 1e-8f) break;
     const float comp = sqrtf(fmaxf(detV / detVd, 0.0f));
-    const float opa = gs_sigmoid_clamped(p[13]);
+    const float opa = kPre ? pf.opa : gs_sigmoid_clamped(p[13]);
     if (opa * comp < cut.aMin) break;
     const float f = cam.f;
     const float mx = f * g.pc[0] / g.pc[2] + cam.cx;
@@ -55,15 +82,16 @@ __device__ __forceinline__ uint32_t project_one(int64_t n, int64_t cap, const Ca
     const float rad = fminf(sqrtf(2.0f * eMax) * sqrtf(mid + disc), cut.radClamp * W);
     if (mx + rad < 0.0f || my + rad < 0.0f || mx - rad > W || my - rad > H) break;
     const float inv = 1.0f / detVd;
-    float col[3] = {gs_sigmoid_clamped(p[10]), gs_sigmoid_clamped(p[11]), gs_sigmoid_clamped(p[12])};
+    float col[3];
+    for (int k = 0; k < 3; k++) col[k] = kPre ? pf.sig[k] : gs_sigmoid_clamped(p[10 + k]);
     if constexpr (kDeg > 0) {
       constexpr int K = ShDims<kDeg>::K;
       float v[3];
       view_dir(cam, p, v);
       float Y[15];
       sh_basis<kDeg, false>(v, Y, nullptr);
-      float sh[ShDims<kDeg>::S];
-      load_sh_any<kDeg, Prec<kPrec>::sh>(SH, cap, i, sh);
+      if constexpr (!kPre) load_sh_any<kDeg, Prec<kPrec>::sh>(SH, cap, i, pf.sh);
+      const float* sh = pf.sh;
       const int aK = cam.shActiveK;
 #pragma unroll
       for (int k = 0; k < K; k++) {
@@ -92,11 +120,13 @@ __device__ __forceinline__ uint32_t project_one(int64_t n, int64_t cap, const Ca
       rec[3 * i + 2] = make_float4(comp, -0.5f * cnorm, 1.0f - opa, opa);
     }
     key = gs_f2u(g.pc[2]);
-    const float TF = (float)cam.tile;   // a power of two: the divisions are exact
-    const int tx0 = (int)clampf(floorf((mx - rad) / TF), 0.0f, (float)(cam.tilesX - 1));
-    const int tx1 = (int)clampf(floorf((mx + rad) / TF), 0.0f, (float)(cam.tilesX - 1));
-    const int ty0 = (int)clampf(floorf((my - rad) / TF), 0.0f, (float)(cam.tilesY - 1));
-    const int ty1 = (int)clampf(floorf((my + rad) / TF), 0.0f, (float)(cam.tilesY - 1));
+    // tile = 2^k: x * (1 / tile) rounds exactly as x / tile (the same real
+    // value, one rounding), so the reference's division is kept bit for bit
+    const float rTF = 1.0f / (float)cam.tile;
+    const int tx0 = (int)clampf(floorf((mx - rad) * rTF), 0.0f, (float)(cam.tilesX - 1));
+    const int tx1 = (int)clampf(floorf((mx + rad) * rTF), 0.0f, (float)(cam.tilesX - 1));
+    const int ty0 = (int)clampf(floorf((my - rad) * rTF), 0.0f, (float)(cam.tilesY - 1));
+    const int ty1 = (int)clampf(floorf((my + rad) * rTF), 0.0f, (float)(cam.tilesY - 1));
     rc = make_uint2((uint32_t)tx0 | ((uint32_t)tx1 << 16), (uint32_t)ty0 | ((uint32_t)ty1 << 16));
     c = (uint32_t)((tx1 - tx0 + 1) * (ty1 - ty0 + 1));
     break;
@@ -107,6 +137,16 @@ __device__ __forceinline__ uint32_t project_one(int64_t n, int64_t cap, const Ca
   return c;
 }
 
+// += the wave's keys to its chunk's count (the wave's 64 Gaussians lie in one
+// chunk: kSegChunk is a multiple of 64); every lane of the wave takes part
+__device__ __forceinline__ void add_chunk_keys(uint32_t* __restrict__ chunkKeys, int64_t i, uint32_t c) {
+  if (!chunkKeys) return;
+  uint32_t s = c;
+  for (int o = 32; o > 0; o >>= 1) s += (uint32_t)__shfl_xor((int)s, o);
+  const int64_t i0 = i - (threadIdx.x & 63);
+  if ((threadIdx.x & 63) == 0 && s != 0u) atomicAdd(&chunkKeys[i0 / kSegChunk], s);
+}
+
 template <int kDeg, int kPrec>
 __device__ __forceinline__ void project_thread(int64_t n, int64_t cap, const Cam& cam, const Cut& cut,
                                                const float4* __restrict__ P, const void* __restrict__ SH,
@@ -114,14 +154,10 @@ __device__ __forceinline__ void project_thread(int64_t n, int64_t cap, const Cam
                                                uint2* __restrict__ rect, uint32_t* __restrict__ cnt,
                                                uint32_t* __restrict__ chunkKeys) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  const uint32_t c = i < n ? project_one<kDeg, kPrec>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, i) : 0u;
-  if (chunkKeys) {
-    // the wave's 64 Gaussians lie in one chunk (kSegChunk is a multiple of 64)
-    uint32_t s = c;
-    for (int o = 32; o > 0; o >>= 1) s += (uint32_t)__shfl_xor((int)s, o);
-    const int64_t i0 = i - (threadIdx.x & 63);
-    if ((threadIdx.x & 63) == 0 && s != 0u) atomicAdd(&chunkKeys[i0 / kSegChunk], s);
-  }
+  ProjFixed<kDeg> pf;
+  const uint32_t c =
+      i < n ? project_one<kDeg, kPrec, false>(cap, cam, cut, P, SH, pf, rec, depthKey, rect, cnt, i) : 0u;
+  add_chunk_keys(chunkKeys, i, c);
 }
 
 template <int kDeg, int kPrec>
@@ -148,27 +184,37 @@ __global__ __launch_bounds__(kBlock) void k_project_view(int64_t n, int64_t cap,
   project_thread<kDeg, kPrec>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys);
 }
 
-// every view of a batched step in one launch: view blockIdx.y, its camera
-// from vb->cams and its buffers from vb->seg
+// every view of a batched step in one launch: one thread per Gaussian loads
+// its parameters and SH and computes the camera-independent terms once,
+// then projects them through views 0..V-1 in turn (cameras from vb->cams,
+// buffers from vb->seg)
 template <int kDeg, int kPrec>
 __global__ __launch_bounds__(kBlock) void k_project_views(int64_t n, int64_t cap, const ViewBlock* __restrict__ vb,
-                                                          Cut cut, const float4* __restrict__ P,
+                                                          int V, Cut cut, const float4* __restrict__ P,
                                                           const void* __restrict__ SH) {
-  const int v = blockIdx.y;
-  const Cam cam = vb->cams[v];
-  const SegJob& j = vb->seg[v];
-  project_thread<kDeg, kPrec>(n, cap, cam, cut, P, SH, j.rec, j.depthKey, j.rect, j.cnt, j.chunkKeys);
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  ProjFixed<kDeg> pf;
+  if (i < n) proj_fixed<kDeg, kPrec>(cap, P, SH, i, pf);
+  for (int v = 0; v < V; v++) {
+    const Cam& cam = vb->cams[v];
+    const SegJob& j = vb->seg[v];
+    const uint32_t c = i < n ? project_one<kDeg, kPrec, true>(cap, cam, cut, P, SH, pf, gs_global(j.rec),
+                                                              gs_global(j.depthKey), gs_global(j.rect),
+                                                              gs_global(j.cnt), i)
+                             : 0u;
+    add_chunk_keys(gs_global(j.chunkKeys), i, c);
+  }
 }
 
 template <int kPrec>
 static void launch_project_views_t(hipStream_t st, int deg, int64_t n, int64_t cap, const ViewBlock* vb, int V,
                                    const Cut& cut, const float4* P, const void* SH) {
-  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock), (unsigned)V);
+  const dim3 grid((unsigned)((n + kBlock - 1) / kBlock));
   switch (deg) {
-    case 0: k_project_views<0, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, cut, P, SH); break;
-    case 1: k_project_views<1, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, cut, P, SH); break;
-    case 2: k_project_views<2, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, cut, P, SH); break;
-    default: k_project_views<3, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, cut, P, SH); break;
+    case 0: k_project_views<0, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, V, cut, P, SH); break;
+    case 1: k_project_views<1, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, V, cut, P, SH); break;
+    case 2: k_project_views<2, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, V, cut, P, SH); break;
+    default: k_project_views<3, kPrec><<<grid, kBlock, 0, st>>>(n, cap, vb, V, cut, P, SH); break;
   }
 }
 

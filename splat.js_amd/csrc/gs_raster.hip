@@ -780,7 +780,7 @@ __device__ __forceinline__ bool next_item(const ViewBlock* __restrict__ vb, uint
   int v = 0;
   while (w >= vb->off[v + 1]) v++;
   it.v = v;
-  it.tile = (int)vb->bufs[v].tileOrder[w - vb->off[v]];
+  it.tile = (int)gs_global(vb->bufs[v].tileOrder)[w - vb->off[v]];
   return true;
 }
 
@@ -795,13 +795,15 @@ k_raster_fwd_bwd_views(const ViewBlock* __restrict__ vb, Cut cut, uint32_t* queu
   while (next_item(vb, queue, it)) {
     // read in place (the block is read-only here): copies held every camera
     // and buffer field in scalar registers across the whole tile and spilled
-    // 10-14 of them (no measurable time either way)
+    // some of them. The buffers' pointers stay generic (flat accesses): a
+    // copy with each loaded as a global pointer (global_bufs) took 0.2-0.4%
+    // longer.
     const Cam& cam = vb->cams[it.v];
     const RasterBufs& b = vb->bufs[it.v];
     PixelState ps;
     raster_fwd_tile<S, 1>(cam, cut, b, it.tile, lds, &ps);
     __syncthreads();   // the forward's last LDS reads are done
-    raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, vb->accum[it.v], it.tile, lds, &ps);
+    raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, gs_global(vb->accum[it.v]), it.tile, lds, &ps);
   }
 }
 
@@ -811,13 +813,13 @@ template <class S, int kAccum, bool kClampColour>
 __global__ __launch_bounds__(S::kThreads) void k_raster_fwd_bwd_view(const ViewBlock* __restrict__ vb, int v,
                                                                      Cut cut) {
   const Cam cam = vb->cams[v];
-  const RasterBufs b = vb->bufs[v];
+  const RasterBufs b = global_bufs(vb->bufs[v]);
   __shared__ float4 lds[RasterLds<S>::kBoth];
   const int tile = tile_of_block(b);
   PixelState ps;
   raster_fwd_tile<S, 1>(cam, cut, b, tile, lds, &ps);
   __syncthreads();
-  raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, vb->accum[v], tile, lds, &ps);
+  raster_bwd_tile<S, kAccum, kClampColour>(cam, cut, b, gs_global(vb->accum[v]), tile, lds, &ps);
 }
 
 // f(Shape<tile, batch>{}) for a supported shape (checked by the callers)

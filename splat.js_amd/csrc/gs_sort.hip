@@ -547,7 +547,7 @@ __device__ __forceinline__ uint32_t seg_val(uint64_t k) {
 // of the batched array
 template <bool kJobs>
 __device__ __forceinline__ SegJob seg_job(const SegJob& arg, const SegJob* __restrict__ jobs, int v) {
-  if constexpr (kJobs) return jobs[v];
+  if constexpr (kJobs) return global_job(jobs[v]);
   else return arg;
 }
 
@@ -1116,12 +1116,15 @@ __device__ uint64_t* seg_sort_chunk(const uint64_t* __restrict__ src, uint32_t m
   // runs of 256 keys: as many as m needs (a segment of 513-768 keys sorts
   // three runs, not four padded ones)
   const uint32_t R = (m + 255u) / 256u;
+  // src is device memory: read through a global pointer (in this
+  // out-of-line function a plain one compiles to flat loads)
+  const __attribute__((address_space(1))) uint64_t* gsrc = (const __attribute__((address_space(1))) uint64_t*)src;
   if (w < R) {
     uint64_t v[4];
 #pragma unroll
     for (int r = 0; r < 4; r++) {
       const uint32_t i = 256u * w + 4u * lane + (uint32_t)r;
-      v[r] = i < m ? src[i] : ~0ull;
+      v[r] = i < m ? gsrc[i] : ~0ull;
     }
     wave_sort256(v);
 #pragma unroll

@@ -815,7 +815,7 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
   if (h->batchedBin && !overlap) {
     // every view's projection, then each binning stage of all views, one launch each
     {
-      Scope sc(h, "project_views", (double)V * n * (56.0 + 4.0 * h->shC + 48.0 + 16.0));
+      Scope sc(h, "project_views", (double)n * (56.0 + 4.0 * h->shC) + (double)V * n * (48.0 + 16.0));
       launch_project_views(h->st, h->shDeg, h->prec, n, h->cap, h->dVB, V, cut, h->P, h->SH);
     }
     {

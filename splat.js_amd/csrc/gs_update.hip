@@ -23,113 +23,162 @@
 
 namespace gs {
 
-// Screen-space gradient sums of Gaussian i (decoded), cleared in place as
-// the reference's chain clears them (shaders.js:727-732).
-template <int kAccum>
-__device__ __forceinline__ void take_sums(void* accum, int64_t i, float* gp) {
+// Screen-space gradient sums of Gaussian i: loaded, decoded, and cleared in
+// place as the reference's chain clears them (shaders.js:727-732). The
+// steps are separate so that a batched update can load the next view's sums
+// while it chains the current view's.
+struct RawSums {
+  int4 a0, a1, a2;
+};
+
+__device__ __forceinline__ RawSums load_sums(const void* accum, int64_t i) {
+  const int4* acc = (const int4*)accum + 3 * i;
+  return RawSums{acc[0], acc[1], acc[2]};
+}
+
+__device__ __forceinline__ void clear_sums(void* accum, int64_t i) {
   int4* acc = (int4*)accum + 3 * i;
-  const int4 a0 = acc[0], a1 = acc[1], a2 = acc[2];
-  const int raw[10] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w, a2.x, a2.y};
   const int4 z = make_int4(0, 0, 0, 0);
   acc[0] = z; acc[1] = z; acc[2] = z;
+}
+
+template <int kAccum>
+__device__ __forceinline__ void decode_sums(const RawSums& r, float* gp) {
+  const int raw[10] = {r.a0.x, r.a0.y, r.a0.z, r.a0.w, r.a1.x, r.a1.y, r.a1.z, r.a1.w, r.a2.x, r.a2.y};
 #pragma unroll
   for (int k = 0; k < 10; k++) gp[k] = kAccum == 0 ? (float)raw[k] / kFixed : __int_as_float(raw[k]);
 }
 
+template <int kAccum>
+__device__ __forceinline__ void take_sums(void* accum, int64_t i, float* gp) {
+  const RawSums r = load_sums(accum, i);
+  clear_sums(accum, i);
+  decode_sums<kAccum>(r, gp);
+}
+
+// Terms of the chain that no camera enters, computed once per Gaussian (a
+// batched step chains up to kMaxViews views with them): the shape, the
+// anisotropy regulariser's gradient (shaders.js:873-884), the quaternion
+// length (:899) and the colour sigmoids (:907-910).
+struct ChainFixed {
+  GeomShape sp;
+  float aniso[3];   // anisoReg * (clamped log-scale - their mean)
+  float ql;         // max(|q|, 1e-6)
+  float sCol[3];
+};
+
+__device__ __forceinline__ void chain_fixed(const AdamHP& hp, const float* p, ChainFixed& cf) {
+  compute_shape(p, cf.sp);
+  float ls[3];
+  for (int k = 0; k < 3; k++) ls[k] = clampf(p[3 + k], -12.0f, 6.0f);
+  const float mls = ((ls[0] + ls[1]) + ls[2]) / 3.0f;
+  for (int k = 0; k < 3; k++) cf.aniso[k] = hp.anisoReg * (ls[k] - mls);
+  cf.ql = fmaxf(sqrtf(((p[6] * p[6] + p[7] * p[7]) + p[8] * p[8]) + p[9] * p[9]), 1e-6f);
+  for (int c = 0; c < 3; c++) cf.sCol[c] = gs_sigmoid_clamped(p[10 + c]);
+}
+
 // The reference's chain (shaders.js:735-947) for one view: the decoded
 // screen-space sums gp[10] of a Gaussian with keys in that view, chained
-// through the view's camera to the 14 parameter gradients gF and the SH
-// coefficient gradients shG (both zero on entry).
+// through the view's camera to the 14 parameter gradients and the SH
+// coefficient gradients. They are added to gF and shG, or written there
+// when `first` (the step's first view with keys): each value is the
+// view's own gradient first, so the sums round as the view's gradients
+// added in view order.
 template <int kDeg>
-__device__ __forceinline__ void chain_one(const Cam& cam, const AdamHP& hp, const float* p, const float* sh,
-                                          float* gp, float* gF, float* shG) {
+__device__ __forceinline__ void chain_one(const Cam& cam, const ChainFixed& cf, const float* p, const float* sh,
+                                          float* gp, bool first, float* gF, float* shG) {
   constexpr int K = ShDims<kDeg>::K;
+  auto put = [first](float& d, float v) { d = first ? v : d + v; };
   Geom g;
-  compute_geom(cam, p, g);
-  if (g.ok) {
-    const float cden = 1.0f + conic_lmax(g.va, g.vb, g.vc);
-    gp[2] = gp[2] * cden;
-    gp[3] = gp[3] * cden;
-    gp[4] = gp[4] * cden;
-    const float iz = 1.0f / g.pc[2];
+  compute_geom(cam, p, cf.sp, g);
+  if (!g.ok) {   // a zero gradient (a Gaussian behind the near plane has no keys)
+    for (int k = 0; k < 14; k++) put(gF[k], 0.0f);
+    for (int k = 0; k < 3 * K; k++) put(shG[k], 0.0f);
+    return;
+  }
+  const float cden = 1.0f + conic_lmax(g.va, g.vb, g.vc);
+  gp[2] = gp[2] * cden;
+  gp[3] = gp[3] * cden;
+  gp[4] = gp[4] * cden;
+  const float iz = 1.0f / g.pc[2];
 
-    // screen space -> Sigma, T, J and the camera-space position (:759-827)
-    ChainScreen cs;
-    chain_screen(cam, g, gp, iz, cs);
-    const float dS00 = cs.dS00, dS01 = cs.dS01, dS02 = cs.dS02;
-    const float dS11 = cs.dS11, dS12 = cs.dS12, dS22 = cs.dS22;
-    const float* dpc = cs.dpc;
-    const float* R = cam.R;
-    gF[0] = (R[0] * dpc[0] + R[3] * dpc[1]) + R[6] * dpc[2];
-    gF[1] = (R[1] * dpc[0] + R[4] * dpc[1]) + R[7] * dpc[2];
-    gF[2] = (R[2] * dpc[0] + R[5] * dpc[1]) + R[8] * dpc[2];
+  // screen space -> Sigma, T, J and the camera-space position (:759-827)
+  ChainScreen cs;
+  chain_screen(cam, g, gp, iz, cs);
+  const float dS00 = cs.dS00, dS01 = cs.dS01, dS02 = cs.dS02;
+  const float dS11 = cs.dS11, dS12 = cs.dS12, dS22 = cs.dS22;
+  const float* dpc = cs.dpc;
+  const float* R = cam.R;
+  float pos[3];
+  pos[0] = (R[0] * dpc[0] + R[3] * dpc[1]) + R[6] * dpc[2];
+  pos[1] = (R[1] * dpc[0] + R[4] * dpc[1]) + R[7] * dpc[2];
+  pos[2] = (R[2] * dpc[0] + R[5] * dpc[1]) + R[8] * dpc[2];
 
-    // Sigma = M M^T backward (:861-884)
-    float m0[3], m1[3], m2[3], dM0[3], dM1[3], dM2[3];
-    for (int k = 0; k < 3; k++) { m0[k] = g.r0[k] * g.s[k]; m1[k] = g.r1[k] * g.s[k]; m2[k] = g.r2[k] * g.s[k]; }
-    for (int k = 0; k < 3; k++) {
-      dM0[k] = 2.0f * ((dS00 * m0[k] + dS01 * m1[k]) + dS02 * m2[k]);
-      dM1[k] = 2.0f * ((dS01 * m0[k] + dS11 * m1[k]) + dS12 * m2[k]);
-      dM2[k] = 2.0f * ((dS02 * m0[k] + dS12 * m1[k]) + dS22 * m2[k]);
-    }
-    float ls[3];
-    for (int k = 0; k < 3; k++) ls[k] = clampf(p[3 + k], -12.0f, 6.0f);
-    const float mls = ((ls[0] + ls[1]) + ls[2]) / 3.0f;
-    for (int k = 0; k < 3; k++) {
-      const float dsv = (dM0[k] * g.r0[k] + dM1[k] * g.r1[k]) + dM2[k] * g.r2[k];
-      gF[3 + k] = dsv * g.s[k] + hp.anisoReg * (ls[k] - mls);
-    }
-
-    // quaternion backward (:887-904)
-    float dR0[3], dR1[3], dR2[3];
-    for (int k = 0; k < 3; k++) { dR0[k] = dM0[k] * g.s[k]; dR1[k] = dM1[k] * g.s[k]; dR2[k] = dM2[k] * g.s[k]; }
-    const float qw = g.q[0], qx = g.q[1], qy = g.q[2], qz = g.q[3];
-    const float gw = 2.0f * (((((dR0[1] * (-qz) + dR0[2] * qy) + dR1[0] * qz) + dR1[2] * (-qx)) + dR2[0] * (-qy)) + dR2[1] * qx);
-    const float gx = 2.0f * (((((((dR0[1] * qy + dR0[2] * qz) + dR1[0] * qy) + dR1[1] * (-2.0f * qx)) + dR1[2] * (-qw)) + dR2[0] * qz) + dR2[1] * qw) + dR2[2] * (-2.0f * qx));
-    const float gy = 2.0f * (((((((dR0[0] * (-2.0f * qy) + dR0[1] * qx) + dR0[2] * qw) + dR1[0] * qx) + dR1[2] * qz) + dR2[0] * (-qw)) + dR2[1] * qz) + dR2[2] * (-2.0f * qy));
-    const float gz = 2.0f * (((((((dR0[0] * (-2.0f * qz) + dR0[1] * (-qw)) + dR0[2] * qx) + dR1[0] * qw) + dR1[1] * (-2.0f * qz)) + dR1[2] * qy) + dR2[0] * qx) + dR2[1] * qy);
-    const float gq[4] = {gw, gx, gy, gz};
-    const float ql = fmaxf(sqrtf(((p[6] * p[6] + p[7] * p[7]) + p[8] * p[8]) + p[9] * p[9]), 1e-6f);
-    const float qdg = ((g.q[0] * gq[0] + g.q[1] * gq[1]) + g.q[2] * gq[2]) + g.q[3] * gq[3];
-    for (int k = 0; k < 4; k++) gF[6 + k] = (gq[k] - g.q[k] * qdg) / ql;
-
-    // colour (:907-947)
-    const float sCol[3] = {gs_sigmoid_clamped(p[10]), gs_sigmoid_clamped(p[11]), gs_sigmoid_clamped(p[12])};
-    float dRGB[3] = {gp[7], gp[8], gp[9]};
-    if constexpr (kDeg > 0) {
-      float v[3];
-      const float ulen = view_dir(cam, p, v);
-      float Y[15], D[45];
-      sh_basis<kDeg, true>(v, Y, D);
-      const int aK = cam.shActiveK;
-      float col[3] = {sCol[0], sCol[1], sCol[2]};
-#pragma unroll
-      for (int k = 0; k < K; k++) {
-        if (k < aK) {
-          col[0] = col[0] + sh[k] * Y[k];
-          col[1] = col[1] + sh[K + k] * Y[k];
-          col[2] = col[2] + sh[2 * K + k] * Y[k];
-        }
-      }
-      for (int c = 0; c < 3; c++) dRGB[c] = dRGB[c] * (col[c] > 0.0f ? 1.0f : 0.0f);
-      float gv[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-      for (int k = 0; k < K; k++) {
-        if (k < aK) {
-          shG[k] = dRGB[0] * Y[k];
-          shG[K + k] = dRGB[1] * Y[k];
-          shG[2 * K + k] = dRGB[2] * Y[k];
-          const float w = (dRGB[0] * sh[k] + dRGB[1] * sh[K + k]) + dRGB[2] * sh[2 * K + k];
-          for (int c = 0; c < 3; c++) gv[c] = gv[c] + w * D[3 * k + c];
-        }
-      }
-      const float vg = dot3(v, gv);
-      for (int c = 0; c < 3; c++) gF[c] = gF[c] + (gv[c] - v[c] * vg) / ulen;
-    }
-    for (int c = 0; c < 3; c++) gF[10 + c] = dRGB[c] * sCol[c] * (1.0f - sCol[c]);
-    gF[13] = gp[6];
+  // Sigma = M M^T backward (:861-884)
+  float m0[3], m1[3], m2[3], dM0[3], dM1[3], dM2[3];
+  for (int k = 0; k < 3; k++) { m0[k] = g.r0[k] * g.s[k]; m1[k] = g.r1[k] * g.s[k]; m2[k] = g.r2[k] * g.s[k]; }
+  for (int k = 0; k < 3; k++) {
+    dM0[k] = 2.0f * ((dS00 * m0[k] + dS01 * m1[k]) + dS02 * m2[k]);
+    dM1[k] = 2.0f * ((dS01 * m0[k] + dS11 * m1[k]) + dS12 * m2[k]);
+    dM2[k] = 2.0f * ((dS02 * m0[k] + dS12 * m1[k]) + dS22 * m2[k]);
+  }
+  for (int k = 0; k < 3; k++) {
+    const float dsv = (dM0[k] * g.r0[k] + dM1[k] * g.r1[k]) + dM2[k] * g.r2[k];
+    put(gF[3 + k], dsv * g.s[k] + cf.aniso[k]);
   }
 
+  // quaternion backward (:887-904)
+  float dR0[3], dR1[3], dR2[3];
+  for (int k = 0; k < 3; k++) { dR0[k] = dM0[k] * g.s[k]; dR1[k] = dM1[k] * g.s[k]; dR2[k] = dM2[k] * g.s[k]; }
+  const float qw = g.q[0], qx = g.q[1], qy = g.q[2], qz = g.q[3];
+  const float gw = 2.0f * (((((dR0[1] * (-qz) + dR0[2] * qy) + dR1[0] * qz) + dR1[2] * (-qx)) + dR2[0] * (-qy)) + dR2[1] * qx);
+  const float gx = 2.0f * (((((((dR0[1] * qy + dR0[2] * qz) + dR1[0] * qy) + dR1[1] * (-2.0f * qx)) + dR1[2] * (-qw)) + dR2[0] * qz) + dR2[1] * qw) + dR2[2] * (-2.0f * qx));
+  const float gy = 2.0f * (((((((dR0[0] * (-2.0f * qy) + dR0[1] * qx) + dR0[2] * qw) + dR1[0] * qx) + dR1[2] * qz) + dR2[0] * (-qw)) + dR2[1] * qz) + dR2[2] * (-2.0f * qy));
+  const float gz = 2.0f * (((((((dR0[0] * (-2.0f * qz) + dR0[1] * (-qw)) + dR0[2] * qx) + dR1[0] * qw) + dR1[1] * (-2.0f * qz)) + dR1[2] * qy) + dR2[0] * qx) + dR2[1] * qy);
+  const float gq[4] = {gw, gx, gy, gz};
+  const float qdg = ((g.q[0] * gq[0] + g.q[1] * gq[1]) + g.q[2] * gq[2]) + g.q[3] * gq[3];
+  for (int k = 0; k < 4; k++) put(gF[6 + k], (gq[k] - g.q[k] * qdg) / cf.ql);
+
+  // colour (:907-947)
+  const float* sCol = cf.sCol;
+  float dRGB[3] = {gp[7], gp[8], gp[9]};
+  if constexpr (kDeg > 0) {
+    float v[3];
+    const float ulen = view_dir(cam, p, v);
+    float Y[15], D[45];
+    sh_basis<kDeg, true>(v, Y, D);
+    const int aK = cam.shActiveK;
+    float col[3] = {sCol[0], sCol[1], sCol[2]};
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      if (k < aK) {
+        col[0] = col[0] + sh[k] * Y[k];
+        col[1] = col[1] + sh[K + k] * Y[k];
+        col[2] = col[2] + sh[2 * K + k] * Y[k];
+      }
+    }
+    for (int c = 0; c < 3; c++) dRGB[c] = dRGB[c] * (col[c] > 0.0f ? 1.0f : 0.0f);
+    float gv[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      if (k < aK) {
+        put(shG[k], dRGB[0] * Y[k]);
+        put(shG[K + k], dRGB[1] * Y[k]);
+        put(shG[2 * K + k], dRGB[2] * Y[k]);
+        const float w = (dRGB[0] * sh[k] + dRGB[1] * sh[K + k]) + dRGB[2] * sh[2 * K + k];
+        for (int c = 0; c < 3; c++) gv[c] = gv[c] + w * D[3 * k + c];
+      } else {
+        put(shG[k], 0.0f);
+        put(shG[K + k], 0.0f);
+        put(shG[2 * K + k], 0.0f);
+      }
+    }
+    const float vg = dot3(v, gv);
+    for (int c = 0; c < 3; c++) pos[c] = pos[c] + (gv[c] - v[c] * vg) / ulen;
+  }
+  for (int c = 0; c < 3; c++) put(gF[c], pos[c]);
+  for (int c = 0; c < 3; c++) put(gF[10 + c], dRGB[c] * sCol[c] * (1.0f - sCol[c]));
+  put(gF[13], gp[6]);
 }
 
 // Adam over one Gaussian's SH coefficients and parameters with gradients gF
@@ -358,7 +407,9 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
   if (keyed) {
     float gp[10];
     take_sums<kAccum>(u.accum, i, gp);
-    chain_one<kDeg>(cam, hp, p, sh, gp, gF, shG);
+    ChainFixed cf;
+    chain_fixed(hp, p, cf);
+    chain_one<kDeg>(cam, cf, p, sh, gp, true, gF, shG);
   }
 
   if (kDebug) {
@@ -373,7 +424,9 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
 // A batched step's update: each view's sums chained through that view's
 // camera, the views' gradients added in view order (the first view with
 // keys gives the value, later ones add to it), then one Adam step with the
-// step's constants (vb->hp). Gaussians without keys in a view skip it.
+// step's constants (vb->hp). Gaussians without keys in a view skip it. The
+// camera-independent terms are computed once for all views, and the views'
+// key counts are read up front, all loads in flight together.
 template <int kDeg, int kAccum, bool kDebug, int kPrec>
 __global__ __launch_bounds__(kBlock) void k_update_views(const ViewBlock* __restrict__ vb, UpdateBufs u) {
   constexpr int NC = ShDims<kDeg>::C;
@@ -381,6 +434,9 @@ __global__ __launch_bounds__(kBlock) void k_update_views(const ViewBlock* __rest
   if (i >= u.n) return;
   const int64_t cap = u.cap;
   const AdamHP hp = vb->hp;
+  const int V = vb->count;
+  uint32_t keyed = 0;   // bit v: Gaussian i has keys in view v
+  for (int v = 0; v < V; v++) keyed |= (gs_global(vb->cnt[v])[i] != 0u ? 1u : 0u) << v;
   float p[16];
   load_params(u.P, cap, i, p);
   float sh[ShDims<kDeg>::S > 0 ? ShDims<kDeg>::S : 1];
@@ -391,25 +447,22 @@ __global__ __launch_bounds__(kBlock) void k_update_views(const ViewBlock* __rest
   float shG[NC > 0 ? NC : 1];
 #pragma unroll
   for (int k = 0; k < NC; k++) shG[k] = 0.0f;
+  ChainFixed cf;
+  if (keyed) chain_fixed(hp, p, cf);
+  // view v's sums in cur; view v + 1's load is in flight during view v's chain
+  RawSums cur{}, nxt{};
+  if (keyed & 1u) cur = load_sums(gs_global(vb->accum[0]), i);
   bool first = true;
-  const int V = vb->count;
   for (int v = 0; v < V; v++) {
-    if (vb->cnt[v][i] == 0u) continue;
-    float gp[10];
-    take_sums<kAccum>(vb->accum[v], i, gp);
-    const Cam cam = vb->cams[v];
-    float gFv[14];
-#pragma unroll
-    for (int k = 0; k < 14; k++) gFv[k] = 0.0f;
-    float shGv[NC > 0 ? NC : 1];
-#pragma unroll
-    for (int k = 0; k < NC; k++) shGv[k] = 0.0f;
-    chain_one<kDeg>(cam, hp, p, sh, gp, gFv, shGv);
-#pragma unroll
-    for (int k = 0; k < 14; k++) gF[k] = first ? gFv[k] : gF[k] + gFv[k];
-#pragma unroll
-    for (int k = 0; k < NC; k++) shG[k] = first ? shGv[k] : shG[k] + shGv[k];
-    first = false;
+    if (v + 1 < V && ((keyed >> (v + 1)) & 1u)) nxt = load_sums(gs_global(vb->accum[v + 1]), i);
+    if ((keyed >> v) & 1u) {
+      clear_sums(gs_global(vb->accum[v]), i);
+      float gp[10];
+      decode_sums<kAccum>(cur, gp);
+      chain_one<kDeg>(vb->cams[v], cf, p, sh, gp, first, gF, shG);
+      first = false;
+    }
+    cur = nxt;
   }
   if (kDebug) {
     for (int k = 0; k < 14; k++) u.dbgGradF[16 * i + k] = gF[k];
