@@ -259,7 +259,8 @@ int raster_views_blocks(int device, const RasterShape& sh);
 // overlapped form of the batched step: forward with the loss and backward
 void launch_raster_fwd_bwd_view(hipStream_t st, const RasterShape& sh, const ViewBlock* vb, int v, int numTiles,
                                 const Cut& cut, int accumMode, bool boundedColourGrad);
-void launch_update_views(hipStream_t st, int deg, int accumMode, int prec, const ViewBlock* vb,
+// V: the views in vb (sizes the kernel's LDS)
+void launch_update_views(hipStream_t st, int deg, int accumMode, int prec, const ViewBlock* vb, int V,
                          const UpdateBufs& u);
 
 // ---- camera pose, focal and exposure gradients (camOpt) ----

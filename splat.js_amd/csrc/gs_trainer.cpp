@@ -961,7 +961,7 @@ int enqueue_views(gs_trainer* h, const ViewBlock& vb, gs_debug_out* dbg, float* 
     u.dbgGradF = dGF;
     u.dbgShGrad = dSG;
     Scope sc(h, "update_views", (double)n * (100.0 + 6 * 64.0 + 6 * 4.0 * h->shC));
-    launch_update_views(h->st, h->shDeg, h->o.gradAccum, h->prec, h->dVB, u);
+    launch_update_views(h->st, h->shDeg, h->o.gradAccum, h->prec, h->dVB, V, u);
   }
   HIPCK(hipGetLastError());
   return GS_OK;

@@ -77,70 +77,29 @@ __device__ __forceinline__ void chain_fixed(const AdamHP& hp, const float* p, Ch
   for (int c = 0; c < 3; c++) cf.sCol[c] = gs_sigmoid_clamped(p[10 + c]);
 }
 
-// The reference's chain (shaders.js:735-947) for one view: the decoded
-// screen-space sums gp[10] of a Gaussian with keys in that view, chained
-// through the view's camera to the 14 parameter gradients and the SH
-// coefficient gradients. They are added to gF and shG, or written there
-// when `first` (the step's first view with keys): each value is the
-// view's own gradient first, so the sums round as the view's gradients
-// added in view order.
+// The reference's chain (shaders.js:735-947) for one view, in two halves
+// that a batched update runs as separate passes over its views. Both take
+// the decoded screen-space sums gp[10] of a Gaussian with keys in the view
+// and add the view's gradients to gF (14 parameter slots) and shG (SH
+// coefficients), or write them there when `first` (the step's first view
+// with keys): each value is the view's own gradient first, so the sums
+// round as the views' gradients added in view order.
+
+// Colour half (:906-947): SH coefficient gradients into shG, colour-logit
+// gradients into gF[10..12]; dpos gets the view-direction term of the
+// position gradient (SH degree > 0), which chain_geometry adds to the
+// projection's term before it adds the sum to gF[0..2].
 template <int kDeg>
-__device__ __forceinline__ void chain_one(const Cam& cam, const ChainFixed& cf, const float* p, const float* sh,
-                                          float* gp, bool first, float* gF, float* shG) {
+__device__ __forceinline__ void chain_colour(const Cam& cam, const float* sCol, const float* p, const float* sh,
+                                             const float* gp, bool first, float* gF, float* shG, float* dpos) {
   constexpr int K = ShDims<kDeg>::K;
   auto put = [first](float& d, float v) { d = first ? v : d + v; };
   Geom g;
-  compute_geom(cam, p, cf.sp, g);
-  if (!g.ok) {   // a zero gradient (a Gaussian behind the near plane has no keys)
-    for (int k = 0; k < 14; k++) put(gF[k], 0.0f);
+  if (!geom_position(cam, p, g)) {   // a zero gradient (a Gaussian behind the near plane has no keys)
+    for (int c = 0; c < 3; c++) put(gF[10 + c], 0.0f);
     for (int k = 0; k < 3 * K; k++) put(shG[k], 0.0f);
     return;
   }
-  const float cden = 1.0f + conic_lmax(g.va, g.vb, g.vc);
-  gp[2] = gp[2] * cden;
-  gp[3] = gp[3] * cden;
-  gp[4] = gp[4] * cden;
-  const float iz = 1.0f / g.pc[2];
-
-  // screen space -> Sigma, T, J and the camera-space position (:759-827)
-  ChainScreen cs;
-  chain_screen(cam, g, gp, iz, cs);
-  const float dS00 = cs.dS00, dS01 = cs.dS01, dS02 = cs.dS02;
-  const float dS11 = cs.dS11, dS12 = cs.dS12, dS22 = cs.dS22;
-  const float* dpc = cs.dpc;
-  const float* R = cam.R;
-  float pos[3];
-  pos[0] = (R[0] * dpc[0] + R[3] * dpc[1]) + R[6] * dpc[2];
-  pos[1] = (R[1] * dpc[0] + R[4] * dpc[1]) + R[7] * dpc[2];
-  pos[2] = (R[2] * dpc[0] + R[5] * dpc[1]) + R[8] * dpc[2];
-
-  // Sigma = M M^T backward (:861-884)
-  float m0[3], m1[3], m2[3], dM0[3], dM1[3], dM2[3];
-  for (int k = 0; k < 3; k++) { m0[k] = g.r0[k] * g.s[k]; m1[k] = g.r1[k] * g.s[k]; m2[k] = g.r2[k] * g.s[k]; }
-  for (int k = 0; k < 3; k++) {
-    dM0[k] = 2.0f * ((dS00 * m0[k] + dS01 * m1[k]) + dS02 * m2[k]);
-    dM1[k] = 2.0f * ((dS01 * m0[k] + dS11 * m1[k]) + dS12 * m2[k]);
-    dM2[k] = 2.0f * ((dS02 * m0[k] + dS12 * m1[k]) + dS22 * m2[k]);
-  }
-  for (int k = 0; k < 3; k++) {
-    const float dsv = (dM0[k] * g.r0[k] + dM1[k] * g.r1[k]) + dM2[k] * g.r2[k];
-    put(gF[3 + k], dsv * g.s[k] + cf.aniso[k]);
-  }
-
-  // quaternion backward (:887-904)
-  float dR0[3], dR1[3], dR2[3];
-  for (int k = 0; k < 3; k++) { dR0[k] = dM0[k] * g.s[k]; dR1[k] = dM1[k] * g.s[k]; dR2[k] = dM2[k] * g.s[k]; }
-  const float qw = g.q[0], qx = g.q[1], qy = g.q[2], qz = g.q[3];
-  const float gw = 2.0f * (((((dR0[1] * (-qz) + dR0[2] * qy) + dR1[0] * qz) + dR1[2] * (-qx)) + dR2[0] * (-qy)) + dR2[1] * qx);
-  const float gx = 2.0f * (((((((dR0[1] * qy + dR0[2] * qz) + dR1[0] * qy) + dR1[1] * (-2.0f * qx)) + dR1[2] * (-qw)) + dR2[0] * qz) + dR2[1] * qw) + dR2[2] * (-2.0f * qx));
-  const float gy = 2.0f * (((((((dR0[0] * (-2.0f * qy) + dR0[1] * qx) + dR0[2] * qw) + dR1[0] * qx) + dR1[2] * qz) + dR2[0] * (-qw)) + dR2[1] * qz) + dR2[2] * (-2.0f * qy));
-  const float gz = 2.0f * (((((((dR0[0] * (-2.0f * qz) + dR0[1] * (-qw)) + dR0[2] * qx) + dR1[0] * qw) + dR1[1] * (-2.0f * qz)) + dR1[2] * qy) + dR2[0] * qx) + dR2[1] * qy);
-  const float gq[4] = {gw, gx, gy, gz};
-  const float qdg = ((g.q[0] * gq[0] + g.q[1] * gq[1]) + g.q[2] * gq[2]) + g.q[3] * gq[3];
-  for (int k = 0; k < 4; k++) put(gF[6 + k], (gq[k] - g.q[k] * qdg) / cf.ql);
-
-  // colour (:907-947)
-  const float* sCol = cf.sCol;
   float dRGB[3] = {gp[7], gp[8], gp[9]};
   if constexpr (kDeg > 0) {
     float v[3];
@@ -174,19 +133,79 @@ __device__ __forceinline__ void chain_one(const Cam& cam, const ChainFixed& cf, 
       }
     }
     const float vg = dot3(v, gv);
-    for (int c = 0; c < 3; c++) pos[c] = pos[c] + (gv[c] - v[c] * vg) / ulen;
+    for (int c = 0; c < 3; c++) dpos[c] = (gv[c] - v[c] * vg) / ulen;
   }
-  for (int c = 0; c < 3; c++) put(gF[c], pos[c]);
   for (int c = 0; c < 3; c++) put(gF[10 + c], dRGB[c] * sCol[c] * (1.0f - sCol[c]));
+}
+
+// Geometry half (:735-904): position, log-scale and rotation gradients into
+// gF[0..9] (the position's with chain_colour's view-direction term dpos),
+// the opacity gradient into gF[13].
+template <int kDeg>
+__device__ __forceinline__ void chain_geometry(const Cam& cam, const ChainFixed& cf, const float* p, float* gp,
+                                               bool first, float* gF, const float* dpos) {
+  auto put = [first](float& d, float v) { d = first ? v : d + v; };
+  Geom g;
+  compute_geom(cam, p, cf.sp, g);
+  if (!g.ok) {   // a zero gradient (see chain_colour)
+    for (int k = 0; k < 10; k++) put(gF[k], 0.0f);
+    put(gF[13], 0.0f);
+    return;
+  }
+  const float cden = 1.0f + conic_lmax(g.va, g.vb, g.vc);
+  gp[2] = gp[2] * cden;
+  gp[3] = gp[3] * cden;
+  gp[4] = gp[4] * cden;
+  const float iz = 1.0f / g.pc[2];
+
+  // screen space -> Sigma, T, J and the camera-space position (:759-827)
+  ChainScreen cs;
+  chain_screen(cam, g, gp, iz, cs);
+  const float dS00 = cs.dS00, dS01 = cs.dS01, dS02 = cs.dS02;
+  const float dS11 = cs.dS11, dS12 = cs.dS12, dS22 = cs.dS22;
+  const float* dpc = cs.dpc;
+  const float* R = cam.R;
+  float pos[3];
+  pos[0] = (R[0] * dpc[0] + R[3] * dpc[1]) + R[6] * dpc[2];
+  pos[1] = (R[1] * dpc[0] + R[4] * dpc[1]) + R[7] * dpc[2];
+  pos[2] = (R[2] * dpc[0] + R[5] * dpc[1]) + R[8] * dpc[2];
+  if constexpr (kDeg > 0)
+    for (int c = 0; c < 3; c++) pos[c] = pos[c] + dpos[c];   // (:914-935)
+  for (int c = 0; c < 3; c++) put(gF[c], pos[c]);
+
+  // Sigma = M M^T backward (:861-884)
+  float m0[3], m1[3], m2[3], dM0[3], dM1[3], dM2[3];
+  for (int k = 0; k < 3; k++) { m0[k] = g.r0[k] * g.s[k]; m1[k] = g.r1[k] * g.s[k]; m2[k] = g.r2[k] * g.s[k]; }
+  for (int k = 0; k < 3; k++) {
+    dM0[k] = 2.0f * ((dS00 * m0[k] + dS01 * m1[k]) + dS02 * m2[k]);
+    dM1[k] = 2.0f * ((dS01 * m0[k] + dS11 * m1[k]) + dS12 * m2[k]);
+    dM2[k] = 2.0f * ((dS02 * m0[k] + dS12 * m1[k]) + dS22 * m2[k]);
+  }
+  for (int k = 0; k < 3; k++) {
+    const float dsv = (dM0[k] * g.r0[k] + dM1[k] * g.r1[k]) + dM2[k] * g.r2[k];
+    put(gF[3 + k], dsv * g.s[k] + cf.aniso[k]);
+  }
+
+  // quaternion backward (:887-904)
+  float dR0[3], dR1[3], dR2[3];
+  for (int k = 0; k < 3; k++) { dR0[k] = dM0[k] * g.s[k]; dR1[k] = dM1[k] * g.s[k]; dR2[k] = dM2[k] * g.s[k]; }
+  const float qw = g.q[0], qx = g.q[1], qy = g.q[2], qz = g.q[3];
+  const float gw = 2.0f * (((((dR0[1] * (-qz) + dR0[2] * qy) + dR1[0] * qz) + dR1[2] * (-qx)) + dR2[0] * (-qy)) + dR2[1] * qx);
+  const float gx = 2.0f * (((((((dR0[1] * qy + dR0[2] * qz) + dR1[0] * qy) + dR1[1] * (-2.0f * qx)) + dR1[2] * (-qw)) + dR2[0] * qz) + dR2[1] * qw) + dR2[2] * (-2.0f * qx));
+  const float gy = 2.0f * (((((((dR0[0] * (-2.0f * qy) + dR0[1] * qx) + dR0[2] * qw) + dR1[0] * qx) + dR1[2] * qz) + dR2[0] * (-qw)) + dR2[1] * qz) + dR2[2] * (-2.0f * qy));
+  const float gz = 2.0f * (((((((dR0[0] * (-2.0f * qz) + dR0[1] * (-qw)) + dR0[2] * qx) + dR1[0] * qw) + dR1[1] * (-2.0f * qz)) + dR1[2] * qy) + dR2[0] * qx) + dR2[1] * qy);
+  const float gq[4] = {gw, gx, gy, gz};
+  const float qdg = ((g.q[0] * gq[0] + g.q[1] * gq[1]) + g.q[2] * gq[2]) + g.q[3] * gq[3];
+  for (int k = 0; k < 4; k++) put(gF[6 + k], (gq[k] - g.q[k] * qdg) / cf.ql);
   put(gF[13], gp[6]);
 }
 
-// Adam over one Gaussian's SH coefficients and parameters with gradients gF
-// and shG (the reference's sh-adam and adam passes), in the storage formats
-// of precision mode kPrec; p and sh hold the current values.
+// Adam over one Gaussian's SH coefficients (the reference's sh-adam pass)
+// with gradients shG, in the storage formats of precision mode kPrec; sh
+// holds the current values.
 template <int kDeg, int kPrec>
-__device__ __forceinline__ void adam_one(const AdamHP& hp, const UpdateBufs& u, int64_t i, float* p,
-                                         const float* sh, const float* gF, const float* shG) {
+__device__ __forceinline__ void adam_sh(const AdamHP& hp, const UpdateBufs& u, int64_t i, const float* sh,
+                                        const float* shG) {
   constexpr int NC = ShDims<kDeg>::C;
   constexpr int P4 = ShDims<kDeg>::P4;
   constexpr int Q8 = ShDims<kDeg>::Q8;
@@ -307,11 +326,17 @@ __device__ __forceinline__ void adam_one(const AdamHP& hp, const UpdateBufs& u, 
       SHV[c * cap + i] = make_float4(vs[0], vs[1], vs[2], vs[3]);
     }
   }
+}
 
+// Adam over one Gaussian's 16 parameter slots (the reference's adam pass)
+// with gradients gF; p holds the current values and takes the new ones.
+template <int kPrec>
+__device__ __forceinline__ void adam_params(const AdamHP& hp, const UpdateBufs& u, int64_t i, float* p,
+                                            const float* gF) {
+  const int64_t cap = u.cap;
   // ---- Adam over the 16 param slots (shaders.js:1000-1035) ----
   // One storage plane at a time (4 f32 or 8 bf16 slots), so only that
-  // plane's moments are live; the SH pass above has already retired the SH
-  // registers.
+  // plane's moments are live.
   auto adam_slot = [&](int j, float& mj, float& vj) {
     const float lr = hp.lr[j];
     if (j >= 14 || lr == 0.0f) return;
@@ -395,41 +420,63 @@ __global__ __launch_bounds__(kBlock) void k_update(Cam cam, AdamHP hp, UpdateBuf
   const bool keyed = u.cnt[i] != 0u;
   float p[16];
   load_params(u.P, cap, i, p);
-  float sh[ShDims<kDeg>::S > 0 ? ShDims<kDeg>::S : 1];
-  if constexpr (kDeg > 0) load_sh_any<kDeg, Prec<kPrec>::sh>(u.SH, cap, i, sh);
-
   float gF[14];
 #pragma unroll
   for (int k = 0; k < 14; k++) gF[k] = 0.0f;
-  float shG[NC > 0 ? NC : 1];
+  float gp[10];
+  if (keyed) take_sums<kAccum>(u.accum, i, gp);
+  // the colour half of the chain and the SH Adam step first, so the SH
+  // registers are free during the geometry half (as in k_update_views)
+  float dpos[3] = {0.0f, 0.0f, 0.0f};
+  {
+    float sh[ShDims<kDeg>::S > 0 ? ShDims<kDeg>::S : 1];
+    if constexpr (kDeg > 0) load_sh_any<kDeg, Prec<kPrec>::sh>(u.SH, cap, i, sh);
+    float shG[NC > 0 ? NC : 1];
 #pragma unroll
-  for (int k = 0; k < NC; k++) shG[k] = 0.0f;
+    for (int k = 0; k < NC; k++) shG[k] = 0.0f;
+    if (keyed) {
+      float sCol[3];
+      for (int c = 0; c < 3; c++) sCol[c] = gs_sigmoid_clamped(p[10 + c]);
+      chain_colour<kDeg>(cam, sCol, p, sh, gp, true, gF, shG, dpos);
+    }
+    if (kDebug)
+      for (int k = 0; k < NC; k++) u.dbgShGrad[(int64_t)NC * i + k] = shG[k];
+    adam_sh<kDeg, kPrec>(hp, u, i, sh, shG);
+  }
   if (keyed) {
-    float gp[10];
-    take_sums<kAccum>(u.accum, i, gp);
     ChainFixed cf;
     chain_fixed(hp, p, cf);
-    chain_one<kDeg>(cam, cf, p, sh, gp, true, gF, shG);
+    chain_geometry<kDeg>(cam, cf, p, gp, true, gF, dpos);
   }
-
   if (kDebug) {
     for (int k = 0; k < 14; k++) u.dbgGradF[16 * i + k] = gF[k];
     u.dbgGradF[16 * i + 14] = 0.0f;
     u.dbgGradF[16 * i + 15] = 0.0f;
-    for (int k = 0; k < NC; k++) u.dbgShGrad[(int64_t)NC * i + k] = shG[k];
   }
-  adam_one<kDeg, kPrec>(hp, u, i, p, sh, gF, shG);
+  adam_params<kPrec>(hp, u, i, p, gF);
 }
 
 // A batched step's update: each view's sums chained through that view's
 // camera, the views' gradients added in view order (the first view with
 // keys gives the value, later ones add to it), then one Adam step with the
-// step's constants (vb->hp). Gaussians without keys in a view skip it. The
-// camera-independent terms are computed once for all views, and the views'
-// key counts are read up front, all loads in flight together.
+// step's constants (vb->hp). Gaussians without keys in a view skip it.
+//
+// Two passes over the views keep the registers of one half of the chain
+// live at a time: the colour pass accumulates the SH and colour-logit
+// gradients and ends with the SH Adam step; the geometry pass then
+// accumulates the position, scale, rotation and opacity gradients and ends
+// with the parameter Adam step. Each view's view-direction term of the
+// position gradient goes from the colour pass to the geometry pass through
+// LDS (dposLds: 3 floats per view and thread, V * 3 * kBlock floats per
+// block). In one pass the chain held 244 VGPRs at SH degree 2 and more than
+// 256 at degree 3 (1 wave per SIMD); in two, 164 and ~230. The
+// camera-independent terms are computed once for all views, the views' key
+// counts are read up front, and the geometry pass loads the next view's
+// sums while it chains the current one.
 template <int kDeg, int kAccum, bool kDebug, int kPrec>
 __global__ __launch_bounds__(kBlock) void k_update_views(const ViewBlock* __restrict__ vb, UpdateBufs u) {
   constexpr int NC = ShDims<kDeg>::C;
+  extern __shared__ float dposLds[];
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= u.n) return;
   const int64_t cap = u.cap;
@@ -437,16 +484,48 @@ __global__ __launch_bounds__(kBlock) void k_update_views(const ViewBlock* __rest
   const int V = vb->count;
   uint32_t keyed = 0;   // bit v: Gaussian i has keys in view v
   for (int v = 0; v < V; v++) keyed |= (gs_global(vb->cnt[v])[i] != 0u ? 1u : 0u) << v;
+  // the colour pass reads only the position and the colour logits; the
+  // geometry pass loads the parameters again, so the other slots hold no
+  // registers meanwhile
   float p[16];
   load_params(u.P, cap, i, p);
-  float sh[ShDims<kDeg>::S > 0 ? ShDims<kDeg>::S : 1];
-  if constexpr (kDeg > 0) load_sh_any<kDeg, Prec<kPrec>::sh>(u.SH, cap, i, sh);
   float gF[14];
 #pragma unroll
   for (int k = 0; k < 14; k++) gF[k] = 0.0f;
-  float shG[NC > 0 ? NC : 1];
+  float sCol[3];
+  if (keyed)
+    for (int c = 0; c < 3; c++) sCol[c] = gs_sigmoid_clamped(p[10 + c]);
+
+  // ---- colour pass: SH and colour-logit gradients, then the SH Adam step
+  {
+    float sh[ShDims<kDeg>::S > 0 ? ShDims<kDeg>::S : 1];
+    if constexpr (kDeg > 0) load_sh_any<kDeg, Prec<kPrec>::sh>(u.SH, cap, i, sh);
+    float shG[NC > 0 ? NC : 1];
 #pragma unroll
-  for (int k = 0; k < NC; k++) shG[k] = 0.0f;
+    for (int k = 0; k < NC; k++) shG[k] = 0.0f;
+    bool first = true;
+    for (int v = 0; v < V; v++) {
+      if (!((keyed >> v) & 1u)) continue;
+      // sums 7..9: the last word of the second int4 and the third int4's first two
+      const int4* acc = (const int4*)gs_global(vb->accum[v]) + 3 * i;
+      const int4 a1 = acc[1], a2 = acc[2];
+      float gp[10];
+      const int raw[3] = {a1.w, a2.x, a2.y};
+      for (int k = 0; k < 3; k++) gp[7 + k] = kAccum == 0 ? (float)raw[k] / kFixed : __int_as_float(raw[k]);
+      float dpos[3] = {0.0f, 0.0f, 0.0f};
+      chain_colour<kDeg>(vb->cams[v], sCol, p, sh, gp, first, gF, shG, dpos);
+      if constexpr (kDeg > 0)
+        for (int c = 0; c < 3; c++) dposLds[(3 * v + c) * kBlock + threadIdx.x] = dpos[c];
+      first = false;
+    }
+    if (kDebug)
+      for (int k = 0; k < NC; k++) u.dbgShGrad[(int64_t)NC * i + k] = shG[k];
+    adam_sh<kDeg, kPrec>(hp, u, i, sh, shG);
+  }
+
+  // ---- geometry pass: position, scale, rotation and opacity gradients,
+  // then the parameter Adam step
+  load_params(u.P, cap, i, p);
   ChainFixed cf;
   if (keyed) chain_fixed(hp, p, cf);
   // view v's sums in cur; view v + 1's load is in flight during view v's chain
@@ -459,7 +538,10 @@ __global__ __launch_bounds__(kBlock) void k_update_views(const ViewBlock* __rest
       clear_sums(gs_global(vb->accum[v]), i);
       float gp[10];
       decode_sums<kAccum>(cur, gp);
-      chain_one<kDeg>(vb->cams[v], cf, p, sh, gp, first, gF, shG);
+      float dpos[3] = {0.0f, 0.0f, 0.0f};
+      if constexpr (kDeg > 0)
+        for (int c = 0; c < 3; c++) dpos[c] = dposLds[(3 * v + c) * kBlock + threadIdx.x];
+      chain_geometry<kDeg>(vb->cams[v], cf, p, gp, first, gF, dpos);
       first = false;
     }
     cur = nxt;
@@ -468,43 +550,44 @@ __global__ __launch_bounds__(kBlock) void k_update_views(const ViewBlock* __rest
     for (int k = 0; k < 14; k++) u.dbgGradF[16 * i + k] = gF[k];
     u.dbgGradF[16 * i + 14] = 0.0f;
     u.dbgGradF[16 * i + 15] = 0.0f;
-    for (int k = 0; k < NC; k++) u.dbgShGrad[(int64_t)NC * i + k] = shG[k];
   }
-  adam_one<kDeg, kPrec>(hp, u, i, p, sh, gF, shG);
+  adam_params<kPrec>(hp, u, i, p, gF);
 }
 
 template <int kDeg, int kPrec>
-static void launch_views_deg(hipStream_t st, int accumMode, bool dbg, const ViewBlock* vb, const UpdateBufs& u) {
+static void launch_views_deg(hipStream_t st, int accumMode, bool dbg, const ViewBlock* vb, int V,
+                             const UpdateBufs& u) {
   const dim3 grid((unsigned)((u.n + kBlock - 1) / kBlock));
+  const size_t lds = kDeg > 0 ? (size_t)V * 3 * kBlock * sizeof(float) : 0;   // dposLds
   if (accumMode == 0) {
-    if (dbg) k_update_views<kDeg, 0, true, kPrec><<<grid, kBlock, 0, st>>>(vb, u);
-    else k_update_views<kDeg, 0, false, kPrec><<<grid, kBlock, 0, st>>>(vb, u);
+    if (dbg) k_update_views<kDeg, 0, true, kPrec><<<grid, kBlock, lds, st>>>(vb, u);
+    else k_update_views<kDeg, 0, false, kPrec><<<grid, kBlock, lds, st>>>(vb, u);
   } else {
-    if (dbg) k_update_views<kDeg, 1, true, kPrec><<<grid, kBlock, 0, st>>>(vb, u);
-    else k_update_views<kDeg, 1, false, kPrec><<<grid, kBlock, 0, st>>>(vb, u);
+    if (dbg) k_update_views<kDeg, 1, true, kPrec><<<grid, kBlock, lds, st>>>(vb, u);
+    else k_update_views<kDeg, 1, false, kPrec><<<grid, kBlock, lds, st>>>(vb, u);
   }
 }
 
 template <int kPrec>
-static void launch_views_prec(hipStream_t st, int deg, int accumMode, bool dbg, const ViewBlock* vb,
+static void launch_views_prec(hipStream_t st, int deg, int accumMode, bool dbg, const ViewBlock* vb, int V,
                               const UpdateBufs& u) {
   switch (deg) {
-    case 0: launch_views_deg<0, kPrec>(st, accumMode, dbg, vb, u); break;
-    case 1: launch_views_deg<1, kPrec>(st, accumMode, dbg, vb, u); break;
-    case 2: launch_views_deg<2, kPrec>(st, accumMode, dbg, vb, u); break;
-    default: launch_views_deg<3, kPrec>(st, accumMode, dbg, vb, u); break;
+    case 0: launch_views_deg<0, kPrec>(st, accumMode, dbg, vb, V, u); break;
+    case 1: launch_views_deg<1, kPrec>(st, accumMode, dbg, vb, V, u); break;
+    case 2: launch_views_deg<2, kPrec>(st, accumMode, dbg, vb, V, u); break;
+    default: launch_views_deg<3, kPrec>(st, accumMode, dbg, vb, V, u); break;
   }
 }
 
-void launch_update_views(hipStream_t st, int deg, int accumMode, int prec, const ViewBlock* vb,
+void launch_update_views(hipStream_t st, int deg, int accumMode, int prec, const ViewBlock* vb, int V,
                          const UpdateBufs& u) {
   if (u.n <= 0) return;
   const bool dbg = u.dbgGradF != nullptr;
   switch (prec) {
-    case 0: launch_views_prec<0>(st, deg, accumMode, dbg, vb, u); break;
-    case 1: launch_views_prec<1>(st, deg, accumMode, dbg, vb, u); break;
-    case 2: launch_views_prec<2>(st, deg, accumMode, dbg, vb, u); break;
-    default: launch_views_prec<3>(st, deg, accumMode, dbg, vb, u); break;
+    case 0: launch_views_prec<0>(st, deg, accumMode, dbg, vb, V, u); break;
+    case 1: launch_views_prec<1>(st, deg, accumMode, dbg, vb, V, u); break;
+    case 2: launch_views_prec<2>(st, deg, accumMode, dbg, vb, V, u); break;
+    default: launch_views_prec<3>(st, deg, accumMode, dbg, vb, V, u); break;
   }
 }
 
