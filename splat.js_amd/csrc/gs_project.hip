@@ -184,12 +184,18 @@ __global__ __launch_bounds__(kBlock) void k_project_view(int64_t n, int64_t cap,
   project_thread<kDeg, kPrec>(n, cap, cam, cut, P, SH, rec, depthKey, rect, cnt, chunkKeys);
 }
 
+// Waves per SIMD the batched projection is compiled for: up to SH degree 2
+// it fits 72 VGPRs, 7 waves (left free it took 79, 6 waves, and 1.7% more
+// time); at degree 3 the 48 coefficients it keeps for all views need ~108.
+template <int kDeg>
+constexpr int kProjViewsWaves = kDeg <= 2 ? 7 : 4;
+
 // every view of a batched step in one launch: one thread per Gaussian loads
 // its parameters and SH and computes the camera-independent terms once,
 // then projects them through views 0..V-1 in turn (cameras from vb->cams,
 // buffers from vb->seg)
 template <int kDeg, int kPrec>
-__global__ __launch_bounds__(kBlock) void k_project_views(int64_t n, int64_t cap, const ViewBlock* __restrict__ vb,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kProjViewsWaves<kDeg>))) void k_project_views(int64_t n, int64_t cap, const ViewBlock* __restrict__ vb,
                                                           int V, Cut cut, const float4* __restrict__ P,
                                                           const void* __restrict__ SH) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
